@@ -1,0 +1,165 @@
+"""CPU oracle for the skew tridiagonalization path -- TEST INFRASTRUCTURE ONLY.
+
+This package wraps ``liboracle.so`` (built from ``skew_oracle.c`` by the Makefile here),
+a C restatement of SkewLinearAlgebra.jl's CPU algorithm:
+
+* ``skewblockedhess``   -- src/hessenberg.jl:144-174 (with skewlatrd! :92-131, skewhess! :70-90,
+                            householder! :26-44, setnb :133-142)
+* ``skewtrieigvals``     -- src/skeweigen.jl:105-156 (+ reducetozero :11-47,
+                            implicitstep_novec :73-103, backup :463-468)
+* ``skewtrieigen``       -- src/skeweigen.jl:195-320 (merged) and :322-457 (divided)
+* ``logabspfaffian`` / ``pfaffian`` -- src/pfaffian.jl:74-94, 167-191
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+use it, and only as the checker / the CPU baseline.  The shipped library never calls it.
+"""
+import ctypes
+import glob
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64 = ctypes.c_int64
+
+
+def build():
+    """Compile liboracle.so in place (gcc via the Makefile here)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def openblas_path():
+    """numpy's bundled ILP64 OpenBLAS (symbols carry the scipy_ prefix), or None."""
+    import numpy
+    libdir = os.path.join(os.path.dirname(os.path.dirname(numpy.__file__)), "numpy.libs")
+    hits = sorted(glob.glob(os.path.join(libdir, "libscipy_openblas64_*.so")))
+    return hits[0] if hits else None
+
+
+def lib(use_blas=True):
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.orc_blas_load.argtypes = [ctypes.c_char_p]
+        L.orc_blas_set_threads.argtypes = [ctypes.c_int]
+        L.orc_setnb.argtypes = [_i64]
+        L.orc_setnb.restype = _i64
+        L.orc_skewblockedhess_d.argtypes = [_i64, _dp, _i64, _dp, _dp, _i64]
+        L.orc_skewblockedhess_z.argtypes = [_i64, ctypes.c_void_p, _i64, ctypes.c_void_p, _dp, _i64]
+        L.orc_skewtrieigvals_d.argtypes = [_i64, _dp, _dp]
+        L.orc_skewtrieigen_d.argtypes = [_i64, _dp, _dp, _dp, _dp, ctypes.c_int]
+        L.orc_logabspfaffian_d.argtypes = [_i64, _dp, _i64, _dp, _dp]
+        L.orc_logabspfaffian_tri_d.argtypes = [_i64, _dp, _dp, _dp]
+        L.orc_pfaffian_d.argtypes = [_i64, _dp, _i64]
+        L.orc_pfaffian_d.restype = ctypes.c_double
+        _LIB = L
+        if use_blas:
+            p = openblas_path()
+            if p is not None:
+                L.orc_blas_load(p.encode())
+    return _LIB
+
+
+def set_threads(t):
+    lib().orc_blas_set_threads(int(t))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def setnb(n):
+    return int(lib().orc_setnb(int(n)))
+
+
+def skewblockedhess(A, nb=0):
+    """Reference-style tridiagonalization.  Returns (factors, tau, eta); A is not modified."""
+    A = np.asarray(A)
+    n = A.shape[0]
+    if np.iscomplexobj(A):
+        F = np.array(A, dtype=np.complex128, order="F", copy=True)
+        tau = np.zeros(max(n - 1, 0), dtype=np.complex128)
+        eta = np.zeros(max(n - 1, 0), dtype=np.float64)
+        lib().orc_skewblockedhess_z(n, F.ctypes.data_as(ctypes.c_void_p), n,
+                                    tau.ctypes.data_as(ctypes.c_void_p), _ptr(eta), int(nb))
+        return F, tau, eta
+    F = np.array(A, dtype=np.float64, order="F", copy=True)
+    tau = np.zeros(max(n - 1, 0))
+    eta = np.zeros(max(n - 1, 0))
+    lib().orc_skewblockedhess_d(n, _ptr(F), n, _ptr(tau), _ptr(eta), int(nb))
+    return F, tau, eta
+
+
+def skewtrieigvals(ev):
+    """Imaginary parts of the eigenvalues of SkewHermTridiagonal(ev), reference order."""
+    ev = np.array(ev, dtype=np.float64, copy=True)
+    N = ev.size + 1
+    vals = np.zeros(N)
+    rc = lib().orc_skewtrieigvals_d(N, _ptr(ev), _ptr(vals))
+    if rc != 0:
+        raise RuntimeError("Maximum number of iterations reached, the algorithm didn't converge")
+    return vals
+
+
+def skewtrieigen(ev, variant="divided"):
+    """(vals_imag, vectors) for SkewHermTridiagonal(ev); variant 'divided' or 'merged'."""
+    ev = np.array(ev, dtype=np.float64, copy=True)
+    N = ev.size + 1
+    vals = np.zeros(N)
+    vr = np.zeros((N, N), order="F")
+    vi = np.zeros((N, N), order="F")
+    rc = lib().orc_skewtrieigen_d(N, _ptr(ev), _ptr(vals), _ptr(vr), _ptr(vi),
+                                  0 if variant == "divided" else 1)
+    if rc != 0:
+        raise RuntimeError("skewtrieigen failed (rc=%d)" % rc)
+    return vals, vr + 1j * vi
+
+
+def logabspfaffian(A):
+    A = np.array(A, dtype=np.float64, order="F", copy=True)
+    n = A.shape[0]
+    lp = ctypes.c_double()
+    sg = ctypes.c_double()
+    lib().orc_logabspfaffian_d(n, _ptr(A), n, ctypes.byref(lp), ctypes.byref(sg))
+    return lp.value, sg.value
+
+
+def logabspfaffian_tri(ev):
+    ev = np.ascontiguousarray(ev, dtype=np.float64)
+    lp = ctypes.c_double()
+    sg = ctypes.c_double()
+    lib().orc_logabspfaffian_tri_d(ev.size + 1, _ptr(ev), ctypes.byref(lp), ctypes.byref(sg))
+    return lp.value, sg.value
+
+
+def pfaffian(A):
+    A = np.array(A, dtype=np.float64, order="F", copy=True)
+    return lib().orc_pfaffian_d(A.shape[0], _ptr(A), A.shape[0])
+
+
+def form_q(F, tau):
+    """Q = H_1 ... H_{n-1} from 'L'-layout factors (LAPACK orgtr('L') convention).
+
+    Test helper: reflector i has v[i+1] = 1 (implicit; the stored entry is not read) and
+    v[i+2:] = F[i+2:, i]; H_i = I - tau_i v v^H.
+    """
+    n = F.shape[0]
+    dt = np.result_type(F.dtype, tau.dtype)
+    Q = np.eye(n, dtype=dt)
+    for i in range(n - 2, -1, -1):
+        t = tau[i]
+        if t == 0:
+            continue
+        v = np.zeros(n, dtype=dt)
+        v[i + 1] = 1.0
+        v[i + 2:] = F[i + 2:, i]
+        # Q <- H_i Q  (apply from the left, building the product right to left)
+        Q -= t * np.outer(v, v.conj() @ Q)
+    return Q
