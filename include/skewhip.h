@@ -1,0 +1,105 @@
+/*
+ * skewhip.h -- C ABI of libskewhip.so, the MI355X (gfx950, FP64) implementation of
+ * SkewLinearAlgebra.jl's skew tridiagonalization path.
+ *
+ * Conventions (what Julia's ccall would bind):
+ *   - column-major matrices, 0-based indices, int64_t sizes and leading dimensions;
+ *   - "host" entry points take caller-owned host memory (Julia arrays under GC.@preserve)
+ *     and block until the result is back in that memory;
+ *   - "_device" entry points take device (HBM) pointers and a hipStream_t (passed as void*),
+ *     and are asynchronous with respect to the host unless stated otherwise;
+ *   - return status: 0 success; -k invalid argument number k (1-based); > 0 runtime error
+ *     (SKH_ERR_*).  skh_last_error() gives a message for the calling thread.
+ * Calls are serialised by an internal mutex; the library owns cached device workspaces.
+ */
+#ifndef SKEWHIP_H
+#define SKEWHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SKH_OK 0
+#define SKH_ERR_HIP 1          /* HIP runtime error (message in skh_last_error) */
+#define SKH_ERR_NOCONV 2       /* implicit QR hit its iteration limit */
+#define SKH_ERR_BARRIER 3      /* a grid barrier of the cooperative panel kernel timed out */
+#define SKH_ERR_NOT_SKEW 4     /* input is not skew-symmetric (logabspfaffian! of a Matrix) */
+#define SKH_ERR_UNSUPPORTED 5  /* requested option is not available */
+
+/* engine selection in skh_opts.flags */
+#define SKH_ENGINE_AUTO 0
+#define SKH_ENGINE_LARGE 1     /* cooperative multi-workgroup panel + MFMA rank-2k update */
+#define SKH_ENGINE_TEAM1 2     /* one workgroup per matrix (the batched engine) */
+#define SKH_FLAG_TIMING 0x100  /* record per-kernel HIP events (read with skh_diag_timings) */
+
+typedef struct skh_opts {
+    int64_t nb;     /* panel width; 0 = autotuned table; must be <= 128 */
+    int64_t flags;  /* SKH_ENGINE_* */
+} skh_opts;
+
+/* Library lifetime.  skh_init selects the HIP device; the other calls init device 0 lazily. */
+int skh_init(int device);
+int skh_finalize(void);
+const char *skh_version(void);
+const char *skh_last_error(void);
+
+/*
+ * Blocked Householder skew tridiagonalization, real Float64.
+ * Replaces skewblockedhess!(S::SkewHermitian{Float64}) -> (tau, eta)
+ *   (reference src/hessenberg.jl:144-174; callers hessenberg! :5-13, skeweigvals! src/eigen.jl:49-73,
+ *    skeweigen! src/eigen.jl:75-91, _pfaffian!/_logabspfaffian! src/pfaffian.jl:74-84,167-179).
+ * On return the strictly lower triangle of A holds the 'L'-layout factors (reflector i in
+ * A[i+1:n, i] with the unit A[i+1,i] = 1 stored; the last entry A[n-1,n-2] keeps eta[n-2]), the
+ * diagonal and upper triangle are left as given.  tau, eta have n-1 entries.
+ * *ntau_nonzero (may be NULL) = number of tau != 0, so det(Q) = (-1)^ntau_nonzero.
+ */
+int skh_sytrd_d(int64_t n, double *A, int64_t lda, double *tau, double *eta, int64_t *ntau_nonzero,
+                const skh_opts *opts);
+int skh_sytrd_d_device(int64_t n, double *dA, int64_t lda, double *dtau, double *deta, void *stream,
+                       const skh_opts *opts);
+
+/*
+ * log|Pf(A)| and sign for a real skew-symmetric A (A is not modified).
+ * Replaces _logabspfaffian!(A::SkewHermitian{<:Real}) (reference src/pfaffian.jl:167-179):
+ * odd n -> (-Inf, 0).
+ */
+int skh_logpf_d(int64_t n, const double *A, int64_t lda, double *logpf, double *sgn, const skh_opts *opts);
+
+/*
+ * Batched log|Pf| + sign over `batch` real skew matrices of uneven sizes, packed column-major
+ * (matrix b is n[b] x n[b] at element offset offset[b] of Apacked, lda = n[b]).
+ * No reference counterpart: the reference user loops logabspfaffian (src/pfaffian.jl:236).
+ * One persistent launch, one workgroup per matrix, largest matrices first.
+ */
+int skh_logpf_batched_d(int64_t batch, const int64_t *n, const int64_t *offset, const double *Apacked,
+                        double *logpf, double *sgn, const skh_opts *opts);
+/* device variant: dA, dlogpf, dsgn in HBM; n and offset are host arrays (used for scheduling). */
+int skh_logpf_batched_d_device(int64_t batch, const int64_t *n, const int64_t *offset, const double *dA,
+                               double *dlogpf, double *dsgn, void *stream, const skh_opts *opts);
+
+/*
+ * Host implicit-QR eigensolver of the real skew tridiagonal SkewHermTridiagonal(ev), n = size.
+ * Replaces skewtrieigvals! (reference src/skeweigen.jl:105-156, backup :463-468): vals receives
+ * the imaginary parts of the eigenvalues in the reference's order; ev (n-1) is overwritten.
+ */
+int skh_skewtrieigvals_d(int64_t n, double *ev, double *vals);
+/*
+ * Eigenvectors as well: variant 0 = skewtrieigen_divided! (src/skeweigen.jl:322-457, used by
+ * eigen(::SkewHermitian)), 1 = skewtrieigen_merged! (:195-320, used by eigen(::SkewHermTridiagonal)).
+ * vr, vi: n x n real and imaginary parts of the eigenvector matrix (column-major).
+ */
+int skh_skewtrieigen_d(int64_t n, double *ev, double *vals, double *vr, double *vi, int variant);
+
+/* Diagnostics: per-kernel times of the last skh_sytrd_d_device call made with SKH_FLAG_TIMING:
+ * out6 = {panel ms, panel launches, rank-2k ms, rank-2k launches, panel algorithmic bytes,
+ *         rank-2k algorithmic flops}.  Waits for the recorded events. */
+int skh_diag_timings(double *out6);
+/* Diagnostics: v_mfma_f64_16x16x4f64 lane-layout probe (768 doubles, see the test suite). */
+int skh_diag_mfma_probe(double *out768);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,528 @@
+// skh_api.cpp -- the C ABI of libskewhip.so (declared in include/skewhip.h).
+//
+// Host-memory entry points copy in, run the device engine on the library's stream and copy
+// back; device entry points run on the caller's stream.  Workspaces are cached per device
+// and only grow.  All entry points are serialised by one mutex.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/skewhip.h"
+#include "skh_internal.h"
+
+namespace {
+
+std::mutex g_mu;
+thread_local std::string g_err;
+bool g_inited = false;
+int g_device = 0;
+int g_num_cu = 0;
+hipStream_t g_stream = nullptr;
+
+struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void* get(size_t want)
+    {
+        if (want > bytes) {
+            if (p) (void)hipFree(p);
+            p = nullptr;
+            bytes = 0;
+            if (hipMalloc(&p, want) != hipSuccess) {
+                p = nullptr;
+                return nullptr;
+            }
+            bytes = want;
+        }
+        return p;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+Buf g_mat, g_ws, g_vec, g_ctl, g_bin, g_bout;
+
+int set_err(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                     \
+    do {                                                                                                 \
+        hipError_t _e = (expr);                                                                          \
+        if (_e != hipSuccess) return set_err(SKH_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+int ensure_init_locked(int device)
+{
+    if (g_inited) return 0;
+    int count = 0;
+    HIPCHK(hipGetDeviceCount(&count));
+    if (count <= 0) return set_err(SKH_ERR_HIP, "no HIP device visible");
+    if (device < 0 || device >= count) return set_err(-1, "device %d out of range (have %d)", device, count);
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    g_num_cu = prop.multiProcessorCount;
+    int coop = 0;
+    HIPCHK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device));
+    if (!coop) return set_err(SKH_ERR_UNSUPPORTED, "device does not support cooperative launch");
+    HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    g_device = device;
+    g_inited = true;
+    return 0;
+}
+
+// autotuned panel width per size (refreshed from measurements; see the bench/tuning notes)
+int64_t pick_nb(int64_t n, const skh_opts* o, bool batched)
+{
+    if (o && o->nb > 0) return std::min<int64_t>(o->nb, skh::NBMAX);
+    if (batched) return n <= 96 ? 16 : 32;
+    return n < 1024 ? 32 : 64;
+}
+
+int engine_of(int64_t n, const skh_opts* o)
+{
+    int64_t f = o ? (o->flags & 0xff) : 0;
+    if (f == SKH_ENGINE_LARGE || f == SKH_ENGINE_TEAM1) return (int)f;
+    return n < 128 ? SKH_ENGINE_TEAM1 : SKH_ENGINE_LARGE;
+}
+
+// Optional per-kernel timing (SKH_FLAG_TIMING): event pairs around every panel and rank-2k
+// launch of the last large-engine factorization, with each launch's algorithmic work.
+struct KTimer {
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kind;        // 0 = panel, 1 = rank-2k
+    std::vector<double> work;     // bytes (panel) or flops (rank-2k)
+    size_t used = 0;
+    bool on = false;
+    int begin(hipStream_t st)
+    {
+        if (used + 2 > ev.size()) {
+            for (int k = 0; k < 64; k++) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) return 1;
+                ev.push_back(e);
+            }
+        }
+        return (int)hipEventRecord(ev[used], st);
+    }
+    int end(hipStream_t st, int k, double w)
+    {
+        int rc = (int)hipEventRecord(ev[used + 1], st);
+        kind.push_back(k);
+        work.push_back(w);
+        used += 2;
+        return rc;
+    }
+    void reset()
+    {
+        used = 0;
+        kind.clear();
+        work.clear();
+    }
+};
+KTimer g_timer;
+
+// control block: [0] barrier counter, [1] error flag, [2] batch queue
+struct Ctl {
+    unsigned* bar;
+    int* err;
+    unsigned* queue;
+};
+
+int get_ctl(Ctl& c)
+{
+    unsigned* p = (unsigned*)g_ctl.get(256);
+    if (!p) return set_err(SKH_ERR_HIP, "hipMalloc of control block failed");
+    c.bar = p;
+    c.err = (int*)(p + 1);
+    c.queue = p + 2;
+    return 0;
+}
+
+// ----------------------------------------------------------------------------------------
+// Large engine: cooperative panel kernel + MFMA rank-2k update per panel.
+// ----------------------------------------------------------------------------------------
+int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, int64_t nb, hipStream_t st)
+{
+    Ctl ctl;
+    int rc = get_ctl(ctl);
+    if (rc) return rc;
+    const int64_t ld = (n + 1) & ~int64_t(1);
+    const int64_t nt = (n + skh::TB - 1) / skh::TB + 1;
+    const int64_t nseg = (nt + skh::SEG - 1) / skh::SEG + 1;
+    size_t doubles = (size_t)(ld * nb + nt + nt * 2 * skh::NBMAX + 2 * skh::NBMAX + nseg * ld + nt * ld + 64);
+    double* ws = (double*)g_ws.get(doubles * sizeof(double));
+    if (!ws) return set_err(SKH_ERR_HIP, "hipMalloc of %zu bytes of workspace failed", doubles * 8);
+    skh::Work w;
+    w.W = ws;
+    w.ldw = ld;
+    w.Pnorm = w.W + ld * nb;
+    w.PS = w.Pnorm + nt;
+    w.Tv = w.PS + nt * 2 * skh::NBMAX;
+    w.Prow = w.Tv + 2 * skh::NBMAX;
+    w.ldp = ld;
+    w.Pcol = w.Prow + nseg * ld;
+
+    int per_cu = 0;
+    HIPCHK((hipError_t)skh_internal_panel_occupancy(&per_cu));
+    per_cu = std::max(1, std::min(per_cu, 2));
+    const int grid = per_cu * g_num_cu;
+    if ((int64_t)grid * (int64_t)(2 * n + 4) > 4000000000LL)
+        return set_err(SKH_ERR_UNSUPPORTED, "n = %lld too large for the barrier counter", (long long)n);
+
+    HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 3, st));
+    unsigned base = 0;
+    for (int64_t c0 = 0; c0 <= n - 3; c0 += nb) {
+        const int nbp = (int)std::min<int64_t>(nb, n - 2 - c0);
+        PanelLaunch pl;
+        pl.A = dA;
+        pl.lda = lda;
+        pl.n = n;
+        pl.c0 = c0;
+        pl.nbp = nbp;
+        pl.w = w;
+        pl.tau = dtau;
+        pl.eta = deta;
+        pl.bar = ctl.bar;
+        pl.base = base;
+        pl.err = ctl.err;
+        pl.grid = grid;
+        const bool tm = g_timer.on;
+        if (tm) HIPCHK((hipError_t)g_timer.begin(st));
+        HIPCHK((hipError_t)skh_internal_launch_panel(&pl, st));
+        if (tm) {
+            // algorithmic bytes: strictly lower triangle of each column's trailing block, once
+            double bytes = 0.0;
+            for (int64_t j = c0; j < c0 + nbp; j++) {
+                const double m = (double)(n - j - 1);
+                bytes += 8.0 * m * (m - 1.0) / 2.0;
+            }
+            HIPCHK((hipError_t)g_timer.end(st, 0, bytes));
+        }
+        base += (unsigned)(2 * nbp * grid);
+        const int64_t e = c0 + nbp;
+        if (tm) HIPCHK((hipError_t)g_timer.begin(st));
+        HIPCHK((hipError_t)skh_internal_launch_rank2k(dA, lda, w.W, w.ldw, n, e, e, c0, nbp, st));
+        if (tm) {
+            const double m = (double)(n - e);
+            HIPCHK((hipError_t)g_timer.end(st, 1, 2.0 * m * (m - 1.0) * nbp));
+        }
+    }
+    HIPCHK((hipError_t)skh_internal_launch_finish(dA, lda, n, dtau, deta, st));
+    return 0;
+}
+
+// ----------------------------------------------------------------------------------------
+// Team-of-one engine (batched).  dA packed; ns/offs host arrays.
+// ----------------------------------------------------------------------------------------
+int batched_run(int64_t batch, const int64_t* ns, const int64_t* offs, const double* dA, double* dlogpf,
+                double* dsgn, const skh_opts* opts, hipStream_t st)
+{
+    if (batch == 0) return 0;
+    int64_t nmax = 0;
+    for (int64_t b = 0; b < batch; b++) nmax = std::max(nmax, ns[b]);
+    const int nb = (int)pick_nb(nmax, opts, true);
+    std::vector<int64_t> order(batch);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return ns[a] > ns[b]; });
+
+    int per_cu = 0;
+    HIPCHK((hipError_t)skh_internal_batched_occupancy(&per_cu));
+    per_cu = std::max(1, per_cu);
+    const int grid = (int)std::min<int64_t>((int64_t)per_cu * g_num_cu, batch);
+    const int64_t slot = skh_slot_doubles(std::max<int64_t>(nmax, 2), nb);
+
+    // control arrays: ns, offs, order (device copies)
+    size_t ctl_bytes = sizeof(int64_t) * 3 * batch;
+    int64_t* dctl = (int64_t*)g_vec.get(ctl_bytes);
+    double* slots = (double*)g_ws.get((size_t)slot * grid * sizeof(double));
+    Ctl ctl;
+    int rc = get_ctl(ctl);
+    if (rc) return rc;
+    if (!dctl || !slots) return set_err(SKH_ERR_HIP, "hipMalloc of batched workspace failed");
+    std::vector<int64_t> h(3 * batch);
+    std::copy(ns, ns + batch, h.begin());
+    std::copy(offs, offs + batch, h.begin() + batch);
+    std::copy(order.begin(), order.end(), h.begin() + 2 * batch);
+    HIPCHK(hipMemcpyAsync(dctl, h.data(), ctl_bytes, hipMemcpyHostToDevice, st));
+    HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 3, st));
+    BatchArgs a;
+    a.A = dA;
+    a.ns = dctl;
+    a.offs = dctl + batch;
+    a.order = dctl + 2 * batch;
+    a.batch = batch;
+    a.slots = slots;
+    a.slot_stride = slot;
+    a.nb = nb;
+    a.queue = ctl.queue;
+    a.logpf = dlogpf;
+    a.sgn = dsgn;
+    HIPCHK((hipError_t)skh_internal_launch_batched(&a, grid, st));
+    // the host copy of the control arrays must outlive the async copy
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int check_barrier_flag()
+{
+    Ctl ctl;
+    int rc = get_ctl(ctl);
+    if (rc) return rc;
+    int herr = 0;
+    HIPCHK(hipMemcpy(&herr, ctl.err, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr) return set_err(SKH_ERR_BARRIER, "grid barrier timed out in the panel kernel");
+    return 0;
+}
+
+int sytrd_device_locked(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, hipStream_t st,
+                        const skh_opts* opts)
+{
+    if (n <= 1) return 0;
+    const int64_t nb = pick_nb(n, opts, engine_of(n, opts) == SKH_ENGINE_TEAM1);
+    g_timer.reset();
+    g_timer.on = opts && (opts->flags & SKH_FLAG_TIMING);
+    if (engine_of(n, opts) == SKH_ENGINE_LARGE) return sytrd_large(n, dA, lda, dtau, deta, nb, st);
+    g_timer.on = false;
+    return set_err(SKH_ERR_UNSUPPORTED, "team-of-one engine for sytrd is driven through the host entry point");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* skh_version(void) { return "skewhip 0.1.0 (gfx950, FP64)"; }
+
+const char* skh_last_error(void) { return g_err.c_str(); }
+
+int skh_init(int device)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_inited && device != g_device) return set_err(SKH_ERR_UNSUPPORTED, "already initialised on device %d", g_device);
+    return ensure_init_locked(device);
+}
+
+int skh_finalize(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_inited) return 0;
+    (void)hipStreamSynchronize(g_stream);
+    for (Buf* b : {&g_mat, &g_ws, &g_vec, &g_ctl, &g_bin, &g_bout}) b->release();
+    (void)hipStreamDestroy(g_stream);
+    g_stream = nullptr;
+    g_inited = false;
+    return 0;
+}
+
+int skh_sytrd_d_device(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, void* stream,
+                       const skh_opts* opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (n < 0) return set_err(-1, "n < 0");
+    if (n > 0 && !dA) return set_err(-2, "A is NULL");
+    if (lda < std::max<int64_t>(n, 1) || (lda & 1)) return set_err(-3, "lda must be >= n and even");
+    if (n > 1 && (!dtau || !deta)) return set_err(-4, "tau/eta NULL");
+    if (((uintptr_t)dA) & 15) return set_err(-2, "A must be 16-byte aligned");
+    if (opts && (opts->nb < 0 || opts->nb > skh::NBMAX)) return set_err(-7, "nb out of range");
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    hipStream_t st = stream ? (hipStream_t)stream : g_stream;
+    if (n <= 1) return 0;
+    skh_opts o = opts ? *opts : skh_opts{0, 0};
+    o.flags = SKH_ENGINE_LARGE | (o.flags & SKH_FLAG_TIMING);
+    return sytrd_device_locked(n, dA, lda, dtau, deta, st, &o);
+}
+
+int skh_sytrd_d(int64_t n, double* A, int64_t lda, double* tau, double* eta, int64_t* ntau_nonzero,
+                const skh_opts* opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (n < 0) return set_err(-1, "n < 0");
+    if (n > 0 && !A) return set_err(-2, "A is NULL");
+    if (lda < std::max<int64_t>(n, 1)) return set_err(-3, "lda < n");
+    if (n > 1 && (!tau || !eta)) return set_err(-4, "tau/eta NULL");
+    if (opts && (opts->nb < 0 || opts->nb > skh::NBMAX)) return set_err(-7, "nb out of range");
+    if (ntau_nonzero) *ntau_nonzero = 0;
+    if (n <= 1) return 0;
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    hipStream_t st = g_stream;
+    const int64_t ld = (n + 1) & ~int64_t(1);
+    double* dA = (double*)g_mat.get((size_t)ld * n * sizeof(double));
+    double* dv = (double*)g_vec.get((size_t)2 * n * sizeof(double));
+    if (!dA || !dv) return set_err(SKH_ERR_HIP, "hipMalloc failed for n = %lld", (long long)n);
+    double* dtau = dv;
+    double* deta = dv + n;
+    HIPCHK(hipMemcpy2DAsync(dA, ld * sizeof(double), A, lda * sizeof(double), n * sizeof(double), n,
+                            hipMemcpyHostToDevice, st));
+    // the team-of-one engine serves batched/logpf calls; single factorizations use the large engine
+    const int eng = (opts && (opts->flags & 0xff) == SKH_ENGINE_TEAM1) ? SKH_ENGINE_TEAM1 : SKH_ENGINE_LARGE;
+    if (eng == SKH_ENGINE_LARGE) {
+        g_timer.reset();
+        g_timer.on = false;
+        rc = sytrd_large(n, dA, ld, dtau, deta, pick_nb(n, opts, false), st);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(st));
+        rc = check_barrier_flag();
+        if (rc) return rc;
+    } else {
+        // one matrix through the batched engine, then read its factors back from the slot
+        return set_err(SKH_ERR_UNSUPPORTED, "team-of-one sytrd not wired yet");
+    }
+    HIPCHK(hipMemcpy2DAsync(A, lda * sizeof(double), dA, ld * sizeof(double), n * sizeof(double), n,
+                            hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(tau, dtau, (n - 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(eta, deta, (n - 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (ntau_nonzero) {
+        int64_t c = 0;
+        for (int64_t k = 0; k < n - 1; k++) c += (tau[k] != 0.0);
+        *ntau_nonzero = c;
+    }
+    return 0;
+}
+
+int skh_logpf_d(int64_t n, const double* A, int64_t lda, double* logpf, double* sgn, const skh_opts* opts)
+{
+    if (n < 0) return set_err(-1, "n < 0");
+    if (n > 0 && !A) return set_err(-2, "A is NULL");
+    if (lda < std::max<int64_t>(n, 1)) return set_err(-3, "lda < n");
+    if (!logpf || !sgn) return set_err(-4, "output NULL");
+    if (n % 2 == 1) {
+        *logpf = -INFINITY;
+        *sgn = 0.0;
+        return 0;
+    }
+    if (n == 0) {
+        *logpf = 0.0;
+        *sgn = 1.0;
+        return 0;
+    }
+    if (engine_of(n, opts) == SKH_ENGINE_TEAM1) {
+        int64_t nn = n, off = 0;
+        std::vector<double> packed((size_t)n * n);
+        for (int64_t c = 0; c < n; c++) std::memcpy(&packed[c * n], A + c * lda, n * sizeof(double));
+        return skh_logpf_batched_d(1, &nn, &off, packed.data(), logpf, sgn, opts);
+    }
+    std::vector<double> F((size_t)n * n), tau(n - 1), eta(n - 1);
+    for (int64_t c = 0; c < n; c++) std::memcpy(&F[c * n], A + c * lda, n * sizeof(double));
+    int64_t nz = 0;
+    int rc = skh_sytrd_d(n, F.data(), n, tau.data(), eta.data(), &nz, opts);
+    if (rc) return rc;
+    // src/pfaffian.jl:171-178, in the reference's summation order
+    double l = 0.0, s = 1.0;
+    for (int64_t i = 0; i + 1 < n; i += 2) {
+        l += std::log(std::fabs(eta[i]));
+        const double m = -eta[i];
+        s *= (m > 0) - (m < 0);
+    }
+    *logpf = l;
+    *sgn = (nz % 2) ? -s : s;
+    return 0;
+}
+
+int skh_logpf_batched_d_device(int64_t batch, const int64_t* n, const int64_t* offset, const double* dA,
+                               double* dlogpf, double* dsgn, void* stream, const skh_opts* opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (batch < 0) return set_err(-1, "batch < 0");
+    if (batch > 0 && (!n || !offset || !dA || !dlogpf || !dsgn)) return set_err(-2, "NULL argument");
+    for (int64_t b = 0; b < batch; b++) {
+        if (n[b] < 0) return set_err(-2, "n[%lld] < 0", (long long)b);
+        if (offset[b] < 0 || (offset[b] & 1)) return set_err(-3, "offset[%lld] must be >= 0 and even", (long long)b);
+    }
+    if (opts && (opts->nb < 0 || opts->nb > skh::NBMAX)) return set_err(-8, "nb out of range");
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    hipStream_t st = stream ? (hipStream_t)stream : g_stream;
+    return batched_run(batch, n, offset, dA, dlogpf, dsgn, opts, st);
+}
+
+int skh_logpf_batched_d(int64_t batch, const int64_t* n, const int64_t* offset, const double* Apacked,
+                        double* logpf, double* sgn, const skh_opts* opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (batch < 0) return set_err(-1, "batch < 0");
+    if (batch == 0) return 0;
+    if (!n || !offset || !Apacked || !logpf || !sgn) return set_err(-2, "NULL argument");
+    for (int64_t b = 0; b < batch; b++) {
+        if (n[b] < 0) return set_err(-2, "n[%lld] < 0", (long long)b);
+        if (offset[b] < 0) return set_err(-3, "offset[%lld] < 0", (long long)b);
+    }
+    if (opts && (opts->nb < 0 || opts->nb > skh::NBMAX)) return set_err(-7, "nb out of range");
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    hipStream_t st = g_stream;
+    // repack with even offsets (16-byte aligned matrices)
+    std::vector<int64_t> offs2(batch);
+    int64_t tot2 = 0;
+    for (int64_t b = 0; b < batch; b++) {
+        offs2[b] = tot2;
+        tot2 += (n[b] * n[b] + 1) & ~int64_t(1);
+    }
+    double* dA = (double*)g_bin.get(std::max<int64_t>(tot2, 1) * sizeof(double));
+    double* dout = (double*)g_bout.get(2 * batch * sizeof(double));
+    if (!dA || !dout) return set_err(SKH_ERR_HIP, "hipMalloc of batched buffers failed");
+    for (int64_t b = 0; b < batch; b++)
+        HIPCHK(hipMemcpyAsync(dA + offs2[b], Apacked + offset[b], n[b] * n[b] * sizeof(double), hipMemcpyHostToDevice, st));
+    rc = batched_run(batch, n, offs2.data(), dA, dout, dout + batch, opts, st);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(logpf, dout, batch * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(sgn, dout + batch, batch * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int skh_diag_timings(double* out6)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!out6) return set_err(-1, "NULL");
+    for (int k = 0; k < 6; k++) out6[k] = 0.0;
+    if (g_timer.used == 0) return 0;
+    HIPCHK(hipEventSynchronize(g_timer.ev[g_timer.used - 1]));
+    for (size_t q = 0; q < g_timer.kind.size(); q++) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, g_timer.ev[2 * q], g_timer.ev[2 * q + 1]));
+        const int k = g_timer.kind[q];
+        out6[2 * k] += ms;
+        out6[2 * k + 1] += 1.0;
+        out6[4 + k] += g_timer.work[q];
+    }
+    return 0;
+}
+
+int skh_diag_mfma_probe(double* out768)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!out768) return set_err(-1, "NULL");
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    double* d = (double*)g_vec.get(768 * sizeof(double));
+    if (!d) return set_err(SKH_ERR_HIP, "hipMalloc failed");
+    HIPCHK((hipError_t)skh_internal_mfma_probe(d, g_stream));
+    HIPCHK(hipMemcpyAsync(out768, d, 768 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+}  // extern "C"

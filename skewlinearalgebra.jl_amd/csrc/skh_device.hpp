@@ -1,0 +1,541 @@
+// skh_device.hpp -- device building blocks of the blocked skew tridiagonalization (gfx950, FP64).
+//
+// The factorization follows the blocked Householder reduction of SkewLinearAlgebra.jl
+// (src/hessenberg.jl:92-174): panels of nb columns, each column computing the reflector of
+// the virtually updated column, w = tau * (A22 v + V (W^T v) - W (V^T v)), and after each
+// panel the skew rank-2k trailing update A22 += V2 W2^T - W2 V2^T (lower triangle only).
+//
+// Every phase below is written for a team of G workgroups (rank g in [0, G)).  The
+// single-matrix engine runs it with G = the whole (co-resident) grid and a grid barrier; the
+// batched engine runs it with G = 1 (one workgroup per matrix) and __syncthreads as barrier.
+//
+// Memory: A column-major, lda even, base 16-byte aligned.  Only the strictly lower triangle
+// of the trailing block is read (the real skew diagonal is zero, the upper half is -A^T).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace skh {
+
+constexpr int WG = 256;       // threads per workgroup (4 waves of 64)
+constexpr int TB = 64;        // matvec tile edge and row-chunk height
+constexpr int SEG = 8;        // tiles per row-strip segment in the matvec
+constexpr int NBMAX = 128;    // largest panel width
+constexpr int BT = 128;       // MFMA output tile edge of the rank-2k update
+constexpr int KC = 16;        // k-chunk staged in LDS by the rank-2k update
+constexpr int LDSLD = BT + 16;  // padded LDS row (doubles) for the staged operands
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// Device workspace of one factorization (one matrix).
+struct Work {
+    double* W;       // n x NBMAX (ldw)
+    int64_t ldw;
+    double* Pnorm;   // [nq]            per-chunk sums of squares of the reflector tail
+    double* PS;      // [nq][2*NBMAX]   per-chunk W^T x and V^T x partials
+    double* Tv;      // [2*NBMAX]       t1 = W^T v, t2 = V^T v
+    double* Prow;    // [nseg][ldp]     row partials of the skew matvec
+    double* Pcol;    // [nt][ldp]       column partials of the skew matvec
+    int64_t ldp;
+};
+
+// Per-column scalars kept in LDS across the phases.
+struct ColScal {
+    double tau, scale, unit, beta;
+};
+
+__device__ __forceinline__ double wave_sum(double x)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+__device__ __forceinline__ int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---------------------------------------------------------------------------------------
+// Reflector scalars from alpha = x[0] and the sum of squares of x[1:] (householder!,
+// src/hessenberg.jl:26-44).  beta = (alpha > 0 ? -1 : +1) * hypot(alpha, xnorm),
+// tau = 1 - alpha/beta, v = [1; x[1:]/(alpha - beta)]; all-zero x gives tau = beta = 0, v = 0.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ ColScal reflector_scalars(double alpha, double xn2)
+{
+    ColScal s;
+    double xnorm = sqrt(xn2);
+    if (xnorm != 0.0 || alpha != 0.0) {
+        double b = ((alpha > 0.0) ? -1.0 : 1.0) * hypot(alpha, xnorm);
+        s.beta = b;
+        s.tau = 1.0 - alpha / b;
+        s.scale = 1.0 / (alpha - b);
+        s.unit = 1.0;
+    } else {
+        s.beta = 0.0;
+        s.tau = 0.0;
+        s.scale = 0.0;
+        s.unit = 0.0;
+    }
+    return s;
+}
+
+// v_c of the current reflector (column j): 0 outside [j+1, n), unit at j+1, scale*x elsewhere.
+__device__ __forceinline__ double vval(const double* __restrict__ A, int64_t lda, int64_t n, int64_t j,
+                                       int64_t c, const ColScal& s)
+{
+    if (c <= j || c >= n) return 0.0;
+    if (c == j + 1) return s.unit;
+    return s.scale * A[c + j * lda];
+}
+
+// ---------------------------------------------------------------------------------------
+// Row pass: finalises w of column j-1 (doC) and/or applies the panel column update to
+// column j and produces its norm and W^T x / V^T x partials (doA).
+//   i = j - c0 = number of previous panel columns that update column j.
+// Rows are processed in absolute 64-row chunks starting at block j/64; chunk q is owned by
+// rank q mod G in both uses, so the row data a rank writes in C(j-1) is the data it reads in
+// A(j).  Row j of w_{j-1} (needed by everyone as W[j, i-1]) is computed redundantly by every
+// rank with one fixed summation order, and the owner stores that same value.
+// ---------------------------------------------------------------------------------------
+struct RowShared {
+    double wrow[NBMAX];
+    double vrow[NBMAX];
+    double t[2 * NBMAX];
+    double red[4][TB];
+    double red2[4][TB];
+    double wspec;
+    ColScal cur;     // scalars of the column finalised by the last phase B
+};
+
+__device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j, int i,
+                         bool doC, bool doA, const Work& w, int g, int G, RowShared& sh)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t jm = j - 1;         // column finalised by the C part
+    const int im = i - 1;             // its local index
+    const ColScal cs = sh.cur;        // scalars of column jm (valid when doC)
+    double* __restrict__ Wm = w.W;
+    const int64_t ldw = w.ldw;
+    const int64_t b0 = j / TB;        // first chunk block
+    const int64_t nt = cdiv(n, TB);
+
+    if (doC) {
+        // t1, t2 of column jm (im entries each)
+        for (int k = tid; k < im; k += WG) {
+            sh.t[k] = w.Tv[k];
+            sh.t[NBMAX + k] = w.Tv[NBMAX + k];
+        }
+    }
+    __syncthreads();
+    if (doC && wave == 0) {
+        // special row r = j of w_{jm}: strip b0 has one segment; column partials I in [b0, nt)
+        double acc = 0.0;
+        if (lane == 0) acc += w.Prow[j];
+        for (int64_t I = b0 + lane; I < nt; I += 64) acc += w.Pcol[I * w.ldp + j];
+        for (int k = lane; k < im; k += 64)
+            acc += A[j + (c0 + k) * lda] * sh.t[k] - Wm[j + k * ldw] * sh.t[NBMAX + k];
+        acc = wave_sum(acc);
+        if (lane == 0) sh.wspec = cs.tau * acc;
+    }
+    __syncthreads();
+    if (doA) {
+        for (int k = tid; k < i; k += WG) {
+            if (doC && k == im) {
+                sh.wrow[k] = sh.wspec;
+                sh.vrow[k] = cs.unit;
+            } else {
+                sh.wrow[k] = Wm[j + k * ldw];
+                sh.vrow[k] = A[j + (c0 + k) * lda];
+            }
+        }
+    }
+    __syncthreads();
+
+    const int kq = (i + 3) / 4;           // k-group size per wave for the A part
+    const int kqm = (im + 3) / 4;         // and for the C correction
+    const int64_t nq = nt - b0;
+    for (int64_t q = g; q < nq; q += G) {
+        const int64_t r = (b0 + q) * TB + lane;
+        const bool rin = r < n;
+        double wr = 0.0, vr = 0.0;
+        if (doC) {
+            double p = 0.0;
+            if (rin && r >= j) {
+                const int64_t b = b0 + q;
+                // row partials: strip b has segments s in [0, (b - b0)/SEG]
+                const int64_t nseg = (b - b0) / SEG + 1;
+                for (int64_t s = wave; s < nseg; s += 4) p += w.Prow[s * w.ldp + r];
+                for (int64_t I = b + wave; I < nt; I += 4) p += w.Pcol[I * w.ldp + r];
+                const int k0 = wave * kqm, k1 = min(im, k0 + kqm);
+                for (int k = k0; k < k1; k++)
+                    p += A[r + (c0 + k) * lda] * sh.t[k] - Wm[r + k * ldw] * sh.t[NBMAX + k];
+            }
+            sh.red[wave][lane] = p;
+        }
+        __syncthreads();
+        if (doC && rin && r >= j) {
+            double s = sh.red[0][lane] + sh.red[1][lane] + sh.red[2][lane] + sh.red[3][lane];
+            wr = (r == j) ? sh.wspec : cs.tau * s;
+            vr = (r == j) ? cs.unit : cs.scale * A[r + jm * lda];
+        }
+        __syncthreads();   // every wave has read A[r, jm] before wave 0 overwrites it
+        if (doC && rin && r >= j && wave == 0) {
+            Wm[r + im * ldw] = wr;
+            A[r + jm * lda] = vr;
+        }
+        if (doA) {
+            double p = 0.0;
+            double xold = 0.0;
+            if (rin && r > j) {
+                xold = A[r + j * lda];
+                const int k0 = wave * kq, k1 = min(i, k0 + kq);
+                for (int k = k0; k < k1; k++) {
+                    double vk, wk;
+                    if (doC && k == im) {
+                        vk = vr;
+                        wk = wr;
+                    } else {
+                        vk = A[r + (c0 + k) * lda];
+                        wk = Wm[r + k * ldw];
+                    }
+                    p += vk * sh.wrow[k] - wk * sh.vrow[k];
+                }
+            }
+            sh.red2[wave][lane] = p;
+            __syncthreads();
+            double x = 0.0;
+            if (rin && r > j)
+                x = xold + (sh.red2[0][lane] + sh.red2[1][lane] + sh.red2[2][lane] + sh.red2[3][lane]);
+            if (i > 0 && rin && r > j && wave == 0) A[r + j * lda] = x;
+            const bool tail = rin && r >= j + 2;
+            if (wave == 0) {
+                double ss = wave_sum(tail ? x * x : 0.0);
+                if (lane == 0) w.Pnorm[q] = ss;
+            }
+            // S partials of column j: sum over the chunk's rows r >= j+2 of W[r,k] x_r, V[r,k] x_r
+            const int k0 = wave * kq, k1 = min(i, k0 + kq);
+            for (int k = k0; k < k1; k++) {
+                double vk = 0.0, wk = 0.0;
+                if (tail) {
+                    if (doC && k == im) {
+                        vk = vr;
+                        wk = wr;
+                    } else {
+                        vk = A[r + (c0 + k) * lda];
+                        wk = Wm[r + k * ldw];
+                    }
+                }
+                double s1 = wave_sum(wk * x);
+                double s2 = wave_sum(vk * x);
+                if (lane == 0) {
+                    w.PS[q * (2 * NBMAX) + k] = s1;
+                    w.PS[q * (2 * NBMAX) + NBMAX + k] = s2;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Phase B scalars: reduce the norm partials of column j, form the reflector scalars.
+// Every rank computes them with the same fixed order (wave 0, strided lanes, then a
+// butterfly), so all ranks hold bit-identical tau/scale.
+// ---------------------------------------------------------------------------------------
+__device__ void column_scalars(const double* __restrict__ A, int64_t lda, int64_t n, int64_t j,
+                               const Work& w, RowShared& sh, double* tau_out, double* eta_out,
+                               bool writer)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (wave == 0) {
+        const int64_t nq = cdiv(n, TB) - j / TB;
+        double acc = 0.0;
+        for (int64_t q = lane; q < nq; q += 64) acc += w.Pnorm[q];
+        acc = wave_sum(acc);
+        if (lane == 0) {
+            ColScal s = reflector_scalars(A[(j + 1) + j * lda], acc);
+            sh.cur = s;
+            if (writer) {
+                tau_out[j] = s.tau;
+                eta_out[j] = s.beta;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// t1 = W^T v, t2 = V^T v for column j from the per-chunk partials (i entries each).
+__device__ void reduce_t(const double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j,
+                         int i, const Work& w, const RowShared& sh, int g, int G)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t nq = cdiv(n, TB) - j / TB;
+    const ColScal s = sh.cur;
+    for (int tv = g * 4 + wave; tv < 2 * i; tv += G * 4) {
+        const int k = tv % i;
+        const int off = (tv < i) ? 0 : NBMAX;
+        double acc = 0.0;
+        for (int64_t q = lane; q < nq; q += 64) acc += w.PS[q * (2 * NBMAX) + off + k];
+        acc = wave_sum(acc);
+        if (lane == 0) {
+            double head = (tv < i) ? w.W[(j + 1) + k * w.ldw] : A[(j + 1) + (c0 + k) * lda];
+            w.Tv[off + k] = s.unit * head + s.scale * acc;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Skew matvec partials of column j over the strictly lower triangle of A[j+1:n, j+1:n].
+// Work item = (strip I, segment s) = tiles (I, J), J in [J0 + s*SEG, min(J0+s*SEG+SEG-1, I)].
+// Thread layout per 64x64 tile: wave -> 16 columns, lane -> 4 columns (lane>>4) x 4 rows
+// {2p, 2p+1, 32+2p, 33+2p} (p = lane&15), rows read as 16-byte pairs.
+//   Prow[s][r] = sum_{c in segment} A[r,c] v_c,   Pcol[I][c] = -sum_{r in strip I} A[r,c] v_r
+// ---------------------------------------------------------------------------------------
+__device__ void skew_matvec(const double* __restrict__ A, int64_t lda, int64_t n, int64_t j,
+                            const Work& w, RowShared& sh, int g, int G)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int p = lane & 15, cg = lane >> 4;
+    const ColScal s = sh.cur;
+    const int64_t nt = cdiv(n, TB);
+    const int64_t J0 = (j + 1) / TB;
+    const int64_t nstrip = nt - J0;
+    const int64_t nsegmax = cdiv(nstrip, SEG);
+    // total items = sum_s (nstrip - s*SEG)
+    int64_t items = 0;
+    for (int64_t ss = 0; ss < nsegmax; ss++) items += nstrip - ss * SEG;
+
+    for (int64_t it = g; it < items; it += G) {
+        int64_t sidx = 0, rem = it, cnt = nstrip;
+        while (rem >= cnt) {
+            rem -= cnt;
+            sidx++;
+            cnt = nstrip - sidx * SEG;
+        }
+        const int64_t I = J0 + sidx * SEG + rem;
+        const int64_t Js = J0 + sidx * SEG;
+        const int64_t Je = min(Js + SEG - 1, I);
+        const int64_t rb = I * TB;
+        int64_t rows[4] = {rb + 2 * p, rb + 2 * p + 1, rb + 32 + 2 * p, rb + 33 + 2 * p};
+        double vr[4], racc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) vr[rr] = vval(A, lda, n, j, rows[rr], s);
+
+        for (int64_t J = Js; J <= Je; J++) {
+            const int64_t cb = J * TB + 16 * wave + 4 * cg;
+            double a[4][4];
+            double vc[4];
+#pragma unroll
+            for (int cc = 0; cc < 4; cc++) {
+                const int64_t c = cb + cc;
+                vc[cc] = vval(A, lda, n, j, c, s);
+                const bool cok = (c > j) && (c < n);
+                const double* col = A + c * lda;
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int64_t r = rows[2 * h];
+                    double x0 = 0.0, x1 = 0.0;
+                    if (cok && r + 1 < n) {
+                        double2 t = *reinterpret_cast<const double2*>(col + r);
+                        x0 = t.x;
+                        x1 = t.y;
+                    } else if (cok && r < n) {
+                        x0 = col[r];
+                    }
+                    a[2 * h][cc] = (r > c) ? x0 : 0.0;
+                    a[2 * h + 1][cc] = (r + 1 > c) ? x1 : 0.0;
+                }
+            }
+            double cp[4];
+#pragma unroll
+            for (int cc = 0; cc < 4; cc++) {
+                cp[cc] = 0.0;
+#pragma unroll
+                for (int rr = 0; rr < 4; rr++) {
+                    racc[rr] += a[rr][cc] * vc[cc];
+                    cp[cc] += a[rr][cc] * vr[rr];
+                }
+            }
+#pragma unroll
+            for (int cc = 0; cc < 4; cc++) {
+                double x = cp[cc];
+                x += __shfl_xor(x, 1, 64);
+                x += __shfl_xor(x, 2, 64);
+                x += __shfl_xor(x, 4, 64);
+                x += __shfl_xor(x, 8, 64);
+                cp[cc] = x;
+            }
+            if (p == 0) {
+#pragma unroll
+                for (int cc = 0; cc < 4; cc++) {
+                    const int64_t c = cb + cc;
+                    if (c > j && c < n) w.Pcol[I * w.ldp + c] = -cp[cc];
+                }
+            }
+        }
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) {
+            double x = racc[rr];
+            x += __shfl_xor(x, 16, 64);
+            x += __shfl_xor(x, 32, 64);
+            racc[rr] = x;
+        }
+        if (cg == 0) {
+            sh.red[wave][2 * p] = racc[0];
+            sh.red[wave][2 * p + 1] = racc[1];
+            sh.red[wave][32 + 2 * p] = racc[2];
+            sh.red[wave][33 + 2 * p] = racc[3];
+        }
+        __syncthreads();
+        if (wave == 0) {
+            const int64_t r = rb + lane;
+            if (r < n) w.Prow[sidx * w.ldp + r] = sh.red[0][lane] + sh.red[1][lane] + sh.red[2][lane] + sh.red[3][lane];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Skew rank-2k update of one BT x BT tile of the lower triangle on FP64 MFMA:
+//   A[r,c] += sum_k L[r,k] R[c,k]   for rows r in [rlo, n), cols c in [clo, n), r > c,
+//   L = [V U], R = [U -V], V[:,k] = A[:, c0+k] (reflectors, rows >= c0+k+1), U[:,k] = U[:, k].
+// v_mfma_f64_16x16x4f64 layout: a-operand lane l = (row l%16, k l/16); b-operand lane l =
+// (k l/16, col l%16); accumulator item q of lane l = (row 4*(l/16)+q, col l%16).
+// The 4 waves tile the 128x128 block as 2x2 sub-blocks of 64x64 (4x4 MFMA blocks each).
+// ---------------------------------------------------------------------------------------
+struct MfmaShared {
+    double L[KC * LDSLD];
+    double R[KC * LDSLD];
+};
+
+__device__ __forceinline__ void rank2k_tile(double* __restrict__ A, int64_t lda, const double* __restrict__ U,
+                                         int64_t ldu, int64_t n, int64_t rlo, int64_t clo, int64_t c0, int nbp,
+                                         int64_t r0, int64_t q0, MfmaShared& sm)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    // a wave whose whole 64x64 sub-block lies on or above the diagonal has nothing to do
+    const bool active = (r0 + wr * 64 + 63 > q0 + wc * 64);
+    const int lr = 4 * (lane >> 4), lc = lane & 15;
+    const int64_t crow = r0 + wr * 64 + lr;   // first row of this lane's accumulator items
+    const int64_t ccol = q0 + wc * 64 + lc;   // column of block bj = 0
+    v4d acc[4][4];
+#pragma unroll
+    for (int bi = 0; bi < 4; bi++) {
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++) {
+            const int64_t row = crow + bi * 16, col = ccol + bj * 16;
+            v4d c = {0.0, 0.0, 0.0, 0.0};
+            if (active && col >= clo && col < n) {
+                const double* p = A + row + col * lda;
+                if (row + 3 < n) {
+                    const double2 t0 = *reinterpret_cast<const double2*>(p);
+                    const double2 t1 = *reinterpret_cast<const double2*>(p + 2);
+                    c[0] = t0.x;
+                    c[1] = t0.y;
+                    c[2] = t1.x;
+                    c[3] = t1.y;
+                } else {
+                    if (row < n) c[0] = p[0];
+                    if (row + 1 < n) c[1] = p[1];
+                    if (row + 2 < n) c[2] = p[2];
+                }
+            }
+            acc[bi][bj] = c;
+        }
+    }
+    // staging: thread -> fixed row (tid & 127) of both operands, k = k0 + (tid >> 7) + 2u
+    const int srow = tid & (BT - 1), skk = tid >> 7;
+    const int64_t rowL = r0 + srow, rowR = q0 + srow;
+    const bool okL = rowL < n, okR = rowR < n;
+    const double* pVL = A + rowL + c0 * lda;   // V[rowL, k] = pVL[k*lda] when rowL > c0+k
+    const double* pVR = A + rowR + c0 * lda;
+    const double* pUL = U + rowL;
+    const double* pUR = U + rowR;
+    const int K = 2 * nbp;
+    for (int k0 = 0; k0 < K; k0 += KC) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < KC / 2; u++) {
+            const int kk = skk + 2 * u;
+            const int k = k0 + kk;
+            double lv = 0.0, rv = 0.0;
+            if (k < nbp) {
+                if (okL && rowL > c0 + k) lv = pVL[(int64_t)k * lda];
+                if (okR) rv = pUR[(int64_t)k * ldu];
+            } else if (k < K) {
+                const int kv = k - nbp;
+                if (okL) lv = pUL[(int64_t)kv * ldu];
+                if (okR && rowR > c0 + kv) rv = -pVR[(int64_t)kv * lda];
+            }
+            sm.L[kk * LDSLD + srow] = lv;
+            sm.R[kk * LDSLD + srow] = rv;
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int ks = 0; ks < KC / 4; ks++) {
+                const int kk = ks * 4 + (lane >> 4);
+                double af[4], bf[4];
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    af[b] = sm.L[kk * LDSLD + wr * 64 + b * 16 + lc];
+                    bf[b] = sm.R[kk * LDSLD + wc * 64 + b * 16 + lc];
+                }
+#pragma unroll
+                for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+                    for (int bj = 0; bj < 4; bj++)
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[bi], bf[bj], acc[bi][bj], 0, 0, 0);
+            }
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int bi = 0; bi < 4; bi++) {
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++) {
+            const int64_t row = crow + bi * 16, col = ccol + bj * 16;
+            if (col < clo || col >= n) continue;
+            double* p = A + row + col * lda;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int64_t rq = row + q;
+                if (rq > col && rq >= rlo && rq < n) p[q] = acc[bi][bj][q];
+            }
+        }
+    }
+}
+
+// Number of BT-tiles of the lower triangle for rows [rlo, n), cols [clo, n); tile origins are
+// absolute multiples of BT.  Tile t -> (bi, bj) with bj <= bi in the row-major triangle order
+// restricted to column blocks >= the first column block.
+struct TileGrid {
+    int64_t rb0, cb0, nrb;   // first row block, first column block, number of row blocks
+    int64_t total;
+};
+
+__host__ __device__ inline TileGrid tile_grid(int64_t n, int64_t rlo, int64_t clo)
+{
+    TileGrid tg;
+    tg.rb0 = rlo / BT;
+    tg.cb0 = clo / BT;
+    const int64_t rb_end = (n + BT - 1) / BT;
+    tg.nrb = rb_end - tg.rb0;
+    tg.total = 0;
+    for (int64_t bi = tg.rb0; bi < rb_end; bi++) tg.total += bi - tg.cb0 + 1;
+    return tg;
+}
+
+__device__ inline void tile_coords(const TileGrid& tg, int64_t t, int64_t& r0, int64_t& q0)
+{
+    // row block bi holds (bi - cb0 + 1) tiles
+    int64_t bi = tg.rb0, cnt = bi - tg.cb0 + 1;
+    while (t >= cnt) {
+        t -= cnt;
+        bi++;
+        cnt = bi - tg.cb0 + 1;
+    }
+    r0 = bi * BT;
+    q0 = (tg.cb0 + t) * BT;
+}
+
+}  // namespace skh

@@ -1,0 +1,58 @@
+// skh_internal.h -- structures shared between the kernels (skh_kernels.hip) and the C ABI
+// (skh_api.cpp).  Not part of the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "skh_device.hpp"
+
+struct PanelLaunch {
+    double* A;
+    int64_t lda;
+    int64_t n;
+    int64_t c0;
+    int nbp;
+    skh::Work w;
+    double* tau;
+    double* eta;
+    unsigned* bar;
+    unsigned base;
+    int* err;
+    int grid;
+};
+
+struct BatchArgs {
+    const double* A;       // packed input matrices (column-major, lda = n_b each)
+    const int64_t* ns;     // sizes
+    const int64_t* offs;   // element offsets into A
+    const int64_t* order;  // processing order (largest first)
+    int64_t batch;
+    double* slots;         // one work slot per workgroup
+    int64_t slot_stride;   // doubles per slot
+    int nb;                // panel width
+    unsigned* queue;       // work-queue counter (zeroed before each launch)
+    double* logpf;
+    double* sgn;
+};
+
+extern "C" {
+int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st);
+int skh_internal_panel_occupancy(int* blocks_per_cu);
+int skh_internal_launch_rank2k(double* A, int64_t lda, const double* U, int64_t ldu, int64_t n, int64_t rlo,
+                               int64_t clo, int64_t c0, int nbp, hipStream_t st);
+int skh_internal_launch_finish(double* A, int64_t lda, int64_t n, double* tau, double* eta, hipStream_t st);
+int skh_internal_zero_u32(unsigned* p, int count, hipStream_t st);
+int skh_internal_launch_batched(const BatchArgs* a, int grid, hipStream_t st);
+int skh_internal_batched_occupancy(int* blocks_per_cu);
+int skh_internal_mfma_probe(double* dout, hipStream_t st);
+}
+
+// doubles needed by one batched work slot for matrices up to nmax with panel width nb
+inline int64_t skh_slot_doubles(int64_t nmax, int nb)
+{
+    const int64_t ld = (nmax + 1) & ~int64_t(1);
+    const int64_t nt = (nmax + skh::TB - 1) / skh::TB + 1;
+    int64_t s = ld * nmax + ld * nb + 2 * ld + nt + nt * 2 * skh::NBMAX + 2 * skh::NBMAX;
+    s += ((nt + skh::SEG - 1) / skh::SEG + 1) * ld + nt * ld;
+    return (s + 31) & ~int64_t(31);   // keep every slot 256-byte aligned
+}

@@ -1,0 +1,376 @@
+"""Host-side mirror of SkewLinearAlgebra.jl's tridiagonalization API over libskewhip (gfx950).
+
+Names follow the reference (Julia ``f!`` -> Python ``f_``):
+
+  SkewHermitian, SkewHermTridiagonal            src/skewhermitian.jl:3-27, src/tridiag.jl:4-51
+  isskewhermitian, isskewsymmetric             src/skewhermitian.jl:84-111
+  skewhermitian, skewhermitian_                 src/skewhermitian.jl:35,119-132
+  hessenberg, hessenberg_                       src/hessenberg.jl:5-15
+  eigvals, eigvals_                             src/eigen.jl:2-16,43-45; src/tridiag.jl:446-460,484-504
+  pfaffian, pfaffian_                           src/pfaffian.jl:74-94,139-158
+  logabspfaffian, logabspfaffian_               src/pfaffian.jl:167-191,235-257
+  logabspfaffian_batched (no reference counterpart: the reference user loops)
+
+The tridiagonalization runs on the GPU (skh_sytrd_d); the O(n^2) implicit QR on the skew
+tridiagonal runs on the host in the same library (skh_skewtrieigvals_d), as in the reference.
+Errors: ArgumentError (a ValueError) for non-skew input, as the reference's ArgumentError.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import ENGINE_AUTO, ENGINE_LARGE, ENGINE_TEAM1, SkhError
+
+__all__ = [
+    "SkewHermitian", "SkewHermTridiagonal", "Hessenberg", "HessenbergQ", "ArgumentError",
+    "isskewhermitian", "isskewsymmetric", "skewhermitian", "skewhermitian_",
+    "hessenberg", "hessenberg_", "eigvals", "eigvals_", "pfaffian", "pfaffian_",
+    "logabspfaffian", "logabspfaffian_", "logabspfaffian_batched", "SkhError",
+    "ENGINE_AUTO", "ENGINE_LARGE", "ENGINE_TEAM1", "version",
+]
+
+
+class ArgumentError(ValueError):
+    """Julia's ArgumentError: invalid (e.g. non skew-symmetric) input."""
+
+
+def version():
+    return _lib.lib().skh_version().decode()
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+# ----------------------------------------------------------------------------------------
+# types
+# ----------------------------------------------------------------------------------------
+def isskewhermitian(A):
+    """A == -A^H exactly (src/skewhermitian.jl:84-94)."""
+    if isinstance(A, SkewHermitian):
+        return True
+    A = np.asarray(A)
+    if A.ndim != 2 or A.shape[0] != A.shape[1]:
+        raise ArgumentError("axes of A do not match: %s" % (A.shape,))
+    return bool(np.array_equal(A, -A.conj().T))
+
+
+def isskewsymmetric(A):
+    """A == -A^T exactly (src/skewhermitian.jl:101-109)."""
+    if isinstance(A, SkewHermitian) and not np.iscomplexobj(A.data):
+        return True
+    A = np.asarray(A)
+    if A.ndim != 2 or A.shape[0] != A.shape[1]:
+        raise ArgumentError("axes of A do not match: %s" % (A.shape,))
+    return bool(np.array_equal(A, -A.T))
+
+
+class SkewHermitian:
+    """Zero-copy wrapper of a skew-Hermitian matrix (both triangles stored), src/skewhermitian.jl:3-27."""
+
+    def __init__(self, A):
+        if isinstance(A, SkewHermitian):
+            self.data = A.data
+            return
+        A = np.asarray(A)
+        if not isskewhermitian(A):
+            raise ArgumentError("matrix `A` must be skew-Hermitian (equal `-A'`)")
+        self.data = A
+
+    @property
+    def shape(self):
+        return self.data.shape
+
+    def __array__(self, dtype=None, copy=None):
+        return np.asarray(self.data, dtype=dtype)
+
+    def copy(self):
+        S = SkewHermitian.__new__(SkewHermitian)
+        S.data = self.data.copy()
+        return S
+
+
+def skewhermitian_(A):
+    """In place (A - A^H)/2 (src/skewhermitian.jl:119-132); returns a SkewHermitian view."""
+    n = A.shape[0]
+    if A.shape != (n, n):
+        raise ArgumentError("matrix is not square")
+    for i in range(n):
+        A[i, i] = 0 if not np.iscomplexobj(A) else 1j * A[i, i].imag
+        a = (A[i, :i] - A[:i, i].conj()) / 2
+        A[i, :i] = a
+        A[:i, i] = -a.conj()
+    return SkewHermitian(A)
+
+
+def skewhermitian(A):
+    """(A - A^H)/2 as a new SkewHermitian (src/skewhermitian.jl:35); a number -> imag(a)."""
+    if np.isscalar(A):
+        return np.imag(A)
+    A = np.asarray(A)
+    dt = np.result_type(A.dtype, np.float64) if A.dtype.kind in "iub" else np.result_type(A.dtype, np.float32)
+    return skewhermitian_(np.array(A, dtype=dt, order="F", copy=True))
+
+
+class SkewHermTridiagonal:
+    """Skew-Hermitian tridiagonal: subdiagonal ev, optional imaginary diagonal dvim (src/tridiag.jl:4-51)."""
+
+    def __init__(self, ev, dvim=None):
+        self.ev = np.asarray(ev)
+        self.dvim = None if dvim is None else np.asarray(dvim, dtype=np.real(self.ev).dtype)
+        if self.dvim is not None and not np.iscomplexobj(self.ev):
+            raise ArgumentError("dvim requires a complex subdiagonal")
+
+    @property
+    def shape(self):
+        n = self.ev.size + 1
+        return (n, n)
+
+    def to_dense(self):
+        """Matrix(::SkewHermTridiagonal) (src/tridiag.jl:131-151)."""
+        n = self.ev.size + 1
+        dt = np.complex128 if (np.iscomplexobj(self.ev) or self.dvim is not None) else np.float64
+        M = np.zeros((n, n), dtype=dt)
+        idx = np.arange(n - 1)
+        M[idx + 1, idx] = self.ev
+        M[idx, idx + 1] = -np.conj(self.ev)
+        if self.dvim is not None:
+            M[np.arange(n), np.arange(n)] = 1j * self.dvim
+        return M
+
+    def copy(self):
+        return SkewHermTridiagonal(self.ev.copy(), None if self.dvim is None else self.dvim.copy())
+
+
+class HessenbergQ:
+    """Q = H_1 ... H_{n-1} from 'L'-layout factors and tau (LAPACK orgtr('L') convention)."""
+
+    def __init__(self, factors, tau):
+        self.factors = factors
+        self.tau = tau
+
+    def det(self):
+        """det(Q) for real reflectors: (-1)^{count(tau != 0)}."""
+        return -1.0 if int(np.count_nonzero(self.tau)) % 2 else 1.0
+
+
+class Hessenberg:
+    """Hessenberg(H::SkewHermTridiagonal, uplo='L', factors, tau) (src/hessenberg.jl:5-13)."""
+
+    def __init__(self, H, factors, tau):
+        self.H = H
+        self.uplo = "L"
+        self.factors = factors
+        self.tau = tau
+
+    @property
+    def Q(self):
+        return HessenbergQ(self.factors, self.tau)
+
+
+# ----------------------------------------------------------------------------------------
+# tridiagonalization
+# ----------------------------------------------------------------------------------------
+def _as_f64_skew(A, check):
+    if isinstance(A, SkewHermitian):
+        D = A.data
+    else:
+        D = np.asarray(A)
+        if check and not isskewhermitian(D):
+            raise ArgumentError("Pfaffian requires a skew-Hermitian matrix")
+    if np.iscomplexobj(D):
+        raise NotImplementedError("complex SkewHermitian input is not supported by this build")
+    return D
+
+
+def _sytrd_inplace(F, nb=0, engine=ENGINE_AUTO):
+    """F: float64 Fortran-ordered n x n, overwritten by the factors.  Returns (tau, eta, ntau)."""
+    n = F.shape[0]
+    tau = np.zeros(max(n - 1, 0))
+    eta = np.zeros(max(n - 1, 0))
+    nz = ctypes.c_int64(0)
+    o = _lib.opts(nb, engine)
+    rc = _lib.lib().skh_sytrd_d(n, _ptr(F), max(n, 1), _ptr(tau), _ptr(eta), ctypes.byref(nz), ctypes.byref(o))
+    _lib.check("skh_sytrd_d", rc)
+    return tau, eta, nz.value
+
+
+def hessenberg_(A, nb=0, engine=ENGINE_AUTO):
+    """hessenberg!(A::SkewHermitian{Float64}): A.data becomes the factors (src/hessenberg.jl:5-13)."""
+    S = A if isinstance(A, SkewHermitian) else SkewHermitian(A)
+    D = S.data
+    if np.iscomplexobj(D):
+        raise NotImplementedError("complex SkewHermitian input is not supported by this build")
+    if D.dtype != np.float64 or not D.flags.f_contiguous:
+        raise ArgumentError("hessenberg_ needs a Float64 column-major (Fortran-ordered) matrix; use hessenberg")
+    tau, eta, _ = _sytrd_inplace(D, nb, engine)
+    return Hessenberg(SkewHermTridiagonal(eta), D, tau)
+
+
+def hessenberg(A, nb=0, engine=ENGINE_AUTO):
+    """hessenberg(A::SkewHermitian) = hessenberg!(copyeigtype(A)) (src/hessenberg.jl:15)."""
+    D = A.data if isinstance(A, SkewHermitian) else np.asarray(A)
+    if not isinstance(A, SkewHermitian) and not isskewhermitian(D):
+        raise ArgumentError("matrix `A` must be skew-Hermitian (equal `-A'`)")
+    F = np.array(D, dtype=np.float64, order="F", copy=True)
+    S = SkewHermitian.__new__(SkewHermitian)
+    S.data = F
+    return hessenberg_(S, nb, engine)
+
+
+# ----------------------------------------------------------------------------------------
+# eigenvalues (host implicit QR on the tridiagonal, as the reference)
+# ----------------------------------------------------------------------------------------
+def _tri_eigvals_host(ev):
+    ev = np.array(ev, dtype=np.float64, copy=True)
+    n = ev.size + 1
+    vals = np.zeros(n)
+    rc = _lib.lib().skh_skewtrieigvals_d(n, _ptr(ev), _ptr(vals))
+    if rc == 2:
+        raise RuntimeError("Maximum number of iterations reached, the algorithm didn't converge")
+    _lib.check("skh_skewtrieigvals_d", rc)
+    return vals
+
+
+def eigvals_(A, sortby=None, nb=0, engine=ENGINE_AUTO):
+    """eigvals!(A::SkewHermitian{<:Real} | SkewHermTridiagonal{<:Real}, sortby) -> complex(0, vals)."""
+    if isinstance(A, SkewHermTridiagonal):
+        if np.iscomplexobj(A.ev) or A.dvim is not None:
+            raise NotImplementedError("complex SkewHermTridiagonal is not supported by this build")
+        vals = _tri_eigvals_host(A.ev)
+    else:
+        S = A if isinstance(A, SkewHermitian) else SkewHermitian(A)
+        n = S.data.shape[0]
+        if n == 1:
+            return np.array([S.data[0, 0]], dtype=np.complex128)   # src/eigen.jl:51
+        H = hessenberg_(S, nb, engine)
+        vals = _tri_eigvals_host(H.H.ev)
+    if sortby is not None:
+        vals = np.array(sorted(vals, key=sortby))
+    return vals * 1j
+
+
+def eigvals(A, sortby=None, nb=0, engine=ENGINE_AUTO):
+    """eigvals(A) = eigvals!(copyeigtype(A), sortby) (src/eigen.jl:43, src/tridiag.jl:484-489)."""
+    if isinstance(A, SkewHermTridiagonal):
+        return eigvals_(A.copy(), sortby)
+    D = A.data if isinstance(A, SkewHermitian) else np.asarray(A)
+    S = SkewHermitian(np.array(D, dtype=np.float64, order="F", copy=True))
+    return eigvals_(S, sortby, nb, engine)
+
+
+# ----------------------------------------------------------------------------------------
+# Pfaffians
+# ----------------------------------------------------------------------------------------
+def _tri_logpf(ev):
+    n = ev.size + 1
+    if n % 2 == 1:
+        return -np.inf, 0.0
+    lp, sg = 0.0, 1.0
+    for i in range(0, n - 1, 2):
+        lp += np.log(abs(ev[i]))
+        sg *= np.sign(-ev[i])
+    return float(lp), float(sg)
+
+
+def logabspfaffian_(A, nb=0, engine=ENGINE_AUTO):
+    """(log|Pf(A)|, sign); odd n -> (-Inf, 0).  A (SkewHermitian) becomes the factors."""
+    if isinstance(A, SkewHermTridiagonal):
+        return _tri_logpf(np.asarray(A.ev, dtype=np.float64))   # src/pfaffian.jl:181-191
+    S = A if isinstance(A, SkewHermitian) else None
+    if S is None:
+        D = np.asarray(A)
+        if np.iscomplexobj(D):
+            raise NotImplementedError("complex skew-symmetric Pfaffians are not supported by this build")
+        if not isskewhermitian(D):
+            raise ArgumentError("Pfaffian requires a skew-Hermitian matrix")
+        S = SkewHermitian(D)
+    n = S.data.shape[0]
+    if n % 2 == 1:
+        return -np.inf, 0.0
+    if S.data.dtype != np.float64 or not S.data.flags.f_contiguous:
+        return logabspfaffian(S, nb, engine)
+    tau, eta, nz = _sytrd_inplace(S.data, nb, engine)
+    lp, sg = _tri_logpf(eta)
+    return lp, (-sg if nz % 2 else sg)
+
+
+def logabspfaffian(A, nb=0, engine=ENGINE_AUTO):
+    """logabspfaffian(A) = logabspfaffian!(copyeigtype(A)) (src/pfaffian.jl:236,246)."""
+    if isinstance(A, SkewHermTridiagonal):
+        return logabspfaffian_(A)
+    D = A.data if isinstance(A, SkewHermitian) else np.asarray(A)
+    if not isinstance(A, SkewHermitian):
+        if np.iscomplexobj(D):
+            raise NotImplementedError("complex skew-symmetric Pfaffians are not supported by this build")
+        if not isskewhermitian(D):
+            raise ArgumentError("Pfaffian requires a skew-Hermitian matrix")
+    F = np.array(D, dtype=np.float64, order="F", copy=True)
+    n = F.shape[0]
+    if n % 2 == 1:
+        return -np.inf, 0.0
+    lp, sg = ctypes.c_double(), ctypes.c_double()
+    o = _lib.opts(nb, engine)
+    rc = _lib.lib().skh_logpf_d(n, _ptr(F), max(n, 1), ctypes.byref(lp), ctypes.byref(sg), ctypes.byref(o))
+    _lib.check("skh_logpf_d", rc)
+    return lp.value, sg.value
+
+
+def pfaffian_(A, nb=0, engine=ENGINE_AUTO):
+    """pfaffian! for real input (src/pfaffian.jl:74-94,155-158): prod(-ev_odd) * sign(det Q)."""
+    if isinstance(A, SkewHermTridiagonal):
+        ev = np.asarray(A.ev, dtype=np.float64)
+        n = ev.size + 1
+        if n % 2 == 1:
+            return 0.0
+        return float(np.prod(-ev[0::2]))
+    S = A if isinstance(A, SkewHermitian) else None
+    if S is None:
+        D = np.asarray(A)
+        if not isskewhermitian(D):
+            raise ArgumentError("Pfaffian requires a skew-symmetric matrix")
+        S = SkewHermitian(np.array(D, dtype=np.float64, order="F", copy=True))
+    n = S.data.shape[0]
+    if n % 2 == 1:
+        return 0.0
+    if S.data.dtype != np.float64 or not S.data.flags.f_contiguous:
+        S = SkewHermitian(np.array(S.data, dtype=np.float64, order="F", copy=True))
+    tau, eta, nz = _sytrd_inplace(S.data, nb, engine)
+    pf = 1.0
+    for i in range(0, n - 1, 2):
+        pf *= -eta[i]
+    return -pf if nz % 2 else pf
+
+
+def pfaffian(A, nb=0, engine=ENGINE_AUTO):
+    """pfaffian(A) = pfaffian!(copyeigtype(A)) (src/pfaffian.jl:148)."""
+    if isinstance(A, SkewHermTridiagonal):
+        return pfaffian_(A)
+    D = A.data if isinstance(A, SkewHermitian) else np.asarray(A)
+    if not isinstance(A, SkewHermitian) and not isskewhermitian(D):
+        raise ArgumentError("Pfaffian requires a skew-symmetric matrix")
+    return pfaffian_(SkewHermitian(np.array(D, dtype=np.float64, order="F", copy=True)), nb, engine)
+
+
+def logabspfaffian_batched(mats, nb=0, engine=ENGINE_AUTO):
+    """log|Pf| and sign of each matrix in `mats` (uneven sizes) in one persistent GPU launch.
+
+    No reference counterpart (the reference user loops logabspfaffian, src/pfaffian.jl:236).
+    Returns (logpf, sign) float64 arrays."""
+    mats = [np.asarray(M, dtype=np.float64) for M in mats]
+    for M in mats:
+        if M.ndim != 2 or M.shape[0] != M.shape[1]:
+            raise ArgumentError("each matrix must be square")
+    ns = np.array([M.shape[0] for M in mats], dtype=np.int64)
+    offs = np.zeros(len(mats), dtype=np.int64)
+    if len(mats) > 1:
+        offs[1:] = np.cumsum(ns[:-1] ** 2)
+    packed = np.concatenate([M.ravel(order="F") for M in mats]) if mats else np.zeros(1)
+    lp = np.zeros(len(mats))
+    sg = np.zeros(len(mats))
+    o = _lib.opts(nb, engine)
+    rc = _lib.lib().skh_logpf_batched_d(len(mats), _ptr(ns), _ptr(offs), _ptr(packed), _ptr(lp), _ptr(sg),
+                                        ctypes.byref(o))
+    _lib.check("skh_logpf_batched_d", rc)
+    return lp, sg

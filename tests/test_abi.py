@@ -1,0 +1,122 @@
+"""CPU-side checks of the C ABI: the library loads, exports every declared symbol, rejects bad
+arguments before touching a device, and its host eigensolver (the part of the path that stays
+on the CPU) agrees bit-for-bit with the oracle's restatement of skewtrieigvals!/skewtrieigen."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import skew_gauss
+
+skl = pytest.importorskip("skewlinearalgebra")
+from skewlinearalgebra import _lib  # noqa: E402
+
+
+def test_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = _lib.declared_symbols()
+    assert len(names) >= 12
+    for name in names:
+        assert hasattr(L, name), name
+    assert L.skh_version().startswith(b"skewhip")
+
+
+def _eig_host(ev):
+    ev = np.array(ev, dtype=np.float64, copy=True)
+    vals = np.zeros(ev.size + 1)
+    rc = _lib.lib().skh_skewtrieigvals_d(ev.size + 1, ev.ctypes.data, vals.ctypes.data)
+    assert rc == 0
+    return vals
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 10, 11, 64, 65, 300, 1001])
+def test_host_eigvals_bitwise_vs_oracle(n):
+    A = skew_gauss(n, 900 + n)
+    _, _, eta = oracle.skewblockedhess(A)
+    np.testing.assert_array_equal(_eig_host(eta), oracle.skewtrieigvals(eta))
+
+
+def test_host_eigvals_fixtures(golden):
+    # some of these reach the symmetric-tridiagonal fallback, where the oracle calls LAPACK
+    # dstev and the library its own QL solver: same eigenvalues, rounding may differ
+    for ev in golden["issue116_ev"] + golden["issue118_ev"] + [[3.0, 0.0, 5.0], [0.0, 1.0, 1.0]]:
+        np.testing.assert_allclose(_eig_host(ev), oracle.skewtrieigvals(ev), rtol=0, atol=1e-14)
+
+
+def test_host_eigvals_fallback_path():
+    """The stagnation detector (src/skeweigen.jl:132-137) hands over to a symmetric
+    tridiagonal solver; the eigenvalues must still be right (sorted comparison)."""
+    ev = np.array([2.0, 1e-11, 2.0, 1e-11, 2.0] * 3)
+    got = np.sort(_eig_host(ev))
+    T = np.diag(ev, -1) - np.diag(ev, 1)
+    ref = np.sort(np.linalg.eigvals(T).imag)
+    np.testing.assert_allclose(got, ref, atol=1e-12)
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("n", [2, 3, 4, 7, 8, 33, 64])
+def test_host_eigen_bitwise_vs_oracle(n, variant):
+    A = skew_gauss(n, 300 + n)
+    _, _, eta = oracle.skewblockedhess(A)
+    ev = eta.copy()
+    N = n
+    vals = np.zeros(N)
+    vr = np.zeros((N, N), order="F")
+    vi = np.zeros((N, N), order="F")
+    rc = _lib.lib().skh_skewtrieigen_d(N, ev.ctypes.data, vals.ctypes.data, vr.ctypes.data, vi.ctypes.data,
+                                       variant)
+    assert rc == 0
+    ov, oV = oracle.skewtrieigen(eta, "divided" if variant == 0 else "merged")
+    np.testing.assert_array_equal(vals, ov)
+    np.testing.assert_array_equal(vr + 1j * vi, oV)
+
+
+def test_host_eigen_fixtures(golden):
+    for ev in golden["issue116_ev"] + golden["issue118_ev"]:
+        ev = np.array(ev, float)
+        N = ev.size + 1
+        vals = np.zeros(N)
+        vr = np.zeros((N, N), order="F")
+        vi = np.zeros((N, N), order="F")
+        e2 = ev.copy()
+        assert _lib.lib().skh_skewtrieigen_d(N, e2.ctypes.data, vals.ctypes.data, vr.ctypes.data,
+                                             vi.ctypes.data, 1) == 0
+        V = vr + 1j * vi
+        T = np.diag(ev, -1) - np.diag(ev, 1)
+        assert np.linalg.norm(V @ np.diag(1j * vals) @ V.conj().T - T) <= 1e-9 * max(1, np.linalg.norm(T))
+
+
+def test_argument_errors_before_device():
+    L = _lib.lib()
+    o = _lib.opts()
+    tau = np.zeros(4)
+    assert L.skh_sytrd_d(-1, None, 1, None, None, None, ctypes.byref(o)) == -1
+    A = np.zeros((4, 4), order="F")
+    assert L.skh_sytrd_d(4, A.ctypes.data, 3, tau.ctypes.data, tau.ctypes.data, None, ctypes.byref(o)) == -3
+    bad = _lib.opts(nb=1000)
+    assert L.skh_sytrd_d(4, A.ctypes.data, 4, tau.ctypes.data, tau.ctypes.data, None, ctypes.byref(bad)) == -7
+    assert b"nb" in L.skh_last_error()
+    lp, sg = ctypes.c_double(), ctypes.c_double()
+    # odd n answers without a device (src/pfaffian.jl:169)
+    A3 = np.zeros((3, 3), order="F")
+    assert L.skh_logpf_d(3, A3.ctypes.data, 3, ctypes.byref(lp), ctypes.byref(sg), None) == 0
+    assert lp.value == -np.inf and sg.value == 0.0
+    assert L.skh_skewtrieigen_d(4, A.ctypes.data, tau.ctypes.data, A.ctypes.data, A.ctypes.data, 7) == -6
+
+
+def test_python_api_host_paths():
+    """Pieces of the Python mirror that need no device."""
+    with pytest.raises(skl.ArgumentError):
+        skl.SkewHermitian(np.ones((3, 3)))
+    with pytest.raises(skl.ArgumentError):
+        skl.logabspfaffian(np.ones((4, 4)))
+    assert skl.logabspfaffian(np.zeros((5, 5))) == (-np.inf, 0.0)
+    S = skl.skewhermitian(np.array([[1, 2], [3, 4]]))
+    np.testing.assert_array_equal(S.data, [[0.0, -0.5], [0.5, 0.0]])   # issue #98
+    T = skl.SkewHermTridiagonal(np.array([3.0, 4.0, 5.0]))
+    np.testing.assert_array_equal(T.to_dense(), [[0, -3, 0, 0], [3, 0, -4, 0], [0, 4, 0, -5], [0, 0, 5, 0]])
+    lp, sg = skl.logabspfaffian(T)
+    assert abs(sg * np.exp(lp) - skl.pfaffian(T)) < 1e-12 and skl.pfaffian(T) == 15.0
+    vals = skl.eigvals(T)
+    np.testing.assert_allclose(np.sort(vals.imag), np.sort(np.linalg.eigvals(T.to_dense()).imag), atol=1e-12)

@@ -1,0 +1,215 @@
+"""GPU parity of the tridiagonalization path against the CPU oracle (calls go through the C ABI).
+
+Bars (written in each test):
+  * eta vs the oracle's eta within 1e-11 * ||A||_F (same reflectors; blocking reorders sums)
+  * ||Q T Q^T - A||_F / ||A||_F <= 1e-13 and ||Q^T Q - I||_F <= 1e-13 * n
+  * eigenvalues (sorted) within 10 * n * eps * ||A||_2 of the oracle's
+  * log|Pf| relative error <= 1e-10 and the sign exactly equal
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import skew_gauss, sparse_to_dense
+
+pytestmark = pytest.mark.gpu
+
+skl = pytest.importorskip("skewlinearalgebra")
+from skewlinearalgebra import _lib  # noqa: E402
+
+EPS = np.finfo(float).eps
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def tri(eta):
+    return np.diag(eta, -1) - np.diag(eta, 1)
+
+
+def check_fact(A, F, tau, eta, tol=1e-13):
+    Q = oracle.form_q(F, tau)
+    n = A.shape[0]
+    nA = max(np.linalg.norm(A), 1e-300)
+    res = np.linalg.norm(Q @ tri(eta) @ Q.T - A) / nA
+    orth = np.linalg.norm(Q.T @ Q - np.eye(n))
+    assert res <= tol, res
+    assert orth <= tol * max(n, 1), orth
+    return Q
+
+
+def test_mfma_lane_layout():
+    out = np.zeros(768)
+    _lib.check("probe", _lib.lib().skh_diag_mfma_probe(out.ctypes.data))
+    lanes = np.arange(64)
+    d1 = out[:256].reshape(64, 4)
+    d2 = out[256:512].reshape(64, 4)
+    d3 = out[512:].reshape(64, 4)
+    rows = 4 * (lanes // 16)[:, None] + np.arange(4)[None, :]
+    cols = np.repeat((lanes % 16)[:, None], 4, axis=1)
+    np.testing.assert_array_equal(d1, rows)
+    np.testing.assert_array_equal(d2, cols)
+    Am = np.array([[np.sin(i * 4 + k) for k in range(4)] for i in range(16)])
+    Bm = np.array([[np.cos(k * 16 + j) for j in range(16)] for k in range(4)])
+    D = Am @ Bm
+    np.testing.assert_allclose(d3, D[rows, cols], rtol=1e-14, atol=1e-14)
+
+
+def gpu_sytrd(A, nb=0, engine=skl.ENGINE_LARGE):
+    F = np.array(A, dtype=np.float64, order="F", copy=True)
+    n = F.shape[0]
+    tau = np.zeros(max(n - 1, 0))
+    eta = np.zeros(max(n - 1, 0))
+    nz = ctypes.c_int64(0)
+    o = _lib.opts(nb, engine)
+    rc = _lib.lib().skh_sytrd_d(n, F.ctypes.data, n, tau.ctypes.data, eta.ctypes.data, ctypes.byref(nz),
+                                ctypes.byref(o))
+    _lib.check("skh_sytrd_d", rc)
+    return F, tau, eta, nz.value
+
+
+@pytest.mark.parametrize("nb", [8, 32, 64])
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 10, 63, 64, 65, 66, 127, 130, 200, 257])
+def test_sytrd_large_vs_oracle(n, nb):
+    say("sytrd large n=%d nb=%d" % (n, nb))
+    A = skew_gauss(n, 10 * n + nb)
+    F, tau, eta, nz = gpu_sytrd(A, nb)
+    Fo, tauo, etao = oracle.skewblockedhess(A)
+    nA = np.linalg.norm(A)
+    np.testing.assert_allclose(eta, etao, rtol=0, atol=1e-11 * nA)
+    np.testing.assert_allclose(tau, tauo, rtol=0, atol=1e-11)
+    assert nz == int(np.count_nonzero(tau))
+    check_fact(A, F, tau, eta)
+    # unit stored at the head of each reflector, last entry keeps eta (hessenberg.jl:27-29)
+    for i in range(n - 2):
+        assert F[i + 1, i] == 1.0
+    assert F[n - 1, n - 2] == eta[-1] and tau[-1] == 0.0
+    # the upper triangle and diagonal are left as given
+    iu = np.triu_indices(n)
+    np.testing.assert_array_equal(F[iu], A[iu])
+
+
+@pytest.mark.parametrize("n", [512, 1000, 2049])
+def test_sytrd_medium(n):
+    say("sytrd medium n=%d" % n)
+    A = skew_gauss(n, n)
+    F, tau, eta, nz = gpu_sytrd(A)
+    _, _, etao = oracle.skewblockedhess(A)
+    nA = np.linalg.norm(A)
+    np.testing.assert_allclose(eta, etao, rtol=0, atol=1e-11 * nA)
+    check_fact(A, F, tau, eta)
+
+
+def test_edge_conventions(golden):
+    say("edge conventions")
+    cases = [np.array(golden["issue121"]["A"], float), np.array(golden["readme"]["A"], float)]
+    A = np.zeros((4, 4))
+    A[1:, 0] = 1
+    A[0, 1:] = -1
+    cases.append(A)
+    A = np.zeros((6, 6))
+    A[1, 0], A[0, 1] = 3.0, -3.0        # alpha != 0, xnorm = 0 -> tau = 2
+    A[5, 2], A[2, 5] = 2.0, -2.0        # alpha = +0 with xnorm > 0 -> beta = +xnorm
+    cases.append(A)
+    cases.append(np.zeros((7, 7)))      # all-zero columns -> tau = 0, v = 0
+    for key in ("m10", "m66"):
+        cases.append(sparse_to_dense(golden["issue130"][key]))
+    for A in cases:
+        for nb in (1, 2, 3, 8):
+            F, tau, eta, nz = gpu_sytrd(A, nb)
+            Fo, tauo, etao = oracle.skewblockedhess(A)
+            np.testing.assert_allclose(eta, etao, rtol=0, atol=1e-12 * max(1.0, np.linalg.norm(A)))
+            np.testing.assert_array_equal(tau != 0, tauo != 0)
+            np.testing.assert_allclose(tau, tauo, rtol=0, atol=1e-12)
+            check_fact(A, F, tau, eta)
+    g = golden["readme"]
+    F, tau, eta, nz = gpu_sytrd(np.array(g["A"], float))
+    np.testing.assert_allclose(eta, g["eta"], rtol=1e-14)
+    np.testing.assert_allclose(oracle.form_q(F, tau), np.array(g["Q"]), atol=1e-14)
+
+
+@pytest.mark.parametrize("n", [2, 4, 10, 64, 130, 512])
+def test_logabspfaffian_vs_oracle(n):
+    say("logpf n=%d" % n)
+    A = skew_gauss(n, 500 + n)
+    lp, sg = skl.logabspfaffian(A)
+    lpo, sgo = oracle.logabspfaffian(A)
+    assert sg == sgo
+    assert abs(lp - lpo) <= 1e-10 * max(1.0, abs(lpo))
+
+
+def test_pfaffian_exact(golden):
+    say("exact pfaffians")
+    from fractions import Fraction
+    cases = golden["integer_pf"] + golden["analytic4"] + [golden["issue49"], golden["issue121"]]
+    for c in cases:
+        A = np.array(c["A"], float)
+        pf = Fraction(c["pfaffian"])
+        n = A.shape[0]
+        lp, sg = skl.logabspfaffian(A)
+        got = skl.pfaffian(A)
+        if n % 2 == 1:
+            assert (lp, sg, got) == (-np.inf, 0.0, 0.0)
+            continue
+        if pf == 0:
+            continue
+        assert sg == float(np.sign(float(pf)))
+        assert abs(lp - np.log(abs(float(pf)))) <= 1e-10 * max(1.0, abs(lp))
+        assert abs(got - float(pf)) <= 1e-10 * abs(float(pf))
+    for c in golden["constructed"]:
+        lp, sg = skl.logabspfaffian(np.array(c["A"]))
+        assert sg == c["sign"]
+        assert abs(lp - c["logpf"]) <= 1e-10 * max(1.0, abs(c["logpf"]))
+    assert skl.logabspfaffian(np.array(golden["readme"]["A"], float)) == (2.1972245773362196, -1.0)
+
+
+@pytest.mark.parametrize("n", [4, 11, 128, 513])
+def test_eigvals_vs_oracle(n):
+    say("eigvals n=%d" % n)
+    A = skew_gauss(n, 60 + n)
+    lam = np.sort(skl.eigvals(A).imag)
+    _, _, etao = oracle.skewblockedhess(A)
+    lamo = np.sort(oracle.skewtrieigvals(etao))
+    assert np.max(np.abs(lam - lamo)) <= 10 * n * EPS * np.linalg.norm(A, 2)
+
+
+def test_batched_vs_single():
+    say("batched logpf")
+    rng = np.random.default_rng(4)
+    sizes = [0, 1, 2, 3, 4, 6, 31, 32, 64, 66, 96, 128, 130, 200, 256, 257, 320, 512] + list(rng.integers(1, 40, 20) * 2)
+    mats = []
+    for k, n in enumerate(sizes):
+        B = rng.standard_normal((n, n)) / np.sqrt(max(n, 1))
+        mats.append(np.asfortranarray(B - B.T))
+    lp, sg = skl.logabspfaffian_batched(mats)
+    for k, M in enumerate(mats):
+        n = M.shape[0]
+        if n == 0:
+            assert (lp[k], sg[k]) == (0.0, 1.0)
+            continue
+        lpo, sgo = oracle.logabspfaffian(M)
+        assert sg[k] == sgo, (k, n)
+        if n % 2 == 1:
+            assert lp[k] == -np.inf
+        else:
+            assert abs(lp[k] - lpo) <= 1e-10 * max(1.0, abs(lpo)), (k, n, lp[k], lpo)
+
+
+def test_batched_nb_variants():
+    say("batched nb variants")
+    rng = np.random.default_rng(5)
+    mats = []
+    for n in (64, 100, 250):
+        B = rng.standard_normal((n, n))
+        mats.append(B - B.T)
+    ref = [oracle.logabspfaffian(M) for M in mats]
+    for nb in (4, 16, 33, 64):
+        lp, sg = skl.logabspfaffian_batched(mats, nb=nb)
+        for k in range(len(mats)):
+            assert sg[k] == ref[k][1]
+            assert abs(lp[k] - ref[k][0]) <= 1e-10 * abs(ref[k][0])
