@@ -80,6 +80,24 @@ int skh_logpf_batched_d_device(int64_t batch, const int64_t *n, const int64_t *o
                                double *dlogpf, double *dsgn, void *stream, const skh_opts *opts);
 
 /*
+ * Explicit Q = H_0 ... H_{n-2} (n x n) from the factors and tau of skh_sytrd_d (device memory).
+ * Replaces Matrix(F.Q) for F = hessenberg(A::SkewHermitian) (HessenbergQ ctor src/hessenberg.jl:3,
+ * used by skeweigen! src/eigen.jl:85; LAPACK orgtr('L') convention: the unit of each reflector is
+ * implicit).  Blocked compact-WY on the GPU.
+ */
+int skh_orgtr_d_device(int64_t n, const double *dF, int64_t ldf, const double *dtau, double *dQ, int64_t ldq,
+                       void *stream);
+
+/*
+ * Eigen-decomposition of a real skew-symmetric A (host memory, A not modified).
+ * Replaces skeweigen!/eigen!(A::SkewHermitian{<:Real}) (reference src/eigen.jl:75-97): GPU
+ * tridiagonalization and Q, host skewtrieigen_divided!, GPU back-transform Q*Qr, Q*Qim.
+ * vals: imaginary parts of the n eigenvalues (reference order); vr, vi: real and imaginary
+ * parts of the eigenvector matrix (n x n, column-major, leading dimension n).
+ */
+int skh_eigen_d(int64_t n, const double *A, int64_t lda, double *vals, double *vr, double *vi, const skh_opts *opts);
+
+/*
  * Host implicit-QR eigensolver of the real skew tridiagonal SkewHermTridiagonal(ev), n = size.
  * Replaces skewtrieigvals! (reference src/skeweigen.jl:105-156, backup :463-468): vals receives
  * the imaginary parts of the eigenvalues in the reference's order; ev (n-1) is overwritten.

@@ -15,6 +15,8 @@
 #include <string>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+
 #include "../../include/skewhip.h"
 #include "skh_internal.h"
 
@@ -52,7 +54,8 @@ struct Buf {
     }
 };
 
-Buf g_mat, g_ws, g_vec, g_ctl, g_bin, g_bout;
+Buf g_mat, g_ws, g_vec, g_ctl, g_bin, g_bout, g_q, g_ev;
+rocblas_handle g_blas = nullptr;
 
 int set_err(int code, const char* fmt, ...)
 {
@@ -284,6 +287,61 @@ int batched_run(int64_t batch, const int64_t* ns, const int64_t* offs, const dou
     return 0;
 }
 
+#define BLASCHK(expr)                                                                         \
+    do {                                                                                      \
+        rocblas_status _s = (expr);                                                           \
+        if (_s != rocblas_status_success) return set_err(SKH_ERR_HIP, "%s failed: %s", #expr, \
+                                                         rocblas_status_to_string(_s));       \
+    } while (0)
+
+int blas_on(hipStream_t st)
+{
+    if (!g_blas) BLASCHK(rocblas_create_handle(&g_blas));
+    BLASCHK(rocblas_set_stream(g_blas, st));
+    BLASCHK(rocblas_set_pointer_mode(g_blas, rocblas_pointer_mode_host));
+    return 0;
+}
+
+// Q (n x n, ldq) = H_0 ... H_{n-2} from factors F and tau (device), blocked compact WY.
+int orgtr_locked(int64_t n, const double* dF, int64_t ldf, const double* dtau, double* dQ, int64_t ldq, hipStream_t st)
+{
+    if (n <= 0) return 0;
+    if (n > 2147483647LL / 2) return set_err(SKH_ERR_UNSUPPORTED, "n too large for the BLAS interface");
+    HIPCHK((hipError_t)skh_internal_eye(dQ, ldq, n, st));
+    if (n <= 1) return 0;
+    int rc = blas_on(st);
+    if (rc) return rc;
+    const int64_t nb = 64;
+    const int64_t nref = n - 1;
+    // workspace: V (n x nb) | Y (nb x n) | Z (nb x n) | Gm (nb x nb) | T (nb x nb)
+    const size_t doubles = (size_t)(n * nb + 2 * nb * n + 2 * nb * nb + 64);
+    double* ws = (double*)g_ws.get(doubles * sizeof(double));
+    if (!ws) return set_err(SKH_ERR_HIP, "hipMalloc of orgtr workspace failed");
+    double* V = ws;
+    double* Y = V + n * nb;
+    double* Z = Y + nb * n;
+    double* Gm = Z + nb * n;
+    double* T = Gm + nb * nb;
+    const double one = 1.0, zero = 0.0, mone = -1.0;
+    const int64_t last = ((nref - 1) / nb) * nb;
+    for (int64_t c0 = last; c0 >= 0; c0 -= nb) {
+        const int nbp = (int)std::min<int64_t>(nb, nref - c0);
+        const int64_t m = n - c0 - 1;
+        HIPCHK((hipError_t)skh_internal_form_v(dF, ldf, n, c0, nbp, V, m, st));
+        BLASCHK(rocblas_dgemm(g_blas, rocblas_operation_transpose, rocblas_operation_none, nbp, nbp, (int)m, &one, V,
+                              (int)m, V, (int)m, &zero, Gm, nbp));
+        HIPCHK((hipError_t)skh_internal_larft(Gm, nbp, dtau + c0, nbp, T, nbp, st));
+        double* Q22 = dQ + (c0 + 1) + (c0 + 1) * ldq;
+        BLASCHK(rocblas_dgemm(g_blas, rocblas_operation_transpose, rocblas_operation_none, nbp, (int)m, (int)m, &one, V,
+                              (int)m, Q22, (int)ldq, &zero, Y, nbp));
+        BLASCHK(rocblas_dgemm(g_blas, rocblas_operation_none, rocblas_operation_none, nbp, (int)m, nbp, &one, T, nbp, Y,
+                              nbp, &zero, Z, nbp));
+        BLASCHK(rocblas_dgemm(g_blas, rocblas_operation_none, rocblas_operation_none, (int)m, (int)m, nbp, &mone, V,
+                              (int)m, Z, nbp, &one, Q22, (int)ldq));
+    }
+    return 0;
+}
+
 int check_barrier_flag()
 {
     Ctl ctl;
@@ -327,7 +385,9 @@ int skh_finalize(void)
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_inited) return 0;
     (void)hipStreamSynchronize(g_stream);
-    for (Buf* b : {&g_mat, &g_ws, &g_vec, &g_ctl, &g_bin, &g_bout}) b->release();
+    for (Buf* b : {&g_mat, &g_ws, &g_vec, &g_ctl, &g_bin, &g_bout, &g_q, &g_ev}) b->release();
+    if (g_blas) (void)rocblas_destroy_handle(g_blas);
+    g_blas = nullptr;
     (void)hipStreamDestroy(g_stream);
     g_stream = nullptr;
     g_inited = false;
@@ -489,6 +549,72 @@ int skh_logpf_batched_d(int64_t batch, const int64_t* n, const int64_t* offset, 
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(logpf, dout, batch * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(sgn, dout + batch, batch * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int skh_orgtr_d_device(int64_t n, const double* dF, int64_t ldf, const double* dtau, double* dQ, int64_t ldq,
+                       void* stream)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (n < 0) return set_err(-1, "n < 0");
+    if (n > 0 && (!dF || !dQ)) return set_err(-2, "NULL matrix");
+    if (ldf < std::max<int64_t>(n, 1)) return set_err(-3, "ldf < n");
+    if (n > 1 && !dtau) return set_err(-4, "tau NULL");
+    if (ldq < std::max<int64_t>(n, 1)) return set_err(-6, "ldq < n");
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    return orgtr_locked(n, dF, ldf, dtau, dQ, ldq, stream ? (hipStream_t)stream : g_stream);
+}
+
+int skh_eigen_d(int64_t n, const double* A, int64_t lda, double* vals, double* vr, double* vi, const skh_opts* opts)
+{
+    if (n < 0) return set_err(-1, "n < 0");
+    if (n > 0 && (!A || !vals || !vr || !vi)) return set_err(-2, "NULL argument");
+    if (lda < std::max<int64_t>(n, 1)) return set_err(-3, "lda < n");
+    if (n == 0) return 0;
+    if (n == 1) {   // src/eigen.jl:77-79
+        vals[0] = 0.0;
+        vr[0] = 1.0;
+        vi[0] = 0.0;
+        return 0;
+    }
+    std::vector<double> F((size_t)n * n), tau(n - 1), eta(n - 1);
+    for (int64_t c = 0; c < n; c++) std::memcpy(&F[c * n], A + c * lda, n * sizeof(double));
+    int rc = skh_sytrd_d(n, F.data(), n, tau.data(), eta.data(), nullptr, opts);
+    if (rc) return rc;
+    // host implicit QR with Givens accumulation (skewtrieigen_divided!, src/eigen.jl:87)
+    std::vector<double> hr((size_t)n * n), hi((size_t)n * n);
+    rc = skh_skewtrieigen_d(n, eta.data(), vals, hr.data(), hi.data(), 0);
+    if (rc) return set_err(rc > 0 ? SKH_ERR_NOCONV : rc, "skewtrieigen failed (%d)", rc);
+    std::lock_guard<std::mutex> lk(g_mu);
+    rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    hipStream_t st = g_stream;
+    const size_t nn = (size_t)n * n;
+    double* d = (double*)g_q.get(5 * nn * sizeof(double) + 2 * n * sizeof(double));
+    if (!d) return set_err(SKH_ERR_HIP, "hipMalloc failed for eigen n = %lld", (long long)n);
+    double* dF = d;
+    double* dQ = dF + nn;
+    double* dR = dQ + nn;
+    double* dI = dR + nn;
+    double* dO = dI + nn;
+    double* dtau = dO + nn;
+    HIPCHK(hipMemcpyAsync(dF, F.data(), nn * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dtau, tau.data(), (n - 1) * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dR, hr.data(), nn * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dI, hi.data(), nn * sizeof(double), hipMemcpyHostToDevice, st));
+    rc = orgtr_locked(n, dF, n, dtau, dQ, n, st);
+    if (rc) return rc;
+    // vectorsreal = Q * Qr, vectorsim = Q * Qim (src/eigen.jl:88-89)
+    const double one = 1.0, zero = 0.0;
+    BLASCHK(rocblas_dgemm(g_blas, rocblas_operation_none, rocblas_operation_none, (int)n, (int)n, (int)n, &one, dQ,
+                          (int)n, dR, (int)n, &zero, dO, (int)n));
+    HIPCHK(hipMemcpyAsync(vr, dO, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    BLASCHK(rocblas_dgemm(g_blas, rocblas_operation_none, rocblas_operation_none, (int)n, (int)n, (int)n, &one, dQ,
+                          (int)n, dI, (int)n, &zero, dO, (int)n));
+    HIPCHK(hipMemcpyAsync(vi, dO, nn * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return 0;
 }

@@ -45,6 +45,11 @@ int skh_internal_zero_u32(unsigned* p, int count, hipStream_t st);
 int skh_internal_launch_batched(const BatchArgs* a, int grid, hipStream_t st);
 int skh_internal_batched_occupancy(int* blocks_per_cu);
 int skh_internal_mfma_probe(double* dout, hipStream_t st);
+int skh_internal_eye(double* Q, int64_t ldq, int64_t n, hipStream_t st);
+int skh_internal_form_v(const double* F, int64_t ldf, int64_t n, int64_t c0, int nbp, double* V, int64_t ldv,
+                        hipStream_t st);
+int skh_internal_larft(const double* Gm, int64_t ldg, const double* tau, int nbp, double* T, int64_t ldt,
+                       hipStream_t st);
 }
 
 // doubles needed by one batched work slot for matrices up to nmax with panel width nb

@@ -12,6 +12,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "skh_device.hpp"
 #include "skh_internal.h"
 
@@ -223,6 +225,54 @@ __global__ __launch_bounds__(WG) void k_batched(BatchArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Explicit Q = H_0 ... H_{n-2} from the 'L'-layout factors (LAPACK orgtr('L') convention:
+// reflector k has v[k+1] = 1 implicit, v[k+2:] = F[k+2:, k]).  Blocked compact WY from the
+// last block to the first: Q22 <- Q22 - V (T (V^T Q22)), T from larft('F','C').
+// ---------------------------------------------------------------------------------------
+__global__ void k_eye(double* Q, int64_t ldq, int64_t n)
+{
+    const int64_t c = blockIdx.y;
+    for (int64_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+        Q[r + c * ldq] = (r == c) ? 1.0 : 0.0;
+}
+
+// V (m x nbp, rows c0+1..n-1) with the implicit unit and explicit zeros above it
+__global__ void k_form_v(const double* F, int64_t ldf, int64_t n, int64_t c0, int nbp, double* V, int64_t ldv)
+{
+    const int k = blockIdx.y;
+    const int64_t m = n - c0 - 1;
+    for (int64_t rr = blockIdx.x * blockDim.x + threadIdx.x; rr < m; rr += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = c0 + 1 + rr;
+        const int64_t head = c0 + k + 1;
+        double v = 0.0;
+        if (r == head) v = 1.0;
+        else if (r > head) v = F[r + (c0 + k) * ldf];
+        V[rr + k * ldv] = v;
+    }
+}
+
+// T (nbp x nbp, upper) from the Gram matrix Gm = V^T V and tau:
+// T[k,k] = tau_k, T[0:k,k] = -tau_k T[0:k,0:k] Gm[0:k,k]   (one workgroup)
+__global__ void k_larft(const double* Gm, int64_t ldg, const double* tau, int nbp, double* T, int64_t ldt)
+{
+    const int i = threadIdx.x;
+    for (int k = 0; k < nbp; k++) {
+        if (i < nbp) {
+            double v = 0.0;
+            if (i < k) {
+                double acc = 0.0;
+                for (int l = i; l < k; l++) acc += T[i + l * ldt] * Gm[l + k * ldg];
+                v = -tau[k] * acc;
+            } else if (i == k) {
+                v = tau[k];
+            }
+            T[i + k * ldt] = v;
+        }
+        __syncthreads();
+    }
+}
+
 // v_mfma_f64_16x16x4f64 lane-layout probe (diagnostics): D = A*B for three operand pairs.
 __global__ void k_mfma_probe(double* out)
 {
@@ -298,5 +348,28 @@ extern "C" int skh_internal_batched_occupancy(int* blocks_per_cu)
 extern "C" int skh_internal_mfma_probe(double* dout, hipStream_t st)
 {
     hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, st, dout);
+    return (int)hipGetLastError();
+}
+
+extern "C" int skh_internal_eye(double* Q, int64_t ldq, int64_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_eye, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 64), (unsigned)n), dim3(256), 0, st, Q,
+                       ldq, n);
+    return (int)hipGetLastError();
+}
+
+extern "C" int skh_internal_form_v(const double* F, int64_t ldf, int64_t n, int64_t c0, int nbp, double* V, int64_t ldv,
+                                   hipStream_t st)
+{
+    const int64_t m = n - c0 - 1;
+    hipLaunchKernelGGL(k_form_v, dim3((unsigned)std::min<int64_t>((m + 255) / 256, 64), (unsigned)nbp), dim3(256), 0,
+                       st, F, ldf, n, c0, nbp, V, ldv);
+    return (int)hipGetLastError();
+}
+
+extern "C" int skh_internal_larft(const double* Gm, int64_t ldg, const double* tau, int nbp, double* T, int64_t ldt,
+                                  hipStream_t st)
+{
+    hipLaunchKernelGGL(k_larft, dim3(1), dim3(128), 0, st, Gm, ldg, tau, nbp, T, ldt);
     return (int)hipGetLastError();
 }

@@ -25,7 +25,7 @@ from ._lib import ENGINE_AUTO, ENGINE_LARGE, ENGINE_TEAM1, SkhError
 __all__ = [
     "SkewHermitian", "SkewHermTridiagonal", "Hessenberg", "HessenbergQ", "ArgumentError",
     "isskewhermitian", "isskewsymmetric", "skewhermitian", "skewhermitian_",
-    "hessenberg", "hessenberg_", "eigvals", "eigvals_", "pfaffian", "pfaffian_",
+    "hessenberg", "hessenberg_", "eigvals", "eigvals_", "eigen", "eigen_", "Eigen", "pfaffian", "pfaffian_",
     "logabspfaffian", "logabspfaffian_", "logabspfaffian_batched", "SkhError",
     "ENGINE_AUTO", "ENGINE_LARGE", "ENGINE_TEAM1", "version",
 ]
@@ -258,6 +258,65 @@ def eigvals(A, sortby=None, nb=0, engine=ENGINE_AUTO):
     D = A.data if isinstance(A, SkewHermitian) else np.asarray(A)
     S = SkewHermitian(np.array(D, dtype=np.float64, order="F", copy=True))
     return eigvals_(S, sortby, nb, engine)
+
+
+class Eigen:
+    """LinearAlgebra.Eigen: values (complex, purely imaginary) and vectors (complex n x n)."""
+
+    def __init__(self, values, vectors):
+        self.values = values
+        self.vectors = vectors
+
+    def __iter__(self):
+        return iter((self.values, self.vectors))
+
+
+def _tri_eigen_host(ev, variant):
+    ev = np.array(ev, dtype=np.float64, copy=True)
+    n = ev.size + 1
+    vals = np.zeros(n)
+    vr = np.zeros((n, n), order="F")
+    vi = np.zeros((n, n), order="F")
+    rc = _lib.lib().skh_skewtrieigen_d(n, _ptr(ev), _ptr(vals), _ptr(vr), _ptr(vi), variant)
+    if rc == 2:
+        raise RuntimeError("Maximum number of iterations reached, the algorithm didn't converge")
+    _lib.check("skh_skewtrieigen_d", rc)
+    return vals, vr, vi
+
+
+def eigen_(A, nb=0, engine=ENGINE_AUTO):
+    """eigen!(A): SkewHermitian{<:Real} -> GPU tridiagonalization + Q + host skewtrieigen_divided!
+    + GPU back-transform (src/eigen.jl:75-97); SkewHermTridiagonal{<:Real} -> skewtrieigen_merged!
+    on the host (src/tridiag.jl:506)."""
+    if isinstance(A, SkewHermTridiagonal):
+        if np.iscomplexobj(A.ev) or A.dvim is not None:
+            raise NotImplementedError("complex SkewHermTridiagonal is not supported by this build")
+        vals, vr, vi = _tri_eigen_host(A.ev, 1)
+        return Eigen(1j * vals, vr + 1j * vi)
+    D = A.data if isinstance(A, SkewHermitian) else np.asarray(A)
+    if np.iscomplexobj(D):
+        raise NotImplementedError("complex SkewHermitian input is not supported by this build")
+    n = D.shape[0]
+    F = np.asfortranarray(D, dtype=np.float64)
+    vals = np.zeros(n)
+    vr = np.zeros((n, n), order="F")
+    vi = np.zeros((n, n), order="F")
+    o = _lib.opts(nb, engine)
+    rc = _lib.lib().skh_eigen_d(n, _ptr(F), max(n, 1), _ptr(vals), _ptr(vr), _ptr(vi), ctypes.byref(o))
+    if rc == 2:
+        raise RuntimeError("Maximum number of iterations reached, the algorithm didn't converge")
+    _lib.check("skh_eigen_d", rc)
+    return Eigen(1j * vals, vr + 1j * vi)
+
+
+def eigen(A, nb=0, engine=ENGINE_AUTO):
+    """eigen(A) = eigen!(copyeigtype(A)) (src/eigen.jl:108, src/tridiag.jl:523)."""
+    if isinstance(A, SkewHermTridiagonal):
+        return eigen_(A.copy())
+    D = A.data if isinstance(A, SkewHermitian) else np.asarray(A)
+    if not isinstance(A, SkewHermitian) and not isskewhermitian(D):
+        raise ArgumentError("matrix `A` must be skew-Hermitian (equal `-A'`)")
+    return eigen_(SkewHermitian(np.array(D, dtype=np.float64, order="F", copy=True)), nb, engine)
 
 
 # ----------------------------------------------------------------------------------------

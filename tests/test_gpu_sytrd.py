@@ -213,3 +213,46 @@ def test_batched_nb_variants():
         for k in range(len(mats)):
             assert sg[k] == ref[k][1]
             assert abs(lp[k] - ref[k][0]) <= 1e-10 * abs(ref[k][0])
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 8, 65, 130, 300])
+def test_eigen_vs_oracle(n):
+    say("eigen n=%d" % n)
+    A = skew_gauss(n, 700 + n)
+    E = skl.eigen(A)
+    X = E.vectors
+    lam = E.values
+    nA = np.linalg.norm(A)
+    assert np.linalg.norm(X @ np.diag(lam) @ X.conj().T - A) <= 1e-12 * n * nA
+    assert np.linalg.norm(X.conj().T @ X - np.eye(n)) <= 1e-12 * n
+    Fo, tauo, etao = oracle.skewblockedhess(A)
+    vo, Vo = oracle.skewtrieigen(etao, "divided")
+    np.testing.assert_allclose(np.sort(lam.imag), np.sort(vo), atol=10 * n * EPS * np.linalg.norm(A, 2))
+
+
+def test_eigen_issue130_118(golden):
+    say("eigen fixtures")
+    mats = [sparse_to_dense(golden["issue130"][k]) for k in ("m10", "m66")]
+    for ev in golden["issue118_ev"]:
+        mats.append(tri(np.array(ev)))
+    for A in mats:
+        E = skl.eigen(A)
+        X = E.vectors
+        assert np.linalg.norm(X @ np.diag(E.values) @ X.conj().T - A) <= 1e-9 * np.linalg.norm(A)
+
+
+def test_orgtr_device_vs_numpy():
+    say("orgtr")
+    import torch
+    n = 333
+    A = skew_gauss(n, 3)
+    F, tau, eta, _ = gpu_sytrd(A)
+    dF = torch.tensor(F.T.copy(), device="cuda")   # row-major (n x n) tensor of F^T = column-major F
+    dtau = torch.tensor(tau, device="cuda")
+    dQ = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    rc = _lib.lib().skh_orgtr_d_device(n, dF.data_ptr(), n, dtau.data_ptr(), dQ.data_ptr(), n,
+                                       torch.cuda.current_stream().cuda_stream)
+    _lib.check("skh_orgtr_d_device", rc)
+    torch.cuda.synchronize()
+    Q = dQ.cpu().numpy().T
+    np.testing.assert_allclose(Q, oracle.form_q(F, tau), atol=1e-13 * n)
