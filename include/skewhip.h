@@ -80,6 +80,19 @@ int skh_logpf_batched_d_device(int64_t batch, const int64_t *n, const int64_t *o
                                double *dlogpf, double *dsgn, void *stream, const skh_opts *opts);
 
 /*
+ * Batched log|Pf| + sign + gradient d log|Pf(A)| / dA = -1/2 A^{-1} (entrywise convention,
+ * i.e. the cotangent a ChainRulesCore rrule of logabspfaffian returns for a unit seed).
+ * No reference counterpart (the reference has no AD code; its CPU oracle for this quantity is
+ * inv(A::SkewHermitian), src/skewhermitian.jl:233).  grad uses the same packing as Apacked.
+ * A^{-1} = Q T^{-1} Q^T is formed from the stored reflectors with the closed-form inverse of the
+ * skew tridiagonal T.  Odd n: logpf = -Inf, sign 0, grad = 0.
+ */
+int skh_logpf_grad_batched_d(int64_t batch, const int64_t *n, const int64_t *offset, const double *Apacked,
+                             double *logpf, double *sgn, double *grad, const skh_opts *opts);
+int skh_logpf_grad_batched_d_device(int64_t batch, const int64_t *n, const int64_t *offset, const double *dA,
+                                    double *dlogpf, double *dsgn, double *dgrad, void *stream, const skh_opts *opts);
+
+/*
  * Explicit Q = H_0 ... H_{n-2} (n x n) from the factors and tau of skh_sytrd_d (device memory).
  * Replaces Matrix(F.Q) for F = hessenberg(A::SkewHermitian) (HessenbergQ ctor src/hessenberg.jl:3,
  * used by skeweigen! src/eigen.jl:85; LAPACK orgtr('L') convention: the unit of each reflector is

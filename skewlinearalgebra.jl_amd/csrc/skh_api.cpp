@@ -239,7 +239,8 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
 // Team-of-one engine (batched).  dA packed; ns/offs host arrays.
 // ----------------------------------------------------------------------------------------
 int batched_run(int64_t batch, const int64_t* ns, const int64_t* offs, const double* dA, double* dlogpf,
-                double* dsgn, const skh_opts* opts, hipStream_t st)
+                double* dsgn, const skh_opts* opts, hipStream_t st, double* dfac = nullptr, double* dtaueta = nullptr,
+                const int64_t* dtoffs = nullptr)
 {
     if (batch == 0) return 0;
     int64_t nmax = 0;
@@ -281,6 +282,9 @@ int batched_run(int64_t batch, const int64_t* ns, const int64_t* offs, const dou
     a.queue = ctl.queue;
     a.logpf = dlogpf;
     a.sgn = dsgn;
+    a.fac = dfac;
+    a.taueta = dtaueta;
+    a.toffs = dtoffs;
     HIPCHK((hipError_t)skh_internal_launch_batched(&a, grid, st));
     // the host copy of the control arrays must outlive the async copy
     HIPCHK(hipStreamSynchronize(st));
@@ -342,6 +346,115 @@ int orgtr_locked(int64_t n, const double* dF, int64_t ldf, const double* dtau, d
     return 0;
 }
 
+// ----------------------------------------------------------------------------------------
+// Gradient of log|Pf|: grad_b = -1/2 A_b^{-1} = -1/2 (K - K^T), K = Q_E C^{-1} Q_O^T, grouped by size.
+// dgrad first receives the factors from the forward kernel (same offsets as the input).
+// ----------------------------------------------------------------------------------------
+Buf g_gq, g_gk, g_gz, g_gv, g_gp, g_gte;
+
+int grad_run(int64_t batch, const int64_t* ns, const int64_t* offs, const double* dA, double* dlogpf, double* dsgn,
+             double* dgrad, const skh_opts* opts, hipStream_t st)
+{
+    if (batch == 0) return 0;
+    // tau|eta offsets
+    std::vector<int64_t> toffs(batch);
+    int64_t tt = 0;
+    for (int64_t b = 0; b < batch; b++) {
+        toffs[b] = tt;
+        tt += 2 * std::max<int64_t>(ns[b], 1);
+    }
+    double* dte = (double*)g_gte.get((size_t)(tt + 2 * batch + 16) * sizeof(double));
+    if (!dte) return set_err(SKH_ERR_HIP, "hipMalloc failed (tau/eta)");
+    int64_t* dtoffs = (int64_t*)(dte + tt);   // stored after the tau|eta area (int64 fits in a double slot)
+    HIPCHK(hipMemcpyAsync(dtoffs, toffs.data(), batch * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    int rc = batched_run(batch, ns, offs, dA, dlogpf, dsgn, opts, st, dgrad, dte, dtoffs);
+    if (rc) return rc;
+    rc = blas_on(st);
+    if (rc) return rc;
+
+    // group by size
+    std::vector<int64_t> idx(batch);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return ns[a] < ns[b]; });
+    const int64_t NB = 32;
+    size_t gi = 0;
+    while (gi < idx.size()) {
+        const int64_t n = ns[idx[gi]];
+        size_t gj = gi;
+        while (gj < idx.size() && ns[idx[gj]] == n) gj++;
+        if (n % 2 == 1 || n == 0) {
+            for (size_t q = gi; q < gj; q++)
+                if (n > 0) HIPCHK(hipMemsetAsync(dgrad + offs[idx[q]], 0, n * n * sizeof(double), st));
+            gi = gj;
+            continue;
+        }
+        // chunk so that every buffer stays below ~1.5 GB
+        const int64_t per = std::max<int64_t>(1, std::min<int64_t>(4096, (int64_t)(1.5e9 / (8.0 * n * n))));
+        for (size_t q0 = gi; q0 < gj; q0 += per) {
+            const int cnt = (int)std::min<int64_t>(per, (int64_t)(gj - q0));
+            const int64_t nn = n * n, h = n / 2;
+            double* Q = (double*)g_gq.get((size_t)cnt * nn * sizeof(double));
+            double* K = (double*)g_gk.get((size_t)cnt * nn * sizeof(double));
+            double* Zt = (double*)g_gz.get((size_t)cnt * (n * h + 2 * NB * n) * sizeof(double));
+            double* V = (double*)g_gv.get((size_t)cnt * (n * NB + 2 * NB * NB) * sizeof(double));
+            void** P = (void**)g_gp.get((size_t)cnt * 3 * sizeof(void*) + (size_t)cnt * sizeof(int64_t));
+            if (!Q || !K || !Zt || !V || !P) return set_err(SKH_ERR_HIP, "hipMalloc failed (gradient buffers)");
+            std::vector<void*> hp(3 * cnt);
+            std::vector<int64_t> htoffs(cnt);
+            for (int i = 0; i < cnt; i++) {
+                const int64_t b = idx[q0 + i];
+                hp[i] = dgrad + offs[b];            // factors in, gradient out
+                hp[cnt + i] = Q + (int64_t)i * nn;
+                hp[2 * cnt + i] = dgrad + offs[b];
+                htoffs[i] = toffs[b];
+            }
+            int64_t* gtoffs = (int64_t*)(P + 3 * cnt);
+            HIPCHK(hipMemcpyAsync(P, hp.data(), 3 * cnt * sizeof(void*), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(gtoffs, htoffs.data(), cnt * sizeof(int64_t), hipMemcpyHostToDevice, st));
+            double* const* Fp = (double* const*)P;
+            double* const* Qp = (double* const*)(P + cnt);
+            double* const* Gp = (double* const*)(P + 2 * cnt);
+            double* Y = Zt;   // Y and Z (NB x n each) live in the Zt area until the solve
+            double* Z = Zt + (int64_t)cnt * NB * n;
+            const int64_t vstride = n * NB, gstride = NB * NB;
+            double* Gm = V + (int64_t)cnt * vstride;
+            double* T = Gm + (int64_t)cnt * gstride;
+            HIPCHK((hipError_t)skh_internal_eye_batched((double* const*)Qp, n, cnt, st));
+            const double one = 1.0, zero = 0.0, mone = -1.0;
+            const int64_t nref = n - 1;
+            for (int64_t c0 = ((nref - 1) / NB) * NB; c0 >= 0; c0 -= NB) {
+                const int nbp = (int)std::min<int64_t>(NB, nref - c0);
+                const int64_t m = n - c0 - 1;
+                HIPCHK((hipError_t)skh_internal_form_v_batched((const double* const*)Fp, nullptr, n, c0, nbp, V, vstride,
+                                                              cnt, st));
+                BLASCHK(rocblas_dgemm_strided_batched(g_blas, rocblas_operation_transpose, rocblas_operation_none, nbp,
+                                                      nbp, (int)m, &one, V, (int)m, vstride, V, (int)m, vstride, &zero,
+                                                      Gm, nbp, gstride, cnt));
+                HIPCHK((hipError_t)skh_internal_larft_batched(Gm, gstride, dte, gtoffs, c0, nbp, T, gstride, cnt, st));
+                double* Q22 = Q + (c0 + 1) + (c0 + 1) * n;
+                BLASCHK(rocblas_dgemm_strided_batched(g_blas, rocblas_operation_transpose, rocblas_operation_none, nbp,
+                                                      (int)m, (int)m, &one, V, (int)m, vstride, Q22, (int)n, nn, &zero,
+                                                      Y, nbp, NB * n, cnt));
+                BLASCHK(rocblas_dgemm_strided_batched(g_blas, rocblas_operation_none, rocblas_operation_none, nbp,
+                                                      (int)m, nbp, &one, T, nbp, gstride, Y, nbp, NB * n, &zero, Z, nbp,
+                                                      NB * n, cnt));
+                BLASCHK(rocblas_dgemm_strided_batched(g_blas, rocblas_operation_none, rocblas_operation_none, (int)m,
+                                                      (int)m, nbp, &mone, V, (int)m, vstride, Z, nbp, NB * n, &one, Q22,
+                                                      (int)n, nn, cnt));
+            }
+            // Zt = (C^{-1} Q_O^T)^T, K = Q_E Zt^T, grad = -1/2 (K - K^T)
+            HIPCHK((hipError_t)skh_internal_bidiag_solve((double* const*)Qp, dte, gtoffs, n, Zt, n * h, cnt, st));
+            BLASCHK(rocblas_dgemm_strided_batched(g_blas, rocblas_operation_none, rocblas_operation_transpose, (int)n,
+                                                  (int)n, (int)h, &one, Q + n, (int)(2 * n), nn, Zt, (int)n, n * h, &zero,
+                                                  K, (int)n, nn, cnt));
+            HIPCHK((hipError_t)skh_internal_skew_out(K, nn, (double* const*)Gp, n, -0.5, cnt, st));
+            HIPCHK(hipStreamSynchronize(st));   // host pointer tables are reused by the next chunk
+        }
+        gi = gj;
+    }
+    return 0;
+}
+
 int check_barrier_flag()
 {
     Ctl ctl;
@@ -385,7 +498,9 @@ int skh_finalize(void)
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_inited) return 0;
     (void)hipStreamSynchronize(g_stream);
-    for (Buf* b : {&g_mat, &g_ws, &g_vec, &g_ctl, &g_bin, &g_bout, &g_q, &g_ev}) b->release();
+    for (Buf* b : {&g_mat, &g_ws, &g_vec, &g_ctl, &g_bin, &g_bout, &g_q, &g_ev, &g_gq, &g_gk, &g_gz, &g_gv, &g_gp,
+                   &g_gte})
+        b->release();
     if (g_blas) (void)rocblas_destroy_handle(g_blas);
     g_blas = nullptr;
     (void)hipStreamDestroy(g_stream);
@@ -615,6 +730,61 @@ int skh_eigen_d(int64_t n, const double* A, int64_t lda, double* vals, double* v
     BLASCHK(rocblas_dgemm(g_blas, rocblas_operation_none, rocblas_operation_none, (int)n, (int)n, (int)n, &one, dQ,
                           (int)n, dI, (int)n, &zero, dO, (int)n));
     HIPCHK(hipMemcpyAsync(vi, dO, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int skh_logpf_grad_batched_d_device(int64_t batch, const int64_t* n, const int64_t* offset, const double* dA,
+                                    double* dlogpf, double* dsgn, double* dgrad, void* stream, const skh_opts* opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (batch < 0) return set_err(-1, "batch < 0");
+    if (batch > 0 && (!n || !offset || !dA || !dlogpf || !dsgn || !dgrad)) return set_err(-2, "NULL argument");
+    for (int64_t b = 0; b < batch; b++) {
+        if (n[b] < 0) return set_err(-2, "n[%lld] < 0", (long long)b);
+        if (offset[b] < 0 || (offset[b] & 1)) return set_err(-3, "offset[%lld] must be >= 0 and even", (long long)b);
+        if (n[b] > 46340) return set_err(-2, "n[%lld] too large", (long long)b);
+    }
+    if (opts && (opts->nb < 0 || opts->nb > skh::NBMAX)) return set_err(-9, "nb out of range");
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    return grad_run(batch, n, offset, dA, dlogpf, dsgn, dgrad, opts, stream ? (hipStream_t)stream : g_stream);
+}
+
+int skh_logpf_grad_batched_d(int64_t batch, const int64_t* n, const int64_t* offset, const double* Apacked,
+                             double* logpf, double* sgn, double* grad, const skh_opts* opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (batch < 0) return set_err(-1, "batch < 0");
+    if (batch == 0) return 0;
+    if (!n || !offset || !Apacked || !logpf || !sgn || !grad) return set_err(-2, "NULL argument");
+    for (int64_t b = 0; b < batch; b++) {
+        if (n[b] < 0) return set_err(-2, "n[%lld] < 0", (long long)b);
+        if (offset[b] < 0) return set_err(-3, "offset[%lld] < 0", (long long)b);
+        if (n[b] > 46340) return set_err(-2, "n[%lld] too large", (long long)b);
+    }
+    if (opts && (opts->nb < 0 || opts->nb > skh::NBMAX)) return set_err(-8, "nb out of range");
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    hipStream_t st = g_stream;
+    std::vector<int64_t> offs2(batch);
+    int64_t tot2 = 0;
+    for (int64_t b = 0; b < batch; b++) {
+        offs2[b] = tot2;
+        tot2 += (n[b] * n[b] + 1) & ~int64_t(1);
+    }
+    double* dA = (double*)g_bin.get(std::max<int64_t>(tot2, 1) * sizeof(double));
+    double* dG = (double*)g_mat.get(std::max<int64_t>(tot2, 1) * sizeof(double));
+    double* dout = (double*)g_bout.get(2 * batch * sizeof(double));
+    if (!dA || !dG || !dout) return set_err(SKH_ERR_HIP, "hipMalloc of batched buffers failed");
+    for (int64_t b = 0; b < batch; b++)
+        HIPCHK(hipMemcpyAsync(dA + offs2[b], Apacked + offset[b], n[b] * n[b] * sizeof(double), hipMemcpyHostToDevice, st));
+    rc = grad_run(batch, n, offs2.data(), dA, dout, dout + batch, dG, opts, st);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(logpf, dout, batch * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(sgn, dout + batch, batch * sizeof(double), hipMemcpyDeviceToHost, st));
+    for (int64_t b = 0; b < batch; b++)
+        HIPCHK(hipMemcpyAsync(grad + offset[b], dG + offs2[b], n[b] * n[b] * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return 0;
 }

@@ -33,6 +33,9 @@ struct BatchArgs {
     unsigned* queue;       // work-queue counter (zeroed before each launch)
     double* logpf;
     double* sgn;
+    double* fac;           // optional: factors out (strictly lower, at offs[b], ld n_b)
+    double* taueta;        // optional: tau | eta out (2 n_b doubles at toffs[b])
+    const int64_t* toffs;
 };
 
 extern "C" {
@@ -50,6 +53,16 @@ int skh_internal_form_v(const double* F, int64_t ldf, int64_t n, int64_t c0, int
                         hipStream_t st);
 int skh_internal_larft(const double* Gm, int64_t ldg, const double* tau, int nbp, double* T, int64_t ldt,
                        hipStream_t st);
+// batched helpers of the gradient (matrix index = blockIdx.z / blockIdx.x as documented in the kernels)
+int skh_internal_eye_batched(double* const* Q, int64_t n, int count, hipStream_t st);
+int skh_internal_form_v_batched(const double* const* F, const int64_t* toffs_tau, int64_t n, int64_t c0, int nbp,
+                                double* V, int64_t vstride, int count, hipStream_t st);
+int skh_internal_larft_batched(const double* Gm, int64_t gstride, const double* taueta, const int64_t* toffs,
+                               int64_t c0, int nbp, double* T, int64_t tstride, int count, hipStream_t st);
+int skh_internal_bidiag_solve(double* const* Q, const double* taueta, const int64_t* toffs, int64_t n, double* Zt,
+                              int64_t zstride, int count, hipStream_t st);
+int skh_internal_skew_out(const double* K, int64_t kstride, double* const* G, int64_t n, double alpha, int count,
+                          hipStream_t st);
 }
 
 // doubles needed by one batched work slot for matrices up to nmax with panel width nb

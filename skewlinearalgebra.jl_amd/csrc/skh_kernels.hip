@@ -222,6 +222,17 @@ __global__ __launch_bounds__(WG) void k_batched(BatchArgs a)
         __syncthreads();
         tridiag_team1(M, ld, n, a.nb, w, tau, eta, sh, sm);
         logpf_reduce(eta, tau, n, a.logpf + b, a.sgn + b);
+        if (a.fac) {
+            double* dst = a.fac + a.offs[b];
+            for (int64_t c = 0; c < n; c++)
+                for (int64_t r = c + 1 + tid; r < n; r += WG) dst[r + c * n] = M[r + c * ld];
+            double* te = a.taueta + a.toffs[b];
+            for (int64_t i = tid; i < n - 1; i += WG) {
+                te[i] = tau[i];
+                te[n + i] = eta[i];
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -270,6 +281,96 @@ __global__ void k_larft(const double* Gm, int64_t ldg, const double* tau, int nb
             T[i + k * ldt] = v;
         }
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Batched pieces of the log|Pf| gradient  d log|Pf(A)| / dA = -1/2 A^{-1},
+//   A^{-1} = Q T^{-1} Q^T,  T^{-1} = P^T [[0, -C^{-T}], [C^{-1}, 0]] P,
+// with C (n/2 x n/2, lower bidiagonal) C[a,a] = -eta[2a], C[a,a-1] = eta[2a-1] the odd/even
+// coupling block of T.  Writing Q_O = Q[:, 0::2], Q_E = Q[:, 1::2] and K = Q_E C^{-1} Q_O^T,
+// A^{-1} = K - K^T.  Q comes from the factors (compact WY, library GEMMs); C^{-1} Q_O^T is a
+// forward substitution, K one GEMM.
+// ---------------------------------------------------------------------------------------
+__global__ void k_eye_batched(double* const* Q, int64_t n)
+{
+    double* q = Q[blockIdx.z];
+    const int64_t c = blockIdx.y;
+    for (int64_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+        q[r + c * n] = (r == c) ? 1.0 : 0.0;
+}
+
+__global__ void k_form_v_batched(const double* const* F, int64_t n, int64_t c0, int nbp, double* V, int64_t vstride)
+{
+    const int k = blockIdx.y;
+    const double* f = F[blockIdx.z];
+    double* v = V + (int64_t)blockIdx.z * vstride;
+    const int64_t m = n - c0 - 1;
+    for (int64_t rr = blockIdx.x * blockDim.x + threadIdx.x; rr < m; rr += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = c0 + 1 + rr, head = c0 + k + 1;
+        v[rr + k * m] = (r == head) ? 1.0 : ((r > head) ? f[r + (c0 + k) * n] : 0.0);
+    }
+}
+
+__global__ void k_larft_batched(const double* Gm, int64_t gstride, const double* taueta, const int64_t* toffs,
+                                int64_t c0, int nbp, double* T, int64_t tstride)
+{
+    const int z = blockIdx.x;
+    const double* g = Gm + (int64_t)z * gstride;
+    const double* tau = taueta + toffs[z] + c0;
+    double* t = T + (int64_t)z * tstride;
+    const int i = threadIdx.x;
+    for (int k = 0; k < nbp; k++) {
+        if (i < nbp) {
+            double v = 0.0;
+            if (i < k) {
+                double acc = 0.0;
+                for (int l = i; l < k; l++) acc += t[i + l * nbp] * g[l + k * nbp];
+                v = -tau[k] * acc;
+            } else if (i == k) {
+                v = tau[k];
+            }
+            t[i + k * nbp] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// Zt (n x n/2) = (C^{-1} Q_O^T)^T: for each row c of Q, forward substitution along a.
+__global__ void k_bidiag_solve(double* const* Q, const double* taueta, const int64_t* toffs, int64_t n, double* Zt,
+                               int64_t zstride)
+{
+    const int z = blockIdx.y;
+    const double* q = Q[z];
+    const double* eta = taueta + toffs[z] + n;
+    double* zt = Zt + (int64_t)z * zstride;
+    const int64_t h = n / 2;
+    for (int64_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        double prev = 0.0;
+        for (int64_t a = 0; a < h; a++) {
+            const double rhs = q[c + (2 * a) * n] - ((a > 0) ? eta[2 * a - 1] * prev : 0.0);
+            prev = rhs / (-eta[2 * a]);
+            zt[c + a * n] = prev;
+        }
+    }
+}
+
+// G = alpha (K - K^T) through 32x32 LDS tiles (grid.z = matrix)
+__global__ void k_skew_out(const double* K, int64_t kstride, double* const* G, int64_t n, double alpha)
+{
+    __shared__ double tile[32][33];
+    const double* k = K + (int64_t)blockIdx.z * kstride;
+    double* g = G[blockIdx.z];
+    const int64_t r0 = (int64_t)blockIdx.x * 32, c0 = (int64_t)blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 threads
+    for (int yy = ty; yy < 32; yy += 8) {
+        const int64_t r = c0 + tx, c = r0 + yy;   // K^T block: K[c0 + tx, r0 + yy]
+        tile[yy][tx] = (r < n && c < n) ? k[r + c * n] : 0.0;
+    }
+    __syncthreads();
+    for (int yy = ty; yy < 32; yy += 8) {
+        const int64_t r = r0 + tx, c = c0 + yy;
+        if (r < n && c < n) g[r + c * n] = alpha * (k[r + c * n] - tile[tx][yy]);
     }
 }
 
@@ -371,5 +472,46 @@ extern "C" int skh_internal_larft(const double* Gm, int64_t ldg, const double* t
                                   hipStream_t st)
 {
     hipLaunchKernelGGL(k_larft, dim3(1), dim3(128), 0, st, Gm, ldg, tau, nbp, T, ldt);
+    return (int)hipGetLastError();
+}
+
+extern "C" int skh_internal_eye_batched(double* const* Q, int64_t n, int count, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_eye_batched, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 16), (unsigned)n, (unsigned)count),
+                       dim3(256), 0, st, Q, n);
+    return (int)hipGetLastError();
+}
+
+extern "C" int skh_internal_form_v_batched(const double* const* F, const int64_t* /*unused*/, int64_t n, int64_t c0,
+                                           int nbp, double* V, int64_t vstride, int count, hipStream_t st)
+{
+    const int64_t m = n - c0 - 1;
+    hipLaunchKernelGGL(k_form_v_batched,
+                       dim3((unsigned)std::min<int64_t>((m + 255) / 256, 16), (unsigned)nbp, (unsigned)count), dim3(256),
+                       0, st, F, n, c0, nbp, V, vstride);
+    return (int)hipGetLastError();
+}
+
+extern "C" int skh_internal_larft_batched(const double* Gm, int64_t gstride, const double* taueta, const int64_t* toffs,
+                                          int64_t c0, int nbp, double* T, int64_t tstride, int count, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_larft_batched, dim3((unsigned)count), dim3(128), 0, st, Gm, gstride, taueta, toffs, c0, nbp, T,
+                       tstride);
+    return (int)hipGetLastError();
+}
+
+extern "C" int skh_internal_bidiag_solve(double* const* Q, const double* taueta, const int64_t* toffs, int64_t n,
+                                         double* Zt, int64_t zstride, int count, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_bidiag_solve, dim3((unsigned)std::max<int64_t>(1, (n + 63) / 64), (unsigned)count), dim3(64), 0,
+                       st, Q, taueta, toffs, n, Zt, zstride);
+    return (int)hipGetLastError();
+}
+
+extern "C" int skh_internal_skew_out(const double* K, int64_t kstride, double* const* G, int64_t n, double alpha,
+                                     int count, hipStream_t st)
+{
+    const unsigned t = (unsigned)((n + 31) / 32);
+    hipLaunchKernelGGL(k_skew_out, dim3(t, t, (unsigned)count), dim3(256), 0, st, K, kstride, G, n, alpha);
     return (int)hipGetLastError();
 }

@@ -26,7 +26,8 @@ __all__ = [
     "SkewHermitian", "SkewHermTridiagonal", "Hessenberg", "HessenbergQ", "ArgumentError",
     "isskewhermitian", "isskewsymmetric", "skewhermitian", "skewhermitian_",
     "hessenberg", "hessenberg_", "eigvals", "eigvals_", "eigen", "eigen_", "Eigen", "pfaffian", "pfaffian_",
-    "logabspfaffian", "logabspfaffian_", "logabspfaffian_batched", "SkhError",
+    "logabspfaffian", "logabspfaffian_", "logabspfaffian_batched", "logabspfaffian_grad_batched",
+    "LogAbsPfaffian", "SkhError",
     "ENGINE_AUTO", "ENGINE_LARGE", "ENGINE_TEAM1", "version",
 ]
 
@@ -433,3 +434,95 @@ def logabspfaffian_batched(mats, nb=0, engine=ENGINE_AUTO):
                                         ctypes.byref(o))
     _lib.check("skh_logpf_batched_d", rc)
     return lp, sg
+
+
+def _pack(mats):
+    mats = [np.asarray(M, dtype=np.float64) for M in mats]
+    for M in mats:
+        if M.ndim != 2 or M.shape[0] != M.shape[1]:
+            raise ArgumentError("each matrix must be square")
+        if not isskewhermitian(M):
+            raise ArgumentError("Pfaffian requires a skew-Hermitian matrix")
+    ns = np.array([M.shape[0] for M in mats], dtype=np.int64)
+    offs = np.zeros(len(mats), dtype=np.int64)
+    if len(mats) > 1:
+        offs[1:] = np.cumsum(ns[:-1] ** 2)
+    packed = np.concatenate([M.ravel(order="F") for M in mats]) if mats else np.zeros(1)
+    return mats, ns, offs, packed
+
+
+def logabspfaffian_grad_batched(mats, nb=0, engine=ENGINE_AUTO):
+    """(logpf, sign, grads) with grads[b] = d log|Pf(A_b)| / dA_b = -1/2 inv(A_b) (entrywise).
+
+    The reverse rule a ChainRulesCore rrule of logabspfaffian would return for a unit seed; the
+    reference has no AD code, its CPU counterpart is inv(A::SkewHermitian) (src/skewhermitian.jl:233)."""
+    mats, ns, offs, packed = _pack(mats)
+    lp = np.zeros(len(mats))
+    sg = np.zeros(len(mats))
+    g = np.zeros_like(packed)
+    o = _lib.opts(nb, engine)
+    rc = _lib.lib().skh_logpf_grad_batched_d(len(mats), _ptr(ns), _ptr(offs), _ptr(packed), _ptr(lp), _ptr(sg),
+                                             _ptr(g), ctypes.byref(o))
+    _lib.check("skh_logpf_grad_batched_d", rc)
+    grads = [g[offs[k]:offs[k] + ns[k] ** 2].reshape(ns[k], ns[k], order="F") for k in range(len(mats))]
+    return lp, sg, grads
+
+
+def _device_batch_call(fn_name, A, want_grad):
+    """A: torch float64 CUDA tensor (B, n, n) of skew matrices (row-major storage of A^T is
+    column-major storage of A, and A^T = -A, so the grad/factor layouts are handled below)."""
+    import torch
+    if A.dim() != 3 or A.shape[1] != A.shape[2] or A.dtype != torch.float64 or not A.is_cuda:
+        raise ArgumentError("expected a float64 CUDA tensor of shape (B, n, n)")
+    B, n = A.shape[0], A.shape[1]
+    # column-major view of each matrix = row-major storage of its transpose
+    At = A.transpose(1, 2).contiguous()
+    ns = np.full(B, n, dtype=np.int64)
+    offs = (np.arange(B, dtype=np.int64) * n * n)
+    if (n * n) % 2 == 1 and B > 1:
+        raise ArgumentError("device batches need an even n*n stride; pad or call per matrix")
+    lp = torch.empty(B, dtype=torch.float64, device=A.device)
+    sg = torch.empty(B, dtype=torch.float64, device=A.device)
+    st = torch.cuda.current_stream(A.device).cuda_stream
+    o = _lib.opts()
+    L = _lib.lib()
+    if want_grad:
+        gt = torch.empty_like(At)
+        rc = L.skh_logpf_grad_batched_d_device(B, _ptr(ns), _ptr(offs), At.data_ptr(), lp.data_ptr(), sg.data_ptr(),
+                                               gt.data_ptr(), st, ctypes.byref(o))
+        _lib.check("skh_logpf_grad_batched_d_device", rc)
+        return lp, sg, gt.transpose(1, 2)
+    rc = L.skh_logpf_batched_d_device(B, _ptr(ns), _ptr(offs), At.data_ptr(), lp.data_ptr(), sg.data_ptr(), st,
+                                      ctypes.byref(o))
+    _lib.check("skh_logpf_batched_d_device", rc)
+    return lp, sg, None
+
+
+def _make_autograd():
+    try:
+        import torch
+    except ImportError:   # pragma: no cover
+        return None
+
+    class LogAbsPfaffian(torch.autograd.Function):
+        """torch.autograd counterpart of a ChainRulesCore rrule for logabspfaffian.
+
+        forward(A) -> (logpf, sign) for a (B, n, n) float64 CUDA batch; backward returns
+        dlogpf[:, None, None] * (-1/2 A^{-1}), computed on the GPU from the stored reflectors."""
+
+        @staticmethod
+        def forward(ctx, A):
+            lp, sg, g = _device_batch_call("grad", A, True)
+            ctx.save_for_backward(g)
+            ctx.mark_non_differentiable(sg)
+            return lp, sg
+
+        @staticmethod
+        def backward(ctx, dlp, dsg):
+            (g,) = ctx.saved_tensors
+            return dlp[:, None, None] * g
+
+    return LogAbsPfaffian
+
+
+LogAbsPfaffian = _make_autograd()

@@ -54,6 +54,8 @@ def lib():
         "skh_diag_timings": ([vp], ctypes.c_int),
         "skh_orgtr_d_device": ([i64, vp, i64, vp, vp, i64, vp], ctypes.c_int),
         "skh_eigen_d": ([i64, vp, i64, vp, vp, vp, op], ctypes.c_int),
+        "skh_logpf_grad_batched_d": ([i64, vp, vp, vp, vp, vp, vp, op], ctypes.c_int),
+        "skh_logpf_grad_batched_d_device": ([i64, vp, vp, vp, vp, vp, vp, vp, op], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
