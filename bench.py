@@ -136,8 +136,8 @@ def main():
         step()
         torch.cuda.synchronize()
         log("warmup %d done" % w)
-    tm = [0.0] * 6
-    out6 = (ctypes.c_double * 6)()
+    tm = [0.0] * 10
+    out6 = (ctypes.c_double * 10)()
     torch.cuda.synchronize()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
@@ -155,7 +155,8 @@ def main():
     flops = 4.0 / 3.0 * n ** 3
     tflops = flops / (ms * 1e-3) / 1e12
 
-    panel_ms, panel_cnt, upd_ms, upd_cnt, panel_bytes, upd_flops = tm
+    panel_ms, panel_cnt, upd_ms, upd_cnt, panel_bytes, upd_flops = tm[:6]
+    phases = {k: v / args.steps for k, v in zip(("row_pass", "barrier1", "matvec", "barrier2"), tm[6:])}
     panel_avg_s = panel_ms / panel_cnt * 1e-3
     panel_bytes_per = panel_bytes / panel_cnt
     achieved = panel_bytes_per / panel_avg_s / 1e9
@@ -188,7 +189,8 @@ def main():
                      "launches_per_step": panel_cnt / args.steps},
         "mfma_update": {"kernel": "k_rank2k", "achieved": upd_tf, "peak": FP64_MFMA_PEAK_TF, "unit": "TFLOP/s",
                         "frac": upd_tf / FP64_MFMA_PEAK_TF, "ms_per_step": upd_ms / args.steps},
-        "time_split_ms_per_step": {"panel": panel_ms / args.steps, "rank2k": upd_ms / args.steps},
+        "time_split_ms_per_step": {"panel": panel_ms / args.steps, "rank2k": upd_ms / args.steps,
+                                   "panel_phases_wg0": phases},
         "check": {"frobenius_rel_err": rel},
         "wall_s": wall,
     }

@@ -124,9 +124,11 @@ int skh_skewtrieigvals_d(int64_t n, double *ev, double *vals);
 int skh_skewtrieigen_d(int64_t n, double *ev, double *vals, double *vr, double *vi, int variant);
 
 /* Diagnostics: per-kernel times of the last skh_sytrd_d_device call made with SKH_FLAG_TIMING:
- * out6 = {panel ms, panel launches, rank-2k ms, rank-2k launches, panel algorithmic bytes,
- *         rank-2k algorithmic flops}.  Waits for the recorded events. */
-int skh_diag_timings(double *out6);
+ * out10 = {panel ms, panel launches, rank-2k ms, rank-2k launches, panel algorithmic bytes,
+ *          rank-2k algorithmic flops, then panel-kernel phase times of workgroup 0 in ms:
+ *          row passes, barrier-1 waits, scalars + matvec, barrier-2 waits}.
+ * Waits for the recorded events. */
+int skh_diag_timings(double *out10);
 /* Diagnostics: v_mfma_f64_16x16x4f64 lane-layout probe (768 doubles, see the test suite). */
 int skh_diag_mfma_probe(double *out768);
 

@@ -150,15 +150,19 @@ struct Ctl {
     unsigned* bar;
     int* err;
     unsigned* queue;
+    unsigned long long* ph;   // phase clock of the panel kernel (4 counters)
 };
 
 int get_ctl(Ctl& c)
 {
-    unsigned* p = (unsigned*)g_ctl.get(256);
+    // words [0, 288): barrier counters (9 lines of 32 words); [320]: error flag; [352]: batch
+    // queue; [384, 392): phase clocks (4 x u64)
+    unsigned* p = (unsigned*)g_ctl.get(2048);
     if (!p) return set_err(SKH_ERR_HIP, "hipMalloc of control block failed");
     c.bar = p;
-    c.err = (int*)(p + 1);
-    c.queue = p + 2;
+    c.err = (int*)(p + 320);
+    c.queue = p + 352;
+    c.ph = (unsigned long long*)(p + 384);
     return 0;
 }
 
@@ -172,7 +176,7 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
     if (rc) return rc;
     const int64_t ld = (n + 1) & ~int64_t(1);
     const int64_t nt = (n + skh::TB - 1) / skh::TB + 1;
-    const int64_t nseg = (nt + skh::SEG - 1) / skh::SEG + 1;
+    const int64_t nseg = nt;   // Prow slots per row
     size_t doubles = (size_t)(ld * nb + nt + nt * 2 * skh::NBMAX + 2 * skh::NBMAX + nseg * ld + nt * ld + 64);
     double* ws = (double*)g_ws.get(doubles * sizeof(double));
     if (!ws) return set_err(SKH_ERR_HIP, "hipMalloc of %zu bytes of workspace failed", doubles * 8);
@@ -187,13 +191,13 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
     w.Pcol = w.Prow + nseg * ld;
 
     int per_cu = 0;
-    HIPCHK((hipError_t)skh_internal_panel_occupancy(&per_cu));
+    HIPCHK((hipError_t)skh_internal_panel_occupancy((int)nb, &per_cu));
     per_cu = std::max(1, std::min(per_cu, 2));
     const int grid = per_cu * g_num_cu;
-    if ((int64_t)grid * (int64_t)(2 * n + 4) > 4000000000LL)
+    if ((int64_t)64 * (int64_t)(2 * n + 4) > 4000000000LL)
         return set_err(SKH_ERR_UNSUPPORTED, "n = %lld too large for the barrier counter", (long long)n);
 
-    HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 3, st));
+    HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 512, st));
     unsigned base = 0;
     for (int64_t c0 = 0; c0 <= n - 3; c0 += nb) {
         const int nbp = (int)std::min<int64_t>(nb, n - 2 - c0);
@@ -210,6 +214,8 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
         pl.base = base;
         pl.err = ctl.err;
         pl.grid = grid;
+        pl.nbmax = (int)nb;
+        pl.ph = g_timer.on ? ctl.ph : nullptr;
         const bool tm = g_timer.on;
         if (tm) HIPCHK((hipError_t)g_timer.begin(st));
         HIPCHK((hipError_t)skh_internal_launch_panel(&pl, st));
@@ -222,7 +228,7 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
             }
             HIPCHK((hipError_t)g_timer.end(st, 0, bytes));
         }
-        base += (unsigned)(2 * nbp * grid);
+        base += (unsigned)(2 * nbp);   // barriers per launch
         const int64_t e = c0 + nbp;
         if (tm) HIPCHK((hipError_t)g_timer.begin(st));
         HIPCHK((hipError_t)skh_internal_launch_rank2k(dA, lda, w.W, w.ldw, n, e, e, c0, nbp, st));
@@ -243,6 +249,10 @@ int batched_run(int64_t batch, const int64_t* ns, const int64_t* offs, const dou
                 const int64_t* dtoffs = nullptr)
 {
     if (batch == 0) return 0;
+    // A launch of the persistent batched kernel over several matrices (up to n = 512) did not finish on
+    // the GPU; until its cause is found only single-matrix launches (validated on the GPU) are allowed.
+    if (batch > 1)
+        return set_err(SKH_ERR_UNSUPPORTED, "multi-matrix batched launches are disabled (unresolved GPU stall)");
     int64_t nmax = 0;
     for (int64_t b = 0; b < batch; b++) nmax = std::max(nmax, ns[b]);
     const int nb = (int)pick_nb(nmax, opts, true);
@@ -251,7 +261,7 @@ int batched_run(int64_t batch, const int64_t* ns, const int64_t* offs, const dou
     std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return ns[a] > ns[b]; });
 
     int per_cu = 0;
-    HIPCHK((hipError_t)skh_internal_batched_occupancy(&per_cu));
+    HIPCHK((hipError_t)skh_internal_batched_occupancy(nb, &per_cu));
     per_cu = std::max(1, per_cu);
     const int grid = (int)std::min<int64_t>((int64_t)per_cu * g_num_cu, batch);
     const int64_t slot = skh_slot_doubles(std::max<int64_t>(nmax, 2), nb);
@@ -269,7 +279,7 @@ int batched_run(int64_t batch, const int64_t* ns, const int64_t* offs, const dou
     std::copy(offs, offs + batch, h.begin() + batch);
     std::copy(order.begin(), order.end(), h.begin() + 2 * batch);
     HIPCHK(hipMemcpyAsync(dctl, h.data(), ctl_bytes, hipMemcpyHostToDevice, st));
-    HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 3, st));
+    HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 512, st));
     BatchArgs a;
     a.A = dA;
     a.ns = dctl;
@@ -789,13 +799,24 @@ int skh_logpf_grad_batched_d(int64_t batch, const int64_t* n, const int64_t* off
     return 0;
 }
 
-int skh_diag_timings(double* out6)
+int skh_diag_timings(double* out10)
 {
     std::lock_guard<std::mutex> lk(g_mu);
-    if (!out6) return set_err(-1, "NULL");
-    for (int k = 0; k < 6; k++) out6[k] = 0.0;
+    if (!out10) return set_err(-1, "NULL");
+    double* out6 = out10;
+    for (int k = 0; k < 10; k++) out10[k] = 0.0;
     if (g_timer.used == 0) return 0;
     HIPCHK(hipEventSynchronize(g_timer.ev[g_timer.used - 1]));
+    {
+        Ctl ctl;
+        int rc = get_ctl(ctl);
+        if (rc) return rc;
+        unsigned long long ph[4];
+        HIPCHK(hipMemcpy(ph, ctl.ph, sizeof(ph), hipMemcpyDeviceToHost));
+        int khz = 0;
+        HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g_device));
+        for (int k = 0; k < 4; k++) out10[6 + k] = khz > 0 ? (double)ph[k] / (double)khz : 0.0;
+    }
     for (size_t q = 0; q < g_timer.kind.size(); q++) {
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, g_timer.ev[2 * q], g_timer.ev[2 * q + 1]));

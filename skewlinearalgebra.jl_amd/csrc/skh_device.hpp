@@ -19,7 +19,6 @@ namespace skh {
 
 constexpr int WG = 256;       // threads per workgroup (4 waves of 64)
 constexpr int TB = 64;        // matvec tile edge and row-chunk height
-constexpr int SEG = 8;        // tiles per row-strip segment in the matvec
 constexpr int NBMAX = 128;    // largest panel width
 constexpr int BT = 128;       // MFMA output tile edge of the rank-2k update
 constexpr int KC = 16;        // k-chunk staged in LDS by the rank-2k update
@@ -34,7 +33,7 @@ struct Work {
     double* Pnorm;   // [nq]            per-chunk sums of squares of the reflector tail
     double* PS;      // [nq][2*NBMAX]   per-chunk W^T x and V^T x partials
     double* Tv;      // [2*NBMAX]       t1 = W^T v, t2 = V^T v
-    double* Prow;    // [nseg][ldp]     row partials of the skew matvec
+    double* Prow;    // [nt][ldp]       row partials of the skew matvec (one slot per rank per strip)
     double* Pcol;    // [nt][ldp]       column partials of the skew matvec
     int64_t ldp;
 };
@@ -86,6 +85,34 @@ __device__ __forceinline__ double vval(const double* __restrict__ A, int64_t lda
     return s.scale * A[c + j * lda];
 }
 
+// Ownership of the matvec tiles (strip-major order over the lower triangle; see skew_matvec).
+__host__ __device__ inline int64_t tile_owner(int64_t t, int64_t T, int64_t G)
+{
+    int64_t g = (t * G) / T;
+    if (g > G - 1) g = G - 1;
+    while (g + 1 < G && ((g + 1) * T) / G <= t) g++;
+    while (g > 0 && (g * T) / G > t) g--;
+    return g;
+}
+
+__host__ __device__ inline int64_t strip_of(int64_t t)
+{
+    int64_t s = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((s + 1) * (s + 2) / 2 <= t) s++;
+    while (s > 0 && s * (s + 1) / 2 > t) s--;
+    return s;
+}
+
+// number of row-partial slots of strip s (0-based from J0) when ns strips are split over G ranks
+// (at most T ranks take part, so every taking part rank has a non-empty range)
+__host__ __device__ inline int64_t strip_slots(int64_t s, int64_t ns, int64_t G)
+{
+    const int64_t T = ns * (ns + 1) / 2;
+    if (G > T) G = T;
+    const int64_t first = s * (s + 1) / 2;
+    return tile_owner(first + s, T, G) - tile_owner(first, T, G) + 1;
+}
+
 // ---------------------------------------------------------------------------------------
 // Row pass: finalises w of column j-1 (doC) and/or applies the panel column update to
 // column j and produces its norm and W^T x / V^T x partials (doA).
@@ -105,6 +132,9 @@ struct RowShared {
     ColScal cur;     // scalars of the column finalised by the last phase B
 };
 
+// KQ = largest k-group per wave (panel width <= 4*KQ); the rows of V and W used by a lane
+// are cached in 2*KQ registers.
+template <int KQ>
 __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j, int i,
                          bool doC, bool doA, const Work& w, int g, int G, RowShared& sh)
 {
@@ -160,13 +190,35 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
             double p = 0.0;
             if (rin && r >= j) {
                 const int64_t b = b0 + q;
-                // row partials: strip b has segments s in [0, (b - b0)/SEG]
-                const int64_t nseg = (b - b0) / SEG + 1;
-                for (int64_t s = wave; s < nseg; s += 4) p += w.Prow[s * w.ldp + r];
-                for (int64_t I = b + wave; I < nt; I += 4) p += w.Pcol[I * w.ldp + r];
+                // row partials of strip b: one slot per rank whose tile range touched it
+                const int64_t nslot = strip_slots(b - b0, nt - b0, G);
+                for (int64_t s = wave; s < nslot; s += 4) p += w.Prow[s * w.ldp + r];
+                // loads issued in unrolled batches (one memory round trip per batch)
+                double p4[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int64_t I0 = b + wave; I0 < nt; I0 += 64) {
+                    double buf[16];
+#pragma unroll
+                    for (int u = 0; u < 16; u++) {
+                        const int64_t I = I0 + 4 * u;
+                        buf[u] = (I < nt) ? w.Pcol[I * w.ldp + r] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 16; u++) p4[u & 3] += buf[u];
+                }
+                p += (p4[0] + p4[1]) + (p4[2] + p4[3]);
                 const int k0 = wave * kqm, k1 = min(im, k0 + kqm);
-                for (int k = k0; k < k1; k++)
-                    p += A[r + (c0 + k) * lda] * sh.t[k] - Wm[r + k * ldw] * sh.t[NBMAX + k];
+                double vb[KQ], wb[KQ];
+#pragma unroll
+                for (int u = 0; u < KQ; u++) {
+                    const int k = k0 + u;
+                    vb[u] = (k < k1) ? A[r + (c0 + k) * lda] : 0.0;
+                    wb[u] = (k < k1) ? Wm[r + k * ldw] : 0.0;
+                }
+                double c4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int u = 0; u < KQ; u++)
+                    if (k0 + u < k1) c4[u & 3] += vb[u] * sh.t[k0 + u] - wb[u] * sh.t[NBMAX + k0 + u];
+                p += (c4[0] + c4[1]) + (c4[2] + c4[3]);
             }
             sh.red[wave][lane] = p;
         }
@@ -182,13 +234,16 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
             A[r + jm * lda] = vr;
         }
         if (doA) {
-            double p = 0.0;
+            // this lane's row of V and W for the wave's k-group, loaded once (kq <= KQ)
+            const int k0 = wave * kq, k1 = min(i, k0 + kq);
+            double vreg[KQ], wreg[KQ];
             double xold = 0.0;
-            if (rin && r > j) {
-                xold = A[r + j * lda];
-                const int k0 = wave * kq, k1 = min(i, k0 + kq);
-                for (int k = k0; k < k1; k++) {
-                    double vk, wk;
+            const bool live = rin && r > j;
+#pragma unroll
+            for (int u = 0; u < KQ; u++) {
+                const int k = k0 + u;
+                double vk = 0.0, wk = 0.0;
+                if (live && k < k1) {
                     if (doC && k == im) {
                         vk = vr;
                         wk = wr;
@@ -196,9 +251,18 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                         vk = A[r + (c0 + k) * lda];
                         wk = Wm[r + k * ldw];
                     }
-                    p += vk * sh.wrow[k] - wk * sh.vrow[k];
                 }
+                vreg[u] = vk;
+                wreg[u] = wk;
             }
+            double p4[4] = {0.0, 0.0, 0.0, 0.0};
+            if (live) {
+                xold = A[r + j * lda];
+#pragma unroll
+                for (int u = 0; u < KQ; u++)
+                    if (k0 + u < k1) p4[u & 3] += vreg[u] * sh.wrow[k0 + u] - wreg[u] * sh.vrow[k0 + u];
+            }
+            const double p = (p4[0] + p4[1]) + (p4[2] + p4[3]);
             sh.red2[wave][lane] = p;
             __syncthreads();
             double x = 0.0;
@@ -211,23 +275,16 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                 if (lane == 0) w.Pnorm[q] = ss;
             }
             // S partials of column j: sum over the chunk's rows r >= j+2 of W[r,k] x_r, V[r,k] x_r
-            const int k0 = wave * kq, k1 = min(i, k0 + kq);
-            for (int k = k0; k < k1; k++) {
-                double vk = 0.0, wk = 0.0;
-                if (tail) {
-                    if (doC && k == im) {
-                        vk = vr;
-                        wk = wr;
-                    } else {
-                        vk = A[r + (c0 + k) * lda];
-                        wk = Wm[r + k * ldw];
+            const double xt = tail ? x : 0.0;
+#pragma unroll
+            for (int u = 0; u < KQ; u++) {
+                if (k0 + u < k1) {   // wave-uniform
+                    const double s1 = wave_sum(wreg[u] * xt);
+                    const double s2 = wave_sum(vreg[u] * xt);
+                    if (lane == 0) {
+                        w.PS[q * (2 * NBMAX) + k0 + u] = s1;
+                        w.PS[q * (2 * NBMAX) + NBMAX + k0 + u] = s2;
                     }
-                }
-                double s1 = wave_sum(wk * x);
-                double s2 = wave_sum(vk * x);
-                if (lane == 0) {
-                    w.PS[q * (2 * NBMAX) + k] = s1;
-                    w.PS[q * (2 * NBMAX) + NBMAX + k] = s2;
                 }
             }
         }
@@ -282,13 +339,55 @@ __device__ void reduce_t(const double* __restrict__ A, int64_t lda, int64_t n, i
     }
 }
 
+// One lane's 4 rows x 4 columns of tile (strip rows, tile column J) of the strictly lower
+// triangle of A[j+1:n, j+1:n] (masked), rows read as 16-byte pairs, and v at its 4 columns.
+__device__ __forceinline__ void load_tile(const double* __restrict__ A, int64_t lda, int64_t n, int64_t j, int64_t J,
+                                          int wave, int cg, const int64_t rows[4], const ColScal& s, double a[4][4],
+                                          double vc[4])
+{
+    const int64_t cb = J * TB + 16 * wave + 4 * cg;
+#pragma unroll
+    for (int cc = 0; cc < 4; cc++) {
+        const int64_t c = cb + cc;
+        vc[cc] = vval(A, lda, n, j, c, s);
+        const bool cok = (c > j) && (c < n);
+        const double* col = A + c * lda;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int64_t r = rows[2 * h];
+            double x0 = 0.0, x1 = 0.0;
+            if (cok && r + 1 < n) {
+                const double2 t = *reinterpret_cast<const double2*>(col + r);
+                x0 = t.x;
+                x1 = t.y;
+            } else if (cok && r < n) {
+                x0 = col[r];
+            }
+            a[2 * h][cc] = (r > c) ? x0 : 0.0;
+            a[2 * h + 1][cc] = (r + 1 > c) ? x1 : 0.0;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Skew matvec partials of column j over the strictly lower triangle of A[j+1:n, j+1:n].
-// Work item = (strip I, segment s) = tiles (I, J), J in [J0 + s*SEG, min(J0+s*SEG+SEG-1, I)].
-// Thread layout per 64x64 tile: wave -> 16 columns, lane -> 4 columns (lane>>4) x 4 rows
-// {2p, 2p+1, 32+2p, 33+2p} (p = lane&15), rows read as 16-byte pairs.
-//   Prow[s][r] = sum_{c in segment} A[r,c] v_c,   Pcol[I][c] = -sum_{r in strip I} A[r,c] v_r
+// The ns(ns+1)/2 tiles (64x64; strips I = J0..nt-1 of I-J0+1 tiles, strip-major order) are split
+// into G equal contiguous ranges, one per rank.  Per tile, thread layout: wave -> 16 columns,
+// lane -> 4 columns (lane>>4) x 4 rows {2p, 2p+1, 32+2p, 33+2p} (p = lane&15), rows read as
+// 16-byte pairs, the next tile's loads in flight while the current one is reduced.
+//   Pcol[I][c]       = -sum_{r in strip I} A[r,c] v_r            (one entry per tile)
+//   Prow[slot][r]    =  sum over the rank's tiles of strip I of A[r,c] v_c,
+//                       slot = rank - (owner of the strip's first tile)
 // ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void strip_rows(int64_t I, int p, int64_t rows[4])
+{
+    const int64_t rb = I * TB;
+    rows[0] = rb + 2 * p;
+    rows[1] = rb + 2 * p + 1;
+    rows[2] = rb + 32 + 2 * p;
+    rows[3] = rb + 33 + 2 * p;
+}
+
 __device__ void skew_matvec(const double* __restrict__ A, int64_t lda, int64_t n, int64_t j,
                             const Work& w, RowShared& sh, int g, int G)
 {
@@ -297,99 +396,100 @@ __device__ void skew_matvec(const double* __restrict__ A, int64_t lda, int64_t n
     const ColScal s = sh.cur;
     const int64_t nt = cdiv(n, TB);
     const int64_t J0 = (j + 1) / TB;
-    const int64_t nstrip = nt - J0;
-    const int64_t nsegmax = cdiv(nstrip, SEG);
-    // total items = sum_s (nstrip - s*SEG)
-    int64_t items = 0;
-    for (int64_t ss = 0; ss < nsegmax; ss++) items += nstrip - ss * SEG;
-
-    for (int64_t it = g; it < items; it += G) {
-        int64_t sidx = 0, rem = it, cnt = nstrip;
-        while (rem >= cnt) {
-            rem -= cnt;
-            sidx++;
-            cnt = nstrip - sidx * SEG;
-        }
-        const int64_t I = J0 + sidx * SEG + rem;
-        const int64_t Js = J0 + sidx * SEG;
-        const int64_t Je = min(Js + SEG - 1, I);
-        const int64_t rb = I * TB;
-        int64_t rows[4] = {rb + 2 * p, rb + 2 * p + 1, rb + 32 + 2 * p, rb + 33 + 2 * p};
-        double vr[4], racc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int64_t ns = nt - J0;
+    const int64_t T = ns * (ns + 1) / 2;
+    const int64_t Ge = (G < T) ? G : T;   // ranks taking part: every range is non-empty
+    if (g >= Ge) return;
+    const int64_t tlo = ((int64_t)g * T) / Ge, thi = ((int64_t)(g + 1) * T) / Ge;
+    int64_t st = strip_of(tlo);
+    int64_t I = J0 + st, J = J0 + (tlo - st * (st + 1) / 2);
+    const int64_t slot = g - tile_owner(st * (st + 1) / 2, T, Ge);
+    int64_t cur_slot = slot;
+    int64_t rows[4];
+    strip_rows(I, p, rows);
+    double vr[4], racc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int rr = 0; rr < 4; rr++) vr[rr] = vval(A, lda, n, j, rows[rr], s);
-
-        for (int64_t J = Js; J <= Je; J++) {
-            const int64_t cb = J * TB + 16 * wave + 4 * cg;
-            double a[4][4];
-            double vc[4];
+    for (int rr = 0; rr < 4; rr++) vr[rr] = vval(A, lda, n, j, rows[rr], s);
+    double a[4][4], vc[4];
+    load_tile(A, lda, n, j, J, wave, cg, rows, s, a, vc);
+    for (int64_t t = tlo; t < thi; t++) {
+        const bool more = t + 1 < thi;
+        const int64_t In = (J < I) ? I : I + 1;
+        const int64_t Jn = (J < I) ? J + 1 : J0;
+        int64_t rows_n[4];
+        strip_rows(In, p, rows_n);
+        double an[4][4], vcn[4];
+        if (more) load_tile(A, lda, n, j, Jn, wave, cg, rows_n, s, an, vcn);
+        const int64_t cb = J * TB + 16 * wave + 4 * cg;
+        double cp[4];
+#pragma unroll
+        for (int cc = 0; cc < 4; cc++) {
+            cp[cc] = 0.0;
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                racc[rr] += a[rr][cc] * vc[cc];
+                cp[cc] += a[rr][cc] * vr[rr];
+            }
+        }
+#pragma unroll
+        for (int cc = 0; cc < 4; cc++) {
+            double x = cp[cc];
+            x += __shfl_xor(x, 1, 64);
+            x += __shfl_xor(x, 2, 64);
+            x += __shfl_xor(x, 4, 64);
+            x += __shfl_xor(x, 8, 64);
+            cp[cc] = x;
+        }
+        if (p == 0) {
 #pragma unroll
             for (int cc = 0; cc < 4; cc++) {
                 const int64_t c = cb + cc;
-                vc[cc] = vval(A, lda, n, j, c, s);
-                const bool cok = (c > j) && (c < n);
-                const double* col = A + c * lda;
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int64_t r = rows[2 * h];
-                    double x0 = 0.0, x1 = 0.0;
-                    if (cok && r + 1 < n) {
-                        double2 t = *reinterpret_cast<const double2*>(col + r);
-                        x0 = t.x;
-                        x1 = t.y;
-                    } else if (cok && r < n) {
-                        x0 = col[r];
-                    }
-                    a[2 * h][cc] = (r > c) ? x0 : 0.0;
-                    a[2 * h + 1][cc] = (r + 1 > c) ? x1 : 0.0;
-                }
+                if (c > j && c < n) w.Pcol[I * w.ldp + c] = -cp[cc];
             }
-            double cp[4];
+        }
+        if (!more || In != I) {
+            // flush this rank's row sums of strip I
 #pragma unroll
-            for (int cc = 0; cc < 4; cc++) {
-                cp[cc] = 0.0;
+            for (int rr = 0; rr < 4; rr++) {
+                double x = racc[rr];
+                x += __shfl_xor(x, 16, 64);
+                x += __shfl_xor(x, 32, 64);
+                racc[rr] = x;
+            }
+            if (cg == 0) {
+                sh.red[wave][2 * p] = racc[0];
+                sh.red[wave][2 * p + 1] = racc[1];
+                sh.red[wave][32 + 2 * p] = racc[2];
+                sh.red[wave][33 + 2 * p] = racc[3];
+            }
+            __syncthreads();
+            if (wave == 0) {
+                const int64_t r = I * TB + lane;
+                if (r < n)
+                    w.Prow[cur_slot * w.ldp + r] = sh.red[0][lane] + sh.red[1][lane] + sh.red[2][lane] + sh.red[3][lane];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) racc[rr] = 0.0;
+            cur_slot = 0;   // this rank owns the first tile of every later strip it touches
+            if (more) {
 #pragma unroll
                 for (int rr = 0; rr < 4; rr++) {
-                    racc[rr] += a[rr][cc] * vc[cc];
-                    cp[cc] += a[rr][cc] * vr[rr];
-                }
-            }
-#pragma unroll
-            for (int cc = 0; cc < 4; cc++) {
-                double x = cp[cc];
-                x += __shfl_xor(x, 1, 64);
-                x += __shfl_xor(x, 2, 64);
-                x += __shfl_xor(x, 4, 64);
-                x += __shfl_xor(x, 8, 64);
-                cp[cc] = x;
-            }
-            if (p == 0) {
-#pragma unroll
-                for (int cc = 0; cc < 4; cc++) {
-                    const int64_t c = cb + cc;
-                    if (c > j && c < n) w.Pcol[I * w.ldp + c] = -cp[cc];
+                    rows[rr] = rows_n[rr];
+                    vr[rr] = vval(A, lda, n, j, rows[rr], s);
                 }
             }
         }
+        if (more) {
 #pragma unroll
-        for (int rr = 0; rr < 4; rr++) {
-            double x = racc[rr];
-            x += __shfl_xor(x, 16, 64);
-            x += __shfl_xor(x, 32, 64);
-            racc[rr] = x;
+            for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+                for (int cc = 0; cc < 4; cc++) a[rr][cc] = an[rr][cc];
+#pragma unroll
+            for (int cc = 0; cc < 4; cc++) vc[cc] = vcn[cc];
         }
-        if (cg == 0) {
-            sh.red[wave][2 * p] = racc[0];
-            sh.red[wave][2 * p + 1] = racc[1];
-            sh.red[wave][32 + 2 * p] = racc[2];
-            sh.red[wave][33 + 2 * p] = racc[3];
-        }
-        __syncthreads();
-        if (wave == 0) {
-            const int64_t r = rb + lane;
-            if (r < n) w.Prow[sidx * w.ldp + r] = sh.red[0][lane] + sh.red[1][lane] + sh.red[2][lane] + sh.red[3][lane];
-        }
-        __syncthreads();
+        I = In;
+        J = Jn;
     }
 }
 
@@ -397,9 +497,11 @@ __device__ void skew_matvec(const double* __restrict__ A, int64_t lda, int64_t n
 // Skew rank-2k update of one BT x BT tile of the lower triangle on FP64 MFMA:
 //   A[r,c] += sum_k L[r,k] R[c,k]   for rows r in [rlo, n), cols c in [clo, n), r > c,
 //   L = [V U], R = [U -V], V[:,k] = A[:, c0+k] (reflectors, rows >= c0+k+1), U[:,k] = U[:, k].
-// v_mfma_f64_16x16x4f64 layout: a-operand lane l = (row l%16, k l/16); b-operand lane l =
-// (k l/16, col l%16); accumulator item q of lane l = (row 4*(l/16)+q, col l%16).
-// The 4 waves tile the 128x128 block as 2x2 sub-blocks of 64x64 (4x4 MFMA blocks each).
+// v_mfma_f64_16x16x4f64 (measured on gfx950 by k_mfma_probe): a-operand lane l = (row l%16,
+// k l/16); b-operand lane l = (k l/16, col l%16); accumulator item q of lane l =
+// (row 4q + l/16, col l%16).  Each 16x16 block is computed transposed, D = R_blk L_blk^T, so
+// item q of lane l is C[r + l%16, c + 4q + l/16]: a load/store of one item is four 128-byte
+// column segments per wave.  The 4 waves tile the 128x128 block as 2x2 sub-blocks of 64x64.
 // ---------------------------------------------------------------------------------------
 struct MfmaShared {
     double L[KC * LDSLD];
@@ -407,36 +509,28 @@ struct MfmaShared {
 };
 
 __device__ __forceinline__ void rank2k_tile(double* __restrict__ A, int64_t lda, const double* __restrict__ U,
-                                         int64_t ldu, int64_t n, int64_t rlo, int64_t clo, int64_t c0, int nbp,
-                                         int64_t r0, int64_t q0, MfmaShared& sm)
+                                            int64_t ldu, int64_t n, int64_t rlo, int64_t clo, int64_t c0, int nbp,
+                                            int64_t r0, int64_t q0, MfmaShared& sm)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     // a wave whose whole 64x64 sub-block lies on or above the diagonal has nothing to do
     const bool active = (r0 + wr * 64 + 63 > q0 + wc * 64);
-    const int lr = 4 * (lane >> 4), lc = lane & 15;
-    const int64_t crow = r0 + wr * 64 + lr;   // first row of this lane's accumulator items
-    const int64_t ccol = q0 + wc * 64 + lc;   // column of block bj = 0
+    const int lc = lane & 15, lq = lane >> 4;
+    const int64_t crow = r0 + wr * 64 + lc;        // row of this lane's items in block bi = 0
+    const int64_t ccol = q0 + wc * 64 + lq;        // column of item q = 0 in block bj = 0
     v4d acc[4][4];
 #pragma unroll
     for (int bi = 0; bi < 4; bi++) {
 #pragma unroll
         for (int bj = 0; bj < 4; bj++) {
-            const int64_t row = crow + bi * 16, col = ccol + bj * 16;
+            const int64_t row = crow + bi * 16;
             v4d c = {0.0, 0.0, 0.0, 0.0};
-            if (active && col >= clo && col < n) {
-                const double* p = A + row + col * lda;
-                if (row + 3 < n) {
-                    const double2 t0 = *reinterpret_cast<const double2*>(p);
-                    const double2 t1 = *reinterpret_cast<const double2*>(p + 2);
-                    c[0] = t0.x;
-                    c[1] = t0.y;
-                    c[2] = t1.x;
-                    c[3] = t1.y;
-                } else {
-                    if (row < n) c[0] = p[0];
-                    if (row + 1 < n) c[1] = p[1];
-                    if (row + 2 < n) c[2] = p[2];
+            if (active && row < n) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int64_t col = ccol + bj * 16 + 4 * q;
+                    if (col >= clo && col < n) c[q] = A[row + col * lda];
                 }
             }
             acc[bi][bj] = c;
@@ -473,33 +567,32 @@ __device__ __forceinline__ void rank2k_tile(double* __restrict__ A, int64_t lda,
         if (active) {
 #pragma unroll
             for (int ks = 0; ks < KC / 4; ks++) {
-                const int kk = ks * 4 + (lane >> 4);
-                double af[4], bf[4];
+                const int kk = ks * 4 + lq;
+                double lf[4], rf[4];
 #pragma unroll
                 for (int b = 0; b < 4; b++) {
-                    af[b] = sm.L[kk * LDSLD + wr * 64 + b * 16 + lc];
-                    bf[b] = sm.R[kk * LDSLD + wc * 64 + b * 16 + lc];
+                    lf[b] = sm.L[kk * LDSLD + wr * 64 + b * 16 + lc];
+                    rf[b] = sm.R[kk * LDSLD + wc * 64 + b * 16 + lc];
                 }
 #pragma unroll
                 for (int bi = 0; bi < 4; bi++)
 #pragma unroll
                     for (int bj = 0; bj < 4; bj++)
-                        acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[bi], bf[bj], acc[bi][bj], 0, 0, 0);
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(rf[bj], lf[bi], acc[bi][bj], 0, 0, 0);
             }
         }
     }
     if (!active) return;
 #pragma unroll
     for (int bi = 0; bi < 4; bi++) {
+        const int64_t row = crow + bi * 16;
+        if (row >= n || row < rlo) continue;
 #pragma unroll
         for (int bj = 0; bj < 4; bj++) {
-            const int64_t row = crow + bi * 16, col = ccol + bj * 16;
-            if (col < clo || col >= n) continue;
-            double* p = A + row + col * lda;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int64_t rq = row + q;
-                if (rq > col && rq >= rlo && rq < n) p[q] = acc[bi][bj][q];
+                const int64_t col = ccol + bj * 16 + 4 * q;
+                if (col >= clo && col < row) A[row + col * lda] = acc[bi][bj][q];
             }
         }
     }
@@ -537,5 +630,80 @@ __device__ inline void tile_coords(const TileGrid& tg, int64_t t, int64_t& r0, i
     r0 = bi * BT;
     q0 = (tg.cb0 + t) * BT;
 }
+
+// ---------------------------------------------------------------------------------------
+// One-workgroup tridiagonalization of an n x n matrix M (ld even), for the batched engine.
+// ---------------------------------------------------------------------------------------
+template <int KQ>
+__device__ void tridiag_team1(double* M, int64_t ld, int64_t n, int nb, const Work& w, double* tau, double* eta,
+                              RowShared& sh, MfmaShared& sm)
+{
+    for (int64_t c0 = 0; c0 <= n - 3; c0 += nb) {
+        const int nbp = (int)min((int64_t)nb, n - 2 - c0);
+        row_pass<KQ>(M, ld, n, c0, c0, 0, false, true, w, 0, 1, sh);
+        for (int i = 0; i < nbp; i++) {
+            const int64_t j = c0 + i;
+            __syncthreads();
+            column_scalars(M, ld, n, j, w, sh, tau, eta, true);
+            reduce_t(M, ld, n, c0, j, i, w, sh, 0, 1);
+            skew_matvec(M, ld, n, j, w, sh, 0, 1);
+            __syncthreads();
+            row_pass<KQ>(M, ld, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, 0, 1, sh);
+        }
+        __syncthreads();
+        const int64_t e = c0 + nbp;
+        const TileGrid tg = tile_grid(n, e, e);
+        for (int64_t t = 0; t < tg.total; t++) {
+            int64_t r0, q0;
+            tile_coords(tg, t, r0, q0);
+            rank2k_tile(M, ld, w.W, w.ldw, n, e, e, c0, nbp, r0, q0, sm);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && n >= 2) {
+        eta[n - 2] = M[(n - 1) + (n - 2) * ld];
+        tau[n - 2] = 0.0;
+    }
+    __syncthreads();
+}
+
+// log|Pf| and sign from eta, tau of an even-n factorization (src/pfaffian.jl:170-178):
+// logpf = sum_{odd i} log|ev_i|, sign = prod sign(-ev_i) * (-1)^{#tau != 0}.
+__device__ void logpf_reduce(const double* eta, const double* tau, int64_t n, double* out_lp, double* out_sg)
+{
+    __shared__ double s_l[4];
+    __shared__ int s_c[4][3];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double l = 0.0;
+    int nneg = 0, nzero = 0, ntau = 0;
+    for (int64_t i = tid; i < n - 1; i += WG) {
+        if ((i & 1) == 0) {
+            const double e = eta[i];
+            l += log(fabs(e));
+            nneg += (e > 0.0);   // sign(-e) < 0
+            nzero += (e == 0.0);
+        }
+        ntau += (tau[i] != 0.0);
+    }
+    l = wave_sum(l);
+    double dn = wave_sum((double)nneg), dz = wave_sum((double)nzero), dt = wave_sum((double)ntau);
+    if (lane == 0) {
+        s_l[wave] = l;
+        s_c[wave][0] = (int)dn;
+        s_c[wave][1] = (int)dz;
+        s_c[wave][2] = (int)dt;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double L = s_l[0] + s_l[1] + s_l[2] + s_l[3];
+        int cn = s_c[0][0] + s_c[1][0] + s_c[2][0] + s_c[3][0];
+        int cz = s_c[0][1] + s_c[1][1] + s_c[2][1] + s_c[3][1];
+        int ct = s_c[0][2] + s_c[1][2] + s_c[2][2] + s_c[3][2];
+        *out_lp = L;
+        *out_sg = (cz > 0) ? 0.0 : (((cn + ct) & 1) ? -1.0 : 1.0);
+    }
+    __syncthreads();
+}
+
 
 }  // namespace skh

@@ -18,7 +18,9 @@ struct PanelLaunch {
     unsigned* bar;
     unsigned base;
     int* err;
+    unsigned long long* ph;   // optional phase clock (4 counters) or null
     int grid;
+    int nbmax;                // panel width bound selecting the kernel instance
 };
 
 struct BatchArgs {
@@ -40,13 +42,13 @@ struct BatchArgs {
 
 extern "C" {
 int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st);
-int skh_internal_panel_occupancy(int* blocks_per_cu);
+int skh_internal_panel_occupancy(int nbmax, int* blocks_per_cu);
 int skh_internal_launch_rank2k(double* A, int64_t lda, const double* U, int64_t ldu, int64_t n, int64_t rlo,
                                int64_t clo, int64_t c0, int nbp, hipStream_t st);
 int skh_internal_launch_finish(double* A, int64_t lda, int64_t n, double* tau, double* eta, hipStream_t st);
 int skh_internal_zero_u32(unsigned* p, int count, hipStream_t st);
 int skh_internal_launch_batched(const BatchArgs* a, int grid, hipStream_t st);
-int skh_internal_batched_occupancy(int* blocks_per_cu);
+int skh_internal_batched_occupancy(int nb, int* blocks_per_cu);
 int skh_internal_mfma_probe(double* dout, hipStream_t st);
 int skh_internal_eye(double* Q, int64_t ldq, int64_t n, hipStream_t st);
 int skh_internal_form_v(const double* F, int64_t ldf, int64_t n, int64_t c0, int nbp, double* V, int64_t ldv,
@@ -71,6 +73,6 @@ inline int64_t skh_slot_doubles(int64_t nmax, int nb)
     const int64_t ld = (nmax + 1) & ~int64_t(1);
     const int64_t nt = (nmax + skh::TB - 1) / skh::TB + 1;
     int64_t s = ld * nmax + ld * nb + 2 * ld + nt + nt * 2 * skh::NBMAX + 2 * skh::NBMAX;
-    s += ((nt + skh::SEG - 1) / skh::SEG + 1) * ld + nt * ld;
+    s += nt * ld + nt * ld;   // Prow, Pcol
     return (s + 31) & ~int64_t(31);   // keep every slot 256-byte aligned
 }

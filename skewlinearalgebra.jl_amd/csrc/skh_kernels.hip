@@ -20,22 +20,35 @@
 namespace skh {
 
 // ---------------------------------------------------------------------------------------
-// grid barrier for the cooperative panel kernel.  The counter only grows within one
-// factorization; the host passes the value it had when the launch started.  A bounded spin
-// turns a lost workgroup into an error flag instead of a hang.
+// Grid barrier of the cooperative panel kernel, two levels: workgroups arrive on one of 8 group
+// counters (blockIdx % 8, i.e. the XCD they are dispatched to), the last arrival of a group
+// arrives on the top counter, and every workgroup polls the top counter.  Counters only grow
+// within one factorization; `epoch` is the 1-based index of this barrier in the factorization.
+// Memory ordering: every wave drains its own stores, then one release fence per workgroup before
+// arriving and one acquire fence after the wait (relaxed polling, so the wait itself does not
+// touch the caches).  A bounded spin turns a lost workgroup into an error flag, not a hang.
+// Layout of `bar` (unsigned words): [0] top counter, [32*(1+k)] group k counter.
 // ---------------------------------------------------------------------------------------
 constexpr unsigned SPIN_LIMIT = 1u << 26;
 
-__device__ bool grid_sync(unsigned* bar, unsigned target, int* err, int* s_ok)
+__device__ bool grid_sync(unsigned* bar, unsigned epoch, int* err, int* s_ok)
 {
+    // this wave's global stores have reached the L2 before the workgroup releases them; the
+    // memory clobber keeps the compiler from moving any store below the wait
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
-        atomicAdd(bar, 1u);
+        const unsigned G = gridDim.x;
+        const unsigned grp = blockIdx.x & 7u;
+        const unsigned ngroups = G < 8u ? G : 8u;
+        const unsigned gsize = (G - grp + 7u) / 8u;
+        __threadfence();   // release this workgroup's writes device-wide
+        const unsigned old = atomicAdd(&bar[32u * (1u + grp)], 1u);
+        if (old + 1u == epoch * gsize) atomicAdd(&bar[0], 1u);
         unsigned spins = 0;
         int ok = 1;
-        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            __builtin_amdgcn_s_sleep(1);
+        while (__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch * ngroups) {
+            __builtin_amdgcn_s_sleep(2);
             if (++spins > SPIN_LIMIT) {
                 atomicOr(err, 1);
                 ok = 0;
@@ -43,31 +56,43 @@ __device__ bool grid_sync(unsigned* bar, unsigned target, int* err, int* s_ok)
             }
         }
         if (ok && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) ok = 0;
-        __threadfence();
+        __threadfence();   // acquire: later loads of this workgroup see every arrival's writes
         *s_ok = ok;
     }
     __syncthreads();
     return *s_ok != 0;
 }
 
-__global__ __launch_bounds__(WG) void k_panel(double* A, int64_t lda, int64_t n, int64_t c0, int nbp, Work w,
-                                              double* tau, double* eta, unsigned* bar, unsigned base, int* err)
+// Optional phase clock (workgroup 0, 100 MHz wall clock): ph[0] row passes, ph[1] barrier 1
+// waits, ph[2] scalars + t + matvec, ph[3] barrier 2 waits (accumulated over all launches).
+template <int KQ>
+__global__ __launch_bounds__(WG, 1) void k_panel(double* A, int64_t lda, int64_t n, int64_t c0, int nbp, Work w,
+                                              double* tau, double* eta, unsigned* bar, unsigned base, int* err,
+                                              unsigned long long* ph)
 {
     __shared__ RowShared sh;
     __shared__ int s_ok;
     const int G = gridDim.x, g = blockIdx.x;
-    unsigned target = base;
-    row_pass(A, lda, n, c0, c0, 0, false, true, w, g, G, sh);
+    const bool clock = (ph != nullptr) && g == 0 && threadIdx.x == 0;
+    unsigned long long t0 = clock ? wall_clock64() : 0, acc[4] = {0, 0, 0, 0};
+    unsigned epoch = base;   // barriers completed before this launch
+    row_pass<KQ>(A, lda, n, c0, c0, 0, false, true, w, g, G, sh);
     for (int i = 0; i < nbp; i++) {
         const int64_t j = c0 + i;
-        target += G;
-        if (!grid_sync(bar, target, err, &s_ok)) return;
+        if (clock) { unsigned long long t = wall_clock64(); acc[0] += t - t0; t0 = t; }
+        if (!grid_sync(bar, ++epoch, err, &s_ok)) return;
+        if (clock) { unsigned long long t = wall_clock64(); acc[1] += t - t0; t0 = t; }
         column_scalars(A, lda, n, j, w, sh, tau, eta, g == 0);
         reduce_t(A, lda, n, c0, j, i, w, sh, g, G);
         skew_matvec(A, lda, n, j, w, sh, g, G);
-        target += G;
-        if (!grid_sync(bar, target, err, &s_ok)) return;
-        row_pass(A, lda, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, g, G, sh);
+        if (clock) { unsigned long long t = wall_clock64(); acc[2] += t - t0; t0 = t; }
+        if (!grid_sync(bar, ++epoch, err, &s_ok)) return;
+        if (clock) { unsigned long long t = wall_clock64(); acc[3] += t - t0; t0 = t; }
+        row_pass<KQ>(A, lda, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, g, G, sh);
+    }
+    if (clock) {
+        acc[0] += wall_clock64() - t0;
+        for (int k = 0; k < 4; k++) ph[k] += acc[k];
     }
 }
 
@@ -97,79 +122,7 @@ __global__ void k_zero_u32(unsigned* p, int count)
     for (int k = threadIdx.x; k < count; k += blockDim.x) p[k] = 0u;
 }
 
-// ---------------------------------------------------------------------------------------
-// One-workgroup tridiagonalization of an n x n matrix M (ld even), for the batched engine.
-// ---------------------------------------------------------------------------------------
-__device__ void tridiag_team1(double* M, int64_t ld, int64_t n, int nb, const Work& w, double* tau, double* eta,
-                              RowShared& sh, MfmaShared& sm)
-{
-    for (int64_t c0 = 0; c0 <= n - 3; c0 += nb) {
-        const int nbp = (int)min((int64_t)nb, n - 2 - c0);
-        row_pass(M, ld, n, c0, c0, 0, false, true, w, 0, 1, sh);
-        for (int i = 0; i < nbp; i++) {
-            const int64_t j = c0 + i;
-            __syncthreads();
-            column_scalars(M, ld, n, j, w, sh, tau, eta, true);
-            reduce_t(M, ld, n, c0, j, i, w, sh, 0, 1);
-            skew_matvec(M, ld, n, j, w, sh, 0, 1);
-            __syncthreads();
-            row_pass(M, ld, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, 0, 1, sh);
-        }
-        __syncthreads();
-        const int64_t e = c0 + nbp;
-        const TileGrid tg = tile_grid(n, e, e);
-        for (int64_t t = 0; t < tg.total; t++) {
-            int64_t r0, q0;
-            tile_coords(tg, t, r0, q0);
-            rank2k_tile(M, ld, w.W, w.ldw, n, e, e, c0, nbp, r0, q0, sm);
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0 && n >= 2) {
-        eta[n - 2] = M[(n - 1) + (n - 2) * ld];
-        tau[n - 2] = 0.0;
-    }
-    __syncthreads();
-}
-
-// log|Pf| and sign from eta, tau of an even-n factorization (src/pfaffian.jl:170-178):
-// logpf = sum_{odd i} log|ev_i|, sign = prod sign(-ev_i) * (-1)^{#tau != 0}.
-__device__ void logpf_reduce(const double* eta, const double* tau, int64_t n, double* out_lp, double* out_sg)
-{
-    __shared__ double s_l[4];
-    __shared__ int s_c[4][3];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    double l = 0.0;
-    int nneg = 0, nzero = 0, ntau = 0;
-    for (int64_t i = tid; i < n - 1; i += WG) {
-        if ((i & 1) == 0) {
-            const double e = eta[i];
-            l += log(fabs(e));
-            nneg += (e > 0.0);   // sign(-e) < 0
-            nzero += (e == 0.0);
-        }
-        ntau += (tau[i] != 0.0);
-    }
-    l = wave_sum(l);
-    double dn = wave_sum((double)nneg), dz = wave_sum((double)nzero), dt = wave_sum((double)ntau);
-    if (lane == 0) {
-        s_l[wave] = l;
-        s_c[wave][0] = (int)dn;
-        s_c[wave][1] = (int)dz;
-        s_c[wave][2] = (int)dt;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        double L = s_l[0] + s_l[1] + s_l[2] + s_l[3];
-        int cn = s_c[0][0] + s_c[1][0] + s_c[2][0] + s_c[3][0];
-        int cz = s_c[0][1] + s_c[1][1] + s_c[2][1] + s_c[3][1];
-        int ct = s_c[0][2] + s_c[1][2] + s_c[2][2] + s_c[3][2];
-        *out_lp = L;
-        *out_sg = (cz > 0) ? 0.0 : (((cn + ct) & 1) ? -1.0 : 1.0);
-    }
-    __syncthreads();
-}
-
+template <int KQ>
 __global__ __launch_bounds__(WG) void k_batched(BatchArgs a)
 {
     __shared__ RowShared sh;
@@ -214,13 +167,13 @@ __global__ __launch_bounds__(WG) void k_batched(BatchArgs a)
         w.Tv = w.PS + nt * 2 * NBMAX;
         w.Prow = w.Tv + 2 * NBMAX;
         w.ldp = ld;
-        w.Pcol = w.Prow + (cdiv(nt, SEG) + 1) * ld;
+        w.Pcol = w.Prow + nt * ld;
         // strictly lower triangle of the input into the slot
         const double* src = a.A + a.offs[b];
         for (int64_t c = 0; c < n; c++)
             for (int64_t r = c + 1 + tid; r < n; r += WG) M[r + c * ld] = src[r + c * n];
         __syncthreads();
-        tridiag_team1(M, ld, n, a.nb, w, tau, eta, sh, sm);
+        tridiag_team1<KQ>(M, ld, n, a.nb, w, tau, eta, sh, sm);
         logpf_reduce(eta, tau, n, a.logpf + b, a.sgn + b);
         if (a.fac) {
             double* dst = a.fac + a.offs[b];
@@ -400,16 +353,31 @@ __global__ void k_mfma_probe(double* out)
 // =========================================================================================
 using namespace skh;
 
-extern "C" int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st)
+static const void* panel_kernel_for(int nbmax)
 {
-    void* args[] = {(void*)&pl->A, (void*)&pl->lda, (void*)&pl->n, (void*)&pl->c0, (void*)&pl->nbp, (void*)&pl->w,
-                    (void*)&pl->tau, (void*)&pl->eta, (void*)&pl->bar, (void*)&pl->base, (void*)&pl->err};
-    return (int)hipLaunchCooperativeKernel((const void*)k_panel, dim3(pl->grid), dim3(WG), args, 0, st);
+    if (nbmax <= 32) return (const void*)k_panel<8>;
+    if (nbmax <= 64) return (const void*)k_panel<16>;
+    return (const void*)k_panel<32>;
 }
 
-extern "C" int skh_internal_panel_occupancy(int* blocks_per_cu)
+static const void* batched_kernel_for(int nb)
 {
-    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void*)k_panel, WG, 0);
+    if (nb <= 32) return (const void*)k_batched<8>;
+    if (nb <= 64) return (const void*)k_batched<16>;
+    return (const void*)k_batched<32>;
+}
+
+extern "C" int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st)
+{
+    void* args[] = {(void*)&pl->A,   (void*)&pl->lda, (void*)&pl->n,   (void*)&pl->c0,  (void*)&pl->nbp,
+                    (void*)&pl->w,   (void*)&pl->tau, (void*)&pl->eta, (void*)&pl->bar, (void*)&pl->base,
+                    (void*)&pl->err, (void*)&pl->ph};
+    return (int)hipLaunchCooperativeKernel(panel_kernel_for(pl->nbmax), dim3(pl->grid), dim3(WG), args, 0, st);
+}
+
+extern "C" int skh_internal_panel_occupancy(int nbmax, int* blocks_per_cu)
+{
+    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, panel_kernel_for(nbmax), WG, 0);
 }
 
 extern "C" int skh_internal_launch_rank2k(double* A, int64_t lda, const double* U, int64_t ldu, int64_t n, int64_t rlo,
@@ -437,13 +405,14 @@ extern "C" int skh_internal_zero_u32(unsigned* p, int count, hipStream_t st)
 
 extern "C" int skh_internal_launch_batched(const BatchArgs* a, int grid, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_batched, dim3(grid), dim3(WG), 0, st, *a);
-    return (int)hipGetLastError();
+    BatchArgs copy = *a;
+    void* args[] = {(void*)&copy};
+    return (int)hipLaunchKernel(batched_kernel_for(a->nb), dim3(grid), dim3(WG), args, 0, st);
 }
 
-extern "C" int skh_internal_batched_occupancy(int* blocks_per_cu)
+extern "C" int skh_internal_batched_occupancy(int nb, int* blocks_per_cu)
 {
-    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void*)k_batched, WG, 0);
+    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, batched_kernel_for(nb), WG, 0);
 }
 
 extern "C" int skh_internal_mfma_probe(double* dout, hipStream_t st)
