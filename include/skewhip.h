@@ -6,8 +6,9 @@
  *   - column-major matrices, 0-based indices, int64_t sizes and leading dimensions;
  *   - "host" entry points take caller-owned host memory (Julia arrays under GC.@preserve)
  *     and block until the result is back in that memory;
- *   - "_device" entry points take device (HBM) pointers and a hipStream_t (passed as void*),
- *     and are asynchronous with respect to the host unless stated otherwise;
+ *   - "_device" entry points take device (HBM) pointers and a hipStream_t (passed as void*; NULL
+ *     is HIP's default stream), and are asynchronous with respect to the host unless stated
+ *     otherwise;
  *   - return status: 0 success; -k invalid argument number k (1-based); > 0 runtime error
  *     (SKH_ERR_*).  skh_last_error() gives a message for the calling thread.
  * Calls are serialised by an internal mutex; the library owns cached device workspaces.

@@ -531,7 +531,7 @@ int skh_sytrd_d_device(int64_t n, double* dA, int64_t lda, double* dtau, double*
     if (opts && (opts->nb < 0 || opts->nb > skh::NBMAX)) return set_err(-7, "nb out of range");
     int rc = ensure_init_locked(g_device);
     if (rc) return rc;
-    hipStream_t st = stream ? (hipStream_t)stream : g_stream;
+    hipStream_t st = (hipStream_t)stream;
     if (n <= 1) return 0;
     skh_opts o = opts ? *opts : skh_opts{0, 0};
     o.flags = SKH_ENGINE_LARGE | (o.flags & SKH_FLAG_TIMING);
@@ -639,7 +639,7 @@ int skh_logpf_batched_d_device(int64_t batch, const int64_t* n, const int64_t* o
     if (opts && (opts->nb < 0 || opts->nb > skh::NBMAX)) return set_err(-8, "nb out of range");
     int rc = ensure_init_locked(g_device);
     if (rc) return rc;
-    hipStream_t st = stream ? (hipStream_t)stream : g_stream;
+    hipStream_t st = (hipStream_t)stream;
     return batched_run(batch, n, offset, dA, dlogpf, dsgn, opts, st);
 }
 
@@ -689,7 +689,7 @@ int skh_orgtr_d_device(int64_t n, const double* dF, int64_t ldf, const double* d
     if (ldq < std::max<int64_t>(n, 1)) return set_err(-6, "ldq < n");
     int rc = ensure_init_locked(g_device);
     if (rc) return rc;
-    return orgtr_locked(n, dF, ldf, dtau, dQ, ldq, stream ? (hipStream_t)stream : g_stream);
+    return orgtr_locked(n, dF, ldf, dtau, dQ, ldq, (hipStream_t)stream);
 }
 
 int skh_eigen_d(int64_t n, const double* A, int64_t lda, double* vals, double* vr, double* vi, const skh_opts* opts)
@@ -758,7 +758,7 @@ int skh_logpf_grad_batched_d_device(int64_t batch, const int64_t* n, const int64
     if (opts && (opts->nb < 0 || opts->nb > skh::NBMAX)) return set_err(-9, "nb out of range");
     int rc = ensure_init_locked(g_device);
     if (rc) return rc;
-    return grad_run(batch, n, offset, dA, dlogpf, dsgn, dgrad, opts, stream ? (hipStream_t)stream : g_stream);
+    return grad_run(batch, n, offset, dA, dlogpf, dsgn, dgrad, opts, (hipStream_t)stream);
 }
 
 int skh_logpf_grad_batched_d(int64_t batch, const int64_t* n, const int64_t* offset, const double* Apacked,
