@@ -95,18 +95,22 @@ int ensure_init_locked(int device)
 }
 
 // autotuned panel width per size (refreshed from measurements; see the bench/tuning notes)
-int64_t pick_nb(int64_t n, const skh_opts* o, bool batched)
+int64_t pick_nb(int64_t n, const skh_opts* o)
 {
     if (o && o->nb > 0) return std::min<int64_t>(o->nb, skh::NBMAX);
-    if (batched) return n <= 96 ? 16 : 32;
     return n < 1024 ? 32 : 64;
 }
+
+// compact engine limits: even n up to this size (one workgroup per matrix)
+constexpr int64_t CMP_NMAX = 1024;
+// automatic choice for a single matrix: the compact engine below this size
+constexpr int64_t CMP_AUTO = 1024;
 
 int engine_of(int64_t n, const skh_opts* o)
 {
     int64_t f = o ? (o->flags & 0xff) : 0;
     if (f == SKH_ENGINE_LARGE || f == SKH_ENGINE_TEAM1) return (int)f;
-    return n < 128 ? SKH_ENGINE_TEAM1 : SKH_ENGINE_LARGE;
+    return (n % 2 == 0 && n <= CMP_AUTO) ? SKH_ENGINE_TEAM1 : SKH_ENGINE_LARGE;
 }
 
 // Optional per-kernel timing (SKH_FLAG_TIMING): event pairs around every panel and rank-2k
@@ -149,20 +153,22 @@ KTimer g_timer;
 struct Ctl {
     unsigned* bar;
     int* err;
-    unsigned* queue;
     unsigned long long* ph;   // phase clock of the panel kernel (4 counters)
+    unsigned* cqueue;         // compact engine: work queue and error flag (own cache lines)
+    int* cerr;
 };
 
 int get_ctl(Ctl& c)
 {
-    // words [0, 288): barrier counters (9 lines of 32 words); [320]: error flag; [352]: batch
-    // queue; [384, 392): phase clocks (4 x u64)
+    // words [0, 288): barrier counters (9 lines of 32 words); [320]: error flag of the panel
+    // kernel; [384, 392): phase clocks (4 x u64); [448]: compact queue; [480]: compact error flag
     unsigned* p = (unsigned*)g_ctl.get(2048);
     if (!p) return set_err(SKH_ERR_HIP, "hipMalloc of control block failed");
     c.bar = p;
     c.err = (int*)(p + 320);
-    c.queue = p + 352;
     c.ph = (unsigned long long*)(p + 384);
+    c.cqueue = p + 448;
+    c.cerr = (int*)(p + 480);
     return 0;
 }
 
@@ -213,7 +219,14 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
         pl.bar = ctl.bar;
         pl.base = base;
         pl.err = ctl.err;
-        pl.grid = grid;
+        // grid sized to the panel's first matvec: ~8 tiles per workgroup (fewer workgroups make
+        // the two grid barriers per column cheaper once the trailing matrix is small)
+        {
+            const int64_t ns0 = (n + skh::TB - 1) / skh::TB - (c0 + 1) / skh::TB;
+            const int64_t tiles = ns0 * (ns0 + 1) / 2;
+            const int64_t want = std::max<int64_t>(8, (tiles + 7) / 8);
+            pl.grid = (int)std::min<int64_t>(grid, want);
+        }
         pl.nbmax = (int)nb;
         pl.ph = g_timer.on ? ctl.ph : nullptr;
         const bool tm = g_timer.on;
@@ -228,7 +241,9 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
             }
             HIPCHK((hipError_t)g_timer.end(st, 0, bytes));
         }
-        base += (unsigned)(2 * nbp);   // barriers per launch
+        base += (unsigned)(2 * nbp);   // barriers per launch (group counters restart below)
+        HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 288, st));
+        base = 0;
         const int64_t e = c0 + nbp;
         if (tm) HIPCHK((hipError_t)g_timer.begin(st));
         HIPCHK((hipError_t)skh_internal_launch_rank2k(dA, lda, w.W, w.ldw, n, e, e, c0, nbp, st));
@@ -242,62 +257,115 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
 }
 
 // ----------------------------------------------------------------------------------------
-// Team-of-one engine (batched).  dA packed; ns/offs host arrays.
+// Batched factorizations.  Matrices with even 2 <= n <= CMP_NMAX go through the compact engine
+// (one workgroup per matrix, one launch for all of them, largest first); larger even ones
+// through the large engine one after another; odd and empty ones get their closed-form result
+// in the compact launch.  With dfac, each factorization runs in place in dfac (input copied
+// in first; offsets as the input) and tau|eta go to dtaueta + toffs[b].
 // ----------------------------------------------------------------------------------------
+Buf g_cslot, g_big, g_bctl;
+
+double watchdog_seconds()
+{
+    const char* e = getenv("SKH_WATCHDOG_SECONDS");
+    double s = e ? atof(e) : 30.0;
+    return s > 0 ? s : 0.0;
+}
+
+int check_barrier_flag();
+
 int batched_run(int64_t batch, const int64_t* ns, const int64_t* offs, const double* dA, double* dlogpf,
                 double* dsgn, const skh_opts* opts, hipStream_t st, double* dfac = nullptr, double* dtaueta = nullptr,
-                const int64_t* dtoffs = nullptr)
+                const int64_t* htoffs = nullptr)
 {
     if (batch == 0) return 0;
-    // A launch of the persistent batched kernel over several matrices (up to n = 512) did not finish on
-    // the GPU; until its cause is found only single-matrix launches (validated on the GPU) are allowed.
-    if (batch > 1)
-        return set_err(SKH_ERR_UNSUPPORTED, "multi-matrix batched launches are disabled (unresolved GPU stall)");
-    int64_t nmax = 0;
-    for (int64_t b = 0; b < batch; b++) nmax = std::max(nmax, ns[b]);
-    const int nb = (int)pick_nb(nmax, opts, true);
-    std::vector<int64_t> order(batch);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return ns[a] > ns[b]; });
-
-    int per_cu = 0;
-    HIPCHK((hipError_t)skh_internal_batched_occupancy(nb, &per_cu));
-    per_cu = std::max(1, per_cu);
-    const int grid = (int)std::min<int64_t>((int64_t)per_cu * g_num_cu, batch);
-    const int64_t slot = skh_slot_doubles(std::max<int64_t>(nmax, 2), nb);
-
-    // control arrays: ns, offs, order (device copies)
-    size_t ctl_bytes = sizeof(int64_t) * 3 * batch;
-    int64_t* dctl = (int64_t*)g_vec.get(ctl_bytes);
-    double* slots = (double*)g_ws.get((size_t)slot * grid * sizeof(double));
     Ctl ctl;
     int rc = get_ctl(ctl);
     if (rc) return rc;
-    if (!dctl || !slots) return set_err(SKH_ERR_HIP, "hipMalloc of batched workspace failed");
-    std::vector<int64_t> h(3 * batch);
+    std::vector<int64_t> small, big;
+    for (int64_t b = 0; b < batch; b++) {
+        if (ns[b] > CMP_NMAX && ns[b] % 2 == 0) big.push_back(b);
+        else small.push_back(b);
+    }
+    std::stable_sort(small.begin(), small.end(), [&](int64_t a, int64_t b) { return ns[a] > ns[b]; });
+
+    // large matrices: one large-engine factorization each, then log|Pf| from eta, tau on the device
+    for (int64_t b : big) {
+        const int64_t n = ns[b];
+        double* M;
+        double* te;
+        if (dfac) {
+            M = dfac + offs[b];
+            te = dtaueta + htoffs[b];
+        } else {
+            double* w = (double*)g_big.get((size_t)(n * n + 2 * n) * sizeof(double));
+            if (!w) return set_err(SKH_ERR_HIP, "hipMalloc of %lld doubles failed", (long long)(n * n + 2 * n));
+            M = w;
+            te = w + n * n;
+        }
+        if (M != dA + offs[b])
+            HIPCHK(hipMemcpyAsync(M, dA + offs[b], (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        g_timer.reset();
+        g_timer.on = false;
+        rc = sytrd_large(n, M, n, te, te + n, pick_nb(n, opts), st);
+        if (rc) return rc;
+        HIPCHK((hipError_t)skh_internal_logpf_from(te + n, te, n, dlogpf + b, dsgn + b, st));
+    }
+    if (!big.empty()) {
+        HIPCHK(hipStreamSynchronize(st));
+        rc = check_barrier_flag();
+        if (rc) return rc;
+    }
+    if (small.empty()) return 0;
+
+    int64_t nmax = 2;
+    for (int64_t b : small) nmax = std::max(nmax, ns[b]);
+    const int kmax = nmax <= 512 ? 512 : 1024;
+    int per_cu = 0;
+    HIPCHK((hipError_t)skh_internal_compact_occupancy(kmax, &per_cu));
+    per_cu = std::max(1, per_cu);
+    const int64_t count = (int64_t)small.size();
+    const int grid = (int)std::min<int64_t>((int64_t)per_cu * g_num_cu, count);
+    const int64_t slot = ((nmax * nmax + 2 * kmax) + 31) & ~int64_t(31);
+    double* slots = nullptr;
+    if (!dfac) {
+        slots = (double*)g_cslot.get((size_t)slot * grid * sizeof(double));
+        if (!slots) return set_err(SKH_ERR_HIP, "hipMalloc of compact work slots failed");
+    }
+    // control arrays: ns, offs, items[, toffs] (device copies)
+    const int64_t nctl = 2 * batch + count + (dtaueta ? batch : 0);
+    const size_t ctl_bytes = sizeof(int64_t) * nctl;
+    int64_t* dctl = (int64_t*)g_bctl.get(ctl_bytes);
+    if (!dctl) return set_err(SKH_ERR_HIP, "hipMalloc of batched control arrays failed");
+    std::vector<int64_t> h(nctl);
     std::copy(ns, ns + batch, h.begin());
     std::copy(offs, offs + batch, h.begin() + batch);
-    std::copy(order.begin(), order.end(), h.begin() + 2 * batch);
+    std::copy(small.begin(), small.end(), h.begin() + 2 * batch);
+    if (dtaueta) std::copy(htoffs, htoffs + batch, h.begin() + 2 * batch + count);
     HIPCHK(hipMemcpyAsync(dctl, h.data(), ctl_bytes, hipMemcpyHostToDevice, st));
-    HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 512, st));
-    BatchArgs a;
+    HIPCHK((hipError_t)skh_internal_zero_u32(ctl.cqueue, 64, st));
+    CompactArgs a;
     a.A = dA;
     a.ns = dctl;
     a.offs = dctl + batch;
-    a.order = dctl + 2 * batch;
-    a.batch = batch;
+    a.items = dctl + 2 * batch;
+    a.count = count;
     a.slots = slots;
     a.slot_stride = slot;
-    a.nb = nb;
-    a.queue = ctl.queue;
+    a.queue = ctl.cqueue;
+    a.err = ctl.cerr;
+    a.budget = (unsigned long long)(watchdog_seconds() * 1e8);
     a.logpf = dlogpf;
     a.sgn = dsgn;
     a.fac = dfac;
     a.taueta = dtaueta;
-    a.toffs = dtoffs;
-    HIPCHK((hipError_t)skh_internal_launch_batched(&a, grid, st));
-    // the host copy of the control arrays must outlive the async copy
+    a.toffs = dtaueta ? dctl + 2 * batch + count : nullptr;
+    HIPCHK((hipError_t)skh_internal_launch_compact(&a, kmax, grid, st));
+    // the host copy of the control arrays must outlive the async copy; the flag is read after it
     HIPCHK(hipStreamSynchronize(st));
+    int herr = 0;
+    HIPCHK(hipMemcpy(&herr, ctl.cerr, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr) return set_err(SKH_ERR_BARRIER, "compact engine stopped by its watchdog after %.0f s", watchdog_seconds());
     return 0;
 }
 
@@ -375,9 +443,7 @@ int grad_run(int64_t batch, const int64_t* ns, const int64_t* offs, const double
     }
     double* dte = (double*)g_gte.get((size_t)(tt + 2 * batch + 16) * sizeof(double));
     if (!dte) return set_err(SKH_ERR_HIP, "hipMalloc failed (tau/eta)");
-    int64_t* dtoffs = (int64_t*)(dte + tt);   // stored after the tau|eta area (int64 fits in a double slot)
-    HIPCHK(hipMemcpyAsync(dtoffs, toffs.data(), batch * sizeof(int64_t), hipMemcpyHostToDevice, st));
-    int rc = batched_run(batch, ns, offs, dA, dlogpf, dsgn, opts, st, dgrad, dte, dtoffs);
+    int rc = batched_run(batch, ns, offs, dA, dlogpf, dsgn, opts, st, dgrad, dte, toffs.data());
     if (rc) return rc;
     rc = blas_on(st);
     if (rc) return rc;
@@ -480,12 +546,10 @@ int sytrd_device_locked(int64_t n, double* dA, int64_t lda, double* dtau, double
                         const skh_opts* opts)
 {
     if (n <= 1) return 0;
-    const int64_t nb = pick_nb(n, opts, engine_of(n, opts) == SKH_ENGINE_TEAM1);
+    const int64_t nb = pick_nb(n, opts);
     g_timer.reset();
     g_timer.on = opts && (opts->flags & SKH_FLAG_TIMING);
-    if (engine_of(n, opts) == SKH_ENGINE_LARGE) return sytrd_large(n, dA, lda, dtau, deta, nb, st);
-    g_timer.on = false;
-    return set_err(SKH_ERR_UNSUPPORTED, "team-of-one engine for sytrd is driven through the host entry point");
+    return sytrd_large(n, dA, lda, dtau, deta, nb, st);
 }
 
 }  // namespace
@@ -509,7 +573,7 @@ int skh_finalize(void)
     if (!g_inited) return 0;
     (void)hipStreamSynchronize(g_stream);
     for (Buf* b : {&g_mat, &g_ws, &g_vec, &g_ctl, &g_bin, &g_bout, &g_q, &g_ev, &g_gq, &g_gk, &g_gz, &g_gv, &g_gp,
-                   &g_gte})
+                   &g_gte, &g_cslot, &g_big, &g_bctl})
         b->release();
     if (g_blas) (void)rocblas_destroy_handle(g_blas);
     g_blas = nullptr;
@@ -560,19 +624,23 @@ int skh_sytrd_d(int64_t n, double* A, int64_t lda, double* tau, double* eta, int
     double* deta = dv + n;
     HIPCHK(hipMemcpy2DAsync(dA, ld * sizeof(double), A, lda * sizeof(double), n * sizeof(double), n,
                             hipMemcpyHostToDevice, st));
-    // the team-of-one engine serves batched/logpf calls; single factorizations use the large engine
-    const int eng = (opts && (opts->flags & 0xff) == SKH_ENGINE_TEAM1) ? SKH_ENGINE_TEAM1 : SKH_ENGINE_LARGE;
-    if (eng == SKH_ENGINE_LARGE) {
+    if (engine_of(n, opts) == SKH_ENGINE_LARGE) {
         g_timer.reset();
         g_timer.on = false;
-        rc = sytrd_large(n, dA, ld, dtau, deta, pick_nb(n, opts, false), st);
+        rc = sytrd_large(n, dA, ld, dtau, deta, pick_nb(n, opts), st);
         if (rc) return rc;
         HIPCHK(hipStreamSynchronize(st));
         rc = check_barrier_flag();
         if (rc) return rc;
     } else {
-        // one matrix through the batched engine, then read its factors back from the slot
-        return set_err(SKH_ERR_UNSUPPORTED, "team-of-one sytrd not wired yet");
+        // compact engine, in place (ld = n for even n); tau | eta land in dv
+        if (n % 2 == 1 || n > CMP_NMAX)
+            return set_err(SKH_ERR_UNSUPPORTED, "the compact engine takes even n <= %lld", (long long)CMP_NMAX);
+        double* dout = (double*)g_bout.get(2 * sizeof(double));
+        if (!dout) return set_err(SKH_ERR_HIP, "hipMalloc failed");
+        int64_t nn = n, off = 0, toff = 0;
+        rc = batched_run(1, &nn, &off, dA, dout, dout + 1, opts, st, dA, dv, &toff);
+        if (rc) return rc;
     }
     HIPCHK(hipMemcpy2DAsync(A, lda * sizeof(double), dA, ld * sizeof(double), n * sizeof(double), n,
                             hipMemcpyDeviceToHost, st));

@@ -5,9 +5,8 @@
 // the virtually updated column, w = tau * (A22 v + V (W^T v) - W (V^T v)), and after each
 // panel the skew rank-2k trailing update A22 += V2 W2^T - W2 V2^T (lower triangle only).
 //
-// Every phase below is written for a team of G workgroups (rank g in [0, G)).  The
-// single-matrix engine runs it with G = the whole (co-resident) grid and a grid barrier; the
-// batched engine runs it with G = 1 (one workgroup per matrix) and __syncthreads as barrier.
+// Every phase below is written for a team of G workgroups (rank g in [0, G)); the large
+// engine runs it with G = the whole (co-resident) grid and a grid barrier between phases.
 //
 // Memory: A column-major, lda even, base 16-byte aligned.  Only the strictly lower triangle
 // of the trailing block is read (the real skew diagonal is zero, the upper half is -A^T).
@@ -629,42 +628,6 @@ __device__ inline void tile_coords(const TileGrid& tg, int64_t t, int64_t& r0, i
     }
     r0 = bi * BT;
     q0 = (tg.cb0 + t) * BT;
-}
-
-// ---------------------------------------------------------------------------------------
-// One-workgroup tridiagonalization of an n x n matrix M (ld even), for the batched engine.
-// ---------------------------------------------------------------------------------------
-template <int KQ>
-__device__ void tridiag_team1(double* M, int64_t ld, int64_t n, int nb, const Work& w, double* tau, double* eta,
-                              RowShared& sh, MfmaShared& sm)
-{
-    for (int64_t c0 = 0; c0 <= n - 3; c0 += nb) {
-        const int nbp = (int)min((int64_t)nb, n - 2 - c0);
-        row_pass<KQ>(M, ld, n, c0, c0, 0, false, true, w, 0, 1, sh);
-        for (int i = 0; i < nbp; i++) {
-            const int64_t j = c0 + i;
-            __syncthreads();
-            column_scalars(M, ld, n, j, w, sh, tau, eta, true);
-            reduce_t(M, ld, n, c0, j, i, w, sh, 0, 1);
-            skew_matvec(M, ld, n, j, w, sh, 0, 1);
-            __syncthreads();
-            row_pass<KQ>(M, ld, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, 0, 1, sh);
-        }
-        __syncthreads();
-        const int64_t e = c0 + nbp;
-        const TileGrid tg = tile_grid(n, e, e);
-        for (int64_t t = 0; t < tg.total; t++) {
-            int64_t r0, q0;
-            tile_coords(tg, t, r0, q0);
-            rank2k_tile(M, ld, w.W, w.ldw, n, e, e, c0, nbp, r0, q0, sm);
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0 && n >= 2) {
-        eta[n - 2] = M[(n - 1) + (n - 2) * ld];
-        tau[n - 2] = 0.0;
-    }
-    __syncthreads();
 }
 
 // log|Pf| and sign from eta, tau of an even-n factorization (src/pfaffian.jl:170-178):

@@ -1,5 +1,5 @@
-// skh_internal.h -- structures shared between the kernels (skh_kernels.hip) and the C ABI
-// (skh_api.cpp).  Not part of the public interface.
+// skh_internal.h -- structures shared between the kernels (skh_kernels.hip, skh_compact.hip)
+// and the C ABI (skh_api.cpp).  Not part of the public interface.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -23,32 +23,35 @@ struct PanelLaunch {
     int nbmax;                // panel width bound selecting the kernel instance
 };
 
-struct BatchArgs {
-    const double* A;       // packed input matrices (column-major, lda = n_b each)
-    const int64_t* ns;     // sizes
-    const int64_t* offs;   // element offsets into A
-    const int64_t* order;  // processing order (largest first)
-    int64_t batch;
-    double* slots;         // one work slot per workgroup
-    int64_t slot_stride;   // doubles per slot
-    int nb;                // panel width
-    unsigned* queue;       // work-queue counter (zeroed before each launch)
+// Compact engine (one workgroup per matrix, n <= nmax of the instantiation).
+struct CompactArgs {
+    const double* A;             // packed inputs, column-major, matrix b at offs[b] (even), ld n_b
+    const int64_t* ns;
+    const int64_t* offs;
+    const int64_t* items;        // matrices of this launch in processing order (largest first)
+    int64_t count;
+    double* slots;               // one work slot per workgroup (when fac is null)
+    int64_t slot_stride;         // doubles per slot: >= nmax^2 + 2*NMAX of the kernel, even
+    unsigned* queue;             // work-queue counter (zeroed before each launch)
+    int* err;                    // error flag: bit 1 = watchdog
+    unsigned long long budget;   // watchdog, 100 MHz ticks (0 = none)
     double* logpf;
     double* sgn;
-    double* fac;           // optional: factors out (strictly lower, at offs[b], ld n_b)
-    double* taueta;        // optional: tau | eta out (2 n_b doubles at toffs[b])
+    double* fac;                 // optional: factorization in place here (input copied in first)
+    double* taueta;              // optional: tau | eta out (2 n_b doubles at toffs[b]); required with fac
     const int64_t* toffs;
 };
 
 extern "C" {
+int skh_internal_compact_occupancy(int nmax, int* blocks_per_cu);
+int skh_internal_launch_compact(const CompactArgs* a, int nmax, int grid, hipStream_t st);
+int skh_internal_logpf_from(const double* eta, const double* tau, int64_t n, double* logpf, double* sgn, hipStream_t st);
 int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st);
 int skh_internal_panel_occupancy(int nbmax, int* blocks_per_cu);
 int skh_internal_launch_rank2k(double* A, int64_t lda, const double* U, int64_t ldu, int64_t n, int64_t rlo,
                                int64_t clo, int64_t c0, int nbp, hipStream_t st);
 int skh_internal_launch_finish(double* A, int64_t lda, int64_t n, double* tau, double* eta, hipStream_t st);
 int skh_internal_zero_u32(unsigned* p, int count, hipStream_t st);
-int skh_internal_launch_batched(const BatchArgs* a, int grid, hipStream_t st);
-int skh_internal_batched_occupancy(int nb, int* blocks_per_cu);
 int skh_internal_mfma_probe(double* dout, hipStream_t st);
 int skh_internal_eye(double* Q, int64_t ldq, int64_t n, hipStream_t st);
 int skh_internal_form_v(const double* F, int64_t ldf, int64_t n, int64_t c0, int nbp, double* V, int64_t ldv,
@@ -67,12 +70,3 @@ int skh_internal_skew_out(const double* K, int64_t kstride, double* const* G, in
                           hipStream_t st);
 }
 
-// doubles needed by one batched work slot for matrices up to nmax with panel width nb
-inline int64_t skh_slot_doubles(int64_t nmax, int nb)
-{
-    const int64_t ld = (nmax + 1) & ~int64_t(1);
-    const int64_t nt = (nmax + skh::TB - 1) / skh::TB + 1;
-    int64_t s = ld * nmax + ld * nb + 2 * ld + nt + nt * 2 * skh::NBMAX + 2 * skh::NBMAX;
-    s += nt * ld + nt * ld;   // Prow, Pcol
-    return (s + 31) & ~int64_t(31);   // keep every slot 256-byte aligned
-}

@@ -5,9 +5,8 @@
 //                  (src/hessenberg.jl:92-131) for one matrix.
 //  * k_rank2k    : skew rank-2k trailing update on FP64 MFMA (src/hessenberg.jl:158-166),
 //                  lower triangle only, no Omega buffer.
-//  * k_batched   : one workgroup per matrix (work queue over a size-sorted batch): the same
-//                  panel phases with G = 1, the MFMA update, then log|Pf| and sign
-//                  (src/pfaffian.jl:167-179) and optionally the gradient -1/2 A^{-1}.
+//  * Q formation and the batched gradient helpers (compact WY with rocBLAS products).
+// The compact engine for batches of small matrices is in skh_compact.hip.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -120,73 +119,6 @@ __global__ void k_finish(double* A, int64_t lda, int64_t n, double* tau, double*
 __global__ void k_zero_u32(unsigned* p, int count)
 {
     for (int k = threadIdx.x; k < count; k += blockDim.x) p[k] = 0u;
-}
-
-template <int KQ>
-__global__ __launch_bounds__(WG) void k_batched(BatchArgs a)
-{
-    __shared__ RowShared sh;
-    __shared__ MfmaShared sm;
-    __shared__ int64_t s_idx;
-    const int tid = threadIdx.x;
-    const int g = blockIdx.x;
-    double* slot = a.slots + (int64_t)g * a.slot_stride;
-    for (;;) {
-        if (tid == 0) s_idx = (int64_t)atomicAdd(a.queue, 1u);
-        __syncthreads();
-        const int64_t idx = s_idx;
-        __syncthreads();
-        if (idx >= a.batch) break;
-        const int64_t b = a.order[idx];
-        const int64_t n = a.ns[b];
-        if (n % 2 == 1) {
-            if (tid == 0) {
-                a.logpf[b] = -INFINITY;
-                a.sgn[b] = 0.0;
-            }
-            continue;
-        }
-        if (n == 0) {
-            if (tid == 0) {
-                a.logpf[b] = 0.0;
-                a.sgn[b] = 1.0;
-            }
-            continue;
-        }
-        const int64_t ld = (n + 1) & ~int64_t(1);
-        // slot layout: M (ld x n) | W (ld x nb) | tau, eta (n) | Pnorm | PS | Tv | Prow | Pcol
-        double* M = slot;
-        Work w;
-        w.W = M + ld * n;
-        w.ldw = ld;
-        double* tau = w.W + ld * a.nb;
-        double* eta = tau + ld;
-        const int64_t nt = cdiv(n, TB) + 1;
-        w.Pnorm = eta + ld;
-        w.PS = w.Pnorm + nt;
-        w.Tv = w.PS + nt * 2 * NBMAX;
-        w.Prow = w.Tv + 2 * NBMAX;
-        w.ldp = ld;
-        w.Pcol = w.Prow + nt * ld;
-        // strictly lower triangle of the input into the slot
-        const double* src = a.A + a.offs[b];
-        for (int64_t c = 0; c < n; c++)
-            for (int64_t r = c + 1 + tid; r < n; r += WG) M[r + c * ld] = src[r + c * n];
-        __syncthreads();
-        tridiag_team1<KQ>(M, ld, n, a.nb, w, tau, eta, sh, sm);
-        logpf_reduce(eta, tau, n, a.logpf + b, a.sgn + b);
-        if (a.fac) {
-            double* dst = a.fac + a.offs[b];
-            for (int64_t c = 0; c < n; c++)
-                for (int64_t r = c + 1 + tid; r < n; r += WG) dst[r + c * n] = M[r + c * ld];
-            double* te = a.taueta + a.toffs[b];
-            for (int64_t i = tid; i < n - 1; i += WG) {
-                te[i] = tau[i];
-                te[n + i] = eta[i];
-            }
-            __syncthreads();
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -360,13 +292,6 @@ static const void* panel_kernel_for(int nbmax)
     return (const void*)k_panel<32>;
 }
 
-static const void* batched_kernel_for(int nb)
-{
-    if (nb <= 32) return (const void*)k_batched<8>;
-    if (nb <= 64) return (const void*)k_batched<16>;
-    return (const void*)k_batched<32>;
-}
-
 extern "C" int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st)
 {
     void* args[] = {(void*)&pl->A,   (void*)&pl->lda, (void*)&pl->n,   (void*)&pl->c0,  (void*)&pl->nbp,
@@ -401,18 +326,6 @@ extern "C" int skh_internal_zero_u32(unsigned* p, int count, hipStream_t st)
 {
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, st, p, count);
     return (int)hipGetLastError();
-}
-
-extern "C" int skh_internal_launch_batched(const BatchArgs* a, int grid, hipStream_t st)
-{
-    BatchArgs copy = *a;
-    void* args[] = {(void*)&copy};
-    return (int)hipLaunchKernel(batched_kernel_for(a->nb), dim3(grid), dim3(WG), args, 0, st);
-}
-
-extern "C" int skh_internal_batched_occupancy(int nb, int* blocks_per_cu)
-{
-    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, batched_kernel_for(nb), WG, 0);
 }
 
 extern "C" int skh_internal_mfma_probe(double* dout, hipStream_t st)

@@ -96,6 +96,53 @@ def test_sytrd_large_vs_oracle(n, nb):
     np.testing.assert_array_equal(F[iu], A[iu])
 
 
+@pytest.mark.parametrize("n", [2, 4, 6, 10, 62, 64, 66, 128, 130, 200, 256, 512, 1024])
+def test_sytrd_compact_vs_oracle(n):
+    """The compact engine (one workgroup per matrix, unblocked fused update + matvec)."""
+    say("sytrd compact n=%d" % n)
+    A = skew_gauss(n, 20 * n + 1)
+    F, tau, eta, nz = gpu_sytrd(A, 0, skl.ENGINE_TEAM1)
+    Fo, tauo, etao = oracle.skewblockedhess(A)
+    nA = np.linalg.norm(A)
+    np.testing.assert_allclose(eta, etao, rtol=0, atol=1e-11 * nA)
+    np.testing.assert_allclose(tau, tauo, rtol=0, atol=1e-11)
+    assert nz == int(np.count_nonzero(tau))
+    check_fact(A, F, tau, eta)
+    for i in range(n - 2):
+        assert F[i + 1, i] == 1.0
+    assert F[n - 1, n - 2] == eta[-1] and tau[-1] == 0.0
+    iu = np.triu_indices(n)
+    np.testing.assert_array_equal(F[iu], A[iu])
+
+
+def test_compact_edge_conventions(golden):
+    say("compact edge conventions")
+    cases = [np.array(golden["readme"]["A"], float)]
+    A = np.zeros((4, 4))
+    A[1:, 0] = 1
+    A[0, 1:] = -1
+    cases.append(A)
+    A = np.zeros((6, 6))
+    A[1, 0], A[0, 1] = 3.0, -3.0        # alpha != 0, xnorm = 0 -> tau = 2
+    A[5, 2], A[2, 5] = 2.0, -2.0        # alpha = +0 with xnorm > 0 -> beta = +xnorm
+    cases.append(A)
+    cases.append(np.zeros((8, 8)))      # all-zero columns -> tau = 0, v = 0
+    for A in cases:
+        F, tau, eta, nz = gpu_sytrd(A, 0, skl.ENGINE_TEAM1)
+        Fo, tauo, etao = oracle.skewblockedhess(A)
+        np.testing.assert_allclose(eta, etao, rtol=0, atol=1e-12 * max(1.0, np.linalg.norm(A)))
+        np.testing.assert_array_equal(tau != 0, tauo != 0)
+        np.testing.assert_allclose(tau, tauo, rtol=0, atol=1e-12)
+        check_fact(A, F, tau, eta)
+    A = sparse_to_dense(golden["issue130"]["m66"])
+    F, tau, eta, nz = gpu_sytrd(A, 0, skl.ENGINE_TEAM1)
+    check_fact(A, F, tau, eta)
+    g = golden["readme"]
+    F, tau, eta, nz = gpu_sytrd(np.array(g["A"], float), 0, skl.ENGINE_TEAM1)
+    np.testing.assert_allclose(eta, g["eta"], rtol=1e-14)
+    np.testing.assert_allclose(oracle.form_q(F, tau), np.array(g["Q"]), atol=1e-14)
+
+
 @pytest.mark.parametrize("n", [512, 1000, 2049])
 def test_sytrd_medium(n):
     say("sytrd medium n=%d" % n)
@@ -190,7 +237,6 @@ def test_eigvals_vs_oracle(n):
     assert np.max(np.abs(lam - lamo)) <= 10 * n * EPS * np.linalg.norm(A, 2)
 
 
-@pytest.mark.skip(reason="the persistent batched kernel stalled on the GPU with a mixed-size batch; disabled until the cause is found (CPU emulation of the workgroup is clean)")
 def test_batched_vs_single():
     say("batched logpf")
     rng = np.random.default_rng(4)
@@ -213,20 +259,39 @@ def test_batched_vs_single():
             assert abs(lp[k] - lpo) <= 1e-10 * max(1.0, abs(lpo)), (k, n, lp[k], lpo)
 
 
-@pytest.mark.skip(reason="the persistent batched kernel stalled on the GPU with a mixed-size batch; disabled until the cause is found (CPU emulation of the workgroup is clean)")
-def test_batched_nb_variants():
-    say("batched nb variants")
+def test_batched_large_members():
+    """Matrices above the compact engine's size limit go through the large engine in the same call."""
+    say("batched with n > 1024 members")
     rng = np.random.default_rng(5)
     mats = []
-    for n in (64, 100, 250):
-        B = rng.standard_normal((n, n))
+    for n in (1026, 64, 1030, 7, 250):
+        B = rng.standard_normal((n, n)) / np.sqrt(n)
         mats.append(B - B.T)
-    ref = [oracle.logabspfaffian(M) for M in mats]
-    for nb in (4, 16, 33, 64):
-        lp, sg = skl.logabspfaffian_batched(mats, nb=nb)
-        for k in range(len(mats)):
-            assert sg[k] == ref[k][1]
-            assert abs(lp[k] - ref[k][0]) <= 1e-10 * abs(ref[k][0])
+    lp, sg = skl.logabspfaffian_batched(mats)
+    for k, M in enumerate(mats):
+        lpo, sgo = oracle.logabspfaffian(M)
+        assert sg[k] == sgo
+        if M.shape[0] % 2:
+            assert lp[k] == -np.inf
+        else:
+            assert abs(lp[k] - lpo) <= 1e-10 * max(1.0, abs(lpo))
+
+
+@pytest.mark.parametrize("n", [128, 256])
+def test_batched_equal_sizes_deterministic(n):
+    """Same matrices in different batch positions and workgroups give bit-identical results."""
+    say("batched equal sizes n=%d" % n)
+    rng = np.random.default_rng(n)
+    mats = []
+    for _ in range(6):
+        B = rng.standard_normal((n, n)) / np.sqrt(n)
+        mats.append(B - B.T)
+    lp1, sg1 = skl.logabspfaffian_batched(mats + mats[::-1])
+    np.testing.assert_array_equal(lp1[:6], lp1[6:][::-1])
+    np.testing.assert_array_equal(sg1[:6], sg1[6:][::-1])
+    for k, M in enumerate(mats):
+        lpo, sgo = oracle.logabspfaffian(M)
+        assert sg1[k] == sgo and abs(lp1[k] - lpo) <= 1e-10 * max(1.0, abs(lpo))
 
 
 @pytest.mark.parametrize("n", [2, 3, 5, 8, 65, 130, 300])
@@ -286,7 +351,6 @@ def _constructed(n, rng):
     return A, float(np.sum(np.log(s))), float(np.sign(np.linalg.det(Q)))
 
 
-@pytest.mark.skip(reason="the persistent batched kernel stalled on the GPU with a mixed-size batch; disabled until the cause is found (CPU emulation of the workgroup is clean)")
 def test_grad_batched_vs_inverse_and_fd():
     """Gradient bar: vs -1/2 inv(A) (LU, the reference's inv(A::SkewHermitian) route) within
     1e-10 relative (Frobenius), and vs central differences h = 1e-6 on skew pairs within 1e-6."""
@@ -315,7 +379,6 @@ def test_grad_batched_vs_inverse_and_fd():
             assert abs(fd - 2 * grads[k][i, j]) <= 1e-6 * max(1.0, abs(fd)), (n, i, j, fd, 2 * grads[k][i, j])
 
 
-@pytest.mark.skip(reason="the persistent batched kernel stalled on the GPU with a mixed-size batch; disabled until the cause is found (CPU emulation of the workgroup is clean)")
 def test_grad_mixed_and_odd():
     say("grad mixed sizes")
     rng = np.random.default_rng(12)
@@ -335,7 +398,6 @@ def test_grad_mixed_and_odd():
         assert np.linalg.norm(grads[k] - G0) <= 1e-9 * np.linalg.norm(G0)
 
 
-@pytest.mark.skip(reason="the persistent batched kernel stalled on the GPU with a mixed-size batch; disabled until the cause is found (CPU emulation of the workgroup is clean)")
 def test_autograd_function():
     say("autograd")
     import torch
