@@ -11,9 +11,12 @@ Also reported:
   mfma_update   the rank-2k trailing update kernel (FP64 MFMA) against the FP64 matrix peak
   cpu_baseline  the oracle (C restatement of the reference algorithm, OpenBLAS, all host
                 threads) timed on a bounded sample: one full factorization at n=4096
-  batched       (with --batched) 4096 uneven-size matrices, log|Pf| + sign, matrices/s
+  batched       4096 uneven-size matrices (n in {32,...,512}), log|Pf| + sign through one tuned
+                plan replayed from a HIP graph, matrices/s, against the per-matrix GPU loop
+  grad          1024 matrices n = 256, log|Pf| + hand-written gradient, matrices/s, against the
+                per-matrix GPU loop
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 8192] [--nb 0] [--no-cpu] [--batched]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 8192] [--nb 0] [--no-cpu] [--no-batched]
 """
 import argparse
 import ctypes
@@ -21,6 +24,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "skewlinearalgebra.jl_amd"))
@@ -52,43 +57,180 @@ def cpu_baseline(n_cpu):
                       % (n_cpu, dt, threads)}
 
 
-def batched_bench(torch, skl, _lib, steps):
-    """4096 real skew matrices, n uniform over {32,64,...,512}, B iid N(0,1/n), in HBM."""
+def cfg4_batch(torch, dev, count=4096, seed=314159):
+    """4096 real skew matrices, n uniform over {32, 64, ..., 512}, A = B - B^T, B iid N(0, 1/n),
+    packed column-major at even offsets in one HBM buffer."""
     import numpy as np
-    dev = torch.device("cuda:0")
-    g = torch.Generator(device=dev)
-    g.manual_seed(314159)
-    rng = np.random.default_rng(314159)
-    count = 4096
+    rng = np.random.default_rng(seed)
     ns = (rng.integers(1, 17, size=count) * 32).astype(np.int64)
     offs = np.zeros(count, dtype=np.int64)
     offs[1:] = np.cumsum(ns[:-1] ** 2)
     total = int(offs[-1] + ns[-1] ** 2)
     buf = torch.empty(total, dtype=torch.float64, device=dev)
-    for k in range(count):
-        n = int(ns[k])
-        Bm = torch.randn((n, n), generator=g, dtype=torch.float64, device=dev) / (n ** 0.5)
-        buf[offs[k]:offs[k] + n * n] = (Bm - Bm.T).T.reshape(-1)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    for n in np.unique(ns):
+        idx = np.nonzero(ns == n)[0]
+        Bm = torch.randn((len(idx), int(n), int(n)), generator=g, dtype=torch.float64, device=dev) / (float(n) ** 0.5)
+        Am = Bm - Bm.transpose(1, 2)
+        for q, k in enumerate(idx):
+            buf[offs[k]:offs[k] + n * n] = Am[q].T.reshape(-1)
+    return ns, offs, buf
+
+
+def cfg5_batch(torch, dev, count=1024, n=256, seed=314159):
+    """1024 real skew matrices n = 256, A = Q blockdiag([0 s_i; -s_i 0]) Q^T, Q Haar (QR of a
+    Gaussian with the sign fix, Mezzadri 2007), s_i log-uniform on [0.1, 10]; exact
+    log|Pf| = sum log s_i.  Built on the host (numpy, seeded) and copied to HBM once."""
+    rng = np.random.default_rng(seed)
+    A = np.empty((count, n, n))
+    exact = np.empty(count)
+    Bm = np.zeros((n, n))
+    k = np.arange(n // 2)
+    for b in range(count):
+        Q, R = np.linalg.qr(rng.standard_normal((n, n)))
+        Q = Q * np.sign(np.diag(R))
+        s = np.exp(rng.uniform(np.log(0.1), np.log(10.0), size=n // 2))
+        Bm[2 * k, 2 * k + 1] = s
+        Bm[2 * k + 1, 2 * k] = -s
+        M = Q @ Bm @ Q.T
+        A[b] = 0.5 * (M - M.T)
+        exact[b] = np.sum(np.log(s))
+    return torch.tensor(A, device=dev), torch.tensor(exact, device=dev)
+
+
+def timed(torch, fn, reps):
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        fn()
+    e1.record(st)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def batched_bench(torch, skl, _lib, steps):
+    """cfg4: the whole batch through one tuned plan replayed from a HIP graph, against the
+    per-matrix GPU loop (same kernels, one launch per matrix) on a stratified sample."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    ns, offs, buf = cfg4_batch(torch, dev)
+    count = len(ns)
     lp = torch.empty(count, dtype=torch.float64, device=dev)
     sg = torch.empty(count, dtype=torch.float64, device=dev)
+    st = torch.cuda.current_stream().cuda_stream
+    plan = skl.LogPfPlan(ns, offs)
+    tuned_ms = plan.tune(buf.data_ptr(), lp.data_ptr(), sg.data_ptr(), st)
+    plan.status(st)
+    info = plan.info()
+    graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        plan.run(buf.data_ptr(), lp.data_ptr(), sg.data_ptr(), side.cuda_stream)   # warm-up off-graph
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph):
+        plan.run(buf.data_ptr(), lp.data_ptr(), sg.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    graph.replay()
+    torch.cuda.synchronize()
+    plan.status(st)
+    ref_lp = lp.clone()
+    ms = timed(torch, graph.replay, max(3, steps))
+    plan.status(st)
+    same = bool(torch.equal(lp, ref_lp))
+    # per-matrix GPU loop on every 16th matrix (all 16 sizes represented), one call per matrix
+    o = _lib.opts()
+    L = _lib.lib()
+    sample = np.arange(0, count, 16)
+    one_n = np.zeros(1, dtype=np.int64)
+    one_off = np.zeros(1, dtype=np.int64)
+
+    def loop():
+        for k in sample:
+            one_n[0] = ns[k]
+            one_off[0] = 0
+            rc = L.skh_logpf_batched_d_device(1, one_n.ctypes.data, one_off.ctypes.data,
+                                              buf.data_ptr() + 8 * int(offs[k]), lp.data_ptr() + 8 * int(k),
+                                              sg.data_ptr() + 8 * int(k), st, ctypes.byref(o))
+            _lib.check("skh_logpf_batched_d_device", rc)
+
+    loop()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loop()
+    torch.cuda.synchronize()
+    loop_s = time.perf_counter() - t0
+    plain_rate = len(sample) / loop_s
+    flops = float(np.sum(4.0 / 3.0 * ns.astype(np.float64) ** 3))
+    loop_agree = bool(torch.equal(lp[sample], ref_lp[sample]))
+    return {"metric": "batched log|Pf|+sign, 4096 matrices n in {32,...,512}", "value": count / (ms * 1e-3),
+            "unit": "matrices/s", "ms_per_batch": ms, "gflops": flops / (ms * 1e-3) / 1e9,
+            "launch": "one tuned plan (%d workgroups, kernel NMAX %d) replayed from a HIP graph; tuner best %.2f ms"
+                      % (info["workgroups"], info["kernel_nmax"], tuned_ms),
+            "graph_replays_identical": same,
+            "plain": {"value": plain_rate, "unit": "matrices/s",
+                      "sample": "per-matrix GPU loop (one device call per matrix) over every 16th matrix (%d)"
+                                % len(sample), "results_identical": loop_agree},
+            "speedup_vs_plain": (count / (ms * 1e-3)) / plain_rate}
+
+
+def grad_bench(torch, skl, _lib, steps):
+    """cfg5: forward log|Pf| + hand-written gradient for 1024 x n=256 in one call, against the
+    per-matrix GPU loop (same entry point, one matrix per call) on a sample."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    A, exact = cfg5_batch(torch, dev)
+    count, n = A.shape[0], A.shape[1]
+    At = A.transpose(1, 2).contiguous()   # column-major storage of each matrix
+    ns = np.full(count, n, dtype=np.int64)
+    offs = np.arange(count, dtype=np.int64) * n * n
+    lp = torch.empty(count, dtype=torch.float64, device=dev)
+    sg = torch.empty(count, dtype=torch.float64, device=dev)
+    gr = torch.empty_like(At)
     st = torch.cuda.current_stream().cuda_stream
     o = _lib.opts()
     L = _lib.lib()
 
     def run():
-        rc = L.skh_logpf_batched_d_device(count, ns.ctypes.data, offs.ctypes.data, buf.data_ptr(), lp.data_ptr(),
-                                          sg.data_ptr(), st, ctypes.byref(o))
-        _lib.check("skh_logpf_batched_d_device", rc)
+        rc = L.skh_logpf_grad_batched_d_device(count, ns.ctypes.data, offs.ctypes.data, At.data_ptr(), lp.data_ptr(),
+                                               sg.data_ptr(), gr.data_ptr(), st, ctypes.byref(o))
+        _lib.check("skh_logpf_grad_batched_d_device", rc)
 
     run()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        run()
+    ms = timed(torch, run, max(3, steps))
+    lp_err = float(torch.max(torch.abs(lp - exact) / torch.clamp(torch.abs(exact), min=1.0)))
+    # gradient check on 4 matrices: -1/2 A^{-1} (LU on the GPU, float64)
+    G = gr[:4].transpose(1, 2)
+    G0 = -0.5 * torch.linalg.inv(A[:4])
+    g_err = float(torch.max(torch.linalg.norm(G - G0, dim=(1, 2)) / torch.linalg.norm(G0, dim=(1, 2))))
+    sample = 32
+    one_n = np.array([n], dtype=np.int64)
+    one_off = np.zeros(1, dtype=np.int64)
+
+    def loop():
+        for k in range(sample):
+            rc = L.skh_logpf_grad_batched_d_device(1, one_n.ctypes.data, one_off.ctypes.data,
+                                                   At.data_ptr() + 8 * k * n * n, lp.data_ptr() + 8 * k,
+                                                   sg.data_ptr() + 8 * k, gr.data_ptr() + 8 * k * n * n, st,
+                                                   ctypes.byref(o))
+            _lib.check("skh_logpf_grad_batched_d_device", rc)
+
+    loop()
     torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / steps
-    return {"metric": "batched log|Pf|+sign, 4096 matrices n in {32..512}", "value": count / dt,
-            "unit": "matrices/s", "ms_per_batch": dt * 1e3, "flop_per_batch": float(np.sum(4.0 / 3.0 * ns ** 3.0))}
+    t0 = time.perf_counter()
+    loop()
+    torch.cuda.synchronize()
+    plain_rate = sample / (time.perf_counter() - t0)
+    return {"metric": "batched log|Pf| + gradient, 1024 matrices n=256", "value": count / (ms * 1e-3),
+            "unit": "matrices/s", "ms_per_batch": ms, "logpf_max_rel_err_vs_exact": lp_err,
+            "grad_max_rel_err_vs_inverse": g_err,
+            "plain": {"value": plain_rate, "unit": "matrices/s",
+                      "sample": "per-matrix GPU loop (one device call per matrix), first %d matrices" % sample},
+            "speedup_vs_plain": (count / (ms * 1e-3)) / plain_rate}
 
 
 def main():
@@ -100,7 +242,7 @@ def main():
     ap.add_argument("--nb", type=int, default=0)
     ap.add_argument("--cpu-n", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--batched", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched log|Pf| and gradient lines")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the single-matrix tridiagonalization is a one-GPU "
@@ -194,8 +336,13 @@ def main():
         "check": {"frobenius_rel_err": rel},
         "wall_s": wall,
     }
-    if args.batched:
-        res["batched"] = batched_bench(torch, skl, _lib, max(1, args.steps))
+    if not args.no_batched:
+        log("batched log|Pf| (4096 matrices) ...")
+        res["batched"] = batched_bench(torch, skl, _lib, args.steps)
+        log(json.dumps(res["batched"]))
+        log("batched gradient (1024 x 256) ...")
+        res["grad"] = grad_bench(torch, skl, _lib, args.steps)
+        log(json.dumps(res["grad"]))
     if not args.no_cpu:
         log("cpu baseline (n=%d) ..." % args.cpu_n)
         res["cpu_baseline"] = cpu_baseline(args.cpu_n)

@@ -94,6 +94,28 @@ int skh_logpf_grad_batched_d_device(int64_t batch, const int64_t *n, const int64
                                     double *dlogpf, double *dsgn, double *dgrad, void *stream, const skh_opts *opts);
 
 /*
+ * Batched log|Pf| plans (no reference counterpart: the reference user loops logabspfaffian,
+ * src/pfaffian.jl:236).  The sizes and offsets of the batch are fixed at creation (even offsets,
+ * every even n <= 1024) and uploaded once; the plan owns its device work slots and control
+ * words.  skh_logpf_batched_plan_exec enqueues one evaluation of the whole batch on `stream`
+ * (a fill kernel and one persistent launch) without allocating, synchronising or copying from
+ * the host, so it can be captured into a hipGraph and replayed.  skh_logpf_batched_plan_tune
+ * times the kernel configurations (work-group count, kernel instance) on real data and keeps
+ * the fastest (*best_ms may be NULL).  skh_plan_status waits for `stream` and reports a
+ * watchdog stop (SKH_ERR_BARRIER).  Overlapping executions of one plan race (they share its
+ * slots and queue).  skh_plan_info: {work-groups, kernel NMAX, tuned, matrices}.
+ */
+typedef struct skh_plan_s *skh_plan;
+int skh_logpf_batched_plan_create(int64_t batch, const int64_t *n, const int64_t *offset, const skh_opts *opts,
+                                  skh_plan *plan);
+int skh_logpf_batched_plan_exec(skh_plan plan, const double *dA, double *dlogpf, double *dsgn, void *stream);
+int skh_logpf_batched_plan_tune(skh_plan plan, const double *dA, double *dlogpf, double *dsgn, void *stream,
+                                double *best_ms);
+int skh_plan_status(skh_plan plan, void *stream);
+int skh_plan_info(skh_plan plan, int64_t *out4);
+int skh_plan_destroy(skh_plan plan);
+
+/*
  * Explicit Q = H_0 ... H_{n-2} (n x n) from the factors and tau of skh_sytrd_d (device memory).
  * Replaces Matrix(F.Q) for F = hessenberg(A::SkewHermitian) (HessenbergQ ctor src/hessenberg.jl:3,
  * used by skeweigen! src/eigen.jl:85; LAPACK orgtr('L') convention: the unit of each reflector is

@@ -554,6 +554,61 @@ int sytrd_device_locked(int64_t n, double* dA, int64_t lda, double* dtau, double
 
 }  // namespace
 
+
+// ----------------------------------------------------------------------------------------
+// Batched log|Pf| plans: sizes fixed at creation; exec enqueues without allocating,
+// synchronising or copying from the host, so it can be captured into a hipGraph.
+// ----------------------------------------------------------------------------------------
+struct skh_plan_s {
+    int64_t batch = 0;
+    int64_t count = 0;
+    int64_t nmax = 0;
+    int kmax = 512;
+    int grid = 0;
+    int64_t slot = 0;
+    int64_t* dctl = nullptr;   // ns | offs | items
+    double* slots = nullptr;
+    unsigned* ctl = nullptr;   // [0] queue, [32] error flag
+    double flops = 0.0;
+    int tuned = 0;
+};
+
+namespace {
+
+int plan_enqueue(skh_plan p, const double* dA, double* dlogpf, double* dsgn, hipStream_t st, int grid, int kmax)
+{
+    HIPCHK((hipError_t)skh_internal_zero_u32(p->ctl, 64, st));
+    CompactArgs a;
+    a.A = dA;
+    a.ns = p->dctl;
+    a.offs = p->dctl + p->batch;
+    a.items = p->dctl + 2 * p->batch;
+    a.count = p->count;
+    a.slots = p->slots;
+    a.slot_stride = p->slot;
+    a.queue = p->ctl;
+    a.err = (int*)(p->ctl + 32);
+    a.budget = (unsigned long long)(watchdog_seconds() * 1e8);
+    a.logpf = dlogpf;
+    a.sgn = dsgn;
+    a.fac = nullptr;
+    a.taueta = nullptr;
+    a.toffs = nullptr;
+    HIPCHK((hipError_t)skh_internal_launch_compact(&a, kmax, grid, st));
+    return 0;
+}
+
+void plan_free(skh_plan p)
+{
+    if (!p) return;
+    if (p->dctl) (void)hipFree(p->dctl);
+    if (p->slots) (void)hipFree(p->slots);
+    if (p->ctl) (void)hipFree(p->ctl);
+    delete p;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* skh_version(void) { return "skewhip 0.1.0 (gfx950, FP64)"; }
@@ -907,6 +962,154 @@ int skh_diag_mfma_probe(double* out768)
     HIPCHK((hipError_t)skh_internal_mfma_probe(d, g_stream));
     HIPCHK(hipMemcpyAsync(out768, d, 768 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int skh_logpf_batched_plan_create(int64_t batch, const int64_t* n, const int64_t* offset, const skh_opts* opts,
+                                  skh_plan* plan)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (batch < 1) return set_err(-1, "batch must be >= 1");
+    if (!n || !offset) return set_err(-2, "NULL argument");
+    if (!plan) return set_err(-5, "plan is NULL");
+    *plan = nullptr;
+    int64_t nmax = 2;
+    for (int64_t b = 0; b < batch; b++) {
+        if (n[b] < 0) return set_err(-2, "n[%lld] < 0", (long long)b);
+        if (n[b] > CMP_NMAX && n[b] % 2 == 0)
+            return set_err(SKH_ERR_UNSUPPORTED, "plans take even n <= %lld (n[%lld] = %lld); use "
+                           "skh_logpf_batched_d_device", (long long)CMP_NMAX, (long long)b, (long long)n[b]);
+        if (offset[b] < 0 || (offset[b] & 1)) return set_err(-3, "offset[%lld] must be >= 0 and even", (long long)b);
+        if (n[b] % 2 == 0) nmax = std::max(nmax, n[b]);
+    }
+    (void)opts;
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    skh_plan p = new skh_plan_s();
+    p->batch = batch;
+    p->count = batch;
+    p->nmax = nmax;
+    p->kmax = nmax <= 512 ? 512 : 1024;
+    int per_cu = 0;
+    hipError_t e = (hipError_t)skh_internal_compact_occupancy(p->kmax, &per_cu);
+    if (e != hipSuccess) { plan_free(p); return set_err(SKH_ERR_HIP, "occupancy query failed"); }
+    per_cu = std::max(1, per_cu);
+    p->grid = (int)std::min<int64_t>((int64_t)per_cu * g_num_cu, batch);
+    p->slot = ((nmax * nmax + 2 * 1024) + 31) & ~int64_t(31);   // fits either kernel instance
+    std::vector<int64_t> order(batch);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return n[a] > n[b]; });
+    std::vector<int64_t> h(3 * batch);
+    std::copy(n, n + batch, h.begin());
+    std::copy(offset, offset + batch, h.begin() + batch);
+    std::copy(order.begin(), order.end(), h.begin() + 2 * batch);
+    for (int64_t b = 0; b < batch; b++) p->flops += 4.0 / 3.0 * (double)n[b] * (double)n[b] * (double)n[b];
+    const int64_t max_grid = (int64_t)per_cu * g_num_cu;
+    if (hipMalloc((void**)&p->dctl, sizeof(int64_t) * 3 * batch) != hipSuccess ||
+        hipMalloc((void**)&p->slots, sizeof(double) * (size_t)p->slot * (size_t)std::min<int64_t>(max_grid, batch)) !=
+            hipSuccess ||
+        hipMalloc((void**)&p->ctl, 256) != hipSuccess) {
+        plan_free(p);
+        return set_err(SKH_ERR_HIP, "hipMalloc of the plan's buffers failed");
+    }
+    if (hipMemcpy(p->dctl, h.data(), sizeof(int64_t) * 3 * batch, hipMemcpyHostToDevice) != hipSuccess) {
+        plan_free(p);
+        return set_err(SKH_ERR_HIP, "upload of the plan's control arrays failed");
+    }
+    *plan = p;
+    return 0;
+}
+
+int skh_logpf_batched_plan_exec(skh_plan plan, const double* dA, double* dlogpf, double* dsgn, void* stream)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!plan) return set_err(-1, "plan is NULL");
+    if (!dA || !dlogpf || !dsgn) return set_err(-2, "NULL device pointer");
+    return plan_enqueue(plan, dA, dlogpf, dsgn, (hipStream_t)stream, plan->grid, plan->kmax);
+}
+
+int skh_logpf_batched_plan_tune(skh_plan plan, const double* dA, double* dlogpf, double* dsgn, void* stream,
+                                double* best_ms)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!plan) return set_err(-1, "plan is NULL");
+    if (!dA || !dlogpf || !dsgn) return set_err(-2, "NULL device pointer");
+    hipStream_t st = (hipStream_t)stream;
+    int per_cu = 0;
+    HIPCHK((hipError_t)skh_internal_compact_occupancy(plan->kmax, &per_cu));
+    per_cu = std::max(1, per_cu);
+    struct Cand { int grid, kmax; };
+    std::vector<Cand> cands;
+    for (int k : {512, 1024}) {
+        if (k < plan->nmax) continue;
+        int pc = 0;
+        HIPCHK((hipError_t)skh_internal_compact_occupancy(k, &pc));
+        pc = std::max(1, pc);
+        for (int per = 1; per <= pc; per++) {
+            const int g = (int)std::min<int64_t>((int64_t)per * g_num_cu, plan->count);
+            bool dup = false;
+            for (auto& c : cands) dup |= (c.grid == g && c.kmax == k);
+            if (!dup) cands.push_back({g, k});
+        }
+    }
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    float best = 1e30f;
+    Cand pick{plan->grid, plan->kmax};
+    for (const Cand& c : cands) {
+        int rc = plan_enqueue(plan, dA, dlogpf, dsgn, st, c.grid, c.kmax);   // warm-up
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(e0, st));
+        for (int r = 0; r < 3; r++) {
+            rc = plan_enqueue(plan, dA, dlogpf, dsgn, st, c.grid, c.kmax);
+            if (rc) return rc;
+        }
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        ms /= 3.f;
+        if (ms < best) {
+            best = ms;
+            pick = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    plan->grid = pick.grid;
+    plan->kmax = pick.kmax;
+    plan->tuned = 1;
+    if (best_ms) *best_ms = best;
+    return 0;
+}
+
+int skh_plan_status(skh_plan plan, void* stream)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!plan) return set_err(-1, "plan is NULL");
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    int herr = 0;
+    HIPCHK(hipMemcpy(&herr, plan->ctl + 32, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr) return set_err(SKH_ERR_BARRIER, "compact engine stopped by its watchdog");
+    return 0;
+}
+
+int skh_plan_info(skh_plan plan, int64_t* out4)
+{
+    if (!plan || !out4) return set_err(-1, "NULL argument");
+    out4[0] = plan->grid;
+    out4[1] = plan->kmax;
+    out4[2] = plan->tuned;
+    out4[3] = plan->count;
+    return 0;
+}
+
+int skh_plan_destroy(skh_plan plan)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (plan) (void)hipDeviceSynchronize();
+    plan_free(plan);
     return 0;
 }
 

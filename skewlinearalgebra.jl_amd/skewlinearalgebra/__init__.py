@@ -28,7 +28,7 @@ __all__ = [
     "hessenberg", "hessenberg_", "eigvals", "eigvals_", "eigen", "eigen_", "Eigen", "pfaffian", "pfaffian_",
     "logabspfaffian", "logabspfaffian_", "logabspfaffian_batched", "logabspfaffian_grad_batched",
     "LogAbsPfaffian", "SkhError",
-    "ENGINE_AUTO", "ENGINE_LARGE", "ENGINE_TEAM1", "version",
+    "ENGINE_AUTO", "ENGINE_LARGE", "ENGINE_TEAM1", "version", "LogPfPlan",
 ]
 
 
@@ -496,6 +496,59 @@ def _device_batch_call(fn_name, A, want_grad):
                                       ctypes.byref(o))
     _lib.check("skh_logpf_batched_d_device", rc)
     return lp, sg, None
+
+
+class LogPfPlan:
+    """A batched log|Pf| plan over device-resident matrices (C ABI skh_logpf_batched_plan_*).
+
+    No reference counterpart (the reference user loops logabspfaffian, src/pfaffian.jl:236).
+    `ns`, `offsets`: host int64 arrays (column-major matrices at even element offsets of one
+    float64 device buffer).  `run(dA, dlogpf, dsgn, stream)` enqueues one evaluation without
+    host synchronisation, allocation or host copies (capturable in a HIP graph, e.g.
+    torch.cuda.graph); `tune(...)` times the kernel configurations and keeps the fastest;
+    `status(stream)` waits and raises if the watchdog stopped a launch."""
+
+    def __init__(self, ns, offsets):
+        self._ns = np.ascontiguousarray(ns, dtype=np.int64)
+        self._offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        if self._ns.shape != self._offs.shape or self._ns.ndim != 1:
+            raise ArgumentError("ns and offsets must be 1-D arrays of the same length")
+        h = ctypes.c_void_p()
+        o = _lib.opts()
+        _lib.check("skh_logpf_batched_plan_create",
+                   _lib.lib().skh_logpf_batched_plan_create(len(self._ns), _ptr(self._ns), _ptr(self._offs),
+                                                            ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+
+    def run(self, dA, dlogpf, dsgn, stream=None):
+        _lib.check("skh_logpf_batched_plan_exec",
+                   _lib.lib().skh_logpf_batched_plan_exec(self._h, dA, dlogpf, dsgn, stream))
+
+    def tune(self, dA, dlogpf, dsgn, stream=None):
+        best = ctypes.c_double(0.0)
+        _lib.check("skh_logpf_batched_plan_tune",
+                   _lib.lib().skh_logpf_batched_plan_tune(self._h, dA, dlogpf, dsgn, stream, ctypes.byref(best)))
+        return best.value
+
+    def status(self, stream=None):
+        _lib.check("skh_plan_status", _lib.lib().skh_plan_status(self._h, stream))
+
+    def info(self):
+        out = np.zeros(4, dtype=np.int64)
+        _lib.check("skh_plan_info", _lib.lib().skh_plan_info(self._h, _ptr(out)))
+        return {"workgroups": int(out[0]), "kernel_nmax": int(out[1]), "tuned": bool(out[2]),
+                "matrices": int(out[3])}
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.lib().skh_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _make_autograd():

@@ -56,6 +56,12 @@ def lib():
         "skh_eigen_d": ([i64, vp, i64, vp, vp, vp, op], ctypes.c_int),
         "skh_logpf_grad_batched_d": ([i64, vp, vp, vp, vp, vp, vp, op], ctypes.c_int),
         "skh_logpf_grad_batched_d_device": ([i64, vp, vp, vp, vp, vp, vp, vp, op], ctypes.c_int),
+        "skh_logpf_batched_plan_create": ([i64, vp, vp, op, ctypes.POINTER(vp)], ctypes.c_int),
+        "skh_logpf_batched_plan_exec": ([vp, vp, vp, vp, vp], ctypes.c_int),
+        "skh_logpf_batched_plan_tune": ([vp, vp, vp, vp, vp, _dp], ctypes.c_int),
+        "skh_plan_status": ([vp, vp], ctypes.c_int),
+        "skh_plan_info": ([vp, vp], ctypes.c_int),
+        "skh_plan_destroy": ([vp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

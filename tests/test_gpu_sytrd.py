@@ -427,3 +427,66 @@ def test_sytrd_n4096_vs_oracle(nb):
     np.testing.assert_allclose(eta, etao, rtol=0, atol=1e-11 * nA)
     np.testing.assert_allclose(tau, tauo, rtol=0, atol=1e-10)
     assert abs(np.sqrt(2 * np.sum(eta ** 2)) - nA) <= 1e-13 * nA
+
+
+def test_plan_tune_and_graph_replay():
+    """A batched plan gives the same bits as the one-shot device call, after tuning and when its
+    launch is captured in a HIP graph and replayed (eager work and synchronisation in between)."""
+    say("plan + graph replay")
+    import torch
+    rng = np.random.default_rng(21)
+    sizes = [0, 3, 2, 64, 96, 130, 256, 510, 512, 31] + list(rng.integers(1, 40, 30) * 2)
+    mats = []
+    for n in sizes:
+        B = rng.standard_normal((n, n)) / np.sqrt(max(n, 1))
+        mats.append(np.asfortranarray(B - B.T))
+    ns = np.array(sizes, dtype=np.int64)
+    offs = np.zeros(len(sizes), dtype=np.int64)
+    tot = 0
+    for k, n in enumerate(sizes):
+        offs[k] = tot
+        tot += (n * n + 1) & ~1
+    host = np.zeros(max(tot, 1))
+    for k, M in enumerate(mats):
+        host[offs[k]:offs[k] + M.size] = M.ravel(order="F")
+    dA = torch.tensor(host, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    lp0 = torch.empty(len(sizes), dtype=torch.float64, device="cuda")
+    sg0 = torch.empty_like(lp0)
+    o = _lib.opts()
+    _lib.check("device", _lib.lib().skh_logpf_batched_d_device(len(sizes), ns.ctypes.data, offs.ctypes.data,
+                                                               dA.data_ptr(), lp0.data_ptr(), sg0.data_ptr(), st,
+                                                               ctypes.byref(o)))
+    torch.cuda.synchronize()
+    for k, M in enumerate(mats):
+        n = M.shape[0]
+        lpo, sgo = (0.0, 1.0) if n == 0 else oracle.logabspfaffian(M)
+        assert sg0[k].item() == sgo
+        if n % 2 == 0 and n > 0:
+            assert abs(lp0[k].item() - lpo) <= 1e-10 * max(1.0, abs(lpo))
+    plan = skl.LogPfPlan(ns, offs)
+    lp = torch.empty_like(lp0)
+    sg = torch.empty_like(lp0)
+    plan.tune(dA.data_ptr(), lp.data_ptr(), sg.data_ptr(), st)
+    plan.status(st)
+    assert plan.info()["tuned"]
+    assert torch.equal(lp, lp0) and torch.equal(sg, sg0)
+    lp.zero_()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        plan.run(dA.data_ptr(), lp.data_ptr(), sg.data_ptr(), side.cuda_stream)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        plan.run(dA.data_ptr(), lp.data_ptr(), sg.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    for r in range(3):
+        lp.fill_(-1.0)
+        sg.fill_(7.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        plan.status(st)
+        assert torch.equal(lp, lp0) and torch.equal(sg, sg0), r
+    plan.close()
