@@ -39,6 +39,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(n, nb):
+    """HBM bytes per k_panel launch for this configuration, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel factors), or None."""
+    if n != 8192 or nb not in (0, 64):
+        return None
+    path = os.path.join(HERE, "profiles", "k_panel_n8192_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        return None
+    return None
+
+
 def cpu_baseline(n_cpu):
     import numpy as np
     import oracle
@@ -326,7 +342,8 @@ def main():
         "config": {"workload": "single real Float64 skew-symmetric matrix, blocked tridiagonalization",
                    "n": n, "nb": args.nb or "auto", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_panel (skew matvec panel)", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(n, args.nb),
                      "bytes_per_launch": panel_bytes_per, "launch_ms": panel_avg_s * 1e3,
                      "launches_per_step": panel_cnt / args.steps},
         "mfma_update": {"kernel": "k_rank2k", "achieved": upd_tf, "peak": FP64_MFMA_PEAK_TF, "unit": "TFLOP/s",
