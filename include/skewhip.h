@@ -125,6 +125,37 @@ int skh_orgtr_d_device(int64_t n, const double *dF, int64_t ldf, const double *d
                        void *stream);
 
 /*
+ * Complex skew-Hermitian tridiagonalization, ComplexF64 (interleaved re/im pairs, i.e. Julia's
+ * Matrix{ComplexF64}; lda counts complex elements).
+ * Replaces the complex branch of skewblockedhess!(S::SkewHermitian{ComplexF64}) -> (tau, eta)
+ *   (reference src/hessenberg.jl:92-131 with :96-104, :123-126, :161, tail :70-90) and
+ *   hessenberg!(A::SkewHermitian{<:Complex}) (:5-13).
+ * On return the strictly lower triangle holds the factors (the unit of every reflector stored,
+ * including the last length-1 reflector), the diagonal holds T's (imaginary) diagonal, the upper
+ * triangle is left as given.  tau: n-1 complex; eta: n-1 real (the subdiagonal of T); dvim: n
+ * (imaginary parts of T's diagonal: SkewHermTridiagonal(eta, dvim)).
+ */
+int skh_sytrd_z(int64_t n, double *A, int64_t lda, double *tau, double *eta, double *dvim, const skh_opts *opts);
+/* device variant: all pointers in HBM (A 16-byte aligned); asynchronous on `stream`. */
+int skh_sytrd_z_device(int64_t n, double *dA, int64_t lda, double *dtau, double *deta, double *ddvim, void *stream,
+                       const skh_opts *opts);
+/* Explicit unitary Q = H_0 ... H_{n-2} from complex factors and tau (device memory, LAPACK
+ * zungtr('L') convention), replacing Matrix(F.Q) for the complex Hessenberg (src/hessenberg.jl:3). */
+int skh_orgtr_z_device(int64_t n, const double *dF, int64_t ldf, const double *dtau, double *dQ, int64_t ldq,
+                       void *stream);
+/*
+ * Eigenvalues / eigen-decomposition of a complex skew-Hermitian A (host memory, not modified).
+ * Replace eigvals!/eigen!(A::SkewHermitian{<:Complex}) (reference src/eigen.jl:18-27, :101-106,
+ * which call LAPACK on Hermitian(iA)): vals[k] = -lambda_k for the ascending eigenvalues lambda of
+ * iA, so the eigenvalues are complex(0, vals[k]); V (eigen only): n x n complex eigenvectors
+ * (interleaved, leading dimension n).  Here: GPU tridiagonalization A = Q T Q^H, the real
+ * symmetric tridiagonal D^H (iT) D (D = diag(i^r)) solved on the host (eigvals) or by GPU divide
+ * and conquer (eigen), vectors Q D Z on the GPU.
+ */
+int skh_eigvals_z(int64_t n, const double *A, int64_t lda, double *vals, const skh_opts *opts);
+int skh_eigen_z(int64_t n, const double *A, int64_t lda, double *vals, double *V, const skh_opts *opts);
+
+/*
  * Eigen-decomposition of a real skew-symmetric A (host memory, A not modified).
  * Replaces skeweigen!/eigen!(A::SkewHermitian{<:Real}) (reference src/eigen.jl:75-97): GPU
  * tridiagonalization and Q, host skewtrieigen_divided!, GPU back-transform Q*Qr, Q*Qim.

@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "skewlinearalgebra", "libskewhip.so")
-SOURCES = ["skh_kernels.hip", "skh_compact.hip", "skh_api.cpp", "host_tridiag.cpp"]
+SOURCES = ["skh_kernels.hip", "skh_compact.hip", "skh_complex.hip", "skh_api.cpp", "host_tridiag.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -24,7 +24,7 @@ def build(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = OUT + ".tmp"
     cmd = [HIPCC] + FLAGS + ["-I", os.path.join(HERE, "..", "include"), "-o", tmp] + srcs + [
-        "-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+        "-L/opt/rocm/lib", "-lrocblas", "-lrocsolver", "-Wl,-rpath,/opt/rocm/lib"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -391,3 +391,12 @@ extern "C" int skh_skewtrieigen_d(int64_t N, double* evp, double* vals, double* 
         for (int64_t j = 1; j <= N; j += 2) VR(j, N) = QO((j + 1) / 2, halfN + 1);
     return 0;
 }
+
+extern "C" int skh_internal_symtri_eigvals(int64_t n, double* d, const double* e)
+{
+    if (n <= 0) return 0;
+    std::vector<double> dv(d, d + n), ev(e, e + (n > 1 ? n - 1 : 0));
+    if (!symtri_ql(n, dv, ev, nullptr)) return 2;
+    std::copy(dv.begin(), dv.end(), d);
+    return 0;
+}

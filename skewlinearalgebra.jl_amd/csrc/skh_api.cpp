@@ -16,6 +16,7 @@
 #include <vector>
 
 #include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include "../../include/skewhip.h"
 #include "skh_internal.h"
@@ -424,6 +425,107 @@ int orgtr_locked(int64_t n, const double* dF, int64_t ldf, const double* dtau, d
     return 0;
 }
 
+
+// ----------------------------------------------------------------------------------------
+// Complex (ComplexF64) path: cooperative unblocked engine, explicit Q, eigen via the real
+// symmetric tridiagonal D^H (iT) D (D = diag(i^r)) and a back-transform by Q.
+// ----------------------------------------------------------------------------------------
+Buf g_zws, g_zq, g_zc, g_zx;
+
+int zsytrd_device(int64_t n, double2* dA, int64_t lda, double2* dtau, double* deta, double* ddvim, hipStream_t st)
+{
+    if (n <= 0) return 0;
+    if (n >= 2) {
+        Ctl ctl;
+        int rc = get_ctl(ctl);
+        if (rc) return rc;
+        const int64_t ldp = (n + 1) & ~int64_t(1);
+        const int64_t nt = (n + skh::TB - 1) / skh::TB + 1;
+        int per_cu = 0;
+        HIPCHK((hipError_t)skh_internal_zfused_occupancy(&per_cu));
+        per_cu = std::max(1, std::min(per_cu, 2));
+        const int64_t ns0 = (n + skh::TB - 1) / skh::TB;
+        const int64_t tiles0 = ns0 * (ns0 + 1) / 2;
+        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * g_num_cu, tiles0));
+        const int64_t gpad = (grid + 1) & ~1;
+        const size_t doubles = (size_t)(4 * nt * ldp + 2 * ldp + 2 * gpad + 2 + 64);
+        double* ws = (double*)g_zws.get(doubles * sizeof(double));
+        if (!ws) return set_err(SKH_ERR_HIP, "hipMalloc of %zu bytes of workspace failed", doubles * 8);
+        skh::zc::ZWork w;
+        w.Prow = (double2*)ws;
+        w.Pcol = w.Prow + nt * ldp;
+        w.ldp = ldp;
+        w.Wv = w.Pcol + nt * ldp;
+        w.Pnrm = (double*)(w.Wv + ldp);
+        w.Pmu = w.Pnrm + gpad;
+        w.alpha = (double2*)(w.Pmu + gpad);
+        HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 512, st));
+        HIPCHK((hipError_t)skh_internal_launch_zfused(dA, lda, n, &w, dtau, deta, ctl.bar, ctl.err, grid, st));
+    }
+    HIPCHK((hipError_t)skh_internal_zdiag(dA, lda, n, ddvim, st));
+    return 0;
+}
+
+// Q (n x n, ldq) = H_0 ... H_{n-2} from complex factors and tau (device), blocked compact WY.
+int zorgtr_locked(int64_t n, const double2* dF, int64_t ldf, const double2* dtau, double2* dQ, int64_t ldq,
+                  hipStream_t st)
+{
+    if (n <= 0) return 0;
+    if (n > 2147483647LL / 2) return set_err(SKH_ERR_UNSUPPORTED, "n too large for the BLAS interface");
+    HIPCHK((hipError_t)skh_internal_zeye(dQ, ldq, n, st));
+    if (n <= 1) return 0;
+    int rc = blas_on(st);
+    if (rc) return rc;
+    const int64_t nb = 64;
+    const int64_t nref = n - 1;
+    const size_t cplx = (size_t)(n * nb + 2 * nb * n + 2 * nb * nb + 64);
+    double2* ws = (double2*)g_ws.get(cplx * sizeof(double2));
+    if (!ws) return set_err(SKH_ERR_HIP, "hipMalloc of orgtr workspace failed");
+    double2* V = ws;
+    double2* Y = V + n * nb;
+    double2* Z = Y + nb * n;
+    double2* Gm = Z + nb * n;
+    double2* T = Gm + nb * nb;
+    const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0}, mone = {-1.0, 0.0};
+    auto rz = [](double2* p) { return (rocblas_double_complex*)p; };
+    auto rzc = [](const double2* p) { return (const rocblas_double_complex*)p; };
+    const int64_t last = ((nref - 1) / nb) * nb;
+    for (int64_t c0 = last; c0 >= 0; c0 -= nb) {
+        const int nbp = (int)std::min<int64_t>(nb, nref - c0);
+        const int64_t m = n - c0 - 1;
+        HIPCHK((hipError_t)skh_internal_zform_v(dF, ldf, n, c0, nbp, V, m, st));
+        BLASCHK(rocblas_zgemm(g_blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, nbp, nbp, (int)m,
+                              &one, rz(V), (int)m, rz(V), (int)m, &zero, rz(Gm), nbp));
+        HIPCHK((hipError_t)skh_internal_zlarft(Gm, nbp, dtau + c0, nbp, T, nbp, st));
+        double2* Q22 = dQ + (c0 + 1) + (c0 + 1) * ldq;
+        BLASCHK(rocblas_zgemm(g_blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, nbp, (int)m,
+                              (int)m, &one, rzc(V), (int)m, rzc(Q22), (int)ldq, &zero, rz(Y), nbp));
+        BLASCHK(rocblas_zgemm(g_blas, rocblas_operation_none, rocblas_operation_none, nbp, (int)m, nbp, &one,
+                              rzc(T), nbp, rzc(Y), nbp, &zero, rz(Z), nbp));
+        BLASCHK(rocblas_zgemm(g_blas, rocblas_operation_none, rocblas_operation_none, (int)m, (int)m, nbp, &mone,
+                              rzc(V), (int)m, rzc(Z), nbp, &one, rz(Q22), (int)ldq));
+    }
+    return 0;
+}
+
+// host copy-in of an n x n complex matrix (interleaved pairs, lda) into a device buffer with ld n
+double2* zupload(int64_t n, const double* A, int64_t lda, hipStream_t st, int* rc)
+{
+    double2* dA = (double2*)g_mat.get((size_t)n * n * sizeof(double2));
+    if (!dA) {
+        *rc = set_err(SKH_ERR_HIP, "hipMalloc failed for n = %lld", (long long)n);
+        return nullptr;
+    }
+    hipError_t e = hipMemcpy2DAsync(dA, n * sizeof(double2), A, lda * sizeof(double2), n * sizeof(double2), n,
+                                    hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) {
+        *rc = set_err(SKH_ERR_HIP, "upload failed: %s", hipGetErrorString(e));
+        return nullptr;
+    }
+    *rc = 0;
+    return dA;
+}
+
 // ----------------------------------------------------------------------------------------
 // Gradient of log|Pf|: grad_b = -1/2 A_b^{-1} = -1/2 (K - K^T), K = Q_E C^{-1} Q_O^T, grouped by size.
 // dgrad first receives the factors from the forward kernel (same offsets as the input).
@@ -628,7 +730,7 @@ int skh_finalize(void)
     if (!g_inited) return 0;
     (void)hipStreamSynchronize(g_stream);
     for (Buf* b : {&g_mat, &g_ws, &g_vec, &g_ctl, &g_bin, &g_bout, &g_q, &g_ev, &g_gq, &g_gk, &g_gz, &g_gv, &g_gp,
-                   &g_gte, &g_cslot, &g_big, &g_bctl})
+                   &g_gte, &g_cslot, &g_big, &g_bctl, &g_zws, &g_zq, &g_zc, &g_zx})
         b->release();
     if (g_blas) (void)rocblas_destroy_handle(g_blas);
     g_blas = nullptr;
@@ -1110,6 +1212,155 @@ int skh_plan_destroy(skh_plan plan)
     std::lock_guard<std::mutex> lk(g_mu);
     if (plan) (void)hipDeviceSynchronize();
     plan_free(plan);
+    return 0;
+}
+
+int skh_sytrd_z_device(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, double* ddvim, void* stream,
+                       const skh_opts* opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (n < 0) return set_err(-1, "n < 0");
+    if (n > 0 && !dA) return set_err(-2, "A is NULL");
+    if (lda < std::max<int64_t>(n, 1)) return set_err(-3, "lda < n");
+    if (n > 1 && (!dtau || !deta)) return set_err(-4, "tau/eta NULL");
+    if (n > 0 && !ddvim) return set_err(-6, "dvim NULL");
+    if (((uintptr_t)dA) & 15) return set_err(-2, "A must be 16-byte aligned");
+    (void)opts;
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    return zsytrd_device(n, (double2*)dA, lda, (double2*)dtau, deta, ddvim, (hipStream_t)stream);
+}
+
+int skh_sytrd_z(int64_t n, double* A, int64_t lda, double* tau, double* eta, double* dvim, const skh_opts* opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (n < 0) return set_err(-1, "n < 0");
+    if (n > 0 && !A) return set_err(-2, "A is NULL");
+    if (lda < std::max<int64_t>(n, 1)) return set_err(-3, "lda < n");
+    if (n > 1 && (!tau || !eta)) return set_err(-4, "tau/eta NULL");
+    if (n > 0 && !dvim) return set_err(-6, "dvim NULL");
+    (void)opts;
+    if (n == 0) return 0;
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    hipStream_t st = g_stream;
+    double2* dA = zupload(n, A, lda, st, &rc);
+    if (!dA) return rc;
+    double* dv = (double*)g_vec.get((size_t)(4 * n + 4) * sizeof(double));
+    if (!dv) return set_err(SKH_ERR_HIP, "hipMalloc failed");
+    double2* dtau = (double2*)dv;
+    double* deta = dv + 2 * n;
+    double* ddvim = deta + n;
+    rc = zsytrd_device(n, dA, n, dtau, deta, ddvim, st);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(st));
+    rc = check_barrier_flag();
+    if (rc) return rc;
+    HIPCHK(hipMemcpy2DAsync(A, lda * sizeof(double2), dA, n * sizeof(double2), n * sizeof(double2), n,
+                            hipMemcpyDeviceToHost, st));
+    if (n > 1) {
+        HIPCHK(hipMemcpyAsync(tau, dtau, (n - 1) * sizeof(double2), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(eta, deta, (n - 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    HIPCHK(hipMemcpyAsync(dvim, ddvim, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int skh_orgtr_z_device(int64_t n, const double* dF, int64_t ldf, const double* dtau, double* dQ, int64_t ldq,
+                       void* stream)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (n < 0) return set_err(-1, "n < 0");
+    if (n > 0 && (!dF || !dQ)) return set_err(-2, "NULL matrix");
+    if (ldf < std::max<int64_t>(n, 1)) return set_err(-3, "ldf < n");
+    if (n > 1 && !dtau) return set_err(-4, "tau NULL");
+    if (ldq < std::max<int64_t>(n, 1)) return set_err(-6, "ldq < n");
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    return zorgtr_locked(n, (const double2*)dF, ldf, (const double2*)dtau, (double2*)dQ, ldq, (hipStream_t)stream);
+}
+
+// eigenvalues of the complex skew-Hermitian A: GPU tridiagonalization, then the eigenvalues of
+// the real symmetric tridiagonal D^H (iT) D (diagonal -dvim, off-diagonal eta) on the host
+int skh_eigvals_z(int64_t n, const double* A, int64_t lda, double* vals, const skh_opts* opts)
+{
+    if (n < 0) return set_err(-1, "n < 0");
+    if (n > 0 && (!A || !vals)) return set_err(-2, "NULL argument");
+    if (lda < std::max<int64_t>(n, 1)) return set_err(-3, "lda < n");
+    if (n == 0) return 0;
+    std::vector<double> F((size_t)2 * n * n), tau(2 * n), eta(n), dvim(n);
+    for (int64_t c = 0; c < n; c++) std::memcpy(&F[2 * c * n], A + 2 * c * lda, 2 * n * sizeof(double));
+    int rc = skh_sytrd_z(n, F.data(), n, tau.data(), eta.data(), dvim.data(), opts);
+    if (rc) return rc;
+    std::vector<double> d(n);
+    for (int64_t k = 0; k < n; k++) d[k] = -dvim[k];
+    if (skh_internal_symtri_eigvals(n, d.data(), eta.data()) != 0)
+        return set_err(SKH_ERR_NOCONV, "tridiagonal QL did not converge");
+    // eigenvalues of A are -i lambda for the ascending eigenvalues lambda of iA (src/eigen.jl:19-22)
+    for (int64_t k = 0; k < n; k++) vals[k] = -d[k];
+    return 0;
+}
+
+// eigen-decomposition of the complex skew-Hermitian A (src/eigen.jl:101-106): values -lambda
+// (imaginary parts) for ascending lambda of iA, vectors X = Q D Z (interleaved complex, ld n)
+int skh_eigen_z(int64_t n, const double* A, int64_t lda, double* vals, double* V, const skh_opts* opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (n < 0) return set_err(-1, "n < 0");
+    if (n > 0 && (!A || !vals || !V)) return set_err(-2, "NULL argument");
+    if (lda < std::max<int64_t>(n, 1)) return set_err(-3, "lda < n");
+    (void)opts;
+    if (n == 0) return 0;
+    if (n > 46340) return set_err(SKH_ERR_UNSUPPORTED, "n too large for the solver interface");
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    hipStream_t st = g_stream;
+    double2* dA = zupload(n, A, lda, st, &rc);
+    if (!dA) return rc;
+    double* dv = (double*)g_vec.get((size_t)(4 * n + 4) * sizeof(double));
+    double2* dQ = (double2*)g_zq.get((size_t)n * n * sizeof(double2));
+    double* dC = (double*)g_zc.get((size_t)(n * n + 2 * n + 2) * sizeof(double));
+    double2* dX = (double2*)g_zx.get((size_t)n * n * sizeof(double2));
+    if (!dv || !dQ || !dC || !dX) return set_err(SKH_ERR_HIP, "hipMalloc failed (eigen buffers)");
+    double2* dtau = (double2*)dv;
+    double* deta = dv + 2 * n;
+    double* ddvim = deta + n;
+    rc = zsytrd_device(n, dA, n, dtau, deta, ddvim, st);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(st));
+    rc = check_barrier_flag();
+    if (rc) return rc;
+    rc = zorgtr_locked(n, dA, n, dtau, dQ, n, st);
+    if (rc) return rc;
+    // tridiagonal eigenpairs on the GPU (divide and conquer)
+    std::vector<double> eta(n), dvim(n), d(n);
+    if (n > 1) HIPCHK(hipMemcpyAsync(eta.data(), deta, (n - 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(dvim.data(), ddvim, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int64_t k = 0; k < n; k++) d[k] = -dvim[k];
+    double* dD = dC + n * n;
+    double* dE = dD + n;
+    int* dinfo = (int*)(dE + n);
+    HIPCHK(hipMemcpyAsync(dD, d.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+    if (n > 1) HIPCHK(hipMemcpyAsync(dE, eta.data(), (n - 1) * sizeof(double), hipMemcpyHostToDevice, st));
+    rc = blas_on(st);
+    if (rc) return rc;
+    BLASCHK(rocsolver_dstedc(g_blas, rocblas_evect_tridiagonal, (rocblas_int)n, dD, dE, dC, (rocblas_int)n, dinfo));
+    int info = 0;
+    HIPCHK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(d.data(), dD, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (info != 0) return set_err(SKH_ERR_NOCONV, "tridiagonal divide and conquer failed (info %d)", info);
+    HIPCHK((hipError_t)skh_internal_zphase(dC, n, dX, st));   // dX = D Z (reuses dX as Zc)
+    // vectors = Q (D Z): write into dA (no longer needed) then copy out
+    const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+    BLASCHK(rocblas_zgemm(g_blas, rocblas_operation_none, rocblas_operation_none, (int)n, (int)n, (int)n, &one,
+                          (const rocblas_double_complex*)dQ, (int)n, (const rocblas_double_complex*)dX, (int)n, &zero,
+                          (rocblas_double_complex*)dA, (int)n));
+    HIPCHK(hipMemcpyAsync(V, dA, (size_t)n * n * sizeof(double2), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int64_t k = 0; k < n; k++) vals[k] = -d[k];
     return 0;
 }
 

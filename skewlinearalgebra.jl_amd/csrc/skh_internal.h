@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "skh_device.hpp"
+#include "skh_zengine.hpp"
 
 struct PanelLaunch {
     double* A;
@@ -43,6 +44,18 @@ struct CompactArgs {
 };
 
 extern "C" {
+int skh_internal_zfused_occupancy(int* blocks_per_cu);
+int skh_internal_launch_zfused(double2* A, int64_t lda, int64_t n, const skh::zc::ZWork* w, double2* tau, double* eta,
+                               unsigned* bar, int* err, int grid, hipStream_t st);
+int skh_internal_zdiag(const double2* A, int64_t lda, int64_t n, double* dvim, hipStream_t st);
+int skh_internal_zform_v(const double2* F, int64_t ldf, int64_t n, int64_t c0, int nbp, double2* V, int64_t ldv,
+                         hipStream_t st);
+int skh_internal_zlarft(const double2* Gm, int64_t ldg, const double2* tau, int nbp, double2* T, int64_t ldt,
+                        hipStream_t st);
+int skh_internal_zeye(double2* Q, int64_t ldq, int64_t n, hipStream_t st);
+int skh_internal_zphase(const double* C, int64_t n, double2* Zc, hipStream_t st);
+// host (host_tridiag.cpp): eigenvalues of the symmetric tridiagonal (d, e), ascending into d; 0 or 2
+int skh_internal_symtri_eigvals(int64_t n, double* d, const double* e);
 int skh_internal_compact_occupancy(int nmax, int* blocks_per_cu);
 int skh_internal_launch_compact(const CompactArgs* a, int nmax, int grid, hipStream_t st);
 int skh_internal_logpf_from(const double* eta, const double* tau, int64_t n, double* logpf, double* sgn, hipStream_t st);

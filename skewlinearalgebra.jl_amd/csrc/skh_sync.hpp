@@ -1,0 +1,51 @@
+// skh_sync.hpp -- grid barrier shared by the cooperative kernels (panel and complex engines).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace skh {
+
+// ---------------------------------------------------------------------------------------
+// Grid barrier of the cooperative panel kernel, two levels: workgroups arrive on one of 8 group
+// counters (blockIdx % 8, i.e. the XCD they are dispatched to), the last arrival of a group
+// arrives on the top counter, and every workgroup polls the top counter.  Counters only grow
+// within one factorization; `epoch` is the 1-based index of this barrier in the factorization.
+// Memory ordering: every wave drains its own stores, then one release fence per workgroup before
+// arriving and one acquire fence after the wait (relaxed polling, so the wait itself does not
+// touch the caches).  A bounded spin turns a lost workgroup into an error flag, not a hang.
+// Layout of `bar` (unsigned words): [0] top counter, [32*(1+k)] group k counter.
+// ---------------------------------------------------------------------------------------
+constexpr unsigned SPIN_LIMIT = 1u << 26;
+
+__device__ inline bool grid_sync(unsigned* bar, unsigned epoch, int* err, int* s_ok)
+{
+    // this wave's global stores have reached the L2 before the workgroup releases them; the
+    // memory clobber keeps the compiler from moving any store below the wait
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned G = gridDim.x;
+        const unsigned grp = blockIdx.x & 7u;
+        const unsigned ngroups = G < 8u ? G : 8u;
+        const unsigned gsize = (G - grp + 7u) / 8u;
+        __threadfence();   // release this workgroup's writes device-wide
+        const unsigned old = atomicAdd(&bar[32u * (1u + grp)], 1u);
+        if (old + 1u == epoch * gsize) atomicAdd(&bar[0], 1u);
+        unsigned spins = 0;
+        int ok = 1;
+        while (__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch * ngroups) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++spins > SPIN_LIMIT) {
+                atomicOr(err, 1);
+                ok = 0;
+                break;
+            }
+        }
+        if (ok && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) ok = 0;
+        __threadfence();   // acquire: later loads of this workgroup see every arrival's writes
+        *s_ok = ok;
+    }
+    __syncthreads();
+    return *s_ok != 0;
+}
+
+}  // namespace skh

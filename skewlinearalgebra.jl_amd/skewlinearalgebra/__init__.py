@@ -202,7 +202,17 @@ def hessenberg_(A, nb=0, engine=ENGINE_AUTO):
     S = A if isinstance(A, SkewHermitian) else SkewHermitian(A)
     D = S.data
     if np.iscomplexobj(D):
-        raise NotImplementedError("complex SkewHermitian input is not supported by this build")
+        # complex branch (src/hessenberg.jl:7-8): T = SkewHermTridiagonal(complex(eta), imag(diag(A)))
+        if D.dtype != np.complex128 or not D.flags.f_contiguous:
+            raise ArgumentError("hessenberg_ needs a ComplexF64 column-major (Fortran-ordered) matrix; use hessenberg")
+        n = D.shape[0]
+        tau = np.zeros(max(n - 1, 0), dtype=np.complex128)
+        eta = np.zeros(max(n - 1, 0))
+        dvim = np.zeros(n)
+        o = _lib.opts(nb, engine)
+        _lib.check("skh_sytrd_z", _lib.lib().skh_sytrd_z(n, _ptr(D), max(n, 1), _ptr(tau), _ptr(eta), _ptr(dvim),
+                                                          ctypes.byref(o)))
+        return Hessenberg(SkewHermTridiagonal(eta.astype(np.complex128), dvim), D, tau)
     if D.dtype != np.float64 or not D.flags.f_contiguous:
         raise ArgumentError("hessenberg_ needs a Float64 column-major (Fortran-ordered) matrix; use hessenberg")
     tau, eta, _ = _sytrd_inplace(D, nb, engine)
@@ -214,10 +224,15 @@ def hessenberg(A, nb=0, engine=ENGINE_AUTO):
     D = A.data if isinstance(A, SkewHermitian) else np.asarray(A)
     if not isinstance(A, SkewHermitian) and not isskewhermitian(D):
         raise ArgumentError("matrix `A` must be skew-Hermitian (equal `-A'`)")
-    F = np.array(D, dtype=np.float64, order="F", copy=True)
+    F = np.array(D, dtype=_eigtype(D), order="F", copy=True)
     S = SkewHermitian.__new__(SkewHermitian)
     S.data = F
     return hessenberg_(S, nb, engine)
+
+
+def _eigtype(D):
+    """LinearAlgebra.eigtype as this build computes it: Float64, or ComplexF64 for complex input."""
+    return np.complex128 if np.iscomplexobj(D) else np.float64
 
 
 # ----------------------------------------------------------------------------------------
@@ -243,6 +258,19 @@ def eigvals_(A, sortby=None, nb=0, engine=ENGINE_AUTO):
     else:
         S = A if isinstance(A, SkewHermitian) else SkewHermitian(A)
         n = S.data.shape[0]
+        if np.iscomplexobj(S.data):
+            # src/eigen.jl:18-27: -eigvals(Hermitian(iA)), with sortby: sort, reverse, negate
+            D = np.asfortranarray(S.data, dtype=np.complex128)
+            vals = np.zeros(n)
+            o = _lib.opts(nb, engine)
+            rc = _lib.lib().skh_eigvals_z(n, _ptr(D), max(n, 1), _ptr(vals), ctypes.byref(o))
+            if rc == 2:
+                raise RuntimeError("Maximum number of iterations reached, the algorithm didn't converge")
+            _lib.check("skh_eigvals_z", rc)
+            if sortby is not None:
+                lam = np.array(sorted(-vals, key=sortby))
+                vals = -lam[::-1]
+            return vals * 1j
         if n == 1:
             return np.array([S.data[0, 0]], dtype=np.complex128)   # src/eigen.jl:51
         H = hessenberg_(S, nb, engine)
@@ -257,7 +285,7 @@ def eigvals(A, sortby=None, nb=0, engine=ENGINE_AUTO):
     if isinstance(A, SkewHermTridiagonal):
         return eigvals_(A.copy(), sortby)
     D = A.data if isinstance(A, SkewHermitian) else np.asarray(A)
-    S = SkewHermitian(np.array(D, dtype=np.float64, order="F", copy=True))
+    S = SkewHermitian(np.array(D, dtype=_eigtype(D), order="F", copy=True))
     return eigvals_(S, sortby, nb, engine)
 
 
@@ -295,9 +323,18 @@ def eigen_(A, nb=0, engine=ENGINE_AUTO):
         vals, vr, vi = _tri_eigen_host(A.ev, 1)
         return Eigen(1j * vals, vr + 1j * vi)
     D = A.data if isinstance(A, SkewHermitian) else np.asarray(A)
-    if np.iscomplexobj(D):
-        raise NotImplementedError("complex SkewHermitian input is not supported by this build")
     n = D.shape[0]
+    if np.iscomplexobj(D):
+        # src/eigen.jl:101-106: Eigen(complex(0, -lambda), eigenvectors of Hermitian(iA))
+        F = np.asfortranarray(D, dtype=np.complex128)
+        vals = np.zeros(n)
+        V = np.zeros((n, n), dtype=np.complex128, order="F")
+        o = _lib.opts(nb, engine)
+        rc = _lib.lib().skh_eigen_z(n, _ptr(F), max(n, 1), _ptr(vals), _ptr(V), ctypes.byref(o))
+        if rc == 2:
+            raise RuntimeError("Maximum number of iterations reached, the algorithm didn't converge")
+        _lib.check("skh_eigen_z", rc)
+        return Eigen(1j * vals, V)
     F = np.asfortranarray(D, dtype=np.float64)
     vals = np.zeros(n)
     vr = np.zeros((n, n), order="F")
@@ -317,7 +354,7 @@ def eigen(A, nb=0, engine=ENGINE_AUTO):
     D = A.data if isinstance(A, SkewHermitian) else np.asarray(A)
     if not isinstance(A, SkewHermitian) and not isskewhermitian(D):
         raise ArgumentError("matrix `A` must be skew-Hermitian (equal `-A'`)")
-    return eigen_(SkewHermitian(np.array(D, dtype=np.float64, order="F", copy=True)), nb, engine)
+    return eigen_(SkewHermitian(np.array(D, dtype=_eigtype(D), order="F", copy=True)), nb, engine)
 
 
 # ----------------------------------------------------------------------------------------

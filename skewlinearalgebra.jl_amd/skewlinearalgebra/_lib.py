@@ -62,6 +62,11 @@ def lib():
         "skh_plan_status": ([vp, vp], ctypes.c_int),
         "skh_plan_info": ([vp, vp], ctypes.c_int),
         "skh_plan_destroy": ([vp], ctypes.c_int),
+        "skh_sytrd_z": ([i64, vp, i64, vp, vp, vp, op], ctypes.c_int),
+        "skh_sytrd_z_device": ([i64, vp, i64, vp, vp, vp, vp, op], ctypes.c_int),
+        "skh_orgtr_z_device": ([i64, vp, i64, vp, vp, i64, vp], ctypes.c_int),
+        "skh_eigvals_z": ([i64, vp, i64, vp, op], ctypes.c_int),
+        "skh_eigen_z": ([i64, vp, i64, vp, vp, op], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
