@@ -171,6 +171,19 @@ int skh_eigen_d(int64_t n, const double *A, int64_t lda, double *vals, double *v
  */
 int skh_skewtrieigvals_d(int64_t n, double *ev, double *vals);
 /*
+ * Eigenvalues by index or value range.  by_value = 0: the eigenvalues of
+ * H = SymTridiagonal(0, eta) with 1-based indices il..iu of its ascending spectrum; by_value = 1:
+ * those in (-vh, -vl]; each negated, so the eigenvalues are complex(0, vals[k]), k < *m.
+ * skh_skewtrieigvals_range_d replaces skewtrieigvals!(S::SkewHermTridiagonal{<:Real}, irange) /
+ * (S, vl, vh) (reference src/tridiag.jl:525-537, via eigvals! :452-460); skh_eigvals_range_d
+ * replaces eigvals!(A::SkewHermitian{<:Real}, irange) / (A, vl, vh) (src/eigen.jl:8-16, 57-73:
+ * GPU tridiagonalization first; n == 1 returns the single entry unless (vl, vh) excludes it).
+ */
+int skh_skewtrieigvals_range_d(int64_t n, const double *ev, int64_t il, int64_t iu, double vl, double vh, int by_value,
+                               double *vals, int64_t *m);
+int skh_eigvals_range_d(int64_t n, const double *A, int64_t lda, int64_t il, int64_t iu, double vl, double vh,
+                        int by_value, double *vals, int64_t *m, const skh_opts *opts);
+/*
  * Eigenvectors as well: variant 0 = skewtrieigen_divided! (src/skeweigen.jl:322-457, used by
  * eigen(::SkewHermitian)), 1 = skewtrieigen_merged! (:195-320, used by eigen(::SkewHermTridiagonal)).
  * vr, vi: n x n real and imaginary parts of the eigenvector matrix (column-major).

@@ -400,3 +400,31 @@ extern "C" int skh_internal_symtri_eigvals(int64_t n, double* d, const double* e
     std::copy(dv.begin(), dv.end(), d);
     return 0;
 }
+
+// Eigenvalues of a real SkewHermTridiagonal(ev) by index or value range, as the reference's
+// skewtrieigvals!(S, irange) / (S, vl, vh) (src/tridiag.jl:525-537): the eigenvalues of the
+// symmetric tridiagonal H = SymTridiagonal(0, ev), ascending, either with 1-based indices
+// il..iu or in (-vh, -vl] (LAPACK stebz's half-open interval), each negated.  vals receives
+// *m values (room for n).
+extern "C" int skh_skewtrieigvals_range_d(int64_t N, const double* ev, int64_t il, int64_t iu, double vl, double vh,
+                                          int by_value, double* vals, int64_t* m)
+{
+    if (N < 0) return -1;
+    if (N > 1 && !ev) return -2;
+    if (!vals || !m) return -8;
+    *m = 0;
+    if (N == 0) return 0;
+    if (!by_value && (il < 1 || iu > N || il > iu + 1)) return -3;
+    std::vector<double> d(N, 0.0);
+    if (N > 1) {
+        std::vector<double> e(ev, ev + (N - 1));
+        if (!symtri_ql(N, d, e, nullptr)) return 2;
+    }
+    int64_t k = 0;
+    for (int64_t i = 0; i < N; i++) {
+        const bool take = by_value ? (d[i] > -vh && d[i] <= -vl) : (i + 1 >= il && i + 1 <= iu);
+        if (take) vals[k++] = -d[i];
+    }
+    *m = k;
+    return 0;
+}

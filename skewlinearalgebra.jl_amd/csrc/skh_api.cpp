@@ -1364,4 +1364,33 @@ int skh_eigen_z(int64_t n, const double* A, int64_t lda, double* vals, double* V
     return 0;
 }
 
+int skh_eigvals_range_d(int64_t n, const double* A, int64_t lda, int64_t il, int64_t iu, double vl, double vh,
+                        int by_value, double* vals, int64_t* m, const skh_opts* opts)
+{
+    if (n < 0) return set_err(-1, "n < 0");
+    if (n > 0 && !A) return set_err(-2, "A is NULL");
+    if (lda < std::max<int64_t>(n, 1)) return set_err(-3, "lda < n");
+    if (!vals || !m) return set_err(-9, "output NULL");
+    *m = 0;
+    if (n == 0) return 0;
+    if (!by_value && (il < 1 || iu > n || il > iu + 1)) return set_err(-4, "irange out of 1:n");
+    if (n == 1) {
+        // src/eigen.jl:58,67: the single entry, unless (vl, vh) excludes it
+        const double a = A[0];
+        if (!by_value || (0.0 > vl && 0.0 < vh)) {
+            vals[0] = a;
+            *m = 1;
+            return 0;
+        }
+    }
+    std::vector<double> F((size_t)n * n), tau(n), eta(n);
+    for (int64_t c = 0; c < n; c++) std::memcpy(&F[c * n], A + c * lda, n * sizeof(double));
+    int64_t nz = 0;
+    int rc = n > 1 ? skh_sytrd_d(n, F.data(), n, tau.data(), eta.data(), &nz, opts) : 0;
+    if (rc) return rc;
+    rc = skh_skewtrieigvals_range_d(n, eta.data(), il, iu, vl, vh, by_value, vals, m);
+    if (rc == 2) return set_err(SKH_ERR_NOCONV, "tridiagonal QL did not converge");
+    return rc;
+}
+
 }  // extern "C"

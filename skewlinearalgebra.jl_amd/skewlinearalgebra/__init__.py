@@ -249,8 +249,61 @@ def _tri_eigvals_host(ev):
     return vals
 
 
-def eigvals_(A, sortby=None, nb=0, engine=ENGINE_AUTO):
-    """eigvals!(A::SkewHermitian{<:Real} | SkewHermTridiagonal{<:Real}, sortby) -> complex(0, vals)."""
+def _range_args(irange, vl, vh):
+    """Julia's eigvals!(A, irange::UnitRange) / eigvals!(A, vl, vh): irange = (il, iu), 1-based, inclusive."""
+    if irange is not None and (vl is not None or vh is not None):
+        raise ArgumentError("give either irange or (vl, vh)")
+    if irange is not None:
+        if isinstance(irange, range):
+            il, iu = irange.start, irange.stop - 1
+        else:
+            il, iu = int(irange[0]), int(irange[-1])
+        return il, iu, 0.0, 0.0, 0
+    if vl is None or vh is None:
+        raise ArgumentError("both vl and vh are needed")
+    return 0, 0, float(vl), float(vh), 1
+
+
+def eigvals_(A, sortby=None, nb=0, engine=ENGINE_AUTO, irange=None, vl=None, vh=None):
+    """eigvals!(A::SkewHermitian | SkewHermTridiagonal{<:Real}, sortby) -> complex(0, vals);
+    with irange = (il, iu) or (vl, vh): the range forms (src/eigen.jl:8-16,29-41,57-73,
+    src/tridiag.jl:452-460,525-537)."""
+    if irange is not None or vl is not None or vh is not None:
+        il, iu, lo, hi, by_value = _range_args(irange, vl, vh)
+        if isinstance(A, SkewHermTridiagonal):
+            if np.iscomplexobj(A.ev) or A.dvim is not None:
+                raise NotImplementedError("complex SkewHermTridiagonal is not supported by this build")
+            ev = np.array(A.ev, dtype=np.float64, copy=True)
+            n = ev.size + 1
+            vals = np.zeros(n)
+            m = ctypes.c_int64(0)
+            rc = _lib.lib().skh_skewtrieigvals_range_d(n, _ptr(ev), il, iu, lo, hi, by_value, _ptr(vals), ctypes.byref(m))
+            _lib.check("skh_skewtrieigvals_range_d", rc)
+            return vals[:m.value] * 1j
+        S = A if isinstance(A, SkewHermitian) else SkewHermitian(A)
+        D = S.data
+        n = D.shape[0]
+        if np.iscomplexobj(D):
+            # src/eigen.jl:36-41: eigenvalues lambda of Hermitian(iA) in (-vh, -vl], ascending, negated.
+            # (The reference's complex irange form passes a negative-step range to LAPACK and cannot
+            # run; it is not offered here.)
+            if not by_value:
+                raise NotImplementedError("eigvals(::SkewHermitian{<:Complex}, irange) has no working "
+                                          "reference form; use (vl, vh)")
+            allv = eigvals_(S, None, nb, engine).imag     # -lambda for ascending lambda
+            lam = -allv
+            keep = (lam > -hi) & (lam <= -lo)
+            return (-lam[keep]) * 1j
+        F = np.asfortranarray(D, dtype=np.float64)
+        vals = np.zeros(max(n, 1))
+        m = ctypes.c_int64(0)
+        o = _lib.opts(nb, engine)
+        rc = _lib.lib().skh_eigvals_range_d(n, _ptr(F), max(n, 1), il, iu, lo, hi, by_value, _ptr(vals),
+                                            ctypes.byref(m), ctypes.byref(o))
+        if rc == 2:
+            raise RuntimeError("Maximum number of iterations reached, the algorithm didn't converge")
+        _lib.check("skh_eigvals_range_d", rc)
+        return vals[:m.value] * 1j
     if isinstance(A, SkewHermTridiagonal):
         if np.iscomplexobj(A.ev) or A.dvim is not None:
             raise NotImplementedError("complex SkewHermTridiagonal is not supported by this build")
@@ -280,13 +333,14 @@ def eigvals_(A, sortby=None, nb=0, engine=ENGINE_AUTO):
     return vals * 1j
 
 
-def eigvals(A, sortby=None, nb=0, engine=ENGINE_AUTO):
-    """eigvals(A) = eigvals!(copyeigtype(A), sortby) (src/eigen.jl:43, src/tridiag.jl:484-489)."""
+def eigvals(A, sortby=None, nb=0, engine=ENGINE_AUTO, irange=None, vl=None, vh=None):
+    """eigvals(A) = eigvals!(copyeigtype(A), sortby | irange | vl, vh) (src/eigen.jl:43-45,
+    src/tridiag.jl:484-489)."""
     if isinstance(A, SkewHermTridiagonal):
-        return eigvals_(A.copy(), sortby)
+        return eigvals_(A.copy(), sortby, irange=irange, vl=vl, vh=vh)
     D = A.data if isinstance(A, SkewHermitian) else np.asarray(A)
     S = SkewHermitian(np.array(D, dtype=_eigtype(D), order="F", copy=True))
-    return eigvals_(S, sortby, nb, engine)
+    return eigvals_(S, sortby, nb, engine, irange=irange, vl=vl, vh=vh)
 
 
 class Eigen:

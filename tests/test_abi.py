@@ -120,3 +120,53 @@ def test_python_api_host_paths():
     assert abs(sg * np.exp(lp) - skl.pfaffian(T)) < 1e-12 and skl.pfaffian(T) == 15.0
     vals = skl.eigvals(T)
     np.testing.assert_allclose(np.sort(vals.imag), np.sort(np.linalg.eigvals(T.to_dense()).imag), atol=1e-12)
+
+
+def test_skewtrieigvals_range_readme(golden):
+    """eigvals(A, 0, 15) and eigvals(A, 1:3) of the README matrix (test/runtests.jl:24-25), on
+    the tridiagonal part the library runs on the host (reference eta as input)."""
+    from skewlinearalgebra import _lib
+    L = _lib.lib()
+    g = golden["readme"]
+    eta = np.array(g["eta"])
+    l1, l2 = g["eig_pos"]
+    vals = np.zeros(4)
+    m = ctypes.c_int64(0)
+    assert L.skh_skewtrieigvals_range_d(4, eta.ctypes.data, 0, 0, 0.0, 15.0, 1, vals.ctypes.data, ctypes.byref(m)) == 0
+    np.testing.assert_allclose(vals[:m.value], [l1, l2], rtol=1e-12)
+    assert L.skh_skewtrieigvals_range_d(4, eta.ctypes.data, 1, 3, 0.0, 0.0, 0, vals.ctypes.data, ctypes.byref(m)) == 0
+    np.testing.assert_allclose(vals[:m.value], [l1, l2, -l2], rtol=1e-12)
+    # empty selections and argument checks
+    assert L.skh_skewtrieigvals_range_d(4, eta.ctypes.data, 0, 0, 20.0, 30.0, 1, vals.ctypes.data, ctypes.byref(m)) == 0
+    assert m.value == 0
+    assert L.skh_skewtrieigvals_range_d(4, eta.ctypes.data, 3, 2, 0.0, 0.0, 0, vals.ctypes.data, ctypes.byref(m)) == 0
+    assert m.value == 0
+    assert L.skh_skewtrieigvals_range_d(4, eta.ctypes.data, 0, 3, 0.0, 0.0, 0, vals.ctypes.data, ctypes.byref(m)) == -3
+    assert L.skh_skewtrieigvals_range_d(4, eta.ctypes.data, 2, 5, 0.0, 0.0, 0, vals.ctypes.data, ctypes.byref(m)) == -3
+    # Python mirror on a SkewHermTridiagonal (host only)
+    T = skl.SkewHermTridiagonal(eta)
+    np.testing.assert_allclose(skl.eigvals(T, vl=0, vh=15).imag, [l1, l2], rtol=1e-12)
+    np.testing.assert_allclose(skl.eigvals(T, irange=(1, 3)).imag, [l1, l2, -l2], rtol=1e-12)
+    np.testing.assert_allclose(skl.eigvals(T, irange=range(2, 5)).imag, [l2, -l2, -l1], rtol=1e-12)
+
+
+def test_skewtrieigvals_range_vs_dense():
+    """Index and value ranges against the dense symmetric tridiagonal eigenvalues (numpy)."""
+    from skewlinearalgebra import _lib
+    L = _lib.lib()
+    rng = np.random.default_rng(77)
+    for n in (1, 2, 5, 40, 41):
+        ev = rng.standard_normal(max(n - 1, 0))
+        H = np.diag(ev, -1) + np.diag(ev, 1)
+        lam = np.linalg.eigvalsh(H) if n > 0 else np.zeros(0)
+        vals = np.zeros(n)
+        m = ctypes.c_int64(0)
+        il, iu = 1 + n // 4, max(1 + n // 4, (3 * n) // 4)
+        assert L.skh_skewtrieigvals_range_d(n, ev.ctypes.data, il, iu, 0.0, 0.0, 0, vals.ctypes.data,
+                                            ctypes.byref(m)) == 0
+        np.testing.assert_allclose(vals[:m.value], -lam[il - 1:iu], atol=1e-13 * max(1.0, np.abs(lam).max()))
+        vl, vh = -0.7, 1.3
+        assert L.skh_skewtrieigvals_range_d(n, ev.ctypes.data, 0, 0, vl, vh, 1, vals.ctypes.data,
+                                            ctypes.byref(m)) == 0
+        sel = lam[(lam > -vh) & (lam <= -vl)]
+        np.testing.assert_allclose(vals[:m.value], -sel, atol=1e-13 * max(1.0, np.abs(lam).max()))

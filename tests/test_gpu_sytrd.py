@@ -490,3 +490,41 @@ def test_plan_tune_and_graph_replay():
         plan.status(st)
         assert torch.equal(lp, lp0) and torch.equal(sg, sg0), r
     plan.close()
+
+
+def test_eigvals_ranges(golden):
+    """eigvals(A, vl, vh) and eigvals(A, irange) (src/eigen.jl:8-16,57-73) after the GPU
+    tridiagonalization: the README values (test/runtests.jl:24-25) and a random n = 130 matrix
+    against the same selection on the oracle's eta."""
+    say("eigvals ranges")
+    g = golden["readme"]
+    A = np.array(g["A"], float)
+    l1, l2 = g["eig_pos"]
+    np.testing.assert_allclose(skl.eigvals(A, vl=0, vh=15).imag, [l1, l2], rtol=1e-12)
+    np.testing.assert_allclose(skl.eigvals(A, irange=(1, 3)).imag, [l1, l2, -l2], rtol=1e-12)
+    assert skl.eigvals(np.zeros((1, 1)), irange=(1, 1)).size == 1
+    assert skl.eigvals(np.zeros((1, 1)), vl=1.0, vh=2.0).size == 0
+    n = 130
+    A = skew_gauss(n, 130)
+    _, _, etao = oracle.skewblockedhess(A)
+    H = np.diag(etao, -1) + np.diag(etao, 1)
+    lam = np.linalg.eigvalsh(H)
+    tol = 10 * n * EPS * np.linalg.norm(A, 2)
+    np.testing.assert_allclose(skl.eigvals(A, irange=(10, 60)).imag, -lam[9:60], atol=tol)
+    sel = lam[(lam > -2.0) & (lam <= 1.0)]
+    got = skl.eigvals(A, vl=-1.0, vh=2.0).imag
+    assert got.size == sel.size
+    np.testing.assert_allclose(got, -sel, atol=tol)
+
+
+def test_complex_eigvals_value_range():
+    say("complex eigvals (vl, vh)")
+    rng = np.random.default_rng(5)
+    n = 64
+    B = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = (B - B.conj().T) / 2
+    lam = np.linalg.eigvalsh(1j * A)
+    got = skl.eigvals(A, vl=-2.0, vh=3.0).imag
+    ref = -lam[(lam > -3.0) & (lam <= 2.0)]
+    assert got.size == ref.size
+    np.testing.assert_allclose(got, ref, atol=10 * n * EPS * np.linalg.norm(A, 2))
