@@ -51,6 +51,9 @@ __device__ __forceinline__ double wave_sum(double x)
 
 __device__ __forceinline__ int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// index of this thread's wave in the workgroup, as a wave-uniform (scalar) value
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
 // ---------------------------------------------------------------------------------------
 // Reflector scalars from alpha = x[0] and the sum of squares of x[1:] (householder!,
 // src/hessenberg.jl:26-44).  beta = (alpha > 0 ? -1 : +1) * hypot(alpha, xnorm),
@@ -132,12 +135,16 @@ struct RowShared {
 };
 
 // KQ = largest k-group per wave (panel width <= 4*KQ); the rows of V and W used by a lane
-// are cached in 2*KQ registers.
+// are cached in 2*KQ registers, shared by the C part (correction of w_{j-1}, k < i-1) and the
+// A part (update of column j, k < i).  Every global load a chunk needs (row and column partials
+// of the matvec, its rows of V and W, the old entries of columns j-1 and j) and the shared
+// inputs (t, row j of V and W, the special row's partials) are issued together, before the
+// first barrier, so a row pass costs one memory round trip.
 template <int KQ>
 __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j, int i,
                          bool doC, bool doA, const Work& w, int g, int G, RowShared& sh)
 {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
     const int64_t jm = j - 1;         // column finalised by the C part
     const int im = i - 1;             // its local index
     const ColScal cs = sh.cur;        // scalars of column jm (valid when doC)
@@ -145,132 +152,141 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
     const int64_t ldw = w.ldw;
     const int64_t b0 = j / TB;        // first chunk block
     const int64_t nt = cdiv(n, TB);
+    const int64_t nq = nt - b0;
+    const int kq = (i + 3) / 4;       // k-group size per wave
+    const int k0 = wave * kq, k1 = min(i, k0 + kq);
+    const int kc = doC ? min(k1, im) : k0;   // C correction uses k in [k0, kc)
+    constexpr int PB = 16;            // column partials per lane per batch (4 waves: 64 strips)
 
-    if (doC) {
-        // t1, t2 of column jm (im entries each)
-        for (int k = tid; k < im; k += WG) {
-            sh.t[k] = w.Tv[k];
-            sh.t[NBMAX + k] = w.Tv[NBMAX + k];
-        }
-    }
-    __syncthreads();
-    if (doC && wave == 0) {
-        // special row r = j of w_{jm}: strip b0 has one segment; column partials I in [b0, nt)
-        double acc = 0.0;
-        if (lane == 0) acc += w.Prow[j];
-        for (int64_t I = b0 + lane; I < nt; I += 64) acc += w.Pcol[I * w.ldp + j];
-        for (int k = lane; k < im; k += 64)
-            acc += A[j + (c0 + k) * lda] * sh.t[k] - Wm[j + k * ldw] * sh.t[NBMAX + k];
-        acc = wave_sum(acc);
-        if (lane == 0) sh.wspec = cs.tau * acc;
-    }
-    __syncthreads();
-    if (doA) {
-        for (int k = tid; k < i; k += WG) {
-            if (doC && k == im) {
-                sh.wrow[k] = sh.wspec;
-                sh.vrow[k] = cs.unit;
-            } else {
-                sh.wrow[k] = Wm[j + k * ldw];
-                sh.vrow[k] = A[j + (c0 + k) * lda];
+    for (int64_t q = g; q < nq; q += G) {
+        const bool first = (q == g);
+        const int64_t b = b0 + q;
+        const int64_t r = b * TB + lane;
+        const bool rin = r < n;
+        const bool rowC = doC && rin && r >= j;
+        // ---- 1. every global load of this step, into registers ----
+        double tv1 = 0.0, tv2 = 0.0, wraw = 0.0, vraw = 0.0, sprow = 0.0, sp[4] = {0.0, 0.0, 0.0, 0.0};
+        if (first) {
+            if (doC && tid < im) {
+                tv1 = w.Tv[tid];
+                tv2 = w.Tv[NBMAX + tid];
+            }
+            if (tid < i) {
+                wraw = Wm[j + tid * ldw];
+                vraw = A[j + (c0 + tid) * lda];
+            }
+            if (doC && wave == 0) {
+                // special row r = j of w_{j-1}: its strip has one segment; column partials I >= b0
+                if (lane == 0) sprow = w.Prow[j];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int64_t I = b0 + lane + 64 * u;
+                    sp[u] = (I < nt) ? w.Pcol[I * w.ldp + j] : 0.0;
+                }
             }
         }
-    }
-    __syncthreads();
-
-    const int kq = (i + 3) / 4;           // k-group size per wave for the A part
-    const int kqm = (im + 3) / 4;         // and for the C correction
-    const int64_t nq = nt - b0;
-    for (int64_t q = g; q < nq; q += G) {
-        const int64_t r = (b0 + q) * TB + lane;
-        const bool rin = r < n;
-        double wr = 0.0, vr = 0.0;
-        if (doC) {
-            double p = 0.0;
-            if (rin && r >= j) {
-                const int64_t b = b0 + q;
-                // row partials of strip b: one slot per rank whose tile range touched it
-                const int64_t nslot = strip_slots(b - b0, nt - b0, G);
-                for (int64_t s = wave; s < nslot; s += 4) p += w.Prow[s * w.ldp + r];
-                // loads issued in unrolled batches (one memory round trip per batch)
+        double vreg[KQ], wreg[KQ];
+#pragma unroll
+        for (int u = 0; u < KQ; u++) {
+            const int k = k0 + u;
+            const bool ok = rin && r >= j && k < k1 && !(doC && k == im);
+            vreg[u] = ok ? A[r + (c0 + k) * lda] : 0.0;
+            wreg[u] = ok ? Wm[r + k * ldw] : 0.0;
+        }
+        const double xo = rowC ? A[r + jm * lda] : 0.0;               // x of column j-1 (unscaled)
+        const double xold = (doA && rin && r > j) ? A[r + j * lda] : 0.0;
+        double pp = 0.0;
+        if (rowC) {
+            const int64_t nslot = strip_slots(b - b0, nt - b0, G);
+            for (int64_t s0 = wave; s0 < nslot; s0 += 16) {
+                double sb[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) sb[u] = (s0 + 4 * u < nslot) ? w.Prow[(s0 + 4 * u) * w.ldp + r] : 0.0;
+                pp += (sb[0] + sb[1]) + (sb[2] + sb[3]);
+            }
+            for (int64_t I0 = b + wave; I0 < nt; I0 += 4 * PB) {
+                double buf[PB];
+#pragma unroll
+                for (int u = 0; u < PB; u++) {
+                    const int64_t I = I0 + 4 * u;
+                    buf[u] = (I < nt) ? w.Pcol[I * w.ldp + r] : 0.0;
+                }
                 double p4[4] = {0.0, 0.0, 0.0, 0.0};
-                for (int64_t I0 = b + wave; I0 < nt; I0 += 64) {
-                    double buf[16];
 #pragma unroll
-                    for (int u = 0; u < 16; u++) {
-                        const int64_t I = I0 + 4 * u;
-                        buf[u] = (I < nt) ? w.Pcol[I * w.ldp + r] : 0.0;
-                    }
+                for (int u = 0; u < PB; u++) p4[u & 3] += buf[u];
+                pp += (p4[0] + p4[1]) + (p4[2] + p4[3]);
+            }
+        }
+        // ---- 2. shared inputs into LDS (first chunk), the special row of w_{j-1} ----
+        if (first) {
+            if (doC && tid < im) {
+                sh.t[tid] = tv1;
+                sh.t[NBMAX + tid] = tv2;
+            }
+            if (tid < i) {
+                sh.wrow[tid] = wraw;
+                sh.vrow[tid] = vraw;
+            }
+            __syncthreads();
+            if (doC && wave == 0) {
+                double acc = sprow;
 #pragma unroll
-                    for (int u = 0; u < 16; u++) p4[u & 3] += buf[u];
+                for (int u = 0; u < 4; u++) acc += sp[u];
+                for (int k = lane; k < im; k += 64) acc += sh.vrow[k] * sh.t[k] - sh.wrow[k] * sh.t[NBMAX + k];
+                acc = wave_sum(acc);
+                if (lane == 0) {
+                    const double ws = cs.tau * acc;
+                    sh.wspec = ws;
+                    sh.wrow[im] = ws;
+                    sh.vrow[im] = cs.unit;
                 }
-                p += (p4[0] + p4[1]) + (p4[2] + p4[3]);
-                const int k0 = wave * kqm, k1 = min(im, k0 + kqm);
-                double vb[KQ], wb[KQ];
-#pragma unroll
-                for (int u = 0; u < KQ; u++) {
-                    const int k = k0 + u;
-                    vb[u] = (k < k1) ? A[r + (c0 + k) * lda] : 0.0;
-                    wb[u] = (k < k1) ? Wm[r + k * ldw] : 0.0;
-                }
-                double c4[4] = {0.0, 0.0, 0.0, 0.0};
+            }
+        }
+        // ---- C part: w_{j-1} on this chunk ----
+        if (doC) {
+            double c4[4] = {0.0, 0.0, 0.0, 0.0};
+            if (rowC) {
 #pragma unroll
                 for (int u = 0; u < KQ; u++)
-                    if (k0 + u < k1) c4[u & 3] += vb[u] * sh.t[k0 + u] - wb[u] * sh.t[NBMAX + k0 + u];
-                p += (c4[0] + c4[1]) + (c4[2] + c4[3]);
+                    if (k0 + u < kc) c4[u & 3] += vreg[u] * sh.t[k0 + u] - wreg[u] * sh.t[NBMAX + k0 + u];
             }
-            sh.red[wave][lane] = p;
+            sh.red[wave][lane] = rowC ? pp + ((c4[0] + c4[1]) + (c4[2] + c4[3])) : 0.0;
         }
         __syncthreads();
-        if (doC && rin && r >= j) {
-            double s = sh.red[0][lane] + sh.red[1][lane] + sh.red[2][lane] + sh.red[3][lane];
-            wr = (r == j) ? sh.wspec : cs.tau * s;
-            vr = (r == j) ? cs.unit : cs.scale * A[r + jm * lda];
-        }
-        __syncthreads();   // every wave has read A[r, jm] before wave 0 overwrites it
-        if (doC && rin && r >= j && wave == 0) {
-            Wm[r + im * ldw] = wr;
-            A[r + jm * lda] = vr;
+        double wr = 0.0, vr = 0.0;
+        if (rowC) {
+            const double sacc = sh.red[0][lane] + sh.red[1][lane] + sh.red[2][lane] + sh.red[3][lane];
+            wr = (r == j) ? sh.wspec : cs.tau * sacc;
+            vr = (r == j) ? cs.unit : cs.scale * xo;
+            if (wave == 0) {
+                Wm[r + im * ldw] = wr;
+                A[r + jm * lda] = vr;
+            }
         }
         if (doA) {
-            // this lane's row of V and W for the wave's k-group, loaded once (kq <= KQ)
-            const int k0 = wave * kq, k1 = min(i, k0 + kq);
-            double vreg[KQ], wreg[KQ];
-            double xold = 0.0;
+            // ---- A part: update column j with the panel's reflectors 0..i-1 ----
             const bool live = rin && r > j;
+            double p4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int u = 0; u < KQ; u++) {
                 const int k = k0 + u;
-                double vk = 0.0, wk = 0.0;
-                if (live && k < k1) {
-                    if (doC && k == im) {
-                        vk = vr;
-                        wk = wr;
-                    } else {
-                        vk = A[r + (c0 + k) * lda];
-                        wk = Wm[r + k * ldw];
-                    }
+                if (doC && k == im) {
+                    vreg[u] = live ? vr : 0.0;
+                    wreg[u] = live ? wr : 0.0;
+                } else if (!live) {
+                    vreg[u] = 0.0;
+                    wreg[u] = 0.0;
                 }
-                vreg[u] = vk;
-                wreg[u] = wk;
+                if (live && k < k1) p4[u & 3] += vreg[u] * sh.wrow[k] - wreg[u] * sh.vrow[k];
             }
-            double p4[4] = {0.0, 0.0, 0.0, 0.0};
-            if (live) {
-                xold = A[r + j * lda];
-#pragma unroll
-                for (int u = 0; u < KQ; u++)
-                    if (k0 + u < k1) p4[u & 3] += vreg[u] * sh.wrow[k0 + u] - wreg[u] * sh.vrow[k0 + u];
-            }
-            const double p = (p4[0] + p4[1]) + (p4[2] + p4[3]);
-            sh.red2[wave][lane] = p;
+            sh.red2[wave][lane] = (p4[0] + p4[1]) + (p4[2] + p4[3]);
             __syncthreads();
             double x = 0.0;
-            if (rin && r > j)
-                x = xold + (sh.red2[0][lane] + sh.red2[1][lane] + sh.red2[2][lane] + sh.red2[3][lane]);
-            if (i > 0 && rin && r > j && wave == 0) A[r + j * lda] = x;
+            if (live) x = xold + (sh.red2[0][lane] + sh.red2[1][lane] + sh.red2[2][lane] + sh.red2[3][lane]);
+            if (i > 0 && live && wave == 0) A[r + j * lda] = x;
             const bool tail = rin && r >= j + 2;
             if (wave == 0) {
-                double ss = wave_sum(tail ? x * x : 0.0);
+                const double ss = wave_sum(tail ? x * x : 0.0);
                 if (lane == 0) w.Pnorm[q] = ss;
             }
             // S partials of column j: sum over the chunk's rows r >= j+2 of W[r,k] x_r, V[r,k] x_r
@@ -300,7 +316,7 @@ __device__ void column_scalars(const double* __restrict__ A, int64_t lda, int64_
                                const Work& w, RowShared& sh, double* tau_out, double* eta_out,
                                bool writer)
 {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
     if (wave == 0) {
         const int64_t nq = cdiv(n, TB) - j / TB;
         double acc = 0.0;
@@ -322,7 +338,7 @@ __device__ void column_scalars(const double* __restrict__ A, int64_t lda, int64_
 __device__ void reduce_t(const double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j,
                          int i, const Work& w, const RowShared& sh, int g, int G)
 {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
     const int64_t nq = cdiv(n, TB) - j / TB;
     const ColScal s = sh.cur;
     for (int tv = g * 4 + wave; tv < 2 * i; tv += G * 4) {
@@ -390,7 +406,7 @@ __device__ __forceinline__ void strip_rows(int64_t I, int p, int64_t rows[4])
 __device__ void skew_matvec(const double* __restrict__ A, int64_t lda, int64_t n, int64_t j,
                             const Work& w, RowShared& sh, int g, int G)
 {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
     const int p = lane & 15, cg = lane >> 4;
     const ColScal s = sh.cur;
     const int64_t nt = cdiv(n, TB);
@@ -511,7 +527,7 @@ __device__ __forceinline__ void rank2k_tile(double* __restrict__ A, int64_t lda,
                                             int64_t ldu, int64_t n, int64_t rlo, int64_t clo, int64_t c0, int nbp,
                                             int64_t r0, int64_t q0, MfmaShared& sm)
 {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
     const int wr = wave >> 1, wc = wave & 1;
     // a wave whose whole 64x64 sub-block lies on or above the diagonal has nothing to do
     const bool active = (r0 + wr * 64 + 63 > q0 + wc * 64);
@@ -636,7 +652,7 @@ __device__ void logpf_reduce(const double* eta, const double* tau, int64_t n, do
 {
     __shared__ double s_l[4];
     __shared__ int s_c[4][3];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
     double l = 0.0;
     int nneg = 0, nzero = 0, ntau = 0;
     for (int64_t i = tid; i < n - 1; i += WG) {

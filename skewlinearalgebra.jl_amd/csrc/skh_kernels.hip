@@ -21,8 +21,9 @@ namespace skh {
 
 // Optional phase clock (workgroup 0, 100 MHz wall clock): ph[0] row passes, ph[1] barrier 1
 // waits, ph[2] scalars + t + matvec, ph[3] barrier 2 waits (accumulated over all launches).
+// panels up to 64 wide fit two workgroups per CU (<= 256 registers per lane); 128 wide takes one
 template <int KQ>
-__global__ __launch_bounds__(WG, 1) void k_panel(double* A, int64_t lda, int64_t n, int64_t c0, int nbp, Work w,
+__global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int64_t lda, int64_t n, int64_t c0, int nbp, Work w,
                                               double* tau, double* eta, unsigned* bar, unsigned base, int* err,
                                               unsigned long long* ph)
 {

@@ -6,7 +6,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libskewhip.so")
+# SKEWHIP_LIB names another build of the same library (the in-tree one by default)
+LIB_PATH = os.environ.get("SKEWHIP_LIB") or os.path.join(_HERE, "libskewhip.so")
 
 ENGINE_AUTO, ENGINE_LARGE, ENGINE_TEAM1 = 0, 1, 2
 FLAG_TIMING = 0x100
