@@ -183,7 +183,10 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
     if (rc) return rc;
     const int64_t ld = (n + 1) & ~int64_t(1);
     const int64_t nt = (n + skh::TB - 1) / skh::TB + 1;
-    const int64_t nseg = nt;   // Prow slots per row
+    int per_cu0 = 0;
+    HIPCHK((hipError_t)skh_internal_panel_occupancy((int)nb, &per_cu0));
+    // row-partial slots of one matvec strip: at most one per workgroup of the grid
+    const int64_t nseg = (int64_t)std::max(1, std::min(per_cu0, 2)) * g_num_cu + 2;
     size_t doubles = (size_t)(ld * nb + nt + nt * 2 * skh::NBMAX + 2 * skh::NBMAX + nseg * ld + nt * ld + 64);
     double* ws = (double*)g_ws.get(doubles * sizeof(double));
     if (!ws) return set_err(SKH_ERR_HIP, "hipMalloc of %zu bytes of workspace failed", doubles * 8);
@@ -204,6 +207,12 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
     if ((int64_t)64 * (int64_t)(2 * n + 4) > 4000000000LL)
         return set_err(SKH_ERR_UNSUPPORTED, "n = %lld too large for the barrier counter", (long long)n);
 
+    // target matvec tiles per workgroup (tuning knob SKH_PANEL_TPW; measured default)
+    static const int64_t tpw = [] {
+        const char* e = getenv("SKH_PANEL_TPW");
+        const long v = e ? atol(e) : 0;
+        return (int64_t)(v > 0 ? v : 8);
+    }();
     HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 512, st));
     unsigned base = 0;
     for (int64_t c0 = 0; c0 <= n - 3; c0 += nb) {
@@ -220,12 +229,11 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
         pl.bar = ctl.bar;
         pl.base = base;
         pl.err = ctl.err;
-        // grid sized to the panel's first matvec: ~8 tiles per workgroup (fewer workgroups make
-        // the two grid barriers per column cheaper once the trailing matrix is small)
+        // grid sized to the panel's first matvec: about `tpw` tiles per workgroup (fewer
+        // workgroups make the two grid barriers per column cheaper once the matrix is small)
         {
-            const int64_t ns0 = (n + skh::TB - 1) / skh::TB - (c0 + 1) / skh::TB;
-            const int64_t tiles = ns0 * (ns0 + 1) / 2;
-            const int64_t want = std::max<int64_t>(8, (tiles + 7) / 8);
+            const int64_t tiles = skh::tall_grid(n, c0 + 1).T;
+            const int64_t want = std::max<int64_t>(8, (tiles + tpw - 1) / tpw);
             pl.grid = (int)std::min<int64_t>(grid, want);
         }
         pl.nbmax = (int)nb;

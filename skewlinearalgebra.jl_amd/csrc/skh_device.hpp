@@ -116,6 +116,54 @@ __host__ __device__ inline int64_t strip_slots(int64_t s, int64_t ns, int64_t G)
 }
 
 // ---------------------------------------------------------------------------------------
+// Tiles of the panel kernel's matvec: MR x MC (256 rows x 16 columns), strip-major.  Strip I
+// holds rows [MR I, MR I + MR); for the trailing block starting at s0 its column blocks are
+// J = s0/MC .. (MR I + MR)/MC - 1.  Tall tiles make every column segment a lane group reads
+// 2 KB long (DRAM row locality), and leave few strips for the column partials of a row.
+// ---------------------------------------------------------------------------------------
+constexpr int MR = 256;
+constexpr int MC = 16;
+constexpr int MPS = MR / MC;   // column blocks per strip row band
+
+struct TallGrid {
+    int64_t s0, Jlo, I0, nts, T;
+};
+
+__host__ __device__ inline int64_t tall_first(const TallGrid& tg, int64_t I)
+{
+    // tiles of strips I0 .. I-1: sum (MPS I' + MPS - Jlo)
+    const int64_t k = I - tg.I0;
+    return MPS * (k * (tg.I0 + I - 1) / 2) + k * (MPS - tg.Jlo);
+}
+
+__host__ __device__ inline TallGrid tall_grid(int64_t n, int64_t s0)
+{
+    TallGrid tg;
+    tg.s0 = s0;
+    tg.Jlo = s0 / MC;
+    tg.I0 = s0 / MR;
+    tg.nts = (n + MR - 1) / MR;
+    tg.T = tall_first(tg, tg.nts);
+    return tg;
+}
+
+__host__ __device__ inline int64_t tall_strip_of(const TallGrid& tg, int64_t t)
+{
+    int64_t I = tg.I0;
+    while (I + 1 < tg.nts && tall_first(tg, I + 1) <= t) I++;
+    return I;
+}
+
+// row-partial slots of strip I (ranks whose tile range touches it) with G ranks
+__host__ __device__ inline int64_t tall_slots(const TallGrid& tg, int64_t I, int64_t G)
+{
+    if (G > tg.T) G = tg.T;
+    const int64_t f = tall_first(tg, I);
+    const int64_t cnt = MPS * I + MPS - tg.Jlo;
+    return tile_owner(f + cnt - 1, tg.T, G) - tile_owner(f, tg.T, G) + 1;
+}
+
+// ---------------------------------------------------------------------------------------
 // Row pass: finalises w of column j-1 (doC) and/or applies the panel column update to
 // column j and produces its norm and W^T x / V^T x partials (doA).
 //   i = j - c0 = number of previous panel columns that update column j.
@@ -130,6 +178,7 @@ struct RowShared {
     double t[2 * NBMAX];
     double red[4][TB];
     double red2[4][TB];
+    double mred[4][MR];   // matvec row sums of one strip, per wave
     double wspec;
     ColScal cur;     // scalars of the column finalised by the last phase B
 };
@@ -157,6 +206,7 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
     const int k0 = wave * kq, k1 = min(i, k0 + kq);
     const int kc = doC ? min(k1, im) : k0;   // C correction uses k in [k0, kc)
     constexpr int PB = 16;            // column partials per lane per batch (4 waves: 64 strips)
+    const TallGrid tg = tall_grid(n, j);     // tiles of the matvec that produced the partials
 
     for (int64_t q = g; q < nq; q += G) {
         const bool first = (q == g);
@@ -176,13 +226,15 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                 vraw = A[j + (c0 + tid) * lda];
             }
             if (doC && wave == 0) {
-                // special row r = j of w_{j-1}: its strip has one segment; column partials I >= b0
-                if (lane == 0) sprow = w.Prow[j];
+                // special row r = j of w_{j-1}: no stored entry of row j lies in the block, so its row
+                // sums are empty; column partials of the strips I >= j / MR
+                const int64_t S0 = j / MR;
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    const int64_t I = b0 + lane + 64 * u;
-                    sp[u] = (I < nt) ? w.Pcol[I * w.ldp + j] : 0.0;
+                    const int64_t I = S0 + lane + 64 * u;
+                    sp[u] = (I < tg.nts) ? w.Pcol[I * w.ldp + j] : 0.0;
                 }
+                for (int64_t I = S0 + lane + 256; I < tg.nts; I += 64) sprow += w.Pcol[I * w.ldp + j];
             }
         }
         double vreg[KQ], wreg[KQ];
@@ -197,19 +249,20 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
         const double xold = (doA && rin && r > j) ? A[r + j * lda] : 0.0;
         double pp = 0.0;
         if (rowC) {
-            const int64_t nslot = strip_slots(b - b0, nt - b0, G);
-            for (int64_t s0 = wave; s0 < nslot; s0 += 16) {
-                double sb[4];
+            const int64_t S = (b * TB) / MR;    // matvec strip holding this chunk
+            const int64_t nslot = tall_slots(tg, S, G);
+            for (int64_t s0 = wave; s0 < nslot; s0 += 32) {
+                double sb[8];
 #pragma unroll
-                for (int u = 0; u < 4; u++) sb[u] = (s0 + 4 * u < nslot) ? w.Prow[(s0 + 4 * u) * w.ldp + r] : 0.0;
-                pp += (sb[0] + sb[1]) + (sb[2] + sb[3]);
+                for (int u = 0; u < 8; u++) sb[u] = (s0 + 4 * u < nslot) ? w.Prow[(s0 + 4 * u) * w.ldp + r] : 0.0;
+                pp += ((sb[0] + sb[1]) + (sb[2] + sb[3])) + ((sb[4] + sb[5]) + (sb[6] + sb[7]));
             }
-            for (int64_t I0 = b + wave; I0 < nt; I0 += 4 * PB) {
+            for (int64_t I0 = S + wave; I0 < tg.nts; I0 += 4 * PB) {
                 double buf[PB];
 #pragma unroll
                 for (int u = 0; u < PB; u++) {
                     const int64_t I = I0 + 4 * u;
-                    buf[u] = (I < nt) ? w.Pcol[I * w.ldp + r] : 0.0;
+                    buf[u] = (I < tg.nts) ? w.Pcol[I * w.ldp + r] : 0.0;
                 }
                 double p4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -354,53 +407,35 @@ __device__ void reduce_t(const double* __restrict__ A, int64_t lda, int64_t n, i
     }
 }
 
-// One lane's 4 rows x 4 columns of tile (strip rows, tile column J) of the strictly lower
-// triangle of A[j+1:n, j+1:n] (masked), rows read as 16-byte pairs, and v at its 4 columns.
-__device__ __forceinline__ void load_tile(const double* __restrict__ A, int64_t lda, int64_t n, int64_t j, int64_t J,
-                                          int wave, int cg, const int64_t rows[4], const ColScal& s, double a[4][4],
-                                          double vc[4])
+// ---------------------------------------------------------------------------------------
+// Skew matvec partials of column j over the strictly lower triangle of A[j+1:n, j+1:n] on
+// MR x MC tiles (strip-major), split into G equal contiguous tile ranges, one per rank.
+// Per tile: wave w -> columns 4w..4w+3 (lane >> 4), lane & 15 = p -> row pairs
+// {32 h + 2p, 32 h + 2p + 1}, h = 0..7 (16 entries per lane, read as 16-byte pairs); the next
+// tile's loads are in flight while the current one is reduced.
+//   Pcol[I][c]    = -sum_{r in strip I} A[r,c] v_r                     (one entry per tile)
+//   Prow[slot][r] =  sum over the rank's tiles of strip I of A[r,c] v_c,
+//                    slot = rank - (owner of the strip's first tile)
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void tall_load(const double* __restrict__ A, int64_t lda, int64_t n, int64_t j,
+                                          int64_t I, int64_t c, int p, double a[16])
 {
-    const int64_t cb = J * TB + 16 * wave + 4 * cg;
+    const bool cok = (c > j) && (c < n);
+    const double* col = A + c * lda;
 #pragma unroll
-    for (int cc = 0; cc < 4; cc++) {
-        const int64_t c = cb + cc;
-        vc[cc] = vval(A, lda, n, j, c, s);
-        const bool cok = (c > j) && (c < n);
-        const double* col = A + c * lda;
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const int64_t r = rows[2 * h];
-            double x0 = 0.0, x1 = 0.0;
-            if (cok && r + 1 < n) {
-                const double2 t = *reinterpret_cast<const double2*>(col + r);
-                x0 = t.x;
-                x1 = t.y;
-            } else if (cok && r < n) {
-                x0 = col[r];
-            }
-            a[2 * h][cc] = (r > c) ? x0 : 0.0;
-            a[2 * h + 1][cc] = (r + 1 > c) ? x1 : 0.0;
+    for (int h = 0; h < 8; h++) {
+        const int64_t r = I * MR + 32 * h + 2 * p;
+        double x0 = 0.0, x1 = 0.0;
+        if (cok && r + 1 < n) {
+            const double2 t = *reinterpret_cast<const double2*>(col + r);
+            x0 = t.x;
+            x1 = t.y;
+        } else if (cok && r < n) {
+            x0 = col[r];
         }
+        a[2 * h] = (r > c) ? x0 : 0.0;
+        a[2 * h + 1] = (r + 1 > c) ? x1 : 0.0;
     }
-}
-
-// ---------------------------------------------------------------------------------------
-// Skew matvec partials of column j over the strictly lower triangle of A[j+1:n, j+1:n].
-// The ns(ns+1)/2 tiles (64x64; strips I = J0..nt-1 of I-J0+1 tiles, strip-major order) are split
-// into G equal contiguous ranges, one per rank.  Per tile, thread layout: wave -> 16 columns,
-// lane -> 4 columns (lane>>4) x 4 rows {2p, 2p+1, 32+2p, 33+2p} (p = lane&15), rows read as
-// 16-byte pairs, the next tile's loads in flight while the current one is reduced.
-//   Pcol[I][c]       = -sum_{r in strip I} A[r,c] v_r            (one entry per tile)
-//   Prow[slot][r]    =  sum over the rank's tiles of strip I of A[r,c] v_c,
-//                       slot = rank - (owner of the strip's first tile)
-// ---------------------------------------------------------------------------------------
-__device__ __forceinline__ void strip_rows(int64_t I, int p, int64_t rows[4])
-{
-    const int64_t rb = I * TB;
-    rows[0] = rb + 2 * p;
-    rows[1] = rb + 2 * p + 1;
-    rows[2] = rb + 32 + 2 * p;
-    rows[3] = rb + 33 + 2 * p;
 }
 
 __device__ void skew_matvec(const double* __restrict__ A, int64_t lda, int64_t n, int64_t j,
@@ -409,102 +444,76 @@ __device__ void skew_matvec(const double* __restrict__ A, int64_t lda, int64_t n
     const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
     const int p = lane & 15, cg = lane >> 4;
     const ColScal s = sh.cur;
-    const int64_t nt = cdiv(n, TB);
-    const int64_t J0 = (j + 1) / TB;
-    const int64_t ns = nt - J0;
-    const int64_t T = ns * (ns + 1) / 2;
-    const int64_t Ge = (G < T) ? G : T;   // ranks taking part: every range is non-empty
+    const TallGrid tg = tall_grid(n, j + 1);
+    const int64_t Ge = (G < tg.T) ? G : tg.T;
     if (g >= Ge) return;
-    const int64_t tlo = ((int64_t)g * T) / Ge, thi = ((int64_t)(g + 1) * T) / Ge;
-    int64_t st = strip_of(tlo);
-    int64_t I = J0 + st, J = J0 + (tlo - st * (st + 1) / 2);
-    const int64_t slot = g - tile_owner(st * (st + 1) / 2, T, Ge);
-    int64_t cur_slot = slot;
-    int64_t rows[4];
-    strip_rows(I, p, rows);
-    double vr[4], racc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int64_t tlo = ((int64_t)g * tg.T) / Ge, thi = ((int64_t)(g + 1) * tg.T) / Ge;
+    int64_t I = tall_strip_of(tg, tlo);
+    int64_t J = tg.Jlo + (tlo - tall_first(tg, I));
+    int64_t cur_slot = g - tile_owner(tall_first(tg, I), tg.T, Ge);
+    double vr[16], racc[16];
 #pragma unroll
-    for (int rr = 0; rr < 4; rr++) vr[rr] = vval(A, lda, n, j, rows[rr], s);
-    double a[4][4], vc[4];
-    load_tile(A, lda, n, j, J, wave, cg, rows, s, a, vc);
+    for (int k = 0; k < 16; k++) {
+        vr[k] = vval(A, lda, n, j, I * MR + 32 * (k >> 1) + 2 * p + (k & 1), s);
+        racc[k] = 0.0;
+    }
+    double a[16];
+    int64_t c = J * MC + 4 * wave + cg;
+    double vc = vval(A, lda, n, j, c, s);
+    tall_load(A, lda, n, j, I, c, p, a);
     for (int64_t t = tlo; t < thi; t++) {
         const bool more = t + 1 < thi;
-        const int64_t In = (J < I) ? I : I + 1;
-        const int64_t Jn = (J < I) ? J + 1 : J0;
-        int64_t rows_n[4];
-        strip_rows(In, p, rows_n);
-        double an[4][4], vcn[4];
-        if (more) load_tile(A, lda, n, j, Jn, wave, cg, rows_n, s, an, vcn);
-        const int64_t cb = J * TB + 16 * wave + 4 * cg;
-        double cp[4];
-#pragma unroll
-        for (int cc = 0; cc < 4; cc++) {
-            cp[cc] = 0.0;
-#pragma unroll
-            for (int rr = 0; rr < 4; rr++) {
-                racc[rr] += a[rr][cc] * vc[cc];
-                cp[cc] += a[rr][cc] * vr[rr];
-            }
+        const bool last_in_strip = (J == MPS * I + MPS - 1);
+        const int64_t In = last_in_strip ? I + 1 : I;
+        const int64_t Jn = last_in_strip ? tg.Jlo : J + 1;
+        const int64_t cn = Jn * MC + 4 * wave + cg;
+        double an[16], vcn = 0.0;
+        if (more) {
+            vcn = vval(A, lda, n, j, cn, s);
+            tall_load(A, lda, n, j, In, cn, p, an);
         }
+        double cp = 0.0;
 #pragma unroll
-        for (int cc = 0; cc < 4; cc++) {
-            double x = cp[cc];
-            x += __shfl_xor(x, 1, 64);
-            x += __shfl_xor(x, 2, 64);
-            x += __shfl_xor(x, 4, 64);
-            x += __shfl_xor(x, 8, 64);
-            cp[cc] = x;
+        for (int k = 0; k < 16; k++) {
+            racc[k] += a[k] * vc;
+            cp += a[k] * vr[k];
         }
-        if (p == 0) {
-#pragma unroll
-            for (int cc = 0; cc < 4; cc++) {
-                const int64_t c = cb + cc;
-                if (c > j && c < n) w.Pcol[I * w.ldp + c] = -cp[cc];
-            }
-        }
+        cp += __shfl_xor(cp, 1, 64);
+        cp += __shfl_xor(cp, 2, 64);
+        cp += __shfl_xor(cp, 4, 64);
+        cp += __shfl_xor(cp, 8, 64);
+        if (p == 0 && c > j && c < n) w.Pcol[I * w.ldp + c] = -cp;
         if (!more || In != I) {
-            // flush this rank's row sums of strip I
+            // flush this rank's row sums of strip I: lanes -> waves -> one slot row
 #pragma unroll
-            for (int rr = 0; rr < 4; rr++) {
-                double x = racc[rr];
+            for (int k = 0; k < 16; k++) {
+                double x = racc[k];
                 x += __shfl_xor(x, 16, 64);
                 x += __shfl_xor(x, 32, 64);
-                racc[rr] = x;
-            }
-            if (cg == 0) {
-                sh.red[wave][2 * p] = racc[0];
-                sh.red[wave][2 * p + 1] = racc[1];
-                sh.red[wave][32 + 2 * p] = racc[2];
-                sh.red[wave][33 + 2 * p] = racc[3];
+                if (cg == 0) sh.mred[wave][32 * (k >> 1) + 2 * p + (k & 1)] = x;
+                racc[k] = 0.0;
             }
             __syncthreads();
-            if (wave == 0) {
-                const int64_t r = I * TB + lane;
+            {
+                const int64_t r = I * MR + tid;
                 if (r < n)
-                    w.Prow[cur_slot * w.ldp + r] = sh.red[0][lane] + sh.red[1][lane] + sh.red[2][lane] + sh.red[3][lane];
+                    w.Prow[cur_slot * w.ldp + r] = (sh.mred[0][tid] + sh.mred[1][tid]) + (sh.mred[2][tid] + sh.mred[3][tid]);
             }
             __syncthreads();
-#pragma unroll
-            for (int rr = 0; rr < 4; rr++) racc[rr] = 0.0;
             cur_slot = 0;   // this rank owns the first tile of every later strip it touches
             if (more) {
 #pragma unroll
-                for (int rr = 0; rr < 4; rr++) {
-                    rows[rr] = rows_n[rr];
-                    vr[rr] = vval(A, lda, n, j, rows[rr], s);
-                }
+                for (int k = 0; k < 16; k++) vr[k] = vval(A, lda, n, j, In * MR + 32 * (k >> 1) + 2 * p + (k & 1), s);
             }
         }
         if (more) {
 #pragma unroll
-            for (int rr = 0; rr < 4; rr++)
-#pragma unroll
-                for (int cc = 0; cc < 4; cc++) a[rr][cc] = an[rr][cc];
-#pragma unroll
-            for (int cc = 0; cc < 4; cc++) vc[cc] = vcn[cc];
+            for (int k = 0; k < 16; k++) a[k] = an[k];
+            vc = vcn;
         }
         I = In;
         J = Jn;
+        c = cn;
     }
 }
 
