@@ -361,61 +361,22 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
 }
 
 // ---------------------------------------------------------------------------------------
-// Phase B scalars: reduce the norm partials of column j, form the reflector scalars.
-// Every rank computes them with the same fixed order (wave 0, strided lanes, then a
-// butterfly), so all ranks hold bit-identical tau/scale.
-// ---------------------------------------------------------------------------------------
-__device__ void column_scalars(const double* __restrict__ A, int64_t lda, int64_t n, int64_t j,
-                               const Work& w, RowShared& sh, double* tau_out, double* eta_out,
-                               bool writer)
-{
-    const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
-    if (wave == 0) {
-        const int64_t nq = cdiv(n, TB) - j / TB;
-        double acc = 0.0;
-        for (int64_t q = lane; q < nq; q += 64) acc += w.Pnorm[q];
-        acc = wave_sum(acc);
-        if (lane == 0) {
-            ColScal s = reflector_scalars(A[(j + 1) + j * lda], acc);
-            sh.cur = s;
-            if (writer) {
-                tau_out[j] = s.tau;
-                eta_out[j] = s.beta;
-            }
-        }
-    }
-    __syncthreads();
-}
-
-// t1 = W^T v, t2 = V^T v for column j from the per-chunk partials (i entries each).
-__device__ void reduce_t(const double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j,
-                         int i, const Work& w, const RowShared& sh, int g, int G)
-{
-    const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
-    const int64_t nq = cdiv(n, TB) - j / TB;
-    const ColScal s = sh.cur;
-    for (int tv = g * 4 + wave; tv < 2 * i; tv += G * 4) {
-        const int k = tv % i;
-        const int off = (tv < i) ? 0 : NBMAX;
-        double acc = 0.0;
-        for (int64_t q = lane; q < nq; q += 64) acc += w.PS[q * (2 * NBMAX) + off + k];
-        acc = wave_sum(acc);
-        if (lane == 0) {
-            double head = (tv < i) ? w.W[(j + 1) + k * w.ldw] : A[(j + 1) + (c0 + k) * lda];
-            w.Tv[off + k] = s.unit * head + s.scale * acc;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------------------
-// Skew matvec partials of column j over the strictly lower triangle of A[j+1:n, j+1:n] on
-// MR x MC tiles (strip-major), split into G equal contiguous tile ranges, one per rank.
-// Per tile: wave w -> columns 4w..4w+3 (lane >> 4), lane & 15 = p -> row pairs
-// {32 h + 2p, 32 h + 2p + 1}, h = 0..7 (16 entries per lane, read as 16-byte pairs); the next
-// tile's loads are in flight while the current one is reduced.
-//   Pcol[I][c]    = -sum_{r in strip I} A[r,c] v_r                     (one entry per tile)
-//   Prow[slot][r] =  sum over the rank's tiles of strip I of A[r,c] v_c,
-//                    slot = rank - (owner of the strip's first tile)
+// Phase B of column j, after the first grid barrier of the column:
+//   (1) the reflector scalars beta, tau, 1/(alpha - beta) from the chunk norm partials, formed by
+//       every rank in the same fixed order (wave 0, strided lanes, a butterfly), so all ranks hold
+//       bit-identical values;
+//   (2) t1 = W^T v, t2 = V^T v of column j from the per-chunk partials (i entries each, spread
+//       over the ranks' waves);
+//   (3) the skew matvec partials of column j over the strictly lower triangle of
+//       A[j+1:n, j+1:n] on MR x MC tiles (strip-major), split into G equal contiguous tile ranges,
+//       one per rank.  Per tile: wave w -> columns 4w..4w+3 (lane >> 4), lane & 15 = p -> row
+//       pairs {32 h + 2p, 32 h + 2p + 1}, h = 0..7 (16 entries per lane, read as 16-byte pairs);
+//       the next tile's loads are in flight while the current one is reduced.
+//         Pcol[I][c]    = -sum_{r in strip I} A[r,c] v_r                     (one entry per tile)
+//         Prow[slot][r] =  sum over the rank's tiles of strip I of A[r,c] v_c,
+//                          slot = rank - (owner of the strip's first tile)
+// The loads of (1), (2) and of the first tile with its v entries (unscaled x) are issued
+// together, so the phase starts with one memory round trip; v = scale * x is applied after.
 // ---------------------------------------------------------------------------------------
 __device__ __forceinline__ void tall_load(const double* __restrict__ A, int64_t lda, int64_t n, int64_t j,
                                           int64_t I, int64_t c, int p, double a[16])
@@ -438,29 +399,100 @@ __device__ __forceinline__ void tall_load(const double* __restrict__ A, int64_t 
     }
 }
 
-__device__ void skew_matvec(const double* __restrict__ A, int64_t lda, int64_t n, int64_t j,
-                            const Work& w, RowShared& sh, int g, int G)
+// unscaled x_c of column j where v_c = scale * x_c (0 elsewhere)
+__device__ __forceinline__ double xraw(const double* __restrict__ A, int64_t lda, int64_t n, int64_t j, int64_t c)
+{
+    return (c > j + 1 && c < n) ? A[c + j * lda] : 0.0;
+}
+
+// v_c from its unscaled x_c
+__device__ __forceinline__ double vfrom(int64_t n, int64_t j, int64_t c, double x, const ColScal& s)
+{
+    if (c <= j || c >= n) return 0.0;
+    return (c == j + 1) ? s.unit : s.scale * x;
+}
+
+__device__ void phase_b(const double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j, int i,
+                        const Work& w, RowShared& sh, double* tau_out, double* eta_out, int g, int G)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
     const int p = lane & 15, cg = lane >> 4;
-    const ColScal s = sh.cur;
+    const int64_t nq = cdiv(n, TB) - j / TB;
+    // ---- loads ----
+    double pn0 = 0.0, pn1 = 0.0, araw = 0.0;
+    if (wave == 0) {
+        pn0 = (lane < nq) ? w.Pnorm[lane] : 0.0;
+        pn1 = (lane + 64 < nq) ? w.Pnorm[lane + 64] : 0.0;
+        if (lane == 0) araw = A[(j + 1) + j * lda];
+    }
+    const int tv0 = g * 4 + wave;
+    double ps0 = 0.0, ps1 = 0.0, head0 = 0.0;
+    if (tv0 < 2 * i) {
+        const int k = tv0 % i;
+        const int off = (tv0 < i) ? 0 : NBMAX;
+        ps0 = (lane < nq) ? w.PS[(int64_t)lane * (2 * NBMAX) + off + k] : 0.0;
+        ps1 = (lane + 64 < nq) ? w.PS[(int64_t)(lane + 64) * (2 * NBMAX) + off + k] : 0.0;
+        if (lane == 0) head0 = (tv0 < i) ? w.W[(j + 1) + k * w.ldw] : A[(j + 1) + (c0 + k) * lda];
+    }
     const TallGrid tg = tall_grid(n, j + 1);
     const int64_t Ge = (G < tg.T) ? G : tg.T;
-    if (g >= Ge) return;
-    const int64_t tlo = ((int64_t)g * tg.T) / Ge, thi = ((int64_t)(g + 1) * tg.T) / Ge;
-    int64_t I = tall_strip_of(tg, tlo);
-    int64_t J = tg.Jlo + (tlo - tall_first(tg, I));
-    int64_t cur_slot = g - tile_owner(tall_first(tg, I), tg.T, Ge);
+    const bool mv = g < Ge;
+    int64_t tlo = 0, thi = 0, I = 0, J = 0, cur_slot = 0, c = 0;
+    double a[16], xr[16], xc = 0.0;
+    if (mv) {
+        tlo = ((int64_t)g * tg.T) / Ge;
+        thi = ((int64_t)(g + 1) * tg.T) / Ge;
+        I = tall_strip_of(tg, tlo);
+        J = tg.Jlo + (tlo - tall_first(tg, I));
+        cur_slot = g - tile_owner(tall_first(tg, I), tg.T, Ge);
+        c = J * MC + 4 * wave + cg;
+        tall_load(A, lda, n, j, I, c, p, a);
+        xc = xraw(A, lda, n, j, c);
+#pragma unroll
+        for (int k = 0; k < 16; k++) xr[k] = xraw(A, lda, n, j, I * MR + 32 * (k >> 1) + 2 * p + (k & 1));
+    }
+    // ---- (1) scalars ----
+    if (wave == 0) {
+        double acc = pn0 + pn1;
+        for (int64_t q = lane + 128; q < nq; q += 64) acc += w.Pnorm[q];
+        acc = wave_sum(acc);
+        if (lane == 0) {
+            const ColScal s = reflector_scalars(araw, acc);
+            sh.cur = s;
+            if (g == 0) {
+                tau_out[j] = s.tau;
+                eta_out[j] = s.beta;
+            }
+        }
+    }
+    __syncthreads();
+    const ColScal s = sh.cur;
+    // ---- (2) t1, t2 ----
+    for (int tv = tv0; tv < 2 * i; tv += G * 4) {
+        const int k = tv % i;
+        const int off = (tv < i) ? 0 : NBMAX;
+        double acc, head;
+        if (tv == tv0) {
+            acc = ps0 + ps1;
+            head = head0;
+        } else {
+            acc = 0.0;
+            for (int64_t q = lane; q < 128 && q < nq; q += 64) acc += w.PS[q * (2 * NBMAX) + off + k];
+            head = (lane == 0) ? ((tv < i) ? w.W[(j + 1) + k * w.ldw] : A[(j + 1) + (c0 + k) * lda]) : 0.0;
+        }
+        for (int64_t q = lane + 128; q < nq; q += 64) acc += w.PS[q * (2 * NBMAX) + off + k];
+        acc = wave_sum(acc);
+        if (lane == 0) w.Tv[off + k] = s.unit * head + s.scale * acc;
+    }
+    // ---- (3) matvec ----
+    if (!mv) return;
     double vr[16], racc[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-        vr[k] = vval(A, lda, n, j, I * MR + 32 * (k >> 1) + 2 * p + (k & 1), s);
+        vr[k] = vfrom(n, j, I * MR + 32 * (k >> 1) + 2 * p + (k & 1), xr[k], s);
         racc[k] = 0.0;
     }
-    double a[16];
-    int64_t c = J * MC + 4 * wave + cg;
-    double vc = vval(A, lda, n, j, c, s);
-    tall_load(A, lda, n, j, I, c, p, a);
+    double vc = vfrom(n, j, c, xc, s);
     for (int64_t t = tlo; t < thi; t++) {
         const bool more = t + 1 < thi;
         const bool last_in_strip = (J == MPS * I + MPS - 1);

@@ -39,9 +39,7 @@ __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int
         if (clock) { unsigned long long t = wall_clock64(); acc[0] += t - t0; t0 = t; }
         if (!grid_sync(bar, ++epoch, err, &s_ok)) return;
         if (clock) { unsigned long long t = wall_clock64(); acc[1] += t - t0; t0 = t; }
-        column_scalars(A, lda, n, j, w, sh, tau, eta, g == 0);
-        reduce_t(A, lda, n, c0, j, i, w, sh, g, G);
-        skew_matvec(A, lda, n, j, w, sh, g, G);
+        phase_b(A, lda, n, c0, j, i, w, sh, tau, eta, g, G);
         if (clock) { unsigned long long t = wall_clock64(); acc[2] += t - t0; t0 = t; }
         if (!grid_sync(bar, ++epoch, err, &s_ok)) return;
         if (clock) { unsigned long long t = wall_clock64(); acc[3] += t - t0; t0 = t; }
