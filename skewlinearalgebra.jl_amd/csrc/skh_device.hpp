@@ -179,6 +179,7 @@ struct RowShared {
     double red[4][TB];
     double red2[4][TB];
     double mred[4][MR];   // matvec row sums of one strip, per wave
+    double vstrip[MR];    // v on the rows of the matvec strip being processed
     double wspec;
     ColScal cur;     // scalars of the column finalised by the last phase B
 };
@@ -438,7 +439,7 @@ __device__ void phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
     const int64_t Ge = (G < tg.T) ? G : tg.T;
     const bool mv = g < Ge;
     int64_t tlo = 0, thi = 0, I = 0, J = 0, cur_slot = 0, c = 0;
-    double a[16], xr[16], xc = 0.0;
+    double a[16], xs = 0.0, xc = 0.0;
     if (mv) {
         tlo = ((int64_t)g * tg.T) / Ge;
         thi = ((int64_t)(g + 1) * tg.T) / Ge;
@@ -448,8 +449,7 @@ __device__ void phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         c = J * MC + 4 * wave + cg;
         tall_load(A, lda, n, j, I, c, p, a);
         xc = xraw(A, lda, n, j, c);
-#pragma unroll
-        for (int k = 0; k < 16; k++) xr[k] = xraw(A, lda, n, j, I * MR + 32 * (k >> 1) + 2 * p + (k & 1));
+        xs = xraw(A, lda, n, j, I * MR + tid);   // this thread's row of the strip
     }
     // ---- (1) scalars ----
     if (wave == 0) {
@@ -467,6 +467,7 @@ __device__ void phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
     }
     __syncthreads();
     const ColScal s = sh.cur;
+    if (mv) sh.vstrip[tid] = vfrom(n, j, I * MR + tid, xs, s);
     // ---- (2) t1, t2 ----
     for (int tv = tv0; tv < 2 * i; tv += G * 4) {
         const int k = tv % i;
@@ -486,12 +487,10 @@ __device__ void phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
     }
     // ---- (3) matvec ----
     if (!mv) return;
-    double vr[16], racc[16];
+    __syncthreads();   // v of the first strip is in LDS
+    double racc[16];
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        vr[k] = vfrom(n, j, I * MR + 32 * (k >> 1) + 2 * p + (k & 1), xr[k], s);
-        racc[k] = 0.0;
-    }
+    for (int k = 0; k < 16; k++) racc[k] = 0.0;
     double vc = vfrom(n, j, c, xc, s);
     for (int64_t t = tlo; t < thi; t++) {
         const bool more = t + 1 < thi;
@@ -499,16 +498,17 @@ __device__ void phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         const int64_t In = last_in_strip ? I + 1 : I;
         const int64_t Jn = last_in_strip ? tg.Jlo : J + 1;
         const int64_t cn = Jn * MC + 4 * wave + cg;
-        double an[16], vcn = 0.0;
+        double an[16], vcn = 0.0, xsn = 0.0;
         if (more) {
             vcn = vval(A, lda, n, j, cn, s);
             tall_load(A, lda, n, j, In, cn, p, an);
+            if (In != I) xsn = xraw(A, lda, n, j, In * MR + tid);
         }
         double cp = 0.0;
 #pragma unroll
         for (int k = 0; k < 16; k++) {
             racc[k] += a[k] * vc;
-            cp += a[k] * vr[k];
+            cp += a[k] * sh.vstrip[32 * (k >> 1) + 2 * p + (k & 1)];
         }
         cp += __shfl_xor(cp, 1, 64);
         cp += __shfl_xor(cp, 2, 64);
@@ -525,18 +525,15 @@ __device__ void phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
                 if (cg == 0) sh.mred[wave][32 * (k >> 1) + 2 * p + (k & 1)] = x;
                 racc[k] = 0.0;
             }
-            __syncthreads();
+            __syncthreads();   // also: every wave is done reading vstrip of strip I
             {
                 const int64_t r = I * MR + tid;
                 if (r < n)
                     w.Prow[cur_slot * w.ldp + r] = (sh.mred[0][tid] + sh.mred[1][tid]) + (sh.mred[2][tid] + sh.mred[3][tid]);
+                if (more) sh.vstrip[tid] = vfrom(n, j, In * MR + tid, xsn, s);
             }
             __syncthreads();
             cur_slot = 0;   // this rank owns the first tile of every later strip it touches
-            if (more) {
-#pragma unroll
-                for (int k = 0; k < 16; k++) vr[k] = vval(A, lda, n, j, In * MR + 32 * (k >> 1) + 2 * p + (k & 1), s);
-            }
         }
         if (more) {
 #pragma unroll
