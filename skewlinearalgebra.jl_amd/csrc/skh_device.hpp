@@ -49,6 +49,28 @@ __device__ __forceinline__ double wave_sum(double x)
     return x;
 }
 
+// Sum each of the 32 values v[0..31] over the 64 lanes of the wave with 32 shuffles (halving
+// exchange): at the step on lane bit b, a lane keeps the upper or lower half of its values by
+// that bit and receives the partner's copy of it.  Afterwards lane l holds the total of value
+// (l >> 1) & 31 (lanes l and l ^ 1 hold the same total).
+__device__ __forceinline__ double wave_sum32(double v[32])
+{
+#pragma unroll
+    for (int step = 0; step < 5; step++) {
+        const int b = 32 >> step;           // lane bit exchanged at this step
+        const int half = 16 >> step;        // values kept after it
+        const bool hi = (threadIdx.x & b) != 0;
+#pragma unroll
+        for (int i = 0; i < half; i++) {
+            const double lo_v = v[i], hi_v = v[i + half];
+            const double keep = hi ? hi_v : lo_v;
+            const double send = hi ? lo_v : hi_v;
+            v[i] = keep + __shfl_xor(send, b, 64);
+        }
+    }
+    return v[0] + __shfl_xor(v[0], 1, 64);
+}
+
 __device__ __forceinline__ int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // index of this thread's wave in the workgroup, as a wave-uniform (scalar) value
@@ -345,14 +367,29 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
             }
             // S partials of column j: sum over the chunk's rows r >= j+2 of W[r,k] x_r, V[r,k] x_r
             const double xt = tail ? x : 0.0;
+            if (KQ <= 16) {
+                // values 0..15: W[r, k0+u] x_r, 16..31: V[r, k0+u] x_r (zero past the wave's group)
+                double v32[32];
 #pragma unroll
-            for (int u = 0; u < KQ; u++) {
-                if (k0 + u < k1) {   // wave-uniform
-                    const double s1 = wave_sum(wreg[u] * xt);
-                    const double s2 = wave_sum(vreg[u] * xt);
-                    if (lane == 0) {
-                        w.PS[q * (2 * NBMAX) + k0 + u] = s1;
-                        w.PS[q * (2 * NBMAX) + NBMAX + k0 + u] = s2;
+                for (int u = 0; u < 16; u++) {
+                    v32[u] = (u < KQ && k0 + u < k1) ? wreg[u < KQ ? u : 0] * xt : 0.0;
+                    v32[16 + u] = (u < KQ && k0 + u < k1) ? vreg[u < KQ ? u : 0] * xt : 0.0;
+                }
+                const double tot = wave_sum32(v32);
+                const int idx = (lane >> 1) & 31;
+                const int u = idx & 15;
+                if ((lane & 1) == 0 && k0 + u < k1)
+                    w.PS[q * (2 * NBMAX) + (idx < 16 ? 0 : NBMAX) + k0 + u] = tot;
+            } else {
+#pragma unroll
+                for (int u = 0; u < KQ; u++) {
+                    if (k0 + u < k1) {   // wave-uniform
+                        const double s1 = wave_sum(wreg[u] * xt);
+                        const double s2 = wave_sum(vreg[u] * xt);
+                        if (lane == 0) {
+                            w.PS[q * (2 * NBMAX) + k0 + u] = s1;
+                            w.PS[q * (2 * NBMAX) + NBMAX + k0 + u] = s2;
+                        }
                     }
                 }
             }
