@@ -233,7 +233,9 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
         // workgroups make the two grid barriers per column cheaper once the matrix is small)
         {
             const int64_t tiles = skh::tall_grid(n, c0 + 1).T;
-            const int64_t want = std::max<int64_t>(8, (tiles + tpw - 1) / tpw);
+            // at least one workgroup per 64-row block of the panel (each keeps its block's rows)
+            const int64_t blocks = (n + skh::TB - 1) / skh::TB - c0 / skh::TB;
+            const int64_t want = std::max<int64_t>(std::max<int64_t>(8, blocks), (tiles + tpw - 1) / tpw);
             pl.grid = (int)std::min<int64_t>(grid, want);
         }
         pl.nbmax = (int)nb;

@@ -198,23 +198,39 @@ struct RowShared {
     double wrow[NBMAX];
     double vrow[NBMAX];
     double t[2 * NBMAX];
-    double red[4][TB];
-    double red2[4][TB];
-    double mred[4][MR];   // matvec row sums of one strip, per wave
+    union {
+        struct {
+            double red[4][TB];    // row pass reductions
+            double red2[4][TB];
+        };
+        double mred[4][MR];       // matvec row sums of one strip, per wave (other phase)
+    };
     double vstrip[MR];    // v on the rows of the matvec strip being processed
     double wspec;
     ColScal cur;     // scalars of the column finalised by the last phase B
 };
 
-// KQ = largest k-group per wave (panel width <= 4*KQ); the rows of V and W used by a lane
-// are cached in 2*KQ registers, shared by the C part (correction of w_{j-1}, k < i-1) and the
-// A part (update of column j, k < i).  Every global load a chunk needs (row and column partials
-// of the matvec, its rows of V and W, the old entries of columns j-1 and j) and the shared
-// inputs (t, row j of V and W, the special row's partials) are issued together, before the
-// first barrier, so a row pass costs one memory round trip.
+// Row data a workgroup keeps for the whole panel: its own 64-row block (absolute block
+// c0/64 + g) of V and W in LDS (wave w holds k in [w*KQ, (w+1)*KQ)), and in registers the last x
+// it computed (column j-1, needed as the unscaled reflector of the next C part) and the
+// prefetched column j+1 (the next A part's input).  Every new V/W column entry of the block is
+// computed by this workgroup, so the block's rows never have to be reloaded from memory.
+template <int KQ>
+struct RowRegs {
+    double* v;   // LDS [4 * KQ][64]: (wave * KQ + u) * 64 + lane
+    double* w;
+    double xprev, xnext;
+};
+
+// KQ = k-group per wave (panel width <= 4*KQ).  The rows of V and W used by a lane are held in
+// 2*KQ registers, shared by the C part (correction of w_{j-1}, k < i-1) and the A part (update of
+// column j, k < i).  Every global load a chunk needs (row and column partials of the matvec;
+// for blocks other than the workgroup's own, also its rows of V and W and the old entries of
+// columns j-1 and j) and the shared inputs (t, row j of V and W, the special row's partials) are
+// issued together, before the first barrier, so a row pass costs one memory round trip.
 template <int KQ>
 __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j, int i,
-                         bool doC, bool doA, const Work& w, int g, int G, RowShared& sh)
+                         bool doC, bool doA, const Work& w, int g, int G, RowShared& sh, RowRegs<KQ>& rr)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
     const int64_t jm = j - 1;         // column finalised by the C part
@@ -222,18 +238,21 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
     const ColScal cs = sh.cur;        // scalars of column jm (valid when doC)
     double* __restrict__ Wm = w.W;
     const int64_t ldw = w.ldw;
-    const int64_t b0 = j / TB;        // first chunk block
+    const int64_t b0 = j / TB;        // block of row j
     const int64_t nt = cdiv(n, TB);
-    const int64_t nq = nt - b0;
-    const int kq = (i + 3) / 4;       // k-group size per wave
-    const int k0 = wave * kq, k1 = min(i, k0 + kq);
+    const int64_t bown = c0 / TB + g; // this workgroup's own block
+    const int k0 = wave * KQ, k1 = min(i, k0 + KQ);
     const int kc = doC ? min(k1, im) : k0;   // C correction uses k in [k0, kc)
     constexpr int PB = 16;            // column partials per lane per batch (4 waves: 64 strips)
     const TallGrid tg = tall_grid(n, j);     // tiles of the matvec that produced the partials
+    bool shared_done = false;
 
-    for (int64_t q = g; q < nq; q += G) {
-        const bool first = (q == g);
-        const int64_t b = b0 + q;
+    for (int64_t b = bown; b < nt; b += G) {
+        if (b < b0) continue;         // block entirely above row j (wave-uniform)
+        const bool first = !shared_done;
+        shared_done = true;
+        const bool own = (b == bown);
+        const int64_t q = b - b0;     // chunk index relative to row j's block
         const int64_t r = b * TB + lane;
         const bool rin = r < n;
         const bool rowC = doC && rin && r >= j;
@@ -260,16 +279,26 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                 for (int64_t I = S0 + lane + 256; I < tg.nts; I += 64) sprow += w.Pcol[I * w.ldp + j];
             }
         }
-        double vreg[KQ], wreg[KQ];
+        double vreg[KQ], wreg[KQ], xo, xold;
+        if (own) {
 #pragma unroll
-        for (int u = 0; u < KQ; u++) {
-            const int k = k0 + u;
-            const bool ok = rin && r >= j && k < k1 && !(doC && k == im);
-            vreg[u] = ok ? A[r + (c0 + k) * lda] : 0.0;
-            wreg[u] = ok ? Wm[r + k * ldw] : 0.0;
+            for (int u = 0; u < KQ; u++) {
+                vreg[u] = rr.v[(wave * KQ + u) * 64 + lane];
+                wreg[u] = rr.w[(wave * KQ + u) * 64 + lane];
+            }
+            xo = rr.xprev;
+            xold = rr.xnext;
+        } else {
+#pragma unroll
+            for (int u = 0; u < KQ; u++) {
+                const int k = k0 + u;
+                const bool ok = rin && r >= j && k < k1 && !(doC && k == im);
+                vreg[u] = ok ? A[r + (c0 + k) * lda] : 0.0;
+                wreg[u] = ok ? Wm[r + k * ldw] : 0.0;
+            }
+            xo = rowC ? A[r + jm * lda] : 0.0;               // x of column j-1 (unscaled)
+            xold = (doA && rin && r > j) ? A[r + j * lda] : 0.0;
         }
-        const double xo = rowC ? A[r + jm * lda] : 0.0;               // x of column j-1 (unscaled)
-        const double xold = (doA && rin && r > j) ? A[r + j * lda] : 0.0;
         double pp = 0.0;
         if (rowC) {
             const int64_t S = (b * TB) / MR;    // matvec strip holding this chunk
@@ -339,6 +368,17 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                 A[r + jm * lda] = vr;
             }
         }
+        if (doC) {
+            // the block's rows of the new V/W column (zero above row j)
+#pragma unroll
+            for (int u = 0; u < KQ; u++) {
+                if (k0 + u == im) {
+                    vreg[u] = vr;
+                    wreg[u] = wr;
+                }
+            }
+        }
+        double x = 0.0;
         if (doA) {
             // ---- A part: update column j with the panel's reflectors 0..i-1 ----
             const bool live = rin && r > j;
@@ -346,18 +386,10 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
 #pragma unroll
             for (int u = 0; u < KQ; u++) {
                 const int k = k0 + u;
-                if (doC && k == im) {
-                    vreg[u] = live ? vr : 0.0;
-                    wreg[u] = live ? wr : 0.0;
-                } else if (!live) {
-                    vreg[u] = 0.0;
-                    wreg[u] = 0.0;
-                }
                 if (live && k < k1) p4[u & 3] += vreg[u] * sh.wrow[k] - wreg[u] * sh.vrow[k];
             }
             sh.red2[wave][lane] = (p4[0] + p4[1]) + (p4[2] + p4[3]);
             __syncthreads();
-            double x = 0.0;
             if (live) x = xold + (sh.red2[0][lane] + sh.red2[1][lane] + sh.red2[2][lane] + sh.red2[3][lane]);
             if (i > 0 && live && wave == 0) A[r + j * lda] = x;
             const bool tail = rin && r >= j + 2;
@@ -393,6 +425,20 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                     }
                 }
             }
+        }
+        if (own) {
+            // keep the block's rows for the next column of the panel; prefetch its input column
+            if (doC) {   // only column im is new
+#pragma unroll
+                for (int u = 0; u < KQ; u++) {
+                    if (k0 + u == im) {
+                        rr.v[(wave * KQ + u) * 64 + lane] = vreg[u];
+                        rr.w[(wave * KQ + u) * 64 + lane] = wreg[u];
+                    }
+                }
+            }
+            rr.xprev = x;
+            rr.xnext = (doA && rin && r > j + 1 && j + 1 < n) ? A[r + (j + 1) * lda] : 0.0;
         }
         __syncthreads();
     }

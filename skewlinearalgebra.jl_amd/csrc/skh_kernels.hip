@@ -33,7 +33,20 @@ __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int
     const bool clock = (ph != nullptr) && g == 0 && threadIdx.x == 0;
     unsigned long long t0 = clock ? wall_clock64() : 0, acc[4] = {0, 0, 0, 0};
     unsigned epoch = base;   // barriers completed before this launch
-    row_pass<KQ>(A, lda, n, c0, c0, 0, false, true, w, g, G, sh);
+    __shared__ double s_pv[4 * KQ * 64], s_pw[4 * KQ * 64];
+    RowRegs<KQ> rr;
+    rr.v = s_pv;
+    rr.w = s_pw;
+    for (int k = threadIdx.x; k < 4 * KQ * 64; k += WG) {
+        s_pv[k] = 0.0;
+        s_pw[k] = 0.0;
+    }
+    rr.xprev = 0.0;
+    {
+        const int64_t r = (c0 / TB + g) * TB + (threadIdx.x & 63);
+        rr.xnext = (r < n && r > c0) ? A[r + c0 * lda] : 0.0;
+    }
+    row_pass<KQ>(A, lda, n, c0, c0, 0, false, true, w, g, G, sh, rr);
     for (int i = 0; i < nbp; i++) {
         const int64_t j = c0 + i;
         if (clock) { unsigned long long t = wall_clock64(); acc[0] += t - t0; t0 = t; }
@@ -43,7 +56,7 @@ __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int
         if (clock) { unsigned long long t = wall_clock64(); acc[2] += t - t0; t0 = t; }
         if (!grid_sync(bar, ++epoch, err, &s_ok)) return;
         if (clock) { unsigned long long t = wall_clock64(); acc[3] += t - t0; t0 = t; }
-        row_pass<KQ>(A, lda, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, g, G, sh);
+        row_pass<KQ>(A, lda, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, g, G, sh, rr);
     }
     if (clock) {
         acc[0] += wall_clock64() - t0;
