@@ -207,11 +207,17 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
     if ((int64_t)64 * (int64_t)(2 * n + 4) > 4000000000LL)
         return set_err(SKH_ERR_UNSUPPORTED, "n = %lld too large for the barrier counter", (long long)n);
 
-    // target matvec tiles per workgroup (tuning knob SKH_PANEL_TPW; measured default)
+    // target matvec tiles per workgroup (tuning knob SKH_PANEL_TPW; measured default) and an
+    // optional cap on the grid (SKH_PANEL_GRID_CAP, for tests of workgroups owning several blocks)
     static const int64_t tpw = [] {
         const char* e = getenv("SKH_PANEL_TPW");
         const long v = e ? atol(e) : 0;
         return (int64_t)(v > 0 ? v : 8);
+    }();
+    const int64_t gcap = [] {
+        const char* e = getenv("SKH_PANEL_GRID_CAP");
+        const long v = e ? atol(e) : 0;
+        return (int64_t)(v > 0 ? v : 1 << 30);
     }();
     HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 512, st));
     unsigned base = 0;
@@ -236,7 +242,7 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
             // at least one workgroup per 64-row block of the panel (each keeps its block's rows)
             const int64_t blocks = (n + skh::TB - 1) / skh::TB - c0 / skh::TB;
             const int64_t want = std::max<int64_t>(std::max<int64_t>(8, blocks), (tiles + tpw - 1) / tpw);
-            pl.grid = (int)std::min<int64_t>(grid, want);
+            pl.grid = (int)std::min<int64_t>(std::min<int64_t>(grid, want), gcap);
         }
         pl.nbmax = (int)nb;
         pl.ph = g_timer.on ? ctl.ph : nullptr;

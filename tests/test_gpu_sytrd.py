@@ -7,6 +7,7 @@ Bars (written in each test):
   * log|Pf| relative error <= 1e-10 and the sign exactly equal
 """
 import ctypes
+import os
 import sys
 
 import numpy as np
@@ -528,3 +529,19 @@ def test_complex_eigvals_value_range():
     ref = -lam[(lam > -3.0) & (lam <= 2.0)]
     assert got.size == ref.size
     np.testing.assert_allclose(got, ref, atol=10 * n * EPS * np.linalg.norm(A, 2))
+
+
+@pytest.mark.parametrize("n,cap", [(1000, 4), (2049, 16)])
+def test_sytrd_small_grid_several_blocks(n, cap):
+    """A capped cooperative grid makes workgroups own several 64-row blocks (the load-based row
+    pass beside the LDS-resident one); the factorization must not change beyond rounding."""
+    say("sytrd small grid n=%d cap=%d" % (n, cap))
+    A = skew_gauss(n, 77)
+    os.environ["SKH_PANEL_GRID_CAP"] = str(cap)
+    try:
+        F, tau, eta, nz = gpu_sytrd(A, 64)
+    finally:
+        del os.environ["SKH_PANEL_GRID_CAP"]
+    _, _, etao = oracle.skewblockedhess(A)
+    np.testing.assert_allclose(eta, etao, rtol=0, atol=1e-11 * np.linalg.norm(A))
+    check_fact(A, F, tau, eta)
