@@ -263,7 +263,7 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
         base = 0;
         const int64_t e = c0 + nbp;
         if (tm) HIPCHK((hipError_t)g_timer.begin(st));
-        HIPCHK((hipError_t)skh_internal_launch_rank2k(dA, lda, w.W, w.ldw, n, e, e, c0, nbp, st));
+        HIPCHK((hipError_t)skh_internal_launch_rank2k(dA, lda, w.W, w.ldw, n, e, c0, nbp, st));
         if (tm) {
             const double m = (double)(n - e);
             HIPCHK((hipError_t)g_timer.end(st, 1, 2.0 * m * (m - 1.0) * nbp));

@@ -19,9 +19,6 @@ namespace skh {
 constexpr int WG = 256;       // threads per workgroup (4 waves of 64)
 constexpr int TB = 64;        // matvec tile edge and row-chunk height
 constexpr int NBMAX = 128;    // largest panel width
-constexpr int BT = 128;       // MFMA output tile edge of the rank-2k update
-constexpr int KC = 16;        // k-chunk staged in LDS by the rank-2k update
-constexpr int LDSLD = BT + 16;  // padded LDS row (doubles) for the staged operands
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -630,141 +627,182 @@ __device__ void phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
 }
 
 // ---------------------------------------------------------------------------------------
-// Skew rank-2k update of one BT x BT tile of the lower triangle on FP64 MFMA:
-//   A[r,c] += sum_k L[r,k] R[c,k]   for rows r in [rlo, n), cols c in [clo, n), r > c,
-//   L = [V U], R = [U -V], V[:,k] = A[:, c0+k] (reflectors, rows >= c0+k+1), U[:,k] = U[:, k].
-// v_mfma_f64_16x16x4f64 (measured on gfx950 by k_mfma_probe): a-operand lane l = (row l%16,
-// k l/16); b-operand lane l = (k l/16, col l%16); accumulator item q of lane l =
-// (row 4q + l/16, col l%16).  Each 16x16 block is computed transposed, D = R_blk L_blk^T, so
-// item q of lane l is C[r + l%16, c + 4q + l/16]: a load/store of one item is four 128-byte
-// column segments per wave.  The 4 waves tile the 128x128 block as 2x2 sub-blocks of 64x64.
+// Skew rank-2k trailing update on FP64 MFMA (src/hessenberg.jl:158-166):
+//   A[r,c] += sum_k L[r,k] R[c,k]   for r > c >= e,
+//   L = [V U], R = [U -V], V[:,k] = A[:, c0+k] (reflector k, its unit element in place),
+//   U[:,k] = W[:,k], k < nbp.
+// One wave owns a 32x64 output tile (rows r0.., cols q0..) and runs without LDS or
+// barriers: its operands stream from L2 straight into the MFMA operand registers, one k-step
+// ahead, through a uniform base pointer plus a 32-bit lane offset.  Independent waves drift
+// out of phase, so one wave's C traffic overlaps another's MFMAs (an LDS-staged 128x128
+// workgroup tile moves its C in lockstep and was 1.3-1.8x slower at n = 8192).
+// v_mfma_f64_16x16x4f64 (lane layout measured on gfx950 by k_mfma_probe): a-operand lane
+// l = (row l%16, k l/16); b-operand lane l = (k l/16, col l%16); accumulator item q of lane
+// l = (row 4q + l/16, col l%16).  Each 16x16 block is computed transposed, D = R_blk L_blk^T,
+// so item q of lane l is C[r + l%16, c + 4q + l/16]: one item of a wave is four 128-byte
+// column segments.
 // ---------------------------------------------------------------------------------------
-struct MfmaShared {
-    double L[KC * LDSLD];
-    double R[KC * LDSLD];
+constexpr int R2K_TR = 32;     // rows of a wave tile
+constexpr int R2K_TC = 64;     // columns of a wave tile (and the edge of a tile block)
+constexpr int R2K_SUPER = 16;  // tile blocks per supertile edge (the L2 reuse unit)
+
+// lane offsets (doubles) of the operand rows: V and W columns are at uniform offsets on top
+struct R2kRows {
+    uint32_t lv[2], lu[2];   // L rows r0 + 16 bi + lane%16, V and W layouts
+    uint32_t rv[4], ru[4];   // R rows q0 + 16 bj + lane%16
 };
 
-__device__ __forceinline__ void rank2k_tile(double* __restrict__ A, int64_t lda, const double* __restrict__ U,
-                                            int64_t ldu, int64_t n, int64_t rlo, int64_t clo, int64_t c0, int nbp,
-                                            int64_t r0, int64_t q0, MfmaShared& sm)
+// k-step s of 4: h = 0 gives L = V, R = W; h = 1 gives L = W, R = -V.  Lanes with
+// 4 ks + lane/16 >= nbp read nothing (zero operands pad nbp to a multiple of 4).
+__device__ __forceinline__ void r2k_operands(const double* __restrict__ A, int64_t lda, const double* __restrict__ U,
+                                             int64_t ldu, int64_t c0, int nbp, int nkh, int s, int lq,
+                                             const R2kRows& o, double lf[2], double rf[4])
 {
-    const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
-    const int wr = wave >> 1, wc = wave & 1;
-    // a wave whose whole 64x64 sub-block lies on or above the diagonal has nothing to do
-    const bool active = (r0 + wr * 64 + 63 > q0 + wc * 64);
-    const int lc = lane & 15, lq = lane >> 4;
-    const int64_t crow = r0 + wr * 64 + lc;        // row of this lane's items in block bi = 0
-    const int64_t ccol = q0 + wc * 64 + lq;        // column of item q = 0 in block bj = 0
-    v4d acc[4][4];
+    const int h = s >= nkh;
+    const int ks = s - h * nkh;
+    const bool ok = ks * 4 + lq < nbp;
+    const double* pv = A + (c0 + ks * 4) * lda;
+    const double* pu = U + (int64_t)ks * 4 * ldu;
+    if (h == 0) {
 #pragma unroll
-    for (int bi = 0; bi < 4; bi++) {
+        for (int b = 0; b < 2; b++) lf[b] = ok ? pv[o.lv[b]] : 0.0;
 #pragma unroll
-        for (int bj = 0; bj < 4; bj++) {
-            const int64_t row = crow + bi * 16;
-            v4d c = {0.0, 0.0, 0.0, 0.0};
-            if (active && row < n) {
+        for (int b = 0; b < 4; b++) rf[b] = ok ? pu[o.ru[b]] : 0.0;
+    } else {
+#pragma unroll
+        for (int b = 0; b < 2; b++) lf[b] = ok ? pu[o.lu[b]] : 0.0;
+#pragma unroll
+        for (int b = 0; b < 4; b++) rf[b] = ok ? -pv[o.rv[b]] : 0.0;
+    }
+}
+
+__device__ __forceinline__ void r2k_mfma(v4d acc[2][4], const double lf[2], const double rf[4])
+{
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++)
+            acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(rf[bj], lf[bi], acc[bi][bj], 0, 0, 0);
+}
+
+// One wave: rows [r0, r0+32) x cols [q0, q0+64) of the update, q0 <= r0, stored where
+// r > c >= e and r < n.  Operand rows past n are clamped to n-1 (their products land in
+// rows or columns that are not stored); rows and columns below e are computed and dropped.
+__device__ __forceinline__ void rank2k_wave(double* __restrict__ A, int64_t lda, const double* __restrict__ U,
+                                            int64_t ldu, int64_t n, int64_t e, int64_t c0, int nbp, int64_t r0,
+                                            int64_t q0)
+{
+    const int lane = threadIdx.x & 63, lc = lane & 15, lq = lane >> 4;
+    R2kRows o;
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+        const int64_t rl = min(r0 + b * 16 + lc, n - 1);
+        o.lv[b] = (uint32_t)(rl + lq * lda);
+        o.lu[b] = (uint32_t)(rl + lq * ldu);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const int64_t rr = min(q0 + b * 16 + lc, n - 1);
+        o.rv[b] = (uint32_t)(rr + lq * lda);
+        o.ru[b] = (uint32_t)(rr + lq * ldu);
+    }
+    // C tile: uniform base + lane offset; item (bi, bj, q) at bi*16 + (bj*16 + 4q)*lda
+    double* C = A + r0 + q0 * lda;
+    const uint32_t co = (uint32_t)(lc + lq * lda);
+    const bool inner = (q0 + R2K_TC - 1 < r0) && (r0 + R2K_TR - 1 < n) && (q0 >= e);
+    v4d acc[2][4];
+    if (inner) {
+#pragma unroll
+        for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+            for (int bj = 0; bj < 4; bj++) {
+                v4d c;
+#pragma unroll
+                for (int q = 0; q < 4; q++) c[q] = C[co + bi * 16 + (bj * 16 + 4 * q) * lda];
+                acc[bi][bj] = c;
+            }
+    } else {
+#pragma unroll
+        for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+            for (int bj = 0; bj < 4; bj++) {
+                const int64_t row = r0 + lc + bi * 16;
+                v4d c = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    const int64_t col = ccol + bj * 16 + 4 * q;
-                    if (col >= clo && col < n) c[q] = A[row + col * lda];
+                    const int64_t col = q0 + lq + bj * 16 + 4 * q;
+                    if (row < n && col >= e && col < row) c[q] = C[co + bi * 16 + (bj * 16 + 4 * q) * lda];
                 }
+                acc[bi][bj] = c;
             }
-            acc[bi][bj] = c;
-        }
     }
-    // staging: thread -> fixed row (tid & 127) of both operands, k = k0 + (tid >> 7) + 2u
-    const int srow = tid & (BT - 1), skk = tid >> 7;
-    const int64_t rowL = r0 + srow, rowR = q0 + srow;
-    const bool okL = rowL < n, okR = rowR < n;
-    const double* pVL = A + rowL + c0 * lda;   // V[rowL, k] = pVL[k*lda] when rowL > c0+k
-    const double* pVR = A + rowR + c0 * lda;
-    const double* pUL = U + rowL;
-    const double* pUR = U + rowR;
-    const int K = 2 * nbp;
-    for (int k0 = 0; k0 < K; k0 += KC) {
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < KC / 2; u++) {
-            const int kk = skk + 2 * u;
-            const int k = k0 + kk;
-            double lv = 0.0, rv = 0.0;
-            if (k < nbp) {
-                if (okL && rowL > c0 + k) lv = pVL[(int64_t)k * lda];
-                if (okR) rv = pUR[(int64_t)k * ldu];
-            } else if (k < K) {
-                const int kv = k - nbp;
-                if (okL) lv = pUL[(int64_t)kv * ldu];
-                if (okR && rowR > c0 + kv) rv = -pVR[(int64_t)kv * lda];
-            }
-            sm.L[kk * LDSLD + srow] = lv;
-            sm.R[kk * LDSLD + srow] = rv;
-        }
-        __syncthreads();
-        if (active) {
-#pragma unroll
-            for (int ks = 0; ks < KC / 4; ks++) {
-                const int kk = ks * 4 + lq;
-                double lf[4], rf[4];
-#pragma unroll
-                for (int b = 0; b < 4; b++) {
-                    lf[b] = sm.L[kk * LDSLD + wr * 64 + b * 16 + lc];
-                    rf[b] = sm.R[kk * LDSLD + wc * 64 + b * 16 + lc];
-                }
-#pragma unroll
-                for (int bi = 0; bi < 4; bi++)
-#pragma unroll
-                    for (int bj = 0; bj < 4; bj++)
-                        acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(rf[bj], lf[bi], acc[bi][bj], 0, 0, 0);
-            }
-        }
+    const int nkh = (nbp + 3) / 4, S = 2 * nkh;
+    double la[2], ra[4], lb[2], rb[4];
+    r2k_operands(A, lda, U, ldu, c0, nbp, nkh, 0, lq, o, la, ra);
+    for (int s = 0; s < S; s += 2) {
+        if (s + 1 < S) r2k_operands(A, lda, U, ldu, c0, nbp, nkh, s + 1, lq, o, lb, rb);
+        r2k_mfma(acc, la, ra);
+        if (s + 1 >= S) break;
+        if (s + 2 < S) r2k_operands(A, lda, U, ldu, c0, nbp, nkh, s + 2, lq, o, la, ra);
+        r2k_mfma(acc, lb, rb);
     }
-    if (!active) return;
+    if (inner) {
 #pragma unroll
-    for (int bi = 0; bi < 4; bi++) {
-        const int64_t row = crow + bi * 16;
-        if (row >= n || row < rlo) continue;
+        for (int bi = 0; bi < 2; bi++)
 #pragma unroll
-        for (int bj = 0; bj < 4; bj++) {
+            for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) C[co + bi * 16 + (bj * 16 + 4 * q) * lda] = acc[bi][bj][q];
+        return;
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++) {
+        const int64_t row = r0 + lc + bi * 16;
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int64_t col = ccol + bj * 16 + 4 * q;
-                if (col >= clo && col < row) A[row + col * lda] = acc[bi][bj][q];
+                const int64_t col = q0 + lq + bj * 16 + 4 * q;
+                if (row < n && col >= e && col < row) C[co + bi * 16 + (bj * 16 + 4 * q) * lda] = acc[bi][bj][q];
             }
+    }
+}
+
+// Triangle index t -> (i, j), j <= i, row-major: row i holds i + 1 entries.
+__host__ __device__ inline void tri_ij(int64_t t, int64_t& i, int64_t& j)
+{
+    int64_t x = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while (x * (x + 1) / 2 > t) x--;
+    while ((x + 1) * (x + 2) / 2 <= t) x++;
+    i = x;
+    j = t - x * (x + 1) / 2;
+}
+
+// Lower triangle of nt x nt tile blocks in supertile order: supertiles of R2K_SUPER x
+// R2K_SUPER blocks in row-major triangle order, blocks row-major inside each supertile, so
+// that consecutive indices (one XCD's share of the launch) reuse the same operand rows in L2.
+__host__ __device__ inline void super_ij(int64_t t, int64_t nt, int64_t& i, int64_t& j)
+{
+    const int64_t S = R2K_SUPER;
+    for (int64_t I = 0;; I++) {
+        const int64_t rI = (nt - I * S < S) ? nt - I * S : S;   // block rows in supertile row I
+        const int64_t full = I * rI * S;                          // blocks left of the diagonal supertile
+        const int64_t cnt = full + rI * (rI + 1) / 2;
+        if (t < cnt) {
+            if (t < full) {
+                const int64_t J = t / (rI * S), u = t - J * rI * S;
+                i = I * S + u / S;
+                j = J * S + u % S;
+            } else {
+                int64_t a, b;
+                tri_ij(t - full, a, b);
+                i = I * S + a;
+                j = I * S + b;
+            }
+            return;
         }
-    }
-}
-
-// Number of BT-tiles of the lower triangle for rows [rlo, n), cols [clo, n); tile origins are
-// absolute multiples of BT.  Tile t -> (bi, bj) with bj <= bi in the row-major triangle order
-// restricted to column blocks >= the first column block.
-struct TileGrid {
-    int64_t rb0, cb0, nrb;   // first row block, first column block, number of row blocks
-    int64_t total;
-};
-
-__host__ __device__ inline TileGrid tile_grid(int64_t n, int64_t rlo, int64_t clo)
-{
-    TileGrid tg;
-    tg.rb0 = rlo / BT;
-    tg.cb0 = clo / BT;
-    const int64_t rb_end = (n + BT - 1) / BT;
-    tg.nrb = rb_end - tg.rb0;
-    tg.total = 0;
-    for (int64_t bi = tg.rb0; bi < rb_end; bi++) tg.total += bi - tg.cb0 + 1;
-    return tg;
-}
-
-__device__ inline void tile_coords(const TileGrid& tg, int64_t t, int64_t& r0, int64_t& q0)
-{
-    // row block bi holds (bi - cb0 + 1) tiles
-    int64_t bi = tg.rb0, cnt = bi - tg.cb0 + 1;
-    while (t >= cnt) {
         t -= cnt;
-        bi++;
-        cnt = bi - tg.cb0 + 1;
     }
-    r0 = bi * BT;
-    q0 = (tg.cb0 + t) * BT;
 }
 
 // log|Pf| and sign from eta, tau of an even-n factorization (src/pfaffian.jl:170-178):

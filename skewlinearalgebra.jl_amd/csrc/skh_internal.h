@@ -61,8 +61,8 @@ int skh_internal_launch_compact(const CompactArgs* a, int nmax, int grid, hipStr
 int skh_internal_logpf_from(const double* eta, const double* tau, int64_t n, double* logpf, double* sgn, hipStream_t st);
 int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st);
 int skh_internal_panel_occupancy(int nbmax, int* blocks_per_cu);
-int skh_internal_launch_rank2k(double* A, int64_t lda, const double* U, int64_t ldu, int64_t n, int64_t rlo,
-                               int64_t clo, int64_t c0, int nbp, hipStream_t st);
+int skh_internal_launch_rank2k(double* A, int64_t lda, const double* U, int64_t ldu, int64_t n, int64_t e,
+                               int64_t c0, int nbp, hipStream_t st);
 int skh_internal_launch_finish(double* A, int64_t lda, int64_t n, double* tau, double* eta, hipStream_t st);
 int skh_internal_zero_u32(unsigned* p, int count, hipStream_t st);
 int skh_internal_mfma_probe(double* dout, hipStream_t st);

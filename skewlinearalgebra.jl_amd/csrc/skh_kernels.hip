@@ -64,17 +64,20 @@ __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int
     }
 }
 
-__global__ __launch_bounds__(WG) void k_rank2k(double* A, int64_t lda, const double* U, int64_t ldu, int64_t n,
-                                               int64_t rlo, int64_t clo, int64_t c0, int nbp, TileGrid tg,
-                                               int64_t per)
+// One wave per 32x64 tile; tile blocks (64x64, two tiles each) of the trailing triangle
+// from block b0 = e/64, consecutive indices on one XCD (blocks are dealt to XCDs round robin).
+__global__ __launch_bounds__(64, 2) void k_rank2k(double* A, int64_t lda, const double* U, int64_t ldu, int64_t n,
+                                                  int64_t e, int64_t c0, int nbp, int64_t b0, int64_t nt,
+                                                  int64_t total, int64_t per)
 {
-    __shared__ MfmaShared sm;
     const int64_t b = blockIdx.x;
-    const int64_t t = (b % 8) * per + b / 8;   // consecutive tiles share an XCD (and its L2)
-    if (t >= tg.total) return;
-    int64_t r0, q0;
-    tile_coords(tg, t, r0, q0);
-    rank2k_tile(A, lda, U, ldu, n, rlo, clo, c0, nbp, r0, q0, sm);
+    const int64_t t = (b % 8) * per + b / 8;
+    if (t >= total) return;
+    int64_t i, j;
+    super_ij(t >> 1, nt, i, j);
+    const int64_t r0 = (b0 + i) * R2K_TC + (t & 1) * R2K_TR, q0 = (b0 + j) * R2K_TC;
+    if (r0 + R2K_TR - 1 <= q0) return;   // the upper half of a diagonal block: nothing below the diagonal
+    rank2k_wave(A, lda, U, ldu, n, e, c0, nbp, r0, q0);
 }
 
 __global__ void k_finish(double* A, int64_t lda, int64_t n, double* tau, double* eta)
@@ -274,14 +277,15 @@ extern "C" int skh_internal_panel_occupancy(int nbmax, int* blocks_per_cu)
     return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, panel_kernel_for(nbmax), WG, 0);
 }
 
-extern "C" int skh_internal_launch_rank2k(double* A, int64_t lda, const double* U, int64_t ldu, int64_t n, int64_t rlo,
-                                          int64_t clo, int64_t c0, int nbp, hipStream_t st)
+extern "C" int skh_internal_launch_rank2k(double* A, int64_t lda, const double* U, int64_t ldu, int64_t n, int64_t e,
+                                          int64_t c0, int nbp, hipStream_t st)
 {
-    TileGrid tg = tile_grid(n, rlo, clo);
-    if (tg.total <= 0) return 0;
-    int64_t per = (tg.total + 7) / 8;
-    hipLaunchKernelGGL(k_rank2k, dim3((unsigned)(per * 8)), dim3(WG), 0, st, A, lda, U, ldu, n, rlo, clo, c0, nbp, tg,
-                       per);
+    if (e >= n) return 0;
+    const int64_t b0 = e / R2K_TC, nt = (n + R2K_TC - 1) / R2K_TC - b0;
+    const int64_t total = nt * (nt + 1);   // two wave tiles per block of the triangle
+    const int64_t per = (total + 7) / 8;
+    hipLaunchKernelGGL(k_rank2k, dim3((unsigned)(per * 8)), dim3(64), 0, st, A, lda, U, ldu, n, e, c0, nbp, b0, nt,
+                       total, per);
     return (int)hipGetLastError();
 }
 
