@@ -50,11 +50,11 @@ __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int
     for (int i = 0; i < nbp; i++) {
         const int64_t j = c0 + i;
         if (clock) { unsigned long long t = wall_clock64(); acc[0] += t - t0; t0 = t; }
-        if (!grid_sync(bar, ++epoch, err, &s_ok)) return;
+        if (!grid_sync_co(bar, ++epoch, err, &s_ok)) return;
         if (clock) { unsigned long long t = wall_clock64(); acc[1] += t - t0; t0 = t; }
         phase_b(A, lda, n, c0, j, i, w, sh, tau, eta, g, G);
         if (clock) { unsigned long long t = wall_clock64(); acc[2] += t - t0; t0 = t; }
-        if (!grid_sync(bar, ++epoch, err, &s_ok)) return;
+        if (!grid_sync_co(bar, ++epoch, err, &s_ok)) return;
         if (clock) { unsigned long long t = wall_clock64(); acc[3] += t - t0; t0 = t; }
         row_pass<KQ>(A, lda, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, g, G, sh, rr);
     }

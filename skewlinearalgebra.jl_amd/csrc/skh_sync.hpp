@@ -48,4 +48,39 @@ __device__ inline bool grid_sync(unsigned* bar, unsigned epoch, int* err, int* s
     return *s_ok != 0;
 }
 
+// The same two-level barrier without fences, for kernels whose cross-workgroup data moves only
+// through agent-scope atomic loads and stores (sc1: coherent across the XCDs' L2 caches).  Each
+// wave's stores are complete (vmcnt 0) before its workgroup arrives, and the wait itself is a
+// compiler barrier, so no load after it is served before the arrival count is seen.  A release
+// and acquire fence per workgroup would write back and invalidate the XCD's L2 each time
+// (about 20 us per barrier at 512 workgroups on MI355X, 3.4 us without).
+__device__ inline bool grid_sync_co(unsigned* bar, unsigned epoch, int* err, int* s_ok)
+{
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned G = gridDim.x;
+        const unsigned grp = blockIdx.x & 7u;
+        const unsigned ngroups = G < 8u ? G : 8u;
+        const unsigned gsize = (G - grp + 7u) / 8u;
+        const unsigned old = __hip_atomic_fetch_add(&bar[32u * (1u + grp)], 1u, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        if (old + 1u == epoch * gsize) __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned spins = 0;
+        int ok = 1;
+        while (__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch * ngroups) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > SPIN_LIMIT) {
+                __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+        }
+        if (ok && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) ok = 0;
+        *s_ok = ok;
+    }
+    __syncthreads();
+    return *s_ok != 0;
+}
+
 }  // namespace skh

@@ -545,3 +545,48 @@ def test_sytrd_small_grid_several_blocks(n, cap):
     _, _, etao = oracle.skewblockedhess(A)
     np.testing.assert_allclose(eta, etao, rtol=0, atol=1e-11 * np.linalg.norm(A))
     check_fact(A, F, tau, eta)
+
+
+def tri_fourth_moment(e):
+    """tr(T^4) = ||T^2||_F^2 for the skew tridiagonal T with subdiagonal e: T^2 is symmetric
+    pentadiagonal with (T^2)_ii = -(e_{i-1}^2 + e_i^2) and (T^2)_{i,i+2} = e_i e_{i+1}."""
+    e2 = np.concatenate(([0.0], e * e, [0.0]))
+    d = e2[:-1] + e2[1:]
+    off = e[:-1] * e[1:]
+    return float(np.sum(d * d) + 2.0 * np.sum(off * off))
+
+
+@pytest.mark.parametrize("nb", [64, 128])
+def test_sytrd_n8192_invariants(nb):
+    """Full-size factorization, the whole cooperative grid (512 workgroups) in the first panels.
+    A = Q T Q^T fixes ||T||_F = ||A||_F, tr(T^4) = ||A^2||_F^2 and log|det A| = 2 log|Pf T|;
+    the right-hand sides come from A itself on the GPU (a product and an LU), so a reflector
+    built from stale or misplaced data fails one of them."""
+    torch = pytest.importorskip("torch")
+    say("sytrd n=8192 nb=%d invariants" % nb)
+    n = 8192
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(8192 + nb)
+    B = torch.randn((n, n), generator=g, dtype=torch.float64, device=dev)
+    A = B - B.T
+    del B
+    work = A.T.contiguous()   # column-major storage of A
+    tau = torch.empty(n, dtype=torch.float64, device=dev)
+    eta = torch.empty(n, dtype=torch.float64, device=dev)
+    o = _lib.opts(nb, skl.ENGINE_LARGE)
+    st = torch.cuda.current_stream().cuda_stream
+    rc = _lib.lib().skh_sytrd_d_device(n, work.data_ptr(), n, tau.data_ptr(), eta.data_ptr(), st, ctypes.byref(o))
+    _lib.check("skh_sytrd_d_device", rc)
+    torch.cuda.synchronize()
+    del work
+    e = eta[: n - 1].cpu().numpy()
+    nA = float(torch.linalg.norm(A))
+    assert abs(np.sqrt(2.0 * np.sum(e * e)) - nA) <= 1e-13 * nA
+    A2 = A @ A
+    m4 = float(torch.sum(A2 * A2))
+    del A2
+    assert abs(tri_fourth_moment(e) - m4) <= 1e-12 * m4, (tri_fourth_moment(e), m4)
+    _, logdet = torch.linalg.slogdet(A)
+    lp = float(np.sum(np.log(np.abs(e[0::2]))))
+    assert abs(2.0 * lp - float(logdet)) <= 1e-10 * abs(float(logdet)), (2.0 * lp, float(logdet))
