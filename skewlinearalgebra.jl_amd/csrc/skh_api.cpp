@@ -162,7 +162,7 @@ struct Ctl {
 int get_ctl(Ctl& c)
 {
     // words [0, 288): barrier counters (9 lines of 32 words); [320]: error flag of the panel
-    // kernel; [384, 392): phase clocks (4 x u64); [448]: compact queue; [480]: compact error flag
+    // kernel; [384, 406): phase clocks (11 x u64); [448]: compact queue; [480]: compact error flag
     unsigned* p = (unsigned*)g_ctl.get(2048);
     if (!p) return set_err(SKH_ERR_HIP, "hipMalloc of control block failed");
     c.bar = p;
@@ -212,7 +212,7 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
     static const int64_t tpw = [] {
         const char* e = getenv("SKH_PANEL_TPW");
         const long v = e ? atol(e) : 0;
-        return (int64_t)(v > 0 ? v : 8);
+        return (int64_t)(v > 0 ? v : 1);
     }();
     const int64_t gcap = [] {
         const char* e = getenv("SKH_PANEL_GRID_CAP");
@@ -1066,6 +1066,23 @@ int skh_diag_timings(double* out10)
         out6[2 * k + 1] += 1.0;
         out6[4 + k] += g_timer.work[q];
     }
+    return 0;
+}
+
+int skh_diag_phase_clock(double* out, int count)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!out) return set_err(-1, "NULL");
+    if (count < 0 || count > 11) return set_err(-2, "count must be in [0, 11]");
+    if (g_timer.used) HIPCHK(hipEventSynchronize(g_timer.ev[g_timer.used - 1]));
+    Ctl ctl;
+    int rc = get_ctl(ctl);
+    if (rc) return rc;
+    unsigned long long ph[11];
+    HIPCHK(hipMemcpy(ph, ctl.ph, sizeof(ph), hipMemcpyDeviceToHost));
+    int khz = 0;
+    HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g_device));
+    for (int k = 0; k < count; k++) out[k] = khz > 0 ? (double)ph[k] / (double)khz : 0.0;
     return 0;
 }
 

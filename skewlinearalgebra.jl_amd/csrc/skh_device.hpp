@@ -73,6 +73,31 @@ __device__ __forceinline__ int64_t cdiv(int64_t a, int64_t b) { return (a + b - 
 // index of this thread's wave in the workgroup, as a wave-uniform (scalar) value
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
 
+// Phase clock of the panel kernel (workgroup 0, thread 0; clk == nullptr when off): slots
+// [0, 11) accumulate 100 MHz wall-clock ticks, PH_INNER / PH_OUTER hold the last stamps of the
+// inner (sub-phase) and outer (phase) clocks.
+constexpr int PH_SLOTS = 16, PH_INNER = 14, PH_OUTER = 15;
+__device__ __forceinline__ void clk_begin(unsigned long long* clk)
+{
+    if (clk) clk[PH_INNER] = wall_clock64();
+}
+__device__ __forceinline__ void clk_inner(unsigned long long* clk, int k)
+{
+    if (clk) {
+        const unsigned long long t = wall_clock64();
+        clk[k] += t - clk[PH_INNER];
+        clk[PH_INNER] = t;
+    }
+}
+__device__ __forceinline__ void clk_outer(unsigned long long* clk, int k)
+{
+    if (clk) {
+        const unsigned long long t = wall_clock64();
+        clk[k] += t - clk[PH_OUTER];
+        clk[PH_OUTER] = t;
+    }
+}
+
 // Data one workgroup of the panel kernel writes and another reads within the same launch (the
 // partial sums, the current column x_j, the rows of V and W) moves through agent-scope relaxed
 // atomic accesses (global_load/global_store with sc1): coherent across the XCDs' L2 caches
@@ -243,8 +268,10 @@ struct RowRegs {
 // issued together, before the first barrier, so a row pass costs one memory round trip.
 template <int KQ>
 __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j, int i,
-                         bool doC, bool doA, const Work& w, int g, int G, RowShared& sh, RowRegs<KQ>& rr)
+                         bool doC, bool doA, const Work& w, int g, int G, RowShared& sh, RowRegs<KQ>& rr,
+                         unsigned long long* clk)
 {
+    clk_begin(clk);
     const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
     const int64_t jm = j - 1;         // column finalised by the C part
     const int im = i - 1;             // its local index
@@ -346,6 +373,7 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                 sh.vrow[tid] = vraw;
             }
             __syncthreads();
+            clk_inner(clk, 4);
             if (doC && wave == 0) {
                 double acc = sprow;
 #pragma unroll
@@ -391,6 +419,7 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                 }
             }
         }
+        clk_inner(clk, 5);
         double x = 0.0;
         if (doA) {
             // ---- A part: update column j with the panel's reflectors 0..i-1 ----
@@ -439,6 +468,7 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                 }
             }
         }
+        clk_inner(clk, 6);
         if (own) {
             // keep the block's rows for the next column of the panel; prefetch its input column
             if (doC) {   // only column im is new
@@ -454,6 +484,7 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
             rr.xnext = (doA && rin && r > j + 1 && j + 1 < n) ? A[r + (j + 1) * lda] : 0.0;
         }
         __syncthreads();
+        clk_inner(clk, 7);
     }
 }
 
@@ -510,8 +541,10 @@ __device__ __forceinline__ double vfrom(int64_t n, int64_t j, int64_t c, double 
 }
 
 __device__ void phase_b(const double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j, int i,
-                        const Work& w, RowShared& sh, double* tau_out, double* eta_out, int g, int G)
+                        const Work& w, RowShared& sh, double* tau_out, double* eta_out, int g, int G,
+                        unsigned long long* clk)
 {
+    clk_begin(clk);
     const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
     const int p = lane & 15, cg = lane >> 4;
     const int64_t nq = cdiv(n, TB) - j / TB;
@@ -562,6 +595,7 @@ __device__ void phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         }
     }
     __syncthreads();
+    clk_inner(clk, 8);
     const ColScal s = sh.cur;
     if (mv) sh.vstrip[tid] = vfrom(n, j, I * MR + tid, xs, s);
     // ---- (2) t1, t2 ----
@@ -581,6 +615,7 @@ __device__ void phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         acc = wave_sum(acc);
         if (lane == 0) st_co(w.Tv + off + k, s.unit * head + s.scale * acc);
     }
+    clk_inner(clk, 9);
     // ---- (3) matvec ----
     if (!mv) return;
     __syncthreads();   // v of the first strip is in LDS
@@ -640,6 +675,7 @@ __device__ void phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         J = Jn;
         c = cn;
     }
+    clk_inner(clk, 10);
 }
 
 // ---------------------------------------------------------------------------------------

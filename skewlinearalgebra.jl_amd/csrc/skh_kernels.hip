@@ -19,8 +19,10 @@
 
 namespace skh {
 
-// Optional phase clock (workgroup 0, 100 MHz wall clock): ph[0] row passes, ph[1] barrier 1
-// waits, ph[2] scalars + t + matvec, ph[3] barrier 2 waits (accumulated over all launches).
+// Optional phase clock (workgroup 0, 100 MHz wall clock), accumulated over all launches:
+// ph[0] row passes, ph[1] barrier 1 waits, ph[2] scalars + t + matvec, ph[3] barrier 2 waits;
+// inside them ph[4..7] the row pass (loads, correction of w, column update with its partials,
+// hand-over to the next column) and ph[8..10] phase B (loads and scalars, t, matvec).
 // panels up to 64 wide fit two workgroups per CU (<= 256 registers per lane); 128 wide takes one
 template <int KQ>
 __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int64_t lda, int64_t n, int64_t c0, int nbp, Work w,
@@ -29,9 +31,14 @@ __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int
 {
     __shared__ RowShared sh;
     __shared__ int s_ok;
+    __shared__ unsigned long long s_clk[PH_SLOTS];
     const int G = gridDim.x, g = blockIdx.x;
-    const bool clock = (ph != nullptr) && g == 0 && threadIdx.x == 0;
-    unsigned long long t0 = clock ? wall_clock64() : 0, acc[4] = {0, 0, 0, 0};
+    unsigned long long* clk = ((ph != nullptr) && g == 0 && threadIdx.x == 0) ? s_clk : nullptr;
+    if (clk) {
+        for (int k = 0; k < PH_SLOTS; k++) s_clk[k] = 0;
+        clk_begin(clk);
+        clk[PH_OUTER] = clk[PH_INNER];
+    }
     unsigned epoch = base;   // barriers completed before this launch
     __shared__ double s_pv[4 * KQ * 64], s_pw[4 * KQ * 64];
     RowRegs<KQ> rr;
@@ -46,21 +53,21 @@ __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int
         const int64_t r = (c0 / TB + g) * TB + (threadIdx.x & 63);
         rr.xnext = (r < n && r > c0) ? A[r + c0 * lda] : 0.0;
     }
-    row_pass<KQ>(A, lda, n, c0, c0, 0, false, true, w, g, G, sh, rr);
+    row_pass<KQ>(A, lda, n, c0, c0, 0, false, true, w, g, G, sh, rr, clk);
     for (int i = 0; i < nbp; i++) {
         const int64_t j = c0 + i;
-        if (clock) { unsigned long long t = wall_clock64(); acc[0] += t - t0; t0 = t; }
+        clk_outer(clk, 0);
         if (!grid_sync_co(bar, ++epoch, err, &s_ok)) return;
-        if (clock) { unsigned long long t = wall_clock64(); acc[1] += t - t0; t0 = t; }
-        phase_b(A, lda, n, c0, j, i, w, sh, tau, eta, g, G);
-        if (clock) { unsigned long long t = wall_clock64(); acc[2] += t - t0; t0 = t; }
+        clk_outer(clk, 1);
+        phase_b(A, lda, n, c0, j, i, w, sh, tau, eta, g, G, clk);
+        clk_outer(clk, 2);
         if (!grid_sync_co(bar, ++epoch, err, &s_ok)) return;
-        if (clock) { unsigned long long t = wall_clock64(); acc[3] += t - t0; t0 = t; }
-        row_pass<KQ>(A, lda, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, g, G, sh, rr);
+        clk_outer(clk, 3);
+        row_pass<KQ>(A, lda, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, g, G, sh, rr, clk);
     }
-    if (clock) {
-        acc[0] += wall_clock64() - t0;
-        for (int k = 0; k < 4; k++) ph[k] += acc[k];
+    if (clk) {
+        clk_outer(clk, 0);
+        for (int k = 0; k < 11; k++) ph[k] += s_clk[k];
     }
 }
 

@@ -53,6 +53,7 @@ def lib():
         "skh_skewtrieigen_d": ([i64, vp, vp, vp, vp, ctypes.c_int], ctypes.c_int),
         "skh_diag_mfma_probe": ([vp], ctypes.c_int),
         "skh_diag_timings": ([vp], ctypes.c_int),
+        "skh_diag_phase_clock": ([vp, ctypes.c_int], ctypes.c_int),
         "skh_orgtr_d_device": ([i64, vp, i64, vp, vp, i64, vp], ctypes.c_int),
         "skh_eigen_d": ([i64, vp, i64, vp, vp, vp, op], ctypes.c_int),
         "skh_logpf_grad_batched_d": ([i64, vp, vp, vp, vp, vp, vp, op], ctypes.c_int),
