@@ -206,6 +206,8 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
     const int grid = per_cu * g_num_cu;
     if ((int64_t)64 * (int64_t)(2 * n + 4) > 4000000000LL)
         return set_err(SKH_ERR_UNSUPPORTED, "n = %lld too large for the barrier counter", (long long)n);
+    if ((double)skh::tall_grid(n, 1).T * (double)grid >= 4294967296.0)
+        return set_err(SKH_ERR_UNSUPPORTED, "n = %lld too large for the 32-bit tile ownership", (long long)n);
 
     // target matvec tiles per workgroup (tuning knob SKH_PANEL_TPW; measured default) and an
     // optional cap on the grid (SKH_PANEL_GRID_CAP, for tests of workgroups owning several blocks)

@@ -153,19 +153,13 @@ __device__ void fused_pass(double* __restrict__ M, int64_t n, int64_t s0, bool u
         }
 #pragma unroll
         for (int cc = 0; cc < 4; cc++) {
-            double x = cacc[cc];
-            x += __shfl_xor(x, 1, 64);
-            x += __shfl_xor(x, 2, 64);
-            x += __shfl_xor(x, 4, 64);
-            x += __shfl_xor(x, 8, 64);
+            const double x = row16_sum(cacc[cc]);
             if (p == 0) s.yc[cb + cc] += x;   // column cb+cc is always owned by this lane
         }
         if (last_in_strip) {
 #pragma unroll
             for (int rr = 0; rr < 4; rr++) {
-                double x = racc[rr];
-                x += __shfl_xor(x, 16, 64);
-                x += __shfl_xor(x, 32, 64);
+                const double x = xor32_sum(xor16_sum(racc[rr]));
                 if (cg == 0) s.yr[wave][I * TB + 32 * (rr >> 1) + 2 * p + (rr & 1)] = x;
                 racc[rr] = 0.0;
             }
