@@ -315,10 +315,11 @@ def main():
 
     panel_ms, panel_cnt, upd_ms, upd_cnt, panel_bytes, upd_flops = tm[:6]
     phases = {k: v / args.steps for k, v in zip(("row_pass", "barrier1", "matvec", "barrier2"), tm[6:])}
-    clk = (ctypes.c_double * 11)()
-    _lib.check("skh_diag_phase_clock", L.skh_diag_phase_clock(clk, 11))   # last step only
+    clk = (ctypes.c_double * 13)()
+    _lib.check("skh_diag_phase_clock", L.skh_diag_phase_clock(clk, 13))   # last step only
     phases["split_last_step"] = dict(zip(("row_loads", "row_correct_w", "row_update_x", "row_handover",
-                                          "b_loads_scalars", "b_t", "b_matvec"), clk[4:11]))
+                                          "b_loads_scalars", "b_t", "b_matvec", "b_first_tile", "b_wait"),
+                                         clk[4:13]))
     panel_avg_s = panel_ms / panel_cnt * 1e-3
     panel_bytes_per = panel_bytes / panel_cnt
     achieved = panel_bytes_per / panel_avg_s / 1e9

@@ -197,9 +197,10 @@ int skh_skewtrieigen_d(int64_t n, double *ev, double *vals, double *vr, double *
  * Waits for the recorded events. */
 int skh_diag_timings(double *out10);
 /* Diagnostics: the panel kernel's phase clock of the last SKH_FLAG_TIMING factorization, in ms
- * (workgroup 0, summed over its launches): [0..3] as in skh_diag_timings, then the row pass split
- * into [4] loads, [5] correction of w, [6] column update and partials, [7] hand-over, and phase B
- * into [8] loads and scalars, [9] t, [10] matvec.  count <= 11 values are written. */
+ * (workgroup 0, summed over its launches): [0..3] as in skh_diag_timings (barrier 1 arrivals in
+ * [1], its wait inside phase B [2]), then the row pass split into [4] loads, [5] correction of w,
+ * [6] column update and partials, [7] hand-over, and phase B into [11] tile range and first tile,
+ * [12] barrier-1 wait, [8] loads and scalars, [9] t, [10] matvec.  count <= 13 values are written. */
 int skh_diag_phase_clock(double *out, int count);
 /* Diagnostics: v_mfma_f64_16x16x4f64 lane-layout probe (768 doubles, see the test suite). */
 int skh_diag_mfma_probe(double *out768);

@@ -162,7 +162,7 @@ struct Ctl {
 int get_ctl(Ctl& c)
 {
     // words [0, 288): barrier counters (9 lines of 32 words); [320]: error flag of the panel
-    // kernel; [384, 406): phase clocks (11 x u64); [448]: compact queue; [480]: compact error flag
+    // kernel; [384, 410): phase clocks (13 x u64); [448]: compact queue; [480]: compact error flag
     unsigned* p = (unsigned*)g_ctl.get(2048);
     if (!p) return set_err(SKH_ERR_HIP, "hipMalloc of control block failed");
     c.bar = p;
@@ -1075,12 +1075,12 @@ int skh_diag_phase_clock(double* out, int count)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!out) return set_err(-1, "NULL");
-    if (count < 0 || count > 11) return set_err(-2, "count must be in [0, 11]");
+    if (count < 0 || count > 13) return set_err(-2, "count must be in [0, 13]");
     if (g_timer.used) HIPCHK(hipEventSynchronize(g_timer.ev[g_timer.used - 1]));
     Ctl ctl;
     int rc = get_ctl(ctl);
     if (rc) return rc;
-    unsigned long long ph[11];
+    unsigned long long ph[13];
     HIPCHK(hipMemcpy(ph, ctl.ph, sizeof(ph), hipMemcpyDeviceToHost));
     int khz = 0;
     HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g_device));

@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "skh_sync.hpp"
+
 namespace skh {
 
 constexpr int WG = 256;       // threads per workgroup (4 waves of 64)
@@ -159,7 +161,7 @@ __device__ __forceinline__ int64_t cdiv(int64_t a, int64_t b) { return (a + b - 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
 
 // Phase clock of the panel kernel (workgroup 0, thread 0; clk == nullptr when off): slots
-// [0, 11) accumulate 100 MHz wall-clock ticks, PH_INNER / PH_OUTER hold the last stamps of the
+// [0, 13) accumulate 100 MHz wall-clock ticks, PH_INNER / PH_OUTER hold the last stamps of the
 // inner (sub-phase) and outer (phase) clocks.
 constexpr int PH_SLOTS = 16, PH_INNER = 14, PH_OUTER = 15;
 __device__ __forceinline__ void clk_begin(unsigned long long* clk)
@@ -320,6 +322,36 @@ __host__ __device__ inline int64_t tall_slots(const TallGrid& tg, int64_t I, int
     return tile_owner(f + cnt - 1, tg.T, G) - tile_owner(f, tg.T, G) + 1;
 }
 
+// Matvec tile map of one panel launch: the tiles of the trailing block of the panel's first column
+// (s0 = c0 + 1) split into Ge contiguous ranges, one per rank, fixed for the whole launch.  A later
+// column j of the panel keeps the same tiles: its columns <= j are masked to zero when loaded, so a
+// tile left of the trailing block adds nothing, and every rank still writes the row slot of each
+// strip it touches.  Computed once per launch instead of once per column.
+struct PanelMap {
+    TallGrid tg;
+    int64_t Ge;                      // ranks taking part in the matvec (G capped at the tile count)
+    int64_t tlo, thi, I, J, slot;    // this rank's tile range, its first tile, that tile's row slot
+    int64_t nslot_own;               // row slots of the strip holding the rank's own row block
+};
+
+__device__ inline PanelMap panel_map(int64_t n, int64_t c0, int g, int G)
+{
+    PanelMap pm;
+    pm.tg = tall_grid(n, c0 + 1);
+    pm.Ge = (G < pm.tg.T) ? G : pm.tg.T;
+    pm.tlo = pm.thi = pm.I = pm.J = pm.slot = 0;
+    if (g < pm.Ge) {
+        pm.tlo = range_lo(g, pm.tg.T, pm.Ge);
+        pm.thi = range_lo(g + 1, pm.tg.T, pm.Ge);
+        pm.I = tall_strip_of(pm.tg, pm.tlo);
+        pm.J = pm.tg.Jlo + (pm.tlo - tall_first(pm.tg, pm.I));
+        pm.slot = g - tile_owner(tall_first(pm.tg, pm.I), pm.tg.T, pm.Ge);
+    }
+    const int64_t S = ((c0 / TB + g) * TB) / MR;
+    pm.nslot_own = (S >= pm.tg.I0 && S < pm.tg.nts) ? tall_slots(pm.tg, S, G) : 0;
+    return pm;
+}
+
 // ---------------------------------------------------------------------------------------
 // Row pass: finalises w of column j-1 (doC) and/or applies the panel column update to
 // column j and produces its norm and W^T x / V^T x partials (doA).
@@ -366,7 +398,7 @@ struct RowRegs {
 template <int KQ>
 __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j, int i,
                          bool doC, bool doA, const Work& w, int g, int G, RowShared& sh, RowRegs<KQ>& rr,
-                         unsigned long long* clk)
+                         const PanelMap& pm, unsigned long long* clk)
 {
     clk_begin(clk);
     const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
@@ -381,7 +413,7 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
     const int k0 = wave * KQ, k1 = min(i, k0 + KQ);
     const int kc = doC ? min(k1, im) : k0;   // C correction uses k in [k0, kc)
     constexpr int PB = 16;            // column partials per lane per batch (4 waves: 64 strips)
-    const TallGrid tg = tall_grid(n, j);     // tiles of the matvec that produced the partials
+    const TallGrid& tg = pm.tg;       // tiles of the matvec that produced the partials
     bool shared_done = false;
 
     for (int64_t b = bown; b < nt; b += G) {
@@ -439,7 +471,7 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
         double pp = 0.0;
         if (rowC) {
             const int64_t S = (b * TB) / MR;    // matvec strip holding this chunk
-            const int64_t nslot = tall_slots(tg, S, G);
+            const int64_t nslot = own ? pm.nslot_own : tall_slots(tg, S, G);
             for (int64_t s0 = wave; s0 < nslot; s0 += 32) {
                 double sb[8];
 #pragma unroll
@@ -637,15 +669,30 @@ __device__ __forceinline__ double vfrom(int64_t n, int64_t j, int64_t c, double 
     return (c == j + 1) ? s.unit : s.scale * x;
 }
 
-__device__ void phase_b(const double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j, int i,
+__device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j, int i,
                         const Work& w, RowShared& sh, double* tau_out, double* eta_out, int g, int G,
+                        const PanelMap& pm, unsigned* bar, unsigned epoch, int* err, int* s_ok,
                         unsigned long long* clk)
 {
     clk_begin(clk);
     const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
     const int p = lane & 15, cg = lane >> 4;
     const int64_t nq = cdiv(n, TB) - j / TB;
-    // ---- loads ----
+    // ---- before the barrier completes: this rank's first tile (tile data is never written during
+    // the launch) ----
+    const TallGrid& tg = pm.tg;
+    const bool mv = g < pm.Ge;
+    const int64_t tlo = pm.tlo, thi = pm.thi;
+    int64_t I = pm.I, J = pm.J, cur_slot = pm.slot, c = 0;
+    double a[16], xs = 0.0, xc = 0.0;
+    if (mv) {
+        c = J * MC + 4 * wave + cg;
+        tall_load(A, lda, n, j, I, c, p, a);
+    }
+    clk_inner(clk, 11);
+    if (!grid_wait_co(bar, epoch, err, s_ok)) return false;
+    clk_inner(clk, 12);
+    // ---- loads of what the row pass produced ----
     double pn0 = 0.0, pn1 = 0.0, araw = 0.0;
     if (wave == 0) {
         pn0 = (lane < nq) ? ld_co(w.Pnorm + lane) : 0.0;
@@ -661,19 +708,7 @@ __device__ void phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         ps1 = (lane + 64 < nq) ? ld_co(w.PS + (int64_t)(lane + 64) * (2 * NBMAX) + off + k) : 0.0;
         if (lane == 0) head0 = (tv0 < i) ? ld_co(w.W + (j + 1) + k * w.ldw) : ld_co(A + (j + 1) + (c0 + k) * lda);
     }
-    const TallGrid tg = tall_grid(n, j + 1);
-    const int64_t Ge = (G < tg.T) ? G : tg.T;
-    const bool mv = g < Ge;
-    int64_t tlo = 0, thi = 0, I = 0, J = 0, cur_slot = 0, c = 0;
-    double a[16], xs = 0.0, xc = 0.0;
     if (mv) {
-        tlo = range_lo(g, tg.T, Ge);
-        thi = range_lo(g + 1, tg.T, Ge);
-        I = tall_strip_of(tg, tlo);
-        J = tg.Jlo + (tlo - tall_first(tg, I));
-        cur_slot = g - tile_owner(tall_first(tg, I), tg.T, Ge);
-        c = J * MC + 4 * wave + cg;
-        tall_load(A, lda, n, j, I, c, p, a);
         xc = xraw(A, lda, n, j, c);
         xs = xraw(A, lda, n, j, I * MR + tid);   // this thread's row of the strip
     }
@@ -714,7 +749,7 @@ __device__ void phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
     }
     clk_inner(clk, 9);
     // ---- (3) matvec ----
-    if (!mv) return;
+    if (!mv) return true;
     __syncthreads();   // v of the first strip is in LDS
     double racc[16];
 #pragma unroll
@@ -769,6 +804,7 @@ __device__ void phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         c = cn;
     }
     clk_inner(clk, 10);
+    return true;
 }
 
 // ---------------------------------------------------------------------------------------

@@ -20,9 +20,10 @@
 namespace skh {
 
 // Optional phase clock (workgroup 0, 100 MHz wall clock), accumulated over all launches:
-// ph[0] row passes, ph[1] barrier 1 waits, ph[2] scalars + t + matvec, ph[3] barrier 2 waits;
-// inside them ph[4..7] the row pass (loads, correction of w, column update with its partials,
-// hand-over to the next column) and ph[8..10] phase B (loads and scalars, t, matvec).
+// ph[0] row passes, ph[1] barrier 1 arrivals, ph[2] phase B (barrier 1 wait, scalars, t, matvec),
+// ph[3] barrier 2; inside them ph[4..7] the row pass (loads, correction of w, column update with
+// its partials, hand-over to the next column) and phase B: ph[11] tile range and first tile,
+// ph[12] barrier 1 wait, ph[8] loads and scalars, ph[9] t, ph[10] matvec.
 // panels up to 64 wide fit two workgroups per CU (<= 256 registers per lane); 128 wide takes one
 template <int KQ>
 __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int64_t lda, int64_t n, int64_t c0, int nbp, Work w,
@@ -53,21 +54,22 @@ __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int
         const int64_t r = (c0 / TB + g) * TB + (threadIdx.x & 63);
         rr.xnext = (r < n && r > c0) ? A[r + c0 * lda] : 0.0;
     }
-    row_pass<KQ>(A, lda, n, c0, c0, 0, false, true, w, g, G, sh, rr, clk);
+    const PanelMap pm = panel_map(n, c0, g, G);
+    row_pass<KQ>(A, lda, n, c0, c0, 0, false, true, w, g, G, sh, rr, pm, clk);
     for (int i = 0; i < nbp; i++) {
         const int64_t j = c0 + i;
         clk_outer(clk, 0);
-        if (!grid_sync_co(bar, ++epoch, err, &s_ok)) return;
+        grid_arrive_co(bar, ++epoch);   // phase B waits once its first tile is in flight
         clk_outer(clk, 1);
-        phase_b(A, lda, n, c0, j, i, w, sh, tau, eta, g, G, clk);
+        if (!phase_b(A, lda, n, c0, j, i, w, sh, tau, eta, g, G, pm, bar, epoch, err, &s_ok, clk)) return;
         clk_outer(clk, 2);
         if (!grid_sync_co(bar, ++epoch, err, &s_ok)) return;
         clk_outer(clk, 3);
-        row_pass<KQ>(A, lda, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, g, G, sh, rr, clk);
+        row_pass<KQ>(A, lda, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, g, G, sh, rr, pm, clk);
     }
     if (clk) {
         clk_outer(clk, 0);
-        for (int k = 0; k < 11; k++) ph[k] += s_clk[k];
+        for (int k = 0; k < 13; k++) ph[k] += s_clk[k];
     }
 }
 

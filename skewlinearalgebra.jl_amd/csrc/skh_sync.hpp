@@ -54,18 +54,27 @@ __device__ inline bool grid_sync(unsigned* bar, unsigned epoch, int* err, int* s
 // compiler barrier, so no load after it is served before the arrival count is seen.  A release
 // and acquire fence per workgroup would write back and invalidate the XCD's L2 each time
 // (about 20 us per barrier at 512 workgroups on MI355X, 3.4 us without).
-__device__ inline bool grid_sync_co(unsigned* bar, unsigned epoch, int* err, int* s_ok)
+__device__ inline void grid_arrive_co(unsigned* bar, unsigned epoch)
 {
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (threadIdx.x == 0) {
         const unsigned G = gridDim.x;
         const unsigned grp = blockIdx.x & 7u;
-        const unsigned ngroups = G < 8u ? G : 8u;
         const unsigned gsize = (G - grp + 7u) / 8u;
         const unsigned old = __hip_atomic_fetch_add(&bar[32u * (1u + grp)], 1u, __ATOMIC_RELAXED,
                                                     __HIP_MEMORY_SCOPE_AGENT);
         if (old + 1u == epoch * gsize) __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Second half of the barrier: work placed between grid_arrive_co and grid_wait_co may load only
+// data no other workgroup writes during the launch.
+__device__ inline bool grid_wait_co(unsigned* bar, unsigned epoch, int* err, int* s_ok)
+{
+    if (threadIdx.x == 0) {
+        const unsigned G = gridDim.x;
+        const unsigned ngroups = G < 8u ? G : 8u;
         unsigned spins = 0;
         int ok = 1;
         while (__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch * ngroups) {
@@ -81,6 +90,12 @@ __device__ inline bool grid_sync_co(unsigned* bar, unsigned epoch, int* err, int
     }
     __syncthreads();
     return *s_ok != 0;
+}
+
+__device__ inline bool grid_sync_co(unsigned* bar, unsigned epoch, int* err, int* s_ok)
+{
+    grid_arrive_co(bar, epoch);
+    return grid_wait_co(bar, epoch, err, s_ok);
 }
 
 }  // namespace skh
