@@ -34,6 +34,7 @@ extern "C" {
 #define SKH_ENGINE_LARGE 1     /* cooperative multi-workgroup panel + MFMA rank-2k update */
 #define SKH_ENGINE_TEAM1 2     /* one workgroup per matrix (the batched engine) */
 #define SKH_FLAG_TIMING 0x100  /* record per-kernel HIP events (read with skh_diag_timings) */
+#define SKH_FLAG_PHASES 0x200  /* also run the panel kernel's phase clock (workgroup 0; slows it) */
 
 typedef struct skh_opts {
     int64_t nb;     /* panel width; 0 = autotuned table; must be <= 128 */
@@ -196,7 +197,7 @@ int skh_skewtrieigen_d(int64_t n, double *ev, double *vals, double *vr, double *
  *          row passes, barrier-1 waits, scalars + matvec, barrier-2 waits}.
  * Waits for the recorded events. */
 int skh_diag_timings(double *out10);
-/* Diagnostics: the panel kernel's phase clock of the last SKH_FLAG_TIMING factorization, in ms
+/* Diagnostics: the panel kernel's phase clock of the last SKH_FLAG_PHASES factorization, in ms
  * (workgroup 0, summed over its launches): [0..3] as in skh_diag_timings (barrier 1 arrivals in
  * [1], its wait inside phase B [2]), then the row pass split into [4] loads, [5] correction of w,
  * [6] column update and partials, [7] hand-over, and phase B into [11] tile range and first tile,

@@ -122,6 +122,7 @@ struct KTimer {
     std::vector<double> work;     // bytes (panel) or flops (rank-2k)
     size_t used = 0;
     bool on = false;
+    bool phases = false;          // SKH_FLAG_PHASES: the panel kernel's phase clock as well
     int begin(hipStream_t st)
     {
         if (used + 2 > ev.size()) {
@@ -247,7 +248,11 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
             pl.grid = (int)std::min<int64_t>(std::min<int64_t>(grid, want), gcap);
         }
         pl.nbmax = (int)nb;
-        pl.ph = g_timer.on ? ctl.ph : nullptr;
+        {
+            const int64_t blocks = (n + skh::TB - 1) / skh::TB - c0 / skh::TB;
+            pl.multi = (int64_t)pl.grid < blocks ? 1 : 0;
+        }
+        pl.ph = g_timer.phases ? ctl.ph : nullptr;
         const bool tm = g_timer.on;
         if (tm) HIPCHK((hipError_t)g_timer.begin(st));
         HIPCHK((hipError_t)skh_internal_launch_panel(&pl, st));
@@ -326,6 +331,7 @@ int batched_run(int64_t batch, const int64_t* ns, const int64_t* offs, const dou
             HIPCHK(hipMemcpyAsync(M, dA + offs[b], (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, st));
         g_timer.reset();
         g_timer.on = false;
+        g_timer.phases = false;
         rc = sytrd_large(n, M, n, te, te + n, pick_nb(n, opts), st);
         if (rc) return rc;
         HIPCHK((hipError_t)skh_internal_logpf_from(te + n, te, n, dlogpf + b, dsgn + b, st));
@@ -669,6 +675,7 @@ int sytrd_device_locked(int64_t n, double* dA, int64_t lda, double* dtau, double
     const int64_t nb = pick_nb(n, opts);
     g_timer.reset();
     g_timer.on = opts && (opts->flags & SKH_FLAG_TIMING);
+    g_timer.phases = opts && (opts->flags & SKH_FLAG_PHASES);
     return sytrd_large(n, dA, lda, dtau, deta, nb, st);
 }
 
@@ -773,7 +780,7 @@ int skh_sytrd_d_device(int64_t n, double* dA, int64_t lda, double* dtau, double*
     hipStream_t st = (hipStream_t)stream;
     if (n <= 1) return 0;
     skh_opts o = opts ? *opts : skh_opts{0, 0};
-    o.flags = SKH_ENGINE_LARGE | (o.flags & SKH_FLAG_TIMING);
+    o.flags = SKH_ENGINE_LARGE | (o.flags & (SKH_FLAG_TIMING | SKH_FLAG_PHASES));
     return sytrd_device_locked(n, dA, lda, dtau, deta, st, &o);
 }
 
@@ -802,6 +809,7 @@ int skh_sytrd_d(int64_t n, double* A, int64_t lda, double* tau, double* eta, int
     if (engine_of(n, opts) == SKH_ENGINE_LARGE) {
         g_timer.reset();
         g_timer.on = false;
+        g_timer.phases = false;
         rc = sytrd_large(n, dA, ld, dtau, deta, pick_nb(n, opts), st);
         if (rc) return rc;
         HIPCHK(hipStreamSynchronize(st));

@@ -395,7 +395,7 @@ struct RowRegs {
 // for blocks other than the workgroup's own, also its rows of V and W and the old entries of
 // columns j-1 and j) and the shared inputs (t, row j of V and W, the special row's partials) are
 // issued together, before the first barrier, so a row pass costs one memory round trip.
-template <int KQ>
+template <int KQ, bool MULTI>
 __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j, int i,
                          bool doC, bool doA, const Work& w, int g, int G, RowShared& sh, RowRegs<KQ>& rr,
                          const PanelMap& pm, unsigned long long* clk)
@@ -416,11 +416,11 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
     const TallGrid& tg = pm.tg;       // tiles of the matvec that produced the partials
     bool shared_done = false;
 
-    for (int64_t b = bown; b < nt; b += G) {
+    for (int64_t b = bown; b < nt; b += (MULTI ? G : nt)) {
         if (b < b0) continue;         // block entirely above row j (wave-uniform)
         const bool first = !shared_done;
         shared_done = true;
-        const bool own = (b == bown);
+        const bool own = !MULTI || (b == bown);
         const int64_t q = b - b0;     // chunk index relative to row j's block
         const int64_t r = b * TB + lane;
         const bool rin = r < n;
@@ -439,13 +439,13 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
             if (doC && wave == 0) {
                 // special row r = j of w_{j-1}: no stored entry of row j lies in the block, so its row
                 // sums are empty; column partials of the strips I >= j / MR
-                const int64_t S0 = j / MR;
+                const int S0 = (int)(j / MR), nts = (int)tg.nts, ldp = (int)w.ldp, j32 = (int)j;
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    const int64_t I = S0 + lane + 64 * u;
-                    sp[u] = (I < tg.nts) ? ld_co(w.Pcol + I * w.ldp + j) : 0.0;
+                    const int I = S0 + lane + 64 * u;
+                    sp[u] = (I < nts) ? ld_co(w.Pcol + (I * ldp + j32)) : 0.0;
                 }
-                for (int64_t I = S0 + lane + 256; I < tg.nts; I += 64) sprow += ld_co(w.Pcol + I * w.ldp + j);
+                for (int I = S0 + lane + 256; I < nts; I += 64) sprow += ld_co(w.Pcol + (I * ldp + j32));
             }
         }
         double vreg[KQ], wreg[KQ], xo, xold;
@@ -470,20 +470,22 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
         }
         double pp = 0.0;
         if (rowC) {
-            const int64_t S = (b * TB) / MR;    // matvec strip holding this chunk
-            const int64_t nslot = own ? pm.nslot_own : tall_slots(tg, S, G);
-            for (int64_t s0 = wave; s0 < nslot; s0 += 32) {
+            // 32-bit element offsets: (slots + strips) * ldp stays far below 2^31 for any matrix in HBM
+            const int S = (int)((b * TB) / MR);    // matvec strip holding this chunk
+            const int nslot = (int)(own ? pm.nslot_own : tall_slots(tg, S, G));
+            const int nts = (int)tg.nts, ldp = (int)w.ldp, rr32 = (int)r;
+            for (int s0 = wave; s0 < nslot; s0 += 32) {
                 double sb[8];
 #pragma unroll
-                for (int u = 0; u < 8; u++) sb[u] = (s0 + 4 * u < nslot) ? ld_co(w.Prow + (s0 + 4 * u) * w.ldp + r) : 0.0;
+                for (int u = 0; u < 8; u++) sb[u] = (s0 + 4 * u < nslot) ? ld_co(w.Prow + ((s0 + 4 * u) * ldp + rr32)) : 0.0;
                 pp += ((sb[0] + sb[1]) + (sb[2] + sb[3])) + ((sb[4] + sb[5]) + (sb[6] + sb[7]));
             }
-            for (int64_t I0 = S + wave; I0 < tg.nts; I0 += 4 * PB) {
+            for (int I0 = S + wave; I0 < nts; I0 += 4 * PB) {
                 double buf[PB];
 #pragma unroll
                 for (int u = 0; u < PB; u++) {
-                    const int64_t I = I0 + 4 * u;
-                    buf[u] = (I < tg.nts) ? ld_co(w.Pcol + I * w.ldp + r) : 0.0;
+                    const int I = I0 + 4 * u;
+                    buf[u] = (I < nts) ? ld_co(w.Pcol + (I * ldp + rr32)) : 0.0;
                 }
                 double p4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll

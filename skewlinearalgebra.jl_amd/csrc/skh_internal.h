@@ -22,6 +22,7 @@ struct PanelLaunch {
     unsigned long long* ph;   // optional phase clock (4 counters) or null
     int grid;
     int nbmax;                // panel width bound selecting the kernel instance
+    int multi;                // 1 if a workgroup may own several 64-row blocks (grid < blocks)
 };
 
 // Compact engine (one workgroup per matrix, n <= nmax of the instantiation).

@@ -24,8 +24,11 @@ namespace skh {
 // ph[3] barrier 2; inside them ph[4..7] the row pass (loads, correction of w, column update with
 // its partials, hand-over to the next column) and phase B: ph[11] tile range and first tile,
 // ph[12] barrier 1 wait, ph[8] loads and scalars, ph[9] t, ph[10] matvec.
-// panels up to 64 wide fit two workgroups per CU (<= 256 registers per lane); 128 wide takes one
-template <int KQ>
+// panels up to 64 wide fit two workgroups per CU (<= 256 registers per lane); 128 wide takes one.
+// MULTI: the grid has fewer workgroups than the panel has 64-row blocks, so a workgroup also
+// processes blocks other than its own (rows of V and W then come from memory); the common
+// instance (one block per workgroup) carries none of that path.
+template <int KQ, bool MULTI>
 __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int64_t lda, int64_t n, int64_t c0, int nbp, Work w,
                                               double* tau, double* eta, unsigned* bar, unsigned base, int* err,
                                               unsigned long long* ph)
@@ -55,7 +58,7 @@ __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int
         rr.xnext = (r < n && r > c0) ? A[r + c0 * lda] : 0.0;
     }
     const PanelMap pm = panel_map(n, c0, g, G);
-    row_pass<KQ>(A, lda, n, c0, c0, 0, false, true, w, g, G, sh, rr, pm, clk);
+    row_pass<KQ, MULTI>(A, lda, n, c0, c0, 0, false, true, w, g, G, sh, rr, pm, clk);
     for (int i = 0; i < nbp; i++) {
         const int64_t j = c0 + i;
         clk_outer(clk, 0);
@@ -65,7 +68,7 @@ __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int
         clk_outer(clk, 2);
         if (!grid_sync_co(bar, ++epoch, err, &s_ok)) return;
         clk_outer(clk, 3);
-        row_pass<KQ>(A, lda, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, g, G, sh, rr, pm, clk);
+        row_pass<KQ, MULTI>(A, lda, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, g, G, sh, rr, pm, clk);
     }
     if (clk) {
         clk_outer(clk, 0);
@@ -266,11 +269,16 @@ __global__ void k_mfma_probe(double* out)
 // =========================================================================================
 using namespace skh;
 
-static const void* panel_kernel_for(int nbmax)
+static const void* panel_kernel_for(int nbmax, int multi)
 {
-    if (nbmax <= 32) return (const void*)k_panel<8>;
-    if (nbmax <= 64) return (const void*)k_panel<16>;
-    return (const void*)k_panel<32>;
+    if (multi) {
+        if (nbmax <= 32) return (const void*)k_panel<8, true>;
+        if (nbmax <= 64) return (const void*)k_panel<16, true>;
+        return (const void*)k_panel<32, true>;
+    }
+    if (nbmax <= 32) return (const void*)k_panel<8, false>;
+    if (nbmax <= 64) return (const void*)k_panel<16, false>;
+    return (const void*)k_panel<32, false>;
 }
 
 extern "C" int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st)
@@ -278,12 +286,17 @@ extern "C" int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st)
     void* args[] = {(void*)&pl->A,   (void*)&pl->lda, (void*)&pl->n,   (void*)&pl->c0,  (void*)&pl->nbp,
                     (void*)&pl->w,   (void*)&pl->tau, (void*)&pl->eta, (void*)&pl->bar, (void*)&pl->base,
                     (void*)&pl->err, (void*)&pl->ph};
-    return (int)hipLaunchCooperativeKernel(panel_kernel_for(pl->nbmax), dim3(pl->grid), dim3(WG), args, 0, st);
+    return (int)hipLaunchCooperativeKernel(panel_kernel_for(pl->nbmax, pl->multi), dim3(pl->grid), dim3(WG), args, 0,
+                                           st);
 }
 
 extern "C" int skh_internal_panel_occupancy(int nbmax, int* blocks_per_cu)
 {
-    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, panel_kernel_for(nbmax), WG, 0);
+    int a = 0, b = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, panel_kernel_for(nbmax, 0), WG, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, panel_kernel_for(nbmax, 1), WG, 0);
+    *blocks_per_cu = a < b ? a : b;
+    return (int)e;
 }
 
 extern "C" int skh_internal_launch_rank2k(double* A, int64_t lda, const double* U, int64_t ldu, int64_t n, int64_t e,

@@ -11,6 +11,7 @@ LIB_PATH = os.environ.get("SKEWHIP_LIB") or os.path.join(_HERE, "libskewhip.so")
 
 ENGINE_AUTO, ENGINE_LARGE, ENGINE_TEAM1 = 0, 1, 2
 FLAG_TIMING = 0x100
+FLAG_PHASES = 0x200
 
 
 class SkhError(RuntimeError):
