@@ -15,8 +15,11 @@ Also reported:
                 plan replayed from a HIP graph, matrices/s, against the per-matrix GPU loop
   grad          1024 matrices n = 256, log|Pf| + hand-written gradient, matrices/s, against the
                 per-matrix GPU loop
+  complex       ComplexF64 n = 4096 skew-Hermitian tridiagonalization, TFLOP/s, with Q formation and
+                the residual / orthogonality of A = Q T Q^H
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 8192] [--nb 0] [--no-cpu] [--no-batched]
+                       [--no-complex]
 """
 import argparse
 import ctypes
@@ -249,6 +252,56 @@ def grad_bench(torch, skl, _lib, steps):
             "speedup_vs_plain": (count / (ms * 1e-3)) / plain_rate}
 
 
+def complex_bench(torch, _lib, steps, n=4096, seed=314159):
+    """ComplexF64 n = 4096 skew-Hermitian A = (B - B^H)/2, Re and Im of B iid N(0,1): the device
+    tridiagonalization (the reference's skewblockedhess! on SkewHermitian{ComplexF64}) timed over
+    fresh HBM copies, then Q from the factors and the residual / orthogonality of A = Q T Q^H."""
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    B = torch.complex(torch.randn((n, n), generator=g, dtype=torch.float64, device=dev),
+                      torch.randn((n, n), generator=g, dtype=torch.float64, device=dev))
+    A = (B - B.conj().T) / 2
+    del B
+    A0 = A.T.contiguous()   # column-major storage of A
+    work = torch.empty_like(A0)
+    dtau = torch.empty(n - 1, dtype=torch.complex128, device=dev)
+    deta = torch.empty(n - 1, dtype=torch.float64, device=dev)
+    ddv = torch.empty(n, dtype=torch.float64, device=dev)
+    st = torch.cuda.current_stream().cuda_stream
+    L = _lib.lib()
+    o = _lib.opts()
+
+    def step():
+        work.copy_(A0)
+        _lib.check("skh_sytrd_z_device", L.skh_sytrd_z_device(n, work.data_ptr(), n, dtau.data_ptr(), deta.data_ptr(),
+                                                             ddv.data_ptr(), st, ctypes.byref(o)))
+
+    step()
+    torch.cuda.synchronize()
+    ms = timed(torch, step, max(3, steps))
+    dQ = torch.empty((n, n), dtype=torch.complex128, device=dev)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    _lib.check("skh_orgtr_z_device", L.skh_orgtr_z_device(n, work.data_ptr(), n, dtau.data_ptr(), dQ.data_ptr(), n, st))
+    e1.record()
+    torch.cuda.synchronize()
+    q_ms = e0.elapsed_time(e1)
+    Q = dQ.T.contiguous()
+    T = torch.diag(1j * ddv.to(torch.complex128))
+    k = torch.arange(n - 1, device=dev)
+    T[k + 1, k] = deta.to(torch.complex128)
+    T[k, k + 1] = -deta.to(torch.complex128)
+    nA = float(torch.linalg.norm(A))
+    res = float(torch.linalg.norm(Q @ T @ Q.conj().T - A)) / nA
+    orth = float(torch.linalg.norm(Q.conj().T @ Q - torch.eye(n, dtype=torch.complex128, device=dev)))
+    flops = 16.0 / 3.0 * n ** 3
+    return {"metric": "complex skew-Hermitian n=4096 tridiagonalization (ComplexF64)",
+            "value": flops / (ms * 1e-3) / 1e12, "unit": "TFLOP/s (16/3 n^3 real flops)", "ms": ms,
+            "q_formation_ms": q_ms, "residual_QTQh_minus_A_rel": res, "orthogonality_QhQ_minus_I": orth}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,6 +312,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched log|Pf| and gradient lines")
+    ap.add_argument("--no-complex", action="store_true", help="skip the complex n=4096 line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the single-matrix tridiagonalization is a one-GPU "
@@ -373,6 +427,10 @@ def main():
         log("batched gradient (1024 x 256) ...")
         res["grad"] = grad_bench(torch, skl, _lib, args.steps)
         log(json.dumps(res["grad"]))
+    if not args.no_complex:
+        log("complex n=4096 ...")
+        res["complex"] = complex_bench(torch, _lib, args.steps)
+        log(json.dumps(res["complex"]))
     if not args.no_cpu:
         log("cpu baseline (n=%d) ..." % args.cpu_n)
         res["cpu_baseline"] = cpu_baseline(args.cpu_n)

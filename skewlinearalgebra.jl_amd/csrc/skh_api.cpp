@@ -484,7 +484,14 @@ int zsytrd_device(int64_t n, double2* dA, int64_t lda, double2* dtau, double* de
         w.Pmu = w.Pnrm + gpad;
         w.alpha = (double2*)(w.Pmu + gpad);
         HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 512, st));
-        HIPCHK((hipError_t)skh_internal_launch_zfused(dA, lda, n, &w, dtau, deta, ctl.bar, ctl.err, grid, st));
+        // one launch per segment of columns whose trailing block starts in the same 64-row block
+        // (the tile grid, and so each workgroup's tiles, stay fixed within a launch)
+        for (int64_t kb = 0; kb <= n - 2;) {
+            const int64_t ke = std::min<int64_t>(n - 1, ((kb + 1) / skh::TB + 1) * skh::TB - 1);
+            HIPCHK((hipError_t)skh_internal_launch_zfused(dA, lda, n, &w, dtau, deta, ctl.bar, ctl.err, grid, kb, ke, st));
+            HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 288, st));
+            kb = ke;
+        }
     }
     HIPCHK((hipError_t)skh_internal_zdiag(dA, lda, n, ddvim, st));
     return 0;

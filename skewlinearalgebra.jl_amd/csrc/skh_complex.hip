@@ -12,17 +12,22 @@
 
 namespace skh {
 
+// Columns k in [k_begin, k_end) (k_end = n - 1 for the last segment); the norm partials and head
+// of column k_begin come from the previous launch (or zinit for k_begin = 0).
 __global__ __launch_bounds__(WG, 1) void k_zfused(double2* A, int64_t lda, int64_t n, zc::ZWork w, double2* tau,
-                                                  double* eta, unsigned* bar, int* err)
+                                                  double* eta, unsigned* bar, int* err, int64_t k_begin,
+                                                  int64_t k_end)
 {
     using namespace zc;
     __shared__ ZShared sh;
     __shared__ int s_ok;
     const int G = gridDim.x, g = blockIdx.x;
     unsigned epoch = 0;
-    zinit(A, lda, n, w, g, G);
-    if (!grid_sync(bar, ++epoch, err, &s_ok)) return;
-    for (int64_t k = 0; k <= n - 2; k++) {
+    if (k_begin == 0) {
+        zinit(A, lda, n, w, g, G);
+        if (!grid_sync_co(bar, ++epoch, err, &s_ok)) return;
+    }
+    for (int64_t k = k_begin; k < k_end; k++) {
         const ZScal s = zscalars(w, G, sh);
         if (g == 0 && threadIdx.x == 0) {
             st(tau + k, s.tau);
@@ -35,9 +40,9 @@ __global__ __launch_bounds__(WG, 1) void k_zfused(double2* A, int64_t lda, int64
             break;
         }
         zpass(A, lda, n, k, k >= 1, s, w, sh, g, G);
-        if (!grid_sync(bar, ++epoch, err, &s_ok)) return;
+        if (!grid_sync_co(bar, ++epoch, err, &s_ok)) return;
         zrows(A, lda, n, k, s, w, sh, g, G, G);
-        if (!grid_sync(bar, ++epoch, err, &s_ok)) return;
+        if (!grid_sync_co(bar, ++epoch, err, &s_ok)) return;
     }
 }
 
@@ -126,10 +131,12 @@ extern "C" int skh_internal_zfused_occupancy(int* blocks_per_cu)
 }
 
 extern "C" int skh_internal_launch_zfused(double2* A, int64_t lda, int64_t n, const zc::ZWork* w, double2* tau,
-                                          double* eta, unsigned* bar, int* err, int grid, hipStream_t st)
+                                          double* eta, unsigned* bar, int* err, int grid, int64_t k_begin,
+                                          int64_t k_end, hipStream_t st)
 {
     zc::ZWork wc = *w;
-    void* args[] = {(void*)&A, (void*)&lda, (void*)&n, (void*)&wc, (void*)&tau, (void*)&eta, (void*)&bar, (void*)&err};
+    void* args[] = {(void*)&A,   (void*)&lda, (void*)&n,   (void*)&wc,      (void*)&tau,
+                    (void*)&eta, (void*)&bar, (void*)&err, (void*)&k_begin, (void*)&k_end};
     return (int)hipLaunchCooperativeKernel((const void*)k_zfused, dim3(grid), dim3(WG), args, 0, st);
 }
 

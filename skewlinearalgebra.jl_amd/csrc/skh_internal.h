@@ -47,7 +47,7 @@ struct CompactArgs {
 extern "C" {
 int skh_internal_zfused_occupancy(int* blocks_per_cu);
 int skh_internal_launch_zfused(double2* A, int64_t lda, int64_t n, const skh::zc::ZWork* w, double2* tau, double* eta,
-                               unsigned* bar, int* err, int grid, hipStream_t st);
+                               unsigned* bar, int* err, int grid, int64_t k_begin, int64_t k_end, hipStream_t st);
 int skh_internal_zdiag(const double2* A, int64_t lda, int64_t n, double* dvim, hipStream_t st);
 int skh_internal_zform_v(const double2* F, int64_t ldf, int64_t n, int64_t c0, int nbp, double2* V, int64_t ldv,
                          hipStream_t st);

@@ -7,9 +7,13 @@
 // length-1 reflector of the last column is a full reflector in the complex case (it makes
 // eta[n-2] real and stores the unit in the factor).  T keeps the imaginary diagonal.
 //
-// This engine applies the reflectors one column at a time (unblocked), in one cooperative
-// persistent launch over the whole matrix (G co-resident workgroups, two grid barriers per
-// column).  Per reflector k (v_k from x_k = A[k+1:, k], scalars computed redundantly by every
+// This engine applies the reflectors one column at a time (unblocked), in cooperative persistent
+// launches of G co-resident workgroups, two grid barriers per column, one launch per segment of
+// columns over which the 64x64 tile grid of the trailing block is the same (k + 1 within one
+// 64-row block): each workgroup then updates and re-reads the same tiles for the whole launch,
+// so tile data stays inside the workgroup (plain cached accesses), and everything that crosses
+// workgroups (partials, the row pass's outputs, column k+1 and the diagonal of the tile update)
+// goes through agent-scope atomic accesses and the fence-free barrier.  Per reflector k (v_k from x_k = A[k+1:, k], scalars computed redundantly by every
 // workgroup from the same partials, bit-identical):
 //   P(k): tiles of the lower triangle (diagonal included) of A[k+1:, k+1:], split into G
 //         contiguous ranges: apply the previous reflector, a += v_{k-1} w_{k-1}^H - w_{k-1} v_{k-1}^H
@@ -44,9 +48,18 @@ __device__ __forceinline__ cd mulc(cd a, cd b) { return cd{a.re * b.re + a.im * 
 __device__ __forceinline__ cd ld(const double2* p) { const double2 t = *p; return cd{t.x, t.y}; }
 __device__ __forceinline__ void st(double2* p, cd v) { *p = make_double2(v.re, v.im); }
 
-__device__ __forceinline__ cd shfl_xor_c(cd x, int o)
+// Data written by one workgroup and read by another within a launch goes through agent-scope
+// atomic accesses (coherent across the XCD L2s; see ld_co/st_co), two doubles per complex entry.
+__device__ __forceinline__ cd ldc(const double2* p)
 {
-    return cd{__shfl_xor(x.re, o, 64), __shfl_xor(x.im, o, 64)};
+    const double* q = reinterpret_cast<const double*>(p);
+    return cd{ld_co(q), ld_co(q + 1)};
+}
+__device__ __forceinline__ void stc(double2* p, cd v)
+{
+    double* q = reinterpret_cast<double*>(p);
+    st_co(q, v.re);
+    st_co(q + 1, v.im);
 }
 
 // Reflector scalars (complex householder!): alpha = x[0], xn2 = ||x[1:]||^2.
@@ -94,7 +107,7 @@ __device__ __forceinline__ cd vval(const double2* __restrict__ A, int64_t lda, i
 {
     if (c <= k || c >= n) return cd{0.0, 0.0};
     if (c == k + 1) return cd{s.unit, 0.0};
-    return s.scale * ld(A + c + k * lda);
+    return s.scale * ldc(A + c + k * lda);
 }
 
 struct ZShared {
@@ -131,8 +144,8 @@ __device__ void zpass(double2* __restrict__ A, int64_t lda, int64_t n, int64_t k
                 rows[rr] = II * TB + 32 * (rr >> 1) + 2 * p + (rr & 1);
                 const int64_t r = rows[rr];
                 const bool in = r >= s0 && r < n;
-                rv[rr] = (upd && in) ? ld(vprev + r) : cd{0.0, 0.0};
-                rw[rr] = (upd && in) ? ld(w.Wv + r) : cd{0.0, 0.0};
+                rv[rr] = (upd && in) ? ldc(vprev + r) : cd{0.0, 0.0};
+                rw[rr] = (upd && in) ? ldc(w.Wv + r) : cd{0.0, 0.0};
                 rn[rr] = vval(A, lda, n, k, r, s);
                 racc[rr] = cd{0.0, 0.0};
             }
@@ -145,8 +158,8 @@ __device__ void zpass(double2* __restrict__ A, int64_t lda, int64_t n, int64_t k
             for (int cc = 0; cc < 4; cc++) {
                 const int64_t c = cb + cc;
                 const bool cin = c >= s0 && c < n;
-                cv[cc] = (upd && cin) ? ld(vprev + c) : cd{0.0, 0.0};
-                cw[cc] = (upd && cin) ? ld(w.Wv + c) : cd{0.0, 0.0};
+                cv[cc] = (upd && cin) ? ldc(vprev + c) : cd{0.0, 0.0};
+                cw[cc] = (upd && cin) ? ldc(w.Wv + c) : cd{0.0, 0.0};
                 cn[cc] = vval(A, lda, n, k, c, s);
                 cacc[cc] = cd{0.0, 0.0};
 #pragma unroll
@@ -165,7 +178,11 @@ __device__ void zpass(double2* __restrict__ A, int64_t lda, int64_t n, int64_t k
                         const int64_t r = rows[rr];
                         if (c >= s0 && c < n && r >= c && r < n) {
                             a[rr][cc] = a[rr][cc] + mulc(rv[rr], cw[cc]) - mulc(rw[rr], cv[cc]);
-                            st(A + r + c * lda, a[rr][cc]);
+                            // column s0 (read by the row pass) and the diagonal (read by the row
+                            // pass and the last reflector) leave this workgroup; the rest of the
+                            // tile is read again only by it, later in the same launch
+                            if (c == s0 || r == c) stc(A + r + c * lda, a[rr][cc]);
+                            else st(A + r + c * lda, a[rr][cc]);
                         }
                     }
                 }
@@ -191,13 +208,9 @@ __device__ void zpass(double2* __restrict__ A, int64_t lda, int64_t n, int64_t k
             }
 #pragma unroll
             for (int cc = 0; cc < 4; cc++) {
-                cd x = cacc[cc];
-                x = x + shfl_xor_c(x, 1);
-                x = x + shfl_xor_c(x, 2);
-                x = x + shfl_xor_c(x, 4);
-                x = x + shfl_xor_c(x, 8);
+                const cd x = cd{row16_sum(cacc[cc].re), row16_sum(cacc[cc].im)};
                 const int64_t c = cb + cc;
-                if (p == 0 && c >= s0 && c < n) st(w.Pcol + I * w.ldp + c, x);
+                if (p == 0 && c >= s0 && c < n) stc(w.Pcol + I * w.ldp + c, x);
             }
             const bool more = t + 1 < thi;
             const int64_t In = (J < I) ? I : I + 1;
@@ -205,10 +218,7 @@ __device__ void zpass(double2* __restrict__ A, int64_t lda, int64_t n, int64_t k
             if (!more || In != I) {
 #pragma unroll
                 for (int rr = 0; rr < 4; rr++) {
-                    cd x = racc[rr];
-                    x = x + shfl_xor_c(x, 16);
-                    x = x + shfl_xor_c(x, 32);
-                    racc[rr] = x;
+                    racc[rr] = cd{xor32_sum(xor16_sum(racc[rr].re)), xor32_sum(xor16_sum(racc[rr].im))};
                 }
                 if (cg == 0) {
                     sh.red[wave][2 * p] = racc[0];
@@ -220,8 +230,8 @@ __device__ void zpass(double2* __restrict__ A, int64_t lda, int64_t n, int64_t k
                 if (wave == 0) {
                     const int64_t r = I * TB + lane;
                     if (r < n)
-                        st(w.Prow + cur_slot * w.ldp + r,
-                           ((sh.red[0][lane] + sh.red[1][lane]) + (sh.red[2][lane] + sh.red[3][lane])));
+                        stc(w.Prow + cur_slot * w.ldp + r,
+                            ((sh.red[0][lane] + sh.red[1][lane]) + (sh.red[2][lane] + sh.red[3][lane])));
                 }
                 __syncthreads();
                 cur_slot = 0;
@@ -234,7 +244,7 @@ __device__ void zpass(double2* __restrict__ A, int64_t lda, int64_t n, int64_t k
     mu = wave_sum(mu);
     if (lane == 0) sh.sred[wave] = mu;
     __syncthreads();
-    if (tid == 0) w.Pmu[g] = (sh.sred[0] + sh.sred[1]) + (sh.sred[2] + sh.sred[3]);
+    if (tid == 0) st_co(w.Pmu + g, (sh.sred[0] + sh.sred[1]) + (sh.sred[2] + sh.sred[3]));
     __syncthreads();
 }
 
@@ -248,8 +258,8 @@ __device__ __forceinline__ cd zrow_partial(const ZWork& w, int64_t r, int64_t b,
 {
     cd acc = cd{0.0, 0.0};
     const int64_t nsl = zslots(b - b0, nt - b0, G);
-    for (int64_t s2 = wave; s2 < nsl; s2 += 4) acc = acc + ld(w.Prow + s2 * w.ldp + r);
-    for (int64_t I = b + wave; I < nt; I += 4) acc = acc + ld(w.Pcol + I * w.ldp + r);
+    for (int64_t s2 = wave; s2 < nsl; s2 += 4) acc = acc + ldc(w.Prow + s2 * w.ldp + r);
+    for (int64_t I = b + wave; I < nt; I += 4) acc = acc + ldc(w.Pcol + I * w.ldp + r);
     return acc;
 }
 
@@ -268,7 +278,7 @@ __device__ void zrows(double2* __restrict__ A, int64_t lda, int64_t n, int64_t k
     // m = sum of the pass partials (every workgroup wrote one), fixed order
     if (wave == 0) {
         double m = 0.0;
-        for (int q = lane; q < Gfull; q += 64) m += w.Pmu[q];
+        for (int q = lane; q < Gfull; q += 64) m += ld_co(w.Pmu + q);
         m = wave_sum(m);
         if (lane == 0) sh.mu = m;
     }
@@ -296,17 +306,17 @@ __device__ void zrows(double2* __restrict__ A, int64_t lda, int64_t n, int64_t k
             const cd y = (sh.red[0][lane] + sh.red[1][lane]) + (sh.red[2][lane] + sh.red[3][lane]);
             const cd vr = vval(A, lda, n, k, r, s);
             const cd wr = (r == s0) ? wspec : (s.tau * y + c * vr);
-            st(w.Wv + r, wr);
-            st(A + r + k * lda, vr);   // factor (unit at k+1)
+            stc(w.Wv + r, wr);
+            stc(A + r + k * lda, vr);   // factor (unit at k+1)
             if (r == s0) {
                 // diagonal (k+1, k+1): += v conj(w) - w conj(v)
-                const cd d = ld(A + r + r * lda);
-                st(A + r + r * lda, d + mulc(vr, wr) - mulc(wr, vr));
+                const cd d = ldc(A + r + r * lda);
+                stc(A + r + r * lda, d + mulc(vr, wr) - mulc(wr, vr));
             } else {
                 // next column: x_r = A[r, k+1] + v_r conj(w_{k+1}) - w_r conj(v_{k+1})
-                const cd x = ld(A + r + (k + 1) * lda) + mulc(vr, wspec) - mulc(wr, vspec);
-                st(A + r + (k + 1) * lda, x);
-                if (r == k + 2) st(w.alpha, x);
+                const cd x = ldc(A + r + (k + 1) * lda) + mulc(vr, wspec) - mulc(wr, vspec);
+                stc(A + r + (k + 1) * lda, x);
+                if (r == k + 2) stc(w.alpha, x);
                 else nrm += norm2(x);
             }
         }
@@ -314,7 +324,7 @@ __device__ void zrows(double2* __restrict__ A, int64_t lda, int64_t n, int64_t k
     }
     if (wave == 0) {
         nrm = wave_sum(nrm);
-        if (lane == 0) w.Pnrm[g] = nrm;
+        if (lane == 0) st_co(w.Pnrm + g, nrm);
     }
 }
 
@@ -323,12 +333,12 @@ __device__ void zrows(double2* __restrict__ A, int64_t lda, int64_t n, int64_t k
 // no pass covers.
 __device__ inline void zlast(double2* __restrict__ A, int64_t lda, int64_t n, const ZWork& w, const ZScal& s)
 {
-    st(A + (n - 1) + (n - 2) * lda, cd{s.unit, 0.0});
+    stc(A + (n - 1) + (n - 2) * lda, cd{s.unit, 0.0});
     if (n >= 3) {
-        const cd v = ld(A + (n - 1) + (n - 3) * lda);
-        const cd wv = ld(w.Wv + (n - 1));
-        const cd d = ld(A + (n - 1) + (n - 1) * lda);
-        st(A + (n - 1) + (n - 1) * lda, d + mulc(v, wv) - mulc(wv, v));
+        const cd v = ldc(A + (n - 1) + (n - 3) * lda);
+        const cd wv = ldc(w.Wv + (n - 1));
+        const cd d = ldc(A + (n - 1) + (n - 1) * lda);
+        stc(A + (n - 1) + (n - 1) * lda, d + mulc(v, wv) - mulc(wv, v));
     }
 }
 
@@ -343,8 +353,8 @@ __device__ void zinit(const double2* __restrict__ A, int64_t lda, int64_t n, con
     if (lane == 0) red[wave] = nrm;
     __syncthreads();
     if (tid == 0) {
-        w.Pnrm[g] = (red[0] + red[1]) + (red[2] + red[3]);
-        if (g == 0) st(w.alpha, n >= 2 ? ld(A + 1) : cd{0.0, 0.0});
+        st_co(w.Pnrm + g, (red[0] + red[1]) + (red[2] + red[3]));
+        if (g == 0) stc(w.alpha, n >= 2 ? ld(A + 1) : cd{0.0, 0.0});
     }
     __syncthreads();
 }
@@ -355,12 +365,12 @@ __device__ ZScal zscalars(const ZWork& w, int Gfull, ZShared& sh)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (wave == 0) {
         double a = 0.0;
-        for (int q = lane; q < Gfull; q += 64) a += w.Pnrm[q];
+        for (int q = lane; q < Gfull; q += 64) a += ld_co(w.Pnrm + q);
         a = wave_sum(a);
         if (lane == 0) sh.sred[0] = a;
     }
     __syncthreads();
-    const ZScal s = zreflector(ld(w.alpha), sh.sred[0]);
+    const ZScal s = zreflector(ldc(w.alpha), sh.sred[0]);
     __syncthreads();
     (void)tid;
     return s;
