@@ -253,13 +253,27 @@ __device__ __forceinline__ int64_t zslots(int64_t sb, int64_t ns, int64_t G) { r
 
 // y_r for one row r of strip b (b0 = first strip): row slots + column partials of strips >= b.
 // Wave-split sums (slots and strips by wave), combined by the caller in a fixed order.
+// The row's partials are numbered q = 0 .. nsl + nt - b - 1 (row slots first, then the column
+// partials of strips b ..); wave w takes q = w, w + 4, ... in batches of 8 loads, so a row costs a
+// few memory round trips instead of one per partial.
 __device__ __forceinline__ cd zrow_partial(const ZWork& w, int64_t r, int64_t b, int64_t b0, int64_t nt, int64_t G,
                                            int wave)
 {
+    constexpr int ZB = 8;
     cd acc = cd{0.0, 0.0};
     const int64_t nsl = zslots(b - b0, nt - b0, G);
-    for (int64_t s2 = wave; s2 < nsl; s2 += 4) acc = acc + ldc(w.Prow + s2 * w.ldp + r);
-    for (int64_t I = b + wave; I < nt; I += 4) acc = acc + ldc(w.Pcol + I * w.ldp + r);
+    const int64_t nall = nsl + (nt - b);
+    for (int64_t q0 = wave; q0 < nall; q0 += 4 * ZB) {
+        cd buf[ZB];
+#pragma unroll
+        for (int u = 0; u < ZB; u++) {
+            const int64_t q = q0 + 4 * u;
+            buf[u] = (q < nsl) ? ldc(w.Prow + q * w.ldp + r)
+                               : ((q < nall) ? ldc(w.Pcol + (b + q - nsl) * w.ldp + r) : cd{0.0, 0.0});
+        }
+#pragma unroll
+        for (int u = 0; u < ZB; u++) acc = acc + buf[u];
+    }
     return acc;
 }
 
