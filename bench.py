@@ -379,9 +379,9 @@ def main():
     _lib.check("skh_diag_timings", L.skh_diag_timings(out6))
     phases = {"note": "separate instrumented factorization (phase clock on); ms of workgroup 0",
               "factorization_ms_with_clock": out6[0] + out6[2]}
-    phases.update(dict(zip(("row_pass", "barrier1_arrive", "phase_b", "barrier2", "row_loads", "row_correct_w",
-                            "row_update_x", "row_handover", "b_loads_scalars", "b_t", "b_matvec", "b_first_tile",
-                            "b_wait"), clk[:13])))
+    phases.update(dict(zip(("first_row_pass", "barrier1_arrive", "phase_b", "row_pass", "row_loads_and_wait",
+                            "row_correct_w", "row_update_x", "row_handover", "b_loads_scalars", "b_t", "b_matvec",
+                            "b_first_tile", "b_wait"), clk[:13])))
     panel_avg_s = panel_ms / panel_cnt * 1e-3
     panel_bytes_per = panel_bytes / panel_cnt
     achieved = panel_bytes_per / panel_avg_s / 1e9

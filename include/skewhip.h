@@ -193,15 +193,16 @@ int skh_skewtrieigen_d(int64_t n, double *ev, double *vals, double *vr, double *
 
 /* Diagnostics: per-kernel times of the last skh_sytrd_d_device call made with SKH_FLAG_TIMING:
  * out10 = {panel ms, panel launches, rank-2k ms, rank-2k launches, panel algorithmic bytes,
- *          rank-2k algorithmic flops, then panel-kernel phase times of workgroup 0 in ms:
- *          row passes, barrier-1 waits, scalars + matvec, barrier-2 waits}.
+ *          rank-2k algorithmic flops, then the first four phase-clock values of
+ *          skh_diag_phase_clock (zero unless SKH_FLAG_PHASES was set)}.
  * Waits for the recorded events. */
 int skh_diag_timings(double *out10);
 /* Diagnostics: the panel kernel's phase clock of the last SKH_FLAG_PHASES factorization, in ms
- * (workgroup 0, summed over its launches): [0..3] as in skh_diag_timings (barrier 1 arrivals in
- * [1], its wait inside phase B [2]), then the row pass split into [4] loads, [5] correction of w,
- * [6] column update and partials, [7] hand-over, and phase B into [11] tile range and first tile,
- * [12] barrier-1 wait, [8] loads and scalars, [9] t, [10] matvec.  count <= 13 values are written. */
+ * (workgroup 0, summed over its launches): [0] each launch's first row pass, [1] barrier arrivals,
+ * [2] phase B (with the barrier wait), [3] row passes (with the wait for the matvec partials), then
+ * the row pass split into [4] loads and that wait, [5] correction of w, [6] column update and
+ * partials, [7] hand-over, and phase B into [11] first tile, [12] barrier wait, [8] loads and
+ * scalars, [9] t, [10] matvec.  count <= 13 values are written. */
 int skh_diag_phase_clock(double *out, int count);
 /* Diagnostics: v_mfma_f64_16x16x4f64 lane-layout probe (768 doubles, see the test suite). */
 int skh_diag_mfma_probe(double *out768);

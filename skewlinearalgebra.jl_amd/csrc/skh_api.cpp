@@ -188,18 +188,25 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
     HIPCHK((hipError_t)skh_internal_panel_occupancy((int)nb, &per_cu0));
     // row-partial slots of one matvec strip: at most one per workgroup of the grid
     const int64_t nseg = (int64_t)std::max(1, std::min(per_cu0, 2)) * g_num_cu + 2;
-    size_t doubles = (size_t)(ld * nb + nt + nt * 2 * skh::NBMAX + 2 * skh::NBMAX + nseg * ld + nt * ld + 64);
+    // W | Pnorm x2 | PS x2 | X x2 | Tv | Prow | Pcol (Tv, Prow, Pcol contiguous: cleared per launch)
+    size_t doubles = (size_t)(ld * nb + 2 * nt + 2 * nt * 2 * skh::NBMAX + 2 * ld + 2 * skh::NBMAX + nseg * ld +
+                              nt * ld + 64);
     double* ws = (double*)g_ws.get(doubles * sizeof(double));
     if (!ws) return set_err(SKH_ERR_HIP, "hipMalloc of %zu bytes of workspace failed", doubles * 8);
     skh::Work w;
     w.W = ws;
     w.ldw = ld;
     w.Pnorm = w.W + ld * nb;
-    w.PS = w.Pnorm + nt;
-    w.Tv = w.PS + nt * 2 * skh::NBMAX;
+    w.pstride = nt;
+    w.PS = w.Pnorm + 2 * nt;
+    w.sstride = nt * 2 * skh::NBMAX;
+    w.X = w.PS + 2 * w.sstride;
+    w.xstride = ld;
+    w.Tv = w.X + 2 * ld;
     w.Prow = w.Tv + 2 * skh::NBMAX;
     w.ldp = ld;
     w.Pcol = w.Prow + nseg * ld;
+    const int64_t nslots = nseg + nt;   // Prow and Pcol slots, ld doubles each, after Tv
 
     int per_cu = 0;
     HIPCHK((hipError_t)skh_internal_panel_occupancy((int)nb, &per_cu));
@@ -253,6 +260,8 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
             pl.multi = (int64_t)pl.grid < blocks ? 1 : 0;
         }
         pl.ph = g_timer.phases ? ctl.ph : nullptr;
+        // tagged hand-over values of the launch start untagged: t and rows >= c0 of the partials
+        HIPCHK((hipError_t)skh_internal_clear_partials(w.Tv, 2 * skh::NBMAX, w.Prow, ld, c0, n, nslots, st));
         const bool tm = g_timer.on;
         if (tm) HIPCHK((hipError_t)g_timer.begin(st));
         HIPCHK((hipError_t)skh_internal_launch_panel(&pl, st));

@@ -34,7 +34,17 @@ struct Work {
     double* Prow;    // [nt][ldp]       row partials of the skew matvec (one slot per rank per strip)
     double* Pcol;    // [nt][ldp]       column partials of the skew matvec
     int64_t ldp;
+    // Pnorm, PS and X hold one copy per column parity: the row pass of column j + 1 writes them
+    // while a slower workgroup may still read the copy of column j in its phase B
+    int64_t pstride; // Pnorm of the other parity at Pnorm + pstride (nq entries)
+    int64_t sstride; // PS of the other parity at PS + sstride
+    double* X;       // [2][xstride]    x_j, the updated column j (unscaled reflector), by parity
+    int64_t xstride;
 };
+
+__device__ __forceinline__ double* pnorm_of(const Work& w, int64_t j) { return w.Pnorm + (j & 1) * w.pstride; }
+__device__ __forceinline__ double* ps_of(const Work& w, int64_t j) { return w.PS + (j & 1) * w.sstride; }
+__device__ __forceinline__ double* x_of(const Work& w, int64_t j) { return w.X + (j & 1) * w.xstride; }
 
 // Per-column scalars kept in LDS across the phases.
 struct ColScal {
@@ -201,6 +211,29 @@ __device__ __forceinline__ void st_co(double* p, double v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Values phase B of column j hands to the next row pass (the matvec partial sums and t) carry the
+// tag of column j in their three low mantissa bits (1..4; 0 = not yet written in this launch,
+// the buffers are zero-filled before every launch).  The row-block owners poll these values
+// instead of waiting at a grid barrier.  Within a launch every value a row pass reads is
+// rewritten at every column, so a slot holds the tag of column j or of j - 1.  The tag changes a
+// value by at most 7 units in the last place (1.6e-15 relative), the size of one rounding of
+// the sums it enters; readers clear the tag bits before use.
+__device__ __forceinline__ unsigned col_tag(int64_t j) { return (unsigned)(j & 3) + 1u; }
+__device__ __forceinline__ double with_tag(double x, unsigned t)
+{
+    return __builtin_bit_cast(double, (__builtin_bit_cast(unsigned long long, x) & ~7ull) | t);
+}
+__device__ __forceinline__ bool has_tag(double x, unsigned t)
+{
+    return (__builtin_bit_cast(unsigned long long, x) & 7ull) == t;
+}
+// the value with its tag bits cleared: a tagged zero reads back as exactly zero (the reflector
+// conventions for zero columns depend on it)
+__device__ __forceinline__ double untag(double x)
+{
+    return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, x) & ~7ull);
+}
+
 // ---------------------------------------------------------------------------------------
 // Reflector scalars from alpha = x[0] and the sum of squares of x[1:] (householder!,
 // src/hessenberg.jl:26-44).  beta = (alpha > 0 ? -1 : +1) * hypot(alpha, xnorm),
@@ -225,13 +258,14 @@ __device__ __forceinline__ ColScal reflector_scalars(double alpha, double xn2)
     return s;
 }
 
-// v_c of the current reflector (column j): 0 outside [j+1, n), unit at j+1, scale*x elsewhere.
-__device__ __forceinline__ double vval(const double* __restrict__ A, int64_t lda, int64_t n, int64_t j,
-                                       int64_t c, const ColScal& s)
+// v_c of the current reflector (column j, x_j in xj): 0 outside [j+1, n), unit at j+1,
+// scale*x elsewhere.
+__device__ __forceinline__ double vval(const double* __restrict__ xj, int64_t n, int64_t j, int64_t c,
+                                       const ColScal& s)
 {
     if (c <= j || c >= n) return 0.0;
     if (c == j + 1) return s.unit;
-    return s.scale * ld_co(A + c + j * lda);
+    return s.scale * ld_co(xj + c);
 }
 
 // Ownership of the matvec tiles (strip-major order over the lower triangle): T tiles split
@@ -398,7 +432,7 @@ struct RowRegs {
 template <int KQ, bool MULTI>
 __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j, int i,
                          bool doC, bool doA, const Work& w, int g, int G, RowShared& sh, RowRegs<KQ>& rr,
-                         const PanelMap& pm, unsigned long long* clk)
+                         const PanelMap& pm, int* err, unsigned long long* clk)
 {
     clk_begin(clk);
     const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
@@ -427,26 +461,10 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
         const bool rowC = doC && rin && r >= j;
         // ---- 1. every global load of this step, into registers ----
         double tv1 = 0.0, tv2 = 0.0, wraw = 0.0, vraw = 0.0, sprow = 0.0, sp[4] = {0.0, 0.0, 0.0, 0.0};
-        if (first) {
-            if (doC && tid < im) {
-                tv1 = ld_co(w.Tv + tid);
-                tv2 = ld_co(w.Tv + NBMAX + tid);
-            }
-            if (tid < i) {
-                wraw = ld_co(Wm + j + tid * ldw);
-                vraw = ld_co(A + j + (c0 + tid) * lda);
-            }
-            if (doC && wave == 0) {
-                // special row r = j of w_{j-1}: no stored entry of row j lies in the block, so its row
-                // sums are empty; column partials of the strips I >= j / MR
-                const int S0 = (int)(j / MR), nts = (int)tg.nts, ldp = (int)w.ldp, j32 = (int)j;
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int I = S0 + lane + 64 * u;
-                    sp[u] = (I < nts) ? ld_co(w.Pcol + (I * ldp + j32)) : 0.0;
-                }
-                for (int I = S0 + lane + 256; I < nts; I += 64) sprow += ld_co(w.Pcol + (I * ldp + j32));
-            }
+        if (first && tid < i) {
+            // rows j of W and V: written by earlier row passes, before the last grid barrier
+            wraw = ld_co(Wm + j + tid * ldw);
+            vraw = ld_co(A + j + (c0 + tid) * lda);
         }
         double vreg[KQ], wreg[KQ], xo, xold;
         if (own) {
@@ -465,32 +483,78 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                 vreg[u] = ok ? ld_co(A + r + (c0 + k) * lda) : 0.0;
                 wreg[u] = ok ? ld_co(Wm + r + k * ldw) : 0.0;
             }
-            xo = rowC ? ld_co(A + r + jm * lda) : 0.0;        // x of column j-1 (unscaled)
+            xo = rowC ? ld_co(x_of(w, jm) + r) : 0.0;         // x of column j-1 (unscaled)
             xold = (doA && rin && r > j) ? A[r + j * lda] : 0.0;
         }
         double pp = 0.0;
-        if (rowC) {
-            // 32-bit element offsets: (slots + strips) * ldp stays far below 2^31 for any matrix in HBM
+        if (doC) {
+            // t, the special row's column partials and this row's matvec partials, written by
+            // phase B of column j-1 with its tag: each wave polls its share until all have arrived
+            // (32-bit element offsets: (slots + strips) * ldp stays far below 2^31)
+            const unsigned tg1 = col_tag(jm);
             const int S = (int)((b * TB) / MR);    // matvec strip holding this chunk
-            const int nslot = (int)(own ? pm.nslot_own : tall_slots(tg, S, G));
+            const int nslot = rowC ? (int)(own ? pm.nslot_own : tall_slots(tg, S, G)) : 0;
             const int nts = (int)tg.nts, ldp = (int)w.ldp, rr32 = (int)r;
-            for (int s0 = wave; s0 < nslot; s0 += 32) {
-                double sb[8];
-#pragma unroll
-                for (int u = 0; u < 8; u++) sb[u] = (s0 + 4 * u < nslot) ? ld_co(w.Prow + ((s0 + 4 * u) * ldp + rr32)) : 0.0;
-                pp += ((sb[0] + sb[1]) + (sb[2] + sb[3])) + ((sb[4] + sb[5]) + (sb[6] + sb[7]));
-            }
-            for (int I0 = S + wave; I0 < nts; I0 += 4 * PB) {
-                double buf[PB];
-#pragma unroll
-                for (int u = 0; u < PB; u++) {
-                    const int I = I0 + 4 * u;
-                    buf[u] = (I < nts) ? ld_co(w.Pcol + (I * ldp + rr32)) : 0.0;
+            const int S0 = (int)(j / MR), j32 = (int)j;
+            for (unsigned spins = 0;; spins++) {
+                bool ok = true;
+                if (first && tid < im) {
+                    tv1 = ld_co(w.Tv + tid);
+                    tv2 = ld_co(w.Tv + NBMAX + tid);
+                    ok = has_tag(tv1, tg1) && has_tag(tv2, tg1);
+                    tv1 = untag(tv1);
+                    tv2 = untag(tv2);
                 }
-                double p4[4] = {0.0, 0.0, 0.0, 0.0};
+                if (first && wave == 0) {
+                    // special row r = j of w_{j-1}: no stored entry of row j lies in the block, so
+                    // its row sums are empty; column partials of the strips I >= j / MR
 #pragma unroll
-                for (int u = 0; u < PB; u++) p4[u & 3] += buf[u];
-                pp += (p4[0] + p4[1]) + (p4[2] + p4[3]);
+                    for (int u = 0; u < 4; u++) {
+                        const int I = S0 + lane + 64 * u;
+                        sp[u] = (I < nts) ? ld_co(w.Pcol + (I * ldp + j32)) : 0.0;
+                        ok = ok && (I >= nts || has_tag(sp[u], tg1));
+                        sp[u] = untag(sp[u]);
+                    }
+                    sprow = 0.0;
+                    for (int I = S0 + lane + 256; I < nts; I += 64) {
+                        const double x = ld_co(w.Pcol + (I * ldp + j32));
+                        ok = ok && has_tag(x, tg1);
+                        sprow += untag(x);
+                    }
+                }
+                if (rowC) {
+                    pp = 0.0;
+                    for (int s0 = wave; s0 < nslot; s0 += 32) {
+                        double sb[8];
+#pragma unroll
+                        for (int u = 0; u < 8; u++) {
+                            sb[u] = (s0 + 4 * u < nslot) ? ld_co(w.Prow + ((s0 + 4 * u) * ldp + rr32)) : 0.0;
+                            ok = ok && (s0 + 4 * u >= nslot || has_tag(sb[u], tg1));
+                            sb[u] = untag(sb[u]);
+                        }
+                        pp += ((sb[0] + sb[1]) + (sb[2] + sb[3])) + ((sb[4] + sb[5]) + (sb[6] + sb[7]));
+                    }
+                    for (int I0 = S + wave; I0 < nts; I0 += 4 * PB) {
+                        double buf[PB];
+#pragma unroll
+                        for (int u = 0; u < PB; u++) {
+                            const int I = I0 + 4 * u;
+                            buf[u] = (I < nts) ? ld_co(w.Pcol + (I * ldp + rr32)) : 0.0;
+                            ok = ok && (I >= nts || has_tag(buf[u], tg1));
+                            buf[u] = untag(buf[u]);
+                        }
+                        double p4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                        for (int u = 0; u < PB; u++) p4[u & 3] += buf[u];
+                        pp += (p4[0] + p4[1]) + (p4[2] + p4[3]);
+                    }
+                }
+                if (__all(ok)) break;
+                if (spins > SPIN_LIMIT) {   // a lost producer: flag it, the next barrier ends the launch
+                    if (lane == 0) __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
             }
         }
         // ---- 2. shared inputs into LDS (first chunk), the special row of w_{j-1} ----
@@ -564,11 +628,11 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
             sh.red2[wave][lane] = (p4[0] + p4[1]) + (p4[2] + p4[3]);
             __syncthreads();
             if (live) x = xold + (sh.red2[0][lane] + sh.red2[1][lane] + sh.red2[2][lane] + sh.red2[3][lane]);
-            if (i > 0 && live && wave == 0) st_co(A + r + j * lda, x);
+            if (live && wave == 0) st_co(x_of(w, j) + r, x);   // column j keeps its old entries until v
             const bool tail = rin && r >= j + 2;
             if (wave == 0) {
                 const double ss = wave_sum(tail ? x * x : 0.0);
-                if (lane == 0) st_co(w.Pnorm + q, ss);
+                if (lane == 0) st_co(pnorm_of(w, j) + q, ss);
             }
             // S partials of column j: sum over the chunk's rows r >= j+2 of W[r,k] x_r, V[r,k] x_r
             const double xt = tail ? x : 0.0;
@@ -584,7 +648,7 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                 const int idx = (lane >> 1) & 31;
                 const int u = idx & 15;
                 if ((lane & 1) == 0 && k0 + u < k1)
-                    st_co(w.PS + q * (2 * NBMAX) + (idx < 16 ? 0 : NBMAX) + k0 + u, tot);
+                    st_co(ps_of(w, j) + q * (2 * NBMAX) + (idx < 16 ? 0 : NBMAX) + k0 + u, tot);
             } else {
 #pragma unroll
                 for (int u = 0; u < KQ; u++) {
@@ -592,8 +656,8 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                         const double s1 = wave_sum(wreg[u] * xt);
                         const double s2 = wave_sum(vreg[u] * xt);
                         if (lane == 0) {
-                            st_co(w.PS + q * (2 * NBMAX) + k0 + u, s1);
-                            st_co(w.PS + q * (2 * NBMAX) + NBMAX + k0 + u, s2);
+                            st_co(ps_of(w, j) + q * (2 * NBMAX) + k0 + u, s1);
+                            st_co(ps_of(w, j) + q * (2 * NBMAX) + NBMAX + k0 + u, s2);
                         }
                     }
                 }
@@ -658,10 +722,10 @@ __device__ __forceinline__ void tall_load(const double* __restrict__ A, int64_t 
     }
 }
 
-// unscaled x_c of column j where v_c = scale * x_c (0 elsewhere)
-__device__ __forceinline__ double xraw(const double* __restrict__ A, int64_t lda, int64_t n, int64_t j, int64_t c)
+// unscaled x_c of column j (x_j in xj) where v_c = scale * x_c (0 elsewhere)
+__device__ __forceinline__ double xraw(const double* __restrict__ xj, int64_t n, int64_t j, int64_t c)
 {
-    return (c > j + 1 && c < n) ? ld_co(A + c + j * lda) : 0.0;
+    return (c > j + 1 && c < n) ? ld_co(xj + c) : 0.0;
 }
 
 // v_c from its unscaled x_c
@@ -697,27 +761,27 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
     // ---- loads of what the row pass produced ----
     double pn0 = 0.0, pn1 = 0.0, araw = 0.0;
     if (wave == 0) {
-        pn0 = (lane < nq) ? ld_co(w.Pnorm + lane) : 0.0;
-        pn1 = (lane + 64 < nq) ? ld_co(w.Pnorm + lane + 64) : 0.0;
-        if (lane == 0) araw = ld_co(A + (j + 1) + j * lda);
+        pn0 = (lane < nq) ? ld_co(pnorm_of(w, j) + lane) : 0.0;
+        pn1 = (lane + 64 < nq) ? ld_co(pnorm_of(w, j) + lane + 64) : 0.0;
+        if (lane == 0) araw = ld_co(x_of(w, j) + (j + 1));
     }
     const int tv0 = g * 4 + wave;
     double ps0 = 0.0, ps1 = 0.0, head0 = 0.0;
     if (tv0 < 2 * i) {
         const int k = tv0 % i;
         const int off = (tv0 < i) ? 0 : NBMAX;
-        ps0 = (lane < nq) ? ld_co(w.PS + (int64_t)lane * (2 * NBMAX) + off + k) : 0.0;
-        ps1 = (lane + 64 < nq) ? ld_co(w.PS + (int64_t)(lane + 64) * (2 * NBMAX) + off + k) : 0.0;
+        ps0 = (lane < nq) ? ld_co(ps_of(w, j) + (int64_t)lane * (2 * NBMAX) + off + k) : 0.0;
+        ps1 = (lane + 64 < nq) ? ld_co(ps_of(w, j) + (int64_t)(lane + 64) * (2 * NBMAX) + off + k) : 0.0;
         if (lane == 0) head0 = (tv0 < i) ? ld_co(w.W + (j + 1) + k * w.ldw) : ld_co(A + (j + 1) + (c0 + k) * lda);
     }
     if (mv) {
-        xc = xraw(A, lda, n, j, c);
-        xs = xraw(A, lda, n, j, I * MR + tid);   // this thread's row of the strip
+        xc = xraw(x_of(w, j), n, j, c);
+        xs = xraw(x_of(w, j), n, j, I * MR + tid);   // this thread's row of the strip
     }
     // ---- (1) scalars ----
     if (wave == 0) {
         double acc = pn0 + pn1;
-        for (int64_t q = lane + 128; q < nq; q += 64) acc += ld_co(w.Pnorm + q);
+        for (int64_t q = lane + 128; q < nq; q += 64) acc += ld_co(pnorm_of(w, j) + q);
         acc = wave_sum(acc);
         if (lane == 0) {
             const ColScal s = reflector_scalars(araw, acc);
@@ -742,12 +806,12 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
             head = head0;
         } else {
             acc = 0.0;
-            for (int64_t q = lane; q < 128 && q < nq; q += 64) acc += ld_co(w.PS + q * (2 * NBMAX) + off + k);
+            for (int64_t q = lane; q < 128 && q < nq; q += 64) acc += ld_co(ps_of(w, j) + q * (2 * NBMAX) + off + k);
             head = (lane == 0) ? ((tv < i) ? ld_co(w.W + (j + 1) + k * w.ldw) : ld_co(A + (j + 1) + (c0 + k) * lda)) : 0.0;
         }
-        for (int64_t q = lane + 128; q < nq; q += 64) acc += ld_co(w.PS + q * (2 * NBMAX) + off + k);
+        for (int64_t q = lane + 128; q < nq; q += 64) acc += ld_co(ps_of(w, j) + q * (2 * NBMAX) + off + k);
         acc = wave_sum(acc);
-        if (lane == 0) st_co(w.Tv + off + k, s.unit * head + s.scale * acc);
+        if (lane == 0) st_co(w.Tv + off + k, with_tag(s.unit * head + s.scale * acc, col_tag(j)));
     }
     clk_inner(clk, 9);
     // ---- (3) matvec ----
@@ -765,9 +829,9 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         const int64_t cn = Jn * MC + 4 * wave + cg;
         double an[16], vcn = 0.0, xsn = 0.0;
         if (more) {
-            vcn = vval(A, lda, n, j, cn, s);
+            vcn = vval(x_of(w, j), n, j, cn, s);
             tall_load(A, lda, n, j, In, cn, p, an);
-            if (In != I) xsn = xraw(A, lda, n, j, In * MR + tid);
+            if (In != I) xsn = xraw(x_of(w, j), n, j, In * MR + tid);
         }
         double cp = 0.0;
 #pragma unroll
@@ -776,7 +840,7 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
             cp += a[k] * sh.vstrip[32 * (k >> 1) + 2 * p + (k & 1)];
         }
         cp = row16_sum(cp);
-        if (p == 0 && c > j && c < n) st_co(w.Pcol + I * w.ldp + c, -cp);
+        if (p == 0 && c > j && c < n) st_co(w.Pcol + I * w.ldp + c, with_tag(-cp, col_tag(j)));
         if (!more || In != I) {
             // flush this rank's row sums of strip I: lanes -> waves -> one slot row
 #pragma unroll
@@ -790,7 +854,8 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
             {
                 const int64_t r = I * MR + tid;
                 if (r < n)
-                    st_co(w.Prow + cur_slot * w.ldp + r, (sh.mred[0][tid] + sh.mred[1][tid]) + (sh.mred[2][tid] + sh.mred[3][tid]));
+                    st_co(w.Prow + cur_slot * w.ldp + r,
+                          with_tag((sh.mred[0][tid] + sh.mred[1][tid]) + (sh.mred[2][tid] + sh.mred[3][tid]), col_tag(j)));
                 if (more) sh.vstrip[tid] = vfrom(n, j, In * MR + tid, xsn, s);
             }
             __syncthreads();

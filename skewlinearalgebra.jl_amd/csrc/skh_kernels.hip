@@ -20,8 +20,9 @@
 namespace skh {
 
 // Optional phase clock (workgroup 0, 100 MHz wall clock), accumulated over all launches:
-// ph[0] row passes, ph[1] barrier 1 arrivals, ph[2] phase B (barrier 1 wait, scalars, t, matvec),
-// ph[3] barrier 2; inside them ph[4..7] the row pass (loads, correction of w, column update with
+// ph[0] time before barrier 1's arrival (0 after the first column), ph[1] barrier 1 arrivals,
+// ph[2] phase B (barrier 1 wait, scalars, t, matvec), ph[3] row passes (with the wait for the
+// tagged partials); inside them ph[4..7] the row pass (loads, correction of w, column update with
 // its partials, hand-over to the next column) and phase B: ph[11] tile range and first tile,
 // ph[12] barrier 1 wait, ph[8] loads and scalars, ph[9] t, ph[10] matvec.
 // panels up to 64 wide fit two workgroups per CU (<= 256 registers per lane); 128 wide takes one.
@@ -58,7 +59,7 @@ __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int
         rr.xnext = (r < n && r > c0) ? A[r + c0 * lda] : 0.0;
     }
     const PanelMap pm = panel_map(n, c0, g, G);
-    row_pass<KQ, MULTI>(A, lda, n, c0, c0, 0, false, true, w, g, G, sh, rr, pm, clk);
+    row_pass<KQ, MULTI>(A, lda, n, c0, c0, 0, false, true, w, g, G, sh, rr, pm, err, clk);
     for (int i = 0; i < nbp; i++) {
         const int64_t j = c0 + i;
         clk_outer(clk, 0);
@@ -66,9 +67,10 @@ __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int
         clk_outer(clk, 1);
         if (!phase_b(A, lda, n, c0, j, i, w, sh, tau, eta, g, G, pm, bar, epoch, err, &s_ok, clk)) return;
         clk_outer(clk, 2);
-        if (!grid_sync_co(bar, ++epoch, err, &s_ok)) return;
+        // no grid barrier here: the row-block owners poll the tagged partial sums of their rows,
+        // every other workgroup goes on to the next barrier
+        row_pass<KQ, MULTI>(A, lda, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, g, G, sh, rr, pm, err, clk);
         clk_outer(clk, 3);
-        row_pass<KQ, MULTI>(A, lda, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, g, G, sh, rr, pm, clk);
     }
     if (clk) {
         clk_outer(clk, 0);
@@ -98,6 +100,16 @@ __global__ void k_finish(double* A, int64_t lda, int64_t n, double* tau, double*
         eta[n - 2] = A[(n - 1) + (n - 2) * lda];
         tau[n - 2] = 0.0;
     }
+}
+
+// zero t (ntv entries) and rows [r0, n) of nslots slots of ld doubles (the matvec partials)
+__global__ void k_clear_partials(double* tv, int64_t ntv, double* slots, int64_t ld, int64_t r0, int64_t n)
+{
+    if (blockIdx.y == 0 && blockIdx.x == 0)
+        for (int64_t k = threadIdx.x; k < ntv; k += blockDim.x) tv[k] = 0.0;
+    double* s = slots + (int64_t)blockIdx.y * ld;
+    for (int64_t r = r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+        s[r] = 0.0;
 }
 
 __global__ void k_zero_u32(unsigned* p, int count)
@@ -288,6 +300,16 @@ extern "C" int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st)
                     (void*)&pl->err, (void*)&pl->ph};
     return (int)hipLaunchCooperativeKernel(panel_kernel_for(pl->nbmax, pl->multi), dim3(pl->grid), dim3(WG), args, 0,
                                            st);
+}
+
+extern "C" int skh_internal_clear_partials(double* tv, int64_t ntv, double* slots, int64_t ld, int64_t r0, int64_t n,
+                                           int64_t nslots, hipStream_t st)
+{
+    if (nslots <= 0) return 0;
+    const int64_t rows = std::max<int64_t>(1, n - r0);
+    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((rows + 255) / 256, 8));
+    hipLaunchKernelGGL(k_clear_partials, dim3(gx, (unsigned)nslots), dim3(256), 0, st, tv, ntv, slots, ld, r0, n);
+    return (int)hipGetLastError();
 }
 
 extern "C" int skh_internal_panel_occupancy(int nbmax, int* blocks_per_cu)
