@@ -590,3 +590,34 @@ def test_sytrd_n8192_invariants(nb):
     _, logdet = torch.linalg.slogdet(A)
     lp = float(np.sum(np.log(np.abs(e[0::2]))))
     assert abs(2.0 * lp - float(logdet)) <= 1e-10 * abs(float(logdet)), (2.0 * lp, float(logdet))
+
+
+@pytest.mark.parametrize("nb", [32, 64])
+def test_sytrd_repeatable_and_clock_neutral(nb):
+    """The cooperative engine hands partial sums between workgroups through tagged values and
+    parity buffers: every result must be independent of timing.  Three factorizations of the same
+    matrix (one with the per-kernel events and the phase clock on) give bit-identical factors,
+    tau and eta."""
+    torch = pytest.importorskip("torch")
+    say("sytrd repeatability n=3000 nb=%d" % nb)
+    n = 3000
+    A = skew_gauss(n, 3000 + nb)
+    dev = torch.device("cuda:0")
+    A0 = torch.tensor(A.T.copy(), device=dev)   # row-major tensor of A^T = column-major A
+    st = torch.cuda.current_stream().cuda_stream
+    outs = []
+    for flags in (skl.ENGINE_LARGE, skl.ENGINE_LARGE, skl.ENGINE_LARGE | _lib.FLAG_TIMING | _lib.FLAG_PHASES):
+        work = A0.clone()
+        tau = torch.empty(n, dtype=torch.float64, device=dev)
+        eta = torch.empty(n, dtype=torch.float64, device=dev)
+        o = _lib.opts(nb, flags)
+        rc = _lib.lib().skh_sytrd_d_device(n, work.data_ptr(), n, tau.data_ptr(), eta.data_ptr(), st, ctypes.byref(o))
+        _lib.check("skh_sytrd_d_device", rc)
+        torch.cuda.synchronize()
+        outs.append((work.cpu().numpy(), tau[: n - 1].cpu().numpy(), eta[: n - 1].cpu().numpy()))
+    for F, tau, eta in outs[1:]:
+        assert np.array_equal(np.tril(F, -1), np.tril(outs[0][0], -1))
+        assert np.array_equal(tau, outs[0][1])
+        assert np.array_equal(eta, outs[0][2])
+    # and they are the orthogonal similarity they should be (Frobenius norm kept)
+    assert abs(np.sqrt(2.0 * np.sum(outs[0][2] ** 2)) - np.linalg.norm(A)) <= 1e-13 * np.linalg.norm(A)
