@@ -8,7 +8,9 @@ a C restatement of SkewLinearAlgebra.jl's CPU algorithm:
 * ``skewtrieigvals``     -- src/skeweigen.jl:105-156 (+ reducetozero :11-47,
                             implicitstep_novec :73-103, backup :463-468)
 * ``skewtrieigen``       -- src/skeweigen.jl:195-320 (merged) and :322-457 (divided)
-* ``logabspfaffian`` / ``pfaffian`` -- src/pfaffian.jl:74-94, 167-191
+* ``logabspfaffian`` / ``pfaffian`` -- src/pfaffian.jl:74-94, 167-191 (real, via the
+                            tridiagonalization) and :95-137, :193-233 (complex skew-symmetric,
+                            Parlett-Reid with pivoting)
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
 use it, and only as the checker / the CPU baseline.  The shipped library never calls it.
@@ -59,6 +61,8 @@ def lib(use_blas=True):
         L.orc_logabspfaffian_tri_d.argtypes = [_i64, _dp, _dp, _dp]
         L.orc_pfaffian_d.argtypes = [_i64, _dp, _i64]
         L.orc_pfaffian_d.restype = ctypes.c_double
+        L.orc_pfaffian_z_out.argtypes = [_i64, _dp, _i64, _dp]
+        L.orc_logabspfaffian_z.argtypes = [_i64, _dp, _i64, _dp, _dp]
         _LIB = L
         if use_blas:
             p = openblas_path()
@@ -123,6 +127,8 @@ def skewtrieigen(ev, variant="divided"):
 
 
 def logabspfaffian(A):
+    if np.iscomplexobj(A):
+        return logabspfaffian_z(A)
     A = np.array(A, dtype=np.float64, order="F", copy=True)
     n = A.shape[0]
     lp = ctypes.c_double()
@@ -140,8 +146,27 @@ def logabspfaffian_tri(ev):
 
 
 def pfaffian(A):
+    if np.iscomplexobj(A):
+        return pfaffian_z(A)
     A = np.array(A, dtype=np.float64, order="F", copy=True)
     return lib().orc_pfaffian_d(A.shape[0], _ptr(A), A.shape[0])
+
+
+def pfaffian_z(A):
+    """Complex skew-symmetric Pfaffian (Parlett-Reid, src/pfaffian.jl:97-137)."""
+    A = np.array(A, dtype=np.complex128, order="F", copy=True)
+    out = np.zeros(2)
+    lib().orc_pfaffian_z_out(A.shape[0], _ptr(A), A.shape[0], _ptr(out))
+    return complex(out[0], out[1])
+
+
+def logabspfaffian_z(A):
+    """(log|Pf|, cis(phase)) of a complex skew-symmetric matrix (src/pfaffian.jl:193-233)."""
+    A = np.array(A, dtype=np.complex128, order="F", copy=True)
+    lp = ctypes.c_double()
+    ph = np.zeros(2)
+    lib().orc_logabspfaffian_z(A.shape[0], _ptr(A), A.shape[0], ctypes.byref(lp), _ptr(ph))
+    return lp.value, complex(ph[0], ph[1])
 
 
 def form_q(F, tau):

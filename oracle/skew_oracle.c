@@ -939,3 +939,102 @@ double orc_pfaffian_d(i64 n, double *A, i64 lda)
     free(eta);
     return (nz % 2 == 1) ? -pf : pf;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Complex skew-symmetric Pfaffian: Parlett-Reid elimination with pivoting on   */
+/* the largest |A[r,k]| (src/pfaffian.jl:95-137 _pfaffian!, :193-233            */
+/* _logabspfaffian! for AbstractMatrix{<:Complex}); the full square is updated, */
+/* as the reference does.                                                       */
+/* ------------------------------------------------------------------------- */
+/* Step k (0-based, even): returns 0 if the pivot A[k+1,k] is zero after the swap, 1 after a
+ * step without a swap, 2 with one; *piv = A[k,k+1]. */
+static int parlett_reid_step_z(i64 n, zc *A, i64 lda, i64 k, zc *piv, zc *tau)
+{
+    /* kp = k + argmaxabs(A[k+1:end, k]): the first index of the largest modulus (:107) */
+    i64 kp = k + 1;
+    double best = -1.0;
+    for (i64 r = k + 1; r < n; r++) {
+        const double a = cabs(A[r + k * lda]);
+        if (a > best) {
+            best = a;
+            kp = r;
+        }
+    }
+    int swapped = 0;
+    if (kp != k + 1) { /* rows k+1 <-> kp, then columns, over l >= k (:108-115) */
+        for (i64 l = k; l < n; l++) {
+            const zc t = A[(k + 1) + l * lda];
+            A[(k + 1) + l * lda] = A[kp + l * lda];
+            A[kp + l * lda] = t;
+        }
+        for (i64 l = k; l < n; l++) {
+            const zc t = A[l + (k + 1) * lda];
+            A[l + (k + 1) * lda] = A[l + kp * lda];
+            A[l + kp * lda] = t;
+        }
+        swapped = 1;
+    }
+    if (A[(k + 1) + k * lda] == 0) return 0; /* (:119, :133) */
+    const zc a = A[k + (k + 1) * lda];
+    const i64 m = n - k - 2;
+    for (i64 c = 0; c < m; c++) tau[c] = A[k + (k + 2 + c) * lda] / a; /* Gauss vector (:120) */
+    for (i64 l1 = 0; l1 < m; l1++) /* rank-2 update of the trailing square (:123-129) */
+        for (i64 l2 = 0; l2 < m; l2++)
+            A[(k + 2 + l2) + (k + 2 + l1) * lda] +=
+                tau[l2] * A[(k + 2 + l1) + (k + 1) * lda] - tau[l1] * A[(k + 2 + l2) + (k + 1) * lda];
+    *piv = a;
+    return 1 + swapped;
+}
+
+/* _pfaffian!(A::AbstractMatrix{<:Complex}) (src/pfaffian.jl:97-137); A is overwritten. */
+zc orc_pfaffian_z(i64 n, zc *A, i64 lda)
+{
+    if (n % 2 == 1) return 0;
+    zc *tau = (zc *)malloc((size_t)(n > 2 ? n : 2) * sizeof(zc));
+    zc pf = 1;
+    for (i64 k = 0; k + 1 < n; k += 2) {
+        zc piv = 0;
+        const int st = parlett_reid_step_z(n, A, lda, k, &piv, tau);
+        if (st == 2) pf = -pf;
+        if (st == 0) {
+            free(tau);
+            return 0;
+        }
+        pf *= piv;
+    }
+    free(tau);
+    return pf;
+}
+
+/* _logabspfaffian!(A::AbstractMatrix{<:Complex}) (src/pfaffian.jl:193-233): log|Pf| and the
+ * unit cis(phase), phase = sum of angle(A[k,k+1]) + pi per pivot swap; odd n or a zero pivot give
+ * (-Inf, 0).  A is overwritten. */
+void orc_logabspfaffian_z(i64 n, zc *A, i64 lda, double *logpf, zc *phase_out)
+{
+    if (n % 2 == 1) {
+        *logpf = -INFINITY;
+        *phase_out = 0;
+        return;
+    }
+    zc *tau = (zc *)malloc((size_t)(n > 2 ? n : 2) * sizeof(zc));
+    double lp = 0.0, ph = 0.0;
+    for (i64 k = 0; k + 1 < n; k += 2) {
+        zc piv = 0;
+        const int st = parlett_reid_step_z(n, A, lda, k, &piv, tau);
+        if (st == 2) ph += 3.14159265358979323846; /* pi, Float64(pi) */
+        if (st == 0) {
+            free(tau);
+            *logpf = -INFINITY;
+            *phase_out = 0;
+            return;
+        }
+        lp += log(cabs(piv));
+        ph += carg(piv);
+    }
+    free(tau);
+    *logpf = lp;
+    *phase_out = cexp(I * ph);
+}
+
+/* orc_pfaffian_z with the result written through a pointer (plain-C callers: ctypes) */
+void orc_pfaffian_z_out(i64 n, zc *A, i64 lda, zc *out) { *out = orc_pfaffian_z(n, A, lda); }

@@ -74,6 +74,56 @@ def pf_expand(M):
     return total
 
 
+class GQ:
+    """Exact Gaussian rational re + im*i (fractions.Fraction parts)."""
+
+    def __init__(self, re, im=0):
+        self.re, self.im = Fraction(re), Fraction(im)
+
+    def __add__(self, o):
+        return GQ(self.re + o.re, self.im + o.im)
+
+    def __sub__(self, o):
+        return GQ(self.re - o.re, self.im - o.im)
+
+    def __neg__(self):
+        return GQ(-self.re, -self.im)
+
+    def __mul__(self, o):
+        return GQ(self.re * o.re - self.im * o.im, self.re * o.im + self.im * o.re)
+
+    def __truediv__(self, o):
+        d = o.re * o.re + o.im * o.im
+        return GQ((self.re * o.re + self.im * o.im) / d, (self.im * o.re - self.re * o.im) / d)
+
+    def iszero(self):
+        return self.re == 0 and self.im == 0
+
+
+def pf_exact_complex(Are, Aim):
+    """Exact Pfaffian of a complex skew-symmetric matrix given as exact real and imaginary parts
+    (skew Gaussian elimination in Gaussian rationals; the same recursion as pf_exact)."""
+    A = [[GQ(Fraction(str(a)), Fraction(str(b))) for a, b in zip(ra, rb)] for ra, rb in zip(Are, Aim)]
+    n = len(A)
+    if n % 2 == 1:
+        return GQ(0)
+    pf = GQ(1)
+    while n > 0:
+        p = next((j for j in range(1, n) if not A[0][j].iszero()), None)
+        if p is None:
+            return GQ(0)
+        if p != 1:
+            A[1], A[p] = A[p], A[1]
+            for row in A:
+                row[1], row[p] = row[p], row[1]
+            pf = -pf
+        a01 = A[0][1]
+        pf = pf * a01
+        A = [[A[i][j] + (A[i][0] * A[1][j] - A[i][1] * A[0][j]) / a01 for j in range(2, n)] for i in range(2, n)]
+        n -= 2
+    return pf
+
+
 def haar(n, rng):
     Z = rng.standard_normal((n, n))
     Q, R = np.linalg.qr(Z)
@@ -173,9 +223,46 @@ def main():
                       "inv": (0.5 * (Ainv - Ainv.T)).tolist()})
     fx["constructed"] = cases
 
+    # complex skew-symmetric matrices as in the reference's Pfaffian tests
+    # (test/runtests.jl:407-420: A = (R - R^T)/2 with R of Gaussian integers in [-10, 10]), plus
+    # cases that force a pivot swap (A[1,0] = 0) or have a zero Pfaffian (a zero row); exact
+    # Pfaffians in Gaussian rationals
+    crng = np.random.default_rng(SEED + 7)
+    cases = []
+    for n in [1, 2, 3, 4, 6, 10, 20]:
+        R = crng.integers(-10, 11, (n, n)) + 1j * crng.integers(-10, 11, (n, n))
+        cases.append((R - R.T) / 2)
+    Sw = cases[4].copy()            # n = 6 with A[1,0] = 0: the first step swaps rows/columns
+    Sw[1, 0] = Sw[0, 1] = 0
+    cases.append(Sw)
+    Sw2 = cases[5].copy()           # n = 10 with A[1:3, 0] = 0
+    Sw2[1:3, 0] = 0
+    Sw2[0, 1:3] = 0
+    cases.append(Sw2)
+    Z = cases[5].copy()             # zero row and column 3: Pf = 0
+    Z[3, :] = 0
+    Z[:, 3] = 0
+    cases.append(Z)
+    out = []
+    for A in cases:
+        pf = pf_exact_complex(A.real.tolist(), A.imag.tolist())
+        out.append({"re": A.real.tolist(), "im": A.imag.tolist(),
+                    "pf": [str(pf.re), str(pf.im)], "pf_float": [float(pf.re), float(pf.im)]})
+    assert tuple(out[3]["pf"]) == _pf4(cases[3]), "4x4 exact Pfaffian disagrees with the analytic formula"
+    fx["complex_pf"] = out
+
     with open(OUT, "w") as f:
         json.dump(fx, f, indent=None, separators=(",", ":"))
     print("wrote", OUT, os.path.getsize(OUT), "bytes")
+
+
+def _pf4(A):
+    """a f - b e + c d for A = [0 a b c; -a 0 d e; -b -d 0 f; -c -e -f 0] in exact Gaussian rationals
+    (test/runtests.jl:426-430 states it for integers)."""
+    g = lambda z: GQ(Fraction(str(z.real)), Fraction(str(z.imag)))
+    a, b, c, d, e, f = g(A[0, 1]), g(A[0, 2]), g(A[0, 3]), g(A[1, 2]), g(A[1, 3]), g(A[2, 3])
+    r = a * f - b * e + c * d
+    return str(r.re), str(r.im)
 
 
 def _issue130():

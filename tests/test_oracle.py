@@ -329,3 +329,79 @@ def test_planted_mistakes():
     for slip in ("sign", "transpose", "drop"):
         with pytest.raises(AssertionError):
             full_check(*unblocked(A, slip))
+
+
+# ---- complex skew-symmetric Pfaffian (Parlett-Reid, src/pfaffian.jl:95-137, 193-233) ----
+
+def _complex_case(c):
+    return np.array(c["re"]) + 1j * np.array(c["im"])
+
+
+def _exact(c):
+    return complex(float(Fraction(c["pf"][0])), float(Fraction(c["pf"][1])))
+
+
+def test_complex_pfaffian_exact(golden):
+    """Oracle against exact Gaussian-rational Pfaffians (the reference's own test inputs, plus
+    cases that pivot and a singular one)."""
+    for c in golden["complex_pf"]:
+        A = _complex_case(c)
+        ex = _exact(c)
+        pf = oracle.pfaffian(A)
+        assert abs(pf - ex) <= 1e-12 * max(1.0, abs(ex)), (A.shape, pf, ex)
+        lp, ph = oracle.logabspfaffian(A)
+        if ex == 0:
+            assert lp == -np.inf and ph == 0
+        else:
+            assert abs(lp - np.log(abs(ex))) <= 1e-12 * max(1.0, abs(np.log(abs(ex))))
+            assert abs(ph - ex / abs(ex)) <= 1e-12
+
+
+@pytest.mark.parametrize("n", [2, 8, 16, 33, 64])
+def test_complex_pfaffian_det_and_congruence(n):
+    """Pf(A)^2 = det(A), and Pf(B A B^T) = det(B) Pf(A) for any complex B."""
+    rng = np.random.default_rng(500 + n)
+    R = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = R - R.T
+    pf = oracle.pfaffian(A)
+    if n % 2:
+        assert pf == 0
+        return
+    d = np.linalg.det(A)
+    assert abs(pf * pf - d) <= 1e-12 * abs(d) * n
+    B = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    pfc = oracle.pfaffian(B @ A @ B.T)
+    assert abs(pfc - np.linalg.det(B) * pf) <= 1e-11 * abs(pfc) * n
+
+
+def _parlett_reid(A, flip_on_swap=True, transposed_update=False):
+    """Plain-loop Parlett-Reid used only to show the complex fixtures catch plausible slips."""
+    A = np.array(A, dtype=complex)
+    n = A.shape[0]
+    pf = 1.0 + 0j
+    for k in range(0, n - 1, 2):
+        kp = k + 1 + int(np.argmax(np.abs(A[k + 1:, k])))
+        if kp != k + 1:
+            A[[k + 1, kp], k:] = A[[kp, k + 1], k:]
+            A[k:, [k + 1, kp]] = A[k:, [kp, k + 1]]
+            if flip_on_swap:
+                pf = -pf
+        if A[k + 1, k] == 0:
+            return 0j
+        tau = A[k, k + 2:] / A[k, k + 1]
+        pf *= A[k, k + 1]
+        col = A[k + 2:, k + 1]
+        upd = np.outer(tau, col) - np.outer(col, tau)
+        A[k + 2:, k + 2:] += upd.T if transposed_update else upd
+    return pf
+
+
+def test_complex_pfaffian_planted_slips(golden):
+    """A missing sign flip on a pivot swap, or the update applied transposed, fails the exact
+    fixtures, while the plain restatement passes them."""
+    def passes(fn):
+        return all(abs(fn(_complex_case(c)) - _exact(c)) <= 1e-10 * max(1.0, abs(_exact(c)))
+                   for c in golden["complex_pf"] if len(c["re"]) % 2 == 0)
+    assert passes(_parlett_reid)
+    assert not passes(lambda A: _parlett_reid(A, flip_on_swap=False))
+    assert not passes(lambda A: _parlett_reid(A, transposed_update=True))
