@@ -157,6 +157,29 @@ int skh_eigvals_z(int64_t n, const double *A, int64_t lda, double *vals, const s
 int skh_eigen_z(int64_t n, const double *A, int64_t lda, double *vals, double *V, const skh_opts *opts);
 
 /*
+ * Pfaffians of a batch of complex skew-SYMMETRIC matrices (A^T = -A, ComplexF64, interleaved
+ * re/im; matrix b is n[b] x n[b] column-major at complex-element offset offset[b], lda = n[b]).
+ * Replaces _pfaffian!(A::AbstractMatrix{<:Complex}) (reference src/pfaffian.jl:97-137, reached
+ * from pfaffian!/pfaffian, :155-158 and :144) and _logabspfaffian!(A::AbstractMatrix{<:Complex})
+ * (:193-233, from logabspfaffian!, :259-263): Parlett-Reid elimination with pivoting.
+ * Only the strictly lower triangle is read (the reference's isskewsymmetric check is the caller's);
+ * A is not modified.  n[b] <= 2048.  pf: 2 doubles (re, im) per matrix; odd n or a zero pivot
+ * give 0.  logpf: log|Pf| per matrix, phase: cis(phase) as 2 doubles per matrix; odd n or a zero
+ * pivot give (-Inf, 0).  No reference counterpart for the batching itself (the reference user
+ * loops pfaffian).  One workgroup per matrix, largest matrices first.
+ */
+int skh_pfaffian_z_batched(int64_t batch, const int64_t *n, const int64_t *offset, const double *Apacked, double *pf,
+                           const skh_opts *opts);
+int skh_logabspfaffian_z_batched(int64_t batch, const int64_t *n, const int64_t *offset, const double *Apacked,
+                                 double *logpf, double *phase, const skh_opts *opts);
+/* device variants: dA (16-byte aligned) and the outputs in HBM; n and offset are host arrays.
+ * Enqueued on `stream`; the call returns once its control arrays are uploaded. */
+int skh_pfaffian_z_batched_device(int64_t batch, const int64_t *n, const int64_t *offset, const double *dA,
+                                  double *dpf, void *stream, const skh_opts *opts);
+int skh_logabspfaffian_z_batched_device(int64_t batch, const int64_t *n, const int64_t *offset, const double *dA,
+                                        double *dlogpf, double *dphase, void *stream, const skh_opts *opts);
+
+/*
  * Eigen-decomposition of a real skew-symmetric A (host memory, A not modified).
  * Replaces skeweigen!/eigen!(A::SkewHermitian{<:Real}) (reference src/eigen.jl:75-97): GPU
  * tridiagonalization and Q, host skewtrieigen_divided!, GPU back-transform Q*Qr, Q*Qim.

@@ -950,12 +950,13 @@ double orc_pfaffian_d(i64 n, double *A, i64 lda)
  * step without a swap, 2 with one; *piv = A[k,k+1]. */
 static int parlett_reid_step_z(i64 n, zc *A, i64 lda, i64 k, zc *piv, zc *tau)
 {
-    /* kp = k + argmaxabs(A[k+1:end, k]): the first index of the largest modulus (:107) */
+    /* kp = k + argmaxabs(A[k+1:end, k]): the first index of the largest modulus (:107); as in
+     * Julia's argmax, NaN ranks above every number */
     i64 kp = k + 1;
     double best = -1.0;
     for (i64 r = k + 1; r < n; r++) {
         const double a = cabs(A[r + k * lda]);
-        if (a > best) {
+        if (!isnan(best) && (isnan(a) || a > best)) {
             best = a;
             kp = r;
         }

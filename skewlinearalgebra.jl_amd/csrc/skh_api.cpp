@@ -750,6 +750,81 @@ void plan_free(skh_plan p)
     delete p;
 }
 
+// ---- complex skew-symmetric Pfaffians (Parlett-Reid, one workgroup per matrix) ----
+Buf g_zpslot, g_zpctl, g_zpin, g_zpout;
+constexpr size_t ZPF_SLOT_BUDGET = size_t(4) << 30;   // bytes of working copies in flight
+
+int zpf_check(int64_t batch, const int64_t* n, const int64_t* offset, const void* A, const void* o1, const void* o2,
+              bool dev)
+{
+    if (batch < 0) return set_err(-1, "batch < 0");
+    if (batch > 0 && (!n || !offset || !A || !o1 || !o2)) return set_err(-2, "NULL argument");
+    const int64_t nmax = skh_internal_zpfaff_nmax();
+    for (int64_t b = 0; b < batch; b++) {
+        if (n[b] < 0) return set_err(-2, "n[%lld] < 0", (long long)b);
+        if (n[b] > nmax) return set_err(-2, "n[%lld] = %lld exceeds %lld", (long long)b, (long long)n[b], (long long)nmax);
+        if (offset[b] < 0) return set_err(-3, "offset[%lld] < 0", (long long)b);
+    }
+    if (dev && batch > 0 && (((uintptr_t)A) & 15)) return set_err(-4, "A must be 16-byte aligned");
+    return 0;
+}
+
+// dA: matrices at complex offsets offs[b]; mode 0: Pf into do1 (2 doubles each); mode 1: log|Pf|
+// into do1 and cis(phase) into do2.  Host n/offs; enqueues on st and waits for the control copy.
+int zpf_run(int64_t batch, const int64_t* ns, const int64_t* offs, const double2* dA, int mode, double* do1,
+            double* do2, hipStream_t st)
+{
+    if (batch == 0) return 0;
+    std::vector<int64_t> order(batch);
+    for (int64_t b = 0; b < batch; b++) order[b] = b;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return ns[a] > ns[b]; });
+    const int64_t nmax = std::max<int64_t>(ns[order[0]], 2);
+    int per_cu = 0;
+    HIPCHK((hipError_t)skh_internal_zpfaff_occupancy(nmax, &per_cu));
+    per_cu = std::max(1, per_cu);
+    const int64_t slot = (nmax * nmax + 15) & ~int64_t(15);   // complex elements
+    const int64_t by_mem = std::max<int64_t>(1, (int64_t)(ZPF_SLOT_BUDGET / ((size_t)slot * sizeof(double2))));
+    const int grid = (int)std::min<int64_t>({(int64_t)per_cu * g_num_cu, batch, by_mem});
+    double2* slots = (double2*)g_zpslot.get((size_t)slot * grid * sizeof(double2));
+    int64_t* dctl = (int64_t*)g_zpctl.get(3 * batch * sizeof(int64_t));
+    if (!slots || !dctl) return set_err(SKH_ERR_HIP, "hipMalloc of Pfaffian work slots failed");
+    std::vector<int64_t> h(3 * batch);
+    std::copy(ns, ns + batch, h.begin());
+    std::copy(offs, offs + batch, h.begin() + batch);
+    std::copy(order.begin(), order.end(), h.begin() + 2 * batch);
+    HIPCHK(hipMemcpyAsync(dctl, h.data(), h.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    HIPCHK((hipError_t)skh_internal_zpfaff(dA, dctl, dctl + batch, dctl + 2 * batch, batch, nmax, slots, slot, grid,
+                                           mode, do1, do2, st));
+    // the host copy of the control arrays must outlive the async copy
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int zpf_host(int64_t batch, const int64_t* n, const int64_t* offset, const double* A, int mode, double* o1, double* o2)
+{
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    hipStream_t st = g_stream;
+    std::vector<int64_t> offs2(batch);
+    int64_t tot = 0;
+    for (int64_t b = 0; b < batch; b++) {
+        offs2[b] = tot;
+        tot += n[b] * n[b];
+    }
+    double2* dA = (double2*)g_zpin.get((size_t)std::max<int64_t>(tot, 1) * sizeof(double2));
+    double* dout = (double*)g_zpout.get((size_t)4 * batch * sizeof(double));
+    if (!dA || !dout) return set_err(SKH_ERR_HIP, "hipMalloc of batched buffers failed");
+    for (int64_t b = 0; b < batch; b++)
+        HIPCHK(hipMemcpyAsync(dA + offs2[b], A + 2 * offset[b], (size_t)n[b] * n[b] * sizeof(double2),
+                              hipMemcpyHostToDevice, st));
+    rc = zpf_run(batch, n, offs2.data(), dA, mode, dout, dout + 2 * batch, st);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(o1, dout, (mode == 0 ? 2 : 1) * batch * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (mode == 1) HIPCHK(hipMemcpyAsync(o2, dout + 2 * batch, 2 * batch * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -771,7 +846,8 @@ int skh_finalize(void)
     if (!g_inited) return 0;
     (void)hipStreamSynchronize(g_stream);
     for (Buf* b : {&g_mat, &g_ws, &g_vec, &g_ctl, &g_bin, &g_bout, &g_q, &g_ev, &g_gq, &g_gk, &g_gz, &g_gv, &g_gp,
-                   &g_gte, &g_cslot, &g_big, &g_bctl, &g_zws, &g_zq, &g_zc, &g_zx})
+                   &g_gte, &g_cslot, &g_big, &g_bctl, &g_zws, &g_zq, &g_zc, &g_zx, &g_zpslot, &g_zpctl, &g_zpin,
+                   &g_zpout})
         b->release();
     if (g_blas) (void)rocblas_destroy_handle(g_blas);
     g_blas = nullptr;
@@ -1421,6 +1497,50 @@ int skh_eigen_z(int64_t n, const double* A, int64_t lda, double* vals, double* V
     HIPCHK(hipStreamSynchronize(st));
     for (int64_t k = 0; k < n; k++) vals[k] = -d[k];
     return 0;
+}
+
+int skh_pfaffian_z_batched(int64_t batch, const int64_t* n, const int64_t* offset, const double* Apacked, double* pf,
+                           const skh_opts* opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    (void)opts;
+    int rc = zpf_check(batch, n, offset, Apacked, pf, pf, false);
+    if (rc || batch == 0) return rc;
+    return zpf_host(batch, n, offset, Apacked, 0, pf, nullptr);
+}
+
+int skh_pfaffian_z_batched_device(int64_t batch, const int64_t* n, const int64_t* offset, const double* dA,
+                                  double* dpf, void* stream, const skh_opts* opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    (void)opts;
+    int rc = zpf_check(batch, n, offset, dA, dpf, dpf, true);
+    if (rc || batch == 0) return rc;
+    rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    return zpf_run(batch, n, offset, (const double2*)dA, 0, dpf, nullptr, (hipStream_t)stream);
+}
+
+int skh_logabspfaffian_z_batched(int64_t batch, const int64_t* n, const int64_t* offset, const double* Apacked,
+                                 double* logpf, double* phase, const skh_opts* opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    (void)opts;
+    int rc = zpf_check(batch, n, offset, Apacked, logpf, phase, false);
+    if (rc || batch == 0) return rc;
+    return zpf_host(batch, n, offset, Apacked, 1, logpf, phase);
+}
+
+int skh_logabspfaffian_z_batched_device(int64_t batch, const int64_t* n, const int64_t* offset, const double* dA,
+                                        double* dlogpf, double* dphase, void* stream, const skh_opts* opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    (void)opts;
+    int rc = zpf_check(batch, n, offset, dA, dlogpf, dphase, true);
+    if (rc || batch == 0) return rc;
+    rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    return zpf_run(batch, n, offset, (const double2*)dA, 1, dlogpf, dphase, (hipStream_t)stream);
 }
 
 int skh_eigvals_range_d(int64_t n, const double* A, int64_t lda, int64_t il, int64_t iu, double vl, double vh,

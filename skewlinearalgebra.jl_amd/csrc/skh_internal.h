@@ -58,6 +58,13 @@ int skh_internal_zphase(const double* C, int64_t n, double2* Zc, hipStream_t st)
 // host (host_tridiag.cpp): eigenvalues of the symmetric tridiagonal (d, e), ascending into d; 0 or 2
 int skh_internal_symtri_eigvals(int64_t n, double* d, const double* e);
 int skh_internal_compact_occupancy(int nmax, int* blocks_per_cu);
+// complex skew-symmetric Pfaffians (skh_zpfaff.hip): mode 0 writes Pf (2 doubles per matrix) to o1,
+// mode 1 writes log|Pf| to o1 and cis(phase) (2 doubles per matrix) to o2
+int skh_internal_zpfaff(const double2* A, const int64_t* ns, const int64_t* offs, const int64_t* order, int64_t batch,
+                        int64_t nmax, double2* slots, int64_t slot, int grid, int mode, double* o1, double* o2,
+                        hipStream_t st);
+int skh_internal_zpfaff_occupancy(int64_t nmax, int* per_cu);
+int skh_internal_zpfaff_nmax();
 int skh_internal_launch_compact(const CompactArgs* a, int nmax, int grid, hipStream_t st);
 int skh_internal_logpf_from(const double* eta, const double* tau, int64_t n, double* logpf, double* sgn, hipStream_t st);
 int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st);

@@ -27,7 +27,7 @@ __all__ = [
     "isskewhermitian", "isskewsymmetric", "skewhermitian", "skewhermitian_",
     "hessenberg", "hessenberg_", "eigvals", "eigvals_", "eigen", "eigen_", "Eigen", "pfaffian", "pfaffian_",
     "logabspfaffian", "logabspfaffian_", "logabspfaffian_batched", "logabspfaffian_grad_batched",
-    "LogAbsPfaffian", "SkhError",
+    "LogAbsPfaffian", "SkhError", "pfaffian_z_batched", "logabspfaffian_z_batched",
     "ENGINE_AUTO", "ENGINE_LARGE", "ENGINE_TEAM1", "version", "LogPfPlan",
 ]
 
@@ -429,11 +429,11 @@ def logabspfaffian_(A, nb=0, engine=ENGINE_AUTO):
     """(log|Pf(A)|, sign); odd n -> (-Inf, 0).  A (SkewHermitian) becomes the factors."""
     if isinstance(A, SkewHermTridiagonal):
         return _tri_logpf(np.asarray(A.ev, dtype=np.float64))   # src/pfaffian.jl:181-191
+    if _is_complex(A):
+        return logabspfaffian(A)
     S = A if isinstance(A, SkewHermitian) else None
     if S is None:
         D = np.asarray(A)
-        if np.iscomplexobj(D):
-            raise NotImplementedError("complex skew-symmetric Pfaffians are not supported by this build")
         if not isskewhermitian(D):
             raise ArgumentError("Pfaffian requires a skew-Hermitian matrix")
         S = SkewHermitian(D)
@@ -451,10 +451,11 @@ def logabspfaffian(A, nb=0, engine=ENGINE_AUTO):
     """logabspfaffian(A) = logabspfaffian!(copyeigtype(A)) (src/pfaffian.jl:236,246)."""
     if isinstance(A, SkewHermTridiagonal):
         return logabspfaffian_(A)
+    if _is_complex(A):
+        lp, ph = logabspfaffian_z_batched([_complex_skew(A)])
+        return float(lp[0]), complex(ph[0])
     D = A.data if isinstance(A, SkewHermitian) else np.asarray(A)
     if not isinstance(A, SkewHermitian):
-        if np.iscomplexobj(D):
-            raise NotImplementedError("complex skew-symmetric Pfaffians are not supported by this build")
         if not isskewhermitian(D):
             raise ArgumentError("Pfaffian requires a skew-Hermitian matrix")
     F = np.array(D, dtype=np.float64, order="F", copy=True)
@@ -476,6 +477,8 @@ def pfaffian_(A, nb=0, engine=ENGINE_AUTO):
         if n % 2 == 1:
             return 0.0
         return float(np.prod(-ev[0::2]))
+    if _is_complex(A):
+        return pfaffian(A)
     S = A if isinstance(A, SkewHermitian) else None
     if S is None:
         D = np.asarray(A)
@@ -498,10 +501,69 @@ def pfaffian(A, nb=0, engine=ENGINE_AUTO):
     """pfaffian(A) = pfaffian!(copyeigtype(A)) (src/pfaffian.jl:148)."""
     if isinstance(A, SkewHermTridiagonal):
         return pfaffian_(A)
+    if _is_complex(A):
+        return complex(pfaffian_z_batched([_complex_skew(A)])[0])
     D = A.data if isinstance(A, SkewHermitian) else np.asarray(A)
     if not isinstance(A, SkewHermitian) and not isskewhermitian(D):
         raise ArgumentError("Pfaffian requires a skew-symmetric matrix")
     return pfaffian_(SkewHermitian(np.array(D, dtype=np.float64, order="F", copy=True)), nb, engine)
+
+
+def _is_complex(A):
+    if isinstance(A, SkewHermTridiagonal):
+        return False
+    return np.iscomplexobj(A.data if isinstance(A, SkewHermitian) else A)
+
+
+def _complex_skew(A):
+    """The complex matrix of A after pfaffian!'s check (src/pfaffian.jl:155-158, :259-263):
+    A == -transpose(A) exactly, else ArgumentError."""
+    D = np.asarray(A.data if isinstance(A, SkewHermitian) else A)
+    if not isskewsymmetric(D):
+        raise ArgumentError("Pfaffian requires a skew-symmetric matrix")
+    return D
+
+
+def _pack_z(mats):
+    mats = [np.asarray(M, dtype=np.complex128) for M in mats]
+    for M in mats:
+        if M.ndim != 2 or M.shape[0] != M.shape[1]:
+            raise ArgumentError("each matrix must be square")
+    ns = np.array([M.shape[0] for M in mats], dtype=np.int64)
+    offs = np.zeros(len(mats), dtype=np.int64)
+    if len(mats) > 1:
+        offs[1:] = np.cumsum(ns[:-1] ** 2)
+    packed = np.concatenate([M.ravel(order="F") for M in mats]) if mats else np.zeros(1, np.complex128)
+    return ns, offs, np.ascontiguousarray(packed)
+
+
+def pfaffian_z_batched(mats):
+    """Pfaffians of complex skew-symmetric matrices (uneven sizes, n <= 2048) in one GPU launch:
+    Parlett-Reid with pivoting as _pfaffian!(A::AbstractMatrix{<:Complex}) (src/pfaffian.jl:97-137).
+    Only the strictly lower triangle is read: the skew-symmetry check is the caller's (pfaffian
+    makes it).  Odd n or a zero pivot give 0.  Returns a complex128 array."""
+    ns, offs, packed = _pack_z(mats)
+    pf = np.zeros(len(ns), dtype=np.complex128)
+    if len(ns) == 0:
+        return pf
+    rc = _lib.lib().skh_pfaffian_z_batched(len(ns), _ptr(ns), _ptr(offs), _ptr(packed), _ptr(pf), None)
+    _lib.check("skh_pfaffian_z_batched", rc)
+    return pf
+
+
+def logabspfaffian_z_batched(mats):
+    """(log|Pf|, cis(phase)) of complex skew-symmetric matrices in one GPU launch, as
+    _logabspfaffian!(A::AbstractMatrix{<:Complex}) (src/pfaffian.jl:193-233); odd n or a zero
+    pivot give (-Inf, 0).  Returns (float64 array, complex128 array)."""
+    ns, offs, packed = _pack_z(mats)
+    lp = np.zeros(len(ns))
+    ph = np.zeros(len(ns), dtype=np.complex128)
+    if len(ns) == 0:
+        return lp, ph
+    rc = _lib.lib().skh_logabspfaffian_z_batched(len(ns), _ptr(ns), _ptr(offs), _ptr(packed), _ptr(lp), _ptr(ph),
+                                                  None)
+    _lib.check("skh_logabspfaffian_z_batched", rc)
+    return lp, ph
 
 
 def logabspfaffian_batched(mats, nb=0, engine=ENGINE_AUTO):

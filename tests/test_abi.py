@@ -103,6 +103,18 @@ def test_argument_errors_before_device():
     assert L.skh_logpf_d(3, A3.ctypes.data, 3, ctypes.byref(lp), ctypes.byref(sg), None) == 0
     assert lp.value == -np.inf and sg.value == 0.0
     assert L.skh_skewtrieigen_d(4, A.ctypes.data, tau.ctypes.data, A.ctypes.data, A.ctypes.data, 7) == -6
+    # complex Pfaffians: sizes beyond 2048 and negative offsets are rejected before any device work
+    ns = np.array([4, 2050], dtype=np.int64)
+    offs = np.array([0, 16], dtype=np.int64)
+    out = np.zeros(8)
+    assert L.skh_pfaffian_z_batched(2, ns.ctypes.data, offs.ctypes.data, out.ctypes.data, out.ctypes.data, None) == -2
+    assert b"2048" in L.skh_last_error()
+    offs[1] = -1
+    ns[1] = 2
+    assert L.skh_logabspfaffian_z_batched(2, ns.ctypes.data, offs.ctypes.data, out.ctypes.data, out.ctypes.data,
+                                          out.ctypes.data, None) == -3
+    assert L.skh_pfaffian_z_batched(-1, None, None, None, None, None) == -1
+    assert L.skh_pfaffian_z_batched(0, None, None, None, None, None) == 0
 
 
 def test_python_api_host_paths():
@@ -112,6 +124,11 @@ def test_python_api_host_paths():
     with pytest.raises(skl.ArgumentError):
         skl.logabspfaffian(np.ones((4, 4)))
     assert skl.logabspfaffian(np.zeros((5, 5))) == (-np.inf, 0.0)
+    # complex input takes the skew-symmetric check of pfaffian! (src/pfaffian.jl:155-158, :259-263)
+    for f in (skl.pfaffian, skl.pfaffian_, skl.logabspfaffian, skl.logabspfaffian_):
+        with pytest.raises(skl.ArgumentError, match="skew-symmetric"):
+            f(np.array([[0, 1j], [1j, 0]]))
+    assert len(skl.pfaffian_z_batched([])) == 0
     S = skl.skewhermitian(np.array([[1, 2], [3, 4]]))
     np.testing.assert_array_equal(S.data, [[0.0, -0.5], [0.5, 0.0]])   # issue #98
     T = skl.SkewHermTridiagonal(np.array([3.0, 4.0, 5.0]))
