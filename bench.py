@@ -17,6 +17,8 @@ Also reported:
                 per-matrix GPU loop
   complex       ComplexF64 n = 4096 skew-Hermitian tridiagonalization, TFLOP/s, with Q formation and
                 the residual / orthogonality of A = Q T Q^H
+  complex_pfaffian  4096 complex skew-symmetric matrices (n in {32,...,256}), log|Pf| and phase by
+                pivoted Parlett-Reid in one batched call, matrices/s, against the oracle on a sample
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 8192] [--nb 0] [--no-cpu] [--no-batched]
                        [--no-complex]
@@ -302,6 +304,63 @@ def complex_bench(torch, _lib, steps, n=4096, seed=314159):
             "q_formation_ms": q_ms, "residual_QTQh_minus_A_rel": res, "orthogonality_QhQ_minus_I": orth}
 
 
+def zpfaff_bench(torch, _lib, steps, count=4096, seed=2718):
+    """Complex skew-symmetric Pfaffians (Parlett-Reid with pivoting, the reference's
+    _logabspfaffian! on Matrix{ComplexF64}): `count` matrices A = R - R^T, n cycling through
+    32, 64, ..., 256, Re and Im of R iid N(0, 1), resident in HBM; one batched device call per step.
+    The oracle (C, one core) on every 65th matrix (each size 8 times, the batch's mix) gives the
+    CPU rate and the agreement."""
+    import numpy as np
+    import oracle
+    dev = torch.device("cuda:0")
+    ns = np.array([32 * (1 + k % 8) for k in range(count)], dtype=np.int64)
+    offs = np.zeros(count, dtype=np.int64)
+    offs[1:] = np.cumsum(ns[:-1] ** 2)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    tot = int(offs[-1] + ns[-1] ** 2)
+    buf = torch.empty(2 * tot, dtype=torch.float64, device=dev)
+    for k in range(count):
+        n = int(ns[k])
+        R = torch.complex(torch.randn((n, n), generator=g, dtype=torch.float64, device=dev),
+                          torch.randn((n, n), generator=g, dtype=torch.float64, device=dev))
+        buf[2 * int(offs[k]):2 * int(offs[k] + n * n)] = torch.view_as_real((R - R.T).T.contiguous()).reshape(-1)
+    lp = torch.empty(count, dtype=torch.float64, device=dev)
+    ph = torch.empty(2 * count, dtype=torch.float64, device=dev)
+    st = torch.cuda.current_stream().cuda_stream
+    L = _lib.lib()
+
+    def step():
+        _lib.check("skh_logabspfaffian_z_batched_device", L.skh_logabspfaffian_z_batched_device(
+            count, ns.ctypes.data, offs.ctypes.data, buf.data_ptr(), lp.data_ptr(), ph.data_ptr(), st, None))
+
+    step()
+    torch.cuda.synchronize()
+    ms = timed(torch, step, max(3, steps))
+    flops = float(np.sum(4.0 / 3.0 * ns.astype(np.float64) ** 3))
+    # CPU leg: the oracle on every 65th matrix (65 = 1 mod 8: every size equally often)
+    host = buf.cpu().numpy()
+    glp = lp.cpu().numpy()
+    sample = np.arange(0, count, 65)
+    worst = 0.0
+    t0 = time.perf_counter()
+    outs = []
+    for k in sample:
+        n = int(ns[k])
+        M = host[2 * int(offs[k]):2 * int(offs[k] + n * n)].view(np.complex128).reshape((n, n), order="F")
+        outs.append(oracle.logabspfaffian_z(M)[0])
+    cpu_s = time.perf_counter() - t0
+    for k, lr in zip(sample, outs):
+        worst = max(worst, abs(glp[k] - lr) / abs(lr))
+    cpu_rate = len(sample) / cpu_s
+    return {"metric": "batched complex skew-symmetric log|Pf| (Parlett-Reid), %d matrices n in {32,...,256}" % count,
+            "value": count / (ms * 1e-3), "unit": "matrices/s", "ms_per_batch": ms,
+            "gflops": flops / (ms * 1e-3) / 1e9, "flop_count": "4/3 n^3 real flops per matrix (lower triangle)",
+            "max_rel_dev_logpf_vs_oracle": worst,
+            "cpu_baseline": {"value": cpu_rate, "unit": "matrices/s (same size mix)", "cores": 1, "kind": "port",
+                             "sample": "oracle on every 65th matrix (%d matrices, %.2f s)" % (len(sample), cpu_s)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -312,7 +371,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched log|Pf| and gradient lines")
-    ap.add_argument("--no-complex", action="store_true", help="skip the complex n=4096 line")
+    ap.add_argument("--no-complex", action="store_true", help="skip the complex n=4096 and complex Pfaffian lines")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the single-matrix tridiagonalization is a one-GPU "
@@ -431,6 +490,9 @@ def main():
         log("complex n=4096 ...")
         res["complex"] = complex_bench(torch, _lib, args.steps)
         log(json.dumps(res["complex"]))
+        log("complex skew-symmetric Pfaffians (4096 matrices) ...")
+        res["complex_pfaffian"] = zpfaff_bench(torch, _lib, args.steps)
+        log(json.dumps(res["complex_pfaffian"]))
     if not args.no_cpu:
         log("cpu baseline (n=%d) ..." % args.cpu_n)
         res["cpu_baseline"] = cpu_baseline(args.cpu_n)

@@ -63,6 +63,8 @@ def lib(use_blas=True):
         L.orc_pfaffian_d.restype = ctypes.c_double
         L.orc_pfaffian_z_out.argtypes = [_i64, _dp, _i64, _dp]
         L.orc_logabspfaffian_z.argtypes = [_i64, _dp, _i64, _dp, _dp]
+        L.orc_skewchol_d.argtypes = [_i64, _dp, _i64, ctypes.c_int, _dp]
+        L.orc_skewchol_d.restype = _i64
         _LIB = L
         if use_blas:
             p = openblas_path()
@@ -167,6 +169,25 @@ def logabspfaffian_z(A):
     ph = np.zeros(2)
     lib().orc_logabspfaffian_z(A.shape[0], _ptr(A), A.shape[0], ctypes.byref(lp), _ptr(ph))
     return lp.value, complex(ph[0], ph[1])
+
+
+def skewchol(A, pivot=True):
+    """Skew-Cholesky R^T J R = A[p, p] (src/cholesky.jl:28-107; RowMaximum if pivot, else
+    NoPivot with p = 0..n-1).  Returns (R upper triangular, p 0-based, steps completed)."""
+    B = np.array(A, dtype=np.float64, order="F", copy=True)
+    n = B.shape[0]
+    p = np.zeros(max(n, 1), dtype=np.int64)
+    steps = lib().orc_skewchol_d(n, _ptr(B), max(n, 1), 1 if pivot else 0, _ptr(p))
+    return np.triu(B), p[:n], int(steps)
+
+
+def jmatrix(n):
+    """JMatrix{+1}(n) dense (src/jmatrix.jl:23-30): J[i, i+1] = 1, J[i+1, i] = -1 for even i."""
+    J = np.zeros((n, n))
+    for i in range(0, n - 1, 2):
+        J[i, i + 1] = 1.0
+        J[i + 1, i] = -1.0
+    return J
 
 
 def form_q(F, tau):

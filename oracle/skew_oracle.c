@@ -1039,3 +1039,117 @@ void orc_logabspfaffian_z(i64 n, zc *A, i64 lda, double *logpf, zc *phase_out)
 
 /* orc_pfaffian_z with the result written through a pointer (plain-C callers: ctypes) */
 void orc_pfaffian_z_out(i64 n, zc *A, i64 lda, zc *out) { *out = orc_pfaffian_z(n, A, lda); }
+
+/* ------------------------------------------------------------------------------------------
+ * Skew-Cholesky (P. Benner et al., "Cholesky-like factorizations of skew-symmetric matrices",
+ * ETNA 11 (2000)): R^T J R = A[p, p] with R upper triangular and J = JMatrix{+1} (the 2 x 2
+ * blocks [0 1; -1 0] on the diagonal).  Restates _skewchol!(A::SkewHermitian{<:Real}, pivot) of
+ * src/cholesky.jl:28-56 (NoPivot) and :58-107 (RowMaximum).  B (n x n, full storage) is
+ * overwritten as in the reference: its upper triangle is R.  p receives the permutation, 0-based.
+ *
+ * Step j (0-based pivot rows i0 = 2j, i1 = 2j + 1), old rows b0 = B[i0, c], b1 = B[i1, c]
+ * (c > i1), r = sqrt(B[i0, i1]) and q = -1/r:
+ *   rows   B[i0, c] = b1 * q,  B[i1, c] = (-b0) * q          (:94-97; J2 = [0 1; -1 0])
+ *   update B[a, b] -= n0[a] * n1[b] - n1[a] * n0[b]          (:98-99, mul! with alpha -1)
+ * The two products of the update are rounded separately (no fused multiply-add), so the
+ * trailing square stays exactly skew-symmetric and its diagonal exactly zero; the reference's
+ * BLAS gemm may fuse, which changes results at the rounding level only.
+ * RowMaximum: the pivot is findmax over the full trailing square (:67-71): the largest value,
+ * the first in column-major order among equal ones (NaN above all numbers, as Julia's findmax);
+ * a value below tol = 1e-15 * ||B||_F ends the factorization, leaving the trailing square as
+ * it is (:71).  Its row index moves to i0 and its column index to i1 by symmetric swaps of whole
+ * rows and columns (:73-90).  NoPivot (:28-56): s = sign(B[i0, i1]), r = sqrt(|B[i0, i1]|),
+ * B[i1, i1] = s r and the first new row is (s b1) * q.
+ * ------------------------------------------------------------------------------------------ */
+static void chol_swap(i64 n, double *B, i64 ldb, i64 x, i64 y)
+{
+    for (i64 t = 0; t < n; t++) { /* columns x <-> y (:77-79) */
+        const double v = B[t + x * ldb];
+        B[t + x * ldb] = B[t + y * ldb];
+        B[t + y * ldb] = v;
+    }
+    for (i64 t = 0; t < n; t++) { /* rows x <-> y (:80-82) */
+        const double v = B[x + t * ldb];
+        B[x + t * ldb] = B[y + t * ldb];
+        B[y + t * ldb] = v;
+    }
+}
+
+/* Julia's isless ordering for findmax: NaN ranks above every number */
+static int chol_greater(double a, double b) { return isnan(b) ? 0 : (isnan(a) || a > b); }
+
+/* returns the number of 2 x 2 steps completed */
+i64 orc_skewchol_d(i64 n, double *B, i64 ldb, int pivot, i64 *p)
+{
+    for (i64 i = 0; i < n; i++) p[i] = i;
+    if (n <= 1) return 0; /* :31, :62 */
+    double tol = 0.0;
+    if (pivot) { /* tol = 1e-15 * norm(B) (:60), column-major sum of squares */
+        double s = 0.0;
+        for (i64 c = 0; c < n; c++)
+            for (i64 r = 0; r < n; r++) s += B[r + c * ldb] * B[r + c * ldb];
+        tol = 1e-15 * sqrt(s);
+    }
+    double *n0 = (double *)malloc((size_t)n * sizeof(double));
+    double *n1 = (double *)malloc((size_t)n * sizeof(double));
+    i64 steps = 0;
+    for (i64 j = 0; j < n / 2; j++) {
+        const i64 i0 = 2 * j, i1 = 2 * j + 1;
+        double s = 1.0;
+        if (pivot) {
+            i64 ii = i0, jj = i0; /* findmax(B[i0:n, i0:n]), column-major, first on ties */
+            double best = B[i0 + i0 * ldb];
+            for (i64 c = i0; c < n; c++)
+                for (i64 r = i0; r < n; r++)
+                    if (chol_greater(B[r + c * ldb], best)) {
+                        best = B[r + c * ldb];
+                        ii = r;
+                        jj = c;
+                    }
+            if (fabs(B[ii + jj * ldb]) < tol) break; /* :71 */
+            const i64 kk = jj == i0 ? ii : jj;      /* :73 */
+            if (ii != i0) {
+                const i64 t = p[ii];
+                p[ii] = p[i0];
+                p[i0] = t;
+                chol_swap(n, B, ldb, ii, i0);
+            }
+            if (kk != i1) {
+                const i64 t = p[kk];
+                p[kk] = p[i1];
+                p[i1] = t;
+                chol_swap(n, B, ldb, kk, i1);
+            }
+        }
+        const double v = B[i0 + i1 * ldb];
+        double r;
+        if (pivot) {
+            r = sqrt(v);
+        } else {
+            s = v > 0 ? 1.0 : v < 0 ? -1.0 : v; /* Julia's sign: +-0 and NaN pass through */
+            r = sqrt(fabs(v));
+        }
+        const double q = -1.0 / r;
+        for (i64 c = i1 + 1; c < n; c++) {
+            const double b0 = B[i0 + c * ldb], b1 = B[i1 + c * ldb];
+            n0[c] = (pivot ? b1 : s * b1) * q;
+            n1[c] = (-b0) * q;
+        }
+        B[i0 + i0 * ldb] = r;
+        B[i1 + i1 * ldb] = pivot ? r : s * r;
+        B[i0 + i1 * ldb] = 0.0;
+        for (i64 c = i1 + 1; c < n; c++) {
+            B[i0 + c * ldb] = n0[c];
+            B[i1 + c * ldb] = n1[c];
+        }
+        for (i64 b = i1 + 1; b < n; b++)
+            for (i64 a = i1 + 1; a < n; a++) {
+                const double x = n0[a] * n1[b], y = n1[a] * n0[b];
+                B[a + b * ldb] -= x - y;
+            }
+        steps++;
+    }
+    free(n0);
+    free(n1);
+    return steps;
+}

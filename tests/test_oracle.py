@@ -405,3 +405,147 @@ def test_complex_pfaffian_planted_slips(golden):
     assert passes(_parlett_reid)
     assert not passes(lambda A: _parlett_reid(A, flip_on_swap=False))
     assert not passes(lambda A: _parlett_reid(A, transposed_update=True))
+
+
+# ---------------------------------------------------------------------------------------------
+# skew-Cholesky (src/cholesky.jl:28-107)
+# ---------------------------------------------------------------------------------------------
+def _skewchol_py(A, pivot=True, slip=None):
+    """Pure-Python restatement of _skewchol! with the oracle's rounding order (small n only).
+    slip plants one mistake: 'row_sign' drops the minus of the second new row, 'row_major_ties'
+    resolves equal maxima in row-major order, 'rows_only' swaps rows but not columns."""
+    B = np.array(A, dtype=np.float64)
+    n = B.shape[0]
+    p = list(range(n))
+    if n <= 1:
+        return np.triu(B), np.array(p), 0
+    tol = 1e-15 * np.sqrt(sum(float(B[r, c]) ** 2 for c in range(n) for r in range(n))) if pivot else 0.0
+
+    def swap(x, y):
+        B[[x, y], :] = B[[y, x], :]
+        if slip != "rows_only":
+            B[:, [x, y]] = B[:, [y, x]]
+
+    steps = 0
+    for j in range(n // 2):
+        i0, i1 = 2 * j, 2 * j + 1
+        s = 1.0
+        if pivot:
+            order = [(r, c) for c in range(i0, n) for r in range(i0, n)]
+            if slip == "row_major_ties":
+                order = [(r, c) for r in range(i0, n) for c in range(i0, n)]
+            ii, jj = order[0]
+            for r, c in order:
+                if B[r, c] > B[ii, jj]:
+                    ii, jj = r, c
+            if abs(B[ii, jj]) < tol:
+                break
+            kk = ii if jj == i0 else jj
+            if ii != i0:
+                p[ii], p[i0] = p[i0], p[ii]
+                swap(ii, i0)
+            if kk != i1:
+                p[kk], p[i1] = p[i1], p[kk]
+                swap(kk, i1)
+        v = B[i0, i1]
+        if pivot:
+            r = np.sqrt(v)
+        else:
+            s = np.sign(v)
+            r = np.sqrt(abs(v))
+        q = -1.0 / r
+        b0 = B[i0, i1 + 1:].copy()
+        b1 = B[i1, i1 + 1:].copy()
+        n0 = (b1 if pivot else s * b1) * q
+        n1 = (b0 if slip == "row_sign" else -b0) * q
+        B[i0, i0] = r
+        B[i1, i1] = r if pivot else s * r
+        B[i0, i1] = 0.0
+        B[i0, i1 + 1:] = n0
+        B[i1, i1 + 1:] = n1
+        B[i1 + 1:, i1 + 1:] -= np.outer(n0, n1) - np.outer(n1, n0)
+        steps += 1
+    return np.triu(B), np.array(p), steps
+
+
+def _chol_residual(A, R, p):
+    n = A.shape[0]
+    return np.abs(R.T @ oracle.jmatrix(n) @ R - A[np.ix_(p, p)]).max() / max(1.0, np.abs(A).max())
+
+
+TIE4 = np.array([[0, 3, -5, 0], [-3, 0, 0, 5], [5, 0, 0, 5], [0, -5, -5, 0]], dtype=float)
+
+
+@pytest.mark.parametrize("n", [1, 2, 10, 11])
+@pytest.mark.parametrize("kind", ["int", "float"])
+def test_skewchol_reference_cases(n, kind):
+    """The reference's own test (test/runtests.jl:436-455): R^T J R = A[p, p] with RowMaximum,
+    R^T J R = A with NoPivot, for skewhermitian(rand(-10:10) * 2) and skewhermitian(randn)."""
+    rng = np.random.default_rng(40 + n)
+    M = rng.integers(-10, 11, size=(n, n)).astype(float) * 2 if kind == "int" else rng.standard_normal((n, n))
+    A = (M - M.T) / 2
+    R, p, steps = oracle.skewchol(A)
+    assert np.array_equal(R, np.triu(R)) and sorted(p) == list(range(n))
+    assert _chol_residual(A, R, p) <= 1e-13
+    R, p, steps = oracle.skewchol(A, pivot=False)
+    assert np.array_equal(p, np.arange(n))
+    assert _chol_residual(A, R, p) <= 1e-12
+
+
+@pytest.mark.parametrize("a", [2.0, -3.0])
+def test_skewchol_closed_forms(a):
+    """n = 2: RowMaximum gives R = sqrt|a| I, with p swapped when a < 0; NoPivot gives
+    R = diag(r, sign(a) r)."""
+    A = np.array([[0.0, a], [-a, 0.0]])
+    r = np.sqrt(abs(a))
+    R, p, steps = oracle.skewchol(A)
+    assert steps == 1 and np.array_equal(R, np.diag([r, r])) and list(p) == ([0, 1] if a > 0 else [1, 0])
+    R, p, steps = oracle.skewchol(A, pivot=False)
+    assert np.array_equal(R, np.diag([r, np.sign(a) * r])) and list(p) == [0, 1]
+
+
+def test_skewchol_tie_rule():
+    """The maximum 5 sits at (2,0), (1,3) and (2,3) of the full square; findmax takes the first
+    in column-major order, (2,0): index 2 moves to position 0, then its partner 0 (now at 2) to
+    position 1, so p starts [2, 0]."""
+    R, p, steps = oracle.skewchol(TIE4)
+    assert list(p[:2]) == [2, 0] and R[0, 0] == np.sqrt(5.0)
+    assert _chol_residual(TIE4, R, p) <= 1e-15
+
+
+def test_skewchol_rank_deficient():
+    """Rank 4 in n = 10: the factorization stops after 2 steps (max below 1e-15 ||A||_F) and
+    R^T J R still reproduces A[p, p]."""
+    rng = np.random.default_rng(8)
+    X = rng.standard_normal((10, 4))
+    A = X @ oracle.jmatrix(4) @ X.T
+    A = (A - A.T) / 2
+    R, p, steps = oracle.skewchol(A)
+    assert steps == 2
+    assert _chol_residual(A, R, p) <= 1e-13
+
+
+@pytest.mark.parametrize("pivot", [True, False])
+def test_skewchol_restatement_matches(pivot):
+    """The pure-Python restatement reproduces the C oracle bit for bit (same rounding order)."""
+    rng = np.random.default_rng(3)
+    for n in (2, 7, 12):
+        M = rng.standard_normal((n, n))
+        A = M - M.T
+        R, p, s = oracle.skewchol(A, pivot)
+        R2, p2, s2 = _skewchol_py(A, pivot)
+        assert np.array_equal(R, R2) and np.array_equal(p, p2) and s == s2
+
+
+def test_skewchol_planted_slips():
+    """Each planted mistake fails a check the oracle passes."""
+    rng = np.random.default_rng(9)
+    M = rng.standard_normal((12, 12))
+    A = M - M.T
+    R, p, _ = _skewchol_py(A, slip="row_sign")
+    assert not _chol_residual(A, R, p) <= 1e-3
+    with np.errstate(invalid="ignore"):
+        R, p, _ = _skewchol_py(A, slip="rows_only")
+    assert not _chol_residual(A, R, p) <= 1e-3
+    R, p, _ = _skewchol_py(TIE4, slip="row_major_ties")
+    assert list(p[:2]) != [2, 0]
