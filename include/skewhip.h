@@ -172,6 +172,29 @@ int skh_pfaffian_z_batched(int64_t batch, const int64_t *n, const int64_t *offse
                            const skh_opts *opts);
 int skh_logabspfaffian_z_batched(int64_t batch, const int64_t *n, const int64_t *offset, const double *Apacked,
                                  double *logpf, double *phase, const skh_opts *opts);
+/*
+ * Skew-Cholesky factorization R^T J R = A[p, p] of a real skew-symmetric A (P. Benner et al., ETNA
+ * 11, 2000), J = JMatrix{+1}: 2 x 2 blocks [0 1; -1 0] on the diagonal.
+ * Replaces skewchol(A, pivot) / skewchol!(A, pivot) (reference src/cholesky.jl:113-151) and
+ * _skewchol!(A::SkewHermitian{<:Real}, ::RowMaximum) (:58-107, pivot = 1, the reference default)
+ * and (..., ::NoPivot) (:28-56, pivot = 0).  Only the strict upper triangle of A is read (A is taken
+ * to be exactly skew-symmetric: the reference's isskewhermitian check is the caller's); A is not
+ * modified.  R (n x n, ldr): upper triangular, strict lower triangle zero.  p: n int64, 0-based
+ * (p = 0..n-1 without pivoting).  steps: 2 x 2 steps completed (n/2 unless the pivot search stops
+ * below tol = 1e-15 ||A||_F, leaving the trailing square in R); may be NULL.  n <= 4096.
+ * One workgroup per matrix, largest matrices first.
+ */
+int skh_skewchol_d(int64_t n, const double *A, int64_t lda, double *R, int64_t ldr, int64_t *p, int pivot,
+                   int64_t *steps, const skh_opts *opts);
+/* Batched: matrix b at element offset offset[b] of Apacked and of Rpacked (ld n[b]); its p at
+ * p + (n[0] + ... + n[b-1]); steps: batch entries (may be NULL).  No reference counterpart for the
+ * batching itself (the reference user loops skewchol). */
+int skh_skewchol_d_batched(int64_t batch, const int64_t *n, const int64_t *offset, const double *Apacked,
+                           double *Rpacked, int64_t *p, int64_t *steps, int pivot, const skh_opts *opts);
+int skh_skewchol_d_batched_device(int64_t batch, const int64_t *n, const int64_t *offset, const double *dA,
+                                  double *dR, int64_t *dp, int64_t *dsteps, int pivot, void *stream,
+                                  const skh_opts *opts);
+
 /* device variants: dA (16-byte aligned) and the outputs in HBM; n and offset are host arrays.
  * Enqueued on `stream`; the call returns once its control arrays are uploaded. */
 int skh_pfaffian_z_batched_device(int64_t batch, const int64_t *n, const int64_t *offset, const double *dA,

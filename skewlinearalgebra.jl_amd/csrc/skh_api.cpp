@@ -825,6 +825,87 @@ int zpf_host(int64_t batch, const int64_t* n, const int64_t* offset, const doubl
     return 0;
 }
 
+// ---- skew-Cholesky (one workgroup per matrix) ----
+Buf g_chctl, g_chin, g_chout, g_chp;
+
+int chol_check(int64_t batch, const int64_t* n, const int64_t* offset, const void* A, const void* R, const void* p,
+               int pivot)
+{
+    if (batch < 0) return set_err(-1, "batch < 0");
+    if (batch > 0 && (!n || !offset || !A || !R || !p)) return set_err(-2, "NULL argument");
+    const int64_t nmax = skh_internal_skewchol_nmax();
+    for (int64_t b = 0; b < batch; b++) {
+        if (n[b] < 0) return set_err(-2, "n[%lld] < 0", (long long)b);
+        if (n[b] > nmax) return set_err(-2, "n[%lld] = %lld exceeds %lld", (long long)b, (long long)n[b], (long long)nmax);
+        if (offset[b] < 0) return set_err(-3, "offset[%lld] < 0", (long long)b);
+    }
+    if (pivot != 0 && pivot != 1) return set_err(-8, "pivot must be 0 (NoPivot) or 1 (RowMaximum)");
+    return 0;
+}
+
+// device arrays; host n/offs.  p of matrix b at dp + (n[0] + ... + n[b-1]).
+int chol_run(int64_t batch, const int64_t* ns, const int64_t* offs, const double* dA, double* dR, int64_t* dp,
+             int64_t* dsteps, int pivot, hipStream_t st)
+{
+    if (batch == 0) return 0;
+    std::vector<int64_t> h(4 * batch);
+    int64_t* order = h.data() + 3 * batch;
+    int64_t pacc = 0;
+    for (int64_t b = 0; b < batch; b++) {
+        h[b] = ns[b];
+        h[batch + b] = offs[b];
+        h[2 * batch + b] = pacc;
+        pacc += ns[b];
+        order[b] = b;
+    }
+    std::stable_sort(order, order + batch, [&](int64_t a, int64_t b) { return ns[a] > ns[b]; });
+    const int64_t nmax = std::max<int64_t>(ns[order[0]], 2);
+    int per_cu = 0;
+    HIPCHK((hipError_t)skh_internal_skewchol_occupancy(nmax, &per_cu));
+    per_cu = std::max(1, per_cu);
+    const int grid = (int)std::min<int64_t>((int64_t)per_cu * g_num_cu, batch);
+    int64_t* dctl = (int64_t*)g_chctl.get(h.size() * sizeof(int64_t));
+    if (!dctl) return set_err(SKH_ERR_HIP, "hipMalloc of skew-Cholesky control arrays failed");
+    HIPCHK(hipMemcpyAsync(dctl, h.data(), h.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    HIPCHK((hipError_t)skh_internal_skewchol(dA, dR, dctl, dctl + batch, dctl + 2 * batch, dctl + 3 * batch, batch,
+                                             nmax, pivot, dp, dsteps, grid, st));
+    // the host copy of the control arrays must outlive the async copy
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int chol_host(int64_t batch, const int64_t* n, const int64_t* offset, const double* A, double* R, int64_t* p,
+              int64_t* steps, int pivot)
+{
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    hipStream_t st = g_stream;
+    std::vector<int64_t> offs2(batch);
+    int64_t tot = 0, ptot = 0;
+    for (int64_t b = 0; b < batch; b++) {
+        offs2[b] = tot;
+        tot += n[b] * n[b];
+        ptot += n[b];
+    }
+    double* dA = (double*)g_chin.get((size_t)std::max<int64_t>(tot, 1) * sizeof(double));
+    double* dR = (double*)g_chout.get((size_t)std::max<int64_t>(tot, 1) * sizeof(double));
+    int64_t* dp = (int64_t*)g_chp.get((size_t)(ptot + batch + 1) * sizeof(int64_t));
+    if (!dA || !dR || !dp) return set_err(SKH_ERR_HIP, "hipMalloc of skew-Cholesky buffers failed");
+    int64_t* dsteps = dp + ptot;
+    for (int64_t b = 0; b < batch; b++)
+        HIPCHK(hipMemcpyAsync(dA + offs2[b], A + offset[b], (size_t)n[b] * n[b] * sizeof(double), hipMemcpyHostToDevice,
+                              st));
+    rc = chol_run(batch, n, offs2.data(), dA, dR, dp, dsteps, pivot, st);
+    if (rc) return rc;
+    for (int64_t b = 0; b < batch; b++)
+        HIPCHK(hipMemcpyAsync(R + offset[b], dR + offs2[b], (size_t)n[b] * n[b] * sizeof(double), hipMemcpyDeviceToHost,
+                              st));
+    HIPCHK(hipMemcpyAsync(p, dp, (size_t)ptot * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    if (steps) HIPCHK(hipMemcpyAsync(steps, dsteps, (size_t)batch * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -847,7 +928,7 @@ int skh_finalize(void)
     (void)hipStreamSynchronize(g_stream);
     for (Buf* b : {&g_mat, &g_ws, &g_vec, &g_ctl, &g_bin, &g_bout, &g_q, &g_ev, &g_gq, &g_gk, &g_gz, &g_gv, &g_gp,
                    &g_gte, &g_cslot, &g_big, &g_bctl, &g_zws, &g_zq, &g_zc, &g_zx, &g_zpslot, &g_zpctl, &g_zpin,
-                   &g_zpout})
+                   &g_zpout, &g_chctl, &g_chin, &g_chout, &g_chp})
         b->release();
     if (g_blas) (void)rocblas_destroy_handle(g_blas);
     g_blas = nullptr;
@@ -1541,6 +1622,50 @@ int skh_logabspfaffian_z_batched_device(int64_t batch, const int64_t* n, const i
     rc = ensure_init_locked(g_device);
     if (rc) return rc;
     return zpf_run(batch, n, offset, (const double2*)dA, 1, dlogpf, dphase, (hipStream_t)stream);
+}
+
+int skh_skewchol_d_batched(int64_t batch, const int64_t* n, const int64_t* offset, const double* Apacked,
+                           double* Rpacked, int64_t* p, int64_t* steps, int pivot, const skh_opts* opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    (void)opts;
+    int rc = chol_check(batch, n, offset, Apacked, Rpacked, p, pivot);
+    if (rc || batch == 0) return rc;
+    return chol_host(batch, n, offset, Apacked, Rpacked, p, steps, pivot);
+}
+
+int skh_skewchol_d_batched_device(int64_t batch, const int64_t* n, const int64_t* offset, const double* dA,
+                                  double* dR, int64_t* dp, int64_t* dsteps, int pivot, void* stream,
+                                  const skh_opts* opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    (void)opts;
+    int rc = chol_check(batch, n, offset, dA, dR, dp, pivot);
+    if (rc || batch == 0) return rc;
+    rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    return chol_run(batch, n, offset, dA, dR, dp, dsteps, pivot, (hipStream_t)stream);
+}
+
+int skh_skewchol_d(int64_t n, const double* A, int64_t lda, double* R, int64_t ldr, int64_t* p, int pivot,
+                   int64_t* steps, const skh_opts* opts)
+{
+    if (n < 0) return set_err(-1, "n < 0");
+    if (n > 0 && (!A || !R || !p)) return set_err(-2, "NULL argument");
+    if (lda < std::max<int64_t>(n, 1)) return set_err(-3, "lda < n");
+    if (ldr < std::max<int64_t>(n, 1)) return set_err(-5, "ldr < n");
+    if (pivot != 0 && pivot != 1) return set_err(-7, "pivot must be 0 (NoPivot) or 1 (RowMaximum)");
+    if (n > skh_internal_skewchol_nmax()) return set_err(-1, "n = %lld exceeds %d", (long long)n,
+                                                         skh_internal_skewchol_nmax());
+    if (steps) *steps = 0;
+    if (n == 0) return 0;
+    std::vector<double> a((size_t)n * n), r((size_t)n * n);
+    for (int64_t c = 0; c < n; c++) std::memcpy(&a[c * n], A + c * lda, n * sizeof(double));
+    const int64_t off = 0;
+    int rc = skh_skewchol_d_batched(1, &n, &off, a.data(), r.data(), p, steps, pivot, opts);
+    if (rc) return rc;
+    for (int64_t c = 0; c < n; c++) std::memcpy(R + c * ldr, &r[c * n], n * sizeof(double));
+    return 0;
 }
 
 int skh_eigvals_range_d(int64_t n, const double* A, int64_t lda, int64_t il, int64_t iu, double vl, double vh,

@@ -65,6 +65,12 @@ int skh_internal_zpfaff(const double2* A, const int64_t* ns, const int64_t* offs
                         hipStream_t st);
 int skh_internal_zpfaff_occupancy(int64_t nmax, int* per_cu);
 int skh_internal_zpfaff_nmax();
+// skew-Cholesky (skh_chol.hip): matrix b at A + offs[b] and R + offs[b] (ld n[b]), p + poffs[b]
+int skh_internal_skewchol(const double* A, double* R, const int64_t* ns, const int64_t* offs, const int64_t* poffs,
+                          const int64_t* order, int64_t batch, int64_t nmax, int pivot, int64_t* p, int64_t* steps,
+                          int grid, hipStream_t st);
+int skh_internal_skewchol_occupancy(int64_t nmax, int* per_cu);
+int skh_internal_skewchol_nmax();
 int skh_internal_launch_compact(const CompactArgs* a, int nmax, int grid, hipStream_t st);
 int skh_internal_logpf_from(const double* eta, const double* tau, int64_t n, double* logpf, double* sgn, hipStream_t st);
 int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st);

@@ -28,6 +28,7 @@ __all__ = [
     "hessenberg", "hessenberg_", "eigvals", "eigvals_", "eigen", "eigen_", "Eigen", "pfaffian", "pfaffian_",
     "logabspfaffian", "logabspfaffian_", "logabspfaffian_batched", "logabspfaffian_grad_batched",
     "LogAbsPfaffian", "SkhError", "pfaffian_z_batched", "logabspfaffian_z_batched",
+    "JMatrix", "SkewCholesky", "RowMaximum", "NoPivot", "skewchol", "skewchol_", "skewchol_batched",
     "ENGINE_AUTO", "ENGINE_LARGE", "ENGINE_TEAM1", "version", "LogPfPlan",
 ]
 
@@ -564,6 +565,115 @@ def logabspfaffian_z_batched(mats):
                                                   None)
     _lib.check("skh_logabspfaffian_z_batched", rc)
     return lp, ph
+
+
+# ----------------------------------------------------------------------------------------
+# skew-Cholesky (src/cholesky.jl, src/jmatrix.jl)
+# ----------------------------------------------------------------------------------------
+class RowMaximum:
+    """Complete pivoting (LinearAlgebra.RowMaximum), the default of skewchol."""
+
+
+class NoPivot:
+    """No pivoting (LinearAlgebra.NoPivot)."""
+
+
+class JMatrix:
+    """JMatrix{T, SGN}(n) (src/jmatrix.jl:9-30): J[i, i+1] = SGN, J[i+1, i] = -SGN for even i
+    (0-based); a structured matrix, materialised by numpy.asarray."""
+
+    def __init__(self, n, sgn=+1):
+        if n < 0:
+            raise ArgumentError("size %d must be >= 0" % n)
+        if sgn not in (+1, -1):
+            raise ArgumentError("SGN parameter must be +-1")
+        self.n = int(n)
+        self.sgn = int(sgn)
+
+    @property
+    def shape(self):
+        return (self.n, self.n)
+
+    def __array__(self, dtype=None, copy=None):
+        M = np.zeros((self.n, self.n), dtype=dtype or np.float64)
+        i = np.arange(0, self.n - 1, 2)
+        M[i, i + 1] = self.sgn
+        M[i + 1, i] = -self.sgn
+        return M
+
+
+class SkewCholesky:
+    """SkewCholesky(R, p) (src/cholesky.jl:3-26): R^T J R = A[p, p], J = JMatrix(n, +1);
+    p is 0-based."""
+
+    def __init__(self, R, p, steps=None):
+        self.R = R
+        self.J = JMatrix(R.shape[0], +1)
+        self.p = p
+        self.steps = steps
+
+    def __iter__(self):
+        return iter((self.R, self.J, self.p))
+
+
+def _pivot_flag(pivot):
+    if pivot is RowMaximum or isinstance(pivot, RowMaximum):
+        return 1
+    if pivot is NoPivot or isinstance(pivot, NoPivot):
+        return 0
+    raise TypeError("pivot must be RowMaximum() or NoPivot()")
+
+
+def _real_skew(A, check):
+    D = A.data if isinstance(A, SkewHermitian) else np.asarray(A)
+    if np.iscomplexobj(D):
+        raise TypeError("skewchol is defined for real skew-symmetric matrices (src/cholesky.jl:28, :58)")
+    if check and not isinstance(A, SkewHermitian) and not isskewhermitian(D):
+        raise ArgumentError("Pfaffian requires a skew-Hermitian matrix")
+    return D
+
+
+def skewchol_batched(mats, pivot=RowMaximum):
+    """skewchol of each matrix (uneven sizes, n <= 4096) in one GPU launch; returns a list of
+    SkewCholesky.  Each matrix must be exactly skew-symmetric (checked)."""
+    flag = _pivot_flag(pivot)
+    mats = [np.asarray(_real_skew(M, True), dtype=np.float64) for M in mats]
+    ns = np.array([M.shape[0] for M in mats], dtype=np.int64)
+    offs = np.zeros(len(mats), dtype=np.int64)
+    if len(mats) > 1:
+        offs[1:] = np.cumsum(ns[:-1] ** 2)
+    packed = np.concatenate([M.ravel(order="F") for M in mats]) if mats else np.zeros(1)
+    R = np.zeros_like(packed)
+    p = np.zeros(max(1, int(ns.sum())), dtype=np.int64)
+    steps = np.zeros(max(1, len(mats)), dtype=np.int64)
+    if mats:
+        rc = _lib.lib().skh_skewchol_d_batched(len(mats), _ptr(ns), _ptr(offs), _ptr(packed), _ptr(R), _ptr(p),
+                                               _ptr(steps), flag, None)
+        _lib.check("skh_skewchol_d_batched", rc)
+    out, pacc = [], 0
+    for k, n in enumerate(ns):
+        n = int(n)
+        Rk = R[offs[k]:offs[k] + n * n].reshape((n, n), order="F").copy()
+        out.append(SkewCholesky(Rk, p[pacc:pacc + n].copy(), int(steps[k])))
+        pacc += n
+    return out
+
+
+def skewchol(A, pivot=RowMaximum):
+    """skewchol(A, pivot) (src/cholesky.jl:134-151): R^T J R = A[p, p] with RowMaximum (the
+    default), R^T J R = A with NoPivot.  A (not modified) must be exactly skew-symmetric."""
+    return skewchol_batched([_real_skew(A, True)], pivot)[0]
+
+
+def skewchol_(A, pivot=RowMaximum):
+    """skewchol!(A, pivot) (src/cholesky.jl:109-127): as skewchol, with A (a float64 array or a
+    SkewHermitian's data) overwritten by R, which the result then shares."""
+    D = _real_skew(A, True)
+    C = skewchol(D, pivot)
+    if isinstance(D, np.ndarray) and D.dtype == np.float64:
+        D[...] = C.R
+        C.R = D
+    return C
 
 
 def logabspfaffian_batched(mats, nb=0, engine=ENGINE_AUTO):

@@ -70,6 +70,9 @@ def lib():
         "skh_orgtr_z_device": ([i64, vp, i64, vp, vp, i64, vp], ctypes.c_int),
         "skh_eigvals_z": ([i64, vp, i64, vp, op], ctypes.c_int),
         "skh_eigen_z": ([i64, vp, i64, vp, vp, op], ctypes.c_int),
+        "skh_skewchol_d": ([i64, vp, i64, vp, i64, vp, ctypes.c_int, vp, op], ctypes.c_int),
+        "skh_skewchol_d_batched": ([i64, vp, vp, vp, vp, vp, vp, ctypes.c_int, op], ctypes.c_int),
+        "skh_skewchol_d_batched_device": ([i64, vp, vp, vp, vp, vp, vp, ctypes.c_int, vp, op], ctypes.c_int),
         "skh_pfaffian_z_batched": ([i64, vp, vp, vp, vp, op], ctypes.c_int),
         "skh_pfaffian_z_batched_device": ([i64, vp, vp, vp, vp, vp, op], ctypes.c_int),
         "skh_logabspfaffian_z_batched": ([i64, vp, vp, vp, vp, vp, op], ctypes.c_int),
