@@ -19,9 +19,11 @@ Also reported:
                 the residual / orthogonality of A = Q T Q^H
   complex_pfaffian  4096 complex skew-symmetric matrices (n in {32,...,256}), log|Pf| and phase by
                 pivoted Parlett-Reid in one batched call, matrices/s, against the oracle on a sample
+  skewchol      4096 real skew matrices (n in {32,...,256}), skew-Cholesky with complete pivoting in
+                one batched call, matrices/s, plus one n = 4096 matrix alone, against the oracle
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 8192] [--nb 0] [--no-cpu] [--no-batched]
-                       [--no-complex]
+                       [--no-complex] [--only complex_pfaffian|skewchol]
 """
 import argparse
 import ctypes
@@ -361,6 +363,83 @@ def zpfaff_bench(torch, _lib, steps, count=4096, seed=2718):
                              "sample": "oracle on every 65th matrix (%d matrices, %.2f s)" % (len(sample), cpu_s)}}
 
 
+def skewchol_bench(torch, _lib, steps, count=4096, seed=1618, big_n=4096):
+    """Skew-Cholesky with complete pivoting (the reference's skewchol(A), RowMaximum): `count`
+    real matrices A = M - M^T, n cycling through 32, 64, ..., 256, M iid N(0, 1), resident in HBM,
+    one batched device call per step; then one n = 4096 matrix alone (one workgroup).  The oracle
+    (C, one core) on every 65th matrix gives the CPU rate and the bitwise agreement."""
+    import numpy as np
+    import oracle
+    dev = torch.device("cuda:0")
+    ns = np.array([32 * (1 + k % 8) for k in range(count)], dtype=np.int64)
+    offs = np.zeros(count, dtype=np.int64)
+    offs[1:] = np.cumsum(ns[:-1] ** 2)
+    tot = int(offs[-1] + ns[-1] ** 2)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    buf = torch.empty(tot, dtype=torch.float64, device=dev)
+    for n in np.unique(ns):
+        idx = np.nonzero(ns == n)[0]
+        Mm = torch.randn((len(idx), int(n), int(n)), generator=g, dtype=torch.float64, device=dev)
+        Am = Mm - Mm.transpose(1, 2)
+        for q, k in enumerate(idx):
+            buf[offs[k]:offs[k] + n * n] = Am[q].T.reshape(-1)
+    R = torch.empty_like(buf)
+    p = torch.empty(int(ns.sum()), dtype=torch.int64, device=dev)
+    stp = torch.empty(count, dtype=torch.int64, device=dev)
+    st = torch.cuda.current_stream().cuda_stream
+    L = _lib.lib()
+
+    def step():
+        _lib.check("skh_skewchol_d_batched_device", L.skh_skewchol_d_batched_device(
+            count, ns.ctypes.data, offs.ctypes.data, buf.data_ptr(), R.data_ptr(), p.data_ptr(), stp.data_ptr(), 1,
+            st, None))
+
+    step()
+    torch.cuda.synchronize()
+    ms = timed(torch, step, max(3, steps))
+    # n^3/3 flops: sum over steps of 2 products + 2 subtractions per entry of the trailing triangle
+    flops = float(np.sum(ns.astype(np.float64) ** 3 / 3.0))
+    host = buf.cpu().numpy()
+    hR = R.cpu().numpy()
+    sample = np.arange(0, count, 65)
+    same = True
+    t0 = time.perf_counter()
+    outs = []
+    for k in sample:
+        n = int(ns[k])
+        outs.append(oracle.skewchol(host[int(offs[k]):int(offs[k] + n * n)].reshape((n, n), order="F")))
+    cpu_s = time.perf_counter() - t0
+    for k, (Ro, po, so) in zip(sample, outs):
+        n = int(ns[k])
+        same = same and bool(np.array_equal(hR[int(offs[k]):int(offs[k] + n * n)].reshape((n, n), order="F"), Ro))
+    # one large matrix
+    Mb = torch.randn((big_n, big_n), generator=g, dtype=torch.float64, device=dev)
+    Ab = (Mb - Mb.T).T.contiguous()
+    del Mb
+    Rb = torch.empty_like(Ab)
+    pb = torch.empty(big_n, dtype=torch.int64, device=dev)
+    one_n = np.array([big_n], dtype=np.int64)
+    one_off = np.zeros(1, dtype=np.int64)
+
+    def big():
+        _lib.check("skh_skewchol_d_batched_device", L.skh_skewchol_d_batched_device(
+            1, one_n.ctypes.data, one_off.ctypes.data, Ab.data_ptr(), Rb.data_ptr(), pb.data_ptr(), None, 1, st,
+            None))
+
+    big()
+    torch.cuda.synchronize()
+    big_ms = timed(torch, big, 1)
+    return {"metric": "batched skew-Cholesky with complete pivoting, %d matrices n in {32,...,256}" % count,
+            "value": count / (ms * 1e-3), "unit": "matrices/s", "ms_per_batch": ms,
+            "gflops": flops / (ms * 1e-3) / 1e9, "flop_count": "n^3/3 flops per matrix",
+            "bitwise_equal_to_oracle_on_sample": same,
+            "single_n%d_ms" % big_n: big_ms,
+            "cpu_baseline": {"value": len(sample) / cpu_s, "unit": "matrices/s (same size mix)", "cores": 1,
+                             "kind": "port",
+                             "sample": "oracle on every 65th matrix (%d matrices, %.2f s)" % (len(sample), cpu_s)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -372,6 +451,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched log|Pf| and gradient lines")
     ap.add_argument("--no-complex", action="store_true", help="skip the complex n=4096 and complex Pfaffian lines")
+    ap.add_argument("--only", choices=["complex_pfaffian", "skewchol"],
+                    help="run just this secondary line and print its JSON (for profiling it alone)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the single-matrix tridiagonalization is a one-GPU "
@@ -381,6 +462,11 @@ def main():
     import torch
     import skewlinearalgebra as skl
     from skewlinearalgebra import _lib
+
+    if args.only:
+        fn = zpfaff_bench if args.only == "complex_pfaffian" else skewchol_bench
+        print(json.dumps(fn(torch, _lib, args.steps)), flush=True)
+        return
 
     n = args.n
     dev = torch.device("cuda:0")
@@ -493,6 +579,10 @@ def main():
         log("complex skew-symmetric Pfaffians (4096 matrices) ...")
         res["complex_pfaffian"] = zpfaff_bench(torch, _lib, args.steps)
         log(json.dumps(res["complex_pfaffian"]))
+    if not args.no_batched:
+        log("skew-Cholesky (4096 matrices, one n=4096) ...")
+        res["skewchol"] = skewchol_bench(torch, _lib, args.steps)
+        log(json.dumps(res["skewchol"]))
     if not args.no_cpu:
         log("cpu baseline (n=%d) ..." % args.cpu_n)
         res["cpu_baseline"] = cpu_baseline(args.cpu_n)

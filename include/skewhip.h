@@ -182,7 +182,8 @@ int skh_logabspfaffian_z_batched(int64_t batch, const int64_t *n, const int64_t 
  * modified.  R (n x n, ldr): upper triangular, strict lower triangle zero.  p: n int64, 0-based
  * (p = 0..n-1 without pivoting).  steps: 2 x 2 steps completed (n/2 unless the pivot search stops
  * below tol = 1e-15 ||A||_F, leaving the trailing square in R); may be NULL.  n <= 4096.
- * One workgroup per matrix, largest matrices first.
+ * Matrices with n >= 512 are factored one at a time by a cooperative kernel over the whole GPU,
+ * smaller ones one workgroup per matrix, largest first; both give the same bits.
  */
 int skh_skewchol_d(int64_t n, const double *A, int64_t lda, double *R, int64_t ldr, int64_t *p, int pivot,
                    int64_t *steps, const skh_opts *opts);
@@ -191,6 +192,8 @@ int skh_skewchol_d(int64_t n, const double *A, int64_t lda, double *R, int64_t l
  * batching itself (the reference user loops skewchol). */
 int skh_skewchol_d_batched(int64_t batch, const int64_t *n, const int64_t *offset, const double *Apacked,
                            double *Rpacked, int64_t *p, int64_t *steps, int pivot, const skh_opts *opts);
+/* device variant (dA, dR, dp, dsteps in HBM; n, offset host arrays): returns once the members with
+ * n >= 512 are done and the control arrays of the rest are uploaded; those finish on `stream`. */
 int skh_skewchol_d_batched_device(int64_t batch, const int64_t *n, const int64_t *offset, const double *dA,
                                   double *dR, int64_t *dp, int64_t *dsteps, int pivot, void *stream,
                                   const skh_opts *opts);

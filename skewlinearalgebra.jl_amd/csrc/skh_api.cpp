@@ -825,8 +825,9 @@ int zpf_host(int64_t batch, const int64_t* n, const int64_t* offset, const doubl
     return 0;
 }
 
-// ---- skew-Cholesky (one workgroup per matrix) ----
-Buf g_chctl, g_chin, g_chout, g_chp;
+// ---- skew-Cholesky: one workgroup per matrix, or the whole GPU for one large matrix ----
+Buf g_chctl, g_chin, g_chout, g_chp, g_chws;
+constexpr int64_t CHOL_COOP_MIN = 512;   // matrices this large get the cooperative kernel
 
 int chol_check(int64_t batch, const int64_t* n, const int64_t* offset, const void* A, const void* R, const void* p,
                int pivot)
@@ -859,16 +860,42 @@ int chol_run(int64_t batch, const int64_t* ns, const int64_t* offs, const double
         order[b] = b;
     }
     std::stable_sort(order, order + batch, [&](int64_t a, int64_t b) { return ns[a] > ns[b]; });
-    const int64_t nmax = std::max<int64_t>(ns[order[0]], 2);
+    // large members: one cooperative launch each over the whole GPU
+    int64_t nbig = 0;
+    while (nbig < batch && ns[order[nbig]] >= CHOL_COOP_MIN) nbig++;
+    if (nbig > 0) {
+        Ctl ctl;
+        int rc = get_ctl(ctl);
+        if (rc) return rc;
+        int per_cu = 0;
+        HIPCHK((hipError_t)skh_internal_skewchol_coop_occupancy(&per_cu));
+        per_cu = std::max(1, std::min(per_cu, 2));
+        for (int64_t q = 0; q < nbig; q++) {
+            const int64_t b = order[q], n = ns[b];
+            const int64_t nt = (n + 63) / 64;
+            const int grid = (int)std::min<int64_t>((int64_t)per_cu * g_num_cu, nt * (nt + 1) / 2);
+            double* ws = (double*)g_chws.get((size_t)(4 * n + 3 * grid + 1) * sizeof(double));
+            if (!ws) return set_err(SKH_ERR_HIP, "hipMalloc of skew-Cholesky workspace failed");
+            int64_t* st1 = dsteps ? dsteps + b : (int64_t*)(ws + 4 * n + 3 * grid);
+            HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 512, st));
+            HIPCHK((hipError_t)skh_internal_skewchol_coop(dA + offs[b], dR + offs[b], n, pivot, dp + h[2 * batch + b],
+                                                          st1, ws, ws + 4 * n, ctl.bar, ctl.err, grid, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        rc = check_barrier_flag();
+        if (rc) return rc;
+        if (nbig == batch) return 0;
+    }
+    const int64_t nmax = std::max<int64_t>(ns[order[nbig]], 2);
     int per_cu = 0;
     HIPCHK((hipError_t)skh_internal_skewchol_occupancy(nmax, &per_cu));
     per_cu = std::max(1, per_cu);
-    const int grid = (int)std::min<int64_t>((int64_t)per_cu * g_num_cu, batch);
+    const int grid = (int)std::min<int64_t>((int64_t)per_cu * g_num_cu, batch - nbig);
     int64_t* dctl = (int64_t*)g_chctl.get(h.size() * sizeof(int64_t));
     if (!dctl) return set_err(SKH_ERR_HIP, "hipMalloc of skew-Cholesky control arrays failed");
     HIPCHK(hipMemcpyAsync(dctl, h.data(), h.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
-    HIPCHK((hipError_t)skh_internal_skewchol(dA, dR, dctl, dctl + batch, dctl + 2 * batch, dctl + 3 * batch, batch,
-                                             nmax, pivot, dp, dsteps, grid, st));
+    HIPCHK((hipError_t)skh_internal_skewchol(dA, dR, dctl, dctl + batch, dctl + 2 * batch, dctl + 3 * batch + nbig,
+                                             batch - nbig, nmax, pivot, dp, dsteps, grid, st));
     // the host copy of the control arrays must outlive the async copy
     HIPCHK(hipStreamSynchronize(st));
     return 0;
@@ -928,7 +955,8 @@ int skh_finalize(void)
     (void)hipStreamSynchronize(g_stream);
     for (Buf* b : {&g_mat, &g_ws, &g_vec, &g_ctl, &g_bin, &g_bout, &g_q, &g_ev, &g_gq, &g_gk, &g_gz, &g_gv, &g_gp,
                    &g_gte, &g_cslot, &g_big, &g_bctl, &g_zws, &g_zq, &g_zc, &g_zx, &g_zpslot, &g_zpctl, &g_zpin,
-                   &g_zpout, &g_chctl, &g_chin, &g_chout, &g_chp})
+                   &g_zpout, &g_chctl, &g_chin, &g_chout, &g_chp,
+                   &g_chws})
         b->release();
     if (g_blas) (void)rocblas_destroy_handle(g_blas);
     g_blas = nullptr;

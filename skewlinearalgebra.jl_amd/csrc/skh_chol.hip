@@ -22,6 +22,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "skh_device.hpp"
+
 namespace skh {
 namespace chol {
 
@@ -272,10 +274,350 @@ __global__ __launch_bounds__(CW) void k_skewchol(const double* __restrict__ A, d
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// One large matrix over the whole GPU (cooperative launch, G workgroups).  The upper triangle, in
+// position space (rows and columns as permuted so far), is cut into 64 x 64 tiles with fixed
+// owners: tile k of the list ordered by decreasing block row goes to workgroup k mod G, so the
+// tiles still inside the shrinking trailing square are always a prefix of the list and spread
+// evenly.  A workgroup only ever reads and writes its own tiles with plain loads and stores; what
+// crosses workgroups goes through agent-scope coherent accesses (ld_co / st_co) around the
+// fence-free grid barrier: the argmax candidates, the norm partials and, each step, the "lines"
+// of the (at most four) positions the step touches: i0, i1 and the pivot's row and column.  The
+// line of position m holds the raw upper-storage entries U[min(t, m), max(t, m)] for every t.
+// The step's swaps (ii <-> i0, then kk <-> i1) form a permutation s of those positions; the new
+// entry at (a, b) is the old one at (s(a), s(b)) (negated when s reverses the order, for the
+// trailing square; unchanged for the finished rows of R above i0).  Per step: candidates -> pivot
+// -> publish lines -> barrier -> the new rows n0, n1 and the pivot block from the lines, the
+// rank-2 update of the own tiles (old values of entries in moved rows or columns from the lines),
+// the moved columns of R, new candidates -> barrier.  The arithmetic is that of chol_one.
+// ------------------------------------------------------------------------------------------
+constexpr int TB = 64;
+
+struct CoopArgs {
+    const double* A;   // n x n, ld n (read above the diagonal)
+    double* U;         // n x n, ld n: R on return
+    int64_t* p;        // n, 0-based
+    int64_t* steps;    // 1
+    double* W;         // 4 lines of n
+    double* part;      // 3 G words: candidate value, packed position, norm partial
+    unsigned* bar;     // grid barrier counters (zeroed before the launch)
+    int* err;
+    int n;
+    int pivot;
+};
+
+__device__ __forceinline__ void tile_of(int64_t k, int nt, int& I, int& J)
+{
+    // k = m (m + 1) / 2 + (J - I) with m = nt - 1 - I
+    int m = (int)((sqrt(8.0 * (double)k + 1.0) - 1.0) * 0.5);
+    while ((int64_t)(m + 1) * (m + 2) / 2 <= k) m++;
+    while ((int64_t)m * (m + 1) / 2 > k) m--;
+    I = nt - 1 - m;
+    J = I + (int)(k - (int64_t)m * (m + 1) / 2);
+}
+
+// packed candidate position (row, column) in a double-sized word
+__device__ __forceinline__ double pack_pos(int r, int c)
+{
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)c << 32) | (unsigned)r);
+}
+__device__ __forceinline__ void unpack_pos(double w, int& r, int& c)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, w);
+    r = (int)(u & 0xffffffffu);
+    c = (int)(u >> 32);
+}
+
+struct CoopShared {
+    double n0r[TB], n1r[TB], n0c[TB], n1c[TB];   // new rows at the tile's row and column ranges
+    double bv[4];
+    int br[4], bc[4];
+    double red[4];
+    double best, tol;
+    int ii, jj;
+    int ok;
+};
+
+// workgroup-wide min-order reduction of candidates into s.best / s.ii / s.jj (all threads call)
+__device__ void coop_reduce(CoopShared& s, double bv, int br, int bc)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const double v2 = __shfl_xor(bv, o, 64);
+        const int r2 = __shfl_xor(br, o, 64), c2 = __shfl_xor(bc, o, 64);
+        if (cand_better(v2, r2, c2, bv, br, bc)) {
+            bv = v2;
+            br = r2;
+            bc = c2;
+        }
+    }
+    if (lane == 0) {
+        s.bv[wave] = bv;
+        s.br[wave] = br;
+        s.bc[wave] = bc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = s.bv[0];
+        int r = s.br[0], c = s.bc[0];
+        for (int w = 1; w < CW / 64; w++)
+            if (cand_better(s.bv[w], s.br[w], s.bc[w], v, r, c)) {
+                v = s.bv[w];
+                r = s.br[w];
+                c = s.bc[w];
+            }
+        s.best = v;
+        s.ii = r;
+        s.jj = c;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(CW) void k_skewchol_coop(CoopArgs g)
+{
+#pragma clang fp contract(off)
+    __shared__ CoopShared s;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n = g.n, G = gridDim.x, gi = blockIdx.x;
+    const int nt = (n + TB - 1) / TB;
+    const int64_t T = (int64_t)nt * (nt + 1) / 2;
+    double* __restrict__ U = g.U;
+    unsigned epoch = 0;
+
+    // ---- copy the own tiles (strict upper part; zeros elsewhere), norm and first candidates ----
+    double ss = 0.0, bv = 0.0;
+    int br = 0, bc = 0;
+    for (int64_t k = gi; k < T; k += G) {
+        int I, J;
+        tile_of(k, nt, I, J);
+        const int a = I * TB + lane;
+        for (int cc = wave; cc < TB; cc += CW / 64) {
+            const int b = J * TB + cc;
+            if (a < n && b < n) {
+                const double v = a < b ? g.A[a + (int64_t)b * n] : 0.0;
+                U[a + (int64_t)b * n] = v;
+                ss += v * v;
+                if (g.pivot && a < b) offer(bv, br, bc, v, a, b);
+            }
+        }
+    }
+    // blocks below the diagonal belong to no tile: zero them (never read in this launch)
+    for (int64_t k = gi; k < (int64_t)nt * nt; k += G) {
+        const int I = (int)(k % nt), J = (int)(k / nt);
+        if (I <= J) continue;
+        const int a = I * TB + lane;
+        for (int cc = wave; cc < TB; cc += CW / 64) {
+            const int b = J * TB + cc;
+            if (a < n && b < n) U[a + (int64_t)b * n] = 0.0;
+        }
+    }
+    if (gi == 0)
+        for (int i = tid; i < n; i += CW) g.p[i] = i;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) ss += __shfl_xor(ss, o, 64);
+    if (lane == 0) s.red[wave] = ss;
+    coop_reduce(s, bv, br, bc);   // its barriers also publish s.red
+    if (tid == 0) {
+        st_co(g.part + 3 * gi, s.best);
+        st_co(g.part + 3 * gi + 1, pack_pos(s.ii, s.jj));
+        st_co(g.part + 3 * gi + 2, (s.red[0] + s.red[1]) + (s.red[2] + s.red[3]));
+    }
+    if (!grid_sync_co(g.bar, ++epoch, g.err, &s.ok)) return;
+    // tol = 1e-15 ||A||_F, the same sum order in every workgroup
+    {
+        double t = 0.0;
+        for (int q = tid; q < G; q += CW) t += ld_co(g.part + 3 * q + 2);
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) t += __shfl_xor(t, o, 64);
+        if (lane == 0) s.red[wave] = t;
+        __syncthreads();
+        if (tid == 0) s.tol = 1e-15 * sqrt(2.0 * ((s.red[0] + s.red[1]) + (s.red[2] + s.red[3])));
+        __syncthreads();
+    }
+
+    int64_t steps = 0;
+    for (int j = 0; j < n / 2; j++) {
+        const int i0 = 2 * j, i1 = 2 * j + 1, t0 = i1 + 1;
+        // ---- pivot from every workgroup's candidate ----
+        int ii = i0, kk = i1;
+        if (g.pivot) {
+            double v = 0.0;
+            int r = i0, c = i0;
+            for (int q = tid; q < G; q += CW) {
+                int r2, c2;
+                const double v2 = ld_co(g.part + 3 * q);
+                unpack_pos(ld_co(g.part + 3 * q + 1), r2, c2);
+                if (cand_better(v2, r2, c2, v, r, c)) {
+                    v = v2;
+                    r = r2;
+                    c = c2;
+                }
+            }
+            coop_reduce(s, v, r, c);
+            if (fabs(s.best) < s.tol) break;   // the same words in every workgroup
+            ii = s.ii;
+            const int jj = s.jj;
+            kk = jj == i0 ? ii : jj;
+            if (gi == 0 && tid == 0) {
+                if (ii != i0) {
+                    const int64_t t = g.p[ii];
+                    g.p[ii] = g.p[i0];
+                    g.p[i0] = t;
+                }
+                if (kk != i1) {
+                    const int64_t t = g.p[kk];
+                    g.p[kk] = g.p[i1];
+                    g.p[i1] = t;
+                }
+            }
+        }
+        // the step's positions: slots 0..3 hold i0, i1, ii, kk (duplicates resolve to the first)
+        const int P[4] = {i0, i1, ii, kk};
+        auto slot = [&](int m) { return m == i0 ? 0 : m == i1 ? 1 : m == ii ? 2 : 3; };
+        auto inP = [&](int m) { return m == i0 || m == i1 || m == ii || m == kk; };
+        // s(a): the old position of new position a (kk <-> i1 first, then ii <-> i0)
+        auto sig = [&](int a) {
+            const int t = a == kk ? i1 : a == i1 ? kk : a;
+            return t == ii ? i0 : t == i0 ? ii : t;
+        };
+        // ---- publish the lines of P from the own tiles ----
+        for (int64_t k = gi; k < T; k += G) {
+            int I, J;
+            tile_of(k, nt, I, J);
+            for (int e = 0; e < 4; e++) {
+                const int m = P[e];
+                if (slot(m) != e) continue;
+                double* Wm = g.W + (int64_t)e * n;
+                if (m / TB == J)   // column m: rows t <= m of this tile
+                    for (int t = I * TB + tid; t < min(I * TB + TB, n); t += CW)
+                        if (t <= m) st_co(Wm + t, U[t + (int64_t)m * n]);
+                if (m / TB == I)   // row m: columns t >= m of this tile
+                    for (int t = J * TB + tid; t < min(J * TB + TB, n); t += CW)
+                        if (t >= m) st_co(Wm + t, U[m + (int64_t)t * n]);
+            }
+        }
+        if (!grid_sync_co(g.bar, ++epoch, g.err, &s.ok)) return;
+        // new trailing entry (x, y), x != y, both >= i0, with x or y moved: the old one at
+        // (s(x), s(y)), whose line is published
+        auto lineval = [&](int u, int v) {
+            return inP(u) ? ld_co(g.W + (int64_t)slot(u) * n + v) : ld_co(g.W + (int64_t)slot(v) * n + u);
+        };
+        auto snew = [&](int x, int y) {
+            const int u = sig(x), v = sig(y);
+            const double w = lineval(u, v);
+            return u < v ? w : -w;
+        };
+        // pivot block
+        const double pv = snew(i0, i1);
+        double sg = 1.0, r;
+        if (g.pivot) {
+            r = sqrt(pv);
+        } else {
+            sg = pv > 0.0 ? 1.0 : pv < 0.0 ? -1.0 : pv;
+            r = sqrt(fabs(pv));
+        }
+        const double q = -1.0 / r;
+        const bool moved = ii != i0 || kk != i1;
+        // ---- own tiles ----
+        bv = 0.0;
+        br = t0;
+        bc = t0;
+        const int I0 = i0 / TB;   // block row of the pivot rows (i0 and i1 share it: TB is even)
+        const int64_t K = (int64_t)(nt - I0) * (nt - I0 + 1) / 2;   // tiles at or below block row I0
+        for (int64_t k = gi; k < T; k += G) {
+            int I, J;
+            tile_of(k, nt, I, J);
+            const int r0 = I * TB, c0 = J * TB;
+            // finished rows of R in moved columns: U[a, b] = old U[a, s(b)], a < i0
+            if (moved)
+                for (int e = 0; e < 4; e++) {
+                    const int b = P[e];
+                    if (slot(b) != e || b / TB != J || sig(b) == b) continue;
+                    const int sb = sig(b);
+                    for (int a = r0 + tid; a < min(r0 + TB, i0); a += CW)
+                        U[a + (int64_t)b * n] = ld_co(g.W + (int64_t)slot(sb) * n + a);
+                }
+            if (k >= K) continue;   // the tile lies above the pivot rows: nothing more to do
+            // new rows at this tile's row and column ranges (positions > i1)
+            for (int t = tid; t < 2 * TB; t += CW) {
+                const int c = (t < TB ? r0 : c0) + (t & (TB - 1));
+                double x0 = 0.0, x1 = 0.0;
+                if (c > i1 && c < n) {
+                    const double b0 = snew(i0, c), b1 = snew(i1, c);
+                    x0 = (g.pivot ? b1 : sg * b1) * q;
+                    x1 = (-b0) * q;
+                }
+                if (t < TB) {
+                    s.n0r[t] = x0;
+                    s.n1r[t] = x1;
+                } else {
+                    s.n0c[t - TB] = x0;
+                    s.n1c[t - TB] = x1;
+                }
+            }
+            __syncthreads();
+            const int a = r0 + lane;
+            const bool amov = moved && a != sig(a);
+            for (int cc = wave; cc < TB; cc += CW / 64) {
+                const int b = c0 + cc;
+                if (b >= n || a >= n || a > b) continue;
+                double* __restrict__ e = U + a + (int64_t)b * n;
+                if (a >= t0) {
+                    const bool mv = amov || (moved && b != sig(b));
+                    const double old = !mv ? *e : a == b ? lineval(sig(a), sig(a)) : snew(a, b);
+                    const double x = s.n0r[lane] * s.n1c[cc], y = s.n1r[lane] * s.n0c[cc];
+                    const double w = old - (x - y);
+                    *e = w;
+                    if (g.pivot && (a < b || isnan(w))) offer(bv, br, bc, w, a, b);
+                } else if (a == i0 || a == i1) {
+                    if (b > i1) *e = a == i0 ? s.n0c[cc] : s.n1c[cc];
+                    else if (b == i1 && a == i0) *e = 0.0;
+                    else if (b == a) *e = (a == i0 || g.pivot) ? r : sg * r;
+                }
+            }
+            __syncthreads();
+        }
+        steps++;
+        if (g.pivot && j + 1 < n / 2) {
+            coop_reduce(s, bv, br, bc);
+            if (tid == 0) {
+                st_co(g.part + 3 * gi, s.best);
+                st_co(g.part + 3 * gi + 1, pack_pos(s.ii, s.jj));
+            }
+        }
+        if (!grid_sync_co(g.bar, ++epoch, g.err, &s.ok)) return;
+    }
+    if (gi == 0 && tid == 0) *g.steps = steps;
+}
+
 }  // namespace chol
 }  // namespace skh
 
 using namespace skh;
+
+extern "C" int skh_internal_skewchol_coop(const double* A, double* U, int64_t n, int pivot, int64_t* p, int64_t* steps,
+                                          double* W, double* part, unsigned* bar, int* err, int grid, hipStream_t st)
+{
+    chol::CoopArgs a;
+    a.A = A;
+    a.U = U;
+    a.p = p;
+    a.steps = steps;
+    a.W = W;
+    a.part = part;
+    a.bar = bar;
+    a.err = err;
+    a.n = (int)n;
+    a.pivot = pivot;
+    void* args[] = {(void*)&a};
+    return (int)hipLaunchCooperativeKernel((const void*)chol::k_skewchol_coop, dim3(grid), dim3(chol::CW), args, 0, st);
+}
+
+extern "C" int skh_internal_skewchol_coop_occupancy(int* per_cu)
+{
+    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, (const void*)chol::k_skewchol_coop, chol::CW, 0);
+}
 
 extern "C" int skh_internal_skewchol(const double* A, double* R, const int64_t* ns, const int64_t* offs,
                                      const int64_t* poffs, const int64_t* order, int64_t batch, int64_t nmax,

@@ -71,6 +71,10 @@ int skh_internal_skewchol(const double* A, double* R, const int64_t* ns, const i
                           int grid, hipStream_t st);
 int skh_internal_skewchol_occupancy(int64_t nmax, int* per_cu);
 int skh_internal_skewchol_nmax();
+// one large matrix over the whole GPU (cooperative): W 4n doubles, part 3 * grid doubles
+int skh_internal_skewchol_coop(const double* A, double* U, int64_t n, int pivot, int64_t* p, int64_t* steps,
+                               double* W, double* part, unsigned* bar, int* err, int grid, hipStream_t st);
+int skh_internal_skewchol_coop_occupancy(int* per_cu);
 int skh_internal_launch_compact(const CompactArgs* a, int nmax, int grid, hipStream_t st);
 int skh_internal_logpf_from(const double* eta, const double* tau, int64_t n, double* logpf, double* sgn, hipStream_t st);
 int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st);

@@ -114,6 +114,19 @@ def test_skewchol_batch_independent():
         assert np.array_equal(C.R, R, equal_nan=True)
 
 
+def test_skewchol_mixed_large_and_small():
+    """Members of 512 and more take the cooperative whole-GPU kernel, the rest the per-matrix
+    kernel, in one call; all bitwise equal to the oracle (with and without pivoting)."""
+    sizes = [600, 40, 513, 7, 512, 130]
+    mats = [gskew(s, 900 + i) for i, s in enumerate(sizes)]
+    for pivot in (skl.RowMaximum(), skl.NoPivot()):
+        say("skewchol mixed %s pivot=%s" % (sizes, type(pivot).__name__))
+        out = skl.skewchol_batched(mats, pivot)
+        for A, C in zip(mats, out):
+            R, p, steps = oracle.skewchol(A, isinstance(pivot, skl.RowMaximum))
+            assert np.array_equal(C.R, R, equal_nan=True) and np.array_equal(C.p, p) and C.steps == steps
+
+
 def test_skewchol_n4096_property():
     """The largest supported size: R^T J R = A[p, p] and R upper triangular with r > 0 blocks."""
     n = 4096
