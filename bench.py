@@ -18,7 +18,8 @@ Also reported:
   complex       ComplexF64 n = 4096 skew-Hermitian tridiagonalization, TFLOP/s, with Q formation and
                 the residual / orthogonality of A = Q T Q^H
   complex_pfaffian  4096 complex skew-symmetric matrices (n in {32,...,256}), log|Pf| and phase by
-                pivoted Parlett-Reid in one batched call, matrices/s, against the oracle on a sample
+                pivoted Parlett-Reid in one batched call, matrices/s, against the oracle on a sample,
+                plus one n = 4096 matrix alone
   skewchol      4096 real skew matrices (n in {32,...,256}), skew-Cholesky with complete pivoting in
                 one batched call, matrices/s, plus one n = 4096 matrix alone, against the oracle
 
@@ -355,10 +356,26 @@ def zpfaff_bench(torch, _lib, steps, count=4096, seed=2718):
     for k, lr in zip(sample, outs):
         worst = max(worst, abs(glp[k] - lr) / abs(lr))
     cpu_rate = len(sample) / cpu_s
+    # one large matrix alone (the whole-GPU kernel)
+    big_n = 4096
+    Rb = torch.complex(torch.randn((big_n, big_n), generator=g, dtype=torch.float64, device=dev),
+                       torch.randn((big_n, big_n), generator=g, dtype=torch.float64, device=dev))
+    Ab = torch.view_as_real((Rb - Rb.T).T.contiguous()).reshape(-1)
+    del Rb
+    one_n = np.array([big_n], dtype=np.int64)
+    one_off = np.zeros(1, dtype=np.int64)
+
+    def big():
+        _lib.check("skh_logabspfaffian_z_batched_device", L.skh_logabspfaffian_z_batched_device(
+            1, one_n.ctypes.data, one_off.ctypes.data, Ab.data_ptr(), lp.data_ptr(), ph.data_ptr(), st, None))
+
+    big()
+    torch.cuda.synchronize()
+    big_ms = timed(torch, big, 1)
     return {"metric": "batched complex skew-symmetric log|Pf| (Parlett-Reid), %d matrices n in {32,...,256}" % count,
             "value": count / (ms * 1e-3), "unit": "matrices/s", "ms_per_batch": ms,
             "gflops": flops / (ms * 1e-3) / 1e9, "flop_count": "4/3 n^3 real flops per matrix (lower triangle)",
-            "max_rel_dev_logpf_vs_oracle": worst,
+            "max_rel_dev_logpf_vs_oracle": worst, "single_n%d_ms" % big_n: big_ms,
             "cpu_baseline": {"value": cpu_rate, "unit": "matrices/s (same size mix)", "cores": 1, "kind": "port",
                              "sample": "oracle on every 65th matrix (%d matrices, %.2f s)" % (len(sample), cpu_s)}}
 

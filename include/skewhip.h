@@ -163,10 +163,11 @@ int skh_eigen_z(int64_t n, const double *A, int64_t lda, double *vals, double *V
  * from pfaffian!/pfaffian, :155-158 and :144) and _logabspfaffian!(A::AbstractMatrix{<:Complex})
  * (:193-233, from logabspfaffian!, :259-263): Parlett-Reid elimination with pivoting.
  * Only the strictly lower triangle is read (the reference's isskewsymmetric check is the caller's);
- * A is not modified.  n[b] <= 2048.  pf: 2 doubles (re, im) per matrix; odd n or a zero pivot
+ * A is not modified.  n[b] <= 8192.  pf: 2 doubles (re, im) per matrix; odd n or a zero pivot
  * give 0.  logpf: log|Pf| per matrix, phase: cis(phase) as 2 doubles per matrix; odd n or a zero
  * pivot give (-Inf, 0).  No reference counterpart for the batching itself (the reference user
- * loops pfaffian).  One workgroup per matrix, largest matrices first.
+ * loops pfaffian).  Even n >= 512: one cooperative launch over the whole GPU per matrix; the
+ * rest one workgroup per matrix, largest first.
  */
 int skh_pfaffian_z_batched(int64_t batch, const int64_t *n, const int64_t *offset, const double *Apacked, double *pf,
                            const skh_opts *opts);
@@ -199,7 +200,8 @@ int skh_skewchol_d_batched_device(int64_t batch, const int64_t *n, const int64_t
                                   const skh_opts *opts);
 
 /* device variants: dA (16-byte aligned) and the outputs in HBM; n and offset are host arrays.
- * Enqueued on `stream`; the call returns once its control arrays are uploaded. */
+ * Enqueued on `stream`; the call returns once the members with even n >= 512 are done and the
+ * control arrays of the rest are uploaded. */
 int skh_pfaffian_z_batched_device(int64_t batch, const int64_t *n, const int64_t *offset, const double *dA,
                                   double *dpf, void *stream, const skh_opts *opts);
 int skh_logabspfaffian_z_batched_device(int64_t batch, const int64_t *n, const int64_t *offset, const double *dA,

@@ -753,13 +753,16 @@ void plan_free(skh_plan p)
 // ---- complex skew-symmetric Pfaffians (Parlett-Reid, one workgroup per matrix) ----
 Buf g_zpslot, g_zpctl, g_zpin, g_zpout;
 constexpr size_t ZPF_SLOT_BUDGET = size_t(4) << 30;   // bytes of working copies in flight
+constexpr int64_t ZPF_COOP_MIN = 512;                  // even members this large use the whole GPU
+constexpr int64_t ZPF_ONE_WG_MAX = 2048;               // largest n of the one-workgroup kernel
+constexpr int64_t ZPF_NMAX = 8192;                     // largest n accepted
 
 int zpf_check(int64_t batch, const int64_t* n, const int64_t* offset, const void* A, const void* o1, const void* o2,
               bool dev)
 {
     if (batch < 0) return set_err(-1, "batch < 0");
     if (batch > 0 && (!n || !offset || !A || !o1 || !o2)) return set_err(-2, "NULL argument");
-    const int64_t nmax = skh_internal_zpfaff_nmax();
+    const int64_t nmax = ZPF_NMAX;
     for (int64_t b = 0; b < batch; b++) {
         if (n[b] < 0) return set_err(-2, "n[%lld] < 0", (long long)b);
         if (n[b] > nmax) return set_err(-2, "n[%lld] = %lld exceeds %lld", (long long)b, (long long)n[b], (long long)nmax);
@@ -775,25 +778,63 @@ int zpf_run(int64_t batch, const int64_t* ns, const int64_t* offs, const double2
             double* do2, hipStream_t st)
 {
     if (batch == 0) return 0;
-    std::vector<int64_t> order(batch);
-    for (int64_t b = 0; b < batch; b++) order[b] = b;
+    std::vector<int64_t> order;
+    std::vector<int64_t> big;
+    for (int64_t b = 0; b < batch; b++) {
+        if (ns[b] % 2 == 0 && ns[b] >= ZPF_COOP_MIN) big.push_back(b);
+        else if (ns[b] % 2 == 1 && ns[b] > ZPF_ONE_WG_MAX) {   // odd: Pf = 0 (log: -Inf, cis 0)
+            static const double zero2[2] = {0.0, 0.0}, minf = -INFINITY;
+            HIPCHK(hipMemcpyAsync(do1 + (mode == 0 ? 2 * b : b), mode == 0 ? zero2 : &minf,
+                                  (mode == 0 ? 2 : 1) * sizeof(double), hipMemcpyHostToDevice, st));
+            if (mode == 1) HIPCHK(hipMemcpyAsync(do2 + 2 * b, zero2, 2 * sizeof(double), hipMemcpyHostToDevice, st));
+        } else {
+            order.push_back(b);
+        }
+    }
+    // large even members: one cooperative launch each over the whole GPU
+    if (!big.empty()) {
+        Ctl ctl;
+        int rc = get_ctl(ctl);
+        if (rc) return rc;
+        int per_cu = 0;
+        HIPCHK((hipError_t)skh_internal_zpfaff_coop_occupancy(&per_cu));
+        per_cu = std::max(1, std::min(per_cu, 2));
+        for (int64_t b : big) {
+            const int64_t n = ns[b], nt = (n + 63) / 64;
+            const int grid = (int)std::min<int64_t>((int64_t)per_cu * g_num_cu, nt * (nt + 1) / 2);
+            double2* work = (double2*)g_zpslot.get((size_t)(n * n + 3 * n + grid) * sizeof(double2));
+            if (!work) return set_err(SKH_ERR_HIP, "hipMalloc of the Pfaffian work copy failed");
+            HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 512, st));
+            HIPCHK((hipError_t)skh_internal_zpfaff_coop(dA + offs[b], work, n, mode, do1 + (mode == 0 ? 2 * b : b),
+                                                        mode == 1 ? do2 + 2 * b : nullptr, work + n * n,
+                                                        (double*)(work + n * n + 3 * n), ctl.bar, ctl.err, grid, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        rc = check_barrier_flag();
+        if (rc) return rc;
+    }
+    if (order.empty()) {
+        HIPCHK(hipStreamSynchronize(st));
+        return 0;
+    }
     std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return ns[a] > ns[b]; });
     const int64_t nmax = std::max<int64_t>(ns[order[0]], 2);
+    const int64_t count = (int64_t)order.size();
     int per_cu = 0;
     HIPCHK((hipError_t)skh_internal_zpfaff_occupancy(nmax, &per_cu));
     per_cu = std::max(1, per_cu);
     const int64_t slot = (nmax * nmax + 15) & ~int64_t(15);   // complex elements
     const int64_t by_mem = std::max<int64_t>(1, (int64_t)(ZPF_SLOT_BUDGET / ((size_t)slot * sizeof(double2))));
-    const int grid = (int)std::min<int64_t>({(int64_t)per_cu * g_num_cu, batch, by_mem});
+    const int grid = (int)std::min<int64_t>({(int64_t)per_cu * g_num_cu, count, by_mem});
     double2* slots = (double2*)g_zpslot.get((size_t)slot * grid * sizeof(double2));
-    int64_t* dctl = (int64_t*)g_zpctl.get(3 * batch * sizeof(int64_t));
+    int64_t* dctl = (int64_t*)g_zpctl.get((2 * batch + count) * sizeof(int64_t));
     if (!slots || !dctl) return set_err(SKH_ERR_HIP, "hipMalloc of Pfaffian work slots failed");
-    std::vector<int64_t> h(3 * batch);
+    std::vector<int64_t> h(2 * batch + count);
     std::copy(ns, ns + batch, h.begin());
     std::copy(offs, offs + batch, h.begin() + batch);
     std::copy(order.begin(), order.end(), h.begin() + 2 * batch);
     HIPCHK(hipMemcpyAsync(dctl, h.data(), h.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
-    HIPCHK((hipError_t)skh_internal_zpfaff(dA, dctl, dctl + batch, dctl + 2 * batch, batch, nmax, slots, slot, grid,
+    HIPCHK((hipError_t)skh_internal_zpfaff(dA, dctl, dctl + batch, dctl + 2 * batch, count, nmax, slots, slot, grid,
                                            mode, do1, do2, st));
     // the host copy of the control arrays must outlive the async copy
     HIPCHK(hipStreamSynchronize(st));

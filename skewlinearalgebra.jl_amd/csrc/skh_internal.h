@@ -65,6 +65,9 @@ int skh_internal_zpfaff(const double2* A, const int64_t* ns, const int64_t* offs
                         hipStream_t st);
 int skh_internal_zpfaff_occupancy(int64_t nmax, int* per_cu);
 int skh_internal_zpfaff_nmax();
+int skh_internal_zpfaff_coop(const double2* A, double2* L, int64_t n, int mode, double* o1, double* o2, double2* W,
+                             double* part, unsigned* bar, int* err, int grid, hipStream_t st);
+int skh_internal_zpfaff_coop_occupancy(int* per_cu);
 // skew-Cholesky (skh_chol.hip): matrix b at A + offs[b] and R + offs[b] (ld n[b]), p + poffs[b]
 int skh_internal_skewchol(const double* A, double* R, const int64_t* ns, const int64_t* offs, const int64_t* poffs,
                           const int64_t* order, int64_t batch, int64_t nmax, int pivot, int64_t* p, int64_t* steps,

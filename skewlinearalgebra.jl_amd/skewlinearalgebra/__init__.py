@@ -539,7 +539,7 @@ def _pack_z(mats):
 
 
 def pfaffian_z_batched(mats):
-    """Pfaffians of complex skew-symmetric matrices (uneven sizes, n <= 2048) in one GPU launch:
+    """Pfaffians of complex skew-symmetric matrices (uneven sizes, n <= 8192) in one GPU call:
     Parlett-Reid with pivoting as _pfaffian!(A::AbstractMatrix{<:Complex}) (src/pfaffian.jl:97-137).
     Only the strictly lower triangle is read: the skew-symmetry check is the caller's (pfaffian
     makes it).  Odd n or a zero pivot give 0.  Returns a complex128 array."""

@@ -103,12 +103,12 @@ def test_argument_errors_before_device():
     assert L.skh_logpf_d(3, A3.ctypes.data, 3, ctypes.byref(lp), ctypes.byref(sg), None) == 0
     assert lp.value == -np.inf and sg.value == 0.0
     assert L.skh_skewtrieigen_d(4, A.ctypes.data, tau.ctypes.data, A.ctypes.data, A.ctypes.data, 7) == -6
-    # complex Pfaffians: sizes beyond 2048 and negative offsets are rejected before any device work
-    ns = np.array([4, 2050], dtype=np.int64)
+    # complex Pfaffians: sizes beyond 8192 and negative offsets are rejected before any device work
+    ns = np.array([4, 8194], dtype=np.int64)
     offs = np.array([0, 16], dtype=np.int64)
     out = np.zeros(8)
     assert L.skh_pfaffian_z_batched(2, ns.ctypes.data, offs.ctypes.data, out.ctypes.data, out.ctypes.data, None) == -2
-    assert b"2048" in L.skh_last_error()
+    assert b"8192" in L.skh_last_error()
     offs[1] = -1
     ns[1] = 2
     assert L.skh_logabspfaffian_z_batched(2, ns.ctypes.data, offs.ctypes.data, out.ctypes.data, out.ctypes.data,

@@ -84,15 +84,53 @@ def test_zpfaffian_log_vs_oracle(n):
     assert abs(ph - hr) <= 1e-10 * n / 64, (ph, hr)
 
 
-def test_zpfaffian_n2048_det():
-    """The largest supported size: Pf^2 = det, i.e. 2 log|Pf| = log|det| and cis(phase)^2 = det/|det|."""
-    n = 2048
+@pytest.mark.parametrize("n", [2048, 4096])
+def test_zpfaffian_large_det(n):
+    """Large sizes (the whole-GPU kernel): Pf^2 = det, i.e. 2 log|Pf| = log|det| and
+    cis(phase)^2 = det/|det|."""
     A = zsym(n, 77)
-    say("zpfaff n=2048 vs slogdet")
+    say("zpfaff n=%d vs slogdet" % n)
     lp, ph = skl.logabspfaffian(A)
     sgn, ld = np.linalg.slogdet(A)
     assert abs(2 * lp - ld) <= 1e-11 * n * abs(ld), (2 * lp, ld)
     assert abs(ph * ph - sgn) <= 1e-8, (ph * ph, sgn)
+
+
+def test_zpfaffian_mixed_large_and_small():
+    """Even members of 512 and more take the whole-GPU kernel, the rest one workgroup each, in
+    one call; odd sizes beyond 2048 are answered on the host (0, and (-Inf, 0))."""
+    sizes = [600, 40, 513, 7, 512, 130, 2049]
+    mats = [zsym(s, 700 + i) for i, s in enumerate(sizes)]
+    say("zpfaff mixed %s" % sizes)
+    lp, ph = skl.logabspfaffian_z_batched(mats)
+    pf = skl.pfaffian_z_batched(mats[1:4] + [mats[5]])
+    for i, A in enumerate(mats):
+        if sizes[i] % 2:
+            assert lp[i] == -np.inf and ph[i] == 0
+            continue
+        lr, hr = oracle.logabspfaffian(A)
+        assert abs(lp[i] - lr) <= 1e-12 * sizes[i] * abs(lr), (sizes[i], lp[i], lr)
+        assert abs(ph[i] - hr) <= 1e-10 * max(1.0, sizes[i] / 64), (sizes[i], ph[i], hr)
+    for A, p in zip(mats[1:4] + [mats[5]], pf):
+        ref = oracle.pfaffian(A)
+        assert (p == 0 and ref == 0) or abs(p - ref) <= 1e-11 * abs(ref)
+
+
+def test_zpfaffian_coop_zero_pivot_and_swaps():
+    """Whole-GPU kernel: a matrix whose later column vanishes gives 0 / (-Inf, 0); a Pfaffian of
+    moderate size that fits Float64 matches the oracle."""
+    n = 512
+    A = zsym(n, 5) * 0.05
+    A[:, 300] = 0
+    A[300, :] = 0
+    say("zpfaff coop zero column")
+    assert skl.pfaffian(A) == 0
+    lp, ph = skl.logabspfaffian(A)
+    assert lp == -np.inf and ph == 0
+    B = zsym(n, 6) * 0.05
+    ref = oracle.pfaffian(B)
+    got = skl.pfaffian(B)
+    assert abs(got - ref) <= 1e-10 * abs(ref), (got, ref)
 
 
 def test_zpfaffian_batch_independent_and_variants():
@@ -180,6 +218,6 @@ def test_zpfaffian_device_entry():
 
 def test_zpfaffian_argument_errors():
     with pytest.raises(ValueError):
-        skl.pfaffian_z_batched([np.zeros((2050, 2050), complex)])
+        skl.pfaffian_z_batched([np.zeros((8194, 8194), complex)])
     with pytest.raises(skl.ArgumentError):
         skl.pfaffian(np.array([[0, 1j], [1j, 0]]))     # skew-Hermitian, not skew-symmetric
