@@ -241,6 +241,12 @@ int skh_eigvals_range_d(int64_t n, const double *A, int64_t lda, int64_t il, int
  * vr, vi: n x n real and imaginary parts of the eigenvector matrix (column-major).
  */
 int skh_skewtrieigen_d(int64_t n, double *ev, double *vals, double *vr, double *vi, int variant);
+/* The same with the sweeps' Givens rotations applied to the eigenvector halves Qodd / Qeven on the
+ * GPU (one thread per row, the chain's carry in a register) instead of on the host: the O(n^3)
+ * part of skewtrieigen_divided!/merged! (src/skeweigen.jl:158-193, :246-262, :376-392).  Host
+ * arrays in and out; results bit-identical to skh_skewtrieigen_d.  skh_eigen_d takes this path
+ * from n = 256. */
+int skh_skewtrieigen_d_gpu(int64_t n, double *ev, double *vals, double *vr, double *vi, int variant);
 
 /* Diagnostics: per-kernel times of the last skh_sytrd_d_device call made with SKH_FLAG_TIMING:
  * out10 = {panel ms, panel launches, rank-2k ms, rank-2k launches, panel algorithmic bytes,

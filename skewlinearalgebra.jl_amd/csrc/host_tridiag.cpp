@@ -32,6 +32,13 @@ inline Rot givens(double a, double b)
     return {a / nm, b / nm};
 }
 
+// Rotations of the QR sweeps logged for a device to apply (instead of applying them on the host):
+// [0] the chain on Qodd, [1] the chain on Qeven; rotation t acts on columns (k - 1, k), 0-based.
+struct RotLog {
+    std::vector<int32_t> k[2];
+    std::vector<double> cs[2];   // c, s interleaved
+};
+
 // 1-based view over a contiguous array
 struct V1 {
     double* p;
@@ -73,7 +80,7 @@ void zero_eigenvalue_sweep(V1 ev, V1 G, int64_t n)
 // One shifted sweep over ev[start..n]; Q-rotations applied when qodd/qeven are given.
 // novec semantics (early return on an interior split) when early_return is set.
 int64_t qr_sweep(V1 ev, int64_t n, int64_t start, double tol, bool early_return, double* qeven, int64_t lde,
-                 double* qodd, int64_t ldo, int64_t nrows)
+                 double* qodd, int64_t ldo, int64_t nrows, RotLog* log = nullptr)
 {
     double bulge = 0.0;
     const double shift = ev(n) * ev(n);
@@ -95,7 +102,12 @@ int64_t qr_sweep(V1 ev, int64_t n, int64_t start, double tol, bool early_return,
                 start = i + 1;
             }
         }
-        if (qodd) {
+        if (log) {
+            const int w = (i % 2 == 1) ? 0 : 1;
+            log->k[w].push_back((int32_t)((i + 1) / 2));
+            log->cs[w].push_back(r.c);
+            log->cs[w].push_back(r.s);
+        } else if (qodd) {
             double* Q = (i % 2 == 1) ? qodd : qeven;
             const int64_t ld = (i % 2 == 1) ? ldo : lde;
             const int64_t k = (i + 1) / 2;
@@ -238,7 +250,17 @@ extern "C" int skh_skewtrieigvals_d(int64_t N, double* evp, double* vals)
     return 2;
 }
 
-extern "C" int skh_skewtrieigen_d(int64_t N, double* evp, double* vals, double* vr, double* vi, int variant)
+// Qo (no x no) and Qe (ne x ne), column-major; the rotations act on rows 0 .. nrows-1 only (the
+// last row of Qo for odd N belongs to the zero eigenvalue and is left to getoddvectors).
+typedef int (*skh_rot_apply_fn)(void* ctx, int64_t nrows, const int32_t* ko, const double* cso, int64_t nro,
+                                double* Qo, int64_t no, const int32_t* ke, const double* cse, int64_t nre, double* Qe,
+                                int64_t ne);
+
+// skewtrieigen_divided! / merged!.  With `apply` set, the sweeps' rotations are logged and handed
+// to it once the iteration has converged (it must leave Qo and Qe as the host would have them);
+// getoddvectors and the assembly of vr, vi then run here as usual.
+extern "C" int skh_internal_skewtrieigen(int64_t N, double* evp, double* vals, double* vr, double* vi, int variant,
+                                         skh_rot_apply_fn apply, void* ctx)
 {
     if (N < 0) return -1;
     if (N > 1 && !evp) return -2;
@@ -264,8 +286,10 @@ extern "C" int skh_skewtrieigen_d(int64_t N, double* evp, double* vals, double* 
     const int64_t max_iter = 30 * n, halfN = n / 2;
     int64_t iter = 0, count_static = 0, start = 1;
     double mem = 1.0;
+    RotLog log;
+    RotLog* lp = apply ? &log : nullptr;
     while (n > 2 && iter < max_iter) {
-        start = qr_sweep(ev, n - 1, start, tol, false, Qe.data(), ne, Qo.data(), no, halfN);
+        start = qr_sweep(ev, n - 1, start, tol, false, Qe.data(), ne, Qo.data(), no, halfN, lp);
         while (n > 2 && std::fabs(ev(n - 1)) <= tol) {
             vals[n - 1] = 0.0;
             n -= 1;
@@ -320,6 +344,11 @@ extern "C" int skh_skewtrieigen_d(int64_t N, double* evp, double* vals, double* 
         vals[1] = -ev(1);
     } else {
         vals[0] = 0.0;
+    }
+    if (apply) {
+        const int rc = apply(ctx, halfN, log.k[0].data(), log.cs[0].data(), (int64_t)log.k[0].size(), Qo.data(), no,
+                             log.k[1].data(), log.cs[1].data(), (int64_t)log.k[1].size(), Qe.data(), ne);
+        if (rc) return rc;
     }
     if (N % 2 == 1) {
         // getoddvectors (src/skeweigen.jl:49-62)
@@ -427,4 +456,9 @@ extern "C" int skh_skewtrieigvals_range_d(int64_t N, const double* ev, int64_t i
     }
     *m = k;
     return 0;
+}
+
+extern "C" int skh_skewtrieigen_d(int64_t N, double* evp, double* vals, double* vr, double* vi, int variant)
+{
+    return skh_internal_skewtrieigen(N, evp, vals, vr, vi, variant, nullptr, nullptr);
 }

@@ -866,6 +866,51 @@ int zpf_host(int64_t batch, const int64_t* n, const int64_t* offset, const doubl
     return 0;
 }
 
+// ---- eigenvector rotations of the skew-tridiagonal QR on the GPU ----
+Buf g_rotq, g_rotk, g_rotcs, g_rotseg;
+constexpr int64_t ROT_GPU_MIN = 256;   // eigen(A) applies the QR rotations on the GPU from this n
+
+std::vector<int64_t> chain_starts(const int32_t* k, int64_t cnt)
+{
+    std::vector<int64_t> s;
+    for (int64_t t = 0; t < cnt; t++)
+        if (t == 0 || k[t] != k[t - 1] + 1) s.push_back(t);
+    s.push_back(cnt);
+    return s;
+}
+
+// skh_rot_apply_fn: Qo, Qe (host, still the identity) receive the logged rotations, applied on the GPU
+int rot_apply_gpu(void*, int64_t nrows, const int32_t* ko, const double* cso, int64_t nro, double* Qo, int64_t no,
+                  const int32_t* ke, const double* cse, int64_t nre, double* Qe, int64_t ne)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    hipStream_t st = g_stream;
+    const std::vector<int64_t> so = chain_starts(ko, nro), se = chain_starts(ke, nre);
+    double* dq = (double*)g_rotq.get((size_t)(no * no + ne * ne + 1) * sizeof(double));
+    int32_t* dk = (int32_t*)g_rotk.get((size_t)(nro + nre + 1) * sizeof(int32_t));
+    double* dcs = (double*)g_rotcs.get((size_t)(2 * (nro + nre) + 1) * sizeof(double));
+    int64_t* dseg = (int64_t*)g_rotseg.get((so.size() + se.size()) * sizeof(int64_t));
+    if (!dq || !dk || !dcs || !dseg) return set_err(SKH_ERR_HIP, "hipMalloc of rotation buffers failed");
+    double* dQo = dq;
+    double* dQe = dq + no * no;
+    HIPCHK(hipMemcpyAsync(dQo, Qo, (size_t)no * no * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dQe, Qe, (size_t)ne * ne * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dk, ko, (size_t)nro * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dk + nro, ke, (size_t)nre * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dcs, cso, (size_t)2 * nro * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dcs + 2 * nro, cse, (size_t)2 * nre * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dseg, so.data(), so.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dseg + so.size(), se.data(), se.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    HIPCHK((hipError_t)skh_internal_rot_chains(dQo, no, dk, dcs, dseg, (int64_t)so.size() - 1, dQe, ne, dk + nro,
+                                               dcs + 2 * nro, dseg + so.size(), (int64_t)se.size() - 1, nrows, st));
+    HIPCHK(hipMemcpyAsync(Qo, dQo, (size_t)no * no * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(Qe, dQe, (size_t)ne * ne * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
 // ---- skew-Cholesky: one workgroup per matrix, or the whole GPU for one large matrix ----
 Buf g_chctl, g_chin, g_chout, g_chp, g_chws;
 constexpr int64_t CHOL_COOP_MIN = 512;   // matrices this large get the cooperative kernel
@@ -997,7 +1042,7 @@ int skh_finalize(void)
     for (Buf* b : {&g_mat, &g_ws, &g_vec, &g_ctl, &g_bin, &g_bout, &g_q, &g_ev, &g_gq, &g_gk, &g_gz, &g_gv, &g_gp,
                    &g_gte, &g_cslot, &g_big, &g_bctl, &g_zws, &g_zq, &g_zc, &g_zx, &g_zpslot, &g_zpctl, &g_zpin,
                    &g_zpout, &g_chctl, &g_chin, &g_chout, &g_chp,
-                   &g_chws})
+                   &g_chws, &g_rotq, &g_rotk, &g_rotcs, &g_rotseg})
         b->release();
     if (g_blas) (void)rocblas_destroy_handle(g_blas);
     g_blas = nullptr;
@@ -1201,10 +1246,13 @@ int skh_eigen_d(int64_t n, const double* A, int64_t lda, double* vals, double* v
     for (int64_t c = 0; c < n; c++) std::memcpy(&F[c * n], A + c * lda, n * sizeof(double));
     int rc = skh_sytrd_d(n, F.data(), n, tau.data(), eta.data(), nullptr, opts);
     if (rc) return rc;
-    // host implicit QR with Givens accumulation (skewtrieigen_divided!, src/eigen.jl:87)
+    // implicit QR with Givens accumulation (skewtrieigen_divided!, src/eigen.jl:87): the sweeps on
+    // the host, their rotations applied to Qodd / Qeven on the GPU from n = 256
     std::vector<double> hr((size_t)n * n), hi((size_t)n * n);
-    rc = skh_skewtrieigen_d(n, eta.data(), vals, hr.data(), hi.data(), 0);
-    if (rc) return set_err(rc > 0 ? SKH_ERR_NOCONV : rc, "skewtrieigen failed (%d)", rc);
+    rc = skh_internal_skewtrieigen(n, eta.data(), vals, hr.data(), hi.data(), 0,
+                                   n >= ROT_GPU_MIN ? rot_apply_gpu : nullptr, nullptr);
+    if (rc == 2) return set_err(SKH_ERR_NOCONV, "skewtrieigen did not converge");
+    if (rc) return rc < 0 ? set_err(rc, "skewtrieigen failed (%d)", rc) : rc;
     std::lock_guard<std::mutex> lk(g_mu);
     rc = ensure_init_locked(g_device);
     if (rc) return rc;
@@ -1735,6 +1783,17 @@ int skh_skewchol_d(int64_t n, const double* A, int64_t lda, double* R, int64_t l
     if (rc) return rc;
     for (int64_t c = 0; c < n; c++) std::memcpy(R + c * ldr, &r[c * n], n * sizeof(double));
     return 0;
+}
+
+int skh_skewtrieigen_d_gpu(int64_t N, double* ev, double* vals, double* vr, double* vi, int variant)
+{
+    if (N < 0) return set_err(-1, "n < 0");
+    if (N > 1 && !ev) return set_err(-2, "ev NULL");
+    if (N > 0 && (!vals || !vr || !vi)) return set_err(-3, "output NULL");
+    if (variant != 0 && variant != 1) return set_err(-6, "variant must be 0 or 1");
+    const int rc = skh_internal_skewtrieigen(N, ev, vals, vr, vi, variant, rot_apply_gpu, nullptr);
+    if (rc == 2) return set_err(SKH_ERR_NOCONV, "tridiagonal QR did not converge");
+    return rc;
 }
 
 int skh_eigvals_range_d(int64_t n, const double* A, int64_t lda, int64_t il, int64_t iu, double vl, double vh,

@@ -68,6 +68,16 @@ int skh_internal_zpfaff_nmax();
 int skh_internal_zpfaff_coop(const double2* A, double2* L, int64_t n, int mode, double* o1, double* o2, double2* W,
                              double* part, unsigned* bar, int* err, int grid, hipStream_t st);
 int skh_internal_zpfaff_coop_occupancy(int* per_cu);
+// host skew-tridiagonal QR (host_tridiag.cpp) with its Qodd / Qeven rotations handed to `apply`
+typedef int (*skh_rot_apply_fn)(void* ctx, int64_t nrows, const int32_t* ko, const double* cso, int64_t nro,
+                                double* Qo, int64_t no, const int32_t* ke, const double* cse, int64_t nre, double* Qe,
+                                int64_t ne);
+int skh_internal_skewtrieigen(int64_t N, double* evp, double* vals, double* vr, double* vi, int variant,
+                              skh_rot_apply_fn apply, void* ctx);
+// rotation chains on the GPU (skh_rot.hip): seg* hold nseg* + 1 chain starts into the lists
+int skh_internal_rot_chains(double* Qo, int64_t ldo, const int32_t* ko, const double* cso, const int64_t* sego,
+                            int64_t nsego, double* Qe, int64_t lde, const int32_t* ke, const double* cse,
+                            const int64_t* sege, int64_t nsege, int64_t nrows, hipStream_t st);
 // skew-Cholesky (skh_chol.hip): matrix b at A + offs[b] and R + offs[b] (ld n[b]), p + poffs[b]
 int skh_internal_skewchol(const double* A, double* R, const int64_t* ns, const int64_t* offs, const int64_t* poffs,
                           const int64_t* order, int64_t batch, int64_t nmax, int pivot, int64_t* p, int64_t* steps,

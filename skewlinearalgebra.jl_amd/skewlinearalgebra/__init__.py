@@ -355,16 +355,19 @@ class Eigen:
         return iter((self.values, self.vectors))
 
 
-def _tri_eigen_host(ev, variant):
+def _tri_eigen_host(ev, variant, gpu=None):
+    """skewtrieigen (host QR); from n = 256 (or with gpu=True) its rotations are applied to the
+    eigenvector halves on the GPU (same bits)."""
     ev = np.array(ev, dtype=np.float64, copy=True)
     n = ev.size + 1
     vals = np.zeros(n)
     vr = np.zeros((n, n), order="F")
     vi = np.zeros((n, n), order="F")
-    rc = _lib.lib().skh_skewtrieigen_d(n, _ptr(ev), _ptr(vals), _ptr(vr), _ptr(vi), variant)
+    name = "skh_skewtrieigen_d_gpu" if (gpu if gpu is not None else n >= 256) else "skh_skewtrieigen_d"
+    rc = getattr(_lib.lib(), name)(n, _ptr(ev), _ptr(vals), _ptr(vr), _ptr(vi), variant)
     if rc == 2:
         raise RuntimeError("Maximum number of iterations reached, the algorithm didn't converge")
-    _lib.check("skh_skewtrieigen_d", rc)
+    _lib.check(name, rc)
     return vals, vr, vi
 
 

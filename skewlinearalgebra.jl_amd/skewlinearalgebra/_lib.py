@@ -52,6 +52,7 @@ def lib():
         "skh_logpf_batched_d_device": ([i64, vp, vp, vp, vp, vp, vp, op], ctypes.c_int),
         "skh_skewtrieigvals_d": ([i64, vp, vp], ctypes.c_int),
         "skh_skewtrieigen_d": ([i64, vp, vp, vp, vp, ctypes.c_int], ctypes.c_int),
+        "skh_skewtrieigen_d_gpu": ([i64, vp, vp, vp, vp, ctypes.c_int], ctypes.c_int),
         "skh_diag_mfma_probe": ([vp], ctypes.c_int),
         "skh_diag_timings": ([vp], ctypes.c_int),
         "skh_diag_phase_clock": ([vp, ctypes.c_int], ctypes.c_int),
