@@ -323,11 +323,14 @@ def zpfaff_bench(torch, _lib, steps, count=4096, seed=2718):
     g.manual_seed(seed)
     tot = int(offs[-1] + ns[-1] ** 2)
     buf = torch.empty(2 * tot, dtype=torch.float64, device=dev)
-    for k in range(count):
-        n = int(ns[k])
-        R = torch.complex(torch.randn((n, n), generator=g, dtype=torch.float64, device=dev),
-                          torch.randn((n, n), generator=g, dtype=torch.float64, device=dev))
-        buf[2 * int(offs[k]):2 * int(offs[k] + n * n)] = torch.view_as_real((R - R.T).T.contiguous()).reshape(-1)
+    for n in np.unique(ns):
+        idx = np.nonzero(ns == n)[0]
+        n = int(n)
+        Rm = torch.complex(torch.randn((len(idx), n, n), generator=g, dtype=torch.float64, device=dev),
+                           torch.randn((len(idx), n, n), generator=g, dtype=torch.float64, device=dev))
+        Am = torch.view_as_real((Rm - Rm.transpose(1, 2)).transpose(1, 2).contiguous())   # column-major each
+        for q, k in enumerate(idx):
+            buf[2 * int(offs[k]):2 * int(offs[k] + n * n)] = Am[q].reshape(-1)
     lp = torch.empty(count, dtype=torch.float64, device=dev)
     ph = torch.empty(2 * count, dtype=torch.float64, device=dev)
     st = torch.cuda.current_stream().cuda_stream

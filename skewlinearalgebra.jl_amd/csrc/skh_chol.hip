@@ -328,8 +328,10 @@ __device__ __forceinline__ void unpack_pos(double w, int& r, int& c)
     c = (int)(u >> 32);
 }
 
+constexpr int MAXT = 4;   // tiles whose loads a workgroup keeps in flight together
+
 struct CoopShared {
-    double n0r[TB], n1r[TB], n0c[TB], n1c[TB];   // new rows at the tile's row and column ranges
+    double nr[MAXT][2 * TB][2];   // new rows (n0, n1) at each tile's row range, then column range
     double bv[4];
     int br[4], bc[4];
     double red[4];
@@ -459,18 +461,6 @@ __global__ __launch_bounds__(CW) void k_skewchol_coop(CoopArgs g)
             ii = s.ii;
             const int jj = s.jj;
             kk = jj == i0 ? ii : jj;
-            if (gi == 0 && tid == 0) {
-                if (ii != i0) {
-                    const int64_t t = g.p[ii];
-                    g.p[ii] = g.p[i0];
-                    g.p[i0] = t;
-                }
-                if (kk != i1) {
-                    const int64_t t = g.p[kk];
-                    g.p[kk] = g.p[i1];
-                    g.p[i1] = t;
-                }
-            }
         }
         // the step's positions: slots 0..3 hold i0, i1, ii, kk (duplicates resolve to the first)
         const int P[4] = {i0, i1, ii, kk};
@@ -498,6 +488,19 @@ __global__ __launch_bounds__(CW) void k_skewchol_coop(CoopArgs g)
             }
         }
         if (!grid_sync_co(g.bar, ++epoch, g.err, &s.ok)) return;
+        // the permutation (workgroup 0 alone; off the path to the barrier above)
+        if (gi == 0 && tid == 0) {
+            if (ii != i0) {
+                const int64_t t = g.p[ii];
+                g.p[ii] = g.p[i0];
+                g.p[i0] = t;
+            }
+            if (kk != i1) {
+                const int64_t t = g.p[kk];
+                g.p[kk] = g.p[i1];
+                g.p[i1] = t;
+            }
+        }
         // new trailing entry (x, y), x != y, both >= i0, with x or y moved: the old one at
         // (s(x), s(y)), whose line is published
         auto lineval = [&](int u, int v) {
@@ -525,12 +528,12 @@ __global__ __launch_bounds__(CW) void k_skewchol_coop(CoopArgs g)
         bc = t0;
         const int I0 = i0 / TB;   // block row of the pivot rows (i0 and i1 share it: TB is even)
         const int64_t K = (int64_t)(nt - I0) * (nt - I0 + 1) / 2;   // tiles at or below block row I0
-        for (int64_t k = gi; k < T; k += G) {
-            int I, J;
-            tile_of(k, nt, I, J);
-            const int r0 = I * TB, c0 = J * TB;
-            // finished rows of R in moved columns: U[a, b] = old U[a, s(b)], a < i0
-            if (moved)
+        // finished rows of R in moved columns: U[a, b] = old U[a, s(b)], a < i0 (any own tile)
+        if (moved)
+            for (int64_t k = gi; k < T; k += G) {
+                int I, J;
+                tile_of(k, nt, I, J);
+                const int r0 = I * TB;
                 for (int e = 0; e < 4; e++) {
                     const int b = P[e];
                     if (slot(b) != e || b / TB != J || sig(b) == b) continue;
@@ -538,42 +541,80 @@ __global__ __launch_bounds__(CW) void k_skewchol_coop(CoopArgs g)
                     for (int a = r0 + tid; a < min(r0 + TB, i0); a += CW)
                         U[a + (int64_t)b * n] = ld_co(g.W + (int64_t)slot(sb) * n + a);
                 }
-            if (k >= K) continue;   // the tile lies above the pivot rows: nothing more to do
-            // new rows at this tile's row and column ranges (positions > i1)
-            for (int t = tid; t < 2 * TB; t += CW) {
-                const int c = (t < TB ? r0 : c0) + (t & (TB - 1));
+            }
+        // the own tiles at or below the pivot rows, MAXT at a time: first the new rows at every
+        // tile's row and column ranges (all line reads in flight together), then every entry's old
+        // value (all tile loads in flight together), then the update
+        for (int64_t kc = gi; kc < K; kc += (int64_t)MAXT * G) {
+            int nc = 0;
+            int tr0[MAXT], tc0[MAXT];
+            for (int u = 0; u < MAXT; u++) {
+                const int64_t k = kc + (int64_t)u * G;
+                int I = 0, J = 0;
+                if (k < K) {
+                    tile_of(k, nt, I, J);
+                    nc = u + 1;
+                }
+                tr0[u] = I * TB;
+                tc0[u] = J * TB;
+            }
+            for (int t = tid; t < nc * 2 * TB; t += CW) {
+                const int u = t / (2 * TB), v = t % (2 * TB);
+                int r0u = tr0[0], c0u = tc0[0];   // constant indices only (no scratch)
+#pragma unroll
+                for (int x = 1; x < MAXT; x++)
+                    if (u == x) {
+                        r0u = tr0[x];
+                        c0u = tc0[x];
+                    }
+                const int c = (v < TB ? r0u : c0u) + (v & (TB - 1));
                 double x0 = 0.0, x1 = 0.0;
                 if (c > i1 && c < n) {
                     const double b0 = snew(i0, c), b1 = snew(i1, c);
                     x0 = (g.pivot ? b1 : sg * b1) * q;
                     x1 = (-b0) * q;
                 }
-                if (t < TB) {
-                    s.n0r[t] = x0;
-                    s.n1r[t] = x1;
-                } else {
-                    s.n0c[t - TB] = x0;
-                    s.n1c[t - TB] = x1;
-                }
+                s.nr[u][v][0] = x0;
+                s.nr[u][v][1] = x1;
             }
             __syncthreads();
-            const int a = r0 + lane;
-            const bool amov = moved && a != sig(a);
-            for (int cc = wave; cc < TB; cc += CW / 64) {
-                const int b = c0 + cc;
-                if (b >= n || a >= n || a > b) continue;
-                double* __restrict__ e = U + a + (int64_t)b * n;
-                if (a >= t0) {
-                    const bool mv = amov || (moved && b != sig(b));
-                    const double old = !mv ? *e : a == b ? lineval(sig(a), sig(a)) : snew(a, b);
-                    const double x = s.n0r[lane] * s.n1c[cc], y = s.n1r[lane] * s.n0c[cc];
-                    const double w = old - (x - y);
-                    *e = w;
-                    if (g.pivot && (a < b || isnan(w))) offer(bv, br, bc, w, a, b);
-                } else if (a == i0 || a == i1) {
-                    if (b > i1) *e = a == i0 ? s.n0c[cc] : s.n1c[cc];
-                    else if (b == i1 && a == i0) *e = 0.0;
-                    else if (b == a) *e = (a == i0 || g.pivot) ? r : sg * r;
+            double old[MAXT][TB / (CW / 64)];
+#pragma unroll
+            for (int u = 0; u < MAXT; u++) {
+                const int a = tr0[u] + lane;
+                const bool amov = moved && a != sig(a);
+#pragma unroll
+                for (int w = 0; w < TB / (CW / 64); w++) {
+                    const int b = tc0[u] + wave + w * (CW / 64);
+                    double o = 0.0;
+                    if (u < nc && a < n && b < n && a >= t0 && a <= b) {
+                        const bool mv = amov || (moved && b != sig(b));
+                        o = !mv ? U[a + (int64_t)b * n] : a == b ? lineval(sig(a), sig(a)) : snew(a, b);
+                    }
+                    old[u][w] = o;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < MAXT; u++) {
+                if (u >= nc) break;
+                const int a = tr0[u] + lane;
+#pragma unroll
+                for (int w = 0; w < TB / (CW / 64); w++) {
+                    const int cc = wave + w * (CW / 64);
+                    const int b = tc0[u] + cc;
+                    if (b >= n || a >= n || a > b) continue;
+                    double* __restrict__ e = U + a + (int64_t)b * n;
+                    if (a >= t0) {
+                        const double x = s.nr[u][lane][0] * s.nr[u][TB + cc][1];
+                        const double y = s.nr[u][lane][1] * s.nr[u][TB + cc][0];
+                        const double wv = old[u][w] - (x - y);
+                        *e = wv;
+                        if (g.pivot && (a < b || isnan(wv))) offer(bv, br, bc, wv, a, b);
+                    } else if (a == i0 || a == i1) {
+                        if (b > i1) *e = a == i0 ? s.nr[u][TB + cc][0] : s.nr[u][TB + cc][1];
+                        else if (b == i1 && a == i0) *e = 0.0;
+                        else if (b == a) *e = (a == i0 || g.pivot) ? r : sg * r;
+                    }
                 }
             }
             __syncthreads();
