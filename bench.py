@@ -11,6 +11,10 @@ Also reported:
   mfma_update   the rank-2k trailing update kernel (FP64 MFMA) against the FP64 matrix peak
   cpu_baseline  the oracle (C restatement of the reference algorithm, OpenBLAS, all host
                 threads) timed on a bounded sample: one full factorization at n=4096
+  nb_sweep      one factorization at each panel width nb in {32, 64, 128}: ms, TFLOP/s, panel
+                HBM GB/s, rank-2k TFLOP/s
+  host_eigvals  the host implicit-QR eigenvalues of the GPU tridiagonal (sum of lambda^2 against
+                ||A||_F^2 as the check)
   batched       4096 uneven-size matrices (n in {32,...,512}), log|Pf| + sign through one tuned
                 plan replayed from a HIP graph, matrices/s, against the per-matrix GPU loop
   grad          1024 matrices n = 256, log|Pf| + hand-written gradient, matrices/s, against the
@@ -585,6 +589,40 @@ def main():
         "check": {"frobenius_rel_err": rel},
         "wall_s": wall,
     }
+    # panel width sweep over nb in {32, 64, 128} (one factorization each, outside the timed steps),
+    # then the host implicit-QR eigenvalues of the tridiagonal (the reference's skewtrieigvals!)
+    sweep = {}
+    for nbv in (32, 64, 128):
+        o_nb = _lib.opts(nbv, _lib.ENGINE_LARGE | _lib.FLAG_TIMING)
+        work.copy_(A0)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        rc = L.skh_sytrd_d_device(n, work.data_ptr(), ld, tau.data_ptr(), eta.data_ptr(), st, ctypes.byref(o_nb))
+        e1.record()
+        torch.cuda.synchronize()
+        _lib.check("skh_sytrd_d_device", rc)
+        _lib.check("skh_diag_timings", L.skh_diag_timings(out6))
+        t_ms = e0.elapsed_time(e1)
+        sweep[str(nbv)] = {"ms": t_ms, "tflops": flops / (t_ms * 1e-3) / 1e12,
+                           "panel_gbs": out6[4] / (out6[0] * 1e-3) / 1e9,
+                           "rank2k_tflops": out6[5] / (out6[2] * 1e-3) / 1e12}
+        log("nb=%d: %.1f ms" % (nbv, t_ms))
+    res["nb_sweep"] = sweep
+    step()
+    torch.cuda.synchronize()
+    ev_h = eta[: n - 1].cpu().numpy().copy()
+    vals = np.zeros(n)
+    t0 = time.perf_counter()
+    rc = L.skh_skewtrieigvals_d(n, ev_h.ctypes.data, vals.ctypes.data)
+    t_eig = time.perf_counter() - t0
+    _lib.check("skh_skewtrieigvals_d", rc)
+    lam2 = float(np.sum(vals * vals))
+    res["host_eigvals"] = {"ms": t_eig * 1e3, "n": n,
+                           "note": "host implicit QR on the GPU-computed tridiagonal (one core)",
+                           "sum_lambda2_vs_frobenius2_rel": abs(lam2 - fro_a * fro_a) / (fro_a * fro_a)}
+    log("host eigvals: %.1f ms" % (t_eig * 1e3))
     if not args.no_batched:
         log("batched log|Pf| (4096 matrices) ...")
         res["batched"] = batched_bench(torch, skl, _lib, args.steps)
