@@ -946,6 +946,27 @@ double orc_pfaffian_d(i64 n, double *A, i64 lda)
 /* _logabspfaffian! for AbstractMatrix{<:Complex}); the full square is updated, */
 /* as the reference does.                                                       */
 /* ------------------------------------------------------------------------- */
+/* Complex arithmetic of the elimination, written out so that its rounding is fixed (products
+ * and sums rounded separately; this file is built with -ffp-contract=off): the product as
+ * (ar br - ai bi, ar bi + ai br), the quotient by Smith's method (scaled by the larger component
+ * of the divisor).  Julia's own complex division is a scaled variant of the same method; the two
+ * can differ in the last bit. */
+static zc zmul(zc a, zc b)
+{
+    const double ar = creal(a), ai = cimag(a), br = creal(b), bi = cimag(b);
+    return CMPLX(ar * br - ai * bi, ar * bi + ai * br);
+}
+static zc zdiv(zc a, zc b)
+{
+    const double ar = creal(a), ai = cimag(a), br = creal(b), bi = cimag(b);
+    if (fabs(br) >= fabs(bi)) {
+        const double r = bi / br, d = br + bi * r;
+        return CMPLX((ar + ai * r) / d, (ai - ar * r) / d);
+    }
+    const double r = br / bi, d = br * r + bi;
+    return CMPLX((ar * r + ai) / d, (ai * r - ar) / d);
+}
+
 /* Step k (0-based, even): returns 0 if the pivot A[k+1,k] is zero after the swap, 1 after a
  * step without a swap, 2 with one; *piv = A[k,k+1]. */
 static int parlett_reid_step_z(i64 n, zc *A, i64 lda, i64 k, zc *piv, zc *tau)
@@ -978,11 +999,12 @@ static int parlett_reid_step_z(i64 n, zc *A, i64 lda, i64 k, zc *piv, zc *tau)
     if (A[(k + 1) + k * lda] == 0) return 0; /* (:119, :133) */
     const zc a = A[k + (k + 1) * lda];
     const i64 m = n - k - 2;
-    for (i64 c = 0; c < m; c++) tau[c] = A[k + (k + 2 + c) * lda] / a; /* Gauss vector (:120) */
+    for (i64 c = 0; c < m; c++) tau[c] = zdiv(A[k + (k + 2 + c) * lda], a); /* Gauss vector (:120) */
     for (i64 l1 = 0; l1 < m; l1++) /* rank-2 update of the trailing square (:123-129) */
-        for (i64 l2 = 0; l2 < m; l2++)
-            A[(k + 2 + l2) + (k + 2 + l1) * lda] +=
-                tau[l2] * A[(k + 2 + l1) + (k + 1) * lda] - tau[l1] * A[(k + 2 + l2) + (k + 1) * lda];
+        for (i64 l2 = 0; l2 < m; l2++) {
+            const zc d = zmul(tau[l2], A[(k + 2 + l1) + (k + 1) * lda]) - zmul(tau[l1], A[(k + 2 + l2) + (k + 1) * lda]);
+            A[(k + 2 + l2) + (k + 2 + l1) * lda] += d;
+        }
     *piv = a;
     return 1 + swapped;
 }
@@ -1001,7 +1023,7 @@ zc orc_pfaffian_z(i64 n, zc *A, i64 lda)
             free(tau);
             return 0;
         }
-        pf *= piv;
+        pf = zmul(pf, piv);
     }
     free(tau);
     return pf;

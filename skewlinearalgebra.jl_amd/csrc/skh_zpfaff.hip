@@ -21,6 +21,10 @@
 
 #include "skh_device.hpp"
 
+// Products and sums are rounded separately everywhere in this file (the complex helpers
+// included), so the Pfaffian is bit-identical to the oracle's, which does the same arithmetic.
+#pragma clang fp contract(off)
+
 namespace skh {
 namespace zpf {
 
