@@ -28,7 +28,7 @@ __all__ = [
     "hessenberg", "hessenberg_", "eigvals", "eigvals_", "eigen", "eigen_", "Eigen", "pfaffian", "pfaffian_",
     "logabspfaffian", "logabspfaffian_", "logabspfaffian_batched", "logabspfaffian_grad_batched",
     "LogAbsPfaffian", "SkhError", "pfaffian_z_batched", "logabspfaffian_z_batched",
-    "JMatrix", "SkewCholesky", "RowMaximum", "NoPivot", "skewchol", "skewchol_", "skewchol_batched",
+    "JMatrix", "SkewCholesky", "RowMaximum", "NoPivot", "skewchol", "skewchol_", "skewchol_batched", "ldiv", "rdiv",
     "ENGINE_AUTO", "ENGINE_LARGE", "ENGINE_TEAM1", "version", "LogPfPlan",
 ]
 
@@ -582,27 +582,126 @@ class NoPivot:
 
 
 class JMatrix:
-    """JMatrix{T, SGN}(n) (src/jmatrix.jl:9-30): J[i, i+1] = SGN, J[i+1, i] = -SGN for even i
-    (0-based); a structured matrix, materialised by numpy.asarray."""
+    """JMatrix{T, SGN}(n) (src/jmatrix.jl:9-121): the block-diagonal skew matrix with blocks
+    [0 SGN; -SGN 0] (J[i, i+1] = SGN, J[i+1, i] = -SGN for even i, 0-based; a trailing zero row and
+    column for odd n).  A structured matrix: products with J permute and negate rows or columns
+    instead of multiplying; numpy.asarray gives the dense form."""
 
-    def __init__(self, n, sgn=+1):
+    def __init__(self, n, sgn=+1, dtype=np.float64):
         if n < 0:
             raise ArgumentError("size %d must be >= 0" % n)
         if sgn not in (+1, -1):
             raise ArgumentError("SGN parameter must be +-1")
         self.n = int(n)
         self.sgn = int(sgn)
+        self.dtype = np.dtype(dtype)
 
     @property
     def shape(self):
         return (self.n, self.n)
 
+    def __getitem__(self, ij):
+        i, j = ij
+        if not (0 <= i < self.n and 0 <= j < self.n):
+            raise IndexError("index %s out of bounds for a %dx%d JMatrix" % (ij, self.n, self.n))
+        if i == j + 1 and j % 2 == 0:
+            return self.dtype.type(-self.sgn)
+        if j == i + 1 and i % 2 == 0:
+            return self.dtype.type(self.sgn)
+        return self.dtype.type(0)
+
     def __array__(self, dtype=None, copy=None):
-        M = np.zeros((self.n, self.n), dtype=dtype or np.float64)
+        M = np.zeros((self.n, self.n), dtype=dtype or self.dtype)
         i = np.arange(0, self.n - 1, 2)
         M[i, i + 1] = self.sgn
         M[i + 1, i] = -self.sgn
         return M
+
+    def __eq__(self, other):
+        if isinstance(other, JMatrix):
+            return self.n == other.n and (self.sgn == other.sgn or self.n < 2)
+        if isinstance(other, SkewHermTridiagonal):
+            other = other.to_dense()
+        other = np.asarray(other)
+        return other.shape == self.shape and bool(np.array_equal(np.asarray(self), other))
+
+    __hash__ = None
+
+    def __neg__(self):
+        return JMatrix(self.n, -self.sgn, self.dtype)
+
+    @property
+    def T(self):
+        """transpose(J) = -J (:88)"""
+        return -self
+
+    def transpose(self):
+        return -self
+
+    def adjoint(self):
+        """J' = -J (:89)"""
+        return -self
+
+    def __matmul__(self, A):
+        """J * A (:51-67): B[i] = SGN A[i+1], B[i+1] = -SGN A[i] for even i; a zero last row for
+        odd n."""
+        A = np.asarray(A)
+        if A.shape[0] != self.n:
+            raise ValueError("DimensionMismatch: J has second dimension %d, A has first dimension %d"
+                             % (self.n, A.shape[0]))
+        B = np.zeros(A.shape, dtype=np.result_type(self.dtype, A.dtype))
+        B[0:self.n - 1:2] = self.sgn * A[1:self.n:2]
+        B[1:self.n:2] = -self.sgn * A[0:self.n - 1:2]
+        return B
+
+    def __rmatmul__(self, A):
+        """A * J (:69-85): B[:, i] = -SGN A[:, i+1], B[:, i+1] = SGN A[:, i] for even i; a zero
+        last column for odd n."""
+        A = np.asarray(A)
+        if A.shape[-1] != self.n:
+            raise ValueError("DimensionMismatch: A has second dimension %d, J has first dimension %d"
+                             % (A.shape[-1], self.n))
+        B = np.zeros(A.shape, dtype=np.result_type(self.dtype, A.dtype))
+        B[..., 0:self.n - 1:2] = -self.sgn * A[..., 1:self.n:2]
+        B[..., 1:self.n:2] = self.sgn * A[..., 0:self.n - 1:2]
+        return B
+
+    def inv(self):
+        """inv(J) = -J for even n; odd n is singular (:90-93)"""
+        if self.n % 2:
+            raise np.linalg.LinAlgError("SingularException(%d)" % self.n)
+        return -self
+
+    def tr(self):
+        return self.dtype.type(0)
+
+    def det(self):
+        return self.dtype.type(self.n % 2 == 0)
+
+    def diag(self, k=0):
+        """diag(J, k) (:97-105)"""
+        v = np.zeros(max(0, self.n - abs(k)), dtype=self.dtype)
+        if k == 1:
+            v[0::2] = self.sgn
+        elif k == -1:
+            v[0::2] = -self.sgn
+        return v
+
+    def to_tridiagonal(self):
+        """SkewHermTridiagonal(J) (:34-38): ev[0::2] = -SGN"""
+        ev = np.zeros(max(self.n - 1, 0), dtype=self.dtype)
+        ev[0::2] = -self.sgn
+        return SkewHermTridiagonal(ev)
+
+
+def ldiv(J, A):
+    """J \\ A = inv(J) * A (src/jmatrix.jl:86)"""
+    return J.inv() @ A
+
+
+def rdiv(A, J):
+    """A / J = A * inv(J) (src/jmatrix.jl:87)"""
+    return np.asarray(A) @ J.inv()
 
 
 class SkewCholesky:
