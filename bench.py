@@ -53,10 +53,12 @@ def log(*a):
 
 def measured_traffic(n, nb):
     """HBM bytes per k_panel launch for this configuration, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel factors), or None."""
-    if n != 8192 or nb not in (0, 64):
+    FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel factors), or None.  nb = 0 is the
+    library's default width, 32 at n = 8192."""
+    if n != 8192 or nb not in (0, 32, 64):
         return None
-    path = os.path.join(HERE, "profiles", "k_panel_n8192_traffic.md")
+    name = "k_panel_n8192_nb64_traffic.md" if nb == 64 else "k_panel_n8192_traffic.md"
+    path = os.path.join(HERE, "profiles", name)
     try:
         with open(path) as f:
             for line in f:

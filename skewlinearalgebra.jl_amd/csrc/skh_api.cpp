@@ -99,7 +99,10 @@ int ensure_init_locked(int device)
 int64_t pick_nb(int64_t n, const skh_opts* o)
 {
     if (o && o->nb > 0) return std::min<int64_t>(o->nb, skh::NBMAX);
-    return n < 1024 ? 32 : 64;
+    // measured on MI355X (whole factorizations): nb = 32 is faster up to n = 8192 (30.4 / 74.3 /
+    // 249 ms at n = 2048 / 4096 / 8192 against 32.9 / 78.1 / 258 ms at 64); at n = 16384 the two
+    // tie (1649 ms) and 64 runs the trailing update closer to the matrix-core peak
+    return n < 12288 ? 32 : 64;
 }
 
 // compact engine limits: even n up to this size (one workgroup per matrix)
