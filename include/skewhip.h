@@ -12,6 +12,9 @@
  *   - return status: 0 success; -k invalid argument number k (1-based); > 0 runtime error
  *     (SKH_ERR_*).  skh_last_error() gives a message for the calling thread.
  * Calls are serialised by an internal mutex; the library owns cached device workspaces.
+ * The device work of one call uses those workspaces and one set of grid-barrier control words,
+ * so the work of two _device calls must not overlap on the GPU: issue them on one stream, or
+ * synchronise between calls on different streams (plans own their buffers and are exempt).
  */
 #ifndef SKEWHIP_H
 #define SKEWHIP_H
