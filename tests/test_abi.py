@@ -187,3 +187,23 @@ def test_skewtrieigvals_range_vs_dense():
                                             ctypes.byref(m)) == 0
         sel = lam[(lam > -vh) & (lam <= -vl)]
         np.testing.assert_allclose(vals[:m.value], -sel, atol=1e-13 * max(1.0, np.abs(lam).max()))
+
+
+def test_skewchol_and_zpfaffian_mirror_arguments():
+    """Argument handling of the skew-Cholesky and complex-Pfaffian mirrors that needs no device:
+    the skew-Hermitian check and its message (src/cholesky.jl:148), complex input to skewchol
+    (defined for real matrices only, :28, :58), the pivot argument, and empty batches."""
+    with pytest.raises(skl.ArgumentError, match="skew-Hermitian"):
+        skl.skewchol(np.ones((3, 3)))
+    with pytest.raises(TypeError):
+        skl.skewchol(np.array([[0, 1j], [1j, 0]]))
+    with pytest.raises(TypeError):
+        skl.skewchol(np.zeros((2, 2)), pivot="rowmaximum")
+    assert skl.skewchol_batched([]) == []
+    lp, ph = skl.logabspfaffian_z_batched([])
+    assert len(lp) == 0 and len(ph) == 0
+    # NaN never passes the exact skew-symmetry checks (NaN != NaN), as in Julia
+    A = np.array([[0, np.nan], [-np.nan, 0]], dtype=complex)
+    with pytest.raises(skl.ArgumentError):
+        skl.pfaffian(A)
+    assert skl.skewchol.__defaults__[0] is skl.RowMaximum   # the reference's default pivoting
