@@ -29,6 +29,7 @@ __all__ = [
     "logabspfaffian", "logabspfaffian_", "logabspfaffian_batched", "logabspfaffian_grad_batched",
     "LogAbsPfaffian", "SkhError", "pfaffian_z_batched", "logabspfaffian_z_batched",
     "JMatrix", "SkewCholesky", "RowMaximum", "NoPivot", "skewchol", "skewchol_", "skewchol_batched", "ldiv", "rdiv",
+    "svd", "svdvals", "SVD",
     "ENGINE_AUTO", "ENGINE_LARGE", "ENGINE_TEAM1", "version", "LogPfPlan",
 ]
 
@@ -413,6 +414,62 @@ def eigen(A, nb=0, engine=ENGINE_AUTO):
     if not isinstance(A, SkewHermitian) and not isskewhermitian(D):
         raise ArgumentError("matrix `A` must be skew-Hermitian (equal `-A'`)")
     return eigen_(SkewHermitian(np.array(D, dtype=_eigtype(D), order="F", copy=True)), nb, engine)
+
+
+# ----------------------------------------------------------------------------------------
+# singular values (src/eigen.jl:110-144, src/tridiag.jl:526-555): from eigvals / eigen
+# ----------------------------------------------------------------------------------------
+class SVD:
+    """LinearAlgebra.SVD(U, S, Vt): A = U diag(S) Vt."""
+
+    def __init__(self, U, S, Vt):
+        self.U = U
+        self.S = S
+        self.Vt = Vt
+
+    def __iter__(self):
+        return iter((self.U, self.S, self.Vt))
+
+
+def _svd_target(A):
+    if isinstance(A, (SkewHermitian, SkewHermTridiagonal)):
+        return A
+    D = np.asarray(A)
+    if not isskewhermitian(D):
+        raise ArgumentError("matrix `A` must be skew-Hermitian (equal `-A'`)")
+    return SkewHermitian(D)
+
+
+def svdvals(A, nb=0, engine=ENGINE_AUTO):
+    """svdvals(A): |imag(eigvals(A))| sorted in decreasing order (src/eigen.jl:110-114, real;
+    src/tridiag.jl:526-530); a complex SkewHermitian goes through Hermitian(iA) as in the
+    reference (:116: LAPACK there, numpy on the host here)."""
+    A = _svd_target(A)
+    if isinstance(A, SkewHermitian) and np.iscomplexobj(A.data):
+        return np.linalg.svd(1j * np.asarray(A.data), compute_uv=False)
+    vals = np.abs(np.imag(eigvals(A, nb=nb, engine=engine)))
+    return np.sort(vals)[::-1].copy()
+
+
+def svd(A, nb=0, engine=ENGINE_AUTO):
+    """svd(A) from eigen(A) (src/eigen.jl:119-137, src/tridiag.jl:534-553): U = the eigenvectors
+    ordered by decreasing |lambda| (a stable sort, as Julia's sortperm), S = |lambda|, V = -i U
+    with the columns of negative lambda negated; returns SVD(U, S, V^H).  A complex SkewHermitian
+    goes through svd(Hermitian(iA)) with Vt scaled by -i (:138-142)."""
+    A = _svd_target(A)
+    if isinstance(A, SkewHermitian) and np.iscomplexobj(A.data):
+        U, S, Vt = np.linalg.svd(1j * np.asarray(A.data))
+        return SVD(U, S, Vt * (-1j))
+    E = eigen(A, nb=nb, engine=engine)
+    vals = np.imag(E.values).copy()
+    order = np.argsort(-np.abs(vals), kind="stable")
+    vals = vals[order]
+    U = E.vectors[:, order]
+    V = U * (-1j)
+    neg = vals < 0
+    vals[neg] = -vals[neg]
+    V[:, neg] = -V[:, neg]
+    return SVD(U, vals, V.conj().T)
 
 
 # ----------------------------------------------------------------------------------------
