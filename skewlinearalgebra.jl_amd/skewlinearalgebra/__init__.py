@@ -29,7 +29,7 @@ __all__ = [
     "logabspfaffian", "logabspfaffian_", "logabspfaffian_batched", "logabspfaffian_grad_batched",
     "LogAbsPfaffian", "SkhError", "pfaffian_z_batched", "logabspfaffian_z_batched",
     "JMatrix", "SkewCholesky", "RowMaximum", "NoPivot", "skewchol", "skewchol_", "skewchol_batched", "ldiv", "rdiv",
-    "svd", "svdvals", "SVD",
+    "svd", "svdvals", "SVD", "schur", "Schur", "eigvecs",
     "ENGINE_AUTO", "ENGINE_LARGE", "ENGINE_TEAM1", "version", "LogPfPlan",
 ]
 
@@ -470,6 +470,32 @@ def svd(A, nb=0, engine=ENGINE_AUTO):
     vals[neg] = -vals[neg]
     V[:, neg] = -V[:, neg]
     return SVD(U, vals, V.conj().T)
+
+
+class Schur:
+    """LinearAlgebra.Schur(T, Z, values): A = Z T Z^H; for a skew matrix T is diagonal."""
+
+    def __init__(self, T, Z, values):
+        self.T = T
+        self.Schur = T
+        self.Z = Z
+        self.vectors = Z
+        self.values = values
+
+    def __iter__(self):
+        return iter((self.T, self.Z, self.values))
+
+
+def schur(A, nb=0, engine=ENGINE_AUTO):
+    """schur(A::SkewHermitian{<:Real}) (src/skewhermitian.jl:238-243): from eigen(A), T =
+    Diagonal(values) as a dense matrix, Z = the eigenvectors."""
+    E = eigen(A, nb=nb, engine=engine)
+    return Schur(np.diag(E.values), E.vectors, E.values)
+
+
+def eigvecs(A, nb=0, engine=ENGINE_AUTO):
+    """eigvecs(A) = eigen(A).vectors (src/tridiag.jl:524)."""
+    return eigen(A, nb=nb, engine=engine).vectors
 
 
 # ----------------------------------------------------------------------------------------

@@ -47,3 +47,15 @@ def test_svd_complex_and_readme():
     R = np.array([[0, 2, -7, 4], [-2, 0, -8, 3], [7, 8, 0, 1], [-4, -3, -1, 0]], dtype=float)
     vals = np.sort(np.abs(np.imag(np.linalg.eigvals(R))))[::-1]
     np.testing.assert_allclose(skl.svdvals(R), vals, rtol=1e-12)
+
+
+@pytest.mark.parametrize("n", [2, 11, 100])
+def test_schur_and_eigvecs(n):
+    """test/runtests.jl:135-136: A = Z T Z^H; eigvecs(T) = eigen(T).vectors (src/tridiag.jl:524)."""
+    A = skew_gauss(n, 170 + n)
+    say("schur n=%d" % n)
+    F = skl.schur(skl.SkewHermitian(A))
+    assert np.linalg.norm(F.vectors @ F.Schur @ F.vectors.conj().T - A) <= 1e-12 * n * max(1.0, np.linalg.norm(A))
+    assert np.array_equal(F.Schur, np.diag(np.diag(F.Schur)))
+    T = skl.SkewHermTridiagonal(np.random.default_rng(n).standard_normal(n - 1))
+    np.testing.assert_array_equal(skl.eigvecs(T), skl.eigen(T).vectors)
