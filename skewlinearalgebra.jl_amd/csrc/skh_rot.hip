@@ -17,10 +17,19 @@ namespace skh {
 namespace rot {
 
 constexpr int RW = 64;   // one wave per workgroup: rows spread over as many CUs as possible
+
+// the double held by lane u (u a compile-time constant), to every lane
+__device__ __forceinline__ double lane_bcast(double x, int u)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, u);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), u);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
 constexpr int RB = 32;   // columns loaded ahead
 
 // Kernel parameters are plain restrict pointers (not a struct) so the compiler knows the
-// rotation lists are read-only and serves the wave-uniform coefficient loads from the scalar cache.
+// rotation lists are read-only.
 __global__ __launch_bounds__(RW) void k_rot_chains(double* __restrict__ Qo, int64_t ldo,
                                                   const int32_t* __restrict__ ko, const double* __restrict__ cso,
                                                   const int64_t* __restrict__ sego, int64_t nsego,
@@ -30,9 +39,12 @@ __global__ __launch_bounds__(RW) void k_rot_chains(double* __restrict__ Qo, int6
 {
 #pragma clang fp contract(off)
     const bool odd = blockIdx.y == 0;
-    const int64_t r = (int64_t)blockIdx.x * RW + threadIdx.x;
-    if (r >= nrows) return;
-    double* __restrict__ q = (odd ? Qo : Qe) + r;
+    const int lane = threadIdx.x;
+    const int64_t r = (int64_t)blockIdx.x * RW + lane;
+    // every lane stays to the end (the coefficients are broadcast from lanes 0..RB-1); lanes past
+    // the last row work on row 0 and store nothing
+    const bool live = r < nrows;
+    double* __restrict__ q = (odd ? Qo : Qe) + (live ? r : 0);
     const int64_t ld = odd ? ldo : lde;
     const int32_t* __restrict__ K = odd ? ko : ke;
     const double* __restrict__ CS = odd ? cso : cse;
@@ -42,33 +54,43 @@ __global__ __launch_bounds__(RW) void k_rot_chains(double* __restrict__ Qo, int6
         const int64_t t0 = seg[g], len = seg[g + 1] - t0;
         const int64_t k0 = K[t0];   // columns k0 - 1, k0, ..., k0 + len - 1
         double carry = q[(k0 - 1) * ld];
-        auto step = [&](int64_t t, double om) {
-            const double c = CS[2 * t], s = CS[2 * t + 1];
+        auto step = [&](int64_t t, double c, double s, double om) {
             const double sg = carry;
-            q[(k0 + (t - t0) - 1) * ld] = c * sg + s * om;
+            const double out = c * sg + s * om;
+            if (live) q[(k0 + (t - t0) - 1) * ld] = out;
             carry = -s * sg + c * om;
         };
         int64_t j0 = 0;
         if (len >= RB) {
-            // full blocks, each with the next block's columns in flight (no branch inside a block,
-            // so the waits can count outstanding accesses instead of draining them)
+            // full blocks, each with the next block's columns and coefficients in flight (no branch
+            // inside a block, so the waits count outstanding accesses instead of draining them); the
+            // coefficients of rotation u of a block sit in lane u and are broadcast with readlane
             double cur[RB], nxt[RB];
+            double cc = lane < RB ? CS[2 * (t0 + lane)] : 0.0, sc = lane < RB ? CS[2 * (t0 + lane) + 1] : 0.0;
+            double cn = 0.0, sn = 0.0;
 #pragma unroll
             for (int u = 0; u < RB; u++) cur[u] = q[(k0 + u) * ld];
             for (; j0 + 2 * RB <= len; j0 += RB) {
+                // coefficients first: loads retire in order, and the next block needs them first
+                if (lane < RB) {
+                    cn = CS[2 * (t0 + j0 + RB + lane)];
+                    sn = CS[2 * (t0 + j0 + RB + lane) + 1];
+                }
 #pragma unroll
                 for (int u = 0; u < RB; u++) nxt[u] = q[(k0 + j0 + RB + u) * ld];
 #pragma unroll
-                for (int u = 0; u < RB; u++) step(t0 + j0 + u, cur[u]);
+                for (int u = 0; u < RB; u++) step(t0 + j0 + u, lane_bcast(cc, u), lane_bcast(sc, u), cur[u]);
 #pragma unroll
                 for (int u = 0; u < RB; u++) cur[u] = nxt[u];
+                cc = cn;
+                sc = sn;
             }
 #pragma unroll
-            for (int u = 0; u < RB; u++) step(t0 + j0 + u, cur[u]);
+            for (int u = 0; u < RB; u++) step(t0 + j0 + u, lane_bcast(cc, u), lane_bcast(sc, u), cur[u]);
             j0 += RB;
         }
-        for (; j0 < len; j0++) step(t0 + j0, q[(k0 + j0) * ld]);
-        q[(k0 + len - 1) * ld] = carry;
+        for (; j0 < len; j0++) step(t0 + j0, CS[2 * (t0 + j0)], CS[2 * (t0 + j0) + 1], q[(k0 + j0) * ld]);
+        if (live) q[(k0 + len - 1) * ld] = carry;
     }
 }
 
