@@ -31,6 +31,8 @@ extern "C" {
 #define SKH_ERR_BARRIER 3      /* a grid barrier of the cooperative panel kernel timed out */
 #define SKH_ERR_NOT_SKEW 4     /* input is not skew-symmetric (logabspfaffian! of a Matrix) */
 #define SKH_ERR_UNSUPPORTED 5  /* requested option is not available */
+#define SKH_ERR_DOMAIN 6       /* log of a singular skew matrix (Julia's DomainError) */
+#define SKH_ERR_SINGULAR 7     /* LU met an exactly zero pivot in tan/cot/tanh/coth (SingularException) */
 
 /* engine selection in skh_opts.flags */
 #define SKH_ENGINE_AUTO 0
@@ -250,6 +252,37 @@ int skh_skewtrieigen_d(int64_t n, double *ev, double *vals, double *vr, double *
  * arrays in and out; results bit-identical to skh_skewtrieigen_d.  skh_eigen_d takes this path
  * from n = 256. */
 int skh_skewtrieigen_d_gpu(int64_t n, double *ev, double *vals, double *vr, double *vi, int variant);
+
+/* ---- matrix functions (src/exp.jl:3-296) -------------------------------------------------------
+ * f(A) of a skew matrix from its eigen-decomposition, with the eigenvectors kept on the GPU: their
+ * columns scaled by a function of the eigenvalues, the products V D V^T / V D V^H in rocBLAS, the
+ * divisions of tan / cot / tanh / coth by LU in rocSOLVER, and the wrappers the reference applies
+ * (skewhermitian!, Hermitian, hermitian!) made dense.  fn selects the function: */
+#define SKH_FN_EXP 0     /* exp(A), a plain matrix (src/exp.jl:3-48) */
+#define SKH_FN_LOG 1     /* log(A); real: even n and no zero eigenvalue, else SKH_ERR_DOMAIN (:50-97) */
+#define SKH_FN_CIS 2     /* cis(A) = exp(iA), Hermitian, always complex output (:99-121, :204) */
+#define SKH_FN_COS 3     /* cos(A), Hermitian (:123-149) */
+#define SKH_FN_SIN 4     /* sin(A), skew-Hermitian (:151-202) */
+#define SKH_FN_SINCOS 5  /* out1 = sin(A), out2 = cos(A) (:208-256) */
+#define SKH_FN_SINH 6    /* skewhermitian!(exp(A)) (:257) */
+#define SKH_FN_COSH 7    /* hermitian!(exp(A)); complex: its conjugate (:258-263) */
+#define SKH_FN_TAN 8     /* skewhermitian!(cos(A) \ sin(A)) (:265-268) */
+#define SKH_FN_TANH 9    /* skewhermitian!((E + I) \ (E - I)), E = exp(2A) (:270-273) */
+#define SKH_FN_COT 10    /* skewhermitian!(sin(A) \ cos(A)) (:275-278) */
+#define SKH_FN_COTH 11   /* skewhermitian!((E - I) \ (E + I)) (:280-284) */
+/* Real skew A (n x n, lda; host).  out1 (and out2 for SKH_FN_SINCOS): n x n, column-major, ld n;
+ * real except for SKH_FN_CIS, whose out1 holds n x n interleaved complex values (2 n^2 doubles).
+ * Replaces exp/log/cis/cos/sin/sincos/sinh/cosh/tan/tanh/cot/coth(::SkewHermitian{<:Real}),
+ * src/exp.jl:48, :97, :204-206, :256-284. */
+int skh_skewfun_d(int64_t n, const double *A, int64_t lda, int fn, double *out1, double *out2,
+                  const skh_opts *opts);
+/* The same for SkewHermTridiagonal{<:Real}(ev) (ev: n - 1 values), through eigen! of the
+ * tridiagonal (skewtrieigen_merged!, src/tridiag.jl:506). */
+int skh_skewfun_tri_d(int64_t n, const double *ev, int fn, double *out1, double *out2);
+/* Complex skew-Hermitian A (interleaved pairs, lda in complex elements); every output interleaved
+ * complex n x n (src/exp.jl:37-46, :86-95, :112-121, :137-149, :181-202, :235-255). */
+int skh_skewfun_z(int64_t n, const double *A, int64_t lda, int fn, double *out1, double *out2,
+                  const skh_opts *opts);
 
 /* Diagnostics: per-kernel times of the last skh_sytrd_d_device call made with SKH_FLAG_TIMING:
  * out10 = {panel ms, panel launches, rank-2k ms, rank-2k launches, panel algorithmic bytes,

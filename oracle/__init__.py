@@ -209,3 +209,142 @@ def form_q(F, tau):
         # Q <- H_i Q  (apply from the left, building the product right to left)
         Q -= t * np.outer(v, v.conj() @ Q)
     return Q
+
+
+# ----------------------------------------------------------------------------------------
+# Matrix functions (src/exp.jl:3-296), restated in numpy on the oracle's own eigen-decomposition
+# ----------------------------------------------------------------------------------------
+def skeweigen(A):
+    """skeweigen! (src/eigen.jl:75-97) of a real skew A: (vals_imag, Qr, Qim) with the eigenvectors
+    Q Qr + i Q Qim from the oracle's tridiagonalization, its Q and skewtrieigen_divided!."""
+    A = np.array(A, dtype=np.float64)
+    n = A.shape[0]
+    if n == 1:
+        return np.zeros(1), np.ones((1, 1)), np.zeros((1, 1))
+    F, tau, eta = skewblockedhess(A)
+    Q = form_q(F, tau)
+    vals, V = skewtrieigen(eta, "divided")
+    return vals, Q @ V.real, Q @ V.imag
+
+
+def _skewhermitian_(X):
+    """skewhermitian!(X) (src/skewhermitian.jl:119-131): lower a = (X[i,j] - conj(X[j,i])) / 2, upper
+    -conj(a), diagonal 0 (real) or i imag (complex)."""
+    X = np.array(X)
+    L = np.tril(X, -1)
+    U = np.triu(X, -0)
+    a = (L - np.conj(U).T) / 2
+    a = np.tril(a, -1)
+    out = a - np.conj(a).T
+    d = np.diagonal(X)
+    out[np.diag_indices_from(out)] = (1j * d.imag) if np.iscomplexobj(X) else 0.0
+    return out
+
+
+def _hermitian_view(X):
+    """Matrix(Hermitian(X)) with uplo 'U': the upper triangle mirrored, the diagonal made real."""
+    U = np.triu(X, 1)
+    out = U + np.conj(U).T
+    out = out.astype(X.dtype)
+    out[np.diag_indices_from(out)] = np.diagonal(X).real
+    return out
+
+
+def _cosh_part(X):
+    """cosh's hermitian!(exp(A)) (+ conj for complex) as a dense matrix (src/exp.jl:258-263, :286-296):
+    upper a = (X[j,i] + conj(X[i,j])) / 2, lower conj(a), real diagonal."""
+    U = np.triu(X, 1)
+    L = np.tril(X, -1)
+    a = (U + np.conj(L).T) / 2
+    out = a + np.conj(a).T
+    out = out.astype(X.dtype)
+    out[np.diag_indices_from(out)] = np.diagonal(X).real
+    return out
+
+
+def _fun_from_eigs_real(vals, Qr, Qim, fn, n):
+    if fn == "log":
+        if n % 2 == 1 or np.any(vals == 0):
+            raise ValueError("DomainError: Logarithm of a singular matrix doesn't exist")
+    if fn in ("exp", "log"):
+        if fn == "exp":
+            a, b = np.cos(vals), np.sin(vals)
+        else:
+            a, b = np.log(np.abs(vals)), np.sign(vals) * np.pi / 2
+        Q1 = Qr * a - Qim * b                      # src/exp.jl:26-28
+        Q2 = Qr * b + Qim * a
+        return Q1 @ Qr.T + Q2 @ Qim.T
+    E = np.exp(-vals)                              # src/exp.jl:134-142, :177-185
+    Q1, Q2 = Qr * E, Qim * E
+    Cos = Q1 @ Qr.T + Q2 @ Qim.T
+    Sin = Q2 @ Qr.T - Q1 @ Qim.T
+    if fn == "cis":                                # src/exp.jl:99-109: Hermitian(Q E Q^H)
+        return _hermitian_view(Cos + 1j * Sin)
+    if fn == "cos":
+        return _hermitian_view(Cos)
+    return _skewhermitian_(Sin), _hermitian_view(Cos)
+
+
+def _fun_from_eigs_complex(vals, V, fn):
+    if fn in ("exp", "log"):
+        d = np.exp(1j * vals) if fn == "exp" else np.log(1j * vals)
+        return V @ (d[:, None] * V.conj().T)          # src/exp.jl:37-46, :86-95
+    P1 = (V * np.exp(-vals)) @ V.conj().T
+    if fn == "cis":
+        return _hermitian_view(P1)
+    P2 = (V * np.exp(vals)) @ V.conj().T
+    if fn == "cos":                                   # src/exp.jl:137-149
+        return _hermitian_view((P1 + P2) / 2)
+    return _skewhermitian_((P1 - P2) * (-0.5j)), _hermitian_view((P1 + P2) / 2)
+
+
+def _fun(eigs, fn, n, cplx):
+    """fn of the matrix whose eigen-decomposition eigs(scale) gives (src/exp.jl:48-284)."""
+    def base(name, scale=1.0):
+        e = eigs(scale)
+        if cplx:
+            return _fun_from_eigs_complex(e[0], e[1], name)
+        return _fun_from_eigs_real(e[0], e[1], e[2], name, n)
+    if fn in ("exp", "log", "cis", "cos"):
+        return base(fn)
+    if fn in ("sin", "sincos", "tan", "cot"):
+        S, C = base("sincos")
+        if fn == "sin":
+            return S
+        if fn == "sincos":
+            return S, C
+        return _skewhermitian_(np.linalg.solve(C, S) if fn == "tan" else np.linalg.solve(S, C))
+    if fn == "sinh":
+        return _skewhermitian_(base("exp"))
+    if fn == "cosh":
+        return _cosh_part(base("exp"))
+    E = base("exp", 2.0)                               # tanh, coth: exp(2A) (src/exp.jl:270-284)
+    I = np.eye(n)
+    return _skewhermitian_(np.linalg.solve(E + I, E - I) if fn == "tanh" else np.linalg.solve(E - I, E + I))
+
+
+def skewfun(A, fn):
+    """exp/log/cis/cos/sin/sincos/sinh/cosh/tan/tanh/cot/coth of a skew matrix (src/exp.jl), dense
+    results; real A through skeweigen!, complex A through eigen of Hermitian(iA) (src/eigen.jl:101-106:
+    values -lambda, the upper triangle of iA)."""
+    A = np.asarray(A)
+    n = A.shape[0]
+    if np.iscomplexobj(A):
+        def eigs(scale):
+            lam, V = np.linalg.eigh(1j * A * scale, UPLO="U")
+            return -lam, V
+        return _fun(eigs, fn, n, True)
+    return _fun(lambda scale: skeweigen(A * scale), fn, n, False)
+
+
+def skewfun_tri(ev, fn):
+    """The same for SkewHermTridiagonal{<:Real}(ev), through skewtrieigen_merged! (src/tridiag.jl:506)."""
+    ev = np.asarray(ev, dtype=np.float64)
+    n = ev.size + 1
+
+    def eigs(scale):
+        if n == 1:
+            return np.zeros(1), np.ones((1, 1)), np.zeros((1, 1))
+        vals, V = skewtrieigen(ev * scale, "merged")
+        return vals, V.real, V.imag
+    return _fun(eigs, fn, n, False)

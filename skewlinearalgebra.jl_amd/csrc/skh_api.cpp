@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -873,6 +874,7 @@ int zpf_host(int64_t batch, const int64_t* n, const int64_t* offset, const doubl
 
 // ---- eigenvector rotations of the skew-tridiagonal QR on the GPU ----
 Buf g_rotq, g_rotk, g_rotcs, g_rotseg;
+Buf g_fvec, g_fpiv, g_fz;   // matrix functions: eigenvalue vectors, LU pivots, complex work matrices
 constexpr int64_t ROT_GPU_MIN = 256;   // eigen(A) applies the QR rotations on the GPU from this n
 
 std::vector<int64_t> chain_starts(const int32_t* k, int64_t cnt)
@@ -1047,7 +1049,7 @@ int skh_finalize(void)
     for (Buf* b : {&g_mat, &g_ws, &g_vec, &g_ctl, &g_bin, &g_bout, &g_q, &g_ev, &g_gq, &g_gk, &g_gz, &g_gv, &g_gp,
                    &g_gte, &g_cslot, &g_big, &g_bctl, &g_zws, &g_zq, &g_zc, &g_zx, &g_zpslot, &g_zpctl, &g_zpin,
                    &g_zpout, &g_chctl, &g_chin, &g_chout, &g_chp,
-                   &g_chws, &g_rotq, &g_rotk, &g_rotcs, &g_rotseg})
+                   &g_chws, &g_rotq, &g_rotk, &g_rotcs, &g_rotseg, &g_fvec, &g_fpiv, &g_fz})
         b->release();
     if (g_blas) (void)rocblas_destroy_handle(g_blas);
     g_blas = nullptr;
@@ -1235,6 +1237,74 @@ int skh_orgtr_d_device(int64_t n, const double* dF, int64_t ldf, const double* d
     return orgtr_locked(n, dF, ldf, dtau, dQ, ldq, (hipStream_t)stream);
 }
 
+// skeweigen! (src/eigen.jl:75-97) of the real skew A times `scale` (1 or 2, exact), the eigenvectors left
+// on the device.  The host part (tridiagonalization call, QR sweeps) runs unlocked; on success lk holds g_mu,
+// vals (host) holds the imaginary parts of the eigenvalues and *base a buffer of `slots` >= 5 n x n
+// matrices with Q Qr in slot EV_R and Q Qim in slot EV_I (slots 1-3 and from 5 on are free).
+}  // extern "C"
+
+namespace {
+constexpr int EV_I = 0, EV_R = 4;
+int eigen_d_dev(int64_t n, const double* A, int64_t lda, double scale, double* vals, const skh_opts* opts, int slots,
+                std::unique_lock<std::mutex>& lk, double** base)
+{
+    const size_t nn = (size_t)n * n;
+    std::vector<double> F(nn), tau(std::max<int64_t>(n - 1, 1)), eta(std::max<int64_t>(n - 1, 1));
+    for (int64_t c = 0; c < n; c++)
+        for (int64_t r = 0; r < n; r++) F[c * n + r] = A[c * lda + r] * scale;
+    // implicit QR with Givens accumulation (skewtrieigen_divided!, src/eigen.jl:87): the sweeps on
+    // the host, their rotations applied to Qodd / Qeven on the GPU from n = 256
+    std::vector<double> hr(nn), hi(nn);
+    if (n == 1) {   // src/eigen.jl:77-79
+        vals[0] = 0.0;
+        hr[0] = 1.0;
+        hi[0] = 0.0;
+    } else {
+        int rc = skh_sytrd_d(n, F.data(), n, tau.data(), eta.data(), nullptr, opts);
+        if (rc) return rc;
+        rc = skh_internal_skewtrieigen(n, eta.data(), vals, hr.data(), hi.data(), 0,
+                                       n >= ROT_GPU_MIN ? rot_apply_gpu : nullptr, nullptr);
+        if (rc == 2) return set_err(SKH_ERR_NOCONV, "skewtrieigen did not converge");
+        if (rc) return rc < 0 ? set_err(rc, "skewtrieigen failed (%d)", rc) : rc;
+    }
+    lk = std::unique_lock<std::mutex>(g_mu);
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    hipStream_t st = g_stream;
+    double* d = (double*)g_q.get((size_t)slots * nn * sizeof(double) + 2 * n * sizeof(double));
+    if (!d) return set_err(SKH_ERR_HIP, "hipMalloc failed for eigen n = %lld", (long long)n);
+    *base = d;
+    if (n == 1) {
+        HIPCHK(hipMemcpyAsync(d + EV_R * nn, hr.data(), sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d + EV_I * nn, hi.data(), sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return 0;
+    }
+    double* dF = d;
+    double* dQ = d + nn;
+    double* dR = d + 2 * nn;
+    double* dI = d + 3 * nn;
+    double* dtau = d + slots * nn;
+    HIPCHK(hipMemcpyAsync(dF, F.data(), nn * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dtau, tau.data(), (n - 1) * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dR, hr.data(), nn * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dI, hi.data(), nn * sizeof(double), hipMemcpyHostToDevice, st));
+    rc = orgtr_locked(n, dF, n, dtau, dQ, n, st);
+    if (rc) return rc;
+    rc = blas_on(st);
+    if (rc) return rc;
+    // vectorsreal = Q * Qr, vectorsim = Q * Qim (src/eigen.jl:88-89); the factors in slot 0 are spent
+    const double one = 1.0, zero = 0.0;
+    BLASCHK(rocblas_dgemm(g_blas, rocblas_operation_none, rocblas_operation_none, (int)n, (int)n, (int)n, &one, dQ,
+                          (int)n, dR, (int)n, &zero, d + EV_R * nn, (int)n));
+    BLASCHK(rocblas_dgemm(g_blas, rocblas_operation_none, rocblas_operation_none, (int)n, (int)n, (int)n, &one, dQ,
+                          (int)n, dI, (int)n, &zero, d + EV_I * nn, (int)n));
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
 int skh_eigen_d(int64_t n, const double* A, int64_t lda, double* vals, double* vr, double* vi, const skh_opts* opts)
 {
     if (n < 0) return set_err(-1, "n < 0");
@@ -1247,46 +1317,14 @@ int skh_eigen_d(int64_t n, const double* A, int64_t lda, double* vals, double* v
         vi[0] = 0.0;
         return 0;
     }
-    std::vector<double> F((size_t)n * n), tau(n - 1), eta(n - 1);
-    for (int64_t c = 0; c < n; c++) std::memcpy(&F[c * n], A + c * lda, n * sizeof(double));
-    int rc = skh_sytrd_d(n, F.data(), n, tau.data(), eta.data(), nullptr, opts);
+    std::unique_lock<std::mutex> lk;
+    double* d = nullptr;
+    int rc = eigen_d_dev(n, A, lda, 1.0, vals, opts, 5, lk, &d);
     if (rc) return rc;
-    // implicit QR with Givens accumulation (skewtrieigen_divided!, src/eigen.jl:87): the sweeps on
-    // the host, their rotations applied to Qodd / Qeven on the GPU from n = 256
-    std::vector<double> hr((size_t)n * n), hi((size_t)n * n);
-    rc = skh_internal_skewtrieigen(n, eta.data(), vals, hr.data(), hi.data(), 0,
-                                   n >= ROT_GPU_MIN ? rot_apply_gpu : nullptr, nullptr);
-    if (rc == 2) return set_err(SKH_ERR_NOCONV, "skewtrieigen did not converge");
-    if (rc) return rc < 0 ? set_err(rc, "skewtrieigen failed (%d)", rc) : rc;
-    std::lock_guard<std::mutex> lk(g_mu);
-    rc = ensure_init_locked(g_device);
-    if (rc) return rc;
-    hipStream_t st = g_stream;
     const size_t nn = (size_t)n * n;
-    double* d = (double*)g_q.get(5 * nn * sizeof(double) + 2 * n * sizeof(double));
-    if (!d) return set_err(SKH_ERR_HIP, "hipMalloc failed for eigen n = %lld", (long long)n);
-    double* dF = d;
-    double* dQ = dF + nn;
-    double* dR = dQ + nn;
-    double* dI = dR + nn;
-    double* dO = dI + nn;
-    double* dtau = dO + nn;
-    HIPCHK(hipMemcpyAsync(dF, F.data(), nn * sizeof(double), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dtau, tau.data(), (n - 1) * sizeof(double), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dR, hr.data(), nn * sizeof(double), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dI, hi.data(), nn * sizeof(double), hipMemcpyHostToDevice, st));
-    rc = orgtr_locked(n, dF, n, dtau, dQ, n, st);
-    if (rc) return rc;
-    // vectorsreal = Q * Qr, vectorsim = Q * Qim (src/eigen.jl:88-89)
-    const double one = 1.0, zero = 0.0;
-    BLASCHK(rocblas_dgemm(g_blas, rocblas_operation_none, rocblas_operation_none, (int)n, (int)n, (int)n, &one, dQ,
-                          (int)n, dR, (int)n, &zero, dO, (int)n));
-    HIPCHK(hipMemcpyAsync(vr, dO, nn * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    BLASCHK(rocblas_dgemm(g_blas, rocblas_operation_none, rocblas_operation_none, (int)n, (int)n, (int)n, &one, dQ,
-                          (int)n, dI, (int)n, &zero, dO, (int)n));
-    HIPCHK(hipMemcpyAsync(vi, dO, nn * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipMemcpyAsync(vr, d + EV_R * nn, nn * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpyAsync(vi, d + EV_I * nn, nn * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
     return 0;
 }
 
@@ -1640,22 +1678,20 @@ int skh_eigvals_z(int64_t n, const double* A, int64_t lda, double* vals, const s
     return 0;
 }
 
-// eigen-decomposition of the complex skew-Hermitian A (src/eigen.jl:101-106): values -lambda
-// (imaginary parts) for ascending lambda of iA, vectors X = Q D Z (interleaved complex, ld n)
-int skh_eigen_z(int64_t n, const double* A, int64_t lda, double* vals, double* V, const skh_opts* opts)
+// eigen-decomposition of the complex skew-Hermitian A times `scale` (1 or 2, exact) (src/eigen.jl:101-106):
+// vals = -lambda (imaginary parts of the eigenvalues) for ascending lambda of iA, eigenvectors X = Q D Z
+// left on the device in *dV (ld n).  The caller holds g_mu.
+}  // extern "C"
+
+namespace {
+int eigen_z_dev_locked(int64_t n, const double* A, int64_t lda, double scale, double* vals, double2** dV)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
-    if (n < 0) return set_err(-1, "n < 0");
-    if (n > 0 && (!A || !vals || !V)) return set_err(-2, "NULL argument");
-    if (lda < std::max<int64_t>(n, 1)) return set_err(-3, "lda < n");
-    (void)opts;
-    if (n == 0) return 0;
-    if (n > 46340) return set_err(SKH_ERR_UNSUPPORTED, "n too large for the solver interface");
     int rc = ensure_init_locked(g_device);
     if (rc) return rc;
     hipStream_t st = g_stream;
     double2* dA = zupload(n, A, lda, st, &rc);
     if (!dA) return rc;
+    if (scale != 1.0) HIPCHK((hipError_t)skh_internal_fun_scale(2 * n * n, (double*)dA, scale, st));
     double* dv = (double*)g_vec.get((size_t)(4 * n + 4) * sizeof(double));
     double2* dQ = (double2*)g_zq.get((size_t)n * n * sizeof(double2));
     double* dC = (double*)g_zc.get((size_t)(n * n + 2 * n + 2) * sizeof(double));
@@ -1691,14 +1727,33 @@ int skh_eigen_z(int64_t n, const double* A, int64_t lda, double* vals, double* V
     HIPCHK(hipStreamSynchronize(st));
     if (info != 0) return set_err(SKH_ERR_NOCONV, "tridiagonal divide and conquer failed (info %d)", info);
     HIPCHK((hipError_t)skh_internal_zphase(dC, n, dX, st));   // dX = D Z (reuses dX as Zc)
-    // vectors = Q (D Z): write into dA (no longer needed) then copy out
+    // vectors = Q (D Z), into dA (no longer needed)
     const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
     BLASCHK(rocblas_zgemm(g_blas, rocblas_operation_none, rocblas_operation_none, (int)n, (int)n, (int)n, &one,
                           (const rocblas_double_complex*)dQ, (int)n, (const rocblas_double_complex*)dX, (int)n, &zero,
                           (rocblas_double_complex*)dA, (int)n));
-    HIPCHK(hipMemcpyAsync(V, dA, (size_t)n * n * sizeof(double2), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
     for (int64_t k = 0; k < n; k++) vals[k] = -d[k];
+    *dV = dA;
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int skh_eigen_z(int64_t n, const double* A, int64_t lda, double* vals, double* V, const skh_opts* opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (n < 0) return set_err(-1, "n < 0");
+    if (n > 0 && (!A || !vals || !V)) return set_err(-2, "NULL argument");
+    if (lda < std::max<int64_t>(n, 1)) return set_err(-3, "lda < n");
+    (void)opts;
+    if (n == 0) return 0;
+    if (n > 46340) return set_err(SKH_ERR_UNSUPPORTED, "n too large for the solver interface");
+    double2* dV = nullptr;
+    int rc = eigen_z_dev_locked(n, A, lda, 1.0, vals, &dV);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(V, dV, (size_t)n * n * sizeof(double2), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
     return 0;
 }
 
@@ -1828,6 +1883,290 @@ int skh_eigvals_range_d(int64_t n, const double* A, int64_t lda, int64_t il, int
     rc = skh_skewtrieigvals_range_d(n, eta.data(), il, iu, vl, vh, by_value, vals, m);
     if (rc == 2) return set_err(SKH_ERR_NOCONV, "tridiagonal QL did not converge");
     return rc;
+}
+
+// ----------------------------------------------------------------------------------------
+// Matrix functions of a skew matrix (src/exp.jl:3-296) from its eigen-decomposition: the eigenvectors
+// stay on the device, their columns are scaled by a function of the eigenvalues, the products
+// V D V^T (real parts and imaginary parts separately, as the reference) / V D V^H run in rocBLAS,
+// tan / cot / tanh / coth solve with rocSOLVER's LU, and the symmetrizations run in skh_fun.hip.
+// ----------------------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+constexpr int FUN_SLOTS = 8;   // real case: EV_I, T1, T2, P1, EV_R, P2, and two for the complex cis output
+
+bool fun_valid(int fn) { return fn >= SKH_FN_EXP && fn <= SKH_FN_COTH; }
+bool fun_expish(int fn)
+{
+    return fn == SKH_FN_EXP || fn == SKH_FN_LOG || fn == SKH_FN_SINH || fn == SKH_FN_COSH || fn == SKH_FN_TANH ||
+           fn == SKH_FN_COTH;
+}
+double fun_scale(int fn) { return fn == SKH_FN_TANH || fn == SKH_FN_COTH ? 2.0 : 1.0; }   // exp(2A)
+
+// L <- LU(L), R <- L \ R (n x n, ld n); an exactly zero pivot is Julia's SingularException
+int lu_solve_locked(int64_t n, double* L, double* R, bool cplx, hipStream_t st)
+{
+    rocblas_int* piv = (rocblas_int*)g_fpiv.get((size_t)(n + 1) * sizeof(rocblas_int));
+    if (!piv) return set_err(SKH_ERR_HIP, "hipMalloc failed (pivots)");
+    rocblas_int* dinfo = piv + n;
+    const rocblas_int m = (rocblas_int)n;
+    if (cplx)
+        BLASCHK(rocsolver_zgetrf(g_blas, m, m, (rocblas_double_complex*)L, m, piv, dinfo));
+    else
+        BLASCHK(rocsolver_dgetrf(g_blas, m, m, L, m, piv, dinfo));
+    rocblas_int info = 0;
+    HIPCHK(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (info > 0) return set_err(SKH_ERR_SINGULAR, "SingularException(%d): the matrix to divide by is singular", info);
+    if (cplx)
+        BLASCHK(rocsolver_zgetrs(g_blas, rocblas_operation_none, m, m, (rocblas_double_complex*)L, m, piv,
+                                 (rocblas_double_complex*)R, m));
+    else
+        BLASCHK(rocsolver_dgetrs(g_blas, rocblas_operation_none, m, m, L, m, piv, R, m));
+    return 0;
+}
+
+// the rest of fn after E = exp(A) (or exp(2A)) in E, with W a free n x n work matrix
+int fun_from_exp_locked(int64_t n, int fn, bool cplx, double* E, double* W, double* out1, hipStream_t st)
+{
+    const size_t bytes = (size_t)n * n * (cplx ? 2 : 1) * sizeof(double);
+    if (fn == SKH_FN_SINH) HIPCHK((hipError_t)skh_internal_fun_sympart(n, E, cplx, 0, st));   // skewhermitian!(exp(A))
+    if (fn == SKH_FN_COSH) HIPCHK((hipError_t)skh_internal_fun_sympart(n, E, cplx, 2, st));   // hermitian!(exp(A))
+    if (fn == SKH_FN_TANH || fn == SKH_FN_COTH) {
+        // tanh: skewhermitian!((E + I) \ (E - I)); coth: skewhermitian!((E - I) \ (E + I))  (src/exp.jl:270-284)
+        const double sl = fn == SKH_FN_TANH ? 1.0 : -1.0;
+        HIPCHK(hipMemcpyAsync(W, E, bytes, hipMemcpyDeviceToDevice, st));
+        HIPCHK((hipError_t)skh_internal_fun_adddiag(n, W, cplx, sl, st));
+        HIPCHK((hipError_t)skh_internal_fun_adddiag(n, E, cplx, -sl, st));
+        int rc = lu_solve_locked(n, W, E, cplx, st);
+        if (rc) return rc;
+        HIPCHK((hipError_t)skh_internal_fun_sympart(n, E, cplx, 0, st));
+    }
+    HIPCHK(hipMemcpyAsync(out1, E, bytes, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// S (raw sin) and C (raw cos) to the outputs of sin / cos / sincos / tan / cot
+int fun_from_sincos_locked(int64_t n, int fn, bool cplx, double* S, double* C, double* out1, double* out2,
+                           hipStream_t st)
+{
+    const size_t bytes = (size_t)n * n * (cplx ? 2 : 1) * sizeof(double);
+    const bool need_s = fn != SKH_FN_COS, need_c = fn != SKH_FN_SIN;
+    if (need_s) HIPCHK((hipError_t)skh_internal_fun_sympart(n, S, cplx, 0, st));   // skewhermitian!(Sin)
+    if (need_c) HIPCHK((hipError_t)skh_internal_fun_sympart(n, C, cplx, 1, st));   // Hermitian(Cos)
+    double* res = fn == SKH_FN_COS ? C : S;
+    if (fn == SKH_FN_TAN || fn == SKH_FN_COT) {
+        // tan: skewhermitian!(C \ S); cot: skewhermitian!(S \ C)  (src/exp.jl:265-278)
+        double* L = fn == SKH_FN_TAN ? C : S;
+        res = fn == SKH_FN_TAN ? S : C;
+        int rc = lu_solve_locked(n, L, res, cplx, st);
+        if (rc) return rc;
+        HIPCHK((hipError_t)skh_internal_fun_sympart(n, res, cplx, 0, st));
+    }
+    HIPCHK(hipMemcpyAsync(out1, res, bytes, hipMemcpyDeviceToHost, st));
+    if (fn == SKH_FN_SINCOS) HIPCHK(hipMemcpyAsync(out2, C, bytes, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// real case: eigenvalues i vals[j], eigenvectors Vr + i Vi in slots EV_R / EV_I of base
+int fun_real_locked(int64_t n, int fn, const double* vals, double* base, double* out1, double* out2)
+{
+    hipStream_t st = g_stream;
+    const size_t nn = (size_t)n * n;
+    double* Vr = base + EV_R * nn;
+    double* Vi = base + EV_I * nn;
+    double* T1 = base + 1 * nn;
+    double* T2 = base + 2 * nn;
+    double* P1 = base + 3 * nn;
+    double* P2 = base + 5 * nn;
+    double* W = base + 6 * nn;   // slots 6 and 7: work matrix, or the complex cis output
+    std::vector<double> a(n), b(n);
+    const bool ex = fun_expish(fn);
+    for (int64_t j = 0; j < n; j++) {
+        if (fn == SKH_FN_LOG) {   // log|lambda| and sign(lambda) pi/2 (src/exp.jl:66-69)
+            if (vals[j] == 0.0) return set_err(SKH_ERR_DOMAIN, "Logarithm of a singular matrix doesn't exist");
+            a[j] = std::log(std::fabs(vals[j]));
+            b[j] = (vals[j] > 0.0 ? 1.0 : -1.0) * M_PI / 2;
+        } else if (ex) {          // sincos(lambda) (src/exp.jl:19-21)
+            a[j] = std::cos(vals[j]);
+            b[j] = std::sin(vals[j]);
+        } else {                  // exp(-lambda): the eigenvalues of iA (src/exp.jl:111, :134, :177)
+            a[j] = std::exp(-vals[j]);
+        }
+    }
+    double* dv = (double*)g_fvec.get((size_t)2 * n * sizeof(double));
+    if (!dv) return set_err(SKH_ERR_HIP, "hipMalloc failed (eigenvalue vectors)");
+    HIPCHK(hipMemcpyAsync(dv, a.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dv + n, b.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK((hipError_t)skh_internal_fun_colmix(n, Vr, Vi, dv, dv + n, ex ? 0 : 1, T1, T2, st));
+    int rc = blas_on(st);
+    if (rc) return rc;
+    const double one = 1.0, zero = 0.0, mone = -1.0;
+    const int m = (int)n;
+    const rocblas_operation N = rocblas_operation_none, T = rocblas_operation_transpose;
+    if (ex) {
+        // (Qr C - Qim S) Qr^T + (Qr S + Qim C) Qim^T  (src/exp.jl:26-33; log: :76-83)
+        BLASCHK(rocblas_dgemm(g_blas, N, T, m, m, m, &one, T1, m, Vr, m, &zero, P1, m));
+        BLASCHK(rocblas_dgemm(g_blas, N, T, m, m, m, &one, T2, m, Vi, m, &one, P1, m));
+        if (fn == SKH_FN_EXP || fn == SKH_FN_LOG) {
+            HIPCHK(hipMemcpyAsync(out1, P1, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            return 0;
+        }
+        return fun_from_exp_locked(n, fn, false, P1, P2, out1, st);
+    }
+    // E = exp(-lambda): Cos = Qr E Qr^T + Qim E Qim^T, Sin = Qim E Qr^T - Qr E Qim^T (src/exp.jl:168-174,
+    // :219-225, :234-241); cis = Cos + i Sin as a Hermitian matrix (src/exp.jl:98-109)
+    const bool need_s = fn != SKH_FN_COS, need_c = fn != SKH_FN_SIN;
+    if (need_c) {
+        BLASCHK(rocblas_dgemm(g_blas, N, T, m, m, m, &one, T1, m, Vr, m, &zero, P2, m));
+        BLASCHK(rocblas_dgemm(g_blas, N, T, m, m, m, &one, T2, m, Vi, m, &one, P2, m));
+    }
+    if (need_s) {
+        BLASCHK(rocblas_dgemm(g_blas, N, T, m, m, m, &one, T2, m, Vr, m, &zero, P1, m));
+        BLASCHK(rocblas_dgemm(g_blas, N, T, m, m, m, &mone, T1, m, Vi, m, &one, P1, m));
+    }
+    if (fn == SKH_FN_CIS) {
+        HIPCHK((hipError_t)skh_internal_fun_cis_out(n, P2, P1, W, st));
+        HIPCHK(hipMemcpyAsync(out1, W, 2 * nn * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return 0;
+    }
+    return fun_from_sincos_locked(n, fn, false, P1, P2, out1, out2, st);
+}
+
+// complex case: eigenvalues i vals[j], eigenvectors V (device, ld n)
+int fun_complex_locked(int64_t n, int fn, const double* vals, const double2* V, double* out1, double* out2)
+{
+    hipStream_t st = g_stream;
+    const size_t nn = (size_t)n * n;
+    double2* Z = (double2*)g_fz.get(3 * nn * sizeof(double2));
+    double2* dv = (double2*)g_fvec.get((size_t)2 * n * sizeof(double2));
+    if (!Z || !dv) return set_err(SKH_ERR_HIP, "hipMalloc failed (complex work matrices)");
+    double2* Tm = Z;
+    double2* P1 = Z + nn;
+    double2* P2 = Z + 2 * nn;
+    const bool ex = fun_expish(fn);
+    std::vector<double2> d1(n), d2(n);
+    for (int64_t j = 0; j < n; j++) {
+        if (fn == SKH_FN_LOG) {   // log.(values), values = i vals (src/exp.jl:87-90)
+            const std::complex<double> z = std::log(std::complex<double>(0.0, vals[j]));
+            d1[j] = make_double2(z.real(), z.imag());
+        } else if (ex) {          // exp.(values) (src/exp.jl:39-41)
+            d1[j] = make_double2(std::cos(vals[j]), std::sin(vals[j]));
+        } else {                  // exp(-imag(values)), exp(imag(values)) (src/exp.jl:113-118, :145-151)
+            d1[j] = make_double2(std::exp(-vals[j]), 0.0);
+            d2[j] = make_double2(std::exp(vals[j]), 0.0);
+        }
+    }
+    HIPCHK(hipMemcpyAsync(dv, d1.data(), n * sizeof(double2), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dv + n, d2.data(), n * sizeof(double2), hipMemcpyHostToDevice, st));
+    int rc = blas_on(st);
+    if (rc) return rc;
+    const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+    const int m = (int)n;
+    const rocblas_operation N = rocblas_operation_none, C = rocblas_operation_conjugate_transpose;
+    auto rz = [](const double2* p) { return (const rocblas_double_complex*)p; };
+    // P = (V diag(d)) V^H
+    auto vdvh = [&](const double2* d, double2* P) -> int {
+        HIPCHK((hipError_t)skh_internal_fun_zcolscale(n, (const double*)V, (const double*)d, (double*)Tm, st));
+        BLASCHK(rocblas_zgemm(g_blas, N, C, m, m, m, &one, rz(Tm), m, rz(V), m, &zero, (rocblas_double_complex*)P, m));
+        return 0;
+    };
+    rc = vdvh(dv, P1);
+    if (rc) return rc;
+    if (ex) {
+        if (fn == SKH_FN_EXP || fn == SKH_FN_LOG) {
+            HIPCHK(hipMemcpyAsync(out1, P1, nn * sizeof(double2), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            return 0;
+        }
+        return fun_from_exp_locked(n, fn, true, (double*)P1, (double*)P2, out1, st);
+    }
+    if (fn == SKH_FN_CIS) {   // Hermitian(V e^{-mu} V^H)
+        HIPCHK((hipError_t)skh_internal_fun_sympart(n, (double*)P1, 1, 1, st));
+        HIPCHK(hipMemcpyAsync(out1, P1, nn * sizeof(double2), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return 0;
+    }
+    rc = vdvh(dv + n, P2);
+    if (rc) return rc;
+    const int mode = fn == SKH_FN_COS ? 0 : fn == SKH_FN_SIN ? 1 : 2;
+    HIPCHK((hipError_t)skh_internal_fun_zsincos(n, (double*)P1, (double*)P2, mode, st));   // P1 <- Sin, P2 <- Cos
+    return fun_from_sincos_locked(n, fn, true, (double*)P1, (double*)P2, out1, out2, st);
+}
+
+int fun_args(int64_t n, const void* A, int64_t lda, int fn, const double* out1, const double* out2)
+{
+    if (n < 0) return set_err(-1, "n < 0");
+    if (n > 0 && !A) return set_err(-2, "A is NULL");
+    if (lda < std::max<int64_t>(n, 1)) return set_err(-3, "lda < n");
+    if (!fun_valid(fn)) return set_err(-4, "unknown matrix function %d", fn);
+    if (n > 0 && !out1) return set_err(-5, "out1 is NULL");
+    if (n > 0 && fn == SKH_FN_SINCOS && !out2) return set_err(-6, "out2 is NULL (sincos has two outputs)");
+    if (n > 46340) return set_err(SKH_ERR_UNSUPPORTED, "n too large for the BLAS interface");
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int skh_skewfun_d(int64_t n, const double* A, int64_t lda, int fn, double* out1, double* out2, const skh_opts* opts)
+{
+    int rc = fun_args(n, A, lda, fn, out1, out2);
+    if (rc || n == 0) return rc;
+    if (fn == SKH_FN_LOG && n % 2) return set_err(SKH_ERR_DOMAIN, "Logarithm of a singular matrix doesn't exist");
+    std::vector<double> vals(n);
+    std::unique_lock<std::mutex> lk;
+    double* base = nullptr;
+    rc = eigen_d_dev(n, A, lda, fun_scale(fn), vals.data(), opts, FUN_SLOTS, lk, &base);
+    if (rc) return rc;
+    return fun_real_locked(n, fn, vals.data(), base, out1, out2);
+}
+
+int skh_skewfun_tri_d(int64_t n, const double* ev, int fn, double* out1, double* out2)
+{
+    int rc = fun_args(n, n > 1 ? (const void*)ev : (const void*)out1, std::max<int64_t>(n, 1), fn, out1, out2);
+    if (rc || n == 0) return rc;
+    if (fn == SKH_FN_LOG && n % 2) return set_err(SKH_ERR_DOMAIN, "Logarithm of a singular matrix doesn't exist");
+    // eigen!(SkewHermTridiagonal) = skewtrieigen_merged! (src/tridiag.jl:506); rotations on the GPU from n = 256
+    const size_t nn = (size_t)n * n;
+    std::vector<double> e(std::max<int64_t>(n - 1, 1)), vals(n), hr(nn), hi(nn);
+    const double sc = fun_scale(fn);
+    for (int64_t k = 0; k + 1 < n; k++) e[k] = ev[k] * sc;
+    if (n == 1) {
+        vals[0] = 0.0;
+        hr[0] = 1.0;
+    } else {
+        rc = skh_internal_skewtrieigen(n, e.data(), vals.data(), hr.data(), hi.data(), 1,
+                                       n >= ROT_GPU_MIN ? rot_apply_gpu : nullptr, nullptr);
+        if (rc == 2) return set_err(SKH_ERR_NOCONV, "tridiagonal QR did not converge");
+        if (rc) return rc < 0 ? set_err(rc, "skewtrieigen failed (%d)", rc) : rc;
+    }
+    std::lock_guard<std::mutex> lk(g_mu);
+    rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    double* base = (double*)g_q.get((size_t)FUN_SLOTS * nn * sizeof(double));
+    if (!base) return set_err(SKH_ERR_HIP, "hipMalloc failed for n = %lld", (long long)n);
+    HIPCHK(hipMemcpyAsync(base + EV_R * nn, hr.data(), nn * sizeof(double), hipMemcpyHostToDevice, g_stream));
+    HIPCHK(hipMemcpyAsync(base + EV_I * nn, hi.data(), nn * sizeof(double), hipMemcpyHostToDevice, g_stream));
+    return fun_real_locked(n, fn, vals.data(), base, out1, out2);
+}
+
+int skh_skewfun_z(int64_t n, const double* A, int64_t lda, int fn, double* out1, double* out2, const skh_opts* opts)
+{
+    (void)opts;
+    int rc = fun_args(n, A, lda, fn, out1, out2);
+    if (rc || n == 0) return rc;
+    std::lock_guard<std::mutex> lk(g_mu);
+    std::vector<double> vals(n);
+    double2* dV = nullptr;
+    rc = eigen_z_dev_locked(n, A, lda, fun_scale(fn), vals.data(), &dV);
+    if (rc) return rc;
+    return fun_complex_locked(n, fn, vals.data(), dV, out1, out2);
 }
 
 }  // extern "C"

@@ -55,6 +55,16 @@ int skh_internal_zlarft(const double2* Gm, int64_t ldg, const double2* tau, int 
                         hipStream_t st);
 int skh_internal_zeye(double2* Q, int64_t ldq, int64_t n, hipStream_t st);
 int skh_internal_zphase(const double* C, int64_t n, double2* Zc, hipStream_t st);
+// matrix functions of a skew matrix (csrc/skh_fun.hip): column mixing of eigenvectors, combinations and
+// symmetrizations (kind 0 skewhermitian!, 1 Hermitian upper made dense, 2 Hermitian part), diagonal shift
+int skh_internal_fun_colmix(int64_t n, const double* X, const double* Y, const double* a, const double* b, int mode,
+                            double* T1, double* T2, hipStream_t st);
+int skh_internal_fun_zcolscale(int64_t n, const double* V, const double* d, double* T, hipStream_t st);
+int skh_internal_fun_zsincos(int64_t n, double* P1, double* P2, int mode, hipStream_t st);
+int skh_internal_fun_sympart(int64_t n, double* A, int cplx, int kind, hipStream_t st);
+int skh_internal_fun_adddiag(int64_t n, double* A, int cplx, double v, hipStream_t st);
+int skh_internal_fun_cis_out(int64_t n, const double* re, const double* im, double* out, hipStream_t st);
+int skh_internal_fun_scale(int64_t cnt, double* p, double s, hipStream_t st);
 // host (host_tridiag.cpp): eigenvalues of the symmetric tridiagonal (d, e), ascending into d; 0 or 2
 int skh_internal_symtri_eigvals(int64_t n, double* d, const double* e);
 int skh_internal_compact_occupancy(int nmax, int* blocks_per_cu);

@@ -29,7 +29,8 @@ __all__ = [
     "logabspfaffian", "logabspfaffian_", "logabspfaffian_batched", "logabspfaffian_grad_batched",
     "LogAbsPfaffian", "SkhError", "pfaffian_z_batched", "logabspfaffian_z_batched",
     "JMatrix", "SkewCholesky", "RowMaximum", "NoPivot", "skewchol", "skewchol_", "skewchol_batched", "ldiv", "rdiv",
-    "svd", "svdvals", "SVD", "schur", "Schur", "eigvecs",
+    "svd", "svdvals", "SVD", "schur", "Schur", "eigvecs", "exp", "log", "cis", "cos", "sin", "sincos", "sinh", "cosh", "tan", "tanh", "cot",
+    "coth", "DomainError", "SingularException",
     "ENGINE_AUTO", "ENGINE_LARGE", "ENGINE_TEAM1", "version", "LogPfPlan",
 ]
 
@@ -496,6 +497,130 @@ def schur(A, nb=0, engine=ENGINE_AUTO):
 def eigvecs(A, nb=0, engine=ENGINE_AUTO):
     """eigvecs(A) = eigen(A).vectors (src/tridiag.jl:524)."""
     return eigen(A, nb=nb, engine=engine).vectors
+
+
+# ----------------------------------------------------------------------------------------
+# matrix functions (src/exp.jl:3-296): eigen-decomposition, products and LU solves on the GPU
+# ----------------------------------------------------------------------------------------
+class DomainError(ValueError):
+    """Julia's DomainError: log of a singular skew matrix (src/exp.jl:52, :68)."""
+
+
+class SingularException(ValueError):
+    """LinearAlgebra.SingularException: the divisor in tan / cot / tanh / coth is singular."""
+
+
+_FN = {"exp": 0, "log": 1, "cis": 2, "cos": 3, "sin": 4, "sincos": 5, "sinh": 6, "cosh": 7, "tan": 8,
+       "tanh": 9, "cot": 10, "coth": 11}
+_SKEW_OUT = ("sin", "sinh", "tan", "tanh", "cot", "coth")
+
+
+def _skewfun(A, name, nb=0, engine=ENGINE_AUTO):
+    """f(A) for f = name, A a SkewHermitian, a SkewHermTridiagonal or a skew-Hermitian matrix; the input
+    is copied (copyeigtype, src/exp.jl:48).  Dense results: exp / log a plain matrix, cis / cos / cosh the
+    Hermitian matrix, sin / sinh / tan / tanh / cot / coth a SkewHermitian, sincos the pair."""
+    fn = _FN[name]
+    L = _lib.lib()
+    if isinstance(A, SkewHermTridiagonal) and not np.iscomplexobj(A.ev) and A.dvim is None:
+        ev = np.array(A.ev, dtype=np.float64)
+        n = ev.size + 1
+        cplx = False
+
+        def call(o1, o2):
+            return L.skh_skewfun_tri_d(n, _ptr(ev), fn, o1, o2)
+    else:
+        if isinstance(A, SkewHermTridiagonal):
+            D = A.to_dense()
+        elif isinstance(A, SkewHermitian):
+            D = A.data
+        else:
+            D = np.asarray(A)
+            if not isskewhermitian(D):
+                raise ArgumentError("matrix `A` must be skew-Hermitian (equal `-A'`)")
+        cplx = np.iscomplexobj(D)
+        F = np.array(D, dtype=np.complex128 if cplx else np.float64, order="F", copy=True)
+        n = F.shape[0]
+        o = _lib.opts(nb, engine)
+        entry = L.skh_skewfun_z if cplx else L.skh_skewfun_d
+
+        def call(o1, o2):
+            return entry(n, _ptr(F), max(n, 1), fn, o1, o2, ctypes.byref(o))
+    dt = np.complex128 if (cplx or name == "cis") else np.float64
+    out1 = np.zeros((n, n), dtype=dt, order="F")
+    out2 = np.zeros((n, n), dtype=dt, order="F") if name == "sincos" else None
+    rc = call(_ptr(out1), _ptr(out2) if out2 is not None else None)
+    if rc == _lib.ERR_DOMAIN:
+        raise DomainError("Logarithm of a singular matrix doesn't exist")
+    if rc == _lib.ERR_SINGULAR:
+        raise SingularException(L.skh_last_error().decode(errors="replace"))
+    if rc == _lib.ERR_NOCONV:
+        raise RuntimeError("Maximum number of iterations reached, the algorithm didn't converge")
+    _lib.check("skh_skewfun(%s)" % name, rc)
+    if name == "sincos":
+        return SkewHermitian(out1), out2
+    return SkewHermitian(out1) if name in _SKEW_OUT else out1
+
+
+def exp(A, nb=0, engine=ENGINE_AUTO):
+    """exp(A) (src/exp.jl:3-48): real Q1 = Qr C - Qim S, Q2 = Qr S + Qim C, exp(A) = Q1 Qr^T + Q2 Qim^T with
+    C, S = cos, sin of the eigenvalues' imaginary parts; complex V exp(Lambda) V^H."""
+    return _skewfun(A, "exp", nb, engine)
+
+
+def log(A, nb=0, engine=ENGINE_AUTO):
+    """log(A) (src/exp.jl:50-97): as exp with log|lambda| and sign(lambda) pi/2; a real A of odd size or
+    with a zero eigenvalue raises DomainError."""
+    return _skewfun(A, "log", nb, engine)
+
+
+def cis(A, nb=0, engine=ENGINE_AUTO):
+    """cis(A) = exp(iA) as a Hermitian matrix (src/exp.jl:99-121, :204)."""
+    return _skewfun(A, "cis", nb, engine)
+
+
+def cos(A, nb=0, engine=ENGINE_AUTO):
+    """cos(A), Hermitian (src/exp.jl:123-149, :205)."""
+    return _skewfun(A, "cos", nb, engine)
+
+
+def sin(A, nb=0, engine=ENGINE_AUTO):
+    """sin(A) = skewhermitian!(...) (src/exp.jl:151-202, :206)."""
+    return _skewfun(A, "sin", nb, engine)
+
+
+def sincos(A, nb=0, engine=ENGINE_AUTO):
+    """(sin(A), cos(A)) (src/exp.jl:208-256)."""
+    return _skewfun(A, "sincos", nb, engine)
+
+
+def sinh(A, nb=0, engine=ENGINE_AUTO):
+    """sinh(A) = skewhermitian!(exp(A)) (src/exp.jl:257)."""
+    return _skewfun(A, "sinh", nb, engine)
+
+
+def cosh(A, nb=0, engine=ENGINE_AUTO):
+    """cosh(A) = hermitian!(exp(A)), conjugated for complex A (src/exp.jl:258-263)."""
+    return _skewfun(A, "cosh", nb, engine)
+
+
+def tan(A, nb=0, engine=ENGINE_AUTO):
+    """tan(A) = skewhermitian!(cos(A) \\ sin(A)) (src/exp.jl:265-268)."""
+    return _skewfun(A, "tan", nb, engine)
+
+
+def tanh(A, nb=0, engine=ENGINE_AUTO):
+    """tanh(A) = skewhermitian!((E + I) \\ (E - I)), E = exp(2A) (src/exp.jl:270-273)."""
+    return _skewfun(A, "tanh", nb, engine)
+
+
+def cot(A, nb=0, engine=ENGINE_AUTO):
+    """cot(A) = skewhermitian!(sin(A) \\ cos(A)) (src/exp.jl:275-278)."""
+    return _skewfun(A, "cot", nb, engine)
+
+
+def coth(A, nb=0, engine=ENGINE_AUTO):
+    """coth(A) = skewhermitian!((E - I) \\ (E + I)), E = exp(2A) (src/exp.jl:280-284)."""
+    return _skewfun(A, "coth", nb, engine)
 
 
 # ----------------------------------------------------------------------------------------

@@ -12,6 +12,7 @@ LIB_PATH = os.environ.get("SKEWHIP_LIB") or os.path.join(_HERE, "libskewhip.so")
 ENGINE_AUTO, ENGINE_LARGE, ENGINE_TEAM1 = 0, 1, 2
 FLAG_TIMING = 0x100
 FLAG_PHASES = 0x200
+ERR_NOCONV, ERR_DOMAIN, ERR_SINGULAR = 2, 6, 7
 
 
 class SkhError(RuntimeError):
@@ -53,6 +54,9 @@ def lib():
         "skh_skewtrieigvals_d": ([i64, vp, vp], ctypes.c_int),
         "skh_skewtrieigen_d": ([i64, vp, vp, vp, vp, ctypes.c_int], ctypes.c_int),
         "skh_skewtrieigen_d_gpu": ([i64, vp, vp, vp, vp, ctypes.c_int], ctypes.c_int),
+        "skh_skewfun_d": ([i64, vp, i64, ctypes.c_int, vp, vp, op], ctypes.c_int),
+        "skh_skewfun_tri_d": ([i64, vp, ctypes.c_int, vp, vp], ctypes.c_int),
+        "skh_skewfun_z": ([i64, vp, i64, ctypes.c_int, vp, vp, op], ctypes.c_int),
         "skh_diag_mfma_probe": ([vp], ctypes.c_int),
         "skh_diag_timings": ([vp], ctypes.c_int),
         "skh_diag_phase_clock": ([vp, ctypes.c_int], ctypes.c_int),

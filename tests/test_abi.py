@@ -207,3 +207,28 @@ def test_skewchol_and_zpfaffian_mirror_arguments():
     with pytest.raises(skl.ArgumentError):
         skl.pfaffian(A)
     assert skl.skewchol.__defaults__[0] is skl.RowMaximum   # the reference's default pivoting
+
+
+def test_skewfun_argument_errors_before_device():
+    """Matrix-function entries reject bad arguments, and log of an odd-sized real skew matrix is a
+    DomainError (src/exp.jl:52), before any device work."""
+    L = _lib.lib()
+    o = _lib.opts()
+    A = np.zeros((3, 3), order="F")
+    out = np.zeros((3, 3), order="F")
+    p, q = A.ctypes.data, out.ctypes.data
+    assert L.skh_skewfun_d(-1, None, 1, 0, None, None, ctypes.byref(o)) == -1
+    assert L.skh_skewfun_d(3, None, 3, 0, q, None, ctypes.byref(o)) == -2
+    assert L.skh_skewfun_d(3, p, 2, 0, q, None, ctypes.byref(o)) == -3
+    assert L.skh_skewfun_d(3, p, 3, 12, q, None, ctypes.byref(o)) == -4
+    assert L.skh_skewfun_z(3, p, 3, -1, q, None, ctypes.byref(o)) == -4
+    assert L.skh_skewfun_d(3, p, 3, 0, None, None, ctypes.byref(o)) == -5
+    assert L.skh_skewfun_d(3, p, 3, 5, q, None, ctypes.byref(o)) == -6
+    assert L.skh_skewfun_d(3, p, 3, 1, q, None, ctypes.byref(o)) == _lib.ERR_DOMAIN
+    assert L.skh_skewfun_tri_d(3, p, 1, q, None) == _lib.ERR_DOMAIN
+    assert L.skh_skewfun_d(0, None, 1, 0, None, None, ctypes.byref(o)) == 0
+    with pytest.raises(skl.DomainError):
+        skl.log(skl.SkewHermitian(np.zeros((5, 5))))
+    with pytest.raises(skl.ArgumentError):
+        skl.exp(np.ones((3, 3)))
+    assert skl.exp(np.zeros((0, 0))).shape == (0, 0)

@@ -549,3 +549,79 @@ def test_skewchol_planted_slips():
     assert not _chol_residual(A, R, p) <= 1e-3
     R, p, _ = _skewchol_py(TIE4, slip="row_major_ties")
     assert list(p[:2]) != [2, 0]
+
+
+# ----------------------------------------------------------------------------------------
+# matrix functions (src/exp.jl): the oracle's restatement against scipy's dense matrix functions,
+# as the reference's own exp.jl test compares against Julia's dense exp / cos / sin / ... (test/runtests.jl:208-281)
+# ----------------------------------------------------------------------------------------
+def _dense_funs(A):
+    inv = np.linalg.inv
+    n = A.shape[0]
+    ref = {"exp": sl.expm(A), "cis": sl.expm(1j * A), "cos": sl.cosm(A), "sin": sl.sinm(A),
+           "sinh": sl.sinhm(A), "cosh": sl.coshm(A), "tan": sl.tanm(A), "tanh": sl.tanhm(A)}
+    if np.iscomplexobj(A) or n % 2 == 0:
+        ref["log"] = sl.logm(A)
+        ref["cot"] = sl.cosm(A) @ inv(sl.sinm(A))
+        ref["coth"] = sl.coshm(A) @ inv(sl.sinhm(A))
+    return ref
+
+
+def _rel(X, R):
+    return np.abs(np.asarray(X) - R).max() / max(1.0, np.abs(R).max())
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 10, 11, 32])
+@pytest.mark.parametrize("kind", ["real", "int", "complex", "tri"])
+def test_skewfun_vs_dense_functions(n, kind):
+    rng = np.random.default_rng(40 + n)
+    if kind == "tri":
+        if n == 1:
+            return
+        ev = rng.standard_normal(n - 1) * 0.6
+        A = tri(ev)
+        got = lambda f: oracle.skewfun_tri(ev, f)   # noqa: E731
+    else:
+        if kind == "int":
+            M = rng.integers(-10, 11, size=(n, n)).astype(float) * 2
+            A = (M - M.T) / 2 / (10 * np.sqrt(n))   # a power-of-ten scale keeps cosh, tan well conditioned
+        elif kind == "real":
+            M = rng.standard_normal((n, n))
+            A = (M - M.T) / (2 * np.sqrt(n))
+        else:
+            M = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+            A = (M - M.conj().T) / (2 * np.sqrt(n))
+        got = lambda f: oracle.skewfun(A, f)   # noqa: E731
+    for f, R in _dense_funs(A).items():
+        tol = 1e-12 if f not in ("cot", "coth") else 1e-10
+        assert _rel(got(f), R) <= tol, (kind, n, f, _rel(got(f), R))
+    S, C = got("sincos")
+    assert _rel(S, sl.sinm(A)) <= 1e-12 and _rel(C, sl.cosm(A)) <= 1e-12
+
+
+def test_skewfun_wrappers_and_domain():
+    """skewhermitian! keeps the lower triangle's rounding, Hermitian reads the upper one; log of an
+    odd-sized or singular real skew matrix is a DomainError (src/exp.jl:52, :68)."""
+    X = np.array([[1.0, 2.0], [5.0, 3.0]])
+    assert np.array_equal(oracle._skewhermitian_(X), [[0.0, -1.5], [1.5, 0.0]])
+    assert np.array_equal(oracle._hermitian_view(X), [[1.0, 2.0], [2.0, 3.0]])
+    Z = np.array([[1 + 2j, 3 + 1j], [5 - 4j, 2j]])
+    assert np.array_equal(oracle._hermitian_view(Z), [[1, 3 + 1j], [3 - 1j, 0]])
+    assert np.array_equal(oracle._skewhermitian_(Z), [[2j, -1 - 1.5j], [1 - 1.5j, 2j]])
+    with pytest.raises(ValueError, match="DomainError"):
+        oracle.skewfun(tri(np.ones(2)), "log")
+    with pytest.raises(ValueError, match="DomainError"):
+        oracle.skewfun_tri(np.array([1.0, 0.0, 0.0]), "log")
+
+
+def test_skewfun_planted_slips():
+    """A sign slip in the eigenvalue convention or swapped real / imaginary eigenvector parts fails the
+    dense comparison that pins the restatement."""
+    rng = np.random.default_rng(3)
+    M = rng.standard_normal((8, 8))
+    A = (M - M.T) / 4
+    vals, Qr, Qim = oracle.skeweigen(A)
+    assert _rel(oracle._fun_from_eigs_real(vals, Qr, Qim, "exp", 8), sl.expm(A)) <= 1e-13
+    assert _rel(oracle._fun_from_eigs_real(-vals, Qr, Qim, "exp", 8), sl.expm(A)) > 1e-3
+    assert _rel(oracle._fun_from_eigs_real(vals, Qim, Qr, "exp", 8), sl.expm(A)) > 1e-3
+    assert _rel(oracle._fun_from_eigs_real(vals, Qr, Qim, "cis", 8), sl.expm(-1j * A)) > 1e-3
