@@ -28,7 +28,7 @@ Also reported:
                 one batched call, matrices/s, plus one n = 4096 matrix alone, against the oracle
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 8192] [--nb 0] [--no-cpu] [--no-batched]
-                       [--no-complex] [--only complex_pfaffian|skewchol]
+                       [--no-complex] [--no-matfun] [--only complex_pfaffian|skewchol|matfun]
 """
 import argparse
 import ctypes
@@ -466,6 +466,43 @@ def skewchol_bench(torch, _lib, steps, count=4096, seed=1618, big_n=4096):
                              "sample": "oracle on every 65th matrix (%d matrices, %.2f s)" % (len(sample), cpu_s)}}
 
 
+def matfun_bench(skl, sizes=(1024, 4096), cpu_n=1024, seed=577):
+    """Matrix functions of a real skew matrix (the reference's exp(A) and sincos(A), src/exp.jl:3-48 and
+    :208-256): whole host-API calls, host arrays in and out, A = (M - M^T) / (2 sqrt(n)), M iid N(0, 1).
+    The GPU runs the tridiagonalization, Q, the eigenvector rotations, the products and the wrappers;
+    the host runs the QR sweeps.  The oracle (numpy + C restatement) times exp at n = cpu_n."""
+    import numpy as np
+    import oracle
+    out = {"metric": "exp(A) / sincos(A) of a real skew matrix, whole call (host arrays)", "unit": "ms"}
+    for n in sizes:
+        rng = np.random.default_rng(seed + n)
+        M = rng.standard_normal((n, n))
+        A = np.asfortranarray((M - M.T) / (2 * np.sqrt(n)))
+        del M
+        skl.exp(A[:64, :64] - A[:64, :64].T)   # library and BLAS start-up outside the timing
+        t0 = time.perf_counter()
+        E = skl.exp(A)
+        t_exp = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        S, C = skl.sincos(A)
+        t_sc = time.perf_counter() - t0
+        cols = E[:, :64]
+        orth = float(np.abs(cols.T @ cols - np.eye(64)).max())
+        pyth = float(np.abs(S.data[:64] @ S.data[:, :64] + C[:64] @ C[:, :64] - np.eye(64)).max())
+        out["n%d" % n] = {"exp_ms": t_exp * 1e3, "sincos_ms": t_sc * 1e3, "exp_orthogonality_64cols": orth,
+                          "sin2_plus_cos2_minus_I_64x64": pyth}
+        if n == cpu_n:
+            t0 = time.perf_counter()
+            Eo = oracle.skewfun(A, "exp")
+            cpu_s = time.perf_counter() - t0
+            out["exp_max_abs_diff_vs_oracle_n%d" % n] = float(np.abs(E - Eo).max())
+            out["cpu_baseline"] = {"value": cpu_s * 1e3, "unit": "ms per exp(A) at n=%d" % n, "cores": 16,
+                                   "kind": "port", "sample": "one exp(A) by the oracle at n=%d (OpenBLAS "
+                                   "threads for its products)" % n, "gpu_over_cpu": cpu_s / t_exp}
+        log("matfun n=%d: exp %.1f ms, sincos %.1f ms" % (n, t_exp * 1e3, t_sc * 1e3))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -477,7 +514,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched log|Pf| and gradient lines")
     ap.add_argument("--no-complex", action="store_true", help="skip the complex n=4096 and complex Pfaffian lines")
-    ap.add_argument("--only", choices=["complex_pfaffian", "skewchol"],
+    ap.add_argument("--no-matfun", action="store_true", help="skip the matrix-function (exp, sincos) line")
+    ap.add_argument("--only", choices=["complex_pfaffian", "skewchol", "matfun"],
                     help="run just this secondary line and print its JSON (for profiling it alone)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -489,6 +527,9 @@ def main():
     import skewlinearalgebra as skl
     from skewlinearalgebra import _lib
 
+    if args.only == "matfun":
+        print(json.dumps(matfun_bench(skl)), flush=True)
+        return
     if args.only:
         fn = zpfaff_bench if args.only == "complex_pfaffian" else skewchol_bench
         print(json.dumps(fn(torch, _lib, args.steps)), flush=True)
@@ -643,6 +684,10 @@ def main():
         log("skew-Cholesky (4096 matrices, one n=4096) ...")
         res["skewchol"] = skewchol_bench(torch, _lib, args.steps)
         log(json.dumps(res["skewchol"]))
+    if not args.no_matfun:
+        log("matrix functions (exp, sincos; n = 1024, 4096) ...")
+        res["matfun"] = matfun_bench(skl)
+        log(json.dumps(res["matfun"]))
     if not args.no_cpu:
         log("cpu baseline (n=%d) ..." % args.cpu_n)
         res["cpu_baseline"] = cpu_baseline(args.cpu_n)

@@ -255,12 +255,15 @@ extern "C" int skh_skewtrieigvals_d(int64_t N, double* evp, double* vals)
 typedef int (*skh_rot_apply_fn)(void* ctx, int64_t nrows, const int32_t* ko, const double* cso, int64_t nro,
                                 double* Qo, int64_t no, const int32_t* ke, const double* cse, int64_t nre, double* Qe,
                                 int64_t ne);
+typedef int (*skh_symtri_fn)(void* ctx, int64_t n, const double* e, double* d, double* Z);
 
 // skewtrieigen_divided! / merged!.  With `apply` set, the sweeps' rotations are logged and handed
 // to it once the iteration has converged (it must leave Qo and Qe as the host would have them);
-// getoddvectors and the assembly of vr, vi then run here as usual.
+// getoddvectors and the assembly of vr, vi then run here as usual.  With `symtri` set, the
+// fallback taken when the sweeps stagnate (the reference's eigen!(SymTridiagonal), LAPACK there)
+// calls it instead of the host QL, whose eigenvector accumulation costs O(n^3) scalar work.
 extern "C" int skh_internal_skewtrieigen(int64_t N, double* evp, double* vals, double* vr, double* vi, int variant,
-                                         skh_rot_apply_fn apply, void* ctx)
+                                         skh_rot_apply_fn apply, skh_symtri_fn symtri, void* ctx)
 {
     if (N < 0) return -1;
     if (N > 1 && !evp) return -2;
@@ -310,7 +313,12 @@ extern "C" int skh_internal_skewtrieigen(int64_t N, double* evp, double* vals, d
             if (++count_static > 4) {
                 // symmetric tridiagonal fallback on the original ev (src/skeweigen.jl:470-490)
                 std::vector<double> d(N, 0.0), Z(N * N);
-                if (!symtri_ql(N, d, backup, Z.data())) return 2;
+                if (symtri) {
+                    const int rc = symtri(ctx, N, backup.data(), d.data(), Z.data());
+                    if (rc) return rc;
+                } else if (!symtri_ql(N, d, backup, Z.data())) {
+                    return 2;
+                }
                 std::fill(vr, vr + N * N, 0.0);
                 std::fill(vi, vi + N * N, 0.0);
                 for (int64_t k = 0; k < N; k++) vals[k] = -d[k];
@@ -460,5 +468,5 @@ extern "C" int skh_skewtrieigvals_range_d(int64_t N, const double* ev, int64_t i
 
 extern "C" int skh_skewtrieigen_d(int64_t N, double* evp, double* vals, double* vr, double* vi, int variant)
 {
-    return skh_internal_skewtrieigen(N, evp, vals, vr, vi, variant, nullptr, nullptr);
+    return skh_internal_skewtrieigen(N, evp, vals, vr, vi, variant, nullptr, nullptr, nullptr);
 }

@@ -875,6 +875,7 @@ int zpf_host(int64_t batch, const int64_t* n, const int64_t* offset, const doubl
 // ---- eigenvector rotations of the skew-tridiagonal QR on the GPU ----
 Buf g_rotq, g_rotk, g_rotcs, g_rotseg;
 Buf g_fvec, g_fpiv, g_fz;   // matrix functions: eigenvalue vectors, LU pivots, complex work matrices
+Buf g_stq;                  // symmetric tridiagonal fallback: eigenvectors, diagonal, off-diagonal, info
 constexpr int64_t ROT_GPU_MIN = 256;   // eigen(A) applies the QR rotations on the GPU from this n
 
 std::vector<int64_t> chain_starts(const int32_t* k, int64_t cnt)
@@ -915,6 +916,36 @@ int rot_apply_gpu(void*, int64_t nrows, const int32_t* ko, const double* cso, in
     HIPCHK(hipMemcpyAsync(Qo, dQo, (size_t)no * no * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(Qe, dQe, (size_t)ne * ne * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// skh_symtri_fn: eigenpairs of SymTridiagonal(0, e) by rocSOLVER's divide and conquer on the GPU, the
+// fallback of the eigenvector QR when its sweeps stagnate (the reference calls LAPACK there,
+// src/skeweigen.jl:470-490); eigenvalues ascending, as LAPACK's
+int symtri_gpu(void*, int64_t n, const double* e, double* d, double* Z)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    if (n > 46340) return set_err(SKH_ERR_UNSUPPORTED, "n too large for the solver interface");
+    hipStream_t st = g_stream;
+    double* dC = (double*)g_stq.get((size_t)(n * n + 2 * n + 2) * sizeof(double));
+    if (!dC) return set_err(SKH_ERR_HIP, "hipMalloc failed (tridiagonal fallback)");
+    double* dD = dC + n * n;
+    double* dE = dD + n;
+    int* dinfo = (int*)(dE + n);
+    const std::vector<double> zeros(n, 0.0);
+    HIPCHK(hipMemcpyAsync(dD, zeros.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+    if (n > 1) HIPCHK(hipMemcpyAsync(dE, e, (n - 1) * sizeof(double), hipMemcpyHostToDevice, st));
+    rc = blas_on(st);
+    if (rc) return rc;
+    BLASCHK(rocsolver_dstedc(g_blas, rocblas_evect_tridiagonal, (rocblas_int)n, dD, dE, dC, (rocblas_int)n, dinfo));
+    int info = 0;
+    HIPCHK(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(d, dD, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(Z, dC, (size_t)n * n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (info != 0) return set_err(SKH_ERR_NOCONV, "tridiagonal divide and conquer failed (info %d)", info);
     return 0;
 }
 
@@ -1049,7 +1080,7 @@ int skh_finalize(void)
     for (Buf* b : {&g_mat, &g_ws, &g_vec, &g_ctl, &g_bin, &g_bout, &g_q, &g_ev, &g_gq, &g_gk, &g_gz, &g_gv, &g_gp,
                    &g_gte, &g_cslot, &g_big, &g_bctl, &g_zws, &g_zq, &g_zc, &g_zx, &g_zpslot, &g_zpctl, &g_zpin,
                    &g_zpout, &g_chctl, &g_chin, &g_chout, &g_chp,
-                   &g_chws, &g_rotq, &g_rotk, &g_rotcs, &g_rotseg, &g_fvec, &g_fpiv, &g_fz})
+                   &g_chws, &g_rotq, &g_rotk, &g_rotcs, &g_rotseg, &g_fvec, &g_fpiv, &g_fz, &g_stq})
         b->release();
     if (g_blas) (void)rocblas_destroy_handle(g_blas);
     g_blas = nullptr;
@@ -1263,7 +1294,7 @@ int eigen_d_dev(int64_t n, const double* A, int64_t lda, double scale, double* v
         int rc = skh_sytrd_d(n, F.data(), n, tau.data(), eta.data(), nullptr, opts);
         if (rc) return rc;
         rc = skh_internal_skewtrieigen(n, eta.data(), vals, hr.data(), hi.data(), 0,
-                                       n >= ROT_GPU_MIN ? rot_apply_gpu : nullptr, nullptr);
+                                       n >= ROT_GPU_MIN ? rot_apply_gpu : nullptr, symtri_gpu, nullptr);
         if (rc == 2) return set_err(SKH_ERR_NOCONV, "skewtrieigen did not converge");
         if (rc) return rc < 0 ? set_err(rc, "skewtrieigen failed (%d)", rc) : rc;
     }
@@ -1851,7 +1882,7 @@ int skh_skewtrieigen_d_gpu(int64_t N, double* ev, double* vals, double* vr, doub
     if (N > 1 && !ev) return set_err(-2, "ev NULL");
     if (N > 0 && (!vals || !vr || !vi)) return set_err(-3, "output NULL");
     if (variant != 0 && variant != 1) return set_err(-6, "variant must be 0 or 1");
-    const int rc = skh_internal_skewtrieigen(N, ev, vals, vr, vi, variant, rot_apply_gpu, nullptr);
+    const int rc = skh_internal_skewtrieigen(N, ev, vals, vr, vi, variant, rot_apply_gpu, symtri_gpu, nullptr);
     if (rc == 2) return set_err(SKH_ERR_NOCONV, "tridiagonal QR did not converge");
     return rc;
 }
@@ -2142,7 +2173,7 @@ int skh_skewfun_tri_d(int64_t n, const double* ev, int fn, double* out1, double*
         hr[0] = 1.0;
     } else {
         rc = skh_internal_skewtrieigen(n, e.data(), vals.data(), hr.data(), hi.data(), 1,
-                                       n >= ROT_GPU_MIN ? rot_apply_gpu : nullptr, nullptr);
+                                       n >= ROT_GPU_MIN ? rot_apply_gpu : nullptr, symtri_gpu, nullptr);
         if (rc == 2) return set_err(SKH_ERR_NOCONV, "tridiagonal QR did not converge");
         if (rc) return rc < 0 ? set_err(rc, "skewtrieigen failed (%d)", rc) : rc;
     }

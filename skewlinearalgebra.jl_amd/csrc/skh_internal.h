@@ -82,8 +82,11 @@ int skh_internal_zpfaff_coop_occupancy(int* per_cu);
 typedef int (*skh_rot_apply_fn)(void* ctx, int64_t nrows, const int32_t* ko, const double* cso, int64_t nro,
                                 double* Qo, int64_t no, const int32_t* ke, const double* cse, int64_t nre, double* Qe,
                                 int64_t ne);
+// eigenpairs of SymTridiagonal(0, e) (n - 1 off-diagonal values): d ascending, Z n x n column-major;
+// 0 on success, 2 if the solver did not converge, another status on failure
+typedef int (*skh_symtri_fn)(void* ctx, int64_t n, const double* e, double* d, double* Z);
 int skh_internal_skewtrieigen(int64_t N, double* evp, double* vals, double* vr, double* vi, int variant,
-                              skh_rot_apply_fn apply, void* ctx);
+                              skh_rot_apply_fn apply, skh_symtri_fn symtri, void* ctx);
 // rotation chains on the GPU (skh_rot.hip): seg* hold nseg* + 1 chain starts into the lists
 int skh_internal_rot_chains(double* Qo, int64_t ldo, const int32_t* ko, const double* cso, const int64_t* sego,
                             int64_t nsego, double* Qe, int64_t lde, const int32_t* ke, const double* cse,
