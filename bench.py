@@ -26,6 +26,8 @@ Also reported:
                 plus one n = 4096 matrix alone
   skewchol      4096 real skew matrices (n in {32,...,256}), skew-Cholesky with complete pivoting in
                 one batched call, matrices/s, plus one n = 4096 matrix alone, against the oracle
+  matfun        exp(A) and sincos(A) of a real skew matrix at n = 1024 and 4096, whole host-API
+                calls (ms), against the oracle's exp at n = 1024
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 8192] [--nb 0] [--no-cpu] [--no-batched]
                        [--no-complex] [--no-matfun] [--only complex_pfaffian|skewchol|matfun]

@@ -443,11 +443,9 @@ def _svd_target(A):
 
 def svdvals(A, nb=0, engine=ENGINE_AUTO):
     """svdvals(A): |imag(eigvals(A))| sorted in decreasing order (src/eigen.jl:110-114, real;
-    src/tridiag.jl:526-530); a complex SkewHermitian goes through Hermitian(iA) as in the
-    reference (:116: LAPACK there, numpy on the host here)."""
+    src/tridiag.jl:526-530).  A complex SkewHermitian takes svdvals(Hermitian(iA)) (:116), i.e. the
+    |eigenvalues| of iA, from the GPU eigenvalues of A."""
     A = _svd_target(A)
-    if isinstance(A, SkewHermitian) and np.iscomplexobj(A.data):
-        return np.linalg.svd(1j * np.asarray(A.data), compute_uv=False)
     vals = np.abs(np.imag(eigvals(A, nb=nb, engine=engine)))
     return np.sort(vals)[::-1].copy()
 
@@ -456,11 +454,21 @@ def svd(A, nb=0, engine=ENGINE_AUTO):
     """svd(A) from eigen(A) (src/eigen.jl:119-137, src/tridiag.jl:534-553): U = the eigenvectors
     ordered by decreasing |lambda| (a stable sort, as Julia's sortperm), S = |lambda|, V = -i U
     with the columns of negative lambda negated; returns SVD(U, S, V^H).  A complex SkewHermitian
-    goes through svd(Hermitian(iA)) with Vt scaled by -i (:138-142)."""
+    goes through svd(Hermitian(iA)) with Vt scaled by -i (:138-142); the stdlib's svd of a Hermitian
+    matrix is its eigen-decomposition sorted by decreasing |mu| with negative-mu columns of V negated,
+    and iA has the eigenvalues mu = -imag(lambda) and the eigenvectors of A (GPU eigen here)."""
     A = _svd_target(A)
     if isinstance(A, SkewHermitian) and np.iscomplexobj(A.data):
-        U, S, Vt = np.linalg.svd(1j * np.asarray(A.data))
-        return SVD(U, S, Vt * (-1j))
+        E = eigen(A, nb=nb, engine=engine)
+        mu = -np.imag(E.values)
+        order = np.argsort(-np.abs(mu), kind="stable")
+        mu = mu[order]
+        U = E.vectors[:, order]
+        V = U.copy()
+        neg = mu < 0
+        mu[neg] = -mu[neg]
+        V[:, neg] = -V[:, neg]
+        return SVD(U, mu, V.conj().T * (-1j))
     E = eigen(A, nb=nb, engine=engine)
     vals = np.imag(E.values).copy()
     order = np.argsort(-np.abs(vals), kind="stable")

@@ -43,6 +43,8 @@ def test_svd_complex_and_readme():
     say("svd complex and README")
     S = skl.svd(skl.SkewHermitian(A))
     assert np.linalg.norm(S.U @ np.diag(S.S) @ S.Vt - A) <= 1e-12 * np.linalg.norm(A)
+    np.testing.assert_allclose(S.S, np.linalg.svd(A, compute_uv=False), rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(skl.svdvals(skl.SkewHermitian(A)), np.linalg.svd(A, compute_uv=False), rtol=1e-12)
     # README 4x4 (test/runtests.jl:27): svdvals = [i l1, i l1, i l2, i l2] in modulus
     R = np.array([[0, 2, -7, 4], [-2, 0, -8, 3], [7, 8, 0, 1], [-4, -3, -1, 0]], dtype=float)
     vals = np.sort(np.abs(np.imag(np.linalg.eigvals(R))))[::-1]
@@ -59,3 +61,20 @@ def test_schur_and_eigvecs(n):
     assert np.array_equal(F.Schur, np.diag(np.diag(F.Schur)))
     T = skl.SkewHermTridiagonal(np.random.default_rng(n).standard_normal(n - 1))
     np.testing.assert_array_equal(skl.eigvecs(T), skl.eigen(T).vectors)
+
+
+@pytest.mark.parametrize("n", [5, 200])
+def test_svd_complex_from_gpu_eigen(n):
+    """svd(Hermitian(iA)) of the stdlib (eigen sorted by decreasing |mu|, negative-mu columns of V
+    negated) with Vt scaled by -i (src/eigen.jl:138-142): A = U S Vt, U and V unitary, S sorted, and S
+    equal to the dense singular values."""
+    rng = np.random.default_rng(50 + n)
+    M = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = (M - M.conj().T) / 2
+    say("svd complex n=%d" % n)
+    S = skl.svd(A)
+    assert np.linalg.norm(S.U @ np.diag(S.S) @ S.Vt - A) <= 1e-12 * n * np.linalg.norm(A)
+    assert np.abs(S.U.conj().T @ S.U - np.eye(n)).max() <= 1e-12 * n
+    assert np.abs(S.Vt @ S.Vt.conj().T - np.eye(n)).max() <= 1e-12 * n
+    assert np.all(np.diff(S.S) <= 0) and np.all(S.S >= 0)
+    np.testing.assert_allclose(S.S, np.linalg.svd(A, compute_uv=False), rtol=1e-11, atol=1e-12)
