@@ -35,7 +35,7 @@ extern "C" {
 #define SKH_ERR_SINGULAR 7     /* LU met an exactly zero pivot in tan/cot/tanh/coth (SingularException) */
 
 /* engine selection in skh_opts.flags */
-#define SKH_ENGINE_AUTO 0
+#define SKH_ENGINE_AUTO 0      /* single-matrix host entries: TEAM1 for even n <= 192, LARGE above */
 #define SKH_ENGINE_LARGE 1     /* cooperative multi-workgroup panel + MFMA rank-2k update */
 #define SKH_ENGINE_TEAM1 2     /* one workgroup per matrix (the batched engine) */
 #define SKH_FLAG_TIMING 0x100  /* record per-kernel HIP events (read with skh_diag_timings) */

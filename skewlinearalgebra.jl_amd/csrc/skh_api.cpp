@@ -108,8 +108,11 @@ int64_t pick_nb(int64_t n, const skh_opts* o)
 
 // compact engine limits: even n up to this size (one workgroup per matrix)
 constexpr int64_t CMP_NMAX = 1024;
-// automatic choice for a single matrix: the compact engine below this size
-constexpr int64_t CMP_AUTO = 1024;
+// automatic choice for a single matrix: the compact engine (one workgroup) up to this size, the large
+// engine above.  Measured host-entry calls: n = 192 5.1 ms against 5.6 ms, n = 256 7.9 against 7.5,
+// n = 512 29.6 against 15.0, n = 1024 163 against 30.5 (batches keep the compact engine to n = 1024:
+// there every workgroup has a matrix of its own)
+constexpr int64_t CMP_AUTO = 192;
 
 int engine_of(int64_t n, const skh_opts* o)
 {
