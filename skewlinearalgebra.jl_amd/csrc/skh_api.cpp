@@ -555,6 +555,37 @@ int zorgtr_locked(int64_t n, const double2* dF, int64_t ldf, const double2* dtau
     return 0;
 }
 
+// Column blocks between pageable host memory (leading dimension lda) and the device (leading dimension
+// ldd), in bytes, as one contiguous copy: direct when the leading dimensions agree, else packed or
+// unpacked through a host staging vector.  (hipMemcpy2DAsync from pageable memory delivered wrong data
+// to the kernels here when run under rocprofv3's kernel tracing; contiguous copies did not.)
+// upload_cols leaves the copy in flight: `stage` must outlive it (the caller synchronizes first).
+hipError_t upload_cols(void* d, size_t ldd, const void* h, size_t lda, size_t rows, int64_t ncols,
+                       std::vector<char>& stage, hipStream_t st)
+{
+    if (ncols <= 0) return hipSuccess;
+    if (lda == ldd) return hipMemcpyAsync(d, h, ldd * (ncols - 1) + rows, hipMemcpyHostToDevice, st);
+    stage.assign(ldd * ncols, 0);
+    for (int64_t c = 0; c < ncols; c++) std::memcpy(stage.data() + c * ldd, (const char*)h + c * lda, rows);
+    return hipMemcpyAsync(d, stage.data(), ldd * ncols, hipMemcpyHostToDevice, st);
+}
+
+// the reverse; complete on return (it synchronizes the stream)
+hipError_t download_cols(void* h, size_t lda, const void* d, size_t ldd, size_t rows, int64_t ncols, hipStream_t st)
+{
+    if (ncols <= 0) return hipSuccess;
+    if (lda == ldd) {
+        const hipError_t e = hipMemcpyAsync(h, d, ldd * (ncols - 1) + rows, hipMemcpyDeviceToHost, st);
+        return e != hipSuccess ? e : hipStreamSynchronize(st);
+    }
+    std::vector<char> stage(ldd * ncols);
+    hipError_t e = hipMemcpyAsync(stage.data(), d, ldd * ncols, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    for (int64_t c = 0; c < ncols; c++) std::memcpy((char*)h + c * lda, stage.data() + c * ldd, rows);
+    return hipSuccess;
+}
+
 // host copy-in of an n x n complex matrix (interleaved pairs, lda) into a device buffer with ld n
 double2* zupload(int64_t n, const double* A, int64_t lda, hipStream_t st, int* rc)
 {
@@ -563,8 +594,9 @@ double2* zupload(int64_t n, const double* A, int64_t lda, hipStream_t st, int* r
         *rc = set_err(SKH_ERR_HIP, "hipMalloc failed for n = %lld", (long long)n);
         return nullptr;
     }
-    hipError_t e = hipMemcpy2DAsync(dA, n * sizeof(double2), A, lda * sizeof(double2), n * sizeof(double2), n,
-                                    hipMemcpyHostToDevice, st);
+    std::vector<char> stage;
+    hipError_t e = upload_cols(dA, n * sizeof(double2), A, lda * sizeof(double2), n * sizeof(double2), n, stage, st);
+    if (e == hipSuccess && !stage.empty()) e = hipStreamSynchronize(st);   // stage ends here
     if (e != hipSuccess) {
         *rc = set_err(SKH_ERR_HIP, "upload failed: %s", hipGetErrorString(e));
         return nullptr;
@@ -1132,8 +1164,9 @@ int skh_sytrd_d(int64_t n, double* A, int64_t lda, double* tau, double* eta, int
     if (!dA || !dv) return set_err(SKH_ERR_HIP, "hipMalloc failed for n = %lld", (long long)n);
     double* dtau = dv;
     double* deta = dv + n;
-    HIPCHK(hipMemcpy2DAsync(dA, ld * sizeof(double), A, lda * sizeof(double), n * sizeof(double), n,
-                            hipMemcpyHostToDevice, st));
+    std::vector<char> stage;
+    HIPCHK(upload_cols(dA, ld * sizeof(double), A, lda * sizeof(double), n * sizeof(double), n, stage, st));
+    if (!stage.empty()) HIPCHK(hipStreamSynchronize(st));   // every return path below outlives the copy
     if (engine_of(n, opts) == SKH_ENGINE_LARGE) {
         g_timer.reset();
         g_timer.on = false;
@@ -1153,8 +1186,7 @@ int skh_sytrd_d(int64_t n, double* A, int64_t lda, double* tau, double* eta, int
         rc = batched_run(1, &nn, &off, dA, dout, dout + 1, opts, st, dA, dv, &toff);
         if (rc) return rc;
     }
-    HIPCHK(hipMemcpy2DAsync(A, lda * sizeof(double), dA, ld * sizeof(double), n * sizeof(double), n,
-                            hipMemcpyDeviceToHost, st));
+    HIPCHK(download_cols(A, lda * sizeof(double), dA, ld * sizeof(double), n * sizeof(double), n, st));
     HIPCHK(hipMemcpyAsync(tau, dtau, (n - 1) * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(eta, deta, (n - 1) * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -1333,6 +1365,8 @@ int eigen_d_dev(int64_t n, const double* A, int64_t lda, double scale, double* v
                           (int)n, dR, (int)n, &zero, d + EV_R * nn, (int)n));
     BLASCHK(rocblas_dgemm(g_blas, rocblas_operation_none, rocblas_operation_none, (int)n, (int)n, (int)n, &one, dQ,
                           (int)n, dI, (int)n, &zero, d + EV_I * nn, (int)n));
+    // the uploads above read F, tau, hr and hi, which end with this function
+    HIPCHK(hipStreamSynchronize(st));
     return 0;
 }
 }  // namespace
@@ -1666,8 +1700,7 @@ int skh_sytrd_z(int64_t n, double* A, int64_t lda, double* tau, double* eta, dou
     HIPCHK(hipStreamSynchronize(st));
     rc = check_barrier_flag();
     if (rc) return rc;
-    HIPCHK(hipMemcpy2DAsync(A, lda * sizeof(double2), dA, n * sizeof(double2), n * sizeof(double2), n,
-                            hipMemcpyDeviceToHost, st));
+    HIPCHK(download_cols(A, lda * sizeof(double2), dA, n * sizeof(double2), n * sizeof(double2), n, st));
     if (n > 1) {
         HIPCHK(hipMemcpyAsync(tau, dtau, (n - 1) * sizeof(double2), hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(eta, deta, (n - 1) * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -2035,6 +2068,7 @@ int fun_real_locked(int64_t n, int fn, const double* vals, double* base, double*
     if (!dv) return set_err(SKH_ERR_HIP, "hipMalloc failed (eigenvalue vectors)");
     HIPCHK(hipMemcpyAsync(dv, a.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dv + n, b.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));   // a and b end with this function, on every return path
     HIPCHK((hipError_t)skh_internal_fun_colmix(n, Vr, Vi, dv, dv + n, ex ? 0 : 1, T1, T2, st));
     int rc = blas_on(st);
     if (rc) return rc;
@@ -2098,6 +2132,7 @@ int fun_complex_locked(int64_t n, int fn, const double* vals, const double2* V, 
     }
     HIPCHK(hipMemcpyAsync(dv, d1.data(), n * sizeof(double2), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dv + n, d2.data(), n * sizeof(double2), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));   // d1 and d2 end with this function, on every return path
     int rc = blas_on(st);
     if (rc) return rc;
     const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
