@@ -252,6 +252,14 @@ int skh_skewtrieigen_d(int64_t n, double *ev, double *vals, double *vr, double *
  * arrays in and out; results bit-identical to skh_skewtrieigen_d.  skh_eigen_d takes this path
  * from n = 256. */
 int skh_skewtrieigen_d_gpu(int64_t n, double *ev, double *vals, double *vr, double *vi, int variant);
+/* Complex SkewHermTridiagonal(ev, dvim) (ev: n - 1 interleaved complex values, dvim: n reals or NULL):
+ * to_symtridiagonal (src/tridiag.jl:558-583) makes i A = Q S Q^H with S real symmetric tridiagonal.
+ * skh_skewtrieigvals_z: mu (n) = the eigenvalues of S ascending, by a host QL (eigvals!, src/tridiag.jl:
+ * 462-468: the eigenvalues of A are -i mu).  skh_skewtrieigen_z: the same mu by rocSOLVER's divide and
+ * conquer on the GPU, and V (n x n interleaved complex) = Diagonal(Q) Z, Z the eigenvectors of S
+ * (eigen!, src/tridiag.jl:508-515: Eigen(-i mu, V)). */
+int skh_skewtrieigvals_z(int64_t n, const double *ev, const double *dvim, double *mu);
+int skh_skewtrieigen_z(int64_t n, const double *ev, const double *dvim, double *mu, double *V);
 
 /* ---- matrix functions (src/exp.jl:3-296) -------------------------------------------------------
  * f(A) of a skew matrix from its eigen-decomposition, with the eigenvectors kept on the GPU: their

@@ -429,6 +429,39 @@ extern "C" int skh_internal_skewtrieigen(int64_t N, double* evp, double* vals, d
     return 0;
 }
 
+// to_symtridiagonal (src/tridiag.jl:558-583) of the complex SkewHermTridiagonal(ev, dvim): V = i ev, made
+// real by diagonal phases Q (Q[0] = 1, Q[i+1] = V[i] / |V[i]|, V[i] = |V[i]|, V[i+1] *= Q[i+1]), so that
+// i A = Q S Q^H with S = SymTridiagonal(-dvim (0 without dvim), real(V)).  ev: n - 1 interleaved complex
+// values; d, e: n and n - 1 reals out; q: n interleaved complex values out.
+extern "C" void skh_internal_to_symtri_z(int64_t n, const double* ev, const double* dvim, double* d, double* e,
+                                         double* q)
+{
+    for (int64_t i = 0; i < n; i++) d[i] = dvim ? -dvim[i] : 0.0;
+    if (n <= 0) return;
+    q[0] = 1.0;
+    q[1] = 0.0;
+    if (n == 1) return;
+    std::vector<double> vr(n - 1), vi(n - 1);
+    for (int64_t k = 0; k + 1 < n; k++) {   // V .*= 1im
+        vr[k] = -ev[2 * k + 1];
+        vi[k] = ev[2 * k];
+    }
+    for (int64_t i = 0; i + 1 < n; i++) {
+        const double nm = std::hypot(vr[i], vi[i]);
+        const double qr = vr[i] / nm, qi = vi[i] / nm;
+        q[2 * (i + 1)] = qr;
+        q[2 * (i + 1) + 1] = qi;
+        vr[i] = nm;
+        vi[i] = 0.0;
+        if (i + 2 < n) {   // V[i+1] *= Q[i+1]
+            const double a = vr[i + 1], b = vi[i + 1];
+            vr[i + 1] = a * qr - b * qi;
+            vi[i + 1] = a * qi + b * qr;
+        }
+    }
+    for (int64_t k = 0; k + 1 < n; k++) e[k] = vr[k];
+}
+
 extern "C" int skh_internal_symtri_eigvals(int64_t n, double* d, const double* e)
 {
     if (n <= 0) return 0;

@@ -957,7 +957,11 @@ int rot_apply_gpu(void*, int64_t nrows, const int32_t* ko, const double* cso, in
 // skh_symtri_fn: eigenpairs of SymTridiagonal(0, e) by rocSOLVER's divide and conquer on the GPU, the
 // fallback of the eigenvector QR when its sweeps stagnate (the reference calls LAPACK there,
 // src/skeweigen.jl:470-490); eigenvalues ascending, as LAPACK's
-int symtri_gpu(void*, int64_t n, const double* e, double* d, double* Z)
+int symtri_gpu_diag(int64_t n, const double* diag, const double* e, double* d, double* Z);
+int symtri_gpu(void*, int64_t n, const double* e, double* d, double* Z) { return symtri_gpu_diag(n, nullptr, e, d, Z); }
+
+// eigenpairs of SymTridiagonal(diag (0 when NULL), e) on the GPU: d ascending, Z n x n
+int symtri_gpu_diag(int64_t n, const double* diag, const double* e, double* d, double* Z)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     int rc = ensure_init_locked(g_device);
@@ -969,8 +973,8 @@ int symtri_gpu(void*, int64_t n, const double* e, double* d, double* Z)
     double* dD = dC + n * n;
     double* dE = dD + n;
     int* dinfo = (int*)(dE + n);
-    const std::vector<double> zeros(n, 0.0);
-    HIPCHK(hipMemcpyAsync(dD, zeros.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+    const std::vector<double> zeros(diag ? 0 : n, 0.0);
+    HIPCHK(hipMemcpyAsync(dD, diag ? diag : zeros.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
     if (n > 1) HIPCHK(hipMemcpyAsync(dE, e, (n - 1) * sizeof(double), hipMemcpyHostToDevice, st));
     rc = blas_on(st);
     if (rc) return rc;
@@ -2236,6 +2240,39 @@ int skh_skewfun_z(int64_t n, const double* A, int64_t lda, int fn, double* out1,
     rc = eigen_z_dev_locked(n, A, lda, fun_scale(fn), vals.data(), &dV);
     if (rc) return rc;
     return fun_complex_locked(n, fn, vals.data(), dV, out1, out2);
+}
+
+int skh_skewtrieigvals_z(int64_t n, const double* ev, const double* dvim, double* mu)
+{
+    if (n < 0) return set_err(-1, "n < 0");
+    if (n > 1 && !ev) return set_err(-2, "ev is NULL");
+    if (n > 0 && !mu) return set_err(-4, "mu is NULL");
+    if (n == 0) return 0;
+    std::vector<double> e(std::max<int64_t>(n - 1, 1)), q(2 * n);
+    skh_internal_to_symtri_z(n, ev, dvim, mu, e.data(), q.data());
+    const int rc = skh_internal_symtri_eigvals(n, mu, e.data());
+    if (rc == 2) return set_err(SKH_ERR_NOCONV, "symmetric tridiagonal QL did not converge");
+    return rc;
+}
+
+int skh_skewtrieigen_z(int64_t n, const double* ev, const double* dvim, double* mu, double* V)
+{
+    if (n < 0) return set_err(-1, "n < 0");
+    if (n > 1 && !ev) return set_err(-2, "ev is NULL");
+    if (n > 0 && (!mu || !V)) return set_err(-4, "output is NULL");
+    if (n == 0) return 0;
+    std::vector<double> d(n), e(std::max<int64_t>(n - 1, 1)), q(2 * n), Z((size_t)n * n);
+    skh_internal_to_symtri_z(n, ev, dvim, d.data(), e.data(), q.data());
+    const int rc = symtri_gpu_diag(n, d.data(), e.data(), mu, Z.data());
+    if (rc) return rc;
+    // Vec = Diagonal(Q) * vectors (src/tridiag.jl:511-513)
+    for (int64_t j = 0; j < n; j++)
+        for (int64_t i = 0; i < n; i++) {
+            const double z = Z[i + j * n];
+            V[2 * (i + j * n)] = q[2 * i] * z;
+            V[2 * (i + j * n) + 1] = q[2 * i + 1] * z;
+        }
+    return 0;
 }
 
 }  // extern "C"

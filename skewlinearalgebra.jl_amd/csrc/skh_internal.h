@@ -87,6 +87,9 @@ typedef int (*skh_rot_apply_fn)(void* ctx, int64_t nrows, const int32_t* ko, con
 typedef int (*skh_symtri_fn)(void* ctx, int64_t n, const double* e, double* d, double* Z);
 int skh_internal_skewtrieigen(int64_t N, double* evp, double* vals, double* vr, double* vi, int variant,
                               skh_rot_apply_fn apply, skh_symtri_fn symtri, void* ctx);
+// to_symtridiagonal of a complex SkewHermTridiagonal and the eigenvalues of a symmetric tridiagonal (host)
+void skh_internal_to_symtri_z(int64_t n, const double* ev, const double* dvim, double* d, double* e, double* q);
+int skh_internal_symtri_eigvals(int64_t n, double* d, const double* e);
 // rotation chains on the GPU (skh_rot.hip): seg* hold nseg* + 1 chain starts into the lists
 int skh_internal_rot_chains(double* Qo, int64_t ldo, const int32_t* ko, const double* cso, const int64_t* sego,
                             int64_t nsego, double* Qe, int64_t lde, const int32_t* ke, const double* cse,

@@ -252,6 +252,30 @@ def _tri_eigvals_host(ev):
     return vals
 
 
+def _tri_is_complex(A):
+    return isinstance(A, SkewHermTridiagonal) and (np.iscomplexobj(A.ev) or A.dvim is not None)
+
+
+def _tri_z(A, vectors):
+    """mu (ascending eigenvalues of to_symtridiagonal(A)[1]) and, with vectors, V = Diagonal(Q) Z for a
+    complex SkewHermTridiagonal (src/tridiag.jl:462-468, :508-515, :558-583)."""
+    ev = np.ascontiguousarray(A.ev, dtype=np.complex128)
+    n = ev.size + 1
+    dv = None if A.dvim is None else np.ascontiguousarray(A.dvim, dtype=np.float64)
+    mu = np.zeros(n)
+    L = _lib.lib()
+    if not vectors:
+        rc = L.skh_skewtrieigvals_z(n, _ptr(ev), _ptr(dv) if dv is not None else None, _ptr(mu))
+        V = None
+    else:
+        V = np.zeros((n, n), dtype=np.complex128, order="F")
+        rc = L.skh_skewtrieigen_z(n, _ptr(ev), _ptr(dv) if dv is not None else None, _ptr(mu), _ptr(V))
+    if rc == 2:
+        raise RuntimeError("Maximum number of iterations reached, the algorithm didn't converge")
+    _lib.check("skh_skewtrieigen_z" if vectors else "skh_skewtrieigvals_z", rc)
+    return mu, V
+
+
 def _range_args(irange, vl, vh):
     """Julia's eigvals!(A, irange::UnitRange) / eigvals!(A, vl, vh): irange = (il, iu), 1-based, inclusive."""
     if irange is not None and (vl is not None or vh is not None):
@@ -273,9 +297,16 @@ def eigvals_(A, sortby=None, nb=0, engine=ENGINE_AUTO, irange=None, vl=None, vh=
     src/tridiag.jl:452-460,525-537)."""
     if irange is not None or vl is not None or vh is not None:
         il, iu, lo, hi, by_value = _range_args(irange, vl, vh)
+        if _tri_is_complex(A):
+            # src/tridiag.jl:477-481: the eigenvalues mu of to_symtridiagonal(A)[1] in (-vh, -vl], returned
+            # as complex(0, mu) (not negated, as the reference does).  The irange form (:470-475) passes a
+            # decreasing range to LAPACK and cannot run; it is not offered here.
+            if not by_value:
+                raise NotImplementedError("eigvals(::SkewHermTridiagonal{<:Complex}, irange) has no working "
+                                          "reference form; use (vl, vh)")
+            mu, _ = _tri_z(A, False)
+            return mu[(mu > -hi) & (mu <= -lo)] * 1j
         if isinstance(A, SkewHermTridiagonal):
-            if np.iscomplexobj(A.ev) or A.dvim is not None:
-                raise NotImplementedError("complex SkewHermTridiagonal is not supported by this build")
             ev = np.array(A.ev, dtype=np.float64, copy=True)
             n = ev.size + 1
             vals = np.zeros(n)
@@ -307,9 +338,13 @@ def eigvals_(A, sortby=None, nb=0, engine=ENGINE_AUTO, irange=None, vl=None, vh=
             raise RuntimeError("Maximum number of iterations reached, the algorithm didn't converge")
         _lib.check("skh_eigvals_range_d", rc)
         return vals[:m.value] * 1j
+    if _tri_is_complex(A):
+        # src/tridiag.jl:462-468: mu ascending, sorted by sortby, reversed, negated
+        mu, _ = _tri_z(A, False)
+        if sortby is not None:
+            mu = np.array(sorted(mu, key=sortby))
+        return -mu[::-1] * 1j
     if isinstance(A, SkewHermTridiagonal):
-        if np.iscomplexobj(A.ev) or A.dvim is not None:
-            raise NotImplementedError("complex SkewHermTridiagonal is not supported by this build")
         vals = _tri_eigvals_host(A.ev)
     else:
         S = A if isinstance(A, SkewHermitian) else SkewHermitian(A)
@@ -377,9 +412,11 @@ def eigen_(A, nb=0, engine=ENGINE_AUTO):
     """eigen!(A): SkewHermitian{<:Real} -> GPU tridiagonalization + Q + host skewtrieigen_divided!
     + GPU back-transform (src/eigen.jl:75-97); SkewHermTridiagonal{<:Real} -> skewtrieigen_merged!
     on the host (src/tridiag.jl:506)."""
+    if _tri_is_complex(A):
+        # src/tridiag.jl:508-515: Eigen(mu .* -im, Diagonal(Q) Z), the eigenpairs of the symmetric part on the GPU
+        mu, V = _tri_z(A, True)
+        return Eigen(-1j * mu, V)
     if isinstance(A, SkewHermTridiagonal):
-        if np.iscomplexobj(A.ev) or A.dvim is not None:
-            raise NotImplementedError("complex SkewHermTridiagonal is not supported by this build")
         vals, vr, vi = _tri_eigen_host(A.ev, 1)
         return Eigen(1j * vals, vr + 1j * vi)
     D = A.data if isinstance(A, SkewHermitian) else np.asarray(A)

@@ -55,6 +55,8 @@ def lib():
         "skh_skewtrieigen_d": ([i64, vp, vp, vp, vp, ctypes.c_int], ctypes.c_int),
         "skh_skewtrieigen_d_gpu": ([i64, vp, vp, vp, vp, ctypes.c_int], ctypes.c_int),
         "skh_skewfun_d": ([i64, vp, i64, ctypes.c_int, vp, vp, op], ctypes.c_int),
+        "skh_skewtrieigvals_z": ([i64, vp, vp, vp], ctypes.c_int),
+        "skh_skewtrieigen_z": ([i64, vp, vp, vp, vp], ctypes.c_int),
         "skh_skewfun_tri_d": ([i64, vp, ctypes.c_int, vp, vp], ctypes.c_int),
         "skh_skewfun_z": ([i64, vp, i64, ctypes.c_int, vp, vp, op], ctypes.c_int),
         "skh_diag_mfma_probe": ([vp], ctypes.c_int),

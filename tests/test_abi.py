@@ -232,3 +232,20 @@ def test_skewfun_argument_errors_before_device():
     with pytest.raises(skl.ArgumentError):
         skl.exp(np.ones((3, 3)))
     assert skl.exp(np.zeros((0, 0))).shape == (0, 0)
+
+
+def test_complex_tridiagonal_entries_before_device():
+    """skh_skewtrieigvals_z runs on the host: argument errors, and the eigenvalues of a 2x2 complex
+    SkewHermTridiagonal([a], [d1, d2]) (S = [[-d1, |a|], [|a|, -d2]]) against the closed form."""
+    L = _lib.lib()
+    mu = np.zeros(2)
+    assert L.skh_skewtrieigvals_z(-1, None, None, mu.ctypes.data) == -1
+    assert L.skh_skewtrieigvals_z(2, None, None, mu.ctypes.data) == -2
+    assert L.skh_skewtrieigen_z(2, mu.ctypes.data, None, None, None) == -4
+    ev = np.array([3.0 + 4.0j])
+    dv = np.array([1.0, -2.0])
+    assert L.skh_skewtrieigvals_z(2, ev.ctypes.data, dv.ctypes.data, mu.ctypes.data) == 0
+    S = np.array([[-1.0, 5.0], [5.0, 2.0]])
+    np.testing.assert_allclose(mu, np.linalg.eigvalsh(S), rtol=1e-15)
+    vals = skl.eigvals(skl.SkewHermTridiagonal(ev, dv))
+    np.testing.assert_allclose(np.imag(vals), -np.linalg.eigvalsh(S)[::-1], rtol=1e-15)
