@@ -176,18 +176,6 @@ class Hessenberg:
 # ----------------------------------------------------------------------------------------
 # tridiagonalization
 # ----------------------------------------------------------------------------------------
-def _as_f64_skew(A, check):
-    if isinstance(A, SkewHermitian):
-        D = A.data
-    else:
-        D = np.asarray(A)
-        if check and not isskewhermitian(D):
-            raise ArgumentError("Pfaffian requires a skew-Hermitian matrix")
-    if np.iscomplexobj(D):
-        raise NotImplementedError("complex SkewHermitian input is not supported by this build")
-    return D
-
-
 def _sytrd_inplace(F, nb=0, engine=ENGINE_AUTO):
     """F: float64 Fortran-ordered n x n, overwritten by the factors.  Returns (tau, eta, ntau)."""
     n = F.shape[0]
