@@ -69,25 +69,27 @@ __global__ void k_zform_v(const double2* F, int64_t ldf, int64_t n, int64_t c0, 
 }
 
 // T (nbp x nbp upper) from Gm = V^H V and tau: T[k,k] = tau_k, T[0:k,k] = -tau_k T[0:k,0:k] Gm[0:k,k]
-__global__ void k_zlarft(const double2* Gm, int64_t ldg, const double2* tau, int nbp, double2* T, int64_t ldt)
+// complex T of the compact WY form (as k_larft): thread i builds row i of T in LDS, no barrier; nbp <= 64
+__global__ __launch_bounds__(64) void k_zlarft(const double2* Gm, int64_t ldg, const double2* tau, int nbp, double2* T,
+                                               int64_t ldt)
 {
     using namespace zc;
+    __shared__ double2 row[64 * 64];
     const int i = threadIdx.x;
+    if (i >= nbp) return;
     for (int k = 0; k < nbp; k++) {
-        if (i < nbp) {
-            cd v = cd{0.0, 0.0};
-            const cd tk = ld(tau + k);
-            if (i < k) {
-                cd acc = cd{0.0, 0.0};
-                for (int l = i; l < k; l++) acc = acc + ld(T + i + l * ldt) * ld(Gm + l + k * ldg);
-                v = cd{0.0, 0.0} - tk * acc;
-            } else if (i == k) {
-                v = tk;
-            }
-            st(T + i + k * ldt, v);
+        cd v = cd{0.0, 0.0};
+        const cd tk = ld(tau + k);
+        if (i < k) {
+            cd acc = cd{0.0, 0.0};
+            for (int l = i; l < k; l++) acc = acc + ld(row + l * 64 + i) * ld(Gm + l + k * ldg);
+            v = cd{0.0, 0.0} - tk * acc;
+        } else if (i == k) {
+            v = tk;
         }
-        __syncthreads();
+        st(row + k * 64 + i, v);
     }
+    for (int k = 0; k < nbp; k++) T[i + k * ldt] = row[k * 64 + i];
 }
 
 __global__ void k_zeye(double2* Q, int64_t ldq, int64_t n)
@@ -159,7 +161,8 @@ extern "C" int skh_internal_zform_v(const double2* F, int64_t ldf, int64_t n, in
 extern "C" int skh_internal_zlarft(const double2* Gm, int64_t ldg, const double2* tau, int nbp, double2* T, int64_t ldt,
                                    hipStream_t st)
 {
-    hipLaunchKernelGGL(k_zlarft, dim3(1), dim3(128), 0, st, Gm, ldg, tau, nbp, T, ldt);
+    if (nbp > 64) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_zlarft, dim3(1), dim3(64), 0, st, Gm, ldg, tau, nbp, T, ldt);
     return (int)hipGetLastError();
 }
 

@@ -146,23 +146,37 @@ __global__ void k_form_v(const double* F, int64_t ldf, int64_t n, int64_t c0, in
 
 // T (nbp x nbp, upper) from the Gram matrix Gm = V^T V and tau:
 // T[k,k] = tau_k, T[0:k,k] = -tau_k T[0:k,0:k] Gm[0:k,k]   (one workgroup)
-__global__ void k_larft(const double* Gm, int64_t ldg, const double* tau, int nbp, double* T, int64_t ldt)
+// T (nbp x nbp, upper triangular, ldt) of the compact WY form I - V T V^T from Gm = V^T V and tau (forward,
+// columnwise): T[k, k] = tau[k], T[i, k] = -tau[k] sum_{l=i}^{k-1} T[i, l] Gm[l, k] for i < k, 0 below.
+// Entry (i, k) needs only row i of T and column k of Gm, so thread i builds row i alone: the row stays in
+// LDS (column-major by thread, no bank conflicts), no barrier is needed, and the Gm loads of one step are
+// independent of each other (coalesced across threads).  nbp <= LARFT_MAX.
+constexpr int LARFT_MAX = 64;
+
+__device__ __forceinline__ void larft_rows(const double* __restrict__ g, int64_t ldg, const double* __restrict__ tau,
+                                           int nbp, double* __restrict__ t, int64_t ldt, double* row)
 {
     const int i = threadIdx.x;
+    if (i >= nbp) return;
     for (int k = 0; k < nbp; k++) {
-        if (i < nbp) {
-            double v = 0.0;
-            if (i < k) {
-                double acc = 0.0;
-                for (int l = i; l < k; l++) acc += T[i + l * ldt] * Gm[l + k * ldg];
-                v = -tau[k] * acc;
-            } else if (i == k) {
-                v = tau[k];
-            }
-            T[i + k * ldt] = v;
+        double v = 0.0;
+        if (i < k) {
+            double acc = 0.0;
+            for (int l = i; l < k; l++) acc += row[l * LARFT_MAX + i] * g[l + k * ldg];
+            v = -tau[k] * acc;
+        } else if (i == k) {
+            v = tau[k];
         }
-        __syncthreads();
+        row[k * LARFT_MAX + i] = v;
     }
+    for (int k = 0; k < nbp; k++) t[i + k * ldt] = row[k * LARFT_MAX + i];
+}
+
+__global__ __launch_bounds__(LARFT_MAX) void k_larft(const double* Gm, int64_t ldg, const double* tau, int nbp, double* T,
+                                                     int64_t ldt)
+{
+    __shared__ double row[LARFT_MAX * LARFT_MAX];
+    larft_rows(Gm, ldg, tau, nbp, T, ldt, row);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -193,28 +207,13 @@ __global__ void k_form_v_batched(const double* const* F, int64_t n, int64_t c0, 
     }
 }
 
-__global__ void k_larft_batched(const double* Gm, int64_t gstride, const double* taueta, const int64_t* toffs,
-                                int64_t c0, int nbp, double* T, int64_t tstride)
+__global__ __launch_bounds__(LARFT_MAX) void k_larft_batched(const double* Gm, int64_t gstride, const double* taueta,
+                                                             const int64_t* toffs, int64_t c0, int nbp, double* T,
+                                                             int64_t tstride)
 {
+    __shared__ double row[LARFT_MAX * LARFT_MAX];
     const int z = blockIdx.x;
-    const double* g = Gm + (int64_t)z * gstride;
-    const double* tau = taueta + toffs[z] + c0;
-    double* t = T + (int64_t)z * tstride;
-    const int i = threadIdx.x;
-    for (int k = 0; k < nbp; k++) {
-        if (i < nbp) {
-            double v = 0.0;
-            if (i < k) {
-                double acc = 0.0;
-                for (int l = i; l < k; l++) acc += t[i + l * nbp] * g[l + k * nbp];
-                v = -tau[k] * acc;
-            } else if (i == k) {
-                v = tau[k];
-            }
-            t[i + k * nbp] = v;
-        }
-        __syncthreads();
-    }
+    larft_rows(Gm + (int64_t)z * gstride, nbp, taueta + toffs[z] + c0, nbp, T + (int64_t)z * tstride, nbp, row);
 }
 
 // Zt (n x n/2) = (C^{-1} Q_O^T)^T: for each row c of Q, forward substitution along a.
@@ -370,7 +369,8 @@ extern "C" int skh_internal_form_v(const double* F, int64_t ldf, int64_t n, int6
 extern "C" int skh_internal_larft(const double* Gm, int64_t ldg, const double* tau, int nbp, double* T, int64_t ldt,
                                   hipStream_t st)
 {
-    hipLaunchKernelGGL(k_larft, dim3(1), dim3(128), 0, st, Gm, ldg, tau, nbp, T, ldt);
+    if (nbp > LARFT_MAX) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_larft, dim3(1), dim3(LARFT_MAX), 0, st, Gm, ldg, tau, nbp, T, ldt);
     return (int)hipGetLastError();
 }
 
@@ -394,8 +394,9 @@ extern "C" int skh_internal_form_v_batched(const double* const* F, const int64_t
 extern "C" int skh_internal_larft_batched(const double* Gm, int64_t gstride, const double* taueta, const int64_t* toffs,
                                           int64_t c0, int nbp, double* T, int64_t tstride, int count, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_larft_batched, dim3((unsigned)count), dim3(128), 0, st, Gm, gstride, taueta, toffs, c0, nbp, T,
-                       tstride);
+    if (nbp > LARFT_MAX) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_larft_batched, dim3((unsigned)count), dim3(LARFT_MAX), 0, st, Gm, gstride, taueta, toffs, c0,
+                       nbp, T, tstride);
     return (int)hipGetLastError();
 }
 
