@@ -590,8 +590,16 @@ def _skewfun(A, name, nb=0, engine=ENGINE_AUTO):
         raise RuntimeError("Maximum number of iterations reached, the algorithm didn't converge")
     _lib.check("skh_skewfun(%s)" % name, rc)
     if name == "sincos":
-        return SkewHermitian(out1), out2
-    return SkewHermitian(out1) if name in _SKEW_OUT else out1
+        return _skew_view(out1), out2
+    return _skew_view(out1) if name in _SKEW_OUT else out1
+
+
+def _skew_view(D):
+    """SkewHermitian over D without the O(n^2) check, for results the library made exactly skew-Hermitian
+    (skewhermitian! on the GPU: D[c, r] = -conj(D[r, c]) bit for bit)."""
+    S = SkewHermitian.__new__(SkewHermitian)
+    S.data = D
+    return S
 
 
 def exp(A, nb=0, engine=ENGINE_AUTO):
