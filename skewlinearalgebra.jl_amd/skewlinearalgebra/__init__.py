@@ -396,6 +396,14 @@ def _tri_eigen_host(ev, variant, gpu=None):
     return vals, vr, vi
 
 
+def _cplx(re, im):
+    """re + i im as one complex array, filled in place (no complex temporaries)."""
+    out = np.empty(re.shape, dtype=np.complex128, order="F")
+    out.real = re
+    out.imag = im
+    return out
+
+
 def eigen_(A, nb=0, engine=ENGINE_AUTO):
     """eigen!(A): SkewHermitian{<:Real} -> GPU tridiagonalization + Q + host skewtrieigen_divided!
     + GPU back-transform (src/eigen.jl:75-97); SkewHermTridiagonal{<:Real} -> skewtrieigen_merged!
@@ -406,7 +414,7 @@ def eigen_(A, nb=0, engine=ENGINE_AUTO):
         return Eigen(-1j * mu, V)
     if isinstance(A, SkewHermTridiagonal):
         vals, vr, vi = _tri_eigen_host(A.ev, 1)
-        return Eigen(1j * vals, vr + 1j * vi)
+        return Eigen(1j * vals, _cplx(vr, vi))
     D = A.data if isinstance(A, SkewHermitian) else np.asarray(A)
     n = D.shape[0]
     if np.iscomplexobj(D):
@@ -429,7 +437,7 @@ def eigen_(A, nb=0, engine=ENGINE_AUTO):
     if rc == 2:
         raise RuntimeError("Maximum number of iterations reached, the algorithm didn't converge")
     _lib.check("skh_eigen_d", rc)
-    return Eigen(1j * vals, vr + 1j * vi)
+    return Eigen(1j * vals, _cplx(vr, vi))
 
 
 def eigen(A, nb=0, engine=ENGINE_AUTO):
