@@ -155,3 +155,17 @@ def test_skewfun_errors():
         skl.coth(skl.SkewHermitian(np.zeros((2, 2))))                   # exp(0) - I = 0
     assert np.array_equal(skl.exp(np.zeros((1, 1))), [[1.0]])
     assert np.array_equal(skl.cis(np.zeros((1, 1))), [[1.0 + 0j]])
+
+
+@pytest.mark.parametrize("n", [2, 10, 11])
+def test_skewfun_complex_tridiagonal_cases(n):
+    """The reference's complex tridiagonal inputs (test/runtests.jl:246-251: SkewHermTridiagonal(rand(T, n - 1),
+    rand(real(T), n)) for T = ComplexF32), every function, against the oracle on Matrix(A) and scipy."""
+    rng = np.random.default_rng(80 + n)
+    ev = (rng.random(n - 1) + 1j * rng.random(n - 1)) * 0.5
+    dv = rng.random(n) * 0.5
+    T = skl.SkewHermTridiagonal(ev, dv)
+    A = T.to_dense()
+    for f in FUNS:
+        say("skewfun complex tridiagonal %s n=%d" % (f, n))
+        check(A, f, getattr(skl, f)(T), oracle.skewfun(A, f))
