@@ -10,15 +10,18 @@ Also reported:
                 launch / mean launch time from HIP events recorded around each launch
   mfma_update   the rank-2k trailing update kernel (FP64 MFMA) against the FP64 matrix peak
   cpu_baseline  the oracle (C restatement of the reference algorithm, OpenBLAS, all host
-                threads) timed on a bounded sample: one full factorization at n=4096
+                threads): median of 3 full factorizations at n=8192 after a warm-up
+  plain512      n = 512 hessenberg + eigvals + logabspfaffian through the host API, against the
+                oracle's same calls
   nb_sweep      one factorization at each panel width nb in {32, 64, 128}: ms, TFLOP/s, panel
                 HBM GB/s, rank-2k TFLOP/s
   host_eigvals  the host implicit-QR eigenvalues of the GPU tridiagonal (sum of lambda^2 against
                 ||A||_F^2 as the check)
   batched       4096 uneven-size matrices (n in {32,...,512}), log|Pf| + sign through one tuned
-                plan replayed from a HIP graph, matrices/s, against the per-matrix GPU loop
-  grad          1024 matrices n = 256, log|Pf| + hand-written gradient, matrices/s, against the
-                per-matrix GPU loop
+                plan replayed from a HIP graph, matrices/s, against the per-matrix GPU loop (the
+                library's single-matrix engine choice) and the oracle's per-matrix loop
+  grad          1024 matrices n = 256, log|Pf| + gradient, matrices/s, against the per-matrix GPU
+                loop and the oracle's logabspfaffian + LU inverse per matrix
   complex       ComplexF64 n = 4096 skew-Hermitian tridiagonalization, TFLOP/s, with Q formation and
                 the residual / orthogonality of A = Q T Q^H
   complex_pfaffian  4096 complex skew-symmetric matrices (n in {32,...,256}), log|Pf| and phase by
@@ -26,11 +29,12 @@ Also reported:
                 plus one n = 4096 matrix alone
   skewchol      4096 real skew matrices (n in {32,...,256}), skew-Cholesky with complete pivoting in
                 one batched call, matrices/s, plus one n = 4096 matrix alone, against the oracle
-  matfun        exp(A) and sincos(A) of a real skew matrix at n = 1024 and 4096, whole host-API
-                calls (ms), against the oracle's exp at n = 1024
+  matfun        (--matfun) exp(A) and sincos(A) of a real skew matrix at n = 1024 and 4096, whole
+                host-API calls (ms), against the oracle's exp at n = 1024
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 8192] [--nb 0] [--no-cpu] [--no-batched]
-                       [--no-complex] [--no-matfun] [--only complex_pfaffian|skewchol|matfun]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 8192] [--nb 0] [--no-cpu] [--cpu-n 8192]
+                       [--cpu-runs 3] [--no-plain] [--no-batched] [--no-complex] [--matfun]
+                       [--only complex_pfaffian|skewchol|matfun]
 """
 import argparse
 import ctypes
@@ -53,40 +57,116 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(n, nb):
-    """HBM bytes per k_panel launch for this configuration, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel factors), or None.  nb = 0 is the
-    library's default width, 32 at n = 8192."""
-    if n != 8192 or nb not in (0, 32, 64):
-        return None
-    name = "k_panel_n8192_nb64_traffic.md" if nb == 64 else "k_panel_n8192_traffic.md"
-    path = os.path.join(HERE, "profiles", name)
+CSRC = os.path.join(HERE, "skewlinearalgebra.jl_amd", "csrc")
+# the sources a kernel's code comes from (its traffic profile is valid for exactly these bytes)
+KERNEL_SOURCES = {
+    "k_panel": ("skh_device.hpp", "skh_sync.hpp", "skh_kernels.hip"),
+    "k_compact": ("skh_device.hpp", "skh_sync.hpp", "skh_compact.hpp", "skh_compact.hip"),
+}
+
+
+def kernel_source_sha(kernel):
+    import hashlib
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES[kernel]:
+        with open(os.path.join(CSRC, name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic(profile, kernel):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (scaled by the copy-kernel factors), or (None, reason) when the profile is missing or was taken
+    on other kernel sources than the ones built here (its kernel_source_sha256 line)."""
+    path = os.path.join(HERE, "profiles", profile)
+    value, sha = None, None
     try:
         with open(path) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":")[1])
+                    value = float(line.split(":")[1])
+                elif line.startswith("kernel_source_sha256:"):
+                    sha = line.split(":")[1].strip()
     except OSError:
-        return None
-    return None
+        return None, "no profile %s" % profile
+    if value is None:
+        return None, "no traffic line in %s" % profile
+    if sha != kernel_source_sha(kernel):
+        return None, "%s was taken on other %s sources (sha %s, built %s)" % (profile, kernel, sha,
+                                                                             kernel_source_sha(kernel))
+    return value, "%s (kernel sources sha %s)" % (profile, sha)
 
 
-def cpu_baseline(n_cpu):
+def cpu_baseline(n_cpu, runs):
+    """The oracle (C restatement of the reference's CPU path: setnb = 60, the full-square gemv per
+    column, the Omega gemm and the serial skew combine, OpenBLAS on all host threads) on the same
+    generator: one warm-up factorization at n = 2048, then the median of `runs` at n_cpu."""
     import numpy as np
     import oracle
     threads = len(os.sched_getaffinity(0))
     oracle.lib()
     oracle.set_threads(threads)
     rng = np.random.default_rng(314159)
+    W = rng.standard_normal((2048, 2048))
+    oracle.skewblockedhess(np.asfortranarray(W - W.T))
+    del W
+    rng = np.random.default_rng(314159)
     B = rng.standard_normal((n_cpu, n_cpu))
     A = np.asfortranarray(B - B.T)
-    t0 = time.perf_counter()
-    oracle.skewblockedhess(A)
-    dt = time.perf_counter() - t0
+    del B
+    times = []
+    for r in range(runs):
+        t0 = time.perf_counter()
+        oracle.skewblockedhess(A)
+        times.append(time.perf_counter() - t0)
+        log("cpu baseline run %d: %.2f s" % (r, times[-1]))
+    dt = float(np.median(times))
     flops = 4.0 / 3.0 * n_cpu ** 3
     return {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
-            "sample": "one full factorization at n=%d (same generator), %.2f s, OpenBLAS threads=%d"
-                      % (n_cpu, dt, threads)}
+            "sample": "full factorizations at n=%d (same generator), median of %d after a warm-up: %s s, "
+                      "OpenBLAS threads=%d" % (n_cpu, runs, ", ".join("%.2f" % t for t in times), threads)}
+
+
+def plain_bench(torch, skl, reps=5, n=512, seed=314159):
+    """The plain n = 512 comparison: hessenberg + eigvals + logabspfaffian/sign of A = B - B^T
+    (B iid N(0,1), seed 314159) through the host API (host arrays in and out, as the reference's
+    calls), each call timed (median of `reps`), against the oracle's same three calls (C restatement
+    of the reference's CPU path, OpenBLAS on all host threads).  Each call factors A itself, as in the
+    reference (src/hessenberg.jl:15, src/eigen.jl:45, src/pfaffian.jl:236)."""
+    import oracle
+    threads = len(os.sched_getaffinity(0))
+    oracle.set_threads(threads)
+    rng = np.random.default_rng(seed)
+    B = rng.standard_normal((n, n))
+    A = np.asfortranarray(B - B.T)
+
+    def med(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)) * 1e3
+
+    g = {"hessenberg": med(lambda: skl.hessenberg(A)), "eigvals": med(lambda: skl.eigvals(A)),
+         "logabspfaffian": med(lambda: skl.logabspfaffian(A))}
+    c = {"hessenberg": med(lambda: oracle.skewblockedhess(A)),
+         "eigvals": med(lambda: oracle.skewtrieigvals(oracle.skewblockedhess(A)[2])),
+         "logabspfaffian": med(lambda: oracle.logabspfaffian(A))}
+    lam = np.sort(skl.eigvals(A).imag)
+    lamo = np.sort(oracle.skewtrieigvals(oracle.skewblockedhess(A)[2]))
+    lp, sg = skl.logabspfaffian(A)
+    lpo, sgo = oracle.logabspfaffian(A)
+    gt, ct = sum(g.values()), sum(c.values())
+    return {"metric": "n=512 hessenberg + eigvals + logabspfaffian, whole host-API calls", "unit": "ms",
+            "value": gt, "gpu_ms": g,
+            "eigvals_max_abs_diff_vs_oracle": float(np.max(np.abs(lam - lamo))),
+            "eigvals_tolerance_10_n_eps_norm2": 10 * n * np.finfo(float).eps * float(np.linalg.norm(A, 2)),
+            "logpf_rel_diff_vs_oracle": abs(lp - lpo) / abs(lpo), "sign_equal": bool(sg == sgo),
+            "cpu_baseline": {"value": ct, "unit": "ms (the three calls)", "cores": threads, "kind": "port",
+                             "per_call_ms": c, "sample": "oracle, median of %d per call" % reps,
+                             "cpu_over_gpu": ct / gt}}
 
 
 def cfg4_batch(torch, dev, count=4096, seed=314159):
@@ -173,21 +253,19 @@ def batched_bench(torch, skl, _lib, steps):
     ms = timed(torch, graph.replay, max(3, steps))
     plan.status(st)
     same = bool(torch.equal(lp, ref_lp))
-    # per-matrix GPU loop on every 16th matrix (all 16 sizes represented), one call per matrix
+    # per-matrix GPU loop on every 16th matrix (all 16 sizes represented): one skh_logpf_d_device
+    # call per matrix, the library's single-matrix engine choice (one workgroup to n = 192, the
+    # cooperative engine above), as a user looping logabspfaffian over device matrices would
     o = _lib.opts()
     L = _lib.lib()
     sample = np.arange(0, count, 16)
-    one_n = np.zeros(1, dtype=np.int64)
-    one_off = np.zeros(1, dtype=np.int64)
 
     def loop():
         for k in sample:
-            one_n[0] = ns[k]
-            one_off[0] = 0
-            rc = L.skh_logpf_batched_d_device(1, one_n.ctypes.data, one_off.ctypes.data,
-                                              buf.data_ptr() + 8 * int(offs[k]), lp.data_ptr() + 8 * int(k),
-                                              sg.data_ptr() + 8 * int(k), st, ctypes.byref(o))
-            _lib.check("skh_logpf_batched_d_device", rc)
+            n = int(ns[k])
+            rc = L.skh_logpf_d_device(n, buf.data_ptr() + 8 * int(offs[k]), n, lp.data_ptr() + 8 * int(k),
+                                      sg.data_ptr() + 8 * int(k), st, ctypes.byref(o))
+            _lib.check("skh_logpf_d_device", rc)
 
     loop()
     torch.cuda.synchronize()
@@ -197,16 +275,43 @@ def batched_bench(torch, skl, _lib, steps):
     loop_s = time.perf_counter() - t0
     plain_rate = len(sample) / loop_s
     flops = float(np.sum(4.0 / 3.0 * ns.astype(np.float64) ** 3))
-    loop_agree = bool(torch.equal(lp[sample], ref_lp[sample]))
+    lp_h = lp.cpu().numpy()
+    sg_h = sg.cpu().numpy()
+    ref_h = ref_lp.cpu().numpy()
+    loop_dev = float(np.max(np.abs(lp_h[sample] - ref_h[sample]) / np.maximum(1.0, np.abs(ref_h[sample]))))
+    # CPU leg: the oracle's logabspfaffian per matrix (the reference user's loop), same sample
+    import oracle
+    threads = len(os.sched_getaffinity(0))
+    oracle.set_threads(threads)
+    host = [buf[int(offs[k]):int(offs[k]) + int(ns[k]) ** 2].cpu().numpy().reshape((int(ns[k]),) * 2, order="F")
+            for k in sample]
+    t0 = time.perf_counter()
+    outs = [oracle.logabspfaffian(M) for M in host]
+    cpu_s = time.perf_counter() - t0
+    worst = max(abs(lp_h[k] - lo) / max(1.0, abs(lo)) for k, (lo, so) in zip(sample, outs))
+    signs = all(sg_h[k] == so for k, (lo, so) in zip(sample, outs))
+    tflops = flops / (ms * 1e-3) / 1e12
+    traffic, tnote = measured_traffic("k_compact_batch4096_traffic.md", "k_compact")
     return {"metric": "batched log|Pf|+sign, 4096 matrices n in {32,...,512}", "value": count / (ms * 1e-3),
             "unit": "matrices/s", "ms_per_batch": ms, "gflops": flops / (ms * 1e-3) / 1e9,
             "launch": "one tuned plan (%d workgroups, kernel NMAX %d) replayed from a HIP graph; tuner best %.2f ms"
                       % (info["workgroups"], info["kernel_nmax"], tuned_ms),
             "graph_replays_identical": same,
+            "roofline": {"bound": "mfma", "kernel": "k_compact (one workgroup per matrix)", "achieved": tflops,
+                         "peak": FP64_MFMA_PEAK_TF, "unit": "TFLOP/s", "frac": tflops / FP64_MFMA_PEAK_TF,
+                         "work": "sum (4/3) n^3 = %.4e flop per launch" % flops, "traffic": traffic,
+                         "traffic_source": tnote,
+                         "note": "the graph replay (a fill kernel and the k_compact launch) timed with HIP events; "
+                                 "FP64 peak (vector and matrix)"},
             "plain": {"value": plain_rate, "unit": "matrices/s",
-                      "sample": "per-matrix GPU loop (one device call per matrix) over every 16th matrix (%d)"
-                                % len(sample), "results_identical": loop_agree},
-            "speedup_vs_plain": (count / (ms * 1e-3)) / plain_rate}
+                      "sample": "per-matrix GPU loop, skh_logpf_d_device per matrix (library engine choice), every "
+                                "16th matrix (%d)" % len(sample), "max_rel_dev_vs_batch": loop_dev},
+            "speedup_vs_plain": (count / (ms * 1e-3)) / plain_rate,
+            "oracle_check": {"matrices": len(sample), "max_rel_dev_logpf": worst, "signs_equal": bool(signs)},
+            "cpu_baseline": {"value": len(sample) / cpu_s, "unit": "matrices/s (same size mix)", "cores": threads,
+                             "kind": "port", "sample": "oracle logabspfaffian per matrix on every 16th matrix "
+                                                        "(%d matrices, %.2f s), OpenBLAS threads=%d"
+                                                        % (len(sample), cpu_s, threads)}}
 
 
 def grad_bench(torch, skl, _lib, steps):
@@ -257,12 +362,37 @@ def grad_bench(torch, skl, _lib, steps):
     loop()
     torch.cuda.synchronize()
     plain_rate = sample / (time.perf_counter() - t0)
+    # CPU leg, the reference user's per-matrix route: logabspfaffian (src/pfaffian.jl:167-179) and the
+    # gradient -1/2 inv(A) by LU (inv(A::SkewHermitian), src/skewhermitian.jl:233)
+    import oracle
+    threads = len(os.sched_getaffinity(0))
+    oracle.set_threads(threads)
+    ncpu = 64
+    host = A[:ncpu].cpu().numpy()
+    t0 = time.perf_counter()
+    for k in range(ncpu):
+        M = np.asfortranarray(host[k])
+        oracle.logabspfaffian(M)
+        G = -0.5 * np.linalg.inv(M)
+        G = 0.5 * (G - G.T)
+    cpu_s = time.perf_counter() - t0
+    # work per matrix: forward (4/3) n^3, backward (7/3) n^3 (Q by blocked reflectors (4/3) n^3 and
+    # K = Q_E C^-1 Q_O^T, n^3)
+    flops = count * (11.0 / 3.0) * n ** 3
+    tflops = flops / (ms * 1e-3) / 1e12
     return {"metric": "batched log|Pf| + gradient, 1024 matrices n=256", "value": count / (ms * 1e-3),
             "unit": "matrices/s", "ms_per_batch": ms, "logpf_max_rel_err_vs_exact": lp_err,
             "grad_max_rel_err_vs_inverse": g_err,
+            "roofline": {"bound": "mfma", "kernel": "whole call (factorization + backward)", "achieved": tflops,
+                         "peak": FP64_MFMA_PEAK_TF, "unit": "TFLOP/s", "frac": tflops / FP64_MFMA_PEAK_TF,
+                         "work": "(11/3) n^3 flop per matrix: forward (4/3) n^3, backward (7/3) n^3",
+                         "traffic": None},
             "plain": {"value": plain_rate, "unit": "matrices/s",
                       "sample": "per-matrix GPU loop (one device call per matrix), first %d matrices" % sample},
-            "speedup_vs_plain": (count / (ms * 1e-3)) / plain_rate}
+            "speedup_vs_plain": (count / (ms * 1e-3)) / plain_rate,
+            "cpu_baseline": {"value": ncpu / cpu_s, "unit": "matrices/s", "cores": threads, "kind": "port",
+                             "sample": "oracle logabspfaffian + numpy LU inverse (-1/2 inv(A)) per matrix, first %d "
+                                       "matrices (%.2f s), OpenBLAS threads=%d" % (ncpu, cpu_s, threads)}}
 
 
 def complex_bench(torch, _lib, steps, n=4096, seed=314159):
@@ -512,11 +642,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--nb", type=int, default=0)
-    ap.add_argument("--cpu-n", type=int, default=4096)
+    ap.add_argument("--cpu-n", type=int, default=8192)
+    ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the n=512 plain-method line")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched log|Pf| and gradient lines")
     ap.add_argument("--no-complex", action="store_true", help="skip the complex n=4096 and complex Pfaffian lines")
-    ap.add_argument("--no-matfun", action="store_true", help="skip the matrix-function (exp, sincos) line")
+    ap.add_argument("--matfun", action="store_true", help="also run the matrix-function (exp, sincos) line")
     ap.add_argument("--only", choices=["complex_pfaffian", "skewchol", "matfun"],
                     help="run just this secondary line and print its JSON (for profiling it alone)")
     args = ap.parse_args()
@@ -596,6 +728,10 @@ def main():
     phases.update(dict(zip(("first_row_pass", "barrier1_arrive", "phase_b", "row_pass", "row_loads_and_wait",
                             "row_correct_w", "row_update_x", "row_handover", "b_loads_scalars", "b_t", "b_matvec",
                             "b_first_tile", "b_wait"), clk[:13])))
+    panel_traffic, panel_tnote = None, "profiles exist for n = 8192 at nb 32 (default) and 64 only"
+    if n == 8192 and args.nb in (0, 32, 64):
+        panel_traffic, panel_tnote = measured_traffic(
+            "k_panel_n8192_nb64_traffic.md" if args.nb == 64 else "k_panel_n8192_traffic.md", "k_panel")
     panel_avg_s = panel_ms / panel_cnt * 1e-3
     panel_bytes_per = panel_bytes / panel_cnt
     achieved = panel_bytes_per / panel_avg_s / 1e9
@@ -624,7 +760,7 @@ def main():
                    "n": n, "nb": args.nb or "auto", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_panel (skew matvec panel)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(n, args.nb),
+                     "traffic": panel_traffic, "traffic_source": panel_tnote,
                      "bytes_per_launch": panel_bytes_per, "launch_ms": panel_avg_s * 1e3,
                      "launches_per_step": panel_cnt / args.steps},
         "mfma_update": {"kernel": "k_rank2k", "achieved": upd_tf, "peak": FP64_MFMA_PEAK_TF, "unit": "TFLOP/s",
@@ -686,13 +822,17 @@ def main():
         log("skew-Cholesky (4096 matrices, one n=4096) ...")
         res["skewchol"] = skewchol_bench(torch, _lib, args.steps)
         log(json.dumps(res["skewchol"]))
-    if not args.no_matfun:
+    if not args.no_plain:
+        log("plain method n=512 (hessenberg, eigvals, logabspfaffian) ...")
+        res["plain512"] = plain_bench(torch, skl)
+        log(json.dumps(res["plain512"]))
+    if args.matfun:
         log("matrix functions (exp, sincos; n = 1024, 4096) ...")
         res["matfun"] = matfun_bench(skl)
         log(json.dumps(res["matfun"]))
     if not args.no_cpu:
-        log("cpu baseline (n=%d) ..." % args.cpu_n)
-        res["cpu_baseline"] = cpu_baseline(args.cpu_n)
+        log("cpu baseline (n=%d, %d runs) ..." % (args.cpu_n, args.cpu_runs))
+        res["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_runs)
         res["cpu_baseline"]["gpu_over_cpu"] = tflops / res["cpu_baseline"]["value"]
     print(json.dumps(res), flush=True)
 

@@ -73,6 +73,11 @@ int skh_sytrd_d_device(int64_t n, double *dA, int64_t lda, double *dtau, double 
  * odd n -> (-Inf, 0).
  */
 int skh_logpf_d(int64_t n, const double *A, int64_t lda, double *logpf, double *sgn, const skh_opts *opts);
+/* device variant (dA 16-byte aligned, *dlogpf and *dsgn in HBM), the same engine choice (SKH_ENGINE_AUTO:
+ * one workgroup for even n <= 192, the cooperative engine above).  Enqueued on `stream`; the one-workgroup
+ * engine returns once its launch is done. */
+int skh_logpf_d_device(int64_t n, const double *dA, int64_t lda, double *dlogpf, double *dsgn, void *stream,
+                       const skh_opts *opts);
 
 /*
  * Batched log|Pf| + sign over `batch` real skew matrices of uneven sizes, packed column-major

@@ -59,6 +59,18 @@ struct Buf {
 Buf g_mat, g_ws, g_vec, g_ctl, g_bin, g_bout, g_q, g_ev;
 rocblas_handle g_blas = nullptr;
 
+// Waits for a stream when it goes out of scope.  Host memory that asynchronous copies on that
+// stream read or write (local vectors, staging areas, the caller's arrays) must outlive those
+// copies on every return path, error returns included: declare the guard after such buffers, so
+// that it is destroyed, and the stream drained, before they are.
+struct StreamDrain {
+    hipStream_t st;
+    explicit StreamDrain(hipStream_t s) : st(s) {}
+    StreamDrain(const StreamDrain&) = delete;
+    StreamDrain& operator=(const StreamDrain&) = delete;
+    ~StreamDrain() { (void)hipStreamSynchronize(st); }
+};
+
 int set_err(int code, const char* fmt, ...)
 {
     char buf[512];
@@ -154,6 +166,12 @@ struct KTimer {
         used = 0;
         kind.clear();
         work.clear();
+    }
+    void release()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        ev.clear();
+        reset();
     }
 };
 KTimer g_timer;
@@ -379,6 +397,7 @@ int batched_run(int64_t batch, const int64_t* ns, const int64_t* offs, const dou
     int64_t* dctl = (int64_t*)g_bctl.get(ctl_bytes);
     if (!dctl) return set_err(SKH_ERR_HIP, "hipMalloc of batched control arrays failed");
     std::vector<int64_t> h(nctl);
+    StreamDrain drain(st);   // h is read by the upload below
     std::copy(ns, ns + batch, h.begin());
     std::copy(offs, offs + batch, h.begin() + batch);
     std::copy(small.begin(), small.end(), h.begin() + 2 * batch);
@@ -871,6 +890,7 @@ int zpf_run(int64_t batch, const int64_t* ns, const int64_t* offs, const double2
     int64_t* dctl = (int64_t*)g_zpctl.get((2 * batch + count) * sizeof(int64_t));
     if (!slots || !dctl) return set_err(SKH_ERR_HIP, "hipMalloc of Pfaffian work slots failed");
     std::vector<int64_t> h(2 * batch + count);
+    StreamDrain drain(st);   // h is read by the upload below
     std::copy(ns, ns + batch, h.begin());
     std::copy(offs, offs + batch, h.begin() + batch);
     std::copy(order.begin(), order.end(), h.begin() + 2 * batch);
@@ -887,6 +907,7 @@ int zpf_host(int64_t batch, const int64_t* n, const int64_t* offset, const doubl
     int rc = ensure_init_locked(g_device);
     if (rc) return rc;
     hipStream_t st = g_stream;
+    StreamDrain drain(st);   // the caller's arrays are copied asynchronously
     std::vector<int64_t> offs2(batch);
     int64_t tot = 0;
     for (int64_t b = 0; b < batch; b++) {
@@ -931,6 +952,7 @@ int rot_apply_gpu(void*, int64_t nrows, const int32_t* ko, const double* cso, in
     if (rc) return rc;
     hipStream_t st = g_stream;
     const std::vector<int64_t> so = chain_starts(ko, nro), se = chain_starts(ke, nre);
+    StreamDrain drain(st);   // so, se and the caller's arrays are copied asynchronously
     double* dq = (double*)g_rotq.get((size_t)(no * no + ne * ne + 1) * sizeof(double));
     int32_t* dk = (int32_t*)g_rotk.get((size_t)(nro + nre + 1) * sizeof(int32_t));
     double* dcs = (double*)g_rotcs.get((size_t)(2 * (nro + nre) + 1) * sizeof(double));
@@ -974,6 +996,7 @@ int symtri_gpu_diag(int64_t n, const double* diag, const double* e, double* d, d
     double* dE = dD + n;
     int* dinfo = (int*)(dE + n);
     const std::vector<double> zeros(diag ? 0 : n, 0.0);
+    StreamDrain drain(st);   // zeros and the caller's arrays are copied asynchronously
     HIPCHK(hipMemcpyAsync(dD, diag ? diag : zeros.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
     if (n > 1) HIPCHK(hipMemcpyAsync(dE, e, (n - 1) * sizeof(double), hipMemcpyHostToDevice, st));
     rc = blas_on(st);
@@ -1013,6 +1036,7 @@ int chol_run(int64_t batch, const int64_t* ns, const int64_t* offs, const double
 {
     if (batch == 0) return 0;
     std::vector<int64_t> h(4 * batch);
+    StreamDrain drain(st);   // h is read by the upload below
     int64_t* order = h.data() + 3 * batch;
     int64_t pacc = 0;
     for (int64_t b = 0; b < batch; b++) {
@@ -1070,6 +1094,7 @@ int chol_host(int64_t batch, const int64_t* n, const int64_t* offset, const doub
     int rc = ensure_init_locked(g_device);
     if (rc) return rc;
     hipStream_t st = g_stream;
+    StreamDrain drain(st);   // the caller's arrays are copied asynchronously
     std::vector<int64_t> offs2(batch);
     int64_t tot = 0, ptot = 0;
     for (int64_t b = 0; b < batch; b++) {
@@ -1121,6 +1146,7 @@ int skh_finalize(void)
                    &g_zpout, &g_chctl, &g_chin, &g_chout, &g_chp,
                    &g_chws, &g_rotq, &g_rotk, &g_rotcs, &g_rotseg, &g_fvec, &g_fpiv, &g_fz, &g_stq})
         b->release();
+    g_timer.release();
     if (g_blas) (void)rocblas_destroy_handle(g_blas);
     g_blas = nullptr;
     (void)hipStreamDestroy(g_stream);
@@ -1169,6 +1195,7 @@ int skh_sytrd_d(int64_t n, double* A, int64_t lda, double* tau, double* eta, int
     double* dtau = dv;
     double* deta = dv + n;
     std::vector<char> stage;
+    StreamDrain drain(st);   // stage and the caller's arrays are copied asynchronously
     HIPCHK(upload_cols(dA, ld * sizeof(double), A, lda * sizeof(double), n * sizeof(double), n, stage, st));
     if (!stage.empty()) HIPCHK(hipStreamSynchronize(st));   // every return path below outlives the copy
     if (engine_of(n, opts) == SKH_ENGINE_LARGE) {
@@ -1241,6 +1268,51 @@ int skh_logpf_d(int64_t n, const double* A, int64_t lda, double* logpf, double* 
     return 0;
 }
 
+int skh_logpf_d_device(int64_t n, const double* dA, int64_t lda, double* dlogpf, double* dsgn, void* stream,
+                       const skh_opts* opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (n < 0) return set_err(-1, "n < 0");
+    if (n > 0 && !dA) return set_err(-2, "A is NULL");
+    if (lda < std::max<int64_t>(n, 1)) return set_err(-3, "lda < n");
+    if (!dlogpf || !dsgn) return set_err(-4, "output NULL");
+    if (((uintptr_t)dA) & 15) return set_err(-2, "A must be 16-byte aligned");
+    if (opts && (opts->nb < 0 || opts->nb > skh::NBMAX)) return set_err(-7, "nb out of range");
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const int eng = engine_of(n, opts);
+    if (n % 2 == 1 || n <= 1 || (eng == SKH_ENGINE_TEAM1 && n <= CMP_NMAX)) {
+        // one workgroup (or the closed form for odd and empty n), on a packed copy when lda != n
+        const double* src = dA;
+        if (lda != n && n > 1) {
+            double* P = (double*)g_big.get((size_t)n * n * sizeof(double));
+            if (!P) return set_err(SKH_ERR_HIP, "hipMalloc of %lld doubles failed", (long long)(n * n));
+            HIPCHK(hipMemcpy2DAsync(P, n * sizeof(double), dA, lda * sizeof(double), n * sizeof(double), n,
+                                    hipMemcpyDeviceToDevice, st));
+            src = P;
+        }
+        int64_t nn = n, off = 0;
+        return batched_run(1, &nn, &off, src, dlogpf, dsgn, opts, st);
+    }
+    if (eng == SKH_ENGINE_TEAM1) return set_err(SKH_ERR_UNSUPPORTED, "the compact engine takes n <= %lld",
+                                                (long long)CMP_NMAX);
+    // large engine on a working copy (even leading dimension), log|Pf| and sign from eta, tau on the device
+    const int64_t ld = (n + 1) & ~int64_t(1);
+    double* w = (double*)g_big.get((size_t)(ld * n + 2 * n) * sizeof(double));
+    if (!w) return set_err(SKH_ERR_HIP, "hipMalloc of %lld doubles failed", (long long)(ld * n + 2 * n));
+    double* te = w + ld * n;
+    HIPCHK(hipMemcpy2DAsync(w, ld * sizeof(double), dA, lda * sizeof(double), n * sizeof(double), n,
+                            hipMemcpyDeviceToDevice, st));
+    g_timer.reset();
+    g_timer.on = false;
+    g_timer.phases = false;
+    rc = sytrd_large(n, w, ld, te, te + n, pick_nb(n, opts), st);
+    if (rc) return rc;
+    HIPCHK((hipError_t)skh_internal_logpf_from(te + n, te, n, dlogpf, dsgn, st));
+    return 0;
+}
+
 int skh_logpf_batched_d_device(int64_t batch, const int64_t* n, const int64_t* offset, const double* dA,
                                double* dlogpf, double* dsgn, void* stream, const skh_opts* opts)
 {
@@ -1273,6 +1345,7 @@ int skh_logpf_batched_d(int64_t batch, const int64_t* n, const int64_t* offset, 
     int rc = ensure_init_locked(g_device);
     if (rc) return rc;
     hipStream_t st = g_stream;
+    StreamDrain drain(st);   // the caller's arrays are copied asynchronously
     // repack with even offsets (16-byte aligned matrices)
     std::vector<int64_t> offs2(batch);
     int64_t tot2 = 0;
@@ -1341,6 +1414,7 @@ int eigen_d_dev(int64_t n, const double* A, int64_t lda, double scale, double* v
     int rc = ensure_init_locked(g_device);
     if (rc) return rc;
     hipStream_t st = g_stream;
+    StreamDrain drain(st);   // F, tau, hr and hi are copied asynchronously
     double* d = (double*)g_q.get((size_t)slots * nn * sizeof(double) + 2 * n * sizeof(double));
     if (!d) return set_err(SKH_ERR_HIP, "hipMalloc failed for eigen n = %lld", (long long)n);
     *base = d;
@@ -1433,6 +1507,7 @@ int skh_logpf_grad_batched_d(int64_t batch, const int64_t* n, const int64_t* off
     int rc = ensure_init_locked(g_device);
     if (rc) return rc;
     hipStream_t st = g_stream;
+    StreamDrain drain(st);   // the caller's arrays are copied asynchronously
     std::vector<int64_t> offs2(batch);
     int64_t tot2 = 0;
     for (int64_t b = 0; b < batch; b++) {
@@ -1692,6 +1767,7 @@ int skh_sytrd_z(int64_t n, double* A, int64_t lda, double* tau, double* eta, dou
     int rc = ensure_init_locked(g_device);
     if (rc) return rc;
     hipStream_t st = g_stream;
+    StreamDrain drain(st);   // the caller's arrays are copied asynchronously
     double2* dA = zupload(n, A, lda, st, &rc);
     if (!dA) return rc;
     double* dv = (double*)g_vec.get((size_t)(4 * n + 4) * sizeof(double));
@@ -1780,6 +1856,7 @@ int eigen_z_dev_locked(int64_t n, const double* A, int64_t lda, double scale, do
     if (rc) return rc;
     // tridiagonal eigenpairs on the GPU (divide and conquer)
     std::vector<double> eta(n), dvim(n), d(n);
+    StreamDrain drain(st);   // eta, dvim and d are copied asynchronously
     if (n > 1) HIPCHK(hipMemcpyAsync(eta.data(), deta, (n - 1) * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(dvim.data(), ddvim, n * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -2197,6 +2274,7 @@ int skh_skewfun_d(int64_t n, const double* A, int64_t lda, int fn, double* out1,
     double* base = nullptr;
     rc = eigen_d_dev(n, A, lda, fun_scale(fn), vals.data(), opts, FUN_SLOTS, lk, &base);
     if (rc) return rc;
+    StreamDrain drain(g_stream);   // out1 / out2 are written asynchronously
     return fun_real_locked(n, fn, vals.data(), base, out1, out2);
 }
 
@@ -2222,6 +2300,7 @@ int skh_skewfun_tri_d(int64_t n, const double* ev, int fn, double* out1, double*
     std::lock_guard<std::mutex> lk(g_mu);
     rc = ensure_init_locked(g_device);
     if (rc) return rc;
+    StreamDrain drain(g_stream);   // hr and hi are copied asynchronously; fun_real_locked may return early
     double* base = (double*)g_q.get((size_t)FUN_SLOTS * nn * sizeof(double));
     if (!base) return set_err(SKH_ERR_HIP, "hipMalloc failed for n = %lld", (long long)n);
     HIPCHK(hipMemcpyAsync(base + EV_R * nn, hr.data(), nn * sizeof(double), hipMemcpyHostToDevice, g_stream));
@@ -2237,6 +2316,9 @@ int skh_skewfun_z(int64_t n, const double* A, int64_t lda, int fn, double* out1,
     std::lock_guard<std::mutex> lk(g_mu);
     std::vector<double> vals(n);
     double2* dV = nullptr;
+    rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    StreamDrain drain(g_stream);   // the caller's arrays are copied asynchronously
     rc = eigen_z_dev_locked(n, A, lda, fun_scale(fn), vals.data(), &dV);
     if (rc) return rc;
     return fun_complex_locked(n, fn, vals.data(), dV, out1, out2);

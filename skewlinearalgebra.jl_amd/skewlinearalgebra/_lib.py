@@ -2,6 +2,7 @@
 
 Loading fails loudly when the HIP library has not been built: there is no CPU fallback.
 """
+import atexit
 import ctypes
 import os
 
@@ -49,6 +50,7 @@ def lib():
         "skh_sytrd_d": ([i64, vp, i64, vp, vp, _i64p, op], ctypes.c_int),
         "skh_sytrd_d_device": ([i64, vp, i64, vp, vp, vp, op], ctypes.c_int),
         "skh_logpf_d": ([i64, vp, i64, _dp, _dp, op], ctypes.c_int),
+        "skh_logpf_d_device": ([i64, vp, i64, vp, vp, vp, op], ctypes.c_int),
         "skh_logpf_batched_d": ([i64, vp, vp, vp, vp, vp, op], ctypes.c_int),
         "skh_logpf_batched_d_device": ([i64, vp, vp, vp, vp, vp, vp, op], ctypes.c_int),
         "skh_skewtrieigvals_d": ([i64, vp, vp], ctypes.c_int),
@@ -94,7 +96,15 @@ def lib():
         f.argtypes = args
         f.restype = res
     _L = L
+    # release the library's stream, BLAS handle and cached device buffers while the HIP runtime is
+    # still up: left to the C exit handlers, they would be torn down after the runtime
+    atexit.register(_finalize)
     return L
+
+
+def _finalize():
+    if _L is not None:
+        _L.skh_finalize()
 
 
 def check(func, code):

@@ -228,7 +228,7 @@ def test_pfaffian_exact(golden):
     assert sg == -1.0 and abs(lp - 2.1972245773362196) <= 4 * EPS * 2.2
 
 
-@pytest.mark.parametrize("n", [4, 11, 128, 513])
+@pytest.mark.parametrize("n", [4, 11, 128, 512, 513])
 def test_eigvals_vs_oracle(n):
     say("eigvals n=%d" % n)
     A = skew_gauss(n, 60 + n)
@@ -556,7 +556,7 @@ def tri_fourth_moment(e):
     return float(np.sum(d * d) + 2.0 * np.sum(off * off))
 
 
-@pytest.mark.parametrize("nb", [64, 128])
+@pytest.mark.parametrize("nb", [32, 64, 128])
 def test_sytrd_n8192_invariants(nb):
     """Full-size factorization, the whole cooperative grid (512 workgroups) in the first panels.
     A = Q T Q^T fixes ||T||_F = ||A||_F, tr(T^4) = ||A^2||_F^2 and log|det A| = 2 log|Pf T|;
