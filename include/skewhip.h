@@ -68,6 +68,20 @@ int skh_sytrd_d_device(int64_t n, double *dA, int64_t lda, double *dtau, double 
                        const skh_opts *opts);
 
 /*
+ * skh_sytrd_d_device without any allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, kernels and results as skh_sytrd_d_device (SKH_FLAG_TIMING and
+ * SKH_FLAG_PHASES are ignored), for n <= SKH_NOALLOC_MAX_N; larger n is rejected (status -1).  The
+ * workspace and grid-barrier control words live in a fixed device array of about 49 MiB (6.4e6
+ * doubles) that each GPU the library runs on holds until the process ends.  Every call on a GPU shares
+ * that GPU's array and starts by clearing its control words: calls that overlap on one GPU race (issue
+ * them on one stream, or synchronise between them).
+ */
+#define SKH_NOALLOC_MAX_N 8192
+int skh_sytrd_d_device_noalloc(int64_t n, double *dA, int64_t lda, double *dtau, double *deta, void *stream,
+                               const skh_opts *opts);
+
+/*
  * log|Pf(A)| and sign for a real skew-symmetric A (A is not modified).
  * Replaces _logabspfaffian!(A::SkewHermitian{<:Real}) (reference src/pfaffian.jl:167-179):
  * odd n -> (-Inf, 0).

@@ -30,6 +30,7 @@ bool g_inited = false;
 int g_device = 0;
 int g_num_cu = 0;
 hipStream_t g_stream = nullptr;
+int g_noalloc_status = 0;   // result of looking up the fixed scratch at init (hipError_t)
 
 struct Buf {
     void* p = nullptr;
@@ -103,6 +104,9 @@ int ensure_init_locked(int device)
     HIPCHK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device));
     if (!coop) return set_err(SKH_ERR_UNSUPPORTED, "device does not support cooperative launch");
     HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    // device address of the allocation-free entry point's scratch, with `device` current; a failure
+    // is reported by that entry point only
+    g_noalloc_status = skh_internal_noalloc_setup(device);
     g_device = device;
     g_inited = true;
     return 0;
@@ -185,28 +189,42 @@ struct Ctl {
     int* cerr;
 };
 
-int get_ctl(Ctl& c)
+// words [0, 288): barrier counters (9 lines of 32 words); [320]: error flag of the panel
+// kernel; [384, 410): phase clocks (13 x u64); [448]: compact queue; [480]: compact error flag
+constexpr size_t CTL_BYTES = 2048;
+
+void ctl_at(unsigned* p, Ctl& c)
 {
-    // words [0, 288): barrier counters (9 lines of 32 words); [320]: error flag of the panel
-    // kernel; [384, 410): phase clocks (13 x u64); [448]: compact queue; [480]: compact error flag
-    unsigned* p = (unsigned*)g_ctl.get(2048);
-    if (!p) return set_err(SKH_ERR_HIP, "hipMalloc of control block failed");
     c.bar = p;
     c.err = (int*)(p + 320);
     c.ph = (unsigned long long*)(p + 384);
     c.cqueue = p + 448;
     c.cerr = (int*)(p + 480);
+}
+
+int get_ctl(Ctl& c)
+{
+    unsigned* p = (unsigned*)g_ctl.get(CTL_BYTES);
+    if (!p) return set_err(SKH_ERR_HIP, "hipMalloc of control block failed");
+    ctl_at(p, c);
     return 0;
 }
 
 // ----------------------------------------------------------------------------------------
 // Large engine: cooperative panel kernel + MFMA rank-2k update per panel.
+// The workspace and control block are the library's cached buffers, or (ws_ext, ctl_ext) given by
+// the caller: ws_cap doubles and CTL_BYTES, nothing allocated then.
 // ----------------------------------------------------------------------------------------
-int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, int64_t nb, hipStream_t st)
+int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, int64_t nb, hipStream_t st,
+                double* ws_ext = nullptr, size_t ws_cap = 0, unsigned* ctl_ext = nullptr)
 {
     Ctl ctl;
-    int rc = get_ctl(ctl);
-    if (rc) return rc;
+    if (ctl_ext) {
+        ctl_at(ctl_ext, ctl);
+    } else {
+        int rc = get_ctl(ctl);
+        if (rc) return rc;
+    }
     const int64_t ld = (n + 1) & ~int64_t(1);
     const int64_t nt = (n + skh::TB - 1) / skh::TB + 1;
     int per_cu0 = 0;
@@ -216,8 +234,16 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
     // W | Pnorm x2 | PS x2 | X x2 | Tv | Prow | Pcol (Tv, Prow, Pcol contiguous: cleared per launch)
     size_t doubles = (size_t)(ld * nb + 2 * nt + 2 * nt * 2 * skh::NBMAX + 2 * ld + 2 * skh::NBMAX + nseg * ld +
                               nt * ld + 64);
-    double* ws = (double*)g_ws.get(doubles * sizeof(double));
-    if (!ws) return set_err(SKH_ERR_HIP, "hipMalloc of %zu bytes of workspace failed", doubles * 8);
+    double* ws;
+    if (ws_ext) {
+        if (doubles > ws_cap)
+            return set_err(SKH_ERR_UNSUPPORTED, "n = %lld, nb = %lld needs %zu doubles of workspace, the scratch "
+                           "holds %zu", (long long)n, (long long)nb, doubles, ws_cap);
+        ws = ws_ext;
+    } else {
+        ws = (double*)g_ws.get(doubles * sizeof(double));
+        if (!ws) return set_err(SKH_ERR_HIP, "hipMalloc of %zu bytes of workspace failed", doubles * 8);
+    }
     skh::Work w;
     w.W = ws;
     w.ldw = ld;
@@ -1172,6 +1198,34 @@ int skh_sytrd_d_device(int64_t n, double* dA, int64_t lda, double* dtau, double*
     skh_opts o = opts ? *opts : skh_opts{0, 0};
     o.flags = SKH_ENGINE_LARGE | (o.flags & (SKH_FLAG_TIMING | SKH_FLAG_PHASES));
     return sytrd_device_locked(n, dA, lda, dtau, deta, st, &o);
+}
+
+int skh_sytrd_d_device_noalloc(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, void* stream,
+                               const skh_opts* opts)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (n < 0) return set_err(-1, "n < 0");
+    if (n > SKH_NOALLOC_MAX_N)
+        return set_err(-1, "n = %lld exceeds the fixed scratch of this entry point (n <= %d)", (long long)n,
+                       SKH_NOALLOC_MAX_N);
+    if (n > 0 && !dA) return set_err(-2, "A is NULL");
+    if (lda < std::max<int64_t>(n, 1) || (lda & 1)) return set_err(-3, "lda must be >= n and even");
+    if (n > 1 && (!dtau || !deta)) return set_err(-4, "tau/eta NULL");
+    if (((uintptr_t)dA) & 15) return set_err(-2, "A must be 16-byte aligned");
+    if (opts && (opts->nb < 0 || opts->nb > skh::NBMAX)) return set_err(-7, "nb out of range");
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    if (n <= 1) return 0;
+    unsigned* ctl = nullptr;
+    double* ws = nullptr;
+    size_t cap = 0;
+    if (g_noalloc_status || skh_internal_noalloc_scratch(g_device, &ctl, &ws, &cap))
+        return set_err(SKH_ERR_HIP, "the fixed scratch could not be located on device %d: %s", g_device,
+                       hipGetErrorString((hipError_t)(g_noalloc_status ? g_noalloc_status : hipErrorNotInitialized)));
+    g_timer.reset();
+    g_timer.on = false;
+    g_timer.phases = false;
+    return sytrd_large(n, dA, lda, dtau, deta, pick_nb(n, opts), (hipStream_t)stream, ws, cap, ctl);
 }
 
 int skh_sytrd_d(int64_t n, double* A, int64_t lda, double* tau, double* eta, int64_t* ntau_nonzero,

@@ -114,6 +114,11 @@ int skh_internal_launch_rank2k(double* A, int64_t lda, const double* U, int64_t 
                                int64_t c0, int nbp, hipStream_t st);
 int skh_internal_launch_finish(double* A, int64_t lda, int64_t n, double* tau, double* eta, hipStream_t st);
 int skh_internal_zero_u32(unsigned* p, int count, hipStream_t st);
+// fixed device scratch of the allocation-free factorization (skh_noalloc.hip): setup looks up the
+// array's address on `device` (which must be current); scratch returns the control block (2 KiB) and
+// the workspace after it
+int skh_internal_noalloc_setup(int device);
+int skh_internal_noalloc_scratch(int device, unsigned** ctl, double** ws, size_t* ws_doubles);
 int skh_internal_mfma_probe(double* dout, hipStream_t st);
 int skh_internal_eye(double* Q, int64_t ldq, int64_t n, hipStream_t st);
 int skh_internal_form_v(const double* F, int64_t ldf, int64_t n, int64_t c0, int nbp, double* V, int64_t ldv,
