@@ -111,7 +111,10 @@ int skh_logpf_batched_d_device(int64_t batch, const int64_t *n, const int64_t *o
  * No reference counterpart (the reference has no AD code; its CPU oracle for this quantity is
  * inv(A::SkewHermitian), src/skewhermitian.jl:233).  grad uses the same packing as Apacked.
  * A^{-1} = Q T^{-1} Q^T is formed from the stored reflectors with the closed-form inverse of the
- * skew tridiagonal T.  Odd n: logpf = -Inf, sign 0, grad = 0.
+ * skew tridiagonal T, every product in a hand-written batched FP64 MFMA GEMM.  Odd n: logpf = -Inf,
+ * sign 0, grad = 0.  The device variant keeps the plan of its last batch layout (sizes and offsets)
+ * and only enqueues when called again with the same layout; it blocks only while it builds that plan
+ * and when members with n > 1024 are present (those are factored one at a time by the cooperative engine).
  */
 int skh_logpf_grad_batched_d(int64_t batch, const int64_t *n, const int64_t *offset, const double *Apacked,
                              double *logpf, double *sgn, double *grad, const skh_opts *opts);
@@ -136,6 +139,18 @@ int skh_logpf_batched_plan_create(int64_t batch, const int64_t *n, const int64_t
 int skh_logpf_batched_plan_exec(skh_plan plan, const double *dA, double *dlogpf, double *dsgn, void *stream);
 int skh_logpf_batched_plan_tune(skh_plan plan, const double *dA, double *dlogpf, double *dsgn, void *stream,
                                 double *best_ms);
+/*
+ * Gradient plans: the same for skh_logpf_grad_batched_d_device (log|Pf|, sign and -1/2 A^{-1} into dgrad,
+ * which uses the input's packing; even n <= 1024).  Creation uploads the control arrays and allocates the
+ * workspaces (tau | eta, the Q formation's buffers for the largest size group); exec enqueues the forward
+ * launch and the backward kernels (hand-written: batched FP64 MFMA GEMMs, compact-WY helpers, the bidiagonal
+ * solve and the skew output) without allocating, synchronising, copying from the host or memset, so it can be
+ * captured into a hipGraph.  skh_plan_status, skh_plan_info and skh_plan_destroy take either kind of plan.
+ */
+int skh_logpf_grad_batched_plan_create(int64_t batch, const int64_t *n, const int64_t *offset, const skh_opts *opts,
+                                       skh_plan *plan);
+int skh_logpf_grad_batched_plan_exec(skh_plan plan, const double *dA, double *dlogpf, double *dsgn, double *dgrad,
+                                     void *stream);
 int skh_plan_status(skh_plan plan, void *stream);
 int skh_plan_info(skh_plan plan, int64_t *out4);
 int skh_plan_destroy(skh_plan plan);

@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "skewlinearalgebra", "libskewhip.so")
-SOURCES = ["skh_kernels.hip", "skh_compact.hip", "skh_complex.hip", "skh_zpfaff.hip", "skh_chol.hip", "skh_rot.hip", "skh_fun.hip", "skh_noalloc.hip", "skh_api.cpp", "host_tridiag.cpp"]
+SOURCES = ["skh_kernels.hip", "skh_compact.hip", "skh_complex.hip", "skh_zpfaff.hip", "skh_chol.hip", "skh_rot.hip", "skh_fun.hip", "skh_noalloc.hip", "skh_grad.hip", "skh_api.cpp", "host_tridiag.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -651,108 +651,251 @@ double2* zupload(int64_t n, const double* A, int64_t lda, hipStream_t st, int* r
 }
 
 // ----------------------------------------------------------------------------------------
-// Gradient of log|Pf|: grad_b = -1/2 A_b^{-1} = -1/2 (K - K^T), K = Q_E C^{-1} Q_O^T, grouped by size.
-// dgrad first receives the factors from the forward kernel (same offsets as the input).
+// Gradient of log|Pf|: grad_b = -1/2 A_b^{-1} = -1/2 (K - K^T), K = Q_E C^{-1} Q_O^T, grouped by size
+// (kernels in skh_grad.hip).  Everything the device work needs (control arrays, tau | eta, the
+// workspaces of the largest size group) is prepared once in a GradPlan, so an evaluation only
+// enqueues kernels: no host copy, synchronisation, allocation or memset (capturable in a graph).
+// The gradient buffer first receives the factors from the forward kernel (same offsets as the
+// input), then the gradient.
 // ----------------------------------------------------------------------------------------
-Buf g_gq, g_gk, g_gz, g_gv, g_gp, g_gte;
+constexpr int64_t GRAD_NB = 32;   // reflectors per compact-WY block of the Q formation
+
+struct GradChunk {
+    int64_t n;     // size of every member
+    int cnt;       // members
+    int64_t at;    // goff[cnt] | toff[cnt] at dgrp + at
+};
+
+struct GradPlan {
+    int64_t batch = 0;
+    std::vector<int64_t> ns, offs;   // host copies: sizes and element offsets (input and gradient)
+    std::vector<int64_t> big;        // even n > CMP_NMAX: forward by the large engine, one at a time
+    std::vector<int64_t> toffs;      // tau | eta offsets (2 max(n, 1) doubles per member)
+    int64_t count = 0;               // members of the compact launch
+    int grid = 0, kmax = 512;
+    std::vector<GradChunk> chunks;
+    int64_t nodd = 0, odd_at = 0;    // odd members: goff[nodd] | n[nodd] at dgrp + odd_at
+    int64_t* dctl = nullptr;         // ns | offs | items | toffs of the compact launch
+    int64_t* dgrp = nullptr;         // per chunk goff | toff, then the odd list
+    unsigned* ctl = nullptr;         // [0] queue, [32] error flag (256 bytes)
+    double* dte = nullptr;           // tau | eta of every member
+    double* ws = nullptr;            // Q | K | Zt | Y | Z | V | Gm | T of the largest chunk
+    double flops = 0.0;              // forward (4/3) n^3 + backward (7/3) n^3 per even member
+};
+
+void grad_plan_free(GradPlan* p)
+{
+    if (!p) return;
+    for (void* q : {(void*)p->dctl, (void*)p->dgrp, (void*)p->ctl, (void*)p->dte, (void*)p->ws})
+        if (q) (void)hipFree(q);
+    delete p;
+}
+
+// doubles of one chunk's workspaces
+size_t grad_ws_doubles(int64_t n, int64_t cnt)
+{
+    const int64_t nn = n * n, h = n / 2, NB = GRAD_NB;
+    return (size_t)cnt * (size_t)(2 * nn + n * h + 2 * NB * n + n * NB + 2 * NB * NB);
+}
+
+// allow_big: members with even n > CMP_NMAX are factored by the large engine (not capturable)
+int grad_plan_build(int64_t batch, const int64_t* n, const int64_t* offset, bool allow_big, GradPlan** out)
+{
+    *out = nullptr;
+    GradPlan* p = new GradPlan();
+    p->batch = batch;
+    p->ns.assign(n, n + batch);
+    p->offs.assign(offset, offset + batch);
+    p->toffs.resize(batch);
+    int64_t tt = 0;
+    for (int64_t b = 0; b < batch; b++) {
+        p->toffs[b] = tt;
+        tt += 2 * std::max<int64_t>(n[b], 1);
+        if (n[b] % 2 == 0 && n[b] > 0)
+            p->flops += (11.0 / 3.0) * (double)n[b] * (double)n[b] * (double)n[b];
+    }
+    std::vector<int64_t> small;
+    for (int64_t b = 0; b < batch; b++) {
+        if (n[b] > CMP_NMAX && n[b] % 2 == 0) {
+            if (!allow_big) {
+                grad_plan_free(p);
+                return set_err(SKH_ERR_UNSUPPORTED, "plans take even n <= %lld (n[%lld] = %lld)", (long long)CMP_NMAX,
+                               (long long)b, (long long)n[b]);
+            }
+            p->big.push_back(b);
+        } else {
+            small.push_back(b);
+        }
+    }
+    std::stable_sort(small.begin(), small.end(), [&](int64_t a, int64_t b) { return n[a] > n[b]; });
+    p->count = (int64_t)small.size();
+    int64_t nmax = 2;
+    for (int64_t b : small) nmax = std::max(nmax, n[b]);
+    p->kmax = nmax <= 512 ? 512 : 1024;
+    int per_cu = 0;
+    if (skh_internal_compact_occupancy(p->kmax, &per_cu) != 0) {
+        grad_plan_free(p);
+        return set_err(SKH_ERR_HIP, "occupancy query failed");
+    }
+    p->grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)std::max(1, per_cu) * g_num_cu, p->count));
+    // size groups (ascending n), chunked so that Q and K stay below about 1.5 GB each
+    std::vector<int64_t> idx;
+    std::vector<int64_t> odd;
+    for (int64_t b = 0; b < batch; b++) {
+        if (n[b] % 2 == 1) odd.push_back(b);
+        else if (n[b] > 0) idx.push_back(b);
+    }
+    std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return n[a] < n[b]; });
+    std::vector<int64_t> g;   // goff | toff per chunk, then the odd list
+    size_t wsmax = 0;
+    for (size_t gi = 0; gi < idx.size();) {
+        const int64_t m = n[idx[gi]];
+        size_t gj = gi;
+        while (gj < idx.size() && n[idx[gj]] == m) gj++;
+        const int64_t per = std::max<int64_t>(1, std::min<int64_t>(4096, (int64_t)(1.5e9 / (8.0 * m * m))));
+        for (size_t q0 = gi; q0 < gj; q0 += per) {
+            const int cnt = (int)std::min<int64_t>(per, (int64_t)(gj - q0));
+            p->chunks.push_back(GradChunk{m, cnt, (int64_t)g.size()});
+            for (int i = 0; i < cnt; i++) g.push_back(offset[idx[q0 + i]]);
+            for (int i = 0; i < cnt; i++) g.push_back(p->toffs[idx[q0 + i]]);
+            wsmax = std::max(wsmax, grad_ws_doubles(m, cnt));
+        }
+        gi = gj;
+    }
+    p->nodd = (int64_t)odd.size();
+    p->odd_at = (int64_t)g.size();
+    for (int64_t b : odd) g.push_back(offset[b]);
+    for (int64_t b : odd) g.push_back(n[b]);
+    // compact control: ns | offs | items | toffs
+    std::vector<int64_t> h(4 * batch);
+    std::copy(n, n + batch, h.begin());
+    std::copy(offset, offset + batch, h.begin() + batch);
+    std::copy(small.begin(), small.end(), h.begin() + 2 * batch);
+    std::copy(p->toffs.begin(), p->toffs.end(), h.begin() + 3 * batch);
+    bool ok = hipMalloc((void**)&p->dctl, sizeof(int64_t) * h.size()) == hipSuccess &&
+              hipMalloc((void**)&p->dgrp, sizeof(int64_t) * std::max<size_t>(g.size(), 1)) == hipSuccess &&
+              hipMalloc((void**)&p->ctl, 256) == hipSuccess &&
+              hipMalloc((void**)&p->dte, sizeof(double) * (size_t)(tt + 16)) == hipSuccess &&
+              hipMalloc((void**)&p->ws, sizeof(double) * std::max<size_t>(wsmax, 1)) == hipSuccess;
+    if (!ok) {
+        grad_plan_free(p);
+        return set_err(SKH_ERR_HIP, "hipMalloc of the gradient plan's buffers failed (%zu doubles of workspace)", wsmax);
+    }
+    if (hipMemcpy(p->dctl, h.data(), sizeof(int64_t) * h.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        (!g.empty() && hipMemcpy(p->dgrp, g.data(), sizeof(int64_t) * g.size(), hipMemcpyHostToDevice) != hipSuccess)) {
+        grad_plan_free(p);
+        return set_err(SKH_ERR_HIP, "upload of the gradient plan's control arrays failed");
+    }
+    *out = p;
+    return 0;
+}
+
+// Enqueue one evaluation on st.  Members with even n > CMP_NMAX (device entry only) are factored
+// first by the large engine, in place in dgrad.
+int grad_enqueue(const GradPlan& p, const double* dA, double* dlogpf, double* dsgn, double* dgrad, int grid,
+                 hipStream_t st)
+{
+    for (int64_t b : p.big) {
+        const int64_t n = p.ns[b];
+        double* M = dgrad + p.offs[b];
+        double* te = p.dte + p.toffs[b];
+        HIPCHK(hipMemcpyAsync(M, dA + p.offs[b], (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        g_timer.reset();
+        g_timer.on = false;
+        g_timer.phases = false;
+        int rc = sytrd_large(n, M, n, te, te + n, pick_nb(n, nullptr), st);
+        if (rc) return rc;
+        HIPCHK((hipError_t)skh_internal_logpf_from(te + n, te, n, dlogpf + b, dsgn + b, st));
+    }
+    if (p.count > 0) {
+        HIPCHK((hipError_t)skh_internal_zero_u32(p.ctl, 64, st));
+        CompactArgs a;
+        a.A = dA;
+        a.ns = p.dctl;
+        a.offs = p.dctl + p.batch;
+        a.items = p.dctl + 2 * p.batch;
+        a.count = p.count;
+        a.slots = nullptr;
+        a.slot_stride = 0;
+        a.queue = p.ctl;
+        a.err = (int*)(p.ctl + 32);
+        a.budget = (unsigned long long)(watchdog_seconds() * 1e8);
+        a.logpf = dlogpf;
+        a.sgn = dsgn;
+        a.fac = dgrad;
+        a.taueta = p.dte;
+        a.toffs = p.dctl + 3 * p.batch;
+        HIPCHK((hipError_t)skh_internal_launch_compact(&a, p.kmax, grid, st));
+    }
+    if (p.nodd > 0)
+        HIPCHK((hipError_t)skh_internal_grad_zero(dgrad, p.dgrp + p.odd_at, p.dgrp + p.odd_at + p.nodd, (int)p.nodd, st));
+    const int64_t NB = GRAD_NB;
+    for (const GradChunk& c : p.chunks) {
+        const int64_t n = c.n, nn = n * n, h = n / 2;
+        const int cnt = c.cnt;
+        const int64_t* goff = p.dgrp + c.at;
+        const int64_t* toff = goff + cnt;
+        double* Q = p.ws;
+        double* K = Q + (int64_t)cnt * nn;
+        double* Zt = K + (int64_t)cnt * nn;
+        double* Y = Zt + (int64_t)cnt * n * h;
+        double* Z = Y + (int64_t)cnt * NB * n;
+        double* V = Z + (int64_t)cnt * NB * n;
+        double* Gm = V + (int64_t)cnt * n * NB;
+        double* T = Gm + (int64_t)cnt * NB * NB;
+        const int64_t vstride = n * NB, gstride = NB * NB, ystride = NB * n;
+        HIPCHK((hipError_t)skh_internal_grad_eye(Q, nn, n, cnt, st));
+        const int64_t nref = n - 1;
+        for (int64_t c0 = ((nref - 1) / NB) * NB; c0 >= 0; c0 -= NB) {
+            const int nbp = (int)std::min<int64_t>(NB, nref - c0);
+            const int64_t m = n - c0 - 1;
+            double* Q22 = Q + (c0 + 1) + (c0 + 1) * n;
+            HIPCHK((hipError_t)skh_internal_grad_form_v(dgrad, goff, n, c0, nbp, V, vstride, cnt, st));
+            HIPCHK((hipError_t)skh_internal_bgemm(1, 0, nbp, nbp, m, 1.0, V, m, vstride, V, m, vstride, 0.0, Gm, nbp,
+                                                  gstride, cnt, st));
+            HIPCHK((hipError_t)skh_internal_larft_batched(Gm, gstride, p.dte, toff, c0, nbp, T, gstride, cnt, st));
+            HIPCHK((hipError_t)skh_internal_bgemm(1, 0, nbp, m, m, 1.0, V, m, vstride, Q22, n, nn, 0.0, Y, nbp, ystride,
+                                                  cnt, st));
+            HIPCHK((hipError_t)skh_internal_bgemm(0, 0, nbp, m, nbp, 1.0, T, nbp, gstride, Y, nbp, ystride, 0.0, Z, nbp,
+                                                  ystride, cnt, st));
+            HIPCHK((hipError_t)skh_internal_bgemm(0, 0, m, m, nbp, -1.0, V, m, vstride, Z, nbp, ystride, 1.0, Q22, n, nn,
+                                                  cnt, st));
+        }
+        // Zt = (C^{-1} Q_O^T)^T, K = Q_E Zt^T, grad = -1/2 (K - K^T)
+        HIPCHK((hipError_t)skh_internal_grad_bidiag(Q, nn, p.dte, toff, n, Zt, n * h, cnt, st));
+        HIPCHK((hipError_t)skh_internal_bgemm(0, 1, n, n, h, 1.0, Q + n, 2 * n, nn, Zt, n, n * h, 0.0, K, n, nn, cnt,
+                                              st));
+        HIPCHK((hipError_t)skh_internal_grad_skew_out(K, nn, dgrad, goff, n, -0.5, cnt, st));
+    }
+    return 0;
+}
+
+// the device entry's plan, rebuilt when the batch's sizes or offsets change
+GradPlan* g_gradplan = nullptr;
 
 int grad_run(int64_t batch, const int64_t* ns, const int64_t* offs, const double* dA, double* dlogpf, double* dsgn,
              double* dgrad, const skh_opts* opts, hipStream_t st)
 {
+    (void)opts;
     if (batch == 0) return 0;
-    // tau|eta offsets
-    std::vector<int64_t> toffs(batch);
-    int64_t tt = 0;
-    for (int64_t b = 0; b < batch; b++) {
-        toffs[b] = tt;
-        tt += 2 * std::max<int64_t>(ns[b], 1);
+    GradPlan* p = g_gradplan;
+    if (!p || p->batch != batch || !std::equal(ns, ns + batch, p->ns.begin()) ||
+        !std::equal(offs, offs + batch, p->offs.begin())) {
+        if (p) {
+            HIPCHK(hipStreamSynchronize(st));   // the previous plan's work may still be running on st
+            grad_plan_free(p);
+            g_gradplan = nullptr;
+        }
+        int rc = grad_plan_build(batch, ns, offs, true, &p);
+        if (rc) return rc;
+        g_gradplan = p;
     }
-    double* dte = (double*)g_gte.get((size_t)(tt + 2 * batch + 16) * sizeof(double));
-    if (!dte) return set_err(SKH_ERR_HIP, "hipMalloc failed (tau/eta)");
-    int rc = batched_run(batch, ns, offs, dA, dlogpf, dsgn, opts, st, dgrad, dte, toffs.data());
+    int rc = grad_enqueue(*p, dA, dlogpf, dsgn, dgrad, p->grid, st);
     if (rc) return rc;
-    rc = blas_on(st);
-    if (rc) return rc;
-
-    // group by size
-    std::vector<int64_t> idx(batch);
-    std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return ns[a] < ns[b]; });
-    const int64_t NB = 32;
-    size_t gi = 0;
-    while (gi < idx.size()) {
-        const int64_t n = ns[idx[gi]];
-        size_t gj = gi;
-        while (gj < idx.size() && ns[idx[gj]] == n) gj++;
-        if (n % 2 == 1 || n == 0) {
-            for (size_t q = gi; q < gj; q++)
-                if (n > 0) HIPCHK(hipMemsetAsync(dgrad + offs[idx[q]], 0, n * n * sizeof(double), st));
-            gi = gj;
-            continue;
-        }
-        // chunk so that every buffer stays below ~1.5 GB
-        const int64_t per = std::max<int64_t>(1, std::min<int64_t>(4096, (int64_t)(1.5e9 / (8.0 * n * n))));
-        for (size_t q0 = gi; q0 < gj; q0 += per) {
-            const int cnt = (int)std::min<int64_t>(per, (int64_t)(gj - q0));
-            const int64_t nn = n * n, h = n / 2;
-            double* Q = (double*)g_gq.get((size_t)cnt * nn * sizeof(double));
-            double* K = (double*)g_gk.get((size_t)cnt * nn * sizeof(double));
-            double* Zt = (double*)g_gz.get((size_t)cnt * (n * h + 2 * NB * n) * sizeof(double));
-            double* V = (double*)g_gv.get((size_t)cnt * (n * NB + 2 * NB * NB) * sizeof(double));
-            void** P = (void**)g_gp.get((size_t)cnt * 3 * sizeof(void*) + (size_t)cnt * sizeof(int64_t));
-            if (!Q || !K || !Zt || !V || !P) return set_err(SKH_ERR_HIP, "hipMalloc failed (gradient buffers)");
-            std::vector<void*> hp(3 * cnt);
-            std::vector<int64_t> htoffs(cnt);
-            for (int i = 0; i < cnt; i++) {
-                const int64_t b = idx[q0 + i];
-                hp[i] = dgrad + offs[b];            // factors in, gradient out
-                hp[cnt + i] = Q + (int64_t)i * nn;
-                hp[2 * cnt + i] = dgrad + offs[b];
-                htoffs[i] = toffs[b];
-            }
-            int64_t* gtoffs = (int64_t*)(P + 3 * cnt);
-            HIPCHK(hipMemcpyAsync(P, hp.data(), 3 * cnt * sizeof(void*), hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(gtoffs, htoffs.data(), cnt * sizeof(int64_t), hipMemcpyHostToDevice, st));
-            double* const* Fp = (double* const*)P;
-            double* const* Qp = (double* const*)(P + cnt);
-            double* const* Gp = (double* const*)(P + 2 * cnt);
-            double* Y = Zt;   // Y and Z (NB x n each) live in the Zt area until the solve
-            double* Z = Zt + (int64_t)cnt * NB * n;
-            const int64_t vstride = n * NB, gstride = NB * NB;
-            double* Gm = V + (int64_t)cnt * vstride;
-            double* T = Gm + (int64_t)cnt * gstride;
-            HIPCHK((hipError_t)skh_internal_eye_batched((double* const*)Qp, n, cnt, st));
-            const double one = 1.0, zero = 0.0, mone = -1.0;
-            const int64_t nref = n - 1;
-            for (int64_t c0 = ((nref - 1) / NB) * NB; c0 >= 0; c0 -= NB) {
-                const int nbp = (int)std::min<int64_t>(NB, nref - c0);
-                const int64_t m = n - c0 - 1;
-                HIPCHK((hipError_t)skh_internal_form_v_batched((const double* const*)Fp, nullptr, n, c0, nbp, V, vstride,
-                                                              cnt, st));
-                BLASCHK(rocblas_dgemm_strided_batched(g_blas, rocblas_operation_transpose, rocblas_operation_none, nbp,
-                                                      nbp, (int)m, &one, V, (int)m, vstride, V, (int)m, vstride, &zero,
-                                                      Gm, nbp, gstride, cnt));
-                HIPCHK((hipError_t)skh_internal_larft_batched(Gm, gstride, dte, gtoffs, c0, nbp, T, gstride, cnt, st));
-                double* Q22 = Q + (c0 + 1) + (c0 + 1) * n;
-                BLASCHK(rocblas_dgemm_strided_batched(g_blas, rocblas_operation_transpose, rocblas_operation_none, nbp,
-                                                      (int)m, (int)m, &one, V, (int)m, vstride, Q22, (int)n, nn, &zero,
-                                                      Y, nbp, NB * n, cnt));
-                BLASCHK(rocblas_dgemm_strided_batched(g_blas, rocblas_operation_none, rocblas_operation_none, nbp,
-                                                      (int)m, nbp, &one, T, nbp, gstride, Y, nbp, NB * n, &zero, Z, nbp,
-                                                      NB * n, cnt));
-                BLASCHK(rocblas_dgemm_strided_batched(g_blas, rocblas_operation_none, rocblas_operation_none, (int)m,
-                                                      (int)m, nbp, &mone, V, (int)m, vstride, Z, nbp, NB * n, &one, Q22,
-                                                      (int)n, nn, cnt));
-            }
-            // Zt = (C^{-1} Q_O^T)^T, K = Q_E Zt^T, grad = -1/2 (K - K^T)
-            HIPCHK((hipError_t)skh_internal_bidiag_solve((double* const*)Qp, dte, gtoffs, n, Zt, n * h, cnt, st));
-            BLASCHK(rocblas_dgemm_strided_batched(g_blas, rocblas_operation_none, rocblas_operation_transpose, (int)n,
-                                                  (int)n, (int)h, &one, Q + n, (int)(2 * n), nn, Zt, (int)n, n * h, &zero,
-                                                  K, (int)n, nn, cnt));
-            HIPCHK((hipError_t)skh_internal_skew_out(K, nn, (double* const*)Gp, n, -0.5, cnt, st));
-            HIPCHK(hipStreamSynchronize(st));   // host pointer tables are reused by the next chunk
-        }
-        gi = gj;
+    if (!p->big.empty()) {
+        HIPCHK(hipStreamSynchronize(st));
+        rc = check_barrier_flag();
+        if (rc) return rc;
     }
     return 0;
 }
@@ -798,6 +941,7 @@ struct skh_plan_s {
     unsigned* ctl = nullptr;   // [0] queue, [32] error flag
     double flops = 0.0;
     int tuned = 0;
+    GradPlan* g = nullptr;     // gradient plans: everything above except grid / kmax / count lives here
 };
 
 namespace {
@@ -828,6 +972,11 @@ int plan_enqueue(skh_plan p, const double* dA, double* dlogpf, double* dsgn, hip
 void plan_free(skh_plan p)
 {
     if (!p) return;
+    if (p->g) {
+        grad_plan_free(p->g);
+        delete p;
+        return;
+    }
     if (p->dctl) (void)hipFree(p->dctl);
     if (p->slots) (void)hipFree(p->slots);
     if (p->ctl) (void)hipFree(p->ctl);
@@ -1167,12 +1316,13 @@ int skh_finalize(void)
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_inited) return 0;
     (void)hipStreamSynchronize(g_stream);
-    for (Buf* b : {&g_mat, &g_ws, &g_vec, &g_ctl, &g_bin, &g_bout, &g_q, &g_ev, &g_gq, &g_gk, &g_gz, &g_gv, &g_gp,
-                   &g_gte, &g_cslot, &g_big, &g_bctl, &g_zws, &g_zq, &g_zc, &g_zx, &g_zpslot, &g_zpctl, &g_zpin,
+    for (Buf* b : {&g_mat, &g_ws, &g_vec, &g_ctl, &g_bin, &g_bout, &g_q, &g_ev, &g_cslot, &g_big, &g_bctl, &g_zws, &g_zq, &g_zc, &g_zx, &g_zpslot, &g_zpctl, &g_zpin,
                    &g_zpout, &g_chctl, &g_chin, &g_chout, &g_chp,
                    &g_chws, &g_rotq, &g_rotk, &g_rotcs, &g_rotseg, &g_fvec, &g_fpiv, &g_fz, &g_stq})
         b->release();
     g_timer.release();
+    grad_plan_free(g_gradplan);
+    g_gradplan = nullptr;
     if (g_blas) (void)rocblas_destroy_handle(g_blas);
     g_blas = nullptr;
     (void)hipStreamDestroy(g_stream);
@@ -1703,8 +1853,49 @@ int skh_logpf_batched_plan_exec(skh_plan plan, const double* dA, double* dlogpf,
 {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!plan) return set_err(-1, "plan is NULL");
+    if (plan->g) return set_err(-1, "a gradient plan runs through skh_logpf_grad_batched_plan_exec");
     if (!dA || !dlogpf || !dsgn) return set_err(-2, "NULL device pointer");
     return plan_enqueue(plan, dA, dlogpf, dsgn, (hipStream_t)stream, plan->grid, plan->kmax);
+}
+
+int skh_logpf_grad_batched_plan_create(int64_t batch, const int64_t* n, const int64_t* offset, const skh_opts* opts,
+                                       skh_plan* plan)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (batch < 1) return set_err(-1, "batch must be >= 1");
+    if (!n || !offset) return set_err(-2, "NULL argument");
+    if (!plan) return set_err(-5, "plan is NULL");
+    *plan = nullptr;
+    for (int64_t b = 0; b < batch; b++) {
+        if (n[b] < 0) return set_err(-2, "n[%lld] < 0", (long long)b);
+        if (offset[b] < 0 || (offset[b] & 1)) return set_err(-3, "offset[%lld] must be >= 0 and even", (long long)b);
+    }
+    (void)opts;
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    GradPlan* g = nullptr;
+    rc = grad_plan_build(batch, n, offset, false, &g);
+    if (rc) return rc;
+    skh_plan p = new skh_plan_s();
+    p->g = g;
+    p->batch = batch;
+    p->count = g->count;
+    p->kmax = g->kmax;
+    p->grid = g->grid;
+    p->ctl = g->ctl;
+    p->flops = g->flops;
+    *plan = p;
+    return 0;
+}
+
+int skh_logpf_grad_batched_plan_exec(skh_plan plan, const double* dA, double* dlogpf, double* dsgn, double* dgrad,
+                                     void* stream)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!plan || !plan->g) return set_err(-1, "plan is NULL or not a gradient plan");
+    if (!dA || !dlogpf || !dsgn || !dgrad) return set_err(-2, "NULL device pointer");
+    if ((((uintptr_t)dA) | ((uintptr_t)dgrad)) & 15) return set_err(-2, "dA and dgrad must be 16-byte aligned");
+    return grad_enqueue(*plan->g, dA, dlogpf, dsgn, dgrad, plan->grid, (hipStream_t)stream);
 }
 
 int skh_logpf_batched_plan_tune(skh_plan plan, const double* dA, double* dlogpf, double* dsgn, void* stream,
@@ -1712,6 +1903,7 @@ int skh_logpf_batched_plan_tune(skh_plan plan, const double* dA, double* dlogpf,
 {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!plan) return set_err(-1, "plan is NULL");
+    if (plan->g) return set_err(-1, "gradient plans are not tuned by this call");
     if (!dA || !dlogpf || !dsgn) return set_err(-2, "NULL device pointer");
     hipStream_t st = (hipStream_t)stream;
     int per_cu = 0;

@@ -135,5 +135,19 @@ int skh_internal_bidiag_solve(double* const* Q, const double* taueta, const int6
                               int64_t zstride, int count, hipStream_t st);
 int skh_internal_skew_out(const double* K, int64_t kstride, double* const* G, int64_t n, double alpha, int count,
                           hipStream_t st);
+// backward pass of the batched log|Pf| gradient (skh_grad.hip): batched FP64 MFMA GEMM
+// C_z = alpha op(A_z) op(B_z) + beta C_z (X_z = X + z sX, ta / tb: transpose), and the helpers on
+// members at base + goff[z] (factors in, gradient out) with strided workspaces
+int skh_internal_bgemm(int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+                       int64_t sA, const double* B, int64_t ldb, int64_t sB, double beta, double* C, int64_t ldc,
+                       int64_t sC, int count, hipStream_t st);
+int skh_internal_grad_eye(double* Q, int64_t qstride, int64_t n, int count, hipStream_t st);
+int skh_internal_grad_form_v(const double* F, const int64_t* goff, int64_t n, int64_t c0, int nbp, double* V,
+                             int64_t vstride, int count, hipStream_t st);
+int skh_internal_grad_bidiag(const double* Q, int64_t qstride, const double* taueta, const int64_t* toff, int64_t n,
+                             double* Zt, int64_t zstride, int count, hipStream_t st);
+int skh_internal_grad_skew_out(const double* K, int64_t kstride, double* G, const int64_t* goff, int64_t n,
+                               double alpha, int count, hipStream_t st);
+int skh_internal_grad_zero(double* G, const int64_t* goff, const int64_t* gn, int count, hipStream_t st);
 }
 

@@ -31,7 +31,7 @@ __all__ = [
     "JMatrix", "SkewCholesky", "RowMaximum", "NoPivot", "skewchol", "skewchol_", "skewchol_batched", "ldiv", "rdiv",
     "svd", "svdvals", "SVD", "schur", "Schur", "eigvecs", "exp", "log", "cis", "cos", "sin", "sincos", "sinh", "cosh", "tan", "tanh", "cot",
     "coth", "DomainError", "SingularException",
-    "ENGINE_AUTO", "ENGINE_LARGE", "ENGINE_TEAM1", "version", "LogPfPlan",
+    "ENGINE_AUTO", "ENGINE_LARGE", "ENGINE_TEAM1", "version", "LogPfPlan", "LogPfGradPlan",
 ]
 
 
@@ -1171,6 +1171,34 @@ class LogPfPlan:
             self.close()
         except Exception:
             pass
+
+
+class LogPfGradPlan(LogPfPlan):
+    """A batched log|Pf| + gradient plan over device-resident matrices (C ABI
+    skh_logpf_grad_batched_plan_*): `run(dA, dlogpf, dsgn, dgrad, stream)` enqueues the forward
+    factorization and the hand-written backward pass (-1/2 A^{-1} into dgrad, the input's packing)
+    without host synchronisation, allocation, host copies or memset, so it can be captured in a HIP
+    graph.  No reference counterpart (the reference has no AD code; its CPU route to the gradient is
+    inv(A::SkewHermitian), src/skewhermitian.jl:233)."""
+
+    def __init__(self, ns, offsets):
+        self._ns = np.ascontiguousarray(ns, dtype=np.int64)
+        self._offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        if self._ns.shape != self._offs.shape or self._ns.ndim != 1:
+            raise ArgumentError("ns and offsets must be 1-D arrays of the same length")
+        h = ctypes.c_void_p()
+        o = _lib.opts()
+        _lib.check("skh_logpf_grad_batched_plan_create",
+                   _lib.lib().skh_logpf_grad_batched_plan_create(len(self._ns), _ptr(self._ns), _ptr(self._offs),
+                                                                 ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+
+    def run(self, dA, dlogpf, dsgn, dgrad, stream=None):
+        _lib.check("skh_logpf_grad_batched_plan_exec",
+                   _lib.lib().skh_logpf_grad_batched_plan_exec(self._h, dA, dlogpf, dsgn, dgrad, stream))
+
+    def tune(self, *a, **k):
+        raise NotImplementedError("gradient plans are not tuned")
 
 
 def _make_autograd():

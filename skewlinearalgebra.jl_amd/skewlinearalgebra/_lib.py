@@ -72,6 +72,8 @@ def lib():
         "skh_logpf_batched_plan_create": ([i64, vp, vp, op, ctypes.POINTER(vp)], ctypes.c_int),
         "skh_logpf_batched_plan_exec": ([vp, vp, vp, vp, vp], ctypes.c_int),
         "skh_logpf_batched_plan_tune": ([vp, vp, vp, vp, vp, _dp], ctypes.c_int),
+        "skh_logpf_grad_batched_plan_create": ([i64, vp, vp, op, ctypes.POINTER(vp)], ctypes.c_int),
+        "skh_logpf_grad_batched_plan_exec": ([vp, vp, vp, vp, vp, vp], ctypes.c_int),
         "skh_plan_status": ([vp, vp], ctypes.c_int),
         "skh_plan_info": ([vp, vp], ctypes.c_int),
         "skh_plan_destroy": ([vp], ctypes.c_int),
