@@ -231,9 +231,10 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
     HIPCHK((hipError_t)skh_internal_panel_occupancy((int)nb, &per_cu0));
     // row-partial slots of one matvec strip: at most one per workgroup of the grid
     const int64_t nseg = (int64_t)std::max(1, std::min(per_cu0, 2)) * g_num_cu + 2;
-    // W | Pnorm x2 | PS x2 | X x2 | Tv | Prow | Pcol (Tv, Prow, Pcol contiguous: cleared per launch)
-    size_t doubles = (size_t)(ld * nb + 2 * nt + 2 * nt * 2 * skh::NBMAX + 2 * ld + 2 * skh::NBMAX + nseg * ld +
-                              nt * ld + 64);
+    // W | Pnorm x2 | PS x2 | X x2 | (Tv | Prow | Pcol) x2: the hand-over buffers of the two column
+    // parities, each contiguous (the launch's first column is filled with the empty marker)
+    const int64_t hsize = 2 * skh::NBMAX + nseg * ld + nt * ld;
+    size_t doubles = (size_t)(ld * nb + 2 * nt + 2 * nt * 2 * skh::NBMAX + 2 * ld + 2 * hsize + 64);
     double* ws;
     if (ws_ext) {
         if (doubles > ws_cap)
@@ -257,6 +258,7 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
     w.Prow = w.Tv + 2 * skh::NBMAX;
     w.ldp = ld;
     w.Pcol = w.Prow + nseg * ld;
+    w.hstride = hsize;
     const int64_t nslots = nseg + nt;   // Prow and Pcol slots, ld doubles each, after Tv
 
     int per_cu = 0;
@@ -311,8 +313,15 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
             pl.multi = (int64_t)pl.grid < blocks ? 1 : 0;
         }
         pl.ph = g_timer.phases ? ctl.ph : nullptr;
-        // tagged hand-over values of the launch start untagged: t and rows >= c0 of the partials
-        HIPCHK((hipError_t)skh_internal_clear_partials(w.Tv, 2 * skh::NBMAX, w.Prow, ld, c0, n, nslots, st));
+        // the hand-over buffers read for the launch's first column start empty (t and rows >= c0 of the
+        // partials, parity c0), and so does t of its second column (the partials of parity c0 + 1 are
+        // emptied by the first column's producers)
+        {
+            double* h0 = w.Tv + (c0 & 1) * hsize;
+            double* h1 = w.Tv + ((c0 + 1) & 1) * hsize;
+            HIPCHK((hipError_t)skh_internal_clear_partials(h0, h1, 2 * skh::NBMAX, h0 + 2 * skh::NBMAX, ld, c0, n, nslots,
+                                                           st));
+        }
         const bool tm = g_timer.on;
         if (tm) HIPCHK((hipError_t)g_timer.begin(st));
         HIPCHK((hipError_t)skh_internal_launch_panel(&pl, st));
@@ -929,6 +938,9 @@ int sytrd_device_locked(int64_t n, double* dA, int64_t lda, double* dtau, double
 // Batched log|Pf| plans: sizes fixed at creation; exec enqueues without allocating,
 // synchronising or copying from the host, so it can be captured into a hipGraph.
 // ----------------------------------------------------------------------------------------
+// grids of up to this many workgroups per CU are timed by the plan tuner
+constexpr int PLAN_TUNE_PER_CU = 4;
+
 struct skh_plan_s {
     int64_t batch = 0;
     int64_t count = 0;
@@ -1833,7 +1845,9 @@ int skh_logpf_batched_plan_create(int64_t batch, const int64_t* n, const int64_t
     std::copy(offset, offset + batch, h.begin() + batch);
     std::copy(order.begin(), order.end(), h.begin() + 2 * batch);
     for (int64_t b = 0; b < batch; b++) p->flops += 4.0 / 3.0 * (double)n[b] * (double)n[b] * (double)n[b];
-    const int64_t max_grid = (int64_t)per_cu * g_num_cu;
+    // work slots for the largest grid the tuner tries (a persistent queue kernel: workgroups beyond the
+    // resident ones start when others finish and leave once the queue is empty)
+    const int64_t max_grid = (int64_t)std::max(per_cu, PLAN_TUNE_PER_CU) * g_num_cu;
     if (hipMalloc((void**)&p->dctl, sizeof(int64_t) * 3 * batch) != hipSuccess ||
         hipMalloc((void**)&p->slots, sizeof(double) * (size_t)p->slot * (size_t)std::min<int64_t>(max_grid, batch)) !=
             hipSuccess ||
@@ -1915,7 +1929,7 @@ int skh_logpf_batched_plan_tune(skh_plan plan, const double* dA, double* dlogpf,
         if (k < plan->nmax) continue;
         int pc = 0;
         HIPCHK((hipError_t)skh_internal_compact_occupancy(k, &pc));
-        pc = std::max(1, pc);
+        pc = std::max(PLAN_TUNE_PER_CU, pc);
         for (int per = 1; per <= pc; per++) {
             const int g = (int)std::min<int64_t>((int64_t)per * g_num_cu, plan->count);
             bool dup = false;

@@ -34,6 +34,7 @@ struct Work {
     double* Prow;    // [nt][ldp]       row partials of the skew matvec (one slot per rank per strip)
     double* Pcol;    // [nt][ldp]       column partials of the skew matvec
     int64_t ldp;
+    int64_t hstride; // Tv, Prow, Pcol of the other column parity at + hstride (the hand-over values)
     // Pnorm, PS and X hold one copy per column parity: the row pass of column j + 1 writes them
     // while a slower workgroup may still read the copy of column j in its phase B
     int64_t pstride; // Pnorm of the other parity at Pnorm + pstride (nq entries)
@@ -45,6 +46,9 @@ struct Work {
 __device__ __forceinline__ double* pnorm_of(const Work& w, int64_t j) { return w.Pnorm + (j & 1) * w.pstride; }
 __device__ __forceinline__ double* ps_of(const Work& w, int64_t j) { return w.PS + (j & 1) * w.sstride; }
 __device__ __forceinline__ double* x_of(const Work& w, int64_t j) { return w.X + (j & 1) * w.xstride; }
+__device__ __forceinline__ double* tv_of(const Work& w, int64_t j) { return w.Tv + (j & 1) * w.hstride; }
+__device__ __forceinline__ double* prow_of(const Work& w, int64_t j) { return w.Prow + (j & 1) * w.hstride; }
+__device__ __forceinline__ double* pcol_of(const Work& w, int64_t j) { return w.Pcol + (j & 1) * w.hstride; }
 
 // Per-column scalars kept in LDS across the phases.
 struct ColScal {
@@ -211,27 +215,23 @@ __device__ __forceinline__ void st_co(double* p, double v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Values phase B of column j hands to the next row pass (the matvec partial sums and t) carry the
-// tag of column j in their three low mantissa bits (1..4; 0 = not yet written in this launch,
-// the buffers are zero-filled before every launch).  The row-block owners poll these values
-// instead of waiting at a grid barrier.  Within a launch every value a row pass reads is
-// rewritten at every column, so a slot holds the tag of column j or of j - 1.  The tag changes a
-// value by at most 7 units in the last place (1.6e-15 relative), the size of one rounding of
-// the sums it enters; readers clear the tag bits before use.
-__device__ __forceinline__ unsigned col_tag(int64_t j) { return (unsigned)(j & 3) + 1u; }
-__device__ __forceinline__ double with_tag(double x, unsigned t)
+// Values phase B of column j hands to the next row pass (the matvec partial sums and t) go to the
+// hand-over buffers of column j's parity, exact.  The row-block owners poll them instead of
+// waiting at a grid barrier, so an empty slot must be told from a written one: empty slots hold
+// the signalling NaN EMPTY, a bit pattern no arithmetic result has (FP64 arithmetic returns quiet
+// NaNs).  Column j's producer stores EMPTY into the same slots of the other parity (t with its
+// value, the partials at the start of phase B): their values (column j - 1) were consumed before
+// the barrier that started column j, and column j + 1 writes them only after the next barrier, so a
+// reader sees EMPTY or the value of its own column.  The buffers of the launch's first column (and
+// t of its second) are filled with EMPTY before it.
+constexpr unsigned long long EMPTY_BITS = 0x7FF0DEADBEEF0001ull;
+__device__ __forceinline__ double empty_slot() { return __builtin_bit_cast(double, EMPTY_BITS); }
+__device__ __forceinline__ bool filled(double x) { return __builtin_bit_cast(unsigned long long, x) != EMPTY_BITS; }
+// store column j's hand-over value at p (a slot of parity j) and EMPTY at the slot of the other parity
+__device__ __forceinline__ void handover(double* p, int64_t hstride, int64_t j, double x)
 {
-    return __builtin_bit_cast(double, (__builtin_bit_cast(unsigned long long, x) & ~7ull) | t);
-}
-__device__ __forceinline__ bool has_tag(double x, unsigned t)
-{
-    return (__builtin_bit_cast(unsigned long long, x) & 7ull) == t;
-}
-// the value with its tag bits cleared: a tagged zero reads back as exactly zero (the reflector
-// conventions for zero columns depend on it)
-__device__ __forceinline__ double untag(double x)
-{
-    return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, x) & ~7ull);
+    st_co(p, x);
+    st_co(p + ((j & 1) ? -hstride : hstride), empty_slot());
 }
 
 // ---------------------------------------------------------------------------------------
@@ -491,19 +491,18 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
             // t, the special row's column partials and this row's matvec partials, written by
             // phase B of column j-1 with its tag: each wave polls its share until all have arrived
             // (32-bit element offsets: (slots + strips) * ldp stays far below 2^31)
-            const unsigned tg1 = col_tag(jm);
             const int S = (int)((b * TB) / MR);    // matvec strip holding this chunk
             const int nslot = rowC ? (int)(own ? pm.nslot_own : tall_slots(tg, S, G)) : 0;
             const int nts = (int)tg.nts, ldp = (int)w.ldp, rr32 = (int)r;
             const int S0 = (int)(j / MR), j32 = (int)j;
+            const double* prowm = prow_of(w, jm);
+            const double* pcolm = pcol_of(w, jm);
             for (unsigned spins = 0;; spins++) {
                 bool ok = true;
                 if (first && tid < im) {
-                    tv1 = ld_co(w.Tv + tid);
-                    tv2 = ld_co(w.Tv + NBMAX + tid);
-                    ok = has_tag(tv1, tg1) && has_tag(tv2, tg1);
-                    tv1 = untag(tv1);
-                    tv2 = untag(tv2);
+                    tv1 = ld_co(tv_of(w, jm) + tid);
+                    tv2 = ld_co(tv_of(w, jm) + NBMAX + tid);
+                    ok = filled(tv1) && filled(tv2);
                 }
                 if (first && wave == 0) {
                     // special row r = j of w_{j-1}: no stored entry of row j lies in the block, so
@@ -511,15 +510,14 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
                         const int I = S0 + lane + 64 * u;
-                        sp[u] = (I < nts) ? ld_co(w.Pcol + (I * ldp + j32)) : 0.0;
-                        ok = ok && (I >= nts || has_tag(sp[u], tg1));
-                        sp[u] = untag(sp[u]);
+                        sp[u] = (I < nts) ? ld_co(pcolm + (I * ldp + j32)) : 0.0;
+                        ok = ok && filled(sp[u]);
                     }
                     sprow = 0.0;
                     for (int I = S0 + lane + 256; I < nts; I += 64) {
-                        const double x = ld_co(w.Pcol + (I * ldp + j32));
-                        ok = ok && has_tag(x, tg1);
-                        sprow += untag(x);
+                        const double x = ld_co(pcolm + (I * ldp + j32));
+                        ok = ok && filled(x);
+                        sprow += x;
                     }
                 }
                 if (rowC) {
@@ -528,9 +526,8 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                         double sb[8];
 #pragma unroll
                         for (int u = 0; u < 8; u++) {
-                            sb[u] = (s0 + 4 * u < nslot) ? ld_co(w.Prow + ((s0 + 4 * u) * ldp + rr32)) : 0.0;
-                            ok = ok && (s0 + 4 * u >= nslot || has_tag(sb[u], tg1));
-                            sb[u] = untag(sb[u]);
+                            sb[u] = (s0 + 4 * u < nslot) ? ld_co(prowm + ((s0 + 4 * u) * ldp + rr32)) : 0.0;
+                            ok = ok && filled(sb[u]);
                         }
                         pp += ((sb[0] + sb[1]) + (sb[2] + sb[3])) + ((sb[4] + sb[5]) + (sb[6] + sb[7]));
                     }
@@ -539,9 +536,8 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
 #pragma unroll
                         for (int u = 0; u < PB; u++) {
                             const int I = I0 + 4 * u;
-                            buf[u] = (I < nts) ? ld_co(w.Pcol + (I * ldp + rr32)) : 0.0;
-                            ok = ok && (I >= nts || has_tag(buf[u], tg1));
-                            buf[u] = untag(buf[u]);
+                            buf[u] = (I < nts) ? ld_co(pcolm + (I * ldp + rr32)) : 0.0;
+                            ok = ok && filled(buf[u]);
                         }
                         double p4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -758,6 +754,33 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
     clk_inner(clk, 11);
     if (!grid_wait_co(bar, epoch, err, s_ok)) return false;
     clk_inner(clk, 12);
+    // EMPTY into this rank's row and column partial slots of the other parity (column j - 1's values,
+    // consumed before this column's barrier; column j + 1 refills them after the next one), issued
+    // here so that the stores complete behind the matvec
+    if (mv) {
+        double* prn = prow_of(w, j + 1);
+        double* pcn = pcol_of(w, j + 1);
+        const double e = empty_slot();
+        int64_t It = I, Jt = J, slot = cur_slot;
+        for (int64_t t = tlo; t < thi; t++) {
+            if (wave == 0 && lane < MC) {
+                const int64_t cc = Jt * MC + lane;
+                if (cc > j && cc < n) st_co(pcn + It * w.ldp + cc, e);
+            }
+            const bool lis = (Jt == MPS * It + MPS - 1);
+            if (lis || t + 1 == thi) {
+                const int64_t r = It * MR + tid;
+                if (r < n) st_co(prn + slot * w.ldp + r, e);
+                slot = 0;
+            }
+            if (lis) {
+                It++;
+                Jt = tg.Jlo;
+            } else {
+                Jt++;
+            }
+        }
+    }
     // ---- loads of what the row pass produced ----
     double pn0 = 0.0, pn1 = 0.0, araw = 0.0;
     if (wave == 0) {
@@ -811,7 +834,7 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         }
         for (int64_t q = lane + 128; q < nq; q += 64) acc += ld_co(ps_of(w, j) + q * (2 * NBMAX) + off + k);
         acc = wave_sum(acc);
-        if (lane == 0) st_co(w.Tv + off + k, with_tag(s.unit * head + s.scale * acc, col_tag(j)));
+        if (lane == 0) handover(tv_of(w, j) + off + k, w.hstride, j, s.unit * head + s.scale * acc);
     }
     clk_inner(clk, 9);
     // ---- (3) matvec ----
@@ -840,7 +863,7 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
             cp += a[k] * sh.vstrip[32 * (k >> 1) + 2 * p + (k & 1)];
         }
         cp = row16_sum(cp);
-        if (p == 0 && c > j && c < n) st_co(w.Pcol + I * w.ldp + c, with_tag(-cp, col_tag(j)));
+        if (p == 0 && c > j && c < n) st_co(pcol_of(w, j) + I * w.ldp + c, -cp);
         if (!more || In != I) {
             // flush this rank's row sums of strip I: lanes -> waves -> one slot row
 #pragma unroll
@@ -854,8 +877,8 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
             {
                 const int64_t r = I * MR + tid;
                 if (r < n)
-                    st_co(w.Prow + cur_slot * w.ldp + r,
-                          with_tag((sh.mred[0][tid] + sh.mred[1][tid]) + (sh.mred[2][tid] + sh.mred[3][tid]), col_tag(j)));
+                    st_co(prow_of(w, j) + cur_slot * w.ldp + r,
+                          (sh.mred[0][tid] + sh.mred[1][tid]) + (sh.mred[2][tid] + sh.mred[3][tid]));
                 if (more) sh.vstrip[tid] = vfrom(n, j, In * MR + tid, xsn, s);
             }
             __syncthreads();

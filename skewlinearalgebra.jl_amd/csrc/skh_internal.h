@@ -107,8 +107,8 @@ int skh_internal_skewchol_coop_occupancy(int* per_cu);
 int skh_internal_launch_compact(const CompactArgs* a, int nmax, int grid, hipStream_t st);
 int skh_internal_logpf_from(const double* eta, const double* tau, int64_t n, double* logpf, double* sgn, hipStream_t st);
 int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st);
-int skh_internal_clear_partials(double* tv, int64_t ntv, double* slots, int64_t ld, int64_t r0, int64_t n,
-                                int64_t nslots, hipStream_t st);
+int skh_internal_clear_partials(double* tv0, double* tv1, int64_t ntv, double* slots, int64_t ld, int64_t r0,
+                                int64_t n, int64_t nslots, hipStream_t st);
 int skh_internal_panel_occupancy(int nbmax, int* blocks_per_cu);
 int skh_internal_launch_rank2k(double* A, int64_t lda, const double* U, int64_t ldu, int64_t n, int64_t e,
                                int64_t c0, int nbp, hipStream_t st);

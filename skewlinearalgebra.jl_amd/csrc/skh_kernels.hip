@@ -102,14 +102,20 @@ __global__ void k_finish(double* A, int64_t lda, int64_t n, double* tau, double*
     }
 }
 
-// zero t (ntv entries) and rows [r0, n) of nslots slots of ld doubles (the matvec partials)
-__global__ void k_clear_partials(double* tv, int64_t ntv, double* slots, int64_t ld, int64_t r0, int64_t n)
+// mark empty t of both parities (ntv entries at tv0 and tv1) and rows [r0, n) of nslots slots of ld
+// doubles (the matvec partials of the launch's first column)
+__global__ void k_clear_partials(double* tv0, double* tv1, int64_t ntv, double* slots, int64_t ld, int64_t r0,
+                                 int64_t n)
 {
+    const double e = empty_slot();
     if (blockIdx.y == 0 && blockIdx.x == 0)
-        for (int64_t k = threadIdx.x; k < ntv; k += blockDim.x) tv[k] = 0.0;
+        for (int64_t k = threadIdx.x; k < ntv; k += blockDim.x) {
+            tv0[k] = e;
+            tv1[k] = e;
+        }
     double* s = slots + (int64_t)blockIdx.y * ld;
     for (int64_t r = r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
-        s[r] = 0.0;
+        s[r] = e;
 }
 
 __global__ void k_zero_u32(unsigned* p, int count)
@@ -301,13 +307,13 @@ extern "C" int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st)
                                            st);
 }
 
-extern "C" int skh_internal_clear_partials(double* tv, int64_t ntv, double* slots, int64_t ld, int64_t r0, int64_t n,
-                                           int64_t nslots, hipStream_t st)
+extern "C" int skh_internal_clear_partials(double* tv0, double* tv1, int64_t ntv, double* slots, int64_t ld, int64_t r0,
+                                           int64_t n, int64_t nslots, hipStream_t st)
 {
     if (nslots <= 0) return 0;
     const int64_t rows = std::max<int64_t>(1, n - r0);
     const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((rows + 255) / 256, 8));
-    hipLaunchKernelGGL(k_clear_partials, dim3(gx, (unsigned)nslots), dim3(256), 0, st, tv, ntv, slots, ld, r0, n);
+    hipLaunchKernelGGL(k_clear_partials, dim3(gx, (unsigned)nslots), dim3(256), 0, st, tv0, tv1, ntv, slots, ld, r0, n);
     return (int)hipGetLastError();
 }
 

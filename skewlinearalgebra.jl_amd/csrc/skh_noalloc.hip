@@ -16,9 +16,9 @@
 namespace {
 
 // control block in doubles (2048 bytes), then the workspace at n = 8192, nb = 128, 514 row-partial
-// slots: ld nb + 2 nt + 4 nt NBMAX + 2 ld + 2 NBMAX + (nseg + nt) ld + 64 = 6,399,042 doubles
+// slots: ld nb + 2 nt + 4 nt NBMAX + 2 ld + 2 (2 NBMAX + (nseg + nt) ld) + 64 = 11,666,754 doubles
 constexpr size_t kCtlDoubles = 256;
-constexpr size_t kScratchDoubles = 6400000;
+constexpr size_t kScratchDoubles = 11700000;
 
 static __device__ __align__(256) double scratch[kScratchDoubles];
 
