@@ -62,6 +62,7 @@ CSRC = os.path.join(HERE, "skewlinearalgebra.jl_amd", "csrc")
 KERNEL_SOURCES = {
     "k_panel": ("skh_device.hpp", "skh_sync.hpp", "skh_kernels.hip"),
     "k_compact": ("skh_device.hpp", "skh_sync.hpp", "skh_compact.hpp", "skh_compact.hip"),
+    "k_cblock": ("skh_device.hpp", "skh_sync.hpp", "skh_cblock.hpp", "skh_compact.hip"),
 }
 
 
@@ -236,7 +237,7 @@ def batched_bench(torch, skl, _lib, steps):
     plan = skl.LogPfPlan(ns, offs)
     tuned_ms = plan.tune(buf.data_ptr(), lp.data_ptr(), sg.data_ptr(), st)
     plan.status(st)
-    info = plan.info()
+    cfg = plan.config()
     graph = torch.cuda.CUDAGraph()
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
@@ -291,18 +292,20 @@ def batched_bench(torch, skl, _lib, steps):
     worst = max(abs(lp_h[k] - lo) / max(1.0, abs(lo)) for k, (lo, so) in zip(sample, outs))
     signs = all(sg_h[k] == so for k, (lo, so) in zip(sample, outs))
     tflops = flops / (ms * 1e-3) / 1e12
-    traffic, tnote = measured_traffic("k_compact_batch4096_traffic.md", "k_compact")
+    kname = "k_cblock" if cfg["engine"] == "blocked" else "k_compact"
+    traffic, tnote = measured_traffic("%s_batch4096_traffic.md" % kname, kname)
     return {"metric": "batched log|Pf|+sign, 4096 matrices n in {32,...,512}", "value": count / (ms * 1e-3),
             "unit": "matrices/s", "ms_per_batch": ms, "gflops": flops / (ms * 1e-3) / 1e9,
-            "launch": "one tuned plan (%d workgroups, kernel NMAX %d) replayed from a HIP graph; tuner best %.2f ms"
-                      % (info["workgroups"], info["kernel_nmax"], tuned_ms),
+            "launch": "one tuned plan replayed from a HIP graph (%s engine, kernel NMAX %d, panel width %d, %d "
+                      "workgroups); tuner best %.2f ms" % (cfg["engine"], cfg["kernel_nmax"], cfg["panel_width"],
+                                                         cfg["workgroups"], tuned_ms),
             "graph_replays_identical": same,
-            "roofline": {"bound": "mfma", "kernel": "k_compact (one workgroup per matrix)", "achieved": tflops,
+            "roofline": {"bound": "mfma", "kernel": kname, "achieved": tflops,
                          "peak": FP64_MFMA_PEAK_TF, "unit": "TFLOP/s", "frac": tflops / FP64_MFMA_PEAK_TF,
                          "work": "sum (4/3) n^3 = %.4e flop per launch" % flops, "traffic": traffic,
                          "traffic_source": tnote,
-                         "note": "the graph replay (a fill kernel and the k_compact launch) timed with HIP events; "
-                                 "FP64 peak (vector and matrix)"},
+                         "note": "the graph replay (a fill kernel and the one-workgroup-per-matrix launch) timed "
+                                 "with HIP events; FP64 peak (vector and matrix)"},
             "plain": {"value": plain_rate, "unit": "matrices/s",
                       "sample": "per-matrix GPU loop, skh_logpf_d_device per matrix (library engine choice), every "
                                 "16th matrix (%d)" % len(sample), "max_rel_dev_vs_batch": loop_dev},

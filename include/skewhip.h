@@ -38,6 +38,8 @@ extern "C" {
 #define SKH_ENGINE_AUTO 0      /* single-matrix host entries: TEAM1 for even n <= 192, LARGE above */
 #define SKH_ENGINE_LARGE 1     /* cooperative multi-workgroup panel + MFMA rank-2k update */
 #define SKH_ENGINE_TEAM1 2     /* one workgroup per matrix (the batched engine) */
+#define SKH_ENGINE_BLOCKED 3   /* one workgroup per matrix, panels of opts.nb = 8 (n <= 512) or 16 (n <= 256)
+                                  reflectors with an MFMA rank-2k update per panel (0 = 8) */
 #define SKH_FLAG_TIMING 0x100  /* record per-kernel HIP events (read with skh_diag_timings) */
 #define SKH_FLAG_PHASES 0x200  /* also run the panel kernel's phase clock (workgroup 0; slows it) */
 
@@ -127,9 +129,10 @@ int skh_logpf_grad_batched_d_device(int64_t batch, const int64_t *n, const int64
  * every even n <= 1024) and uploaded once; the plan owns its device work slots and control
  * words.  skh_logpf_batched_plan_exec enqueues one evaluation of the whole batch on `stream`
  * (a fill kernel and one persistent launch) without allocating, synchronising or copying from
- * the host, so it can be captured into a hipGraph and replayed.  skh_logpf_batched_plan_tune
- * times the kernel configurations (work-group count, kernel instance) on real data and keeps
- * the fastest (*best_ms may be NULL).  skh_plan_status waits for `stream` and reports a
+ * the host, so it can be captured into a hipGraph and replayed.  An untuned plan takes the blocked
+ * engine (panels of 8 reflectors, MFMA update) when every member has n <= 512, else the unblocked
+ * one.  skh_logpf_batched_plan_tune times the kernel configurations (engine, kernel NMAX, panel
+ * width 8 or 16, 1 to 4 work-groups per CU) on real data and keeps the fastest (*best_ms may be NULL).  skh_plan_status waits for `stream` and reports a
  * watchdog stop (SKH_ERR_BARRIER).  Overlapping executions of one plan race (they share its
  * slots and queue).  skh_plan_info: {work-groups, kernel NMAX, tuned, matrices}.
  */
@@ -152,6 +155,11 @@ int skh_logpf_grad_batched_plan_create(int64_t batch, const int64_t *n, const in
 int skh_logpf_grad_batched_plan_exec(skh_plan plan, const double *dA, double *dlogpf, double *dsgn, double *dgrad,
                                      void *stream);
 int skh_plan_status(skh_plan plan, void *stream);
+/* The plan's launch configuration, out4 = {engine (SKH_ENGINE_TEAM1: unblocked, SKH_ENGINE_BLOCKED), kernel
+ * NMAX, panel width (0 for the unblocked engine), work-groups}; skh_plan_set_config sets it (log|Pf| plans),
+ * e.g. from a table of tuned configurations kept by the caller (the Python mirror keeps one as JSON). */
+int skh_plan_config(skh_plan plan, int64_t *out4);
+int skh_plan_set_config(skh_plan plan, const int64_t *in4);
 int skh_plan_info(skh_plan plan, int64_t *out4);
 int skh_plan_destroy(skh_plan plan);
 

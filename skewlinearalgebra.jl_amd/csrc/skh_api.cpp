@@ -124,6 +124,26 @@ int64_t pick_nb(int64_t n, const skh_opts* o)
 
 // compact engine limits: even n up to this size (one workgroup per matrix)
 constexpr int64_t CMP_NMAX = 1024;
+// blocked one-workgroup engine: even n up to this size; panel width 8 (n <= 512) or 16 (n <= 256)
+constexpr int64_t BLK_NMAX = 512;
+
+// (kernel NMAX, panel width) of the blocked engine for matrices up to nmax, panel width nb (0 = 8)
+bool blocked_instance(int64_t nmax, int64_t nb, int* kmax, int* knb)
+{
+    const int w = nb == 16 ? 16 : 8;
+    if (nb != 0 && nb != 8 && nb != 16) return false;
+    if (nmax <= 256) {
+        *kmax = 256;
+        *knb = w;
+        return true;
+    }
+    if (nmax <= 512) {
+        *kmax = 512;
+        *knb = w;
+        return true;
+    }
+    return false;
+}
 // automatic choice for a single matrix: the compact engine (one workgroup) up to this size, the large
 // engine above.  Measured host-entry calls: n = 192 5.1 ms against 5.6 ms, n = 256 7.9 against 7.5,
 // n = 512 29.6 against 15.0, n = 1024 163 against 30.5 (batches keep the compact engine to n = 1024:
@@ -133,7 +153,7 @@ constexpr int64_t CMP_AUTO = 192;
 int engine_of(int64_t n, const skh_opts* o)
 {
     int64_t f = o ? (o->flags & 0xff) : 0;
-    if (f == SKH_ENGINE_LARGE || f == SKH_ENGINE_TEAM1) return (int)f;
+    if (f == SKH_ENGINE_LARGE || f == SKH_ENGINE_TEAM1 || f == SKH_ENGINE_BLOCKED) return (int)f;
     return (n % 2 == 0 && n <= CMP_AUTO) ? SKH_ENGINE_TEAM1 : SKH_ENGINE_LARGE;
 }
 
@@ -356,7 +376,7 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
 // in the compact launch.  With dfac, each factorization runs in place in dfac (input copied
 // in first; offsets as the input) and tau|eta go to dtaueta + toffs[b].
 // ----------------------------------------------------------------------------------------
-Buf g_cslot, g_big, g_bctl;
+Buf g_cslot, g_big, g_bctl, g_cwbuf;
 
 double watchdog_seconds()
 {
@@ -414,9 +434,18 @@ int batched_run(int64_t batch, const int64_t* ns, const int64_t* offs, const dou
 
     int64_t nmax = 2;
     for (int64_t b : small) nmax = std::max(nmax, ns[b]);
-    const int kmax = nmax <= 512 ? 512 : 1024;
+    // the blocked engine on request, and by default for batches whose members all fit it; the
+    // unblocked one (SKH_ENGINE_TEAM1) on request and for larger members
+    const int64_t eflag = opts ? (opts->flags & 0xff) : 0;
+    const bool blocked = eflag == SKH_ENGINE_BLOCKED || (eflag != SKH_ENGINE_TEAM1 && nmax <= BLK_NMAX);
+    int kmax = nmax <= 512 ? 512 : 1024, knb = 0;
+    if (blocked && !blocked_instance(nmax, eflag == SKH_ENGINE_BLOCKED ? opts->nb : 0, &kmax, &knb))
+        return set_err(SKH_ERR_UNSUPPORTED, "the blocked engine takes even n <= %lld with panel width 8, n <= 256 "
+                       "with width 16 (largest n %lld, nb %lld)", (long long)BLK_NMAX, (long long)nmax,
+                       (long long)opts->nb);
     int per_cu = 0;
-    HIPCHK((hipError_t)skh_internal_compact_occupancy(kmax, &per_cu));
+    if (blocked) HIPCHK((hipError_t)skh_internal_cblock_occupancy(kmax, knb, &per_cu));
+    else HIPCHK((hipError_t)skh_internal_compact_occupancy(kmax, &per_cu));
     per_cu = std::max(1, per_cu);
     const int64_t count = (int64_t)small.size();
     const int grid = (int)std::min<int64_t>((int64_t)per_cu * g_num_cu, count);
@@ -455,7 +484,16 @@ int batched_run(int64_t batch, const int64_t* ns, const int64_t* offs, const dou
     a.fac = dfac;
     a.taueta = dtaueta;
     a.toffs = dtaueta ? dctl + 2 * batch + count : nullptr;
-    HIPCHK((hipError_t)skh_internal_launch_compact(&a, kmax, grid, st));
+    a.wbuf = nullptr;
+    a.wstride = 0;
+    if (blocked) {
+        a.wstride = (int64_t)kmax * knb;
+        a.wbuf = (double*)g_cwbuf.get((size_t)a.wstride * grid * sizeof(double));
+        if (!a.wbuf) return set_err(SKH_ERR_HIP, "hipMalloc of the blocked engine's panel workspaces failed");
+        HIPCHK((hipError_t)skh_internal_launch_cblock(&a, kmax, knb, grid, st));
+    } else {
+        HIPCHK((hipError_t)skh_internal_launch_compact(&a, kmax, grid, st));
+    }
     // the host copy of the control arrays must outlive the async copy; the flag is read after it
     HIPCHK(hipStreamSynchronize(st));
     int herr = 0;
@@ -682,6 +720,8 @@ struct GradPlan {
     std::vector<int64_t> toffs;      // tau | eta offsets (2 max(n, 1) doubles per member)
     int64_t count = 0;               // members of the compact launch
     int grid = 0, kmax = 512;
+    int knb = 0;                     // blocked engine panel width for the forward (0: unblocked engine)
+    double* wbuf = nullptr;          // its W workspaces (kmax * knb doubles per workgroup)
     std::vector<GradChunk> chunks;
     int64_t nodd = 0, odd_at = 0;    // odd members: goff[nodd] | n[nodd] at dgrp + odd_at
     int64_t* dctl = nullptr;         // ns | offs | items | toffs of the compact launch
@@ -695,7 +735,7 @@ struct GradPlan {
 void grad_plan_free(GradPlan* p)
 {
     if (!p) return;
-    for (void* q : {(void*)p->dctl, (void*)p->dgrp, (void*)p->ctl, (void*)p->dte, (void*)p->ws})
+    for (void* q : {(void*)p->dctl, (void*)p->dgrp, (void*)p->ctl, (void*)p->dte, (void*)p->ws, (void*)p->wbuf})
         if (q) (void)hipFree(q);
     delete p;
 }
@@ -741,8 +781,17 @@ int grad_plan_build(int64_t batch, const int64_t* n, const int64_t* offset, bool
     int64_t nmax = 2;
     for (int64_t b : small) nmax = std::max(nmax, n[b]);
     p->kmax = nmax <= 512 ? 512 : 1024;
+    {   // the forward on the blocked engine (panel width 8) when every member fits it
+        int bk = 0, bnb = 0;
+        if (nmax <= BLK_NMAX && blocked_instance(nmax, 0, &bk, &bnb)) {
+            p->kmax = bk;
+            p->knb = bnb;
+        }
+    }
     int per_cu = 0;
-    if (skh_internal_compact_occupancy(p->kmax, &per_cu) != 0) {
+    const int occ = p->knb ? skh_internal_cblock_occupancy(p->kmax, p->knb, &per_cu)
+                           : skh_internal_compact_occupancy(p->kmax, &per_cu);
+    if (occ != 0) {
         grad_plan_free(p);
         return set_err(SKH_ERR_HIP, "occupancy query failed");
     }
@@ -785,7 +834,9 @@ int grad_plan_build(int64_t batch, const int64_t* n, const int64_t* offset, bool
               hipMalloc((void**)&p->dgrp, sizeof(int64_t) * std::max<size_t>(g.size(), 1)) == hipSuccess &&
               hipMalloc((void**)&p->ctl, 256) == hipSuccess &&
               hipMalloc((void**)&p->dte, sizeof(double) * (size_t)(tt + 16)) == hipSuccess &&
-              hipMalloc((void**)&p->ws, sizeof(double) * std::max<size_t>(wsmax, 1)) == hipSuccess;
+              hipMalloc((void**)&p->ws, sizeof(double) * std::max<size_t>(wsmax, 1)) == hipSuccess &&
+              (p->knb == 0 || hipMalloc((void**)&p->wbuf, sizeof(double) * (size_t)p->kmax * p->knb * p->grid) ==
+                                  hipSuccess);
     if (!ok) {
         grad_plan_free(p);
         return set_err(SKH_ERR_HIP, "hipMalloc of the gradient plan's buffers failed (%zu doubles of workspace)", wsmax);
@@ -834,7 +885,10 @@ int grad_enqueue(const GradPlan& p, const double* dA, double* dlogpf, double* ds
         a.fac = dgrad;
         a.taueta = p.dte;
         a.toffs = p.dctl + 3 * p.batch;
-        HIPCHK((hipError_t)skh_internal_launch_compact(&a, p.kmax, grid, st));
+        a.wbuf = p.wbuf;
+        a.wstride = (int64_t)p.kmax * p.knb;
+        if (p.knb > 0) HIPCHK((hipError_t)skh_internal_launch_cblock(&a, p.kmax, p.knb, grid, st));
+        else HIPCHK((hipError_t)skh_internal_launch_compact(&a, p.kmax, grid, st));
     }
     if (p.nodd > 0)
         HIPCHK((hipError_t)skh_internal_grad_zero(dgrad, p.dgrp + p.odd_at, p.dgrp + p.odd_at + p.nodd, (int)p.nodd, st));
@@ -940,6 +994,8 @@ int sytrd_device_locked(int64_t n, double* dA, int64_t lda, double* dtau, double
 // ----------------------------------------------------------------------------------------
 // grids of up to this many workgroups per CU are timed by the plan tuner
 constexpr int PLAN_TUNE_PER_CU = 4;
+// W workspace of one workgroup of the blocked engine, enough for every instance (NMAX x NB)
+constexpr int64_t PLAN_WSTRIDE = 512 * 16;
 
 struct skh_plan_s {
     int64_t batch = 0;
@@ -954,11 +1010,14 @@ struct skh_plan_s {
     double flops = 0.0;
     int tuned = 0;
     GradPlan* g = nullptr;     // gradient plans: everything above except grid / kmax / count lives here
+    int knb = 0;               // blocked engine panel width (0: the unblocked engine)
+    double* wbuf = nullptr;    // blocked engine: W workspace per workgroup (PLAN_WSTRIDE doubles each)
 };
 
 namespace {
 
-int plan_enqueue(skh_plan p, const double* dA, double* dlogpf, double* dsgn, hipStream_t st, int grid, int kmax)
+int plan_enqueue(skh_plan p, const double* dA, double* dlogpf, double* dsgn, hipStream_t st, int grid, int kmax,
+                 int knb)
 {
     HIPCHK((hipError_t)skh_internal_zero_u32(p->ctl, 64, st));
     CompactArgs a;
@@ -977,7 +1036,10 @@ int plan_enqueue(skh_plan p, const double* dA, double* dlogpf, double* dsgn, hip
     a.fac = nullptr;
     a.taueta = nullptr;
     a.toffs = nullptr;
-    HIPCHK((hipError_t)skh_internal_launch_compact(&a, kmax, grid, st));
+    a.wbuf = p->wbuf;
+    a.wstride = PLAN_WSTRIDE;
+    if (knb > 0) HIPCHK((hipError_t)skh_internal_launch_cblock(&a, kmax, knb, grid, st));
+    else HIPCHK((hipError_t)skh_internal_launch_compact(&a, kmax, grid, st));
     return 0;
 }
 
@@ -990,6 +1052,7 @@ void plan_free(skh_plan p)
         return;
     }
     if (p->dctl) (void)hipFree(p->dctl);
+    if (p->wbuf) (void)hipFree(p->wbuf);
     if (p->slots) (void)hipFree(p->slots);
     if (p->ctl) (void)hipFree(p->ctl);
     delete p;
@@ -1328,7 +1391,7 @@ int skh_finalize(void)
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_inited) return 0;
     (void)hipStreamSynchronize(g_stream);
-    for (Buf* b : {&g_mat, &g_ws, &g_vec, &g_ctl, &g_bin, &g_bout, &g_q, &g_ev, &g_cslot, &g_big, &g_bctl, &g_zws, &g_zq, &g_zc, &g_zx, &g_zpslot, &g_zpctl, &g_zpin,
+    for (Buf* b : {&g_mat, &g_ws, &g_vec, &g_ctl, &g_bin, &g_bout, &g_q, &g_ev, &g_cslot, &g_big, &g_bctl, &g_cwbuf, &g_zws, &g_zq, &g_zc, &g_zx, &g_zpslot, &g_zpctl, &g_zpin,
                    &g_zpout, &g_chctl, &g_chin, &g_chout, &g_chp,
                    &g_chws, &g_rotq, &g_rotk, &g_rotcs, &g_rotseg, &g_fvec, &g_fpiv, &g_fz, &g_stq})
         b->release();
@@ -1424,7 +1487,7 @@ int skh_sytrd_d(int64_t n, double* A, int64_t lda, double* tau, double* eta, int
         rc = check_barrier_flag();
         if (rc) return rc;
     } else {
-        // compact engine, in place (ld = n for even n); tau | eta land in dv
+        // one-workgroup engines, in place (ld = n for even n); tau | eta land in dv
         if (n % 2 == 1 || n > CMP_NMAX)
             return set_err(SKH_ERR_UNSUPPORTED, "the compact engine takes even n <= %lld", (long long)CMP_NMAX);
         double* dout = (double*)g_bout.get(2 * sizeof(double));
@@ -1831,12 +1894,19 @@ int skh_logpf_batched_plan_create(int64_t batch, const int64_t* n, const int64_t
     p->count = batch;
     p->nmax = nmax;
     p->kmax = nmax <= 512 ? 512 : 1024;
+    // untuned: the blocked engine (panel width 8) when every member fits it, else the unblocked one
+    int bk = 0, bnb = 0;
+    if (nmax <= BLK_NMAX && blocked_instance(nmax, 0, &bk, &bnb)) {
+        p->kmax = bk;
+        p->knb = bnb;
+    }
     int per_cu = 0;
-    hipError_t e = (hipError_t)skh_internal_compact_occupancy(p->kmax, &per_cu);
+    hipError_t e = p->knb ? (hipError_t)skh_internal_cblock_occupancy(p->kmax, p->knb, &per_cu)
+                          : (hipError_t)skh_internal_compact_occupancy(p->kmax, &per_cu);
     if (e != hipSuccess) { plan_free(p); return set_err(SKH_ERR_HIP, "occupancy query failed"); }
     per_cu = std::max(1, per_cu);
     p->grid = (int)std::min<int64_t>((int64_t)per_cu * g_num_cu, batch);
-    p->slot = ((nmax * nmax + 2 * 1024) + 31) & ~int64_t(31);   // fits either kernel instance
+    p->slot = ((nmax * nmax + 2 * 1024) + 31) & ~int64_t(31);   // fits every kernel instance
     std::vector<int64_t> order(batch);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return n[a] > n[b]; });
@@ -1851,7 +1921,9 @@ int skh_logpf_batched_plan_create(int64_t batch, const int64_t* n, const int64_t
     if (hipMalloc((void**)&p->dctl, sizeof(int64_t) * 3 * batch) != hipSuccess ||
         hipMalloc((void**)&p->slots, sizeof(double) * (size_t)p->slot * (size_t)std::min<int64_t>(max_grid, batch)) !=
             hipSuccess ||
-        hipMalloc((void**)&p->ctl, 256) != hipSuccess) {
+        hipMalloc((void**)&p->ctl, 256) != hipSuccess ||
+        hipMalloc((void**)&p->wbuf, sizeof(double) * (size_t)PLAN_WSTRIDE * (size_t)std::min<int64_t>(max_grid, batch)) !=
+            hipSuccess) {
         plan_free(p);
         return set_err(SKH_ERR_HIP, "hipMalloc of the plan's buffers failed");
     }
@@ -1869,7 +1941,7 @@ int skh_logpf_batched_plan_exec(skh_plan plan, const double* dA, double* dlogpf,
     if (!plan) return set_err(-1, "plan is NULL");
     if (plan->g) return set_err(-1, "a gradient plan runs through skh_logpf_grad_batched_plan_exec");
     if (!dA || !dlogpf || !dsgn) return set_err(-2, "NULL device pointer");
-    return plan_enqueue(plan, dA, dlogpf, dsgn, (hipStream_t)stream, plan->grid, plan->kmax);
+    return plan_enqueue(plan, dA, dlogpf, dsgn, (hipStream_t)stream, plan->grid, plan->kmax, plan->knb);
 }
 
 int skh_logpf_grad_batched_plan_create(int64_t batch, const int64_t* n, const int64_t* offset, const skh_opts* opts,
@@ -1920,34 +1992,36 @@ int skh_logpf_batched_plan_tune(skh_plan plan, const double* dA, double* dlogpf,
     if (plan->g) return set_err(-1, "gradient plans are not tuned by this call");
     if (!dA || !dlogpf || !dsgn) return set_err(-2, "NULL device pointer");
     hipStream_t st = (hipStream_t)stream;
-    int per_cu = 0;
-    HIPCHK((hipError_t)skh_internal_compact_occupancy(plan->kmax, &per_cu));
-    per_cu = std::max(1, per_cu);
-    struct Cand { int grid, kmax; };
+    // candidates: the unblocked engine (kernel NMAX 512 or 1024) and the blocked engine (panel width
+    // 8 or 16 where an instance exists), each at 1..PLAN_TUNE_PER_CU workgroups per CU
+    struct Cand { int grid, kmax, knb; };
     std::vector<Cand> cands;
-    for (int k : {512, 1024}) {
-        if (k < plan->nmax) continue;
-        int pc = 0;
-        HIPCHK((hipError_t)skh_internal_compact_occupancy(k, &pc));
-        pc = std::max(PLAN_TUNE_PER_CU, pc);
-        for (int per = 1; per <= pc; per++) {
+    auto add = [&](int kmax, int knb) {
+        for (int per = 1; per <= PLAN_TUNE_PER_CU; per++) {
             const int g = (int)std::min<int64_t>((int64_t)per * g_num_cu, plan->count);
             bool dup = false;
-            for (auto& c : cands) dup |= (c.grid == g && c.kmax == k);
-            if (!dup) cands.push_back({g, k});
+            for (auto& c : cands) dup |= (c.grid == g && c.kmax == kmax && c.knb == knb);
+            if (!dup) cands.push_back({g, kmax, knb});
         }
+    };
+    for (int k : {512, 1024})
+        if (k >= plan->nmax) add(k, 0);
+    for (int w : {8, 16}) {
+        int bk = 0, bnb = 0;
+        if (plan->nmax <= BLK_NMAX && blocked_instance(plan->nmax, w, &bk, &bnb) && !(bk == 512 && bnb == 16))
+            add(bk, bnb);
     }
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     float best = 1e30f;
-    Cand pick{plan->grid, plan->kmax};
+    Cand pick{plan->grid, plan->kmax, plan->knb};
     for (const Cand& c : cands) {
-        int rc = plan_enqueue(plan, dA, dlogpf, dsgn, st, c.grid, c.kmax);   // warm-up
+        int rc = plan_enqueue(plan, dA, dlogpf, dsgn, st, c.grid, c.kmax, c.knb);   // warm-up
         if (rc) return rc;
         HIPCHK(hipEventRecord(e0, st));
         for (int r = 0; r < 3; r++) {
-            rc = plan_enqueue(plan, dA, dlogpf, dsgn, st, c.grid, c.kmax);
+            rc = plan_enqueue(plan, dA, dlogpf, dsgn, st, c.grid, c.kmax, c.knb);
             if (rc) return rc;
         }
         HIPCHK(hipEventRecord(e1, st));
@@ -1964,8 +2038,46 @@ int skh_logpf_batched_plan_tune(skh_plan plan, const double* dA, double* dlogpf,
     (void)hipEventDestroy(e1);
     plan->grid = pick.grid;
     plan->kmax = pick.kmax;
+    plan->knb = pick.knb;
     plan->tuned = 1;
     if (best_ms) *best_ms = best;
+    return 0;
+}
+
+int skh_plan_config(skh_plan plan, int64_t* out4)
+{
+    if (!plan || !out4) return set_err(-1, "NULL argument");
+    const bool blk = plan->g ? plan->g->knb > 0 : plan->knb > 0;
+    out4[0] = blk ? SKH_ENGINE_BLOCKED : SKH_ENGINE_TEAM1;
+    out4[1] = plan->g ? plan->g->kmax : plan->kmax;
+    out4[2] = plan->g ? plan->g->knb : plan->knb;
+    out4[3] = plan->g ? plan->g->grid : plan->grid;
+    return 0;
+}
+
+int skh_plan_set_config(skh_plan plan, const int64_t* in4)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!plan || !in4) return set_err(-1, "NULL argument");
+    if (plan->g) return set_err(-1, "gradient plans take their configuration at creation");
+    const int64_t eng = in4[0], kmax = in4[1], knb = in4[2], grid = in4[3];
+    const int64_t max_grid = (int64_t)PLAN_TUNE_PER_CU * g_num_cu;
+    if (grid < 1 || grid > std::min<int64_t>(max_grid, plan->count)) return set_err(-2, "grid out of range");
+    if (eng == SKH_ENGINE_BLOCKED) {
+        int bk = 0, bnb = 0;
+        if (plan->nmax > BLK_NMAX || !blocked_instance(plan->nmax, knb, &bk, &bnb) || bk != kmax)
+            return set_err(-2, "no blocked instance with NMAX %lld and panel width %lld for this batch",
+                           (long long)kmax, (long long)knb);
+        plan->knb = (int)knb;
+    } else if (eng == SKH_ENGINE_TEAM1) {
+        if ((kmax != 512 && kmax != 1024) || kmax < plan->nmax) return set_err(-2, "kernel NMAX out of range");
+        plan->knb = 0;
+    } else {
+        return set_err(-2, "engine must be SKH_ENGINE_TEAM1 or SKH_ENGINE_BLOCKED");
+    }
+    plan->kmax = (int)kmax;
+    plan->grid = (int)grid;
+    plan->tuned = 1;
     return 0;
 }
 

@@ -48,6 +48,10 @@ struct TileRegs {
     double a[4][4];
 };
 
+// Every lane loads, entries outside the triangle from the matrix's first two elements (which always
+// exist): a load under a per-lane branch, or a select right after it, is waited for on the spot,
+// which would serialise the tile's eight loads.  The garbage in those entries is never stored (the
+// store is masked) and never reaches a sum (the matvec masks again).
 __device__ __forceinline__ void tile_load(const double* __restrict__ M, int64_t n, int64_t s0, int64_t I, int64_t J,
                                           int wave, int cg, int p, TileRegs& t)
 {
@@ -55,18 +59,13 @@ __device__ __forceinline__ void tile_load(const double* __restrict__ M, int64_t 
 #pragma unroll
     for (int cc = 0; cc < 4; cc++) {
         const int64_t c = cb + cc;
-        const double* col = M + c * n;
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int64_t r = I * TB + 32 * h + 2 * p;   // even; r + 1 < n whenever r < n (n even)
-            double x0 = 0.0, x1 = 0.0;
-            if (c >= s0 && c < n && r < n && r + 1 > c) {
-                const double2 v2 = *reinterpret_cast<const double2*>(col + r);
-                x0 = (r > c) ? v2.x : 0.0;
-                x1 = v2.y;
-            }
-            t.a[2 * h][cc] = x0;
-            t.a[2 * h + 1][cc] = x1;
+            const bool ok = c >= s0 && c < n && r < n && r + 1 > c;
+            const double2 v2 = *reinterpret_cast<const double2*>(M + (ok ? c * n + r : 0));
+            t.a[2 * h][cc] = v2.x;
+            t.a[2 * h + 1][cc] = v2.y;
         }
     }
 }
@@ -139,7 +138,7 @@ __device__ void fused_pass(double* __restrict__ M, int64_t n, int64_t s0, bool u
                 for (int cc = 0; cc < 4; cc++) cur.a[rr][cc] += rv[rr] * cw[cc] - rw[rr] * cv[cc];
             tile_store(M, n, s0, I, J, wave, cg, p, cur);
         }
-        // masked entries hold 0 (load) but the update may have made them nonzero: mask again
+        // entries outside the triangle hold whatever was loaded for them: mask
 #pragma unroll
         for (int cc = 0; cc < 4; cc++) {
             const int64_t c = cb + cc;

@@ -42,6 +42,8 @@ struct CompactArgs {
     double* fac;                 // optional: factorization in place here (input copied in first)
     double* taueta;              // optional: tau | eta out (2 n_b doubles at toffs[b]); required with fac
     const int64_t* toffs;
+    double* wbuf;                // blocked engine: W panel workspace of each workgroup (NMAX x NB doubles)
+    int64_t wstride;             // doubles between two workgroups' W workspaces
 };
 
 extern "C" {
@@ -105,6 +107,9 @@ int skh_internal_skewchol_coop(const double* A, double* U, int64_t n, int pivot,
                                double* W, double* part, unsigned* bar, int* err, int grid, hipStream_t st);
 int skh_internal_skewchol_coop_occupancy(int* per_cu);
 int skh_internal_launch_compact(const CompactArgs* a, int nmax, int grid, hipStream_t st);
+// blocked engine (skh_cblock.hpp): instances (nmax, nb) in {(256, 8), (256, 16), (512, 8)}; nonzero if absent
+int skh_internal_launch_cblock(const CompactArgs* a, int nmax, int nb, int grid, hipStream_t st);
+int skh_internal_cblock_occupancy(int nmax, int nb, int* blocks_per_cu);
 int skh_internal_logpf_from(const double* eta, const double* tau, int64_t n, double* logpf, double* sgn, hipStream_t st);
 int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st);
 int skh_internal_clear_partials(double* tv0, double* tv1, int64_t ntv, double* slots, int64_t ld, int64_t r0,

@@ -16,11 +16,14 @@ tridiagonal runs on the host in the same library (skh_skewtrieigvals_d), as in t
 Errors: ArgumentError (a ValueError) for non-skew input, as the reference's ArgumentError.
 """
 import ctypes
+import hashlib
+import json
+import os
 
 import numpy as np
 
 from . import _lib
-from ._lib import ENGINE_AUTO, ENGINE_LARGE, ENGINE_TEAM1, SkhError
+from ._lib import ENGINE_AUTO, ENGINE_BLOCKED, ENGINE_LARGE, ENGINE_TEAM1, SkhError
 
 __all__ = [
     "SkewHermitian", "SkewHermTridiagonal", "Hessenberg", "HessenbergQ", "ArgumentError",
@@ -31,7 +34,7 @@ __all__ = [
     "JMatrix", "SkewCholesky", "RowMaximum", "NoPivot", "skewchol", "skewchol_", "skewchol_batched", "ldiv", "rdiv",
     "svd", "svdvals", "SVD", "schur", "Schur", "eigvecs", "exp", "log", "cis", "cos", "sin", "sincos", "sinh", "cosh", "tan", "tanh", "cot",
     "coth", "DomainError", "SingularException",
-    "ENGINE_AUTO", "ENGINE_LARGE", "ENGINE_TEAM1", "version", "LogPfPlan", "LogPfGradPlan",
+    "ENGINE_AUTO", "ENGINE_LARGE", "ENGINE_TEAM1", "ENGINE_BLOCKED", "version", "LogPfPlan", "LogPfGradPlan",
 ]
 
 
@@ -1120,6 +1123,31 @@ def _device_batch_call(fn_name, A, want_grad):
     return lp, sg, None
 
 
+def _batch_key(ns):
+    """Tuning-table key of a batch layout: its size histogram (hashed), count and largest n."""
+    ns = np.asarray(ns, dtype=np.int64)
+    vals, cnts = np.unique(ns, return_counts=True)
+    h = hashlib.sha1(np.concatenate([vals, cnts]).astype(np.int64).tobytes()).hexdigest()[:16]
+    return "%s/%d/%d" % (h, len(ns), int(ns.max()) if len(ns) else 0)
+
+
+def _read_table(path):
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get("plans", {}) if isinstance(d, dict) else {}
+    except (OSError, ValueError):
+        return {}
+
+
+def _write_table(path, entries):
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump({"format": "skewhip tuning table 1", "library": version(), "plans": entries}, f, indent=1,
+                  sort_keys=True)
+    os.replace(tmp, path)
+
+
 class LogPfPlan:
     """A batched log|Pf| plan over device-resident matrices (C ABI skh_logpf_batched_plan_*).
 
@@ -1146,11 +1174,38 @@ class LogPfPlan:
         _lib.check("skh_logpf_batched_plan_exec",
                    _lib.lib().skh_logpf_batched_plan_exec(self._h, dA, dlogpf, dsgn, stream))
 
-    def tune(self, dA, dlogpf, dsgn, stream=None):
+    def tune(self, dA, dlogpf, dsgn, stream=None, table=None):
+        """Time the launch configurations (engine, kernel NMAX, panel width, work-groups) on these
+        inputs and keep the fastest; returns its time in ms.  With a tuning table (a JSON file: the
+        `table` path, else $SKEWHIP_TUNE_TABLE), a batch layout already in the table takes its stored
+        configuration without timing, and a new one is timed and added."""
+        path = table if table is not None else os.environ.get("SKEWHIP_TUNE_TABLE")
+        key = _batch_key(self._ns)
+        entries = _read_table(path) if path else {}
+        if key in entries:
+            self.set_config(entries[key]["config"])
+            return float(entries[key]["ms"])
         best = ctypes.c_double(0.0)
         _lib.check("skh_logpf_batched_plan_tune",
                    _lib.lib().skh_logpf_batched_plan_tune(self._h, dA, dlogpf, dsgn, stream, ctypes.byref(best)))
+        if path:
+            entries[key] = {"config": self.config(), "ms": best.value, "matrices": int(len(self._ns)),
+                            "nmax": int(self._ns.max()) if len(self._ns) else 0}
+            _write_table(path, entries)
         return best.value
+
+    def config(self):
+        """The launch configuration: engine ("blocked" or "unblocked"), kernel NMAX, panel width,
+        work-groups."""
+        out = np.zeros(4, dtype=np.int64)
+        _lib.check("skh_plan_config", _lib.lib().skh_plan_config(self._h, _ptr(out)))
+        return {"engine": "blocked" if int(out[0]) == ENGINE_BLOCKED else "unblocked", "kernel_nmax": int(out[1]),
+                "panel_width": int(out[2]), "workgroups": int(out[3])}
+
+    def set_config(self, cfg):
+        v = np.array([ENGINE_BLOCKED if cfg["engine"] == "blocked" else ENGINE_TEAM1, cfg["kernel_nmax"],
+                      cfg["panel_width"], cfg["workgroups"]], dtype=np.int64)
+        _lib.check("skh_plan_set_config", _lib.lib().skh_plan_set_config(self._h, _ptr(v)))
 
     def status(self, stream=None):
         _lib.check("skh_plan_status", _lib.lib().skh_plan_status(self._h, stream))

@@ -10,7 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # SKEWHIP_LIB names another build of the same library (the in-tree one by default)
 LIB_PATH = os.environ.get("SKEWHIP_LIB") or os.path.join(_HERE, "libskewhip.so")
 
-ENGINE_AUTO, ENGINE_LARGE, ENGINE_TEAM1 = 0, 1, 2
+ENGINE_AUTO, ENGINE_LARGE, ENGINE_TEAM1, ENGINE_BLOCKED = 0, 1, 2, 3
 FLAG_TIMING = 0x100
 FLAG_PHASES = 0x200
 ERR_NOCONV, ERR_DOMAIN, ERR_SINGULAR = 2, 6, 7
@@ -76,6 +76,8 @@ def lib():
         "skh_logpf_grad_batched_plan_exec": ([vp, vp, vp, vp, vp, vp], ctypes.c_int),
         "skh_plan_status": ([vp, vp], ctypes.c_int),
         "skh_plan_info": ([vp, vp], ctypes.c_int),
+        "skh_plan_config": ([vp, vp], ctypes.c_int),
+        "skh_plan_set_config": ([vp, vp], ctypes.c_int),
         "skh_plan_destroy": ([vp], ctypes.c_int),
         "skh_sytrd_z": ([i64, vp, i64, vp, vp, vp, op], ctypes.c_int),
         "skh_sytrd_z_device": ([i64, vp, i64, vp, vp, vp, vp, op], ctypes.c_int),

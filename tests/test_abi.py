@@ -249,3 +249,21 @@ def test_complex_tridiagonal_entries_before_device():
     np.testing.assert_allclose(mu, np.linalg.eigvalsh(S), rtol=1e-15)
     vals = skl.eigvals(skl.SkewHermTridiagonal(ev, dv))
     np.testing.assert_allclose(np.imag(vals), -np.linalg.eigvalsh(S)[::-1], rtol=1e-15)
+
+
+def test_tuning_table_roundtrip(tmp_path):
+    """The plan tuner's JSON table: the batch key depends only on the size histogram, count and
+    largest n (not on the order of the matrices), and entries survive a write and a read."""
+    import numpy as np
+    import skewlinearalgebra as skl
+    a = np.array([32, 64, 64, 512, 96], dtype=np.int64)
+    b = np.array([64, 512, 32, 96, 64], dtype=np.int64)
+    c = np.array([64, 512, 32, 96, 96], dtype=np.int64)
+    assert skl._batch_key(a) == skl._batch_key(b)
+    assert skl._batch_key(a) != skl._batch_key(c)
+    path = str(tmp_path / "tune.json")
+    entry = {"config": {"engine": "blocked", "kernel_nmax": 512, "panel_width": 8, "workgroups": 512}, "ms": 1.5}
+    skl._write_table(path, {skl._batch_key(a): entry})
+    back = skl._read_table(path)
+    assert back[skl._batch_key(b)] == entry
+    assert skl._read_table(str(tmp_path / "missing.json")) == {}

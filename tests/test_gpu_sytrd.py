@@ -621,3 +621,106 @@ def test_sytrd_repeatable_and_clock_neutral(nb):
         assert np.array_equal(eta, outs[0][2])
     # and they are the orthogonal similarity they should be (Frobenius norm kept)
     assert abs(np.sqrt(2.0 * np.sum(outs[0][2] ** 2)) - np.linalg.norm(A)) <= 1e-13 * np.linalg.norm(A)
+
+
+@pytest.mark.parametrize("nb", [8, 16])
+@pytest.mark.parametrize("n", [2, 4, 6, 8, 10, 16, 18, 30, 62, 64, 66, 96, 128, 130, 200, 256, 258, 320, 512])
+def test_sytrd_blocked_vs_oracle(n, nb):
+    """The blocked one-workgroup engine (panels of nb reflectors, MFMA rank-2k update per panel):
+    the same reflectors as the oracle within the blocking tolerance, the factor layout, and the
+    orthogonal similarity."""
+    if nb == 16 and n > 256:
+        pytest.skip("panel width 16 is built for n <= 256")
+    say("sytrd blocked n=%d nb=%d" % (n, nb))
+    A = skew_gauss(n, 30 * n + nb)
+    F, tau, eta, nz = gpu_sytrd(A, nb, skl.ENGINE_BLOCKED)
+    Fo, tauo, etao = oracle.skewblockedhess(A)
+    nA = np.linalg.norm(A)
+    np.testing.assert_allclose(eta, etao, rtol=0, atol=1e-11 * nA)
+    np.testing.assert_allclose(tau, tauo, rtol=0, atol=1e-11)
+    assert nz == int(np.count_nonzero(tau))
+    check_fact(A, F, tau, eta)
+    for i in range(n - 2):
+        assert F[i + 1, i] == 1.0
+    assert F[n - 1, n - 2] == eta[-1] and tau[-1] == 0.0
+    iu = np.triu_indices(n)
+    np.testing.assert_array_equal(F[iu], A[iu])
+
+
+def test_blocked_batch_vs_compact():
+    """A mixed batch (odd, empty and every even size class up to 512) through the blocked engine
+    against the one-workgroup unblocked engine and the oracle."""
+    say("blocked batch")
+    rng = np.random.default_rng(41)
+    sizes = [0, 1, 2, 3, 4, 32, 64, 96, 130, 256, 300, 384, 510, 512, 7] + list(rng.integers(1, 40, 24) * 2)
+    mats = []
+    for n in sizes:
+        B = rng.standard_normal((n, n)) / np.sqrt(max(n, 1))
+        mats.append(np.asfortranarray(B - B.T))
+    lp_b, sg_b = skl.logabspfaffian_batched(mats, engine=skl.ENGINE_BLOCKED)
+    lp_c, sg_c = skl.logabspfaffian_batched(mats, engine=skl.ENGINE_TEAM1)
+    for k, M in enumerate(mats):
+        n = M.shape[0]
+        assert sg_b[k] == sg_c[k], (k, n)
+        if n % 2 == 1:
+            assert lp_b[k] == -np.inf
+            continue
+        if n == 0:
+            assert lp_b[k] == 0.0
+            continue
+        lpo, sgo = oracle.logabspfaffian(M)
+        assert sg_b[k] == sgo
+        assert abs(lp_b[k] - lpo) <= 1e-10 * max(1.0, abs(lpo)), (n, lp_b[k], lpo)
+
+
+def test_plan_tune_table(tmp_path):
+    """The tuner over engines and panel widths, with its JSON table: a second plan of the same layout
+    takes the stored configuration without timing and gives the same bits; a configuration set by
+    hand (the unblocked engine) agrees with the oracle too."""
+    say("plan tuning table")
+    import torch
+    rng = np.random.default_rng(23)
+    sizes = list(rng.integers(1, 17, 300) * 32) + [2, 3, 0]
+    mats = []
+    for n in sizes:
+        B = rng.standard_normal((n, n)) / np.sqrt(max(n, 1))
+        mats.append(np.asfortranarray(B - B.T))
+    ns = np.array(sizes, dtype=np.int64)
+    offs = np.zeros(len(sizes), dtype=np.int64)
+    tot = 0
+    for k, n in enumerate(sizes):
+        offs[k] = tot
+        tot += (n * n + 1) & ~1
+    host = np.zeros(max(tot, 1))
+    for k, M in enumerate(mats):
+        host[offs[k]:offs[k] + M.size] = M.ravel(order="F")
+    dA = torch.tensor(host, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    path = str(tmp_path / "tune.json")
+    p1 = skl.LogPfPlan(ns, offs)
+    lp1 = torch.empty(len(sizes), dtype=torch.float64, device="cuda")
+    sg1 = torch.empty_like(lp1)
+    ms1 = p1.tune(dA.data_ptr(), lp1.data_ptr(), sg1.data_ptr(), st, table=path)
+    p1.run(dA.data_ptr(), lp1.data_ptr(), sg1.data_ptr(), st)
+    p1.status(st)
+    p2 = skl.LogPfPlan(ns, offs)
+    lp2 = torch.empty_like(lp1)
+    sg2 = torch.empty_like(lp1)
+    ms2 = p2.tune(dA.data_ptr(), lp2.data_ptr(), sg2.data_ptr(), st, table=path)
+    assert ms2 == ms1 and p2.config() == p1.config(), (p1.config(), p2.config())
+    p2.run(dA.data_ptr(), lp2.data_ptr(), sg2.data_ptr(), st)
+    p2.status(st)
+    assert torch.equal(lp1, lp2) and torch.equal(sg1, sg2)
+    say("tuned configuration: %s, %.2f ms" % (p1.config(), ms1))
+    p2.set_config({"engine": "unblocked", "kernel_nmax": 512, "panel_width": 0, "workgroups": 256})
+    p2.run(dA.data_ptr(), lp2.data_ptr(), sg2.data_ptr(), st)
+    p2.status(st)
+    hl, hs = lp2.cpu().numpy(), sg2.cpu().numpy()
+    for k in range(0, len(sizes), 7):
+        n = sizes[k]
+        lpo, sgo = (0.0, 1.0) if n == 0 else oracle.logabspfaffian(mats[k])
+        assert hs[k] == sgo
+        if n % 2 == 0 and n > 0:
+            assert abs(hl[k] - lpo) <= 1e-10 * max(1.0, abs(lpo))
+    p1.close()
+    p2.close()
