@@ -74,8 +74,8 @@ int skh_sytrd_d_device(int64_t n, double *dA, int64_t lda, double *dtau, double 
  * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc).  Same arguments, kernels and results as skh_sytrd_d_device (SKH_FLAG_TIMING and
  * SKH_FLAG_PHASES are ignored), for n <= SKH_NOALLOC_MAX_N; larger n is rejected (status -1).  The
- * workspace and grid-barrier control words live in a fixed device array of about 90 MiB (1.17e7
- * doubles) that each GPU the library runs on holds until the process ends.  Every call on a GPU shares
+ * workspace and control words live in a fixed device array of about 120 MiB (1.571e7 doubles) that
+ * each GPU the library runs on holds until the process ends.  Every call on a GPU shares
  * that GPU's array and starts by clearing its control words: calls that overlap on one GPU race (issue
  * them on one stream, or synchronise between them).
  */

@@ -247,14 +247,16 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
     }
     const int64_t ld = (n + 1) & ~int64_t(1);
     const int64_t nt = (n + skh::TB - 1) / skh::TB + 1;
+    const int64_t nts = (n + skh::MR - 1) / skh::MR + 1;
     int per_cu0 = 0;
     HIPCHK((hipError_t)skh_internal_panel_occupancy((int)nb, &per_cu0));
     // row-partial slots of one matvec strip: at most one per workgroup of the grid
     const int64_t nseg = (int64_t)std::max(1, std::min(per_cu0, 2)) * g_num_cu + 2;
-    // W | Pnorm x2 | PS x2 | X x2 | (Tv | Prow | Pcol) x2: the hand-over buffers of the two column
-    // parities, each contiguous (the launch's first column is filled with the empty marker)
-    const int64_t hsize = 2 * skh::NBMAX + nseg * ld + nt * ld;
-    size_t doubles = (size_t)(ld * nb + 2 * nt + 2 * nt * 2 * skh::NBMAX + 2 * ld + 2 * hsize + 64);
+    // W | V2 | (Pnorm | PS | X) x 4 | (Tv | Prow | Pcol) x 3: the copies of the hand-over slots by
+    // column mod 4 (written by the row passes) and mod 3 (written by phase B)
+    const int64_t ostride = (nt + nt * 2 * skh::NBMAX + ld + 1) & ~int64_t(1);
+    const int64_t hsize = 2 * skh::NBMAX + nseg * ld + nts * ld;
+    size_t doubles = (size_t)(2 * ld * nb + skh::OWN_COPIES * ostride + skh::HAND_COPIES * hsize + 64);
     double* ws;
     if (ws_ext) {
         if (doubles > ws_cap)
@@ -267,26 +269,22 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
     }
     skh::Work w;
     w.W = ws;
+    w.V2 = w.W + ld * nb;
     w.ldw = ld;
-    w.Pnorm = w.W + ld * nb;
-    w.pstride = nt;
-    w.PS = w.Pnorm + 2 * nt;
-    w.sstride = nt * 2 * skh::NBMAX;
-    w.X = w.PS + 2 * w.sstride;
-    w.xstride = ld;
-    w.Tv = w.X + 2 * ld;
+    w.Pnorm = w.V2 + ld * nb;
+    w.PS = w.Pnorm + nt;
+    w.X = w.PS + nt * 2 * skh::NBMAX;
+    w.ostride = ostride;
+    w.Tv = w.Pnorm + skh::OWN_COPIES * ostride;
     w.Prow = w.Tv + 2 * skh::NBMAX;
     w.ldp = ld;
     w.Pcol = w.Prow + nseg * ld;
     w.hstride = hsize;
-    const int64_t nslots = nseg + nt;   // Prow and Pcol slots, ld doubles each, after Tv
 
     int per_cu = 0;
     HIPCHK((hipError_t)skh_internal_panel_occupancy((int)nb, &per_cu));
     per_cu = std::max(1, std::min(per_cu, 2));
     const int grid = per_cu * g_num_cu;
-    if ((int64_t)64 * (int64_t)(2 * n + 4) > 4000000000LL)
-        return set_err(SKH_ERR_UNSUPPORTED, "n = %lld too large for the barrier counter", (long long)n);
     if ((double)skh::tall_grid(n, 1).T * (double)grid >= 4294967296.0)
         return set_err(SKH_ERR_UNSUPPORTED, "n = %lld too large for the 32-bit tile ownership", (long long)n);
 
@@ -302,8 +300,8 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
         const long v = e ? atol(e) : 0;
         return (int64_t)(v > 0 ? v : 1 << 30);
     }();
+    // error flag and phase clocks start at zero
     HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 512, st));
-    unsigned base = 0;
     for (int64_t c0 = 0; c0 <= n - 3; c0 += nb) {
         const int nbp = (int)std::min<int64_t>(nb, n - 2 - c0);
         PanelLaunch pl;
@@ -315,16 +313,13 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
         pl.w = w;
         pl.tau = dtau;
         pl.eta = deta;
-        pl.bar = ctl.bar;
-        pl.base = base;
         pl.err = ctl.err;
-        // grid sized to the panel's first matvec: about `tpw` tiles per workgroup (fewer
-        // workgroups make the two grid barriers per column cheaper once the matrix is small)
+        // grid sized to the panel's first matvec: about `tpw` tiles per workgroup
+        const skh::TallGrid tg = skh::tall_grid(n, c0 + 1);
         {
-            const int64_t tiles = skh::tall_grid(n, c0 + 1).T;
             // at least one workgroup per 64-row block of the panel (each keeps its block's rows)
             const int64_t blocks = (n + skh::TB - 1) / skh::TB - c0 / skh::TB;
-            const int64_t want = std::max<int64_t>(std::max<int64_t>(8, blocks), (tiles + tpw - 1) / tpw);
+            const int64_t want = std::max<int64_t>(std::max<int64_t>(8, blocks), (tg.T + tpw - 1) / tpw);
             pl.grid = (int)std::min<int64_t>(std::min<int64_t>(grid, want), gcap);
         }
         pl.nbmax = (int)nb;
@@ -333,14 +328,36 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
             pl.multi = (int64_t)pl.grid < blocks ? 1 : 0;
         }
         pl.ph = g_timer.phases ? ctl.ph : nullptr;
-        // the hand-over buffers read for the launch's first column start empty (t and rows >= c0 of the
-        // partials, parity c0), and so does t of its second column (the partials of parity c0 + 1 are
-        // emptied by the first column's producers)
+        // the slots polled before any producer of this launch has refilled them start empty: the
+        // hand-over copies of the launch's first two columns (t, rows >= c0 of the row-partial slots in
+        // use and of the column partials), the row-pass copies of its first two columns, and the
+        // panel's columns of W and V2
         {
-            double* h0 = w.Tv + (c0 & 1) * hsize;
-            double* h1 = w.Tv + ((c0 + 1) & 1) * hsize;
-            HIPCHK((hipError_t)skh_internal_clear_partials(h0, h1, 2 * skh::NBMAX, h0 + 2 * skh::NBMAX, ld, c0, n, nslots,
-                                                           st));
+            int64_t maxslot = 1;
+            for (int64_t I = tg.I0; I < tg.nts; I++) maxslot = std::max(maxslot, skh::tall_slots(tg, I, pl.grid));
+            FillSegs fs;
+            int ns = 0;
+            auto seg = [&](double* p, int64_t rows, int64_t ldr, int64_t lo, int64_t hi) {
+                fs.p[ns] = p;
+                fs.rows[ns] = rows;
+                fs.ld[ns] = ldr;
+                fs.lo[ns] = lo;
+                fs.hi[ns] = hi;
+                ns++;
+            };
+            for (int64_t cc = c0; cc < c0 + 2; cc++) {
+                double* h = w.Tv + (cc % skh::HAND_COPIES) * hsize;
+                seg(h, 1, 0, 0, 2 * skh::NBMAX);
+                seg(h + 2 * skh::NBMAX, maxslot, ld, c0, n);
+                seg(h + 2 * skh::NBMAX + nseg * ld, tg.nts, ld, c0, n);
+                seg(w.Pnorm + (cc & (skh::OWN_COPIES - 1)) * ostride, 1, 0, 0, ostride);
+            }
+            // t of the third copy too: an element that has no part in the launch's first columns
+            // empties its slots only from the column it becomes live
+            seg(w.Tv + ((c0 + 2) % skh::HAND_COPIES) * hsize, 1, 0, 0, 2 * skh::NBMAX);
+            seg(w.W, nbp, ld, c0, n);
+            seg(w.V2, nbp, ld, c0, n);
+            HIPCHK((hipError_t)skh_internal_fill_empty(&fs, ns, st));
         }
         const bool tm = g_timer.on;
         if (tm) HIPCHK((hipError_t)g_timer.begin(st));
@@ -354,9 +371,6 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
             }
             HIPCHK((hipError_t)g_timer.end(st, 0, bytes));
         }
-        base += (unsigned)(2 * nbp);   // barriers per launch (group counters restart below)
-        HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 288, st));
-        base = 0;
         const int64_t e = c0 + nbp;
         if (tm) HIPCHK((hipError_t)g_timer.begin(st));
         HIPCHK((hipError_t)skh_internal_launch_rank2k(dA, lda, w.W, w.ldw, n, e, c0, nbp, st));
@@ -968,9 +982,18 @@ int check_barrier_flag()
     Ctl ctl;
     int rc = get_ctl(ctl);
     if (rc) return rc;
-    int herr = 0;
-    HIPCHK(hipMemcpy(&herr, ctl.err, sizeof(int), hipMemcpyDeviceToHost));
-    if (herr) return set_err(SKH_ERR_BARRIER, "grid barrier timed out in the panel kernel");
+    int hv[3] = {0, 0, 0};
+    HIPCHK(hipMemcpy(hv, ctl.err, sizeof(hv), hipMemcpyDeviceToHost));
+    const int herr = hv[0];
+    if (herr) {
+        if ((herr & 1) && (herr >> 1) != 0) {
+            static const char* site[] = {"?", "row pass", "phase B", "phase B column entry"};
+            return set_err(SKH_ERR_BARRIER, "a bounded wait of the panel kernel ran out (%s, column %d mod 65536, "
+                           "workgroup %d; inputs missing 0x%x, index %d)", site[(herr >> 1) & 3], (herr >> 3) & 0xFFFF,
+                           (herr >> 19) & 0x3FF, hv[1], hv[2] - 1);
+        }
+        return set_err(SKH_ERR_BARRIER, "a bounded wait of a cooperative kernel ran out (lost workgroup)");
+    }
     return 0;
 }
 

@@ -24,31 +24,40 @@ constexpr int NBMAX = 128;    // largest panel width
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-// Device workspace of one factorization (one matrix).
+// Device workspace of one factorization (one matrix).  Every value one workgroup of the panel
+// kernel hands to another travels through a slot that holds the empty marker until the value is
+// stored (see EMPTY_BITS), so consumers poll their inputs instead of meeting at grid barriers:
+//   row pass of column j -> phase B of column j:  Pnorm, PS and X of column j (OWN_COPIES copies,
+//                                                  one per column mod 4) and the rows of W and V2;
+//   phase B of column j -> row pass of column j+1: Tv, Prow and Pcol of column j (HAND_COPIES
+//                                                  copies, one per column mod 3).
+// A slot of column j + OWN_COPIES (j + HAND_COPIES) is refilled with the empty marker by its own
+// producer once every consumer of column j's value is known to be done (row_pass and phase_b
+// state when).
+constexpr int OWN_COPIES = 4, HAND_COPIES = 3;
 struct Work {
-    double* W;       // n x NBMAX (ldw)
+    double* W;       // n x NBMAX (ldw)   w of the panel's reflectors (empty marker where not stored yet)
+    double* V2;      // n x NBMAX (ldw)   the panel's reflectors (as stored in A), for the row hand-over
     int64_t ldw;
-    double* Pnorm;   // [nq]            per-chunk sums of squares of the reflector tail
-    double* PS;      // [nq][2*NBMAX]   per-chunk W^T x and V^T x partials
-    double* Tv;      // [2*NBMAX]       t1 = W^T v, t2 = V^T v
-    double* Prow;    // [nt][ldp]       row partials of the skew matvec (one slot per rank per strip)
-    double* Pcol;    // [nt][ldp]       column partials of the skew matvec
+    double* Pnorm;   // [nt]              per-block sums of squares of the reflector tail
+    double* PS;      // [nt][2*NBMAX]     per-block W^T x and V^T x partials
+    double* X;       // [ld]              x_j, the updated column j (unscaled reflector)
+    int64_t ostride; // Pnorm, PS and X of column j at + (j mod 4) * ostride
+    double* Tv;      // [2*NBMAX]         scale * (partial t1 = W^T x, t2 = V^T x); the consumer adds the head
+    double* Prow;    // [nseg][ldp]       row partials of the skew matvec (one slot per rank per strip)
+    double* Pcol;    // [nts][ldp]        column partials of the skew matvec
     int64_t ldp;
-    int64_t hstride; // Tv, Prow, Pcol of the other column parity at + hstride (the hand-over values)
-    // Pnorm, PS and X hold one copy per column parity: the row pass of column j + 1 writes them
-    // while a slower workgroup may still read the copy of column j in its phase B
-    int64_t pstride; // Pnorm of the other parity at Pnorm + pstride (nq entries)
-    int64_t sstride; // PS of the other parity at PS + sstride
-    double* X;       // [2][xstride]    x_j, the updated column j (unscaled reflector), by parity
-    int64_t xstride;
+    int64_t hstride; // Tv, Prow and Pcol of column j at + (j mod 3) * hstride
 };
 
-__device__ __forceinline__ double* pnorm_of(const Work& w, int64_t j) { return w.Pnorm + (j & 1) * w.pstride; }
-__device__ __forceinline__ double* ps_of(const Work& w, int64_t j) { return w.PS + (j & 1) * w.sstride; }
-__device__ __forceinline__ double* x_of(const Work& w, int64_t j) { return w.X + (j & 1) * w.xstride; }
-__device__ __forceinline__ double* tv_of(const Work& w, int64_t j) { return w.Tv + (j & 1) * w.hstride; }
-__device__ __forceinline__ double* prow_of(const Work& w, int64_t j) { return w.Prow + (j & 1) * w.hstride; }
-__device__ __forceinline__ double* pcol_of(const Work& w, int64_t j) { return w.Pcol + (j & 1) * w.hstride; }
+__device__ __forceinline__ int64_t own_copy(int64_t j) { return j & (OWN_COPIES - 1); }
+__device__ __forceinline__ int64_t hand_copy(int64_t j) { return j % HAND_COPIES; }
+__device__ __forceinline__ double* pnorm_of(const Work& w, int64_t j) { return w.Pnorm + own_copy(j) * w.ostride; }
+__device__ __forceinline__ double* ps_of(const Work& w, int64_t j) { return w.PS + own_copy(j) * w.ostride; }
+__device__ __forceinline__ double* x_of(const Work& w, int64_t j) { return w.X + own_copy(j) * w.ostride; }
+__device__ __forceinline__ double* tv_of(const Work& w, int64_t j) { return w.Tv + hand_copy(j) * w.hstride; }
+__device__ __forceinline__ double* prow_of(const Work& w, int64_t j) { return w.Prow + hand_copy(j) * w.hstride; }
+__device__ __forceinline__ double* pcol_of(const Work& w, int64_t j) { return w.Pcol + hand_copy(j) * w.hstride; }
 
 // Per-column scalars kept in LDS across the phases.
 struct ColScal {
@@ -215,23 +224,72 @@ __device__ __forceinline__ void st_co(double* p, double v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Values phase B of column j hands to the next row pass (the matvec partial sums and t) go to the
-// hand-over buffers of column j's parity, exact.  The row-block owners poll them instead of
-// waiting at a grid barrier, so an empty slot must be told from a written one: empty slots hold
-// the signalling NaN EMPTY, a bit pattern no arithmetic result has (FP64 arithmetic returns quiet
-// NaNs).  Column j's producer stores EMPTY into the same slots of the other parity (t with its
-// value, the partials at the start of phase B): their values (column j - 1) were consumed before
-// the barrier that started column j, and column j + 1 writes them only after the next barrier, so a
-// reader sees EMPTY or the value of its own column.  The buffers of the launch's first column (and
-// t of its second) are filled with EMPTY before it.
+// Hand-over slots hold their values exactly.  An empty slot holds the signalling NaN EMPTY, a bit
+// pattern no arithmetic result has (FP64 arithmetic returns quiet NaNs), so a consumer polling a
+// slot sees EMPTY or the value of its own column, never an older one, as long as the producer
+// empties the slot after the older value's last reader and that store completes before any reader
+// of the new value starts polling.  The stores complete because every wave executes
+// `s_waitcnt vmcnt(0)` (drain) when it starts polling its own next inputs, and the order follows
+// from the data: a consumer only polls column j's slots after it has seen a value that the
+// producer stored after its drain.
 constexpr unsigned long long EMPTY_BITS = 0x7FF0DEADBEEF0001ull;
 __device__ __forceinline__ double empty_slot() { return __builtin_bit_cast(double, EMPTY_BITS); }
 __device__ __forceinline__ bool filled(double x) { return __builtin_bit_cast(unsigned long long, x) != EMPTY_BITS; }
-// store column j's hand-over value at p (a slot of parity j) and EMPTY at the slot of the other parity
-__device__ __forceinline__ void handover(double* p, int64_t hstride, int64_t j, double x)
+__device__ __forceinline__ void drain() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// One more round of a poll loop?  False (the launch is abandoned) after POLL_LIMIT rounds (about
+// a second; a round is at least one memory round trip), which flags the error, or once another
+// workgroup has flagged one (looked at every 64 rounds).  The first wait to run out records where
+// (poll_code) in the flag.
+constexpr unsigned POLL_LIMIT = 1u << 20;
+enum PollSite { POLL_ROW = 1, POLL_B = 2, POLL_SETTLE = 3 };
+__device__ __forceinline__ int poll_code(int site, int64_t j)
 {
-    st_co(p, x);
-    st_co(p + ((j & 1) ? -hstride : hstride), empty_slot());
+    return 1 | (site << 1) | ((int)(j & 0xFFFF) << 3) | ((int)(blockIdx.x & 0x3FF) << 19);
+}
+__device__ __forceinline__ bool poll_more(unsigned spins, int* err, int code)
+{
+    if (spins > POLL_LIMIT) {
+        int expect = 0;
+        __hip_atomic_compare_exchange_strong(err, &expect, code, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+        return false;
+    }
+    if ((spins & 63u) == 63u && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+#ifndef SKH_POLL_SLEEP
+#define SKH_POLL_SLEEP 1
+#endif
+#ifdef SKH_POLL_FAST
+    if (spins < SKH_POLL_FAST) __builtin_amdgcn_s_sleep(1);
+    else
+#endif
+    __builtin_amdgcn_s_sleep(SKH_POLL_SLEEP);
+    return true;
+}
+
+// what an abandoned poll still missed (err[1]: OR of the missing input kinds, err[2]: 1 + an index)
+__device__ __forceinline__ void poll_report(int* err, unsigned miss, int64_t idx)
+{
+    if (miss) {
+        __hip_atomic_fetch_or(err + 1, (int)miss, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int expect = 0;
+        __hip_atomic_compare_exchange_strong(err + 2, &expect, (int)(idx + 1), __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// x (already loaded from p) once it is filled: polls p again in the rare case that the producer's
+// store is still on its way (0 if the launch is abandoned)
+__device__ __forceinline__ double settle(double x, const double* p, int* err, int code)
+{
+    if (__builtin_expect(!filled(x), 0)) {
+        for (unsigned spins = 0;; spins++) {
+            if (!poll_more(spins, err, code)) return 0.0;
+            x = ld_co(p);
+            if (filled(x)) break;
+        }
+    }
+    return x;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -390,10 +448,17 @@ __device__ inline PanelMap panel_map(int64_t n, int64_t c0, int g, int G)
 // Row pass: finalises w of column j-1 (doC) and/or applies the panel column update to
 // column j and produces its norm and W^T x / V^T x partials (doA).
 //   i = j - c0 = number of previous panel columns that update column j.
-// Rows are processed in absolute 64-row chunks starting at block j/64; chunk q is owned by
-// rank q mod G in both uses, so the row data a rank writes in C(j-1) is the data it reads in
-// A(j).  Row j of w_{j-1} (needed by everyone as W[j, i-1]) is computed redundantly by every
-// rank with one fixed summation order, and the owner stores that same value.
+// Rows are processed in absolute 64-row blocks starting at block j/64; block b is owned by
+// rank (b - c0/64) mod G in both uses, so the row data a rank writes in C(j-1) is the data it
+// reads in A(j).  Row j of w_{j-1} (needed by everyone as W[j, i-1]) is computed redundantly by
+// every rank with one fixed summation order, and the owner stores that same value.
+// Hand-over: the block's norm partial (slot b), its W^T x / V^T x partials (row b) and its rows of
+// x_j go to copy j mod 4 of Pnorm / PS / X, and its rows of the new W and V columns to W and V2,
+// all polled by their consumers.  At its start the pass empties the block's slots of column j + 2
+// (copy (j + 2) mod 4 = (j - 2) mod 4): every phase B that read column j - 2 had finished before
+// the owners of all blocks stored their norm partials of column j - 1, which this rank's phase B
+// of column j - 1 has seen; the next pass's drain completes those stores before this rank stores
+// anything of column j + 1, and readers of column j + 2 poll only after seeing column j + 1.
 // ---------------------------------------------------------------------------------------
 struct RowShared {
     double wrow[NBMAX];
@@ -428,9 +493,11 @@ struct RowRegs {
 // column j, k < i).  Every global load a chunk needs (row and column partials of the matvec;
 // for blocks other than the workgroup's own, also its rows of V and W and the old entries of
 // columns j-1 and j) and the shared inputs (t, row j of V and W, the special row's partials) are
-// issued together, before the first barrier, so a row pass costs one memory round trip.
+// issued together and polled together, so a row pass costs one memory round trip once its inputs
+// are there.  Returns false when the launch is abandoned (a poll ran out, or another workgroup
+// flagged an error).
 template <int KQ, bool MULTI>
-__device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j, int i,
+__device__ bool row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j, int i,
                          bool doC, bool doA, const Work& w, int g, int G, RowShared& sh, RowRegs<KQ>& rr,
                          const PanelMap& pm, int* err, unsigned long long* clk)
 {
@@ -440,6 +507,7 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
     const int im = i - 1;             // its local index
     const ColScal cs = sh.cur;        // scalars of column jm (valid when doC)
     double* __restrict__ Wm = w.W;
+    double* __restrict__ V2 = w.V2;
     const int64_t ldw = w.ldw;
     const int64_t b0 = j / TB;        // block of row j
     const int64_t nt = cdiv(n, TB);
@@ -455,17 +523,24 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
         const bool first = !shared_done;
         shared_done = true;
         const bool own = !MULTI || (b == bown);
-        const int64_t q = b - b0;     // chunk index relative to row j's block
         const int64_t r = b * TB + lane;
         const bool rin = r < n;
         const bool rowC = doC && rin && r >= j;
+        // ---- 0. empty the block's slots of column j + 2 ----
+        if (doA) {
+            const double e = empty_slot();
+            if (wave == 0) {
+                if (rin) st_co(x_of(w, j + 2) + r, e);
+                if (lane == 0) st_co(pnorm_of(w, j + 2) + b, e);
+            }
+            if (lane < KQ) {
+                double* ps2 = ps_of(w, j + 2) + b * (2 * NBMAX) + k0 + lane;
+                st_co(ps2, e);
+                st_co(ps2 + NBMAX, e);
+            }
+        }
         // ---- 1. every global load of this step, into registers ----
         double tv1 = 0.0, tv2 = 0.0, wraw = 0.0, vraw = 0.0, sprow = 0.0, sp[4] = {0.0, 0.0, 0.0, 0.0};
-        if (first && tid < i) {
-            // rows j of W and V: written by earlier row passes, before the last grid barrier
-            wraw = ld_co(Wm + j + tid * ldw);
-            vraw = ld_co(A + j + (c0 + tid) * lda);
-        }
         double vreg[KQ], wreg[KQ], xo, xold;
         if (own) {
 #pragma unroll
@@ -488,21 +563,27 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
         }
         double pp = 0.0;
         if (doC) {
-            // t, the special row's column partials and this row's matvec partials, written by
-            // phase B of column j-1 with its tag: each wave polls its share until all have arrived
-            // (32-bit element offsets: (slots + strips) * ldp stays far below 2^31)
+            // t and the special row's column partials (phase B of column j-1), row j of W and V
+            // (earlier row passes) and this row's matvec partials: each wave polls its share until
+            // all have arrived (32-bit element offsets: (slots + strips) * ldp stays far below 2^31)
             const int S = (int)((b * TB) / MR);    // matvec strip holding this chunk
             const int nslot = rowC ? (int)(own ? pm.nslot_own : tall_slots(tg, S, G)) : 0;
             const int nts = (int)tg.nts, ldp = (int)w.ldp, rr32 = (int)r;
             const int S0 = (int)(j / MR), j32 = (int)j;
             const double* prowm = prow_of(w, jm);
             const double* pcolm = pcol_of(w, jm);
+            bool alive = true;
             for (unsigned spins = 0;; spins++) {
                 bool ok = true;
+                unsigned miss = 0;
                 if (first && tid < im) {
                     tv1 = ld_co(tv_of(w, jm) + tid);
                     tv2 = ld_co(tv_of(w, jm) + NBMAX + tid);
-                    ok = filled(tv1) && filled(tv2);
+                    wraw = ld_co(Wm + j + tid * ldw);
+                    vraw = ld_co(V2 + j + tid * ldw);
+                    if (!(filled(tv1) && filled(tv2))) miss |= 1;
+                    if (!(filled(wraw) && filled(vraw))) miss |= 2;
+                    ok = miss == 0;
                 }
                 if (first && wave == 0) {
                     // special row r = j of w_{j-1}: no stored entry of row j lies in the block, so
@@ -511,6 +592,7 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                     for (int u = 0; u < 4; u++) {
                         const int I = S0 + lane + 64 * u;
                         sp[u] = (I < nts) ? ld_co(pcolm + (I * ldp + j32)) : 0.0;
+                        if (!filled(sp[u])) miss |= 4;
                         ok = ok && filled(sp[u]);
                     }
                     sprow = 0.0;
@@ -527,6 +609,7 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
 #pragma unroll
                         for (int u = 0; u < 8; u++) {
                             sb[u] = (s0 + 4 * u < nslot) ? ld_co(prowm + ((s0 + 4 * u) * ldp + rr32)) : 0.0;
+                            if (!filled(sb[u])) miss |= 8;
                             ok = ok && filled(sb[u]);
                         }
                         pp += ((sb[0] + sb[1]) + (sb[2] + sb[3])) + ((sb[4] + sb[5]) + (sb[6] + sb[7]));
@@ -537,6 +620,7 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                         for (int u = 0; u < PB; u++) {
                             const int I = I0 + 4 * u;
                             buf[u] = (I < nts) ? ld_co(pcolm + (I * ldp + rr32)) : 0.0;
+                            if (!filled(buf[u])) miss |= 16;
                             ok = ok && filled(buf[u]);
                         }
                         double p4[4] = {0.0, 0.0, 0.0, 0.0};
@@ -546,20 +630,23 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                     }
                 }
                 if (__all(ok)) break;
-                if (spins > SPIN_LIMIT) {   // a lost producer: flag it, the next barrier ends the launch
-                    if (lane == 0) __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (!poll_more(spins, err, poll_code(POLL_ROW, j))) {
+                    poll_report(err, miss, r);
+                    alive = false;
                     break;
                 }
-                __builtin_amdgcn_s_sleep(1);
             }
+            // this wave's earlier stores (the empty markers above and of the last pass, phase B's
+            // partials) have completed before anything of column j is stored
+            drain();
+            if (__syncthreads_or(!alive)) return false;
         }
         // ---- 2. shared inputs into LDS (first chunk), the special row of w_{j-1} ----
         if (first) {
             if (doC && tid < im) {
-                sh.t[tid] = tv1;
-                sh.t[NBMAX + tid] = tv2;
-            }
-            if (tid < i) {
+                // t of column j-1 = unit * (row j of W, V) + scale * (sums over rows > j)
+                sh.t[tid] = cs.unit * wraw + tv1;
+                sh.t[NBMAX + tid] = cs.unit * vraw + tv2;
                 sh.wrow[tid] = wraw;
                 sh.vrow[tid] = vraw;
             }
@@ -597,6 +684,7 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
             vr = (r == j) ? cs.unit : cs.scale * xo;
             if (wave == 0) {
                 st_co(Wm + r + im * ldw, wr);
+                st_co(V2 + r + im * ldw, vr);
                 st_co(A + r + jm * lda, vr);
             }
         }
@@ -628,10 +716,11 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
             const bool tail = rin && r >= j + 2;
             if (wave == 0) {
                 const double ss = wave_sum(tail ? x * x : 0.0);
-                if (lane == 0) st_co(pnorm_of(w, j) + q, ss);
+                if (lane == 0) st_co(pnorm_of(w, j) + b, ss);
             }
             // S partials of column j: sum over the chunk's rows r >= j+2 of W[r,k] x_r, V[r,k] x_r
             const double xt = tail ? x : 0.0;
+            double* psb = ps_of(w, j) + b * (2 * NBMAX);
             if (KQ <= 16) {
                 // values 0..15: W[r, k0+u] x_r, 16..31: V[r, k0+u] x_r (zero past the wave's group)
                 double v32[32];
@@ -643,8 +732,7 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                 const double tot = wave_sum32(v32);
                 const int idx = (lane >> 1) & 31;
                 const int u = idx & 15;
-                if ((lane & 1) == 0 && k0 + u < k1)
-                    st_co(ps_of(w, j) + q * (2 * NBMAX) + (idx < 16 ? 0 : NBMAX) + k0 + u, tot);
+                if ((lane & 1) == 0 && k0 + u < k1) st_co(psb + (idx < 16 ? 0 : NBMAX) + k0 + u, tot);
             } else {
 #pragma unroll
                 for (int u = 0; u < KQ; u++) {
@@ -652,8 +740,8 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                         const double s1 = wave_sum(wreg[u] * xt);
                         const double s2 = wave_sum(vreg[u] * xt);
                         if (lane == 0) {
-                            st_co(ps_of(w, j) + q * (2 * NBMAX) + k0 + u, s1);
-                            st_co(ps_of(w, j) + q * (2 * NBMAX) + NBMAX + k0 + u, s2);
+                            st_co(psb + k0 + u, s1);
+                            st_co(psb + NBMAX + k0 + u, s2);
                         }
                     }
                 }
@@ -677,15 +765,17 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
         __syncthreads();
         clk_inner(clk, 7);
     }
+    return true;
 }
 
 // ---------------------------------------------------------------------------------------
-// Phase B of column j, after the first grid barrier of the column:
-//   (1) the reflector scalars beta, tau, 1/(alpha - beta) from the chunk norm partials, formed by
+// Phase B of column j, once the row pass's outputs of column j are in:
+//   (1) the reflector scalars beta, tau, 1/(alpha - beta) from the block norm partials, formed by
 //       every rank in the same fixed order (wave 0, strided lanes, a butterfly), so all ranks hold
 //       bit-identical values;
-//   (2) t1 = W^T v, t2 = V^T v of column j from the per-chunk partials (i entries each, spread
-//       over the ranks' waves);
+//   (2) scale * (sums over rows >= j+2 of W^T x and V^T x) of column j from the per-block
+//       partials, element e of t (W part e < NBMAX, V part NBMAX + k) by wave e mod 4 of rank
+//       (e / 4) mod G for the whole launch, live while k < i (the consumer adds unit * row j+1);
 //   (3) the skew matvec partials of column j over the strictly lower triangle of
 //       A[j+1:n, j+1:n] on MR x MC tiles (strip-major), split into G equal contiguous tile ranges,
 //       one per rank.  Per tile: wave w -> columns 4w..4w+3 (lane >> 4), lane & 15 = p -> row
@@ -694,8 +784,13 @@ __device__ void row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
 //         Pcol[I][c]    = -sum_{r in strip I} A[r,c] v_r                     (one entry per tile)
 //         Prow[slot][r] =  sum over the rank's tiles of strip I of A[r,c] v_c,
 //                          slot = rank - (owner of the strip's first tile)
-// The loads of (1), (2) and of the first tile with its v entries (unscaled x) are issued
-// together, so the phase starts with one memory round trip; v = scale * x is applied after.
+// The first tile is loaded before the poll (tile data is not written while any rank can still read
+// it: a column c > j is stored as a factor in row pass c + 1, after every rank's matvec of column
+// j has been consumed); the poll of (1), (2) and the first tile's v entries (unscaled x) is one
+// memory round trip once they are stored; v = scale * x is applied after.
+// Only ranks with a part in column j take it (tiles, a live t element, or rows of w_j to finish
+// in the next row pass): every value such a rank reads is consumed by a row pass, which is what
+// lets producers refill the slots with the empty marker safely.
 // ---------------------------------------------------------------------------------------
 __device__ __forceinline__ void tall_load(const double* __restrict__ A, int64_t lda, int64_t n, int64_t j,
                                           int64_t I, int64_t c, int p, double a[16])
@@ -731,41 +826,122 @@ __device__ __forceinline__ double vfrom(int64_t n, int64_t j, int64_t c, double 
     return (c == j + 1) ? s.unit : s.scale * x;
 }
 
+// does rank g own a t element that is live at local column i?
+__device__ __forceinline__ bool t_live(int g, int G, int i)
+{
+    for (int q = 4 * g; q < 2 * NBMAX; q += 4 * G)
+        if ((q % NBMAX) < i) return true;
+    return false;
+}
+
 __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j, int i,
                         const Work& w, RowShared& sh, double* tau_out, double* eta_out, int g, int G,
-                        const PanelMap& pm, unsigned* bar, unsigned epoch, int* err, int* s_ok,
-                        unsigned long long* clk)
+                        const PanelMap& pm, bool rown, int* err, unsigned long long* clk)
 {
+    const bool mv = g < pm.Ge;
+    if (!(mv || rown || t_live(g, G, i))) return true;
     clk_begin(clk);
     const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
     const int p = lane & 15, cg = lane >> 4;
-    const int64_t nq = cdiv(n, TB) - j / TB;
-    // ---- before the barrier completes: this rank's first tile (tile data is never written during
-    // the launch) ----
+    const int64_t b0 = j / TB;
+    const int64_t nt = cdiv(n, TB);
+    const int64_t nq = nt - b0;
     const TallGrid& tg = pm.tg;
-    const bool mv = g < pm.Ge;
     const int64_t tlo = pm.tlo, thi = pm.thi;
     int64_t I = pm.I, J = pm.J, cur_slot = pm.slot, c = 0;
-    double a[16], xs = 0.0, xc = 0.0;
+    double a[16];
     if (mv) {
         c = J * MC + 4 * wave + cg;
         tall_load(A, lda, n, j, I, c, p, a);
     }
     clk_inner(clk, 11);
-    if (!grid_wait_co(bar, epoch, err, s_ok)) return false;
+    // ---- poll what the row pass of column j produced ----
+    const double* pnj = pnorm_of(w, j);
+    const double* psj = ps_of(w, j);
+    const double* xj = x_of(w, j);
+    const int e0 = 4 * g + wave;                                 // this wave's first t element
+    const bool t0 = e0 < 2 * NBMAX && (e0 % NBMAX) < i;          // wave-uniform
+    const int64_t rs = I * MR + tid;                             // this thread's row of the first strip
+    double pn0 = 0.0, pn1 = 0.0, araw = 0.0, ps0 = 0.0, ps1 = 0.0, xc = 0.0, xs = 0.0;
+    bool alive = true;
+    for (unsigned spins = 0;; spins++) {
+        unsigned miss = 0;
+        if (wave == 0) {
+            if (lane < nq) {
+                pn0 = ld_co(pnj + b0 + lane);
+                if (!filled(pn0)) miss |= 32;
+            }
+            if (lane + 64 < nq) {
+                pn1 = ld_co(pnj + b0 + lane + 64);
+                if (!filled(pn1)) miss |= 32;
+            }
+            if (lane == 0) {
+                araw = ld_co(xj + (j + 1));
+                if (!filled(araw)) miss |= 64;
+            }
+        }
+        if (t0) {
+            if (lane < nq) {
+                ps0 = ld_co(psj + (b0 + lane) * (2 * NBMAX) + e0);
+                if (!filled(ps0)) miss |= 128;
+            }
+            if (lane + 64 < nq) {
+                ps1 = ld_co(psj + (b0 + lane + 64) * (2 * NBMAX) + e0);
+                if (!filled(ps1)) miss |= 128;
+            }
+        }
+        if (mv) {
+            if (c > j + 1 && c < n) {
+                xc = ld_co(xj + c);
+                if (!filled(xc)) miss |= 256;
+            }
+            if (rs > j + 1 && rs < n) {
+                xs = ld_co(xj + rs);
+                if (!filled(xs)) miss |= 512;
+            }
+        }
+        if (__all(miss == 0)) break;
+        if (!poll_more(spins, err, poll_code(POLL_B, j))) {
+            poll_report(err, miss, (miss & 256) ? c : rs);
+            alive = false;
+            break;
+        }
+    }
+    // the empty markers this rank stored in the last phase B have completed (they must before any
+    // value of column j + 1 leaves this rank)
+    drain();
     clk_inner(clk, 12);
-    // EMPTY into this rank's row and column partial slots of the other parity (column j - 1's values,
-    // consumed before this column's barrier; column j + 1 refills them after the next one), issued
-    // here so that the stores complete behind the matvec
+    // ---- (1) scalars ----
+    if (wave == 0 && alive) {
+        double acc = pn0 + pn1;
+        for (int64_t q = lane + 128; q < nq; q += 64) {
+            const double x = ld_co(pnj + b0 + q);
+            acc += settle(x, pnj + b0 + q, err, poll_code(POLL_SETTLE, j));
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) {
+            const ColScal s = reflector_scalars(araw, acc);
+            sh.cur = s;
+            if (g == 0) {
+                tau_out[j] = s.tau;
+                eta_out[j] = s.beta;
+            }
+        }
+    }
+    if (__syncthreads_or(!alive)) return false;
+    // ---- empty this rank's hand-over slots of column j + 2 (copy (j + 2) mod 3 = (j - 1) mod 3):
+    // column j - 1's values there were consumed by the row passes of column j, which all finished
+    // before their norm partials of column j, which wave 0 has seen (hence after the barrier: the
+    // other waves may have had nothing to poll) ----
     if (mv) {
-        double* prn = prow_of(w, j + 1);
-        double* pcn = pcol_of(w, j + 1);
+        double* prn = prow_of(w, j + 2);
+        double* pcn = pcol_of(w, j + 2);
         const double e = empty_slot();
         int64_t It = I, Jt = J, slot = cur_slot;
         for (int64_t t = tlo; t < thi; t++) {
             if (wave == 0 && lane < MC) {
                 const int64_t cc = Jt * MC + lane;
-                if (cc > j && cc < n) st_co(pcn + It * w.ldp + cc, e);
+                if (cc > j + 2 && cc < n) st_co(pcn + It * w.ldp + cc, e);
             }
             const bool lis = (Jt == MPS * It + MPS - 1);
             if (lis || t + 1 == thi) {
@@ -781,60 +957,30 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
             }
         }
     }
-    // ---- loads of what the row pass produced ----
-    double pn0 = 0.0, pn1 = 0.0, araw = 0.0;
-    if (wave == 0) {
-        pn0 = (lane < nq) ? ld_co(pnorm_of(w, j) + lane) : 0.0;
-        pn1 = (lane + 64 < nq) ? ld_co(pnorm_of(w, j) + lane + 64) : 0.0;
-        if (lane == 0) araw = ld_co(x_of(w, j) + (j + 1));
-    }
-    const int tv0 = g * 4 + wave;
-    double ps0 = 0.0, ps1 = 0.0, head0 = 0.0;
-    if (tv0 < 2 * i) {
-        const int k = tv0 % i;
-        const int off = (tv0 < i) ? 0 : NBMAX;
-        ps0 = (lane < nq) ? ld_co(ps_of(w, j) + (int64_t)lane * (2 * NBMAX) + off + k) : 0.0;
-        ps1 = (lane + 64 < nq) ? ld_co(ps_of(w, j) + (int64_t)(lane + 64) * (2 * NBMAX) + off + k) : 0.0;
-        if (lane == 0) head0 = (tv0 < i) ? ld_co(w.W + (j + 1) + k * w.ldw) : ld_co(A + (j + 1) + (c0 + k) * lda);
-    }
-    if (mv) {
-        xc = xraw(x_of(w, j), n, j, c);
-        xs = xraw(x_of(w, j), n, j, I * MR + tid);   // this thread's row of the strip
-    }
-    // ---- (1) scalars ----
-    if (wave == 0) {
-        double acc = pn0 + pn1;
-        for (int64_t q = lane + 128; q < nq; q += 64) acc += ld_co(pnorm_of(w, j) + q);
-        acc = wave_sum(acc);
-        if (lane == 0) {
-            const ColScal s = reflector_scalars(araw, acc);
-            sh.cur = s;
-            if (g == 0) {
-                tau_out[j] = s.tau;
-                eta_out[j] = s.beta;
-            }
-        }
-    }
-    __syncthreads();
+    if (lane == 0)
+        for (int e = e0; e < 2 * NBMAX; e += 4 * G) st_co(tv_of(w, j + 2) + e, empty_slot());
     clk_inner(clk, 8);
     const ColScal s = sh.cur;
-    if (mv) sh.vstrip[tid] = vfrom(n, j, I * MR + tid, xs, s);
-    // ---- (2) t1, t2 ----
-    for (int tv = tv0; tv < 2 * i; tv += G * 4) {
-        const int k = tv % i;
-        const int off = (tv < i) ? 0 : NBMAX;
-        double acc, head;
-        if (tv == tv0) {
+    if (mv) sh.vstrip[tid] = vfrom(n, j, rs, xs, s);
+    // ---- (2) t of the live elements of this wave ----
+    for (int e = e0; e < 2 * NBMAX; e += 4 * G) {
+        if ((e % NBMAX) >= i) continue;   // wave-uniform
+        double acc;
+        if (e == e0) {
             acc = ps0 + ps1;
-            head = head0;
         } else {
             acc = 0.0;
-            for (int64_t q = lane; q < 128 && q < nq; q += 64) acc += ld_co(ps_of(w, j) + q * (2 * NBMAX) + off + k);
-            head = (lane == 0) ? ((tv < i) ? ld_co(w.W + (j + 1) + k * w.ldw) : ld_co(A + (j + 1) + (c0 + k) * lda)) : 0.0;
+            for (int64_t q = lane; q < 128 && q < nq; q += 64) {
+                const double* pq = psj + (b0 + q) * (2 * NBMAX) + e;
+                acc += settle(ld_co(pq), pq, err, poll_code(POLL_SETTLE, j));
+            }
         }
-        for (int64_t q = lane + 128; q < nq; q += 64) acc += ld_co(ps_of(w, j) + q * (2 * NBMAX) + off + k);
+        for (int64_t q = lane + 128; q < nq; q += 64) {
+            const double* pq = psj + (b0 + q) * (2 * NBMAX) + e;
+            acc += settle(ld_co(pq), pq, err, poll_code(POLL_SETTLE, j));
+        }
         acc = wave_sum(acc);
-        if (lane == 0) handover(tv_of(w, j) + off + k, w.hstride, j, s.unit * head + s.scale * acc);
+        if (lane == 0) st_co(tv_of(w, j) + e, s.scale * acc);
     }
     clk_inner(clk, 9);
     // ---- (3) matvec ----
@@ -850,11 +996,11 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         const int64_t In = last_in_strip ? I + 1 : I;
         const int64_t Jn = last_in_strip ? tg.Jlo : J + 1;
         const int64_t cn = Jn * MC + 4 * wave + cg;
-        double an[16], vcn = 0.0, xsn = 0.0;
+        double an[16], xcn = 0.0, xsn = 0.0;
         if (more) {
-            vcn = vval(x_of(w, j), n, j, cn, s);
+            xcn = xraw(xj, n, j, cn);
             tall_load(A, lda, n, j, In, cn, p, an);
-            if (In != I) xsn = xraw(x_of(w, j), n, j, In * MR + tid);
+            if (In != I) xsn = xraw(xj, n, j, In * MR + tid);
         }
         double cp = 0.0;
 #pragma unroll
@@ -879,7 +1025,7 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
                 if (r < n)
                     st_co(prow_of(w, j) + cur_slot * w.ldp + r,
                           (sh.mred[0][tid] + sh.mred[1][tid]) + (sh.mred[2][tid] + sh.mred[3][tid]));
-                if (more) sh.vstrip[tid] = vfrom(n, j, In * MR + tid, xsn, s);
+                if (more) sh.vstrip[tid] = vfrom(n, j, In * MR + tid, settle(xsn, xj + In * MR + tid, err, poll_code(POLL_SETTLE, j)), s);
             }
             __syncthreads();
             cur_slot = 0;   // this rank owns the first tile of every later strip it touches
@@ -887,7 +1033,7 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         if (more) {
 #pragma unroll
             for (int k = 0; k < 16; k++) a[k] = an[k];
-            vc = vcn;
+            vc = vfrom(n, j, cn, settle(xcn, xj + cn, err, poll_code(POLL_SETTLE, j)), s);
         }
         I = In;
         J = Jn;

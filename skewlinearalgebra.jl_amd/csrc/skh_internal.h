@@ -16,8 +16,6 @@ struct PanelLaunch {
     skh::Work w;
     double* tau;
     double* eta;
-    unsigned* bar;
-    unsigned base;
     int* err;
     unsigned long long* ph;   // optional phase clock (4 counters) or null
     int grid;
@@ -112,8 +110,17 @@ int skh_internal_launch_cblock(const CompactArgs* a, int nmax, int nb, int grid,
 int skh_internal_cblock_occupancy(int nmax, int nb, int* blocks_per_cu);
 int skh_internal_logpf_from(const double* eta, const double* tau, int64_t n, double* logpf, double* sgn, hipStream_t st);
 int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st);
-int skh_internal_clear_partials(double* tv0, double* tv1, int64_t ntv, double* slots, int64_t ld, int64_t r0,
-                                int64_t n, int64_t nslots, hipStream_t st);
+// Segments of device memory to fill with the panel kernel's empty marker: entries [lo, hi) of
+// `rows` rows, ld doubles apart, at p (one launch for all segments).
+struct FillSegs {
+    static constexpr int MAXSEG = 12;
+    double* p[MAXSEG];
+    int64_t rows[MAXSEG];
+    int64_t ld[MAXSEG];
+    int64_t lo[MAXSEG];
+    int64_t hi[MAXSEG];
+};
+int skh_internal_fill_empty(const FillSegs* fs, int nseg, hipStream_t st);
 int skh_internal_panel_occupancy(int nbmax, int* blocks_per_cu);
 int skh_internal_launch_rank2k(double* A, int64_t lda, const double* U, int64_t ldu, int64_t n, int64_t e,
                                int64_t c0, int nbp, hipStream_t st);

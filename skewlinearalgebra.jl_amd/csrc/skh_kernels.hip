@@ -20,22 +20,23 @@
 namespace skh {
 
 // Optional phase clock (workgroup 0, 100 MHz wall clock), accumulated over all launches:
-// ph[0] time before barrier 1's arrival (0 after the first column), ph[1] barrier 1 arrivals,
-// ph[2] phase B (barrier 1 wait, scalars, t, matvec), ph[3] row passes (with the wait for the
-// tagged partials); inside them ph[4..7] the row pass (loads, correction of w, column update with
-// its partials, hand-over to the next column) and phase B: ph[11] tile range and first tile,
-// ph[12] barrier 1 wait, ph[8] loads and scalars, ph[9] t, ph[10] matvec.
+// ph[2] phase B (poll of the row pass's outputs, scalars, t, matvec), ph[3] row passes (with the
+// poll of the matvec partials); inside them ph[4..7] the row pass (loads and poll, correction of
+// w, column update with its partials, hand-over to the next column) and phase B: ph[11] tile
+// range and first tile, ph[12] poll, ph[8] scalars, ph[9] t, ph[10] matvec.  ph[0] and ph[1] are
+// the time between the phases (no grid barrier any more: both stay near zero).
 // panels up to 64 wide fit two workgroups per CU (<= 256 registers per lane); 128 wide takes one.
 // MULTI: the grid has fewer workgroups than the panel has 64-row blocks, so a workgroup also
 // processes blocks other than its own (rows of V and W then come from memory); the common
 // instance (one block per workgroup) carries none of that path.
+// The launch has no grid barrier: each workgroup polls the hand-over slots of its inputs (see
+// Work).  All workgroups are co-resident (cooperative launch), and every poll is bounded: a lost
+// producer flags `err` and ends the launch.
 template <int KQ, bool MULTI>
 __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int64_t lda, int64_t n, int64_t c0, int nbp, Work w,
-                                              double* tau, double* eta, unsigned* bar, unsigned base, int* err,
-                                              unsigned long long* ph)
+                                              double* tau, double* eta, int* err, unsigned long long* ph)
 {
     __shared__ RowShared sh;
-    __shared__ int s_ok;
     __shared__ unsigned long long s_clk[PH_SLOTS];
     const int G = gridDim.x, g = blockIdx.x;
     unsigned long long* clk = ((ph != nullptr) && g == 0 && threadIdx.x == 0) ? s_clk : nullptr;
@@ -44,7 +45,6 @@ __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int
         clk_begin(clk);
         clk[PH_OUTER] = clk[PH_INNER];
     }
-    unsigned epoch = base;   // barriers completed before this launch
     __shared__ double s_pv[4 * KQ * 64], s_pw[4 * KQ * 64];
     RowRegs<KQ> rr;
     rr.v = s_pv;
@@ -59,17 +59,16 @@ __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int
         rr.xnext = (r < n && r > c0) ? A[r + c0 * lda] : 0.0;
     }
     const PanelMap pm = panel_map(n, c0, g, G);
-    row_pass<KQ, MULTI>(A, lda, n, c0, c0, 0, false, true, w, g, G, sh, rr, pm, err, clk);
-    for (int i = 0; i < nbp; i++) {
+    // last 64-row block this workgroup finalises rows of (-1: none)
+    const int64_t nt = cdiv(n, TB), bown = c0 / TB + g;
+    const int64_t blast = (bown < nt) ? (MULTI ? bown + (int64_t)G * ((nt - 1 - bown) / G) : bown) : -1;
+    bool ok = row_pass<KQ, MULTI>(A, lda, n, c0, c0, 0, false, true, w, g, G, sh, rr, pm, err, clk);
+    for (int i = 0; ok && i < nbp; i++) {
         const int64_t j = c0 + i;
         clk_outer(clk, 0);
-        grid_arrive_co(bar, ++epoch);   // phase B waits once its first tile is in flight
-        clk_outer(clk, 1);
-        if (!phase_b(A, lda, n, c0, j, i, w, sh, tau, eta, g, G, pm, bar, epoch, err, &s_ok, clk)) return;
+        ok = phase_b(A, lda, n, c0, j, i, w, sh, tau, eta, g, G, pm, blast >= (j + 1) / TB, err, clk);
         clk_outer(clk, 2);
-        // no grid barrier here: the row-block owners poll the tagged partial sums of their rows,
-        // every other workgroup goes on to the next barrier
-        row_pass<KQ, MULTI>(A, lda, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, g, G, sh, rr, pm, err, clk);
+        if (ok) ok = row_pass<KQ, MULTI>(A, lda, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, g, G, sh, rr, pm, err, clk);
         clk_outer(clk, 3);
     }
     if (clk) {
@@ -102,19 +101,16 @@ __global__ void k_finish(double* A, int64_t lda, int64_t n, double* tau, double*
     }
 }
 
-// mark empty t of both parities (ntv entries at tv0 and tv1) and rows [r0, n) of nslots slots of ld
-// doubles (the matvec partials of the launch's first column)
-__global__ void k_clear_partials(double* tv0, double* tv1, int64_t ntv, double* slots, int64_t ld, int64_t r0,
-                                 int64_t n)
+// fill entries [lo, hi) of `rows` rows (ld doubles apart) at each segment's base with the empty marker
+__global__ void k_fill_empty(FillSegs fs)
 {
     const double e = empty_slot();
-    if (blockIdx.y == 0 && blockIdx.x == 0)
-        for (int64_t k = threadIdx.x; k < ntv; k += blockDim.x) {
-            tv0[k] = e;
-            tv1[k] = e;
-        }
-    double* s = slots + (int64_t)blockIdx.y * ld;
-    for (int64_t r = r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+    const int sgi = blockIdx.z;
+    const int64_t row = blockIdx.y;
+    if (row >= fs.rows[sgi]) return;
+    double* s = fs.p[sgi] + row * fs.ld[sgi];
+    for (int64_t r = fs.lo[sgi] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < fs.hi[sgi];
+         r += (int64_t)gridDim.x * blockDim.x)
         s[r] = e;
 }
 
@@ -301,19 +297,24 @@ static const void* panel_kernel_for(int nbmax, int multi)
 extern "C" int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st)
 {
     void* args[] = {(void*)&pl->A,   (void*)&pl->lda, (void*)&pl->n,   (void*)&pl->c0,  (void*)&pl->nbp,
-                    (void*)&pl->w,   (void*)&pl->tau, (void*)&pl->eta, (void*)&pl->bar, (void*)&pl->base,
-                    (void*)&pl->err, (void*)&pl->ph};
+                    (void*)&pl->w,   (void*)&pl->tau, (void*)&pl->eta, (void*)&pl->err, (void*)&pl->ph};
     return (int)hipLaunchCooperativeKernel(panel_kernel_for(pl->nbmax, pl->multi), dim3(pl->grid), dim3(WG), args, 0,
                                            st);
 }
 
-extern "C" int skh_internal_clear_partials(double* tv0, double* tv1, int64_t ntv, double* slots, int64_t ld, int64_t r0,
-                                           int64_t n, int64_t nslots, hipStream_t st)
+extern "C" int skh_internal_fill_empty(const FillSegs* fs, int nseg, hipStream_t st)
 {
-    if (nslots <= 0) return 0;
-    const int64_t rows = std::max<int64_t>(1, n - r0);
-    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((rows + 255) / 256, 8));
-    hipLaunchKernelGGL(k_clear_partials, dim3(gx, (unsigned)nslots), dim3(256), 0, st, tv0, tv1, ntv, slots, ld, r0, n);
+    if (nseg < 0 || nseg > FillSegs::MAXSEG) return (int)hipErrorInvalidValue;
+    if (nseg == 0) return 0;
+    FillSegs f = *fs;
+    int64_t ymax = 1, cnt = 1;
+    for (int k = 0; k < nseg; k++) {
+        ymax = std::max(ymax, f.rows[k]);
+        cnt = std::max(cnt, f.hi[k] - f.lo[k]);
+    }
+    for (int k = nseg; k < FillSegs::MAXSEG; k++) f.rows[k] = 0;
+    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((cnt + 255) / 256, 16));
+    hipLaunchKernelGGL(k_fill_empty, dim3(gx, (unsigned)ymax, (unsigned)nseg), dim3(256), 0, st, f);
     return (int)hipGetLastError();
 }
 

@@ -2,8 +2,8 @@
 //
 // One file-scope device array per GPU holds what the large engine (cooperative panel kernel +
 // MFMA rank-2k update) otherwise takes from the library's cached buffers: the 2 KiB control
-// block (grid-barrier counters, error flag, phase clocks) followed by the workspace (W, the norm
-// and W^T x / V^T x partials, the double-buffered column, t, the matvec row and column partials).
+// block (error flag, phase clocks) followed by the workspace (W and V of the panel, the copies of
+// the norm and W^T x / V^T x partials and of the column, t, the matvec row and column partials).
 // It is sized for n <= SKH_NOALLOC_MAX_N at any panel width up to skh::NBMAX on a 256-CU device.
 //
 // In host code `scratch` names a host-side stand-in, not device memory: its device address is
@@ -16,9 +16,10 @@
 namespace {
 
 // control block in doubles (2048 bytes), then the workspace at n = 8192, nb = 128, 514 row-partial
-// slots: ld nb + 2 nt + 4 nt NBMAX + 2 ld + 2 (2 NBMAX + (nseg + nt) ld) + 64 = 11,666,754 doubles
+// slots: 2 ld nb + 4 (nt + 2 nt NBMAX + ld) + 3 (2 NBMAX + (nseg + nts) ld) + 64 = 15,706,440
+// doubles (ld = 8192, nt = 129, nts = 33)
 constexpr size_t kCtlDoubles = 256;
-constexpr size_t kScratchDoubles = 11700000;
+constexpr size_t kScratchDoubles = 15710000;
 
 static __device__ __align__(256) double scratch[kScratchDoubles];
 
