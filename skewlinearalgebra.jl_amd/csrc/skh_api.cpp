@@ -245,6 +245,9 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
         int rc = get_ctl(ctl);
         if (rc) return rc;
     }
+    // the matvec reads 16-byte row pairs at even rows < n: an even leading dimension keeps them
+    // inside their column
+    if ((lda & 1) || lda < n) return set_err(-3, "lda must be >= n and even");
     const int64_t ld = (n + 1) & ~int64_t(1);
     const int64_t nt = (n + skh::TB - 1) / skh::TB + 1;
     const int64_t nts = (n + skh::MR - 1) / skh::MR + 1;

@@ -514,7 +514,7 @@ __device__ bool row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
     const int64_t bown = c0 / TB + g; // this workgroup's own block
     const int k0 = wave * KQ, k1 = min(i, k0 + KQ);
     const int kc = doC ? min(k1, im) : k0;   // C correction uses k in [k0, kc)
-    constexpr int PB = 16;            // column partials per lane per batch (4 waves: 64 strips)
+    constexpr int PB = 8;             // column partials per lane per batch (4 waves: 32 strips)
     const TallGrid& tg = pm.tg;       // tiles of the matvec that produced the partials
     bool shared_done = false;
 
@@ -564,72 +564,113 @@ __device__ bool row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
         double pp = 0.0;
         if (doC) {
             // t and the special row's column partials (phase B of column j-1), row j of W and V
-            // (earlier row passes) and this row's matvec partials: each wave polls its share until
-            // all have arrived (32-bit element offsets: (slots + strips) * ldp stays far below 2^31)
+            // (earlier row passes) and this row's matvec partials, polled until all have arrived.
+            // Every load of a round is issued before any is looked at (atomic loads keep their
+            // order, and a check between them would make the compiler wait for each); lanes that
+            // need a value load a valid slot instead and ignore it.  32-bit element offsets:
+            // (slots + strips) * ldp stays far below 2^31.
             const int S = (int)((b * TB) / MR);    // matvec strip holding this chunk
-            const int nslot = rowC ? (int)(own ? pm.nslot_own : tall_slots(tg, S, G)) : 0;
-            const int nts = (int)tg.nts, ldp = (int)w.ldp, rr32 = (int)r;
+            const int nsl = (int)(own ? pm.nslot_own : tall_slots(tg, S, G));
+            const int nts = (int)tg.nts, ldp = (int)w.ldp;
+            const int rr32 = (int)(rowC ? r : j);
             const int S0 = (int)(j / MR), j32 = (int)j;
             const double* prowm = prow_of(w, jm);
             const double* pcolm = pcol_of(w, jm);
+            const double* tvm = tv_of(w, jm);
+            const bool ga = first && im > 0;      // wave-uniform
+            const int kt = (tid < im) ? tid : 0;
+            const bool spw = first && wave == 0;  // wave-uniform
             bool alive = true;
             for (unsigned spins = 0;; spins++) {
-                bool ok = true;
-                unsigned miss = 0;
-                if (first && tid < im) {
-                    tv1 = ld_co(tv_of(w, jm) + tid);
-                    tv2 = ld_co(tv_of(w, jm) + NBMAX + tid);
-                    wraw = ld_co(Wm + j + tid * ldw);
-                    vraw = ld_co(V2 + j + tid * ldw);
-                    if (!(filled(tv1) && filled(tv2))) miss |= 1;
-                    if (!(filled(wraw) && filled(vraw))) miss |= 2;
-                    ok = miss == 0;
+                if (ga) {
+                    tv1 = ld_co(tvm + kt);
+                    tv2 = ld_co(tvm + NBMAX + kt);
+                    wraw = ld_co(Wm + j + kt * ldw);
+                    vraw = ld_co(V2 + j + kt * ldw);
                 }
-                if (first && wave == 0) {
-                    // special row r = j of w_{j-1}: no stored entry of row j lies in the block, so
-                    // its row sums are empty; column partials of the strips I >= j / MR
+                if (spw) {
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
                         const int I = S0 + lane + 64 * u;
-                        sp[u] = (I < nts) ? ld_co(pcolm + (I * ldp + j32)) : 0.0;
-                        if (!filled(sp[u])) miss |= 4;
-                        ok = ok && filled(sp[u]);
+                        sp[u] = ld_co(pcolm + ((I < nts ? I : S0) * ldp + j32));
                     }
-                    sprow = 0.0;
+                }
+                double sb[8], buf[PB];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int sl = wave + 4 * u;
+                    sb[u] = ld_co(prowm + ((sl < nsl ? sl : 0) * ldp + rr32));
+                }
+#pragma unroll
+                for (int u = 0; u < PB; u++) {
+                    const int I = S + wave + 4 * u;
+                    buf[u] = ld_co(pcolm + ((I < nts ? I : S) * ldp + rr32));
+                }
+                unsigned miss = 0;
+                if (ga && tid < im) {
+                    if (!(filled(tv1) && filled(tv2))) miss |= 1;
+                    if (!(filled(wraw) && filled(vraw))) miss |= 2;
+                }
+                if (spw) {
+#pragma unroll
+                    for (int u = 0; u < 4; u++)
+                        if (S0 + lane + 64 * u < nts && !filled(sp[u])) miss |= 4;
+                }
+                pp = 0.0;
+                if (rowC) {
+                    // first batches of row slots and strips, in a fixed order ahead of the extra ones
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        if (wave + 4 * u >= nsl) sb[u] = 0.0;
+                        if (!filled(sb[u])) miss |= 8;
+                    }
+                    double p4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int u = 0; u < PB; u++) {
+                        if (S + wave + 4 * u >= nts) buf[u] = 0.0;
+                        if (!filled(buf[u])) miss |= 16;
+                        p4[u & 3] += buf[u];
+                    }
+                    pp = (((sb[0] + sb[1]) + (sb[2] + sb[3])) + ((sb[4] + sb[5]) + (sb[6] + sb[7]))) +
+                         ((p4[0] + p4[1]) + (p4[2] + p4[3]));
+                }
+                // rare extra batches: strips beyond the first 256 for the special row (n > 65536),
+                // more than 32 row slots, more than 64 strips below the chunk (n > 16384)
+                sprow = 0.0;
+                if (spw) {
                     for (int I = S0 + lane + 256; I < nts; I += 64) {
                         const double x = ld_co(pcolm + (I * ldp + j32));
-                        ok = ok && filled(x);
+                        if (!filled(x)) miss |= 4;
                         sprow += x;
                     }
                 }
                 if (rowC) {
-                    pp = 0.0;
-                    for (int s0 = wave; s0 < nslot; s0 += 32) {
-                        double sb[8];
+                    for (int s0 = wave + 32; s0 < nsl; s0 += 32) {
+                        double xb[8];
+#pragma unroll
+                        for (int u = 0; u < 8; u++) xb[u] = ld_co(prowm + ((s0 + 4 * u < nsl ? s0 + 4 * u : 0) * ldp + rr32));
 #pragma unroll
                         for (int u = 0; u < 8; u++) {
-                            sb[u] = (s0 + 4 * u < nslot) ? ld_co(prowm + ((s0 + 4 * u) * ldp + rr32)) : 0.0;
-                            if (!filled(sb[u])) miss |= 8;
-                            ok = ok && filled(sb[u]);
+                            if (s0 + 4 * u >= nsl) xb[u] = 0.0;
+                            if (!filled(xb[u])) miss |= 8;
                         }
-                        pp += ((sb[0] + sb[1]) + (sb[2] + sb[3])) + ((sb[4] + sb[5]) + (sb[6] + sb[7]));
+                        pp += ((xb[0] + xb[1]) + (xb[2] + xb[3])) + ((xb[4] + xb[5]) + (xb[6] + xb[7]));
                     }
-                    for (int I0 = S + wave; I0 < nts; I0 += 4 * PB) {
-                        double buf[PB];
+                    for (int I0 = S + wave + 4 * PB; I0 < nts; I0 += 4 * PB) {
+                        double xb[PB];
 #pragma unroll
-                        for (int u = 0; u < PB; u++) {
-                            const int I = I0 + 4 * u;
-                            buf[u] = (I < nts) ? ld_co(pcolm + (I * ldp + rr32)) : 0.0;
-                            if (!filled(buf[u])) miss |= 16;
-                            ok = ok && filled(buf[u]);
-                        }
+                        for (int u = 0; u < PB; u++) xb[u] = ld_co(pcolm + ((I0 + 4 * u < nts ? I0 + 4 * u : S) * ldp + rr32));
                         double p4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                        for (int u = 0; u < PB; u++) p4[u & 3] += buf[u];
+                        for (int u = 0; u < PB; u++) {
+                            if (I0 + 4 * u >= nts) xb[u] = 0.0;
+                            if (!filled(xb[u])) miss |= 16;
+                            p4[u & 3] += xb[u];
+                        }
                         pp += (p4[0] + p4[1]) + (p4[2] + p4[3]);
                     }
                 }
-                if (__all(ok)) break;
+                if (__all(miss == 0)) break;
                 if (!poll_more(spins, err, poll_code(POLL_ROW, j))) {
                     poll_report(err, miss, r);
                     alive = false;
@@ -640,6 +681,11 @@ __device__ bool row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
             // partials) have completed before anything of column j is stored
             drain();
             if (__syncthreads_or(!alive)) return false;
+            if (spw) {
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (S0 + lane + 64 * u >= nts) sp[u] = 0.0;
+            }
         }
         // ---- 2. shared inputs into LDS (first chunk), the special row of w_{j-1} ----
         if (first) {
@@ -792,25 +838,31 @@ __device__ bool row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
 // in the next row pass): every value such a rank reads is consumed by a row pass, which is what
 // lets producers refill the slots with the empty marker safely.
 // ---------------------------------------------------------------------------------------
-__device__ __forceinline__ void tall_load(const double* __restrict__ A, int64_t lda, int64_t n, int64_t j,
-                                          int64_t I, int64_t c, int p, double a[16])
+// Raw tile entries of one lane: rows I*MR + 32 h + 2p (+1), column c, as 16-byte pairs.  The loads
+// are unconditional and always in bounds (a row or column past the matrix reads row 0 or column
+// n-1; lda is even and >= n, so the pair at an even row < n stays inside its column), and nothing
+// is masked here: a mask right after a load makes the compiler wait for it, which would serialise
+// the prefetch with the reduction of the current tile.  tall_ok masks at use.
+__device__ __forceinline__ void tall_load(const double* __restrict__ A, int64_t lda, int64_t n, int64_t I, int64_t c,
+                                          int p, double a[16])
 {
-    const bool cok = (c > j) && (c < n);
-    const double* col = A + c * lda;
+    const double* col = A + ((c < n) ? c : n - 1) * lda + I * MR;
+    const int rn = (int)(n - I * MR), rb = 2 * p;   // rows relative to the strip
 #pragma unroll
     for (int h = 0; h < 8; h++) {
-        const int64_t r = I * MR + 32 * h + 2 * p;
-        double x0 = 0.0, x1 = 0.0;
-        if (cok && r + 1 < n) {
-            const double2 t = *reinterpret_cast<const double2*>(col + r);
-            x0 = t.x;
-            x1 = t.y;
-        } else if (cok && r < n) {
-            x0 = col[r];
-        }
-        a[2 * h] = (r > c) ? x0 : 0.0;
-        a[2 * h + 1] = (r + 1 > c) ? x1 : 0.0;
+        const int r = rb + 32 * h;
+        const double2 t = *reinterpret_cast<const double2*>(col + ((r < rn) ? r : -(int)(I * MR)));
+        a[2 * h] = t.x;
+        a[2 * h + 1] = t.y;
     }
+}
+
+// entry k of a lane's tile (row I*MR + 32 (k>>1) + 2p + (k&1), column c) lies in the strictly lower
+// triangle of the trailing block of column j: lo = c - I*MR and hi = n - I*MR bound the local row
+__device__ __forceinline__ bool tall_ok(int k, int p, int64_t lo, int64_t hi)
+{
+    const int rl = 32 * (k >> 1) + 2 * p + (k & 1);
+    return rl > lo && rl < hi;
 }
 
 // unscaled x_c of column j (x_j in xj) where v_c = scale * x_c (0 elsewhere)
@@ -852,7 +904,7 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
     double a[16];
     if (mv) {
         c = J * MC + 4 * wave + cg;
-        tall_load(A, lda, n, j, I, c, p, a);
+        tall_load(A, lda, n, I, c, p, a);
     }
     clk_inner(clk, 11);
     // ---- poll what the row pass of column j produced ----
@@ -863,43 +915,38 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
     const bool t0 = e0 < 2 * NBMAX && (e0 % NBMAX) < i;          // wave-uniform
     const int64_t rs = I * MR + tid;                             // this thread's row of the first strip
     double pn0 = 0.0, pn1 = 0.0, araw = 0.0, ps0 = 0.0, ps1 = 0.0, xc = 0.0, xs = 0.0;
+    // every load of a round is issued before any is looked at (the loads are atomic, so the compiler
+    // keeps their order and would otherwise wait for each before issuing the next); out-of-range
+    // lanes load a valid slot and ignore it
+    const bool in0 = lane < nq, in1 = lane + 64 < nq;
+    const int64_t q0 = b0 + (in0 ? lane : 0), q1 = b0 + (in1 ? lane + 64 : 0);
+    const bool xcin = mv && c > j + 1 && c < n, xsin = mv && rs > j + 1 && rs < n;
+    const double* pxc = xj + (xcin ? c : j + 1);
+    const double* pxs = xj + (xsin ? rs : j + 1);
+    const int64_t e0c = t0 ? e0 : 0;
     bool alive = true;
     for (unsigned spins = 0;; spins++) {
-        unsigned miss = 0;
-        if (wave == 0) {
-            if (lane < nq) {
-                pn0 = ld_co(pnj + b0 + lane);
-                if (!filled(pn0)) miss |= 32;
-            }
-            if (lane + 64 < nq) {
-                pn1 = ld_co(pnj + b0 + lane + 64);
-                if (!filled(pn1)) miss |= 32;
-            }
-            if (lane == 0) {
-                araw = ld_co(xj + (j + 1));
-                if (!filled(araw)) miss |= 64;
-            }
+        if (wave == 0) {   // wave-uniform branches: no per-lane exec mask around the loads
+            pn0 = ld_co(pnj + q0);
+            pn1 = ld_co(pnj + q1);
+            araw = ld_co(xj + (j + 1));
         }
         if (t0) {
-            if (lane < nq) {
-                ps0 = ld_co(psj + (b0 + lane) * (2 * NBMAX) + e0);
-                if (!filled(ps0)) miss |= 128;
-            }
-            if (lane + 64 < nq) {
-                ps1 = ld_co(psj + (b0 + lane + 64) * (2 * NBMAX) + e0);
-                if (!filled(ps1)) miss |= 128;
-            }
+            ps0 = ld_co(psj + q0 * (2 * NBMAX) + e0c);
+            ps1 = ld_co(psj + q1 * (2 * NBMAX) + e0c);
         }
         if (mv) {
-            if (c > j + 1 && c < n) {
-                xc = ld_co(xj + c);
-                if (!filled(xc)) miss |= 256;
-            }
-            if (rs > j + 1 && rs < n) {
-                xs = ld_co(xj + rs);
-                if (!filled(xs)) miss |= 512;
-            }
+            xc = ld_co(pxc);
+            xs = ld_co(pxs);
         }
+        unsigned miss = 0;
+        if (wave == 0) {
+            if ((in0 && !filled(pn0)) || (in1 && !filled(pn1))) miss |= 32;
+            if (!filled(araw)) miss |= 64;
+        }
+        if (t0 && ((in0 && !filled(ps0)) || (in1 && !filled(ps1)))) miss |= 128;
+        if (xcin && !filled(xc)) miss |= 256;
+        if (xsin && !filled(xs)) miss |= 512;
         if (__all(miss == 0)) break;
         if (!poll_more(spins, err, poll_code(POLL_B, j))) {
             poll_report(err, miss, (miss & 256) ? c : rs);
@@ -907,6 +954,16 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
             break;
         }
     }
+    if (!in0) {
+        pn0 = 0.0;
+        ps0 = 0.0;
+    }
+    if (!in1) {
+        pn1 = 0.0;
+        ps1 = 0.0;
+    }
+    if (!xcin) xc = 0.0;
+    if (!xsin) xs = 0.0;
     // the empty markers this rank stored in the last phase B have completed (they must before any
     // value of column j + 1 leaves this rank)
     drain();
@@ -999,14 +1056,20 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         double an[16], xcn = 0.0, xsn = 0.0;
         if (more) {
             xcn = xraw(xj, n, j, cn);
-            tall_load(A, lda, n, j, In, cn, p, an);
+            tall_load(A, lda, n, In, cn, p, an);
             if (In != I) xsn = xraw(xj, n, j, In * MR + tid);
         }
         double cp = 0.0;
+        {
+            // columns outside (j, n) have vc = 0, but their raw entries are masked all the same
+            // (0 * Inf or 0 * NaN of an unused entry must not reach the sums)
+            const int64_t lo = (c > j && c < n) ? c - I * MR : (int64_t)MR, hi = n - I * MR;
 #pragma unroll
-        for (int k = 0; k < 16; k++) {
-            racc[k] += a[k] * vc;
-            cp += a[k] * sh.vstrip[32 * (k >> 1) + 2 * p + (k & 1)];
+            for (int k = 0; k < 16; k++) {
+                const double av = tall_ok(k, p, lo, hi) ? a[k] : 0.0;
+                racc[k] += av * vc;
+                cp += av * sh.vstrip[32 * (k >> 1) + 2 * p + (k & 1)];
+            }
         }
         cp = row16_sum(cp);
         if (p == 0 && c > j && c < n) st_co(pcol_of(w, j) + I * w.ldp + c, -cp);
