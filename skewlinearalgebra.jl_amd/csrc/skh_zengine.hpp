@@ -101,13 +101,14 @@ struct ZWork {
     double2* alpha;  // [1] head of the next column
 };
 
-// v_k at index c: unit at k+1, scale * x_c below (x_k kept unscaled in A[:, k] until R(k)).
-__device__ __forceinline__ cd vval(const double2* __restrict__ A, int64_t lda, int64_t n, int64_t k, int64_t c,
-                                   const ZScal& s)
+// v_k at index c from its raw column entry x = A[c, k] (x_k is kept unscaled in A[:, k] until R(k)):
+// unit at k+1, scale * x below, zero outside (k+1, n).  The entry is loaded unconditionally from a
+// valid address and masked here: a use of a load right after it makes the compiler wait for it.
+__device__ __forceinline__ cd vfrom_raw(int64_t n, int64_t k, int64_t c, cd x, const ZScal& s)
 {
     if (c <= k || c >= n) return cd{0.0, 0.0};
     if (c == k + 1) return cd{s.unit, 0.0};
-    return s.scale * ldc(A + c + k * lda);
+    return s.scale * x;
 }
 
 struct ZShared {
@@ -138,15 +139,23 @@ __device__ void zpass(double2* __restrict__ A, int64_t lda, int64_t n, int64_t k
         const double2* vprev = upd ? A + (k - 1) * lda : A;   // v_{k-1} (factor stored in R(k-1))
         cd racc[4], rv[4], rw[4], rn[4];
         int64_t rows[4];
+        // every load of a step is issued before any is used, from valid addresses (index s0 stands
+        // in for one outside [s0, n)), and masked afterwards
         auto set_rows = [&](int64_t II) {
 #pragma unroll
             for (int rr = 0; rr < 4; rr++) {
                 rows[rr] = II * TB + 32 * (rr >> 1) + 2 * p + (rr & 1);
                 const int64_t r = rows[rr];
-                const bool in = r >= s0 && r < n;
-                rv[rr] = (upd && in) ? ldc(vprev + r) : cd{0.0, 0.0};
-                rw[rr] = (upd && in) ? ldc(w.Wv + r) : cd{0.0, 0.0};
-                rn[rr] = vval(A, lda, n, k, r, s);
+                const int64_t rc = (r >= s0 && r < n) ? r : s0;
+                rv[rr] = ldc(vprev + rc);
+                rw[rr] = ldc(w.Wv + rc);
+                rn[rr] = ldc(A + rc + k * lda);
+            }
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                const int64_t r = rows[rr];
+                if (!(upd && r >= s0 && r < n)) rv[rr] = rw[rr] = cd{0.0, 0.0};
+                rn[rr] = vfrom_raw(n, k, r, rn[rr], s);
                 racc[rr] = cd{0.0, 0.0};
             }
         };
@@ -157,16 +166,27 @@ __device__ void zpass(double2* __restrict__ A, int64_t lda, int64_t n, int64_t k
 #pragma unroll
             for (int cc = 0; cc < 4; cc++) {
                 const int64_t c = cb + cc;
+                const int64_t ci = (c >= s0 && c < n) ? c : s0;
+                cv[cc] = ldc(vprev + ci);
+                cw[cc] = ldc(w.Wv + ci);
+                cn[cc] = ldc(A + ci + k * lda);
+#pragma unroll
+                for (int rr = 0; rr < 4; rr++) {
+                    const int64_t r = rows[rr];
+                    a[rr][cc] = ld(A + ((r < n) ? r : s0) + ci * lda);
+                }
+            }
+#pragma unroll
+            for (int cc = 0; cc < 4; cc++) {
+                const int64_t c = cb + cc;
                 const bool cin = c >= s0 && c < n;
-                cv[cc] = (upd && cin) ? ldc(vprev + c) : cd{0.0, 0.0};
-                cw[cc] = (upd && cin) ? ldc(w.Wv + c) : cd{0.0, 0.0};
-                cn[cc] = vval(A, lda, n, k, c, s);
+                if (!(upd && cin)) cv[cc] = cw[cc] = cd{0.0, 0.0};
+                cn[cc] = vfrom_raw(n, k, c, cn[cc], s);
                 cacc[cc] = cd{0.0, 0.0};
 #pragma unroll
                 for (int rr = 0; rr < 4; rr++) {
                     const int64_t r = rows[rr];
-                    const bool ok = cin && r >= c && r < n;
-                    a[rr][cc] = ok ? ld(A + r + c * lda) : cd{0.0, 0.0};
+                    if (!(cin && r >= c && r < n)) a[rr][cc] = cd{0.0, 0.0};
                 }
             }
             if (upd) {
@@ -291,8 +311,13 @@ __device__ void zrows(double2* __restrict__ A, int64_t lda, int64_t n, int64_t k
     const int64_t Ge = (G < T) ? G : T;
     // m = sum of the pass partials (every workgroup wrote one), fixed order
     if (wave == 0) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = ld_co(w.Pmu + ((lane + 64 * u < Gfull) ? lane + 64 * u : 0));
         double m = 0.0;
-        for (int q = lane; q < Gfull; q += 64) m += ld_co(w.Pmu + q);
+#pragma unroll
+        for (int u = 0; u < 8; u++) m += (lane + 64 * u < Gfull) ? v[u] : 0.0;
+        for (int q = lane + 512; q < Gfull; q += 64) m += ld_co(w.Pmu + q);
         m = wave_sum(m);
         if (lane == 0) sh.mu = m;
     }
@@ -317,18 +342,20 @@ __device__ void zrows(double2* __restrict__ A, int64_t lda, int64_t n, int64_t k
         sh.red[wave][lane] = part;
         __syncthreads();
         if (wave == 0 && rin) {
+            // loads first (column k; the diagonal or column k+1), then the arithmetic
+            const cd xr = ldc(A + r + k * lda);
+            const cd o = ldc(A + r + ((r == s0) ? r : k + 1) * lda);
             const cd y = (sh.red[0][lane] + sh.red[1][lane]) + (sh.red[2][lane] + sh.red[3][lane]);
-            const cd vr = vval(A, lda, n, k, r, s);
+            const cd vr = vfrom_raw(n, k, r, xr, s);
             const cd wr = (r == s0) ? wspec : (s.tau * y + c * vr);
             stc(w.Wv + r, wr);
             stc(A + r + k * lda, vr);   // factor (unit at k+1)
             if (r == s0) {
                 // diagonal (k+1, k+1): += v conj(w) - w conj(v)
-                const cd d = ldc(A + r + r * lda);
-                stc(A + r + r * lda, d + mulc(vr, wr) - mulc(wr, vr));
+                stc(A + r + r * lda, o + mulc(vr, wr) - mulc(wr, vr));
             } else {
                 // next column: x_r = A[r, k+1] + v_r conj(w_{k+1}) - w_r conj(v_{k+1})
-                const cd x = ldc(A + r + (k + 1) * lda) + mulc(vr, wspec) - mulc(wr, vspec);
+                const cd x = o + mulc(vr, wspec) - mulc(wr, vspec);
                 stc(A + r + (k + 1) * lda, x);
                 if (r == k + 2) stc(w.alpha, x);
                 else nrm += norm2(x);
@@ -378,13 +405,23 @@ __device__ ZScal zscalars(const ZWork& w, int Gfull, ZShared& sh)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (wave == 0) {
+        // up to 512 partials: all loads issued before the sum (a sum between them serialises them)
+        double v[8];
+        const cd al = ldc(w.alpha);
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = ld_co(w.Pnrm + ((lane + 64 * u < Gfull) ? lane + 64 * u : 0));
         double a = 0.0;
-        for (int q = lane; q < Gfull; q += 64) a += ld_co(w.Pnrm + q);
+#pragma unroll
+        for (int u = 0; u < 8; u++) a += (lane + 64 * u < Gfull) ? v[u] : 0.0;
+        for (int q = lane + 512; q < Gfull; q += 64) a += ld_co(w.Pnrm + q);
         a = wave_sum(a);
-        if (lane == 0) sh.sred[0] = a;
+        if (lane == 0) {
+            sh.sred[0] = a;
+            sh.wspec = al;
+        }
     }
     __syncthreads();
-    const ZScal s = zreflector(ldc(w.alpha), sh.sred[0]);
+    const ZScal s = zreflector(sh.wspec, sh.sred[0]);
     __syncthreads();
     (void)tid;
     return s;
