@@ -579,21 +579,121 @@ int orgtr_locked(int64_t n, const double* dF, int64_t ldf, const double* dtau, d
 // Complex (ComplexF64) path: cooperative unblocked engine, explicit Q, eigen via the real
 // symmetric tridiagonal D^H (iT) D (D = diag(i^r)) and a back-transform by Q.
 // ----------------------------------------------------------------------------------------
-Buf g_zws, g_zq, g_zc, g_zx;
+Buf g_zws, g_zq, g_zc, g_zx, g_zbws;
 
-int zsytrd_device(int64_t n, double2* dA, int64_t lda, double2* dtau, double* deta, double* ddvim, hipStream_t st)
+// columns the unblocked engine finishes after the blocked panels (and the size below which it
+// takes the whole matrix)
+constexpr int64_t ZTAIL = 128, ZBLOCKED_MIN_N = 512;
+
+// Blocked complex panels (skewlatrd! + the complex rank-2k update) for columns [0, kb_end); the
+// trailing matrix A[kb_end:, kb_end:] is fully updated on return.  grid >= the 64-row blocks of the
+// matrix (checked by the caller).
+int zsytrd_blocked(int64_t n, double2* dA, int64_t lda, double2* dtau, double* deta, int64_t kb_end, int grid, Ctl& ctl,
+                   hipStream_t st)
+{
+    const int64_t nb = skh::zp::ZNB;
+    const int64_t ld = (n + 1) & ~int64_t(1);
+    const int64_t nt = (n + skh::TB - 1) / skh::TB + 1;
+    const int64_t ntp = (nt + 1) & ~int64_t(1);
+    const int64_t nts = (n + skh::MR - 1) / skh::MR + 1;
+    const int64_t nseg = grid + 2;
+    const int64_t gp = (grid + 1) & ~int64_t(1);
+    // W | V2 | (Pnorm | PS | X) x 4 | (Tv | Pmu | Prow | Pcol) x 3 | Dfin, in doubles
+    const int64_t ostride = ntp + 2 * nt * 2 * nb + 2 * ld;
+    const int64_t hstride = 2 * 2 * nb + gp + 2 * nseg * ld + 2 * nts * ld;
+    const size_t doubles = (size_t)(2 * 2 * ld * nb + skh::OWN_COPIES * ostride + skh::HAND_COPIES * hstride + 2 * ld + 64);
+    double* ws = (double*)g_zbws.get(doubles * sizeof(double));
+    if (!ws) return set_err(SKH_ERR_HIP, "hipMalloc of %zu bytes of workspace failed", doubles * 8);
+    skh::zp::ZPWork w;
+    w.W = (double2*)ws;
+    w.V2 = w.W + ld * nb;
+    w.ldw = ld;
+    double* own0 = ws + 2 * 2 * ld * nb;
+    w.Pnorm = own0;
+    w.PS = (double2*)(own0 + ntp);
+    w.X = (double2*)(own0 + ntp + 2 * nt * 2 * nb);
+    w.pn_stride = ostride;
+    w.ps_stride = ostride / 2;
+    w.x_stride = ostride / 2;
+    double* hand0 = own0 + skh::OWN_COPIES * ostride;
+    w.Tv = (double2*)hand0;
+    w.Pmu = hand0 + 2 * 2 * nb;
+    w.Prow = (double2*)(hand0 + 2 * 2 * nb + gp);
+    w.Pcol = w.Prow + nseg * ld;
+    w.ldp = ld;
+    w.tv_stride = hstride / 2;
+    w.mu_stride = hstride;
+    w.pr_stride = hstride / 2;
+    w.pc_stride = hstride / 2;
+    w.Dfin = (double2*)(hand0 + skh::HAND_COPIES * hstride);
+    for (int64_t c0 = 0; c0 < kb_end; c0 += nb) {
+        const int nbp = (int)std::min<int64_t>(nb, kb_end - c0);
+        const skh::TallGrid tg = skh::tall_grid(n, c0 + 1);
+        const int64_t blocks = (n + skh::TB - 1) / skh::TB - c0 / skh::TB;
+        const int g = (int)std::min<int64_t>(grid, std::max<int64_t>(std::max<int64_t>(8, blocks), tg.T));
+        // the slots polled before any producer of this launch refills them start empty
+        {
+            int64_t maxslot = 1;
+            for (int64_t I = tg.I0; I < tg.nts; I++) maxslot = std::max(maxslot, skh::tall_slots(tg, I, g));
+            FillSegs fs;
+            int ns = 0;
+            auto seg = [&](double* p, int64_t rows, int64_t ldr, int64_t lo, int64_t hi) {
+                fs.p[ns] = p;
+                fs.rows[ns] = rows;
+                fs.ld[ns] = ldr;
+                fs.lo[ns] = lo;
+                fs.hi[ns] = hi;
+                ns++;
+            };
+            for (int64_t cc = c0; cc < c0 + 2; cc++) {
+                double* h = hand0 + (cc % skh::HAND_COPIES) * hstride;
+                seg(h, 1, 0, 0, 2 * 2 * nb + gp);                                   // t and m partials
+                seg(h + 2 * 2 * nb + gp, maxslot, 2 * ld, 2 * c0, 2 * n);           // row partials
+                seg(h + 2 * 2 * nb + gp + 2 * nseg * ld, tg.nts, 2 * ld, 2 * c0, 2 * n);   // column partials
+                seg(own0 + (cc & (skh::OWN_COPIES - 1)) * ostride, 1, 0, 0, ostride);
+            }
+            seg(hand0 + ((c0 + 2) % skh::HAND_COPIES) * hstride, 1, 0, 0, 2 * 2 * nb);
+            seg((double*)w.W, nbp, 2 * ld, 2 * c0, 2 * n);
+            seg((double*)w.V2, nbp, 2 * ld, 2 * c0, 2 * n);
+            HIPCHK((hipError_t)skh_internal_fill_empty(&fs, ns, st));
+        }
+        HIPCHK((hipError_t)skh_internal_launch_zpanel(dA, lda, n, c0, nbp, &w, dtau, deta, ctl.err, g, st));
+        HIPCHK((hipError_t)skh_internal_launch_zdfin(dA, lda, w.Dfin, c0, nbp, st));
+        HIPCHK((hipError_t)skh_internal_launch_zrank2k(dA, lda, w.W, w.ldw, n, c0 + nbp, c0, nbp, st));
+    }
+    return 0;
+}
+
+// engine: SKH_ENGINE_TEAM1 keeps the whole matrix on the unblocked engine, SKH_ENGINE_LARGE asks
+// for the blocked panels, AUTO takes them from n = ZBLOCKED_MIN_N
+int zsytrd_device(int64_t n, double2* dA, int64_t lda, double2* dtau, double* deta, double* ddvim, hipStream_t st,
+                  int engine = SKH_ENGINE_AUTO)
 {
     if (n <= 0) return 0;
     if (n >= 2) {
         Ctl ctl;
         int rc = get_ctl(ctl);
         if (rc) return rc;
+        HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 512, st));
+        // blocked panels first (one workgroup per CU, one 64-row block each)
+        int64_t kb = 0;
+        if (engine != SKH_ENGINE_TEAM1 && (engine == SKH_ENGINE_LARGE || n >= ZBLOCKED_MIN_N)) {
+            int zpc = 0;
+            HIPCHK((hipError_t)skh_internal_zpanel_occupancy(&zpc));
+            const int zgrid = std::max(1, std::min(zpc, 1)) * g_num_cu;
+            const int64_t blocks = (n + skh::TB - 1) / skh::TB;
+            if (zpc >= 1 && blocks <= zgrid && n > ZTAIL + skh::zp::ZNB) {
+                kb = ((n - ZTAIL) / skh::zp::ZNB) * skh::zp::ZNB;
+                rc = zsytrd_blocked(n, dA, lda, dtau, deta, kb, zgrid, ctl, st);
+                if (rc) return rc;
+            }
+        }
         const int64_t ldp = (n + 1) & ~int64_t(1);
         const int64_t nt = (n + skh::TB - 1) / skh::TB + 1;
         int per_cu = 0;
         HIPCHK((hipError_t)skh_internal_zfused_occupancy(&per_cu));
         per_cu = std::max(1, std::min(per_cu, 2));
-        const int64_t ns0 = (n + skh::TB - 1) / skh::TB;
+        const int64_t ns0 = (n - kb + skh::TB - 1) / skh::TB + 1;
         const int64_t tiles0 = ns0 * (ns0 + 1) / 2;
         const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * g_num_cu, tiles0));
         const int64_t gpad = (grid + 1) & ~1;
@@ -608,14 +708,16 @@ int zsytrd_device(int64_t n, double2* dA, int64_t lda, double2* dtau, double* de
         w.Pnrm = (double*)(w.Wv + ldp);
         w.Pmu = w.Pnrm + gpad;
         w.alpha = (double2*)(w.Pmu + gpad);
-        HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 512, st));
+        // after the blocked panels nothing is pending: w of the "previous reflector" is zero
+        if (kb > 0) HIPCHK((hipError_t)skh_internal_zero_u32((unsigned*)w.Wv, (int)(ldp * 4), st));
         // one launch per segment of columns whose trailing block starts in the same 64-row block
         // (the tile grid, and so each workgroup's tiles, stay fixed within a launch)
-        for (int64_t kb = 0; kb <= n - 2;) {
-            const int64_t ke = std::min<int64_t>(n - 1, ((kb + 1) / skh::TB + 1) * skh::TB - 1);
-            HIPCHK((hipError_t)skh_internal_launch_zfused(dA, lda, n, &w, dtau, deta, ctl.bar, ctl.err, grid, kb, ke, st));
+        for (int64_t k0 = kb; k0 <= n - 2;) {
+            const int64_t ke = std::min<int64_t>(n - 1, ((k0 + 1) / skh::TB + 1) * skh::TB - 1);
+            HIPCHK((hipError_t)skh_internal_launch_zfused(dA, lda, n, &w, dtau, deta, ctl.bar, ctl.err, grid, k0, ke,
+                                                          (k0 == kb && kb > 0) ? 1 : 0, st));
             HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 288, st));
-            kb = ke;
+            k0 = ke;
         }
     }
     HIPCHK((hipError_t)skh_internal_zdiag(dA, lda, n, ddvim, st));
@@ -2146,10 +2248,10 @@ int skh_sytrd_z_device(int64_t n, double* dA, int64_t lda, double* dtau, double*
     if (n > 1 && (!dtau || !deta)) return set_err(-4, "tau/eta NULL");
     if (n > 0 && !ddvim) return set_err(-6, "dvim NULL");
     if (((uintptr_t)dA) & 15) return set_err(-2, "A must be 16-byte aligned");
-    (void)opts;
     int rc = ensure_init_locked(g_device);
     if (rc) return rc;
-    return zsytrd_device(n, (double2*)dA, lda, (double2*)dtau, deta, ddvim, (hipStream_t)stream);
+    return zsytrd_device(n, (double2*)dA, lda, (double2*)dtau, deta, ddvim, (hipStream_t)stream,
+                         opts ? (int)(opts->flags & 0xFF) : SKH_ENGINE_AUTO);
 }
 
 int skh_sytrd_z(int64_t n, double* A, int64_t lda, double* tau, double* eta, double* dvim, const skh_opts* opts)
@@ -2160,7 +2262,6 @@ int skh_sytrd_z(int64_t n, double* A, int64_t lda, double* tau, double* eta, dou
     if (lda < std::max<int64_t>(n, 1)) return set_err(-3, "lda < n");
     if (n > 1 && (!tau || !eta)) return set_err(-4, "tau/eta NULL");
     if (n > 0 && !dvim) return set_err(-6, "dvim NULL");
-    (void)opts;
     if (n == 0) return 0;
     int rc = ensure_init_locked(g_device);
     if (rc) return rc;
@@ -2173,7 +2274,7 @@ int skh_sytrd_z(int64_t n, double* A, int64_t lda, double* tau, double* eta, dou
     double2* dtau = (double2*)dv;
     double* deta = dv + 2 * n;
     double* ddvim = deta + n;
-    rc = zsytrd_device(n, dA, n, dtau, deta, ddvim, st);
+    rc = zsytrd_device(n, dA, n, dtau, deta, ddvim, st, opts ? (int)(opts->flags & 0xFF) : SKH_ENGINE_AUTO);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(st));
     rc = check_barrier_flag();

@@ -9,22 +9,25 @@
 #include "skh_internal.h"
 #include "skh_sync.hpp"
 #include "skh_zengine.hpp"
+#include "skh_zpanel.hpp"
 
 namespace skh {
 
 // Columns k in [k_begin, k_end) (k_end = n - 1 for the last segment); the norm partials and head
-// of column k_begin come from the previous launch (or zinit for k_begin = 0).
+// of column k_begin come from the previous launch, or from zinit when the launch starts with no
+// pending reflector (k_begin = 0, or `fresh`: after the blocked panels, with w.Wv zeroed).
 __global__ __launch_bounds__(WG, 1) void k_zfused(double2* A, int64_t lda, int64_t n, zc::ZWork w, double2* tau,
                                                   double* eta, unsigned* bar, int* err, int64_t k_begin,
-                                                  int64_t k_end)
+                                                  int64_t k_end, int fresh)
 {
     using namespace zc;
     __shared__ ZShared sh;
     __shared__ int s_ok;
     const int G = gridDim.x, g = blockIdx.x;
     unsigned epoch = 0;
-    if (k_begin == 0) {
-        zinit(A, lda, n, w, g, G);
+    const bool start = k_begin == 0 || fresh;
+    if (start) {
+        zinit(A, lda, n, w, g, G, k_begin);
         if (!grid_sync_co(bar, ++epoch, err, &s_ok)) return;
     }
     for (int64_t k = k_begin; k < k_end; k++) {
@@ -39,10 +42,166 @@ __global__ __launch_bounds__(WG, 1) void k_zfused(double2* A, int64_t lda, int64
             if (g == 0 && threadIdx.x == 0) zlast(A, lda, n, w, s);
             break;
         }
-        zpass(A, lda, n, k, k >= 1, s, w, sh, g, G);
+        zpass(A, lda, n, k, k >= 1 && !(start && k == k_begin), s, w, sh, g, G);
         if (!grid_sync_co(bar, ++epoch, err, &s_ok)) return;
         zrows(A, lda, n, k, s, w, sh, g, G, G);
         if (!grid_sync_co(bar, ++epoch, err, &s_ok)) return;
+    }
+}
+
+// One panel of the blocked complex reduction (device code in skh_zpanel.hpp): cooperative, one
+// workgroup per CU, one 64-row block per workgroup (grid >= blocks of the trailing matrix).
+__global__ __launch_bounds__(WG, 1) void k_zpanel(double2* A, int64_t lda, int64_t n, int64_t c0, int nbp, zp::ZPWork w,
+                                                  double2* tau, double* eta, int* err)
+{
+    using namespace zp;
+    __shared__ ZPShared sh;
+    __shared__ cd s_pv[4 * ZKQ * 64], s_pw[4 * ZKQ * 64];
+    const int G = gridDim.x, g = blockIdx.x;
+    ZRowRegs<ZKQ> rr;
+    rr.v = s_pv;
+    rr.w = s_pw;
+    for (int k = threadIdx.x; k < 4 * ZKQ * 64; k += WG) {
+        s_pv[k] = czero();
+        s_pw[k] = czero();
+    }
+    rr.xprev = czero();
+    const int64_t nt = cdiv(n, TB), bown = c0 / TB + g;
+    {
+        const int64_t r = bown * TB + (threadIdx.x & 63);
+        rr.xnext = (r < n && r >= c0) ? zc::ld(A + r + c0 * lda) : czero();   // column c0 from its diagonal
+    }
+    const PanelMap pm = panel_map(n, c0, g, G);
+    bool ok = zrow_pass<ZKQ>(A, lda, n, c0, c0, 0, false, true, w, g, G, sh, rr, pm, err);
+    for (int i = 0; ok && i < nbp; i++) {
+        const int64_t j = c0 + i;
+        ok = zphase_b(A, lda, n, c0, j, i, w, sh, tau, eta, g, G, pm, bown < nt && bown >= (j + 1) / TB, err);
+        if (ok) ok = zrow_pass<ZKQ>(A, lda, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, g, G, sh, rr, pm, err);
+    }
+}
+
+// final diagonal entries of the panel columns [c0, c0 + nbp) into A
+__global__ void k_zdfin(double2* A, int64_t lda, const double2* Dfin, int64_t c0, int nbp)
+{
+    const int64_t j = c0 + threadIdx.x;
+    if (threadIdx.x < nbp) A[j + j * lda] = Dfin[j];
+}
+
+// Complex skew rank-2k update after a panel (src/hessenberg.jl:158-166, complex case):
+// A[r, c] += sum_k L[r, k] conj(R[c, k]) for r >= c >= e, L = [V W], R = [W -V], on the diagonal the
+// imaginary part only (the reference adds Omega - Omega^H there).  One wave per 32 x 64 tile, as
+// k_rank2k, each 16 x 16 block transposed: Re += Rre Lre^T + Rim Lim^T, Im += Rre Lim^T - Rim Lre^T
+// (four v_mfma_f64_16x16x4f64 per block and k-step; real and imaginary parts of the interleaved
+// operands are the two halves of one 16-byte load).
+__device__ __forceinline__ void zr2k_operands(const double2* __restrict__ A, int64_t lda, const double2* __restrict__ U,
+                                              int64_t ldu, int64_t c0, int nbp, int nkh, int s, int lq,
+                                              const R2kRows& o, double2 lf[2], double2 rf[4])
+{
+    const int h = s >= nkh;
+    const int ks = s - h * nkh;
+    const bool ok = ks * 4 + lq < nbp;
+    const double2* pv = A + (c0 + ks * 4) * lda;
+    const double2* pu = U + (int64_t)ks * 4 * ldu;
+    const double2 z = make_double2(0.0, 0.0);
+    if (h == 0) {
+#pragma unroll
+        for (int b = 0; b < 2; b++) lf[b] = ok ? pv[o.lv[b]] : z;
+#pragma unroll
+        for (int b = 0; b < 4; b++) rf[b] = ok ? pu[o.ru[b]] : z;
+    } else {
+#pragma unroll
+        for (int b = 0; b < 2; b++) lf[b] = ok ? pu[o.lu[b]] : z;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const double2 t = ok ? pv[o.rv[b]] : z;
+            rf[b] = make_double2(-t.x, -t.y);
+        }
+    }
+}
+
+__device__ __forceinline__ void zr2k_mfma(v4d re[2][4], v4d im[2][4], const double2 lf[2], const double2 rf[4])
+{
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++) {
+            re[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(rf[bj].x, lf[bi].x, re[bi][bj], 0, 0, 0);
+            re[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(rf[bj].y, lf[bi].y, re[bi][bj], 0, 0, 0);
+            im[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(rf[bj].x, lf[bi].y, im[bi][bj], 0, 0, 0);
+            im[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(-rf[bj].y, lf[bi].x, im[bi][bj], 0, 0, 0);
+        }
+}
+
+__global__ __launch_bounds__(64, 2) void k_zrank2k(double2* A, int64_t lda, const double2* U, int64_t ldu, int64_t n,
+                                                   int64_t e, int64_t c0, int nbp, int64_t b0, int64_t nt,
+                                                   int64_t total, int64_t per)
+{
+    const int64_t bb = blockIdx.x;
+    const int64_t t = (bb % 8) * per + bb / 8;
+    if (t >= total) return;
+    int64_t ti, tj;
+    super_ij(t >> 1, nt, ti, tj);
+    const int64_t r0 = (b0 + ti) * R2K_TC + (t & 1) * R2K_TR, q0 = (b0 + tj) * R2K_TC;
+    if (r0 + R2K_TR - 1 < q0) return;   // the upper half of a diagonal block: nothing on or below the diagonal
+    const int lane = threadIdx.x & 63, lc = lane & 15, lq = lane >> 4;
+    R2kRows o;
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+        const int64_t rl = min(r0 + b * 16 + lc, n - 1);
+        o.lv[b] = (uint32_t)(rl + lq * lda);
+        o.lu[b] = (uint32_t)(rl + lq * ldu);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const int64_t rr = min(q0 + b * 16 + lc, n - 1);
+        o.rv[b] = (uint32_t)(rr + lq * lda);
+        o.ru[b] = (uint32_t)(rr + lq * ldu);
+    }
+    double2* C = A + r0 + q0 * lda;
+    const uint32_t co = (uint32_t)(lc + lq * lda);
+    // strictly below the diagonal and inside the trailing block everywhere: plain loads and stores
+    const bool inner = (q0 + R2K_TC - 1 < r0) && (r0 + R2K_TR - 1 < n) && (q0 >= e);
+    v4d re[2][4], im[2][4];
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++) {
+            const int64_t row = r0 + lc + bi * 16;
+            v4d cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int64_t col = q0 + lq + bj * 16 + 4 * q;
+                if (inner || (row < n && col >= e && col <= row)) {
+                    const double2 x = C[co + bi * 16 + (bj * 16 + 4 * q) * lda];
+                    cr[q] = x.x;
+                    ci[q] = x.y;
+                }
+            }
+            re[bi][bj] = cr;
+            im[bi][bj] = ci;
+        }
+    const int nkh = (nbp + 3) / 4, S = 2 * nkh;
+    double2 la[2], ra[4], lb[2], rb[4];
+    zr2k_operands(A, lda, U, ldu, c0, nbp, nkh, 0, lq, o, la, ra);
+    for (int s = 0; s < S; s += 2) {
+        if (s + 1 < S) zr2k_operands(A, lda, U, ldu, c0, nbp, nkh, s + 1, lq, o, lb, rb);
+        zr2k_mfma(re, im, la, ra);
+        if (s + 1 >= S) break;
+        if (s + 2 < S) zr2k_operands(A, lda, U, ldu, c0, nbp, nkh, s + 2, lq, o, la, ra);
+        zr2k_mfma(re, im, lb, rb);
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++) {
+        const int64_t row = r0 + lc + bi * 16;
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int64_t col = q0 + lq + bj * 16 + 4 * q;
+                double2* pc = C + co + bi * 16 + (bj * 16 + 4 * q) * lda;
+                if (inner || (row < n && col >= e && col < row)) *pc = make_double2(re[bi][bj][q], im[bi][bj][q]);
+                else if (row < n && col >= e && col == row) reinterpret_cast<double*>(pc)[1] = im[bi][bj][q];
+            }
     }
 }
 
@@ -134,12 +293,49 @@ extern "C" int skh_internal_zfused_occupancy(int* blocks_per_cu)
 
 extern "C" int skh_internal_launch_zfused(double2* A, int64_t lda, int64_t n, const zc::ZWork* w, double2* tau,
                                           double* eta, unsigned* bar, int* err, int grid, int64_t k_begin,
-                                          int64_t k_end, hipStream_t st)
+                                          int64_t k_end, int fresh, hipStream_t st)
 {
     zc::ZWork wc = *w;
-    void* args[] = {(void*)&A,   (void*)&lda, (void*)&n,   (void*)&wc,      (void*)&tau,
-                    (void*)&eta, (void*)&bar, (void*)&err, (void*)&k_begin, (void*)&k_end};
+    void* args[] = {(void*)&A,   (void*)&lda, (void*)&n,   (void*)&wc,      (void*)&tau,  (void*)&eta,
+                    (void*)&bar, (void*)&err, (void*)&k_begin, (void*)&k_end, (void*)&fresh};
     return (int)hipLaunchCooperativeKernel((const void*)k_zfused, dim3(grid), dim3(WG), args, 0, st);
+}
+
+extern "C" int skh_internal_zpanel_occupancy(int* blocks_per_cu)
+{
+    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void*)k_zpanel, WG, 0);
+}
+
+extern "C" int skh_internal_launch_zpanel(double2* A, int64_t lda, int64_t n, int64_t c0, int nbp, const zp::ZPWork* w,
+                                          double2* tau, double* eta, int* err, int grid, hipStream_t st)
+{
+    zp::ZPWork wc = *w;
+    void* args[] = {(void*)&A, (void*)&lda, (void*)&n, (void*)&c0, (void*)&nbp, (void*)&wc, (void*)&tau, (void*)&eta,
+                    (void*)&err};
+    return (int)hipLaunchCooperativeKernel((const void*)k_zpanel, dim3(grid), dim3(WG), args, 0, st);
+}
+
+extern "C" int skh_internal_launch_zdfin(double2* A, int64_t lda, const double2* Dfin, int64_t c0, int nbp,
+                                         hipStream_t st)
+{
+    if (nbp <= 0 || nbp > 256) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_zdfin, dim3(1), dim3(256), 0, st, A, lda, Dfin, c0, nbp);
+    return (int)hipGetLastError();
+}
+
+// trailing update of rows / columns >= e after the panel [c0, c0 + nbp): 64 x 64 tile blocks of the
+// lower triangle from block e / 64, two waves each, dealt to the XCDs as k_rank2k
+extern "C" int skh_internal_launch_zrank2k(double2* A, int64_t lda, const double2* U, int64_t ldu, int64_t n, int64_t e,
+                                           int64_t c0, int nbp, hipStream_t st)
+{
+    if (e >= n) return 0;
+    const int64_t b0 = e / R2K_TC;
+    const int64_t nt = (n + R2K_TC - 1) / R2K_TC - b0;
+    const int64_t total = nt * (nt + 1);   // two 32-row wave tiles per 64 x 64 block
+    const int64_t per = (total + 7) / 8;
+    hipLaunchKernelGGL(k_zrank2k, dim3((unsigned)(per * 8)), dim3(64), 0, st, A, lda, U, ldu, n, e, c0, nbp, b0, nt, total,
+                       per);
+    return (int)hipGetLastError();
 }
 
 extern "C" int skh_internal_zdiag(const double2* A, int64_t lda, int64_t n, double* dvim, hipStream_t st)

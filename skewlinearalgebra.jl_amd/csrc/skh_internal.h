@@ -6,6 +6,7 @@
 
 #include "skh_device.hpp"
 #include "skh_zengine.hpp"
+#include "skh_zpanel.hpp"
 
 struct PanelLaunch {
     double* A;
@@ -47,7 +48,15 @@ struct CompactArgs {
 extern "C" {
 int skh_internal_zfused_occupancy(int* blocks_per_cu);
 int skh_internal_launch_zfused(double2* A, int64_t lda, int64_t n, const skh::zc::ZWork* w, double2* tau, double* eta,
-                               unsigned* bar, int* err, int grid, int64_t k_begin, int64_t k_end, hipStream_t st);
+                               unsigned* bar, int* err, int grid, int64_t k_begin, int64_t k_end, int fresh,
+                               hipStream_t st);
+// blocked complex path: panel kernel (one workgroup per CU), diagonal write-back, MFMA rank-2k update
+int skh_internal_zpanel_occupancy(int* blocks_per_cu);
+int skh_internal_launch_zpanel(double2* A, int64_t lda, int64_t n, int64_t c0, int nbp, const skh::zp::ZPWork* w,
+                               double2* tau, double* eta, int* err, int grid, hipStream_t st);
+int skh_internal_launch_zdfin(double2* A, int64_t lda, const double2* Dfin, int64_t c0, int nbp, hipStream_t st);
+int skh_internal_launch_zrank2k(double2* A, int64_t lda, const double2* U, int64_t ldu, int64_t n, int64_t e,
+                                int64_t c0, int nbp, hipStream_t st);
 int skh_internal_zdiag(const double2* A, int64_t lda, int64_t n, double* dvim, hipStream_t st);
 int skh_internal_zform_v(const double2* F, int64_t ldf, int64_t n, int64_t c0, int nbp, double2* V, int64_t ldv,
                          hipStream_t st);

@@ -383,19 +383,21 @@ __device__ inline void zlast(double2* __restrict__ A, int64_t lda, int64_t n, co
     }
 }
 
-// Initial norm partials of column 0 (rows >= 2) and its head.
-__device__ void zinit(const double2* __restrict__ A, int64_t lda, int64_t n, const ZWork& w, int g, int G)
+// Initial norm partials of column k (rows >= k + 2) and its head, for a start with no pending
+// reflector (column 0, or the first column after the blocked panels).
+__device__ void zinit(const double2* __restrict__ A, int64_t lda, int64_t n, const ZWork& w, int g, int G, int64_t k)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     __shared__ double red[4];
     double nrm = 0.0;
-    for (int64_t r = 2 + (int64_t)g * 256 + tid; r < n; r += (int64_t)G * 256) nrm += norm2(ld(A + r));
+    const double2* col = A + k * lda;
+    for (int64_t r = k + 2 + (int64_t)g * 256 + tid; r < n; r += (int64_t)G * 256) nrm += norm2(ld(col + r));
     nrm = wave_sum(nrm);
     if (lane == 0) red[wave] = nrm;
     __syncthreads();
     if (tid == 0) {
         st_co(w.Pnrm + g, (red[0] + red[1]) + (red[2] + red[3]));
-        if (g == 0) stc(w.alpha, n >= 2 ? ld(A + 1) : cd{0.0, 0.0});
+        if (g == 0) stc(w.alpha, k + 1 < n ? ld(col + k + 1) : cd{0.0, 0.0});
     }
     __syncthreads();
 }

@@ -39,13 +39,13 @@ def ztri(eta, dvim):
     return T
 
 
-def gpu_zsytrd(A):
+def gpu_zsytrd(A, engine=None):
     F = np.array(A, dtype=np.complex128, order="F", copy=True)
     n = F.shape[0]
     tau = np.zeros(max(n - 1, 0), dtype=np.complex128)
     eta = np.zeros(max(n - 1, 0))
     dvim = np.zeros(n)
-    o = _lib.opts()
+    o = _lib.opts() if engine is None else _lib.opts(0, engine)
     rc = _lib.lib().skh_sytrd_z(n, F.ctypes.data, max(n, 1), tau.ctypes.data, eta.ctypes.data, dvim.ctypes.data,
                                 ctypes.byref(o))
     _lib.check("skh_sytrd_z", rc)
@@ -80,6 +80,26 @@ def test_zsytrd_vs_oracle(n):
     # the Python mirror: hessenberg(A) -> SkewHermTridiagonal(complex(eta), dvim)
     H = skl.hessenberg(A)
     np.testing.assert_allclose(H.H.to_dense(), ztri(eta, dvim), rtol=0, atol=1e-12 * nA)
+
+
+@pytest.mark.parametrize("engine", ["blocked", "unblocked"])
+@pytest.mark.parametrize("n", [200, 513, 700, 1111])
+def test_zsytrd_engines_vs_oracle(n, engine):
+    """Both complex engines against the oracle: the blocked panels (complex skewlatrd! with the
+    complex rank-2k update, the unblocked engine finishing the last columns) and the unblocked one."""
+    say("zsytrd %s n=%d" % (engine, n))
+    A = zskew(n, 3 * n + 1)
+    eng = _lib.ENGINE_LARGE if engine == "blocked" else _lib.ENGINE_TEAM1
+    F, tau, eta, dvim = gpu_zsytrd(A, eng)
+    Fo, tauo, etao = oracle.skewblockedhess(A)
+    nA = max(1.0, np.linalg.norm(A))
+    np.testing.assert_allclose(eta, etao, rtol=0, atol=1e-11 * nA)
+    np.testing.assert_allclose(tau, tauo, rtol=0, atol=1e-11)
+    np.testing.assert_allclose(dvim, np.imag(np.diag(Fo)), rtol=0, atol=1e-11 * nA)
+    np.testing.assert_allclose(np.tril(F, -1), np.tril(Fo, -1), rtol=0, atol=1e-10)
+    iu = np.triu_indices(n, 1)
+    np.testing.assert_array_equal(F[iu], A[iu])
+    check_zfact(A, F, tau, eta, dvim)
 
 
 def test_zsytrd_structured():
