@@ -658,9 +658,10 @@ int zsytrd_blocked(int64_t n, double2* dA, int64_t lda, double2* dtau, double* d
             HIPCHK((hipError_t)skh_internal_fill_empty(&fs, ns, st));
         }
         HIPCHK((hipError_t)skh_internal_launch_zpanel(dA, lda, n, c0, nbp, &w, dtau, deta, ctl.err, g, st));
-        HIPCHK((hipError_t)skh_internal_launch_zdfin(dA, lda, w.Dfin, c0, nbp, st));
         HIPCHK((hipError_t)skh_internal_launch_zrank2k(dA, lda, w.W, w.ldw, n, c0 + nbp, c0, nbp, st));
     }
+    // the panel columns' final diagonal entries into A (no later launch reads them)
+    HIPCHK((hipError_t)skh_internal_launch_zdfin(dA, lda, w.Dfin, 0, (int)kb_end, st));
     return 0;
 }
 

@@ -80,11 +80,11 @@ __global__ __launch_bounds__(WG, 1) void k_zpanel(double2* A, int64_t lda, int64
     }
 }
 
-// final diagonal entries of the panel columns [c0, c0 + nbp) into A
-__global__ void k_zdfin(double2* A, int64_t lda, const double2* Dfin, int64_t c0, int nbp)
+// final diagonal entries of the panel columns [c0, c0 + count) into A
+__global__ void k_zdfin(double2* A, int64_t lda, const double2* Dfin, int64_t c0, int64_t count)
 {
-    const int64_t j = c0 + threadIdx.x;
-    if (threadIdx.x < nbp) A[j + j * lda] = Dfin[j];
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < count; k += (int64_t)gridDim.x * blockDim.x)
+        A[(c0 + k) + (c0 + k) * lda] = Dfin[c0 + k];
 }
 
 // Complex skew rank-2k update after a panel (src/hessenberg.jl:158-166, complex case):
@@ -318,8 +318,9 @@ extern "C" int skh_internal_launch_zpanel(double2* A, int64_t lda, int64_t n, in
 extern "C" int skh_internal_launch_zdfin(double2* A, int64_t lda, const double2* Dfin, int64_t c0, int nbp,
                                          hipStream_t st)
 {
-    if (nbp <= 0 || nbp > 256) return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_zdfin, dim3(1), dim3(256), 0, st, A, lda, Dfin, c0, nbp);
+    if (nbp <= 0) return 0;
+    hipLaunchKernelGGL(k_zdfin, dim3((unsigned)std::min<int64_t>((nbp + 255) / 256, 64)), dim3(256), 0, st, A, lda, Dfin, c0,
+                       (int64_t)nbp);
     return (int)hipGetLastError();
 }
 

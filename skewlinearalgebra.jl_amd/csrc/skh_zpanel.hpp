@@ -404,16 +404,19 @@ __device__ bool zrow_pass(double2* __restrict__ A, int64_t lda, int64_t n, int64
     return true;
 }
 
-// raw tile entries of one lane (16 complex: rows I*MR + 32 h + 2p (+1), column c), unconditional
-// and in bounds; masked at use
+// local row of entry k of a lane's tile: rows 32 (k>>1) + 16 (k&1) + p, so that the 16 lanes of a
+// column read 16 consecutive entries (256 bytes) per load
+__device__ __forceinline__ int ztile_row(int k, int p) { return 32 * (k >> 1) + 16 * (k & 1) + p; }
+
+// raw tile entries of one lane (16 complex, column c), unconditional and in bounds; masked at use
 __device__ __forceinline__ void ztall_load(const double2* __restrict__ A, int64_t lda, int64_t n, int64_t I, int64_t c,
                                            int p, cd a[16])
 {
     const double2* col = A + ((c < n) ? c : n - 1) * lda + I * MR;
-    const int rn = (int)(n - I * MR), rb = 2 * p;
+    const int rn = (int)(n - I * MR);
 #pragma unroll
     for (int h = 0; h < 8; h++) {
-        const int r0 = rb + 32 * h, r1 = r0 + 1;
+        const int r0 = ztile_row(2 * h, p), r1 = ztile_row(2 * h + 1, p);
         const double2 t0 = col[(r0 < rn) ? r0 : -(int)(I * MR)];
         const double2 t1 = col[(r1 < rn) ? r1 : -(int)(I * MR)];
         a[2 * h] = cd{t0.x, t0.y};
@@ -609,7 +612,7 @@ __device__ bool zphase_b(const double2* __restrict__ A, int64_t lda, int64_t n, 
             const int64_t lo = cok ? c - I * MR : (int64_t)MR, hi = n - I * MR;
 #pragma unroll
             for (int k = 0; k < 16; k++) {
-                const int rl = 32 * (k >> 1) + 2 * p + (k & 1);
+                const int rl = ztile_row(k, p);
                 const bool below = rl > lo && rl < hi;
                 const bool diag = cok && rl == lo;
                 const cd av = (below || diag) ? a[k] : czero();
@@ -630,7 +633,7 @@ __device__ bool zphase_b(const double2* __restrict__ A, int64_t lda, int64_t n, 
             for (int k = 0; k < 16; k++) {
                 cd x = racc[k];
                 x = cd{xor32_sum(xor16_sum(x.re)), xor32_sum(xor16_sum(x.im))};
-                if (cg == 0) sh.mred[wave][32 * (k >> 1) + 2 * p + (k & 1)] = x;
+                if (cg == 0) sh.mred[wave][ztile_row(k, p)] = x;
                 racc[k] = czero();
             }
             __syncthreads();
