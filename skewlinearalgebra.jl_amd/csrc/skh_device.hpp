@@ -460,6 +460,10 @@ __device__ inline PanelMap panel_map(int64_t n, int64_t c0, int g, int G)
 // of column j - 1 has seen; the next pass's drain completes those stores before this rank stores
 // anything of column j + 1, and readers of column j + 2 poll only after seeing column j + 1.
 // ---------------------------------------------------------------------------------------
+// MROWS: rows of the matvec flush buffer: 16 (one per wave and column group, the LDS transposition
+// of the row sums; the 32-wide instance, where the LDS fits) or 4 (one per wave, after a
+// cross-lane reduction over the column groups)
+template <int MROWS>
 struct RowShared {
     double wrow[NBMAX];
     double vrow[NBMAX];
@@ -469,7 +473,7 @@ struct RowShared {
             double red[4][TB];    // row pass reductions
             double red2[4][TB];
         };
-        double mred[4][MR];       // matvec row sums of one strip, per wave (other phase)
+        double mred[MROWS][MR + 4];  // matvec row sums of one strip (other phase)
     };
     double vstrip[MR];    // v on the rows of the matvec strip being processed
     double wspec;
@@ -496,9 +500,9 @@ struct RowRegs {
 // issued together and polled together, so a row pass costs one memory round trip once its inputs
 // are there.  Returns false when the launch is abandoned (a poll ran out, or another workgroup
 // flagged an error).
-template <int KQ, bool MULTI>
+template <int KQ, bool MULTI, int MROWS>
 __device__ bool row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j, int i,
-                         bool doC, bool doA, const Work& w, int g, int G, RowShared& sh, RowRegs<KQ>& rr,
+                         bool doC, bool doA, const Work& w, int g, int G, RowShared<MROWS>& sh, RowRegs<KQ>& rr,
                          const PanelMap& pm, int* err, unsigned long long* clk)
 {
     clk_begin(clk);
@@ -886,8 +890,9 @@ __device__ __forceinline__ bool t_live(int g, int G, int i)
     return false;
 }
 
+template <int MROWS>
 __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j, int i,
-                        const Work& w, RowShared& sh, double* tau_out, double* eta_out, int g, int G,
+                        const Work& w, RowShared<MROWS>& sh, double* tau_out, double* eta_out, int g, int G,
                         const PanelMap& pm, bool rown, int* err, unsigned long long* clk)
 {
     const bool mv = g < pm.Ge;
@@ -1074,20 +1079,42 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         cp = row16_sum(cp);
         if (p == 0 && c > j && c < n) st_co(pcol_of(w, j) + I * w.ldp + c, -cp);
         if (!more || In != I) {
-            // flush this rank's row sums of strip I: lanes -> waves -> one slot row
+            // flush this rank's row sums of strip I into one slot row
+            if constexpr (MROWS == 16) {
+                // each (wave, column group) writes its 16 rows (8 row pairs) to its own LDS row; thread
+                // t then sums the 16 values of strip row t in a fixed order (an LDS transposition
+                // instead of 16 cross-lane reduction chains)
+                double* mr = sh.mred[wave * 4 + cg];
 #pragma unroll
-            for (int k = 0; k < 16; k++) {
-                double x = racc[k];
-                x = xor32_sum(xor16_sum(x));
-                if (cg == 0) sh.mred[wave][32 * (k >> 1) + 2 * p + (k & 1)] = x;
-                racc[k] = 0.0;
+                for (int h = 0; h < 8; h++) {
+                    *reinterpret_cast<double2*>(mr + 32 * h + 2 * p) = make_double2(racc[2 * h], racc[2 * h + 1]);
+                    racc[2 * h] = 0.0;
+                    racc[2 * h + 1] = 0.0;
+                }
+            } else {
+                // lanes -> one row per wave
+#pragma unroll
+                for (int k = 0; k < 16; k++) {
+                    double x = racc[k];
+                    x = xor32_sum(xor16_sum(x));
+                    if (cg == 0) sh.mred[wave][32 * (k >> 1) + 2 * p + (k & 1)] = x;
+                    racc[k] = 0.0;
+                }
             }
             __syncthreads();   // also: every wave is done reading vstrip of strip I
             {
                 const int64_t r = I * MR + tid;
-                if (r < n)
-                    st_co(prow_of(w, j) + cur_slot * w.ldp + r,
-                          (sh.mred[0][tid] + sh.mred[1][tid]) + (sh.mred[2][tid] + sh.mred[3][tid]));
+                double tot;
+                if constexpr (MROWS == 16) {
+                    double q4[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+                        q4[q] = (sh.mred[4 * q][tid] + sh.mred[4 * q + 1][tid]) + (sh.mred[4 * q + 2][tid] + sh.mred[4 * q + 3][tid]);
+                    tot = (q4[0] + q4[1]) + (q4[2] + q4[3]);
+                } else {
+                    tot = (sh.mred[0][tid] + sh.mred[1][tid]) + (sh.mred[2][tid] + sh.mred[3][tid]);
+                }
+                if (r < n) st_co(prow_of(w, j) + cur_slot * w.ldp + r, tot);
                 if (more) sh.vstrip[tid] = vfrom(n, j, In * MR + tid, settle(xsn, xj + In * MR + tid, err, poll_code(POLL_SETTLE, j)), s);
             }
             __syncthreads();

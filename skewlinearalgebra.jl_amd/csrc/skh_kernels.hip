@@ -36,7 +36,8 @@ template <int KQ, bool MULTI>
 __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int64_t lda, int64_t n, int64_t c0, int nbp, Work w,
                                               double* tau, double* eta, int* err, unsigned long long* ph)
 {
-    __shared__ RowShared sh;
+    constexpr int MROWS = (KQ == 8) ? 16 : 4;
+    __shared__ RowShared<MROWS> sh;
     __shared__ unsigned long long s_clk[PH_SLOTS];
     const int G = gridDim.x, g = blockIdx.x;
     unsigned long long* clk = ((ph != nullptr) && g == 0 && threadIdx.x == 0) ? s_clk : nullptr;
@@ -62,13 +63,13 @@ __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int
     // last 64-row block this workgroup finalises rows of (-1: none)
     const int64_t nt = cdiv(n, TB), bown = c0 / TB + g;
     const int64_t blast = (bown < nt) ? (MULTI ? bown + (int64_t)G * ((nt - 1 - bown) / G) : bown) : -1;
-    bool ok = row_pass<KQ, MULTI>(A, lda, n, c0, c0, 0, false, true, w, g, G, sh, rr, pm, err, clk);
+    bool ok = row_pass<KQ, MULTI, MROWS>(A, lda, n, c0, c0, 0, false, true, w, g, G, sh, rr, pm, err, clk);
     for (int i = 0; ok && i < nbp; i++) {
         const int64_t j = c0 + i;
         clk_outer(clk, 0);
-        ok = phase_b(A, lda, n, c0, j, i, w, sh, tau, eta, g, G, pm, blast >= (j + 1) / TB, err, clk);
+        ok = phase_b<MROWS>(A, lda, n, c0, j, i, w, sh, tau, eta, g, G, pm, blast >= (j + 1) / TB, err, clk);
         clk_outer(clk, 2);
-        if (ok) ok = row_pass<KQ, MULTI>(A, lda, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, g, G, sh, rr, pm, err, clk);
+        if (ok) ok = row_pass<KQ, MULTI, MROWS>(A, lda, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, g, G, sh, rr, pm, err, clk);
         clk_outer(clk, 3);
     }
     if (clk) {
