@@ -49,6 +49,8 @@ __global__ __launch_bounds__(WG, 1) void k_zfused(double2* A, int64_t lda, int64
     }
 }
 
+constexpr int ZR2K_TR = 32;   // rows of a complex rank-2k wave tile (two per 64 x 64 block)
+
 // One panel of the blocked complex reduction (device code in skh_zpanel.hpp): cooperative, one
 // workgroup per CU, one 64-row block per workgroup (grid >= blocks of the trailing matrix).
 __global__ __launch_bounds__(WG, 1) void k_zpanel(double2* A, int64_t lda, int64_t n, int64_t c0, int nbp, zp::ZPWork w,
@@ -141,8 +143,8 @@ __global__ __launch_bounds__(64, 2) void k_zrank2k(double2* A, int64_t lda, cons
     if (t >= total) return;
     int64_t ti, tj;
     super_ij(t >> 1, nt, ti, tj);
-    const int64_t r0 = (b0 + ti) * R2K_TC + (t & 1) * R2K_TR, q0 = (b0 + tj) * R2K_TC;
-    if (r0 + R2K_TR - 1 < q0) return;   // the upper half of a diagonal block: nothing on or below the diagonal
+    const int64_t r0 = (b0 + ti) * R2K_BLK + (t & 1) * ZR2K_TR, q0 = (b0 + tj) * R2K_BLK;
+    if (r0 + ZR2K_TR - 1 < q0) return;   // the upper half of a diagonal block: nothing on or below the diagonal
     const int lane = threadIdx.x & 63, lc = lane & 15, lq = lane >> 4;
     R2kRows o;
 #pragma unroll
@@ -160,7 +162,7 @@ __global__ __launch_bounds__(64, 2) void k_zrank2k(double2* A, int64_t lda, cons
     double2* C = A + r0 + q0 * lda;
     const uint32_t co = (uint32_t)(lc + lq * lda);
     // strictly below the diagonal and inside the trailing block everywhere: plain loads and stores
-    const bool inner = (q0 + R2K_TC - 1 < r0) && (r0 + R2K_TR - 1 < n) && (q0 >= e);
+    const bool inner = (q0 + R2K_BLK - 1 < r0) && (r0 + ZR2K_TR - 1 < n) && (q0 >= e);
     v4d re[2][4], im[2][4];
 #pragma unroll
     for (int bi = 0; bi < 2; bi++)
@@ -330,8 +332,8 @@ extern "C" int skh_internal_launch_zrank2k(double2* A, int64_t lda, const double
                                            int64_t c0, int nbp, hipStream_t st)
 {
     if (e >= n) return 0;
-    const int64_t b0 = e / R2K_TC;
-    const int64_t nt = (n + R2K_TC - 1) / R2K_TC - b0;
+    const int64_t b0 = e / R2K_BLK;
+    const int64_t nt = (n + R2K_BLK - 1) / R2K_BLK - b0;
     const int64_t total = nt * (nt + 1);   // two 32-row wave tiles per 64 x 64 block
     const int64_t per = (total + 7) / 8;
     hipLaunchKernelGGL(k_zrank2k, dim3((unsigned)(per * 8)), dim3(64), 0, st, A, lda, U, ldu, n, e, c0, nbp, b0, nt, total,

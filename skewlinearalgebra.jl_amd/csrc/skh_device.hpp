@@ -1149,21 +1149,31 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
 // so item q of lane l is C[r + l%16, c + 4q + l/16]: one item of a wave is four 128-byte
 // column segments.
 // ---------------------------------------------------------------------------------------
-constexpr int R2K_TR = 32;     // rows of a wave tile
-constexpr int R2K_TC = 64;     // columns of a wave tile (and the edge of a tile block)
+#ifndef SKH_R2K_TR
+#define SKH_R2K_TR 32
+#endif
+#ifndef SKH_R2K_TC
+#define SKH_R2K_TC 64
+#endif
+constexpr int R2K_BLK = 64;          // edge of a tile block (the unit of the supertile order)
+constexpr int R2K_TR = SKH_R2K_TR;   // rows of a wave tile
+constexpr int R2K_TC = SKH_R2K_TC;   // columns of a wave tile
+constexpr int R2K_BI = R2K_TR / 16;  // 16-row MFMA blocks of a wave tile
+constexpr int R2K_BJ = R2K_TC / 16;  // 16-column MFMA blocks
+constexpr int R2K_WPB = (R2K_BLK / R2K_TR) * (R2K_BLK / R2K_TC);   // wave tiles per tile block
 constexpr int R2K_SUPER = 16;  // tile blocks per supertile edge (the L2 reuse unit)
 
 // lane offsets (doubles) of the operand rows: V and W columns are at uniform offsets on top
 struct R2kRows {
-    uint32_t lv[2], lu[2];   // L rows r0 + 16 bi + lane%16, V and W layouts
-    uint32_t rv[4], ru[4];   // R rows q0 + 16 bj + lane%16
+    uint32_t lv[R2K_BI], lu[R2K_BI];   // L rows r0 + 16 bi + lane%16, V and W layouts
+    uint32_t rv[R2K_BJ], ru[R2K_BJ];   // R rows q0 + 16 bj + lane%16
 };
 
 // k-step s of 4: h = 0 gives L = V, R = W; h = 1 gives L = W, R = -V.  Lanes with
 // 4 ks + lane/16 >= nbp read nothing (zero operands pad nbp to a multiple of 4).
 __device__ __forceinline__ void r2k_operands(const double* __restrict__ A, int64_t lda, const double* __restrict__ U,
                                              int64_t ldu, int64_t c0, int nbp, int nkh, int s, int lq,
-                                             const R2kRows& o, double lf[2], double rf[4])
+                                             const R2kRows& o, double lf[R2K_BI], double rf[R2K_BJ])
 {
     const int h = s >= nkh;
     const int ks = s - h * nkh;
@@ -1172,23 +1182,23 @@ __device__ __forceinline__ void r2k_operands(const double* __restrict__ A, int64
     const double* pu = U + (int64_t)ks * 4 * ldu;
     if (h == 0) {
 #pragma unroll
-        for (int b = 0; b < 2; b++) lf[b] = ok ? pv[o.lv[b]] : 0.0;
+        for (int b = 0; b < R2K_BI; b++) lf[b] = ok ? pv[o.lv[b]] : 0.0;
 #pragma unroll
-        for (int b = 0; b < 4; b++) rf[b] = ok ? pu[o.ru[b]] : 0.0;
+        for (int b = 0; b < R2K_BJ; b++) rf[b] = ok ? pu[o.ru[b]] : 0.0;
     } else {
 #pragma unroll
-        for (int b = 0; b < 2; b++) lf[b] = ok ? pu[o.lu[b]] : 0.0;
+        for (int b = 0; b < R2K_BI; b++) lf[b] = ok ? pu[o.lu[b]] : 0.0;
 #pragma unroll
-        for (int b = 0; b < 4; b++) rf[b] = ok ? -pv[o.rv[b]] : 0.0;
+        for (int b = 0; b < R2K_BJ; b++) rf[b] = ok ? -pv[o.rv[b]] : 0.0;
     }
 }
 
-__device__ __forceinline__ void r2k_mfma(v4d acc[2][4], const double lf[2], const double rf[4])
+__device__ __forceinline__ void r2k_mfma(v4d acc[R2K_BI][R2K_BJ], const double lf[R2K_BI], const double rf[R2K_BJ])
 {
 #pragma unroll
-    for (int bi = 0; bi < 2; bi++)
+    for (int bi = 0; bi < R2K_BI; bi++)
 #pragma unroll
-        for (int bj = 0; bj < 4; bj++)
+        for (int bj = 0; bj < R2K_BJ; bj++)
             acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(rf[bj], lf[bi], acc[bi][bj], 0, 0, 0);
 }
 
@@ -1202,13 +1212,13 @@ __device__ __forceinline__ void rank2k_wave(double* __restrict__ A, int64_t lda,
     const int lane = threadIdx.x & 63, lc = lane & 15, lq = lane >> 4;
     R2kRows o;
 #pragma unroll
-    for (int b = 0; b < 2; b++) {
+    for (int b = 0; b < R2K_BI; b++) {
         const int64_t rl = min(r0 + b * 16 + lc, n - 1);
         o.lv[b] = (uint32_t)(rl + lq * lda);
         o.lu[b] = (uint32_t)(rl + lq * ldu);
     }
 #pragma unroll
-    for (int b = 0; b < 4; b++) {
+    for (int b = 0; b < R2K_BJ; b++) {
         const int64_t rr = min(q0 + b * 16 + lc, n - 1);
         o.rv[b] = (uint32_t)(rr + lq * lda);
         o.ru[b] = (uint32_t)(rr + lq * ldu);
@@ -1217,12 +1227,12 @@ __device__ __forceinline__ void rank2k_wave(double* __restrict__ A, int64_t lda,
     double* C = A + r0 + q0 * lda;
     const uint32_t co = (uint32_t)(lc + lq * lda);
     const bool inner = (q0 + R2K_TC - 1 < r0) && (r0 + R2K_TR - 1 < n) && (q0 >= e);
-    v4d acc[2][4];
+    v4d acc[R2K_BI][R2K_BJ];
     if (inner) {
 #pragma unroll
-        for (int bi = 0; bi < 2; bi++)
+        for (int bi = 0; bi < R2K_BI; bi++)
 #pragma unroll
-            for (int bj = 0; bj < 4; bj++) {
+            for (int bj = 0; bj < R2K_BJ; bj++) {
                 v4d c;
 #pragma unroll
                 for (int q = 0; q < 4; q++) c[q] = C[co + bi * 16 + (bj * 16 + 4 * q) * lda];
@@ -1230,9 +1240,9 @@ __device__ __forceinline__ void rank2k_wave(double* __restrict__ A, int64_t lda,
             }
     } else {
 #pragma unroll
-        for (int bi = 0; bi < 2; bi++)
+        for (int bi = 0; bi < R2K_BI; bi++)
 #pragma unroll
-            for (int bj = 0; bj < 4; bj++) {
+            for (int bj = 0; bj < R2K_BJ; bj++) {
                 const int64_t row = r0 + lc + bi * 16;
                 v4d c = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -1244,7 +1254,7 @@ __device__ __forceinline__ void rank2k_wave(double* __restrict__ A, int64_t lda,
             }
     }
     const int nkh = (nbp + 3) / 4, S = 2 * nkh;
-    double la[2], ra[4], lb[2], rb[4];
+    double la[R2K_BI], ra[R2K_BJ], lb[R2K_BI], rb[R2K_BJ];
     r2k_operands(A, lda, U, ldu, c0, nbp, nkh, 0, lq, o, la, ra);
     for (int s = 0; s < S; s += 2) {
         if (s + 1 < S) r2k_operands(A, lda, U, ldu, c0, nbp, nkh, s + 1, lq, o, lb, rb);
@@ -1255,18 +1265,18 @@ __device__ __forceinline__ void rank2k_wave(double* __restrict__ A, int64_t lda,
     }
     if (inner) {
 #pragma unroll
-        for (int bi = 0; bi < 2; bi++)
+        for (int bi = 0; bi < R2K_BI; bi++)
 #pragma unroll
-            for (int bj = 0; bj < 4; bj++)
+            for (int bj = 0; bj < R2K_BJ; bj++)
 #pragma unroll
                 for (int q = 0; q < 4; q++) C[co + bi * 16 + (bj * 16 + 4 * q) * lda] = acc[bi][bj][q];
         return;
     }
 #pragma unroll
-    for (int bi = 0; bi < 2; bi++) {
+    for (int bi = 0; bi < R2K_BI; bi++) {
         const int64_t row = r0 + lc + bi * 16;
 #pragma unroll
-        for (int bj = 0; bj < 4; bj++)
+        for (int bj = 0; bj < R2K_BJ; bj++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int64_t col = q0 + lq + bj * 16 + 4 * q;

@@ -88,8 +88,10 @@ __global__ __launch_bounds__(64, 2) void k_rank2k(double* A, int64_t lda, const 
     const int64_t t = (b % 8) * per + b / 8;
     if (t >= total) return;
     int64_t i, j;
-    super_ij(t >> 1, nt, i, j);
-    const int64_t r0 = (b0 + i) * R2K_TC + (t & 1) * R2K_TR, q0 = (b0 + j) * R2K_TC;
+    super_ij(t / R2K_WPB, nt, i, j);
+    const int wt = (int)(t % R2K_WPB);   // wave tile inside the block: row-major over (TR x TC) sub-tiles
+    const int64_t r0 = (b0 + i) * R2K_BLK + (wt / (R2K_BLK / R2K_TC)) * R2K_TR;
+    const int64_t q0 = (b0 + j) * R2K_BLK + (wt % (R2K_BLK / R2K_TC)) * R2K_TC;
     if (r0 + R2K_TR - 1 <= q0) return;   // the upper half of a diagonal block: nothing below the diagonal
     rank2k_wave(A, lda, U, ldu, n, e, c0, nbp, r0, q0);
 }
@@ -332,8 +334,8 @@ extern "C" int skh_internal_launch_rank2k(double* A, int64_t lda, const double* 
                                           int64_t c0, int nbp, hipStream_t st)
 {
     if (e >= n) return 0;
-    const int64_t b0 = e / R2K_TC, nt = (n + R2K_TC - 1) / R2K_TC - b0;
-    const int64_t total = nt * (nt + 1);   // two wave tiles per block of the triangle
+    const int64_t b0 = e / R2K_BLK, nt = (n + R2K_BLK - 1) / R2K_BLK - b0;
+    const int64_t total = nt * (nt + 1) / 2 * R2K_WPB;   // wave tiles of the blocks of the triangle
     const int64_t per = (total + 7) / 8;
     hipLaunchKernelGGL(k_rank2k, dim3((unsigned)(per * 8)), dim3(64), 0, st, A, lda, U, ldu, n, e, c0, nbp, b0, nt,
                        total, per);
