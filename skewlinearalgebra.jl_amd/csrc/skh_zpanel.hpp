@@ -86,7 +86,7 @@ struct ZPShared {
             cd red[4][TB];
             cd red2[4][TB];
         };
-        cd mred[4][MR];
+        cd mred[16][MR + 2];   // matvec row sums of one strip per (wave, column group)
     };
     cd vstrip[MR];
     cd wspec;
@@ -629,19 +629,24 @@ __device__ bool zphase_b(const double2* __restrict__ A, int64_t lda, int64_t n, 
         cp = cd{row16_sum(cp.re), row16_sum(cp.im)};
         if (p == 0 && c > j && c < n) cst(pcol_of(w, j) + I * w.ldp + c, cd{-cp.re, -cp.im});
         if (!more || In != I) {
+            // flush: each (wave, column group) writes its 16 rows to its own LDS row, thread t sums
+            // the 16 values of strip row t in a fixed order (an LDS transposition)
+            {
+                cd* mr = sh.mred[wave * 4 + cg];
 #pragma unroll
-            for (int k = 0; k < 16; k++) {
-                cd x = racc[k];
-                x = cd{xor32_sum(xor16_sum(x.re)), xor32_sum(xor16_sum(x.im))};
-                if (cg == 0) sh.mred[wave][ztile_row(k, p)] = x;
-                racc[k] = czero();
+                for (int k = 0; k < 16; k++) {
+                    mr[ztile_row(k, p)] = racc[k];
+                    racc[k] = czero();
+                }
             }
             __syncthreads();
             {
                 const int64_t r = I * MR + tid;
-                if (r < n)
-                    cst(prow_of(w, j) + cur_slot * w.ldp + r,
-                        cadd(cadd(sh.mred[0][tid], sh.mred[1][tid]), cadd(sh.mred[2][tid], sh.mred[3][tid])));
+                cd q4[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    q4[q] = cadd(cadd(sh.mred[4 * q][tid], sh.mred[4 * q + 1][tid]), cadd(sh.mred[4 * q + 2][tid], sh.mred[4 * q + 3][tid]));
+                if (r < n) cst(prow_of(w, j) + cur_slot * w.ldp + r, cadd(cadd(q4[0], q4[1]), cadd(q4[2], q4[3])));
                 if (more) {
                     const double2* px = xj + In * MR + tid;
                     xsn.re = settle(xsn.re, reinterpret_cast<const double*>(px), err, poll_code(POLL_SETTLE, j));
