@@ -224,6 +224,41 @@ def timed(torch, fn, reps):
     return e0.elapsed_time(e1) / reps
 
 
+BATCH_BYTES_RULE = {
+    True: "per matrix, the strictly lower triangle of each column's trailing block read once (the matvec) "
+          "and, per panel, that of the block after it read and written once (the rank-2k update)",
+    False: "per matrix, the strictly lower triangle of each column's trailing block read and written once",
+}
+
+
+def batch_bytes(ns, nb):
+    """Algorithmic bytes of one batched launch: the blocked engine (panel width nb) reads the strictly
+    lower triangle of each column's trailing block [j+1, n)^2 once and reads and writes that of the
+    block [c0+nb, n)^2 after each panel; the unblocked engine (nb = 0) reads and writes each
+    column's trailing triangle once."""
+    total = 0.0
+    for n in ns:
+        n = int(n)
+        if n < 2 or n % 2:
+            continue
+        cols = 0.0
+        for j in range(n - 2):
+            m = n - j - 1
+            cols += m * (m - 1) / 2.0
+        if nb == 0:
+            total += 16.0 * cols
+            continue
+        upd = 0.0
+        for c0 in range(0, n - 1, nb):
+            e = min(c0 + nb, n - 1)
+            if e >= n - 1:
+                continue
+            m = n - e
+            upd += m * (m - 1) / 2.0
+        total += 8.0 * cols + 16.0 * upd
+    return total
+
+
 def batched_bench(torch, skl, _lib, steps):
     """cfg4: the whole batch through one tuned plan replayed from a HIP graph, against the
     per-matrix GPU loop (same kernels, one launch per matrix) on a stratified sample."""
@@ -294,18 +329,22 @@ def batched_bench(torch, skl, _lib, steps):
     tflops = flops / (ms * 1e-3) / 1e12
     kname = "k_cblock" if cfg["engine"] == "blocked" else "k_compact"
     traffic, tnote = measured_traffic("%s_batch4096_traffic.md" % kname, kname)
+    abytes = batch_bytes(ns, cfg["panel_width"] if cfg["engine"] == "blocked" else 0)
+    gbs = abytes / (ms * 1e-3) / 1e9
     return {"metric": "batched log|Pf|+sign, 4096 matrices n in {32,...,512}", "value": count / (ms * 1e-3),
             "unit": "matrices/s", "ms_per_batch": ms, "gflops": flops / (ms * 1e-3) / 1e9,
             "launch": "one tuned plan replayed from a HIP graph (%s engine, kernel NMAX %d, panel width %d, %d "
                       "workgroups); tuner best %.2f ms" % (cfg["engine"], cfg["kernel_nmax"], cfg["panel_width"],
                                                          cfg["workgroups"], tuned_ms),
             "graph_replays_identical": same,
-            "roofline": {"bound": "mfma", "kernel": kname, "achieved": tflops,
-                         "peak": FP64_MFMA_PEAK_TF, "unit": "TFLOP/s", "frac": tflops / FP64_MFMA_PEAK_TF,
-                         "work": "sum (4/3) n^3 = %.4e flop per launch" % flops, "traffic": traffic,
-                         "traffic_source": tnote,
+            "roofline": {"bound": "hbm", "kernel": kname, "achieved": gbs,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                         "bytes_per_launch": abytes, "traffic": traffic, "traffic_source": tnote,
+                         "fp64_tflops": tflops, "fp64_frac": tflops / FP64_MFMA_PEAK_TF,
+                         "work": "%.4e bytes per launch: %s; sum (4/3) n^3 = %.4e flop (0.8 flop/byte)"
+                                 % (abytes, BATCH_BYTES_RULE[cfg["engine"] == "blocked"], flops),
                          "note": "the graph replay (a fill kernel and the one-workgroup-per-matrix launch) timed "
-                                 "with HIP events; FP64 peak (vector and matrix)"},
+                                 "with HIP events"},
             "plain": {"value": plain_rate, "unit": "matrices/s",
                       "sample": "per-matrix GPU loop, skh_logpf_d_device per matrix (library engine choice), every "
                                 "16th matrix (%d)" % len(sample), "max_rel_dev_vs_batch": loop_dev},
