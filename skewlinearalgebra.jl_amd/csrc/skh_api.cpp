@@ -17,6 +17,7 @@
 #include <vector>
 
 #include <rocblas/rocblas.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 #include <rocsolver/rocsolver.h>
 
 #include "../../include/skewhip.h"
@@ -70,6 +71,23 @@ struct StreamDrain {
     StreamDrain(const StreamDrain&) = delete;
     StreamDrain& operator=(const StreamDrain&) = delete;
     ~StreamDrain() { (void)hipStreamSynchronize(st); }
+};
+
+// roctx range around the host-side enqueue of an engine or a launch (rocprofv3 --marker-trace shows
+// them beside the kernels; without a profiler a push / pop costs a call into the roctx library).
+struct Range {
+    explicit Range(const char* fmt, ...) __attribute__((format(printf, 2, 3)))
+    {
+        char buf[128];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        roctxRangePushA(buf);
+    }
+    Range(const Range&) = delete;
+    Range& operator=(const Range&) = delete;
+    ~Range() { roctxRangePop(); }
 };
 
 int set_err(int code, const char* fmt, ...)
@@ -238,6 +256,7 @@ int get_ctl(Ctl& c)
 int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, int64_t nb, hipStream_t st,
                 double* ws_ext = nullptr, size_t ws_cap = 0, unsigned* ctl_ext = nullptr)
 {
+    Range rg("skh sytrd n=%lld nb=%lld", (long long)n, (long long)nb);
     Ctl ctl;
     if (ctl_ext) {
         ctl_at(ctl_ext, ctl);
@@ -363,6 +382,7 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
             HIPCHK((hipError_t)skh_internal_fill_empty(&fs, ns, st));
         }
         const bool tm = g_timer.on;
+        Range rp("skh panel + rank2k c0=%lld", (long long)c0);   // both launches of this panel
         if (tm) HIPCHK((hipError_t)g_timer.begin(st));
         HIPCHK((hipError_t)skh_internal_launch_panel(&pl, st));
         if (tm) {
@@ -409,6 +429,7 @@ int batched_run(int64_t batch, const int64_t* ns, const int64_t* offs, const dou
                 const int64_t* htoffs = nullptr)
 {
     if (batch == 0) return 0;
+    Range rg("skh logpf batched %lld matrices", (long long)batch);
     Ctl ctl;
     int rc = get_ctl(ctl);
     if (rc) return rc;
@@ -538,6 +559,7 @@ int blas_on(hipStream_t st)
 int orgtr_locked(int64_t n, const double* dF, int64_t ldf, const double* dtau, double* dQ, int64_t ldq, hipStream_t st)
 {
     if (n <= 0) return 0;
+    Range rg("skh orgtr n=%lld", (long long)n);
     if (n > 2147483647LL / 2) return set_err(SKH_ERR_UNSUPPORTED, "n too large for the BLAS interface");
     HIPCHK((hipError_t)skh_internal_eye(dQ, ldq, n, st));
     if (n <= 1) return 0;
@@ -591,6 +613,7 @@ constexpr int64_t ZTAIL = 128, ZBLOCKED_MIN_N = 512;
 int zsytrd_blocked(int64_t n, double2* dA, int64_t lda, double2* dtau, double* deta, int64_t kb_end, int grid, Ctl& ctl,
                    hipStream_t st)
 {
+    Range rg("skh zsytrd blocked n=%lld", (long long)n);
     const int64_t nb = skh::zp::ZNB;
     const int64_t ld = (n + 1) & ~int64_t(1);
     const int64_t nt = (n + skh::TB - 1) / skh::TB + 1;
@@ -671,6 +694,7 @@ int zsytrd_device(int64_t n, double2* dA, int64_t lda, double2* dtau, double* de
                   int engine = SKH_ENGINE_AUTO)
 {
     if (n <= 0) return 0;
+    Range rg("skh zsytrd n=%lld", (long long)n);
     if (n >= 2) {
         Ctl ctl;
         int rc = get_ctl(ctl);
@@ -730,6 +754,7 @@ int zorgtr_locked(int64_t n, const double2* dF, int64_t ldf, const double2* dtau
                   hipStream_t st)
 {
     if (n <= 0) return 0;
+    Range rg("skh zorgtr n=%lld", (long long)n);
     if (n > 2147483647LL / 2) return set_err(SKH_ERR_UNSUPPORTED, "n too large for the BLAS interface");
     HIPCHK((hipError_t)skh_internal_zeye(dQ, ldq, n, st));
     if (n <= 1) return 0;
@@ -1061,6 +1086,7 @@ int grad_run(int64_t batch, const int64_t* ns, const int64_t* offs, const double
 {
     (void)opts;
     if (batch == 0) return 0;
+    Range rg("skh logpf grad batched %lld matrices", (long long)batch);
     GradPlan* p = g_gradplan;
     if (!p || p->batch != batch || !std::equal(ns, ns + batch, p->ns.begin()) ||
         !std::equal(offs, offs + batch, p->offs.begin())) {
@@ -1148,6 +1174,7 @@ namespace {
 int plan_enqueue(skh_plan p, const double* dA, double* dlogpf, double* dsgn, hipStream_t st, int grid, int kmax,
                  int knb)
 {
+    Range rg("skh logpf plan %lld matrices", (long long)p->batch);
     HIPCHK((hipError_t)skh_internal_zero_u32(p->ctl, 64, st));
     CompactArgs a;
     a.A = dA;
