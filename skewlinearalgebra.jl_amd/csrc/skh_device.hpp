@@ -991,34 +991,6 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         }
     }
     if (__syncthreads_or(!alive)) return false;
-    // ---- empty this rank's hand-over slots of column j + 2 (copy (j + 2) mod 3 = (j - 1) mod 3):
-    // column j - 1's values there were consumed by the row passes of column j, which all finished
-    // before their norm partials of column j, which wave 0 has seen (hence after the barrier: the
-    // other waves may have had nothing to poll) ----
-    if (mv) {
-        double* prn = prow_of(w, j + 2);
-        double* pcn = pcol_of(w, j + 2);
-        const double e = empty_slot();
-        int64_t It = I, Jt = J, slot = cur_slot;
-        for (int64_t t = tlo; t < thi; t++) {
-            if (wave == 0 && lane < MC) {
-                const int64_t cc = Jt * MC + lane;
-                if (cc > j + 2 && cc < n) st_co(pcn + It * w.ldp + cc, e);
-            }
-            const bool lis = (Jt == MPS * It + MPS - 1);
-            if (lis || t + 1 == thi) {
-                const int64_t r = It * MR + tid;
-                if (r < n) st_co(prn + slot * w.ldp + r, e);
-                slot = 0;
-            }
-            if (lis) {
-                It++;
-                Jt = tg.Jlo;
-            } else {
-                Jt++;
-            }
-        }
-    }
     if (lane == 0)
         for (int e = e0; e < 2 * NBMAX; e += 4 * G) st_co(tv_of(w, j + 2) + e, empty_slot());
     clk_inner(clk, 8);
@@ -1130,6 +1102,36 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         c = cn;
     }
     clk_inner(clk, 10);
+    // ---- empty this rank's matvec hand-over slots of column j + 2 (copy (j + 2) mod 3 = (j - 1) mod 3),
+    // after its own partials of column j are on their way, off the path from the poll to the matvec.
+    // Column j - 1's values there were consumed by the row passes of column j, which all finished
+    // before their norm partials of column j, which wave 0 saw before the barrier above.  The next
+    // phase B drains these stores before it stores anything of column j + 1, and a reader polls
+    // column j + 2's slots only after a row pass of column j + 2 consumed column j + 1's partials. ----
+    {
+        double* prn = prow_of(w, j + 2);
+        double* pcn = pcol_of(w, j + 2);
+        const double e = empty_slot();
+        int64_t It = pm.I, Jt = pm.J, slot = pm.slot;
+        for (int64_t t = tlo; t < thi; t++) {
+            if (wave == 0 && lane < MC) {
+                const int64_t cc = Jt * MC + lane;
+                if (cc > j + 2 && cc < n) st_co(pcn + It * w.ldp + cc, e);
+            }
+            const bool lis = (Jt == MPS * It + MPS - 1);
+            if (lis || t + 1 == thi) {
+                const int64_t r = It * MR + tid;
+                if (r < n) st_co(prn + slot * w.ldp + r, e);
+                slot = 0;
+            }
+            if (lis) {
+                It++;
+                Jt = tg.Jlo;
+            } else {
+                Jt++;
+            }
+        }
+    }
     return true;
 }
 
