@@ -890,10 +890,28 @@ __device__ __forceinline__ bool t_live(int g, int G, int i)
     return false;
 }
 
-template <int MROWS>
+// RES: the rank's first matvec tile (tile tlo, the same for every column of the launch) is kept in
+// LDS for the whole launch (rt: WG x 8 double2, thread tid's 16 entries at [h * WG + tid], loaded
+// once by tile_keep), so each column streams one tile fewer per rank from memory; the loads of the
+// rank's second tile then go out before the poll, as the first tile's did.
+// RES: thread tid's 16 entries of the rank's first matvec tile into rt (read back only by the same
+// thread, so no barrier); the tile data is the trailing block at the launch's start, which the matvec
+// of every column of the launch reads (columns <= j are masked at use)
+__device__ __forceinline__ void tile_keep(const double* __restrict__ A, int64_t lda, int64_t n, const PanelMap& pm,
+                                          int g, double2* rt)
+{
+    if (g >= pm.Ge) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = wave_id();
+    double a[16];
+    tall_load(A, lda, n, pm.I, pm.J * MC + 4 * wave + (lane >> 4), lane & 15, a);
+#pragma unroll
+    for (int h = 0; h < 8; h++) rt[h * WG + tid] = make_double2(a[2 * h], a[2 * h + 1]);
+}
+
+template <int MROWS, bool RES>
 __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j, int i,
                         const Work& w, RowShared<MROWS>& sh, double* tau_out, double* eta_out, int g, int G,
-                        const PanelMap& pm, bool rown, int* err, unsigned long long* clk)
+                        const PanelMap& pm, bool rown, const double2* rt, int* err, unsigned long long* clk)
 {
     const bool mv = g < pm.Ge;
     if (!(mv || rown || t_live(g, G, i))) return true;
@@ -906,10 +924,22 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
     const TallGrid& tg = pm.tg;
     const int64_t tlo = pm.tlo, thi = pm.thi;
     int64_t I = pm.I, J = pm.J, cur_slot = pm.slot, c = 0;
-    double a[16];
+    double a[16], an[16];
     if (mv) {
         c = J * MC + 4 * wave + cg;
-        tall_load(A, lda, n, I, c, p, a);
+        if constexpr (RES) {
+            if (tlo + 1 < thi) {
+                const bool lis = (J == MPS * I + MPS - 1);
+                // wave-uniform, and opaque per column: keeps the compiler from hoisting this address math
+                // (lane-dependent, 64-bit) out of the column loop into registers it would have to spill
+                int I2 = __builtin_amdgcn_readfirstlane((int)(lis ? I + 1 : I));
+                int J2 = __builtin_amdgcn_readfirstlane((int)(lis ? tg.Jlo : J + 1));
+                asm volatile("" : "+s"(I2), "+s"(J2));
+                tall_load(A, lda, n, I2, J2 * MC + 4 * wave + cg, p, an);
+            }
+        } else {
+            tall_load(A, lda, n, I, c, p, a);
+        }
     }
     clk_inner(clk, 11);
     // ---- poll what the row pass of column j produced ----
@@ -1020,6 +1050,14 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
     // ---- (3) matvec ----
     if (!mv) return true;
     __syncthreads();   // v of the first strip is in LDS
+    if constexpr (RES) {
+#pragma unroll
+        for (int h = 0; h < 8; h++) {
+            const double2 t2 = rt[h * WG + tid];
+            a[2 * h] = t2.x;
+            a[2 * h + 1] = t2.y;
+        }
+    }
     double racc[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) racc[k] = 0.0;
@@ -1030,10 +1068,10 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         const int64_t In = last_in_strip ? I + 1 : I;
         const int64_t Jn = last_in_strip ? tg.Jlo : J + 1;
         const int64_t cn = Jn * MC + 4 * wave + cg;
-        double an[16], xcn = 0.0, xsn = 0.0;
+        double xcn = 0.0, xsn = 0.0;
         if (more) {
             xcn = xraw(xj, n, j, cn);
-            tall_load(A, lda, n, In, cn, p, an);
+            if (!RES || t != tlo) tall_load(A, lda, n, In, cn, p, an);   // (RES: tile tlo + 1 is loaded)
             if (In != I) xsn = xraw(xj, n, j, In * MR + tid);
         }
         double cp = 0.0;

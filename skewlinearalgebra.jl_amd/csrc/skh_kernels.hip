@@ -36,8 +36,16 @@ template <int KQ, bool MULTI>
 __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int64_t lda, int64_t n, int64_t c0, int nbp, Work w,
                                               double* tau, double* eta, int* err, unsigned long long* ph)
 {
-    constexpr int MROWS = (KQ == 8) ? 16 : 4;
+    // the 32-wide instance keeps each rank's first matvec tile in LDS (32 KB, room made by the
+    // 4-row flush buffer: two workgroups per CU still fit the 160 KB); SKH_PANEL_RESIDENT=0 builds the
+    // form without it (16-row flush buffer)
+#ifndef SKH_PANEL_RESIDENT
+#define SKH_PANEL_RESIDENT 1
+#endif
+    constexpr bool RES = SKH_PANEL_RESIDENT && KQ == 8;
+    constexpr int MROWS = (KQ == 8 && !RES) ? 16 : 4;
     __shared__ RowShared<MROWS> sh;
+    __shared__ double2 s_rt[RES ? 8 * WG : 1];
     __shared__ unsigned long long s_clk[PH_SLOTS];
     const int G = gridDim.x, g = blockIdx.x;
     unsigned long long* clk = ((ph != nullptr) && g == 0 && threadIdx.x == 0) ? s_clk : nullptr;
@@ -60,6 +68,7 @@ __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int
         rr.xnext = (r < n && r > c0) ? A[r + c0 * lda] : 0.0;
     }
     const PanelMap pm = panel_map(n, c0, g, G);
+    if constexpr (RES) tile_keep(A, lda, n, pm, g, s_rt);
     // last 64-row block this workgroup finalises rows of (-1: none)
     const int64_t nt = cdiv(n, TB), bown = c0 / TB + g;
     const int64_t blast = (bown < nt) ? (MULTI ? bown + (int64_t)G * ((nt - 1 - bown) / G) : bown) : -1;
@@ -67,7 +76,7 @@ __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int
     for (int i = 0; ok && i < nbp; i++) {
         const int64_t j = c0 + i;
         clk_outer(clk, 0);
-        ok = phase_b<MROWS>(A, lda, n, c0, j, i, w, sh, tau, eta, g, G, pm, blast >= (j + 1) / TB, err, clk);
+        ok = phase_b<MROWS, RES>(A, lda, n, c0, j, i, w, sh, tau, eta, g, G, pm, blast >= (j + 1) / TB, s_rt, err, clk);
         clk_outer(clk, 2);
         if (ok) ok = row_pass<KQ, MULTI, MROWS>(A, lda, n, c0, j + 1, i + 1, true, i + 1 < nbp, w, g, G, sh, rr, pm, err, clk);
         clk_outer(clk, 3);

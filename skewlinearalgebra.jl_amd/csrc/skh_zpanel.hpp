@@ -530,32 +530,6 @@ __device__ bool zphase_b(const double2* __restrict__ A, int64_t lda, int64_t n, 
         }
     }
     if (__syncthreads_or(!alive)) return false;
-    // ---- empty this rank's hand-over slots of column j + 2 (after the barrier: wave 0 has seen
-    // every norm partial of column j) ----
-    if (mv) {
-        double2* prn = prow_of(w, j + 2);
-        double2* pcn = pcol_of(w, j + 2);
-        int64_t It = I, Jt = J, slot = cur_slot;
-        for (int64_t t = tlo; t < thi; t++) {
-            if (wave == 0 && lane < MC) {
-                const int64_t cc = Jt * MC + lane;
-                if (cc > j + 2 && cc < n) cempty(pcn + It * w.ldp + cc);
-            }
-            const bool lis = (Jt == MPS * It + MPS - 1);
-            if (lis || t + 1 == thi) {
-                const int64_t rr = It * MR + tid;
-                if (rr < n) cempty(prn + slot * w.ldp + rr);
-                slot = 0;
-            }
-            if (lis) {
-                It++;
-                Jt = tg.Jlo;
-            } else {
-                Jt++;
-            }
-        }
-        if (tid == 0) st_co(pmu_of(w, j + 2) + g, empty_slot());
-    }
     if (lane == 0)
         for (int e = e0; e < 2 * ZNB; e += 4 * G) cempty(tv_of(w, j + 2) + e);
     const ZScal s = sh.cur;
@@ -674,6 +648,35 @@ __device__ bool zphase_b(const double2* __restrict__ A, int64_t lda, int64_t n, 
     if (lane == 0) sh.mu[wave] = mu;
     __syncthreads();
     if (tid == 0) st_co(pmu_of(w, j) + g, (sh.mu[0] + sh.mu[1]) + (sh.mu[2] + sh.mu[3]));
+    // ---- empty this rank's matvec hand-over slots of column j + 2 (copy (j + 2) mod 3 = (j - 1) mod 3)
+    // after its own outputs of column j are on their way, off the path from the poll to the matvec:
+    // column j - 1's values there were consumed by the row passes of column j, which all finished
+    // before their norm partials of column j, which wave 0 saw before the first barrier above; the
+    // next phase B drains these stores before storing anything of column j + 1 ----
+    {
+        double2* prn = prow_of(w, j + 2);
+        double2* pcn = pcol_of(w, j + 2);
+        int64_t It = pm.I, Jt = pm.J, slot = pm.slot;
+        for (int64_t t = tlo; t < thi; t++) {
+            if (wave == 0 && lane < MC) {
+                const int64_t cc = Jt * MC + lane;
+                if (cc > j + 2 && cc < n) cempty(pcn + It * w.ldp + cc);
+            }
+            const bool lis = (Jt == MPS * It + MPS - 1);
+            if (lis || t + 1 == thi) {
+                const int64_t rr = It * MR + tid;
+                if (rr < n) cempty(prn + slot * w.ldp + rr);
+                slot = 0;
+            }
+            if (lis) {
+                It++;
+                Jt = tg.Jlo;
+            } else {
+                Jt++;
+            }
+        }
+        if (tid == 0) st_co(pmu_of(w, j + 2) + g, empty_slot());
+    }
     __syncthreads();
     return true;
 }
