@@ -274,11 +274,15 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
     HIPCHK((hipError_t)skh_internal_panel_occupancy((int)nb, &per_cu0));
     // row-partial slots of one matvec strip: at most one per workgroup of the grid
     const int64_t nseg = (int64_t)std::max(1, std::min(per_cu0, 2)) * g_num_cu + 2;
-    // W | V2 | (Pnorm | PS | X) x 4 | (Tv | Prow | Pcol) x 3: the copies of the hand-over slots by
-    // column mod 4 (written by the row passes) and mod 3 (written by phase B)
+    // W | V2 (two pairs when double-buffered) | (Pnorm | PS | X) x 4 | (Tv | Prow | Pcol) x 3: the
+    // copies of the hand-over slots by column mod 4 (written by the row passes) and mod 3 (written by
+    // phase B).  Up to nb = 64 the panels alternate between two W | V2 pairs, so the rank-2k launch of
+    // one panel (which reads its W) can fill the next panel's slots with the empty marker itself.
+    const bool dbuf = nb <= 64;
+    const int64_t wpair = 2 * ld * nb;
     const int64_t ostride = (nt + nt * 2 * skh::NBMAX + ld + 1) & ~int64_t(1);
     const int64_t hsize = 2 * skh::NBMAX + nseg * ld + nts * ld;
-    size_t doubles = (size_t)(2 * ld * nb + skh::OWN_COPIES * ostride + skh::HAND_COPIES * hsize + 64);
+    size_t doubles = (size_t)((dbuf ? 2 : 1) * wpair + skh::OWN_COPIES * ostride + skh::HAND_COPIES * hsize + 64);
     double* ws;
     if (ws_ext) {
         if (doubles > ws_cap)
@@ -293,7 +297,7 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
     w.W = ws;
     w.V2 = w.W + ld * nb;
     w.ldw = ld;
-    w.Pnorm = w.V2 + ld * nb;
+    w.Pnorm = ws + (dbuf ? 2 : 1) * wpair;
     w.PS = w.Pnorm + nt;
     w.X = w.PS + nt * 2 * skh::NBMAX;
     w.ostride = ostride;
@@ -322,6 +326,54 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
         const long v = e ? atol(e) : 0;
         return (int64_t)(v > 0 ? v : 1 << 30);
     }();
+    // grid of the panel starting at c0, sized to its first matvec: about `tpw` tiles per workgroup, and
+    // at least one workgroup per 64-row block of the panel (each keeps its block's rows)
+    auto panel_grid = [&](int64_t c0) {
+        const skh::TallGrid tg = skh::tall_grid(n, c0 + 1);
+        const int64_t blocks = (n + skh::TB - 1) / skh::TB - c0 / skh::TB;
+        const int64_t want = std::max<int64_t>(std::max<int64_t>(8, blocks), (tg.T + tpw - 1) / tpw);
+        return (int)std::min<int64_t>(std::min<int64_t>(grid, want), gcap);
+    };
+    // the panel's W | V2 pair
+    auto panel_w = [&](int64_t c0) {
+        skh::Work pw = w;
+        if (dbuf && ((c0 / nb) & 1)) {
+            pw.W = ws + wpair;
+            pw.V2 = pw.W + ld * nb;
+        }
+        return pw;
+    };
+    // the slots polled before any producer of the panel's launch has refilled them start empty: the
+    // hand-over copies of the launch's first two columns (t, rows >= c0 of the row-partial slots in
+    // use and of the column partials), the row-pass copies of its first two columns, and the panel's
+    // columns of W and V2
+    auto panel_fill = [&](int64_t c0, int nbp, int pgrid, const skh::Work& pw, FillSegs& fs) {
+        const skh::TallGrid tg = skh::tall_grid(n, c0 + 1);
+        int64_t maxslot = 1;
+        for (int64_t I = tg.I0; I < tg.nts; I++) maxslot = std::max(maxslot, skh::tall_slots(tg, I, pgrid));
+        int ns = 0;
+        auto seg = [&](double* p, int64_t rows, int64_t ldr, int64_t lo, int64_t hi) {
+            fs.p[ns] = p;
+            fs.rows[ns] = rows;
+            fs.ld[ns] = ldr;
+            fs.lo[ns] = lo;
+            fs.hi[ns] = hi;
+            ns++;
+        };
+        for (int64_t cc = c0; cc < c0 + 2; cc++) {
+            double* h = w.Tv + (cc % skh::HAND_COPIES) * hsize;
+            seg(h, 1, 0, 0, 2 * skh::NBMAX);
+            seg(h + 2 * skh::NBMAX, maxslot, ld, c0, n);
+            seg(h + 2 * skh::NBMAX + nseg * ld, tg.nts, ld, c0, n);
+            seg(w.Pnorm + (cc & (skh::OWN_COPIES - 1)) * ostride, 1, 0, 0, ostride);
+        }
+        // t of the third copy too: an element that has no part in the launch's first columns
+        // empties its slots only from the column it becomes live
+        seg(w.Tv + ((c0 + 2) % skh::HAND_COPIES) * hsize, 1, 0, 0, 2 * skh::NBMAX);
+        seg(pw.W, nbp, ld, c0, n);
+        seg(pw.V2, nbp, ld, c0, n);
+        return ns;
+    };
     // error flag and phase clocks start at zero
     HIPCHK((hipError_t)skh_internal_zero_u32(ctl.bar, 512, st));
     for (int64_t c0 = 0; c0 <= n - 3; c0 += nb) {
@@ -332,53 +384,20 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
         pl.n = n;
         pl.c0 = c0;
         pl.nbp = nbp;
-        pl.w = w;
+        pl.w = panel_w(c0);
         pl.tau = dtau;
         pl.eta = deta;
         pl.err = ctl.err;
-        // grid sized to the panel's first matvec: about `tpw` tiles per workgroup
-        const skh::TallGrid tg = skh::tall_grid(n, c0 + 1);
-        {
-            // at least one workgroup per 64-row block of the panel (each keeps its block's rows)
-            const int64_t blocks = (n + skh::TB - 1) / skh::TB - c0 / skh::TB;
-            const int64_t want = std::max<int64_t>(std::max<int64_t>(8, blocks), (tg.T + tpw - 1) / tpw);
-            pl.grid = (int)std::min<int64_t>(std::min<int64_t>(grid, want), gcap);
-        }
+        pl.grid = panel_grid(c0);
         pl.nbmax = (int)nb;
         {
             const int64_t blocks = (n + skh::TB - 1) / skh::TB - c0 / skh::TB;
             pl.multi = (int64_t)pl.grid < blocks ? 1 : 0;
         }
         pl.ph = g_timer.phases ? ctl.ph : nullptr;
-        // the slots polled before any producer of this launch has refilled them start empty: the
-        // hand-over copies of the launch's first two columns (t, rows >= c0 of the row-partial slots in
-        // use and of the column partials), the row-pass copies of its first two columns, and the
-        // panel's columns of W and V2
-        {
-            int64_t maxslot = 1;
-            for (int64_t I = tg.I0; I < tg.nts; I++) maxslot = std::max(maxslot, skh::tall_slots(tg, I, pl.grid));
+        if (c0 == 0 || !dbuf) {   // otherwise the previous rank-2k launch did it
             FillSegs fs;
-            int ns = 0;
-            auto seg = [&](double* p, int64_t rows, int64_t ldr, int64_t lo, int64_t hi) {
-                fs.p[ns] = p;
-                fs.rows[ns] = rows;
-                fs.ld[ns] = ldr;
-                fs.lo[ns] = lo;
-                fs.hi[ns] = hi;
-                ns++;
-            };
-            for (int64_t cc = c0; cc < c0 + 2; cc++) {
-                double* h = w.Tv + (cc % skh::HAND_COPIES) * hsize;
-                seg(h, 1, 0, 0, 2 * skh::NBMAX);
-                seg(h + 2 * skh::NBMAX, maxslot, ld, c0, n);
-                seg(h + 2 * skh::NBMAX + nseg * ld, tg.nts, ld, c0, n);
-                seg(w.Pnorm + (cc & (skh::OWN_COPIES - 1)) * ostride, 1, 0, 0, ostride);
-            }
-            // t of the third copy too: an element that has no part in the launch's first columns
-            // empties its slots only from the column it becomes live
-            seg(w.Tv + ((c0 + 2) % skh::HAND_COPIES) * hsize, 1, 0, 0, 2 * skh::NBMAX);
-            seg(w.W, nbp, ld, c0, n);
-            seg(w.V2, nbp, ld, c0, n);
+            const int ns = panel_fill(c0, nbp, pl.grid, pl.w, fs);
             HIPCHK((hipError_t)skh_internal_fill_empty(&fs, ns, st));
         }
         const bool tm = g_timer.on;
@@ -395,8 +414,17 @@ int sytrd_large(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, 
             HIPCHK((hipError_t)g_timer.end(st, 0, bytes));
         }
         const int64_t e = c0 + nbp;
+        // the next panel's slots, filled by extra workgroups of this rank-2k launch
+        FillSegs fnext;
+        int nsnext = 0;
+        const int64_t c1 = c0 + nb;
+        if (dbuf && c1 <= n - 3) {
+            const int nbp1 = (int)std::min<int64_t>(nb, n - 2 - c1);
+            nsnext = panel_fill(c1, nbp1, panel_grid(c1), panel_w(c1), fnext);
+        }
         if (tm) HIPCHK((hipError_t)g_timer.begin(st));
-        HIPCHK((hipError_t)skh_internal_launch_rank2k(dA, lda, w.W, w.ldw, n, e, c0, nbp, st));
+        HIPCHK((hipError_t)skh_internal_launch_rank2k(dA, lda, pl.w.W, w.ldw, n, e, c0, nbp, nsnext ? &fnext : nullptr,
+                                                      nsnext, st));
         if (tm) {
             const double m = (double)(n - e);
             HIPCHK((hipError_t)g_timer.end(st, 1, 2.0 * m * (m - 1.0) * nbp));

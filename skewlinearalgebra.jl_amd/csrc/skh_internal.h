@@ -120,7 +120,8 @@ int skh_internal_cblock_occupancy(int nmax, int nb, int* blocks_per_cu);
 int skh_internal_logpf_from(const double* eta, const double* tau, int64_t n, double* logpf, double* sgn, hipStream_t st);
 int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st);
 // Segments of device memory to fill with the panel kernel's empty marker: entries [lo, hi) of
-// `rows` rows, ld doubles apart, at p (one launch for all segments).
+// `rows` rows, ld doubles apart, at p (one launch for all segments).  first / chunks: the block plan
+// of a fill done by 64-thread workgroups of another launch (set by the launcher).
 struct FillSegs {
     static constexpr int MAXSEG = 12;
     double* p[MAXSEG];
@@ -128,11 +129,15 @@ struct FillSegs {
     int64_t ld[MAXSEG];
     int64_t lo[MAXSEG];
     int64_t hi[MAXSEG];
+    int64_t first[MAXSEG + 1];   // first workgroup of each segment, [nseg] = total
+    int64_t chunks[MAXSEG];      // workgroups per row of each segment
+    int nseg;
 };
 int skh_internal_fill_empty(const FillSegs* fs, int nseg, hipStream_t st);
 int skh_internal_panel_occupancy(int nbmax, int* blocks_per_cu);
+// the rank-2k update after a panel; with fs (nseg > 0) the same launch also fills those segments
 int skh_internal_launch_rank2k(double* A, int64_t lda, const double* U, int64_t ldu, int64_t n, int64_t e,
-                               int64_t c0, int nbp, hipStream_t st);
+                               int64_t c0, int nbp, const FillSegs* fs, int nseg, hipStream_t st);
 int skh_internal_launch_finish(double* A, int64_t lda, int64_t n, double* tau, double* eta, hipStream_t st);
 int skh_internal_zero_u32(unsigned* p, int count, hipStream_t st);
 // fixed device scratch of the allocation-free factorization (skh_noalloc.hip): setup looks up the
