@@ -716,15 +716,43 @@ __device__ bool row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                 }
             }
         }
-        // ---- C part: w_{j-1} on this chunk ----
-        if (doC) {
-            double c4[4] = {0.0, 0.0, 0.0, 0.0};
-            if (rowC) {
+        // ---- C part: w_{j-1} on this chunk; with it, the A part's terms of the reflectors before the
+        // newest one (k < ka), which need nothing from the C part: one barrier publishes both sums ----
+        // Every LDS operand of the wave's k-group is read first and masked at use (a read under a
+        // per-term condition would be waited for on its own); entries past the live ones may hold
+        // anything, and the select drops them.
+        const bool live = doA && rin && r > j;
+        const int ka = doC ? im : i;
+        {
+            // the 32-wide instance reads all its operands at once; wider ones in groups of 4 k (all of
+            // them would not fit in registers)
+            constexpr int KG = KQ <= 8 ? KQ : 4;
+            double c4[4] = {0.0, 0.0, 0.0, 0.0}, p4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int u = 0; u < KQ; u++)
-                    if (k0 + u < kc) c4[u & 3] += vreg[u] * sh.t[k0 + u] - wreg[u] * sh.t[NBMAX + k0 + u];
+            for (int u0 = 0; u0 < KQ; u0 += KG) {
+                double ta[KG], tb[KG], wa[KG], va[KG];
+#pragma unroll
+                for (int q = 0; q < KG; q++) {
+                    ta[q] = sh.t[k0 + u0 + q];
+                    tb[q] = sh.t[NBMAX + k0 + u0 + q];
+                    wa[q] = sh.wrow[k0 + u0 + q];
+                    va[q] = sh.vrow[k0 + u0 + q];
+                }
+#pragma unroll
+                for (int q = 0; q < KG; q++) {
+                    const int u = u0 + q;
+                    if (doC) {
+                        const double term = vreg[u] * ta[q] - wreg[u] * tb[q];
+                        c4[u & 3] += (rowC && k0 + u < kc) ? term : 0.0;
+                    }
+                    if (doA) {
+                        const double term = vreg[u] * wa[q] - wreg[u] * va[q];
+                        p4[u & 3] += (live && k0 + u < ka) ? term : 0.0;
+                    }
+                }
             }
-            sh.red[wave][lane] = rowC ? pp + ((c4[0] + c4[1]) + (c4[2] + c4[3])) : 0.0;
+            if (doC) sh.red[wave][lane] = rowC ? pp + ((c4[0] + c4[1]) + (c4[2] + c4[3])) : 0.0;
+            if (doA) sh.red2[wave][lane] = (p4[0] + p4[1]) + (p4[2] + p4[3]);
         }
         __syncthreads();
         double wr = 0.0, vr = 0.0;
@@ -751,17 +779,12 @@ __device__ bool row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
         clk_inner(clk, 5);
         double x = 0.0;
         if (doA) {
-            // ---- A part: update column j with the panel's reflectors 0..i-1 ----
-            const bool live = rin && r > j;
-            double p4[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int u = 0; u < KQ; u++) {
-                const int k = k0 + u;
-                if (live && k < k1) p4[u & 3] += vreg[u] * sh.wrow[k] - wreg[u] * sh.vrow[k];
+            // ---- A part: column j updated with the panel's reflectors 0..i-1: the sums over k < ka from
+            // before the barrier, plus the newest reflector's term (V[r,im] W[j,im] - W[r,im] V[j,im]) ----
+            if (live) {
+                x = xold + (sh.red2[0][lane] + sh.red2[1][lane] + sh.red2[2][lane] + sh.red2[3][lane]);
+                if (doC) x += vr * sh.wspec - wr * cs.unit;
             }
-            sh.red2[wave][lane] = (p4[0] + p4[1]) + (p4[2] + p4[3]);
-            __syncthreads();
-            if (live) x = xold + (sh.red2[0][lane] + sh.red2[1][lane] + sh.red2[2][lane] + sh.red2[3][lane]);
             if (live && wave == 0) st_co(x_of(w, j) + r, x);   // column j keeps its old entries until v
             const bool tail = rin && r >= j + 2;
             if (wave == 0) {
