@@ -464,7 +464,7 @@ __device__ inline PanelMap panel_map(int64_t n, int64_t c0, int g, int G)
 // of the row sums; the 32-wide instance, where the LDS fits) or 4 (one per wave, after a
 // cross-lane reduction over the column groups)
 template <int MROWS>
-struct RowShared {
+struct alignas(16) RowShared {
     double wrow[NBMAX];
     double vrow[NBMAX];
     double t[2 * NBMAX];
@@ -886,7 +886,7 @@ __device__ __forceinline__ void tall_load(const double* __restrict__ A, int64_t 
 
 // entry k of a lane's tile (row I*MR + 32 (k>>1) + 2p + (k&1), column c) lies in the strictly lower
 // triangle of the trailing block of column j: lo = c - I*MR and hi = n - I*MR bound the local row
-__device__ __forceinline__ bool tall_ok(int k, int p, int64_t lo, int64_t hi)
+__device__ __forceinline__ bool tall_ok(int k, int p, int lo, int hi)
 {
     const int rl = 32 * (k >> 1) + 2 * p + (k & 1);
     return rl > lo && rl < hi;
@@ -984,7 +984,8 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
     const int64_t e0c = t0 ? e0 : 0;
     bool alive = true;
     for (unsigned spins = 0;; spins++) {
-        if (wave == 0) {   // wave-uniform branches: no per-lane exec mask around the loads
+        // every wave forms the scalars itself (no barrier between them and the wave's matvec)
+        if (wave == 0) {
             pn0 = ld_co(pnj + q0);
             pn1 = ld_co(pnj + q1);
             araw = ld_co(xj + (j + 1));
@@ -998,10 +999,8 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
             xs = ld_co(pxs);
         }
         unsigned miss = 0;
-        if (wave == 0) {
-            if ((in0 && !filled(pn0)) || (in1 && !filled(pn1))) miss |= 32;
-            if (!filled(araw)) miss |= 64;
-        }
+        if (wave == 0 && ((in0 && !filled(pn0)) || (in1 && !filled(pn1)))) miss |= 32;
+        if (wave == 0 && !filled(araw)) miss |= 64;
         if (t0 && ((in0 && !filled(ps0)) || (in1 && !filled(ps1)))) miss |= 128;
         if (xcin && !filled(xc)) miss |= 256;
         if (xsin && !filled(xs)) miss |= 512;
@@ -1026,7 +1025,7 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
     // value of column j + 1 leaves this rank)
     drain();
     clk_inner(clk, 12);
-    // ---- (1) scalars ----
+    // ---- (1) scalars, formed by every wave from the same values in the same order (bit-identical) ----
     if (wave == 0 && alive) {
         double acc = pn0 + pn1;
         for (int64_t q = lane + 128; q < nq; q += 64) {
@@ -1034,8 +1033,8 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
             acc += settle(x, pnj + b0 + q, err, poll_code(POLL_SETTLE, j));
         }
         acc = wave_sum(acc);
+        const ColScal s = reflector_scalars(araw, acc);
         if (lane == 0) {
-            const ColScal s = reflector_scalars(araw, acc);
             sh.cur = s;
             if (g == 0) {
                 tau_out[j] = s.tau;
@@ -1044,10 +1043,13 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         }
     }
     if (__syncthreads_or(!alive)) return false;
+    const ColScal s = sh.cur;
+    // ---- empty this rank's t slots of column j + 2 (copy (j + 2) mod 3 = (j - 1) mod 3): column
+    // j - 1's t was consumed by the row passes of column j, which all finished before their norm
+    // partials of column j, which this wave has seen ----
     if (lane == 0)
         for (int e = e0; e < 2 * NBMAX; e += 4 * G) st_co(tv_of(w, j + 2) + e, empty_slot());
     clk_inner(clk, 8);
-    const ColScal s = sh.cur;
     if (mv) sh.vstrip[tid] = vfrom(n, j, rs, xs, s);
     // ---- (2) t of the live elements of this wave ----
     for (int e = e0; e < 2 * NBMAX; e += 4 * G) {
@@ -1070,99 +1072,128 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         if (lane == 0) st_co(tv_of(w, j) + e, s.scale * acc);
     }
     clk_inner(clk, 9);
-    // ---- (3) matvec ----
-    if (!mv) return true;
     __syncthreads();   // v of the first strip is in LDS
-    if constexpr (RES) {
+    // ---- (3) matvec ----
+    if (mv) {
+        if constexpr (RES) {
 #pragma unroll
-        for (int h = 0; h < 8; h++) {
-            const double2 t2 = rt[h * WG + tid];
-            a[2 * h] = t2.x;
-            a[2 * h + 1] = t2.y;
-        }
-    }
-    double racc[16];
-#pragma unroll
-    for (int k = 0; k < 16; k++) racc[k] = 0.0;
-    double vc = vfrom(n, j, c, xc, s);
-    for (int64_t t = tlo; t < thi; t++) {
-        const bool more = t + 1 < thi;
-        const bool last_in_strip = (J == MPS * I + MPS - 1);
-        const int64_t In = last_in_strip ? I + 1 : I;
-        const int64_t Jn = last_in_strip ? tg.Jlo : J + 1;
-        const int64_t cn = Jn * MC + 4 * wave + cg;
-        double xcn = 0.0, xsn = 0.0;
-        if (more) {
-            xcn = xraw(xj, n, j, cn);
-            if (!RES || t != tlo) tall_load(A, lda, n, In, cn, p, an);   // (RES: tile tlo + 1 is loaded)
-            if (In != I) xsn = xraw(xj, n, j, In * MR + tid);
-        }
-        double cp = 0.0;
-        {
-            // columns outside (j, n) have vc = 0, but their raw entries are masked all the same
-            // (0 * Inf or 0 * NaN of an unused entry must not reach the sums)
-            const int64_t lo = (c > j && c < n) ? c - I * MR : (int64_t)MR, hi = n - I * MR;
-#pragma unroll
-            for (int k = 0; k < 16; k++) {
-                const double av = tall_ok(k, p, lo, hi) ? a[k] : 0.0;
-                racc[k] += av * vc;
-                cp += av * sh.vstrip[32 * (k >> 1) + 2 * p + (k & 1)];
+            for (int h = 0; h < 8; h++) {
+                const double2 t2 = rt[h * WG + tid];
+                a[2 * h] = t2.x;
+                a[2 * h + 1] = t2.y;
             }
         }
-        cp = row16_sum(cp);
-        if (p == 0 && c > j && c < n) st_co(pcol_of(w, j) + I * w.ldp + c, -cp);
-        if (!more || In != I) {
-            // flush this rank's row sums of strip I into one slot row
-            if constexpr (MROWS == 16) {
-                // each (wave, column group) writes its 16 rows (8 row pairs) to its own LDS row; thread
-                // t then sums the 16 values of strip row t in a fixed order (an LDS transposition
-                // instead of 16 cross-lane reduction chains)
-                double* mr = sh.mred[wave * 4 + cg];
+        double racc[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) racc[k] = 0.0;
+        double vc = vfrom(n, j, c, xc, s);
+        for (int64_t t = tlo; t < thi; t++) {
+            const bool more = t + 1 < thi;
+            const bool last_in_strip = (J == MPS * I + MPS - 1);
+            const int64_t In = last_in_strip ? I + 1 : I;
+            const int64_t Jn = last_in_strip ? tg.Jlo : J + 1;
+            const int64_t cn = Jn * MC + 4 * wave + cg;
+            double xcn = 0.0, xsn = 0.0;
+            if (more) {
+                xcn = xraw(xj, n, j, cn);
+                if (!RES || t != tlo) tall_load(A, lda, n, In, cn, p, an);   // (RES: tile tlo + 1 is loaded)
+                if (In != I) xsn = xraw(xj, n, j, In * MR + tid);
+            }
+            double cp = 0.0;
+            {
+                // v of the lane's 16 strip rows, all read before use (eight 16-byte LDS reads); the
+                // column sum in four chains (a single chain of 16 dependent FMAs is a latency chain)
+                double vsr[16];
 #pragma unroll
                 for (int h = 0; h < 8; h++) {
-                    *reinterpret_cast<double2*>(mr + 32 * h + 2 * p) = make_double2(racc[2 * h], racc[2 * h + 1]);
-                    racc[2 * h] = 0.0;
-                    racc[2 * h + 1] = 0.0;
+                    const double2 t2 = *reinterpret_cast<const double2*>(&sh.vstrip[32 * h + 2 * p]);
+                    vsr[2 * h] = t2.x;
+                    vsr[2 * h + 1] = t2.y;
                 }
-            } else {
-                // lanes -> one row per wave
+                // columns outside (j, n) have vc = 0, but their raw entries are masked all the same
+                // (0 * Inf or 0 * NaN of an unused entry must not reach the sums)
+                const int lo = (c > j && c < n) ? (int)(c - I * MR) : MR, hi = (int)min<int64_t>(n - I * MR, MR + 1);
+                double cq[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                 for (int k = 0; k < 16; k++) {
-                    double x = racc[k];
-                    x = xor32_sum(xor16_sum(x));
-                    if (cg == 0) sh.mred[wave][32 * (k >> 1) + 2 * p + (k & 1)] = x;
-                    racc[k] = 0.0;
+                    const double av = tall_ok(k, p, lo, hi) ? a[k] : 0.0;
+                    racc[k] += av * vc;
+                    cq[k & 3] += av * vsr[k];
                 }
+                cp = (cq[0] + cq[1]) + (cq[2] + cq[3]);
             }
-            __syncthreads();   // also: every wave is done reading vstrip of strip I
-            {
-                const int64_t r = I * MR + tid;
-                double tot;
+            cp = row16_sum(cp);
+            if (p == 0 && c > j && c < n) st_co(pcol_of(w, j) + I * w.ldp + c, -cp);
+            if (!more || In != I) {
+                // flush this rank's row sums of strip I into one slot row
                 if constexpr (MROWS == 16) {
-                    double q4[4];
+                    // each (wave, column group) writes its 16 rows (8 row pairs) to its own LDS row; thread
+                    // t then sums the 16 values of strip row t in a fixed order (an LDS transposition
+                    // instead of 16 cross-lane reduction chains)
+                    double* mr = sh.mred[wave * 4 + cg];
 #pragma unroll
-                    for (int q = 0; q < 4; q++)
-                        q4[q] = (sh.mred[4 * q][tid] + sh.mred[4 * q + 1][tid]) + (sh.mred[4 * q + 2][tid] + sh.mred[4 * q + 3][tid]);
-                    tot = (q4[0] + q4[1]) + (q4[2] + q4[3]);
+                    for (int h = 0; h < 8; h++) {
+                        *reinterpret_cast<double2*>(mr + 32 * h + 2 * p) = make_double2(racc[2 * h], racc[2 * h + 1]);
+                        racc[2 * h] = 0.0;
+                        racc[2 * h + 1] = 0.0;
+                    }
                 } else {
-                    tot = (sh.mred[0][tid] + sh.mred[1][tid]) + (sh.mred[2][tid] + sh.mred[3][tid]);
-                }
-                if (r < n) st_co(prow_of(w, j) + cur_slot * w.ldp + r, tot);
-                if (more) sh.vstrip[tid] = vfrom(n, j, In * MR + tid, settle(xsn, xj + In * MR + tid, err, poll_code(POLL_SETTLE, j)), s);
-            }
-            __syncthreads();
-            cur_slot = 0;   // this rank owns the first tile of every later strip it touches
-        }
-        if (more) {
+                    // each row's sum over the wave's 4 column groups (lanes p, p + 16, p + 32, p + 48) by
+                    // halving exchanges: 8 swaps between the wave's halves, then 4 between rows of 16 lanes;
+                    // lane (cg, p) ends with the totals of values 4 cg .. 4 cg + 3, i.e. rows
+                    // 64 cg + 32 h + 2 p + e (h, e in {0, 1}), one LDS row per wave
+                    double v[16];
 #pragma unroll
-            for (int k = 0; k < 16; k++) a[k] = an[k];
-            vc = vfrom(n, j, cn, settle(xcn, xj + cn, err, poll_code(POLL_SETTLE, j)), s);
+                    for (int k = 0; k < 16; k++) {
+                        v[k] = racc[k];
+                        racc[k] = 0.0;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        double a0, b0v;
+                        swap32(v[q], v[q + 8], a0, b0v);
+                        v[q] = a0 + b0v;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        double a0, b0v;
+                        swap16(v[q], v[q + 4], a0, b0v);
+                        v[q] = a0 + b0v;
+                    }
+                    *reinterpret_cast<double2*>(&sh.mred[wave][64 * cg + 2 * p]) = make_double2(v[0], v[1]);
+                    *reinterpret_cast<double2*>(&sh.mred[wave][64 * cg + 32 + 2 * p]) = make_double2(v[2], v[3]);
+                }
+                __syncthreads();   // also: every wave is done reading vstrip of strip I
+                {
+                    const int64_t r = I * MR + tid;
+                    double tot;
+                    if constexpr (MROWS == 16) {
+                        double q4[4];
+#pragma unroll
+                        for (int q = 0; q < 4; q++)
+                            q4[q] = (sh.mred[4 * q][tid] + sh.mred[4 * q + 1][tid]) + (sh.mred[4 * q + 2][tid] + sh.mred[4 * q + 3][tid]);
+                        tot = (q4[0] + q4[1]) + (q4[2] + q4[3]);
+                    } else {
+                        tot = (sh.mred[0][tid] + sh.mred[1][tid]) + (sh.mred[2][tid] + sh.mred[3][tid]);
+                    }
+                    if (r < n) st_co(prow_of(w, j) + cur_slot * w.ldp + r, tot);
+                    if (more) sh.vstrip[tid] = vfrom(n, j, In * MR + tid, settle(xsn, xj + In * MR + tid, err, poll_code(POLL_SETTLE, j)), s);
+                }
+                __syncthreads();
+                cur_slot = 0;   // this rank owns the first tile of every later strip it touches
+            }
+            if (more) {
+#pragma unroll
+                for (int k = 0; k < 16; k++) a[k] = an[k];
+                vc = vfrom(n, j, cn, settle(xcn, xj + cn, err, poll_code(POLL_SETTLE, j)), s);
+            }
+            I = In;
+            J = Jn;
+            c = cn;
         }
-        I = In;
-        J = Jn;
-        c = cn;
+        clk_inner(clk, 10);
     }
-    clk_inner(clk, 10);
+    if (!mv) return true;
     // ---- empty this rank's matvec hand-over slots of column j + 2 (copy (j + 2) mod 3 = (j - 1) mod 3),
     // after its own partials of column j are on their way, off the path from the poll to the matvec.
     // Column j - 1's values there were consumed by the row passes of column j, which all finished
