@@ -309,15 +309,40 @@ __device__ bool zrow_pass(double2* __restrict__ A, int64_t lda, int64_t n, int64
             sh.vrow[im] = cd{cs.unit, 0.0};
         }
     }
-    // ---- C part: w_{j-1} on this block ----
-    if (doC) {
-        cd c4 = czero();
-        if (rowC) {
+    // ---- C part: w_{j-1} on this block; with it, the A part's terms of the reflectors before the
+    // newest one (k < ka), which need nothing from the C part: one barrier publishes both sums.  The
+    // LDS operands are read in groups of 4 k before use and masked by select (entries past the live
+    // ones may hold anything) ----
+    const bool live = doA && rin && r >= j;
+    const int ka = doC ? im : i;
+    {
+        constexpr int KG = 4;
+        cd c4 = czero(), p4 = czero();
 #pragma unroll
-            for (int u = 0; u < KQ; u++)
-                if (k0 + u < kc) c4 = cadd(c4, csub(cmul(vreg[u], sh.t[k0 + u]), cmul(wreg[u], sh.t[ZNB + k0 + u])));
+        for (int u0 = 0; u0 < KQ; u0 += KG) {
+            cd ta[KG], tb[KG], wa[KG], va[KG];
+#pragma unroll
+            for (int q = 0; q < KG; q++) {
+                ta[q] = sh.t[k0 + u0 + q];
+                tb[q] = sh.t[ZNB + k0 + u0 + q];
+                wa[q] = sh.wrow[k0 + u0 + q];
+                va[q] = sh.vrow[k0 + u0 + q];
+            }
+#pragma unroll
+            for (int q = 0; q < KG; q++) {
+                const int u = u0 + q;
+                if (doC) {
+                    const cd term = csub(cmul(vreg[u], ta[q]), cmul(wreg[u], tb[q]));
+                    c4 = cadd(c4, (rowC && k0 + u < kc) ? term : czero());
+                }
+                if (doA) {
+                    const cd term = csub(cmulc(vreg[u], wa[q]), cmulc(wreg[u], va[q]));
+                    p4 = cadd(p4, (live && k0 + u < ka) ? term : czero());
+                }
+            }
         }
-        sh.red[wave][lane] = rowC ? cadd(pp, c4) : czero();
+        if (doC) sh.red[wave][lane] = rowC ? cadd(pp, c4) : czero();
+        if (doA) sh.red2[wave][lane] = p4;
     }
     __syncthreads();
     cd wr = czero(), vr = czero();
@@ -342,18 +367,13 @@ __device__ bool zrow_pass(double2* __restrict__ A, int64_t lda, int64_t n, int64
     }
     cd x = czero();
     if (doA) {
-        // ---- A part: update column j (rows >= j) with the panel's reflectors 0..i-1 ----
-        const bool live = rin && r >= j;
-        cd p4 = czero();
-#pragma unroll
-        for (int u = 0; u < KQ; u++) {
-            const int k = k0 + u;
-            if (live && k < k1) p4 = cadd(p4, csub(cmulc(vreg[u], sh.wrow[k]), cmulc(wreg[u], sh.vrow[k])));
-        }
-        sh.red2[wave][lane] = p4;
-        __syncthreads();
-        if (live)
+        // ---- A part: column j (rows >= j) updated with the panel's reflectors 0..i-1: the sums over
+        // k < ka from before the barrier, plus the newest reflector's term
+        // V[r,im] conj(W[j,im]) - W[r,im] conj(V[j,im]) ----
+        if (live) {
             x = cadd(xold, cadd(cadd(sh.red2[0][lane], sh.red2[1][lane]), cadd(sh.red2[2][lane], sh.red2[3][lane])));
+            if (doC) x = cadd(x, csub(cmulc(vr, sh.wspec), cmulc(wr, cd{cs.unit, 0.0})));
+        }
         if (wave == 0 && live) {
             if (r == j) w.Dfin[j] = make_double2(x.re, x.im);   // final diagonal entry of column j
             else cst(x_of(w, j) + r, x);
