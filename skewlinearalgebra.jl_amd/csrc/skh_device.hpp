@@ -256,8 +256,10 @@ __device__ __forceinline__ bool poll_more(unsigned spins, int* err, int code)
         return false;
     }
     if ((spins & 63u) == 63u && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+// s_sleep units (64 clocks) between poll rounds: fewer rounds while waiting leaves more of the memory
+// system to the producers (16 measured 0.6% faster than 1 at n = 8192)
 #ifndef SKH_POLL_SLEEP
-#define SKH_POLL_SLEEP 1
+#define SKH_POLL_SLEEP 16
 #endif
 #ifdef SKH_POLL_FAST
     if (spins < SKH_POLL_FAST) __builtin_amdgcn_s_sleep(1);
