@@ -986,8 +986,7 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
     const int64_t e0c = t0 ? e0 : 0;
     bool alive = true;
     for (unsigned spins = 0;; spins++) {
-        // every wave forms the scalars itself (no barrier between them and the wave's matvec)
-        if (wave == 0) {
+        if (wave == 0) {   // wave-uniform branches: no per-lane exec mask around the loads
             pn0 = ld_co(pnj + q0);
             pn1 = ld_co(pnj + q1);
             araw = ld_co(xj + (j + 1));
