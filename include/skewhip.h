@@ -35,7 +35,7 @@ extern "C" {
 #define SKH_ERR_SINGULAR 7     /* LU met an exactly zero pivot in tan/cot/tanh/coth (SingularException) */
 
 /* engine selection in skh_opts.flags */
-#define SKH_ENGINE_AUTO 0      /* single-matrix host entries: TEAM1 for even n <= 192, LARGE above */
+#define SKH_ENGINE_AUTO 0      /* single-matrix entries: one workgroup for even n <= 320, LARGE above */
 #define SKH_ENGINE_LARGE 1     /* cooperative multi-workgroup panel + MFMA rank-2k update */
 #define SKH_ENGINE_TEAM1 2     /* one workgroup per matrix (the batched engine) */
 #define SKH_ENGINE_BLOCKED 3   /* one workgroup per matrix, panels of opts.nb = 8 (n <= 512) or 16 (n <= 256)
@@ -90,7 +90,7 @@ int skh_sytrd_d_device_noalloc(int64_t n, double *dA, int64_t lda, double *dtau,
  */
 int skh_logpf_d(int64_t n, const double *A, int64_t lda, double *logpf, double *sgn, const skh_opts *opts);
 /* device variant (dA 16-byte aligned, *dlogpf and *dsgn in HBM), the same engine choice (SKH_ENGINE_AUTO:
- * one workgroup for even n <= 192, the cooperative engine above).  Enqueued on `stream`; the one-workgroup
+ * one workgroup for even n <= 320, the cooperative engine above).  Enqueued on `stream`; the one-workgroup
  * engine returns once its launch is done. */
 int skh_logpf_d_device(int64_t n, const double *dA, int64_t lda, double *dlogpf, double *dsgn, void *stream,
                        const skh_opts *opts);

@@ -162,11 +162,11 @@ bool blocked_instance(int64_t nmax, int64_t nb, int* kmax, int* knb)
     }
     return false;
 }
-// automatic choice for a single matrix: the compact engine (one workgroup) up to this size, the large
-// engine above.  Measured host-entry calls: n = 192 5.1 ms against 5.6 ms, n = 256 7.9 against 7.5,
-// n = 512 29.6 against 15.0, n = 1024 163 against 30.5 (batches keep the compact engine to n = 1024:
-// there every workgroup has a matrix of its own)
-constexpr int64_t CMP_AUTO = 192;
+// automatic choice for a single matrix: one workgroup (the blocked engine, k_cblock) up to this size, the
+// large engine above.  Measured device log|Pf| calls (MI355X): n = 128 0.71 ms against 1.47 ms, n = 256
+// 2.23 against 3.02, n = 320 3.64 against 3.62, n = 384 5.38 against 4.37, n = 512 10.4 against 5.94
+// (batches keep one workgroup per matrix to n = 1024: there every workgroup has a matrix of its own)
+constexpr int64_t CMP_AUTO = 320;
 
 int engine_of(int64_t n, const skh_opts* o)
 {
@@ -1745,7 +1745,8 @@ int skh_logpf_d_device(int64_t n, const double* dA, int64_t lda, double* dlogpf,
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     const int eng = engine_of(n, opts);
-    if (n % 2 == 1 || n <= 1 || (eng == SKH_ENGINE_TEAM1 && n <= CMP_NMAX)) {
+    const bool one_wg = eng == SKH_ENGINE_TEAM1 || eng == SKH_ENGINE_BLOCKED;   // one workgroup per matrix
+    if (n % 2 == 1 || n <= 1 || (one_wg && n <= CMP_NMAX)) {
         // one workgroup (or the closed form for odd and empty n), on a packed copy when lda != n
         const double* src = dA;
         if (lda != n && n > 1) {
@@ -1758,8 +1759,7 @@ int skh_logpf_d_device(int64_t n, const double* dA, int64_t lda, double* dlogpf,
         int64_t nn = n, off = 0;
         return batched_run(1, &nn, &off, src, dlogpf, dsgn, opts, st);
     }
-    if (eng == SKH_ENGINE_TEAM1) return set_err(SKH_ERR_UNSUPPORTED, "the compact engine takes n <= %lld",
-                                                (long long)CMP_NMAX);
+    if (one_wg) return set_err(SKH_ERR_UNSUPPORTED, "the one-workgroup engines take n <= %lld", (long long)CMP_NMAX);
     // large engine on a working copy (even leading dimension), log|Pf| and sign from eta, tau on the device
     const int64_t ld = (n + 1) & ~int64_t(1);
     double* w = (double*)g_big.get((size_t)(ld * n + 2 * n) * sizeof(double));
