@@ -374,6 +374,7 @@ constexpr int MPS = MR / MC;   // column blocks per strip row band
 
 struct TallGrid {
     int64_t s0, Jlo, I0, nts, T;
+    int64_t nblk;   // 64-row blocks from the one holding row s0 - 1 to the end (the panel's row-block owners)
 };
 
 __host__ __device__ inline int64_t tall_first(const TallGrid& tg, int64_t I)
@@ -391,7 +392,50 @@ __host__ __device__ inline TallGrid tall_grid(int64_t n, int64_t s0)
     tg.I0 = s0 / MR;
     tg.nts = (n + MR - 1) / MR;
     tg.T = tall_first(tg, tg.nts);
+    tg.nblk = (n + TB - 1) / TB - (s0 - 1) / TB;
     return tg;
+}
+
+// Weighted split of the T matvec tiles over Ge ranks: the first O ranks (the panel's row-block owners,
+// which also run the row pass and start their matvec after it) take WNUM / WDEN of the others' share,
+// so that they do not finish it last.  Rank g's range starts at floor(T vpos(g) / Lv) with vpos(g) =
+// WNUM min(g, O) + WDEN max(g - O, 0) and Lv = vpos(Ge).  Applied only while T >= 2 Ge (every range then
+// holds at least one tile), else the split is uniform.  64-bit arithmetic (computed once per launch).
+#ifndef SKH_OWNER_WNUM
+#define SKH_OWNER_WNUM 3
+#endif
+#ifndef SKH_OWNER_WDEN
+#define SKH_OWNER_WDEN 4
+#endif
+struct TileSplit {
+    uint64_t T, Ge, O, wn, wd, Lv;
+};
+__host__ __device__ inline TileSplit tile_split(const TallGrid& tg, int64_t Ge)
+{
+    TileSplit ts;
+    ts.T = (uint64_t)tg.T;
+    ts.Ge = (uint64_t)Ge;
+    ts.O = (uint64_t)(tg.nblk < Ge ? tg.nblk : Ge);
+    const bool w = tg.T >= 2 * Ge;
+    ts.wn = w ? SKH_OWNER_WNUM : 1;
+    ts.wd = w ? SKH_OWNER_WDEN : 1;
+    ts.Lv = ts.wn * ts.O + ts.wd * (ts.Ge - ts.O);
+    return ts;
+}
+__host__ __device__ inline int64_t split_lo(const TileSplit& ts, int64_t g)
+{
+    const uint64_t ug = (uint64_t)g;
+    const uint64_t v = ug <= ts.O ? ts.wn * ug : ts.wn * ts.O + ts.wd * (ug - ts.O);
+    return (int64_t)(ts.T * v / ts.Lv);
+}
+// the rank whose range holds tile t: the largest g with T vpos(g) <= (t + 1) Lv - 1
+__host__ __device__ inline int64_t split_owner(const TileSplit& ts, int64_t t)
+{
+    const uint64_t X = ((uint64_t)t + 1) * ts.Lv - 1;
+    const uint64_t g1 = X / (ts.T * ts.wn);
+    if (g1 < ts.O) return (int64_t)g1;
+    const uint64_t g2 = ts.O + (X - ts.T * ts.wn * ts.O) / (ts.T * ts.wd);
+    return (int64_t)(g2 < ts.Ge - 1 ? g2 : ts.Ge - 1);
 }
 
 // strip holding tile t: tall_first is 8 k^2 + (16 I0 + 8 - Jlo) k in k = I - I0, inverted in
@@ -411,9 +455,10 @@ __host__ __device__ inline int64_t tall_strip_of(const TallGrid& tg, int64_t t)
 __host__ __device__ inline int64_t tall_slots(const TallGrid& tg, int64_t I, int64_t G)
 {
     if (G > tg.T) G = tg.T;
+    const TileSplit ts = tile_split(tg, G);
     const int64_t f = tall_first(tg, I);
     const int64_t cnt = MPS * I + MPS - tg.Jlo;
-    return tile_owner(f + cnt - 1, tg.T, G) - tile_owner(f, tg.T, G) + 1;
+    return split_owner(ts, f + cnt - 1) - split_owner(ts, f) + 1;
 }
 
 // Matvec tile map of one panel launch: the tiles of the trailing block of the panel's first column
@@ -435,11 +480,12 @@ __device__ inline PanelMap panel_map(int64_t n, int64_t c0, int g, int G)
     pm.Ge = (G < pm.tg.T) ? G : pm.tg.T;
     pm.tlo = pm.thi = pm.I = pm.J = pm.slot = 0;
     if (g < pm.Ge) {
-        pm.tlo = range_lo(g, pm.tg.T, pm.Ge);
-        pm.thi = range_lo(g + 1, pm.tg.T, pm.Ge);
+        const TileSplit ts = tile_split(pm.tg, pm.Ge);
+        pm.tlo = split_lo(ts, g);
+        pm.thi = split_lo(ts, g + 1);
         pm.I = tall_strip_of(pm.tg, pm.tlo);
         pm.J = pm.tg.Jlo + (pm.tlo - tall_first(pm.tg, pm.I));
-        pm.slot = g - tile_owner(tall_first(pm.tg, pm.I), pm.tg.T, pm.Ge);
+        pm.slot = g - split_owner(ts, tall_first(pm.tg, pm.I));
     }
     const int64_t S = ((c0 / TB + g) * TB) / MR;
     pm.nslot_own = (S >= pm.tg.I0 && S < pm.tg.nts) ? tall_slots(pm.tg, S, G) : 0;
