@@ -15,6 +15,11 @@
  * The device work of one call uses those workspaces and one set of grid-barrier control words,
  * so the work of two _device calls must not overlap on the GPU: issue them on one stream, or
  * synchronise between calls on different streams (plans own their buffers and are exempt).
+ * The large engine's panel kernels need all their workgroups resident at once; their grid is the
+ * occupancy bound and they use a plain launch, so work of other streams or processes running on
+ * the GPU at the same time can keep workgroups out past the bounded waits (SKH_ERR_BARRIER).
+ * SKH_PANEL_COOPERATIVE=1 in the environment selects cooperative launches, which the runtime
+ * places co-resident beside other work, at about 22 us more per panel.
  */
 #ifndef SKEWHIP_H
 #define SKEWHIP_H
@@ -28,7 +33,7 @@ extern "C" {
 #define SKH_OK 0
 #define SKH_ERR_HIP 1          /* HIP runtime error (message in skh_last_error) */
 #define SKH_ERR_NOCONV 2       /* implicit QR hit its iteration limit */
-#define SKH_ERR_BARRIER 3      /* a grid barrier of the cooperative panel kernel timed out */
+#define SKH_ERR_BARRIER 3      /* a bounded wait of a persistent kernel timed out (a workgroup not resident or lost) */
 #define SKH_ERR_NOT_SKEW 4     /* input is not skew-symmetric (logabspfaffian! of a Matrix) */
 #define SKH_ERR_UNSUPPORTED 5  /* requested option is not available */
 #define SKH_ERR_DOMAIN 6       /* log of a singular skew matrix (Julia's DomainError) */
@@ -36,7 +41,7 @@ extern "C" {
 
 /* engine selection in skh_opts.flags */
 #define SKH_ENGINE_AUTO 0      /* single-matrix entries: one workgroup for even n <= 320, LARGE above */
-#define SKH_ENGINE_LARGE 1     /* cooperative multi-workgroup panel + MFMA rank-2k update */
+#define SKH_ENGINE_LARGE 1     /* persistent multi-workgroup panel + MFMA rank-2k update */
 #define SKH_ENGINE_TEAM1 2     /* one workgroup per matrix (the batched engine) */
 #define SKH_ENGINE_BLOCKED 3   /* one workgroup per matrix, panels of opts.nb = 8 (n <= 512) or 16 (n <= 256)
                                   reflectors with an MFMA rank-2k update per panel (0 = 8) */

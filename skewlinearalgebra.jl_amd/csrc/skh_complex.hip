@@ -51,7 +51,7 @@ __global__ __launch_bounds__(WG, 1) void k_zfused(double2* A, int64_t lda, int64
 
 constexpr int ZR2K_TR = 32;   // rows of a complex rank-2k wave tile (two per 64 x 64 block)
 
-// One panel of the blocked complex reduction (device code in skh_zpanel.hpp): cooperative, one
+// One panel of the blocked complex reduction (device code in skh_zpanel.hpp): co-resident, one
 // workgroup per CU, one 64-row block per workgroup (grid >= blocks of the trailing matrix).
 __global__ __launch_bounds__(WG, 1) void k_zpanel(double2* A, int64_t lda, int64_t n, int64_t c0, int nbp, zp::ZPWork w,
                                                   double2* tau, double* eta, int* err)
@@ -314,7 +314,11 @@ extern "C" int skh_internal_launch_zpanel(double2* A, int64_t lda, int64_t n, in
     zp::ZPWork wc = *w;
     void* args[] = {(void*)&A, (void*)&lda, (void*)&n, (void*)&c0, (void*)&nbp, (void*)&wc, (void*)&tau, (void*)&eta,
                     (void*)&err};
-    return (int)hipLaunchCooperativeKernel((const void*)k_zpanel, dim3(grid), dim3(WG), args, 0, st);
+    // barrier-free like the real panel kernel: the same plain launch unless SKH_PANEL_COOPERATIVE=1
+    // (the grid is at most one workgroup per CU, the occupancy bound)
+    if (skh_internal_panel_cooperative())
+        return (int)hipLaunchCooperativeKernel((const void*)k_zpanel, dim3(grid), dim3(WG), args, 0, st);
+    return (int)hipLaunchKernel((const void*)k_zpanel, dim3(grid), dim3(WG), args, 0, st);
 }
 
 extern "C" int skh_internal_launch_zdfin(double2* A, int64_t lda, const double2* Dfin, int64_t c0, int nbp,

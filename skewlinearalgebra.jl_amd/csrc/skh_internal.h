@@ -119,6 +119,8 @@ int skh_internal_launch_cblock(const CompactArgs* a, int nmax, int nb, int grid,
 int skh_internal_cblock_occupancy(int nmax, int nb, int* blocks_per_cu);
 int skh_internal_logpf_from(const double* eta, const double* tau, int64_t n, double* logpf, double* sgn, hipStream_t st);
 int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st);
+// 1 if SKH_PANEL_COOPERATIVE asks for cooperative launches of the barrier-free panel kernels
+int skh_internal_panel_cooperative();
 // Segments of device memory to fill with the panel kernel's empty marker: entries [lo, hi) of
 // `rows` rows, ld doubles apart, at p (one launch for all segments).  first / chunks: the block plan
 // of a fill done by 64-thread workgroups of another launch (set by the launcher).

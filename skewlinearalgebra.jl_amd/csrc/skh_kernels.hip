@@ -1,8 +1,8 @@
 // skh_kernels.hip -- gfx950 kernels of the skew tridiagonalization and the engines that drive them.
 //
-//  * k_panel     : one panel of the blocked reduction, a cooperative persistent kernel over the
-//                  whole chip (two grid barriers per column), replacing skewlatrd!
-//                  (src/hessenberg.jl:92-131) for one matrix.
+//  * k_panel     : one panel of the blocked reduction, a persistent kernel over the whole chip
+//                  whose workgroups hand values over through polled slots (no grid barrier),
+//                  replacing skewlatrd! (src/hessenberg.jl:92-131) for one matrix.
 //  * k_rank2k    : skew rank-2k trailing update on FP64 MFMA (src/hessenberg.jl:158-166),
 //                  lower triangle only, no Omega buffer.
 //  * Q formation and the batched gradient helpers (compact WY with rocBLAS products).
@@ -30,8 +30,9 @@ namespace skh {
 // processes blocks other than its own (rows of V and W then come from memory); the common
 // instance (one block per workgroup) carries none of that path.
 // The launch has no grid barrier: each workgroup polls the hand-over slots of its inputs (see
-// Work).  All workgroups are co-resident (cooperative launch), and every poll is bounded: a lost
-// producer flags `err` and ends the launch.
+// Work).  All workgroups are co-resident (the grid is at most the occupancy bound; see
+// skh_internal_launch_panel), and every poll is bounded: a lost producer flags `err` and ends the
+// launch.
 template <int KQ, bool MULTI>
 __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int64_t lda, int64_t n, int64_t c0, int nbp, Work w,
                                               double* tau, double* eta, int* err, unsigned long long* ph)
@@ -326,12 +327,27 @@ static const void* panel_kernel_for(int nbmax, int multi)
     return (const void*)k_panel<32, false>;
 }
 
+// The panel's workgroups wait on each other through bounded polls, so all of them must be resident
+// at once.  The grid never exceeds the occupancy bound (blocks per CU x CUs) and launches on one
+// stream run in order, so on a device that runs nothing beside the factorization a plain launch
+// places every workgroup at once, and it costs about 22 us less per panel than the cooperative one
+// (n = 8192, 256 panels: 212.4 -> 206.7 ms on MI355X).  SKH_PANEL_COOPERATIVE=1 takes the cooperative
+// launch, whose co-residency the runtime guarantees also beside other work on the device.  A plain
+// launch that does not become resident in time ends with the bounded polls' error, not a hang.
+extern "C" int skh_internal_panel_cooperative()
+{
+    const char* e = getenv("SKH_PANEL_COOPERATIVE");
+    return (e && atoi(e) != 0) ? 1 : 0;
+}
+
 extern "C" int skh_internal_launch_panel(const PanelLaunch* pl, hipStream_t st)
 {
     void* args[] = {(void*)&pl->A,   (void*)&pl->lda, (void*)&pl->n,   (void*)&pl->c0,  (void*)&pl->nbp,
                     (void*)&pl->w,   (void*)&pl->tau, (void*)&pl->eta, (void*)&pl->err, (void*)&pl->ph};
-    return (int)hipLaunchCooperativeKernel(panel_kernel_for(pl->nbmax, pl->multi), dim3(pl->grid), dim3(WG), args, 0,
-                                           st);
+    const void* k = panel_kernel_for(pl->nbmax, pl->multi);
+    if (skh_internal_panel_cooperative())
+        return (int)hipLaunchCooperativeKernel(k, dim3(pl->grid), dim3(WG), args, 0, st);
+    return (int)hipLaunchKernel(k, dim3(pl->grid), dim3(WG), args, 0, st);
 }
 
 extern "C" int skh_internal_fill_empty(const FillSegs* fs, int nseg, hipStream_t st)

@@ -6,6 +6,7 @@ on Hermitian(iA) (src/eigen.jl:18-41, :101-106).  Tolerances: eta, tau, dvim, fa
 1e-11 * ||A||; ||Q T Q^H - A|| / ||A|| <= 1e-13; eigenvalues within 10 n eps ||A||.
 """
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -100,6 +101,22 @@ def test_zsytrd_engines_vs_oracle(n, engine):
     iu = np.triu_indices(n, 1)
     np.testing.assert_array_equal(F[iu], A[iu])
     check_zfact(A, F, tau, eta, dvim)
+
+
+def test_zsytrd_cooperative_launch_same_bits():
+    """The blocked complex panels under the cooperative launch (SKH_PANEL_COOPERATIVE=1) and the
+    default plain launch: same grid, so the same bits."""
+    say("zsytrd cooperative launch n=700")
+    A = zskew(700, 71)
+    r0 = gpu_zsytrd(A, _lib.ENGINE_LARGE)
+    os.environ["SKH_PANEL_COOPERATIVE"] = "1"
+    try:
+        r1 = gpu_zsytrd(A, _lib.ENGINE_LARGE)
+    finally:
+        del os.environ["SKH_PANEL_COOPERATIVE"]
+    for a, b in zip(r0, r1):
+        assert np.array_equal(a, b)
+    check_zfact(A, *r1)
 
 
 def test_zsytrd_structured():

@@ -547,6 +547,23 @@ def test_sytrd_small_grid_several_blocks(n, cap):
     check_fact(A, F, tau, eta)
 
 
+@pytest.mark.parametrize("n,nb", [(1000, 32), (2049, 64), (700, 128)])
+def test_sytrd_cooperative_launch_same_bits(n, nb):
+    """The panel kernels' cooperative launch (SKH_PANEL_COOPERATIVE=1) and the default plain launch
+    run the same grid, so the factorization must agree bit for bit."""
+    say("sytrd cooperative launch n=%d nb=%d" % (n, nb))
+    A = skew_gauss(n, 91)
+    F0, tau0, eta0, _ = gpu_sytrd(A, nb)
+    os.environ["SKH_PANEL_COOPERATIVE"] = "1"
+    try:
+        F1, tau1, eta1, _ = gpu_sytrd(A, nb)
+    finally:
+        del os.environ["SKH_PANEL_COOPERATIVE"]
+    assert np.array_equal(F0, F1) and np.array_equal(tau0, tau1) and np.array_equal(eta0, eta1)
+    _, _, etao = oracle.skewblockedhess(A)
+    np.testing.assert_allclose(eta1, etao, rtol=0, atol=1e-11 * np.linalg.norm(A))
+
+
 def tri_fourth_moment(e):
     """tr(T^4) = ||T^2||_F^2 for the skew tridiagonal T with subdiagonal e: T^2 is symmetric
     pentadiagonal with (T^2)_ii = -(e_{i-1}^2 + e_i^2) and (T^2)_{i,i+2} = e_i e_{i+1}."""
