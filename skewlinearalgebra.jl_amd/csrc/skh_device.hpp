@@ -43,7 +43,7 @@ struct Work {
     double* PS;      // [nt][2*NBMAX]     per-block W^T x and V^T x partials
     double* X;       // [ld]              x_j, the updated column j (unscaled reflector)
     int64_t ostride; // Pnorm, PS and X of column j at + (j mod 4) * ostride
-    double* Tv;      // [2*NBMAX]         scale * (partial t1 = W^T x, t2 = V^T x); the consumer adds the head
+    double* Tv;      // [2*NBMAX]         partial t1 = W^T x, t2 = V^T x (unscaled; the consumer scales, adds the head)
     double* Prow;    // [nseg][ldp]       row partials of the skew matvec (one slot per rank per strip)
     double* Pcol;    // [nts][ldp]        column partials of the skew matvec
     int64_t ldp;
@@ -59,9 +59,11 @@ __device__ __forceinline__ double* tv_of(const Work& w, int64_t j) { return w.Tv
 __device__ __forceinline__ double* prow_of(const Work& w, int64_t j) { return w.Prow + hand_copy(j) * w.hstride; }
 __device__ __forceinline__ double* pcol_of(const Work& w, int64_t j) { return w.Pcol + hand_copy(j) * w.hstride; }
 
-// Per-column scalars kept in LDS across the phases.
+// Per-column scalars kept in LDS across the phases.  The matvec runs on the unscaled column x
+// (x[j+1] = alpha at the reflector's unit position), so its consumer forms
+// A v = scale * (A x) + corr * A[:, j+1] with corr = unit - scale * alpha.
 struct ColScal {
-    double tau, scale, unit, beta;
+    double tau, scale, unit, beta, corr;
 };
 
 // ---------------------------------------------------------------------------------------
@@ -315,17 +317,8 @@ __device__ __forceinline__ ColScal reflector_scalars(double alpha, double xn2)
         s.scale = 0.0;
         s.unit = 0.0;
     }
+    s.corr = s.unit - s.scale * alpha;   // |scale * alpha| <= 1/2: no cancellation
     return s;
-}
-
-// v_c of the current reflector (column j, x_j in xj): 0 outside [j+1, n), unit at j+1,
-// scale*x elsewhere.
-__device__ __forceinline__ double vval(const double* __restrict__ xj, int64_t n, int64_t j, int64_t c,
-                                       const ColScal& s)
-{
-    if (c <= j || c >= n) return 0.0;
-    if (c == j + 1) return s.unit;
-    return s.scale * ld_co(xj + c);
 }
 
 // Ownership of the matvec tiles (strip-major order over the lower triangle): T tiles split
@@ -611,7 +604,7 @@ __device__ bool row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                 wreg[u] = ok ? ld_co(Wm + r + k * ldw) : 0.0;
             }
             xo = rowC ? ld_co(x_of(w, jm) + r) : 0.0;         // x of column j-1 (unscaled)
-            xold = (doA && rin && r > j) ? A[r + j * lda] : 0.0;
+            xold = ((doA || doC) && rin && r > j) ? A[r + j * lda] : 0.0;
         }
         double pp = 0.0;
         if (doC) {
@@ -742,18 +735,20 @@ __device__ bool row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
         // ---- 2. shared inputs into LDS (first chunk), the special row of w_{j-1} ----
         if (first) {
             if (doC && tid < im) {
-                // t of column j-1 = unit * (row j of W, V) + scale * (sums over rows > j)
-                sh.t[tid] = cs.unit * wraw + tv1;
-                sh.t[NBMAX + tid] = cs.unit * vraw + tv2;
+                // t of column j-1 = unit * (row j of W, V) + scale * (sums over rows > j of W^T x, V^T x)
+                sh.t[tid] = cs.unit * wraw + cs.scale * tv1;
+                sh.t[NBMAX + tid] = cs.unit * vraw + cs.scale * tv2;
                 sh.wrow[tid] = wraw;
                 sh.vrow[tid] = vraw;
             }
             __syncthreads();
             clk_inner(clk, 4);
             if (doC && wave == 0) {
+                // (A v)_j = -sum_{r > j} A[r, j] v_r: column partials of the unscaled x, times scale
                 double acc = sprow;
 #pragma unroll
                 for (int u = 0; u < 4; u++) acc += sp[u];
+                acc *= cs.scale;
                 for (int k = lane; k < im; k += 64) acc += sh.vrow[k] * sh.t[k] - sh.wrow[k] * sh.t[NBMAX + k];
                 acc = wave_sum(acc);
                 if (lane == 0) {
@@ -799,7 +794,11 @@ __device__ bool row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
                     }
                 }
             }
-            if (doC) sh.red[wave][lane] = rowC ? pp + ((c4[0] + c4[1]) + (c4[2] + c4[3])) : 0.0;
+            // (A v)_r = scale * (A x)_r + corr * A[r, j] (x_j = alpha where v_j = unit; xold = A[r, j], 0
+            // for r <= j); pp is this wave's part of the partials, the corr term is wave 0's
+            if (doC)
+                sh.red[wave][lane] = rowC ? (cs.scale * pp + (wave == 0 ? cs.corr * xold : 0.0)) + ((c4[0] + c4[1]) + (c4[2] + c4[3]))
+                                          : 0.0;
             if (doA) sh.red2[wave][lane] = (p4[0] + p4[1]) + (p4[2] + p4[3]);
         }
         __syncthreads();
@@ -890,25 +889,27 @@ __device__ bool row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
 }
 
 // ---------------------------------------------------------------------------------------
-// Phase B of column j, once the row pass's outputs of column j are in:
-//   (1) the reflector scalars beta, tau, 1/(alpha - beta) from the block norm partials, formed by
-//       every rank in the same fixed order (wave 0, strided lanes, a butterfly), so all ranks hold
-//       bit-identical values;
-//   (2) scale * (sums over rows >= j+2 of W^T x and V^T x) of column j from the per-block
-//       partials, element e of t (W part e < NBMAX, V part NBMAX + k) by wave e mod 4 of rank
-//       (e / 4) mod G for the whole launch, live while k < i (the consumer adds unit * row j+1);
-//   (3) the skew matvec partials of column j over the strictly lower triangle of
-//       A[j+1:n, j+1:n] on MR x MC tiles (strip-major), split into G equal contiguous tile ranges,
-//       one per rank.  Per tile: wave w -> columns 4w..4w+3 (lane >> 4), lane & 15 = p -> row
-//       pairs {32 h + 2p, 32 h + 2p + 1}, h = 0..7 (16 entries per lane, read as 16-byte pairs);
-//       the next tile's loads are in flight while the current one is reduced.
-//         Pcol[I][c]    = -sum_{r in strip I} A[r,c] v_r                     (one entry per tile)
-//         Prow[slot][r] =  sum over the rank's tiles of strip I of A[r,c] v_c,
+// Phase B of column j, once the row pass's outputs of column j it needs are in:
+//   (1) the sums over rows >= j+2 of W^T x and V^T x of column j from the per-block partials
+//       (unscaled), element e of t (W part e < NBMAX, V part NBMAX + k) by wave e mod 4 of rank
+//       (e / 4) mod G for the whole launch, live while k < i (the consumer scales them and adds
+//       unit * row j+1);
+//   (2) the skew matvec partials of column j over the strictly lower triangle of
+//       A[j+1:n, j+1:n] on MR x MC tiles (strip-major), split into G contiguous tile ranges, one
+//       per rank, on the unscaled x (x_{j+1} = alpha): the consumer forms
+//       A v = scale * A x + corr * A[:, j+1], so the matvec waits for no reduction.  Per tile:
+//       wave w -> columns 4w..4w+3 (lane >> 4), lane & 15 = p -> row pairs
+//       {32 h + 2p, 32 h + 2p + 1}, h = 0..7 (16 entries per lane, read as 16-byte pairs); the
+//       next tile's loads are in flight while the current one is reduced.
+//         Pcol[I][c]    = -sum_{r in strip I} A[r,c] x_r                     (one entry per tile)
+//         Prow[slot][r] =  sum over the rank's tiles of strip I of A[r,c] x_c,
 //                          slot = rank - (owner of the strip's first tile)
+//   (3) after the matvec, the reflector scalars beta, tau, 1/(alpha - beta), corr from the block
+//       norm partials (the next row pass's inputs), then the slot emptying.
 // The first tile is loaded before the poll (tile data is not written while any rank can still read
 // it: a column c > j is stored as a factor in row pass c + 1, after every rank's matvec of column
-// j has been consumed); the poll of (1), (2) and the first tile's v entries (unscaled x) is one
-// memory round trip once they are stored; v = scale * x is applied after.
+// j has been consumed); the poll of (1) and the first tile's x entries is one memory round trip
+// once they are stored.
 // Only ranks with a part in column j take it (tiles, a live t element, or rows of w_j to finish
 // in the next row pass): every value such a rank reads is consumed by a row pass, which is what
 // lets producers refill the slots with the empty marker safely.
@@ -940,17 +941,17 @@ __device__ __forceinline__ bool tall_ok(int k, int p, int lo, int hi)
     return rl > lo && rl < hi;
 }
 
-// unscaled x_c of column j (x_j in xj) where v_c = scale * x_c (0 elsewhere)
+// unscaled x_c of column j (x_j in xj) on the rows of a strip: rows past j + 1 (row j + 1 meets
+// only columns <= j, which are masked)
 __device__ __forceinline__ double xraw(const double* __restrict__ xj, int64_t n, int64_t j, int64_t c)
 {
     return (c > j + 1 && c < n) ? ld_co(xj + c) : 0.0;
 }
 
-// v_c from its unscaled x_c
-__device__ __forceinline__ double vfrom(int64_t n, int64_t j, int64_t c, double x, const ColScal& s)
+// unscaled x_c of column j for a tile column: alpha at c = j + 1, 0 outside (j, n)
+__device__ __forceinline__ double xcol(const double* __restrict__ xj, int64_t n, int64_t j, int64_t c)
 {
-    if (c <= j || c >= n) return 0.0;
-    return (c == j + 1) ? s.unit : s.scale * x;
+    return (c > j && c < n) ? ld_co(xj + c) : 0.0;
 }
 
 // does rank g own a t element that is live at local column i?
@@ -1013,31 +1014,26 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         }
     }
     clk_inner(clk, 11);
-    // ---- poll what the row pass of column j produced ----
+    // ---- poll what the row pass of column j produced: this wave's t partials, the first tile's x ----
     const double* pnj = pnorm_of(w, j);
     const double* psj = ps_of(w, j);
     const double* xj = x_of(w, j);
     const int e0 = 4 * g + wave;                                 // this wave's first t element
     const bool t0 = e0 < 2 * NBMAX && (e0 % NBMAX) < i;          // wave-uniform
     const int64_t rs = I * MR + tid;                             // this thread's row of the first strip
-    double pn0 = 0.0, pn1 = 0.0, araw = 0.0, ps0 = 0.0, ps1 = 0.0, xc = 0.0, xs = 0.0;
+    double ps0 = 0.0, ps1 = 0.0, xc = 0.0, xs = 0.0;
     // every load of a round is issued before any is looked at (the loads are atomic, so the compiler
     // keeps their order and would otherwise wait for each before issuing the next); out-of-range
     // lanes load a valid slot and ignore it
     const bool in0 = lane < nq, in1 = lane + 64 < nq;
     const int64_t q0 = b0 + (in0 ? lane : 0), q1 = b0 + (in1 ? lane + 64 : 0);
-    const bool xcin = mv && c > j + 1 && c < n, xsin = mv && rs > j + 1 && rs < n;
+    const bool xcin = mv && c > j && c < n, xsin = mv && rs > j + 1 && rs < n;
     const double* pxc = xj + (xcin ? c : j + 1);
     const double* pxs = xj + (xsin ? rs : j + 1);
     const int64_t e0c = t0 ? e0 : 0;
     bool alive = true;
     for (unsigned spins = 0;; spins++) {
-        if (wave == 0) {   // wave-uniform branches: no per-lane exec mask around the loads
-            pn0 = ld_co(pnj + q0);
-            pn1 = ld_co(pnj + q1);
-            araw = ld_co(xj + (j + 1));
-        }
-        if (t0) {
+        if (t0) {   // wave-uniform branches: no per-lane exec mask around the loads
             ps0 = ld_co(psj + q0 * (2 * NBMAX) + e0c);
             ps1 = ld_co(psj + q1 * (2 * NBMAX) + e0c);
         }
@@ -1046,8 +1042,6 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
             xs = ld_co(pxs);
         }
         unsigned miss = 0;
-        if (wave == 0 && ((in0 && !filled(pn0)) || (in1 && !filled(pn1)))) miss |= 32;
-        if (wave == 0 && !filled(araw)) miss |= 64;
         if (t0 && ((in0 && !filled(ps0)) || (in1 && !filled(ps1)))) miss |= 128;
         if (xcin && !filled(xc)) miss |= 256;
         if (xsin && !filled(xs)) miss |= 512;
@@ -1058,48 +1052,17 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
             break;
         }
     }
-    if (!in0) {
-        pn0 = 0.0;
-        ps0 = 0.0;
-    }
-    if (!in1) {
-        pn1 = 0.0;
-        ps1 = 0.0;
-    }
+    if (!in0) ps0 = 0.0;
+    if (!in1) ps1 = 0.0;
     if (!xcin) xc = 0.0;
     if (!xsin) xs = 0.0;
     // the empty markers this rank stored in the last phase B have completed (they must before any
     // value of column j + 1 leaves this rank)
     drain();
     clk_inner(clk, 12);
-    // ---- (1) scalars, formed by every wave from the same values in the same order (bit-identical) ----
-    if (wave == 0 && alive) {
-        double acc = pn0 + pn1;
-        for (int64_t q = lane + 128; q < nq; q += 64) {
-            const double x = ld_co(pnj + b0 + q);
-            acc += settle(x, pnj + b0 + q, err, poll_code(POLL_SETTLE, j));
-        }
-        acc = wave_sum(acc);
-        const ColScal s = reflector_scalars(araw, acc);
-        if (lane == 0) {
-            sh.cur = s;
-            if (g == 0) {
-                tau_out[j] = s.tau;
-                eta_out[j] = s.beta;
-            }
-        }
-    }
-    if (__syncthreads_or(!alive)) return false;
-    const ColScal s = sh.cur;
-    // ---- empty this rank's t slots of column j + 2 (copy (j + 2) mod 3 = (j - 1) mod 3): column
-    // j - 1's t was consumed by the row passes of column j, which all finished before their norm
-    // partials of column j, which this wave has seen ----
-    if (lane == 0)
-        for (int e = e0; e < 2 * NBMAX; e += 4 * G) st_co(tv_of(w, j + 2) + e, empty_slot());
-    clk_inner(clk, 8);
-    if (mv) sh.vstrip[tid] = vfrom(n, j, rs, xs, s);
-    // ---- (2) t of the live elements of this wave ----
-    for (int e = e0; e < 2 * NBMAX; e += 4 * G) {
+    if (mv) sh.vstrip[tid] = xs;
+    // ---- (2) t of the live elements of this wave (unscaled: the consumer applies scale) ----
+    for (int e = e0; e < 2 * NBMAX && alive; e += 4 * G) {
         if ((e % NBMAX) >= i) continue;   // wave-uniform
         double acc;
         if (e == e0) {
@@ -1116,10 +1079,10 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
             acc += settle(ld_co(pq), pq, err, poll_code(POLL_SETTLE, j));
         }
         acc = wave_sum(acc);
-        if (lane == 0) st_co(tv_of(w, j) + e, s.scale * acc);
+        if (lane == 0) st_co(tv_of(w, j) + e, acc);
     }
     clk_inner(clk, 9);
-    __syncthreads();   // v of the first strip is in LDS
+    if (__syncthreads_or(!alive)) return false;   // x of the first strip is in LDS
     // ---- (3) matvec ----
     if (mv) {
         if constexpr (RES) {
@@ -1133,7 +1096,7 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         double racc[16];
 #pragma unroll
         for (int k = 0; k < 16; k++) racc[k] = 0.0;
-        double vc = vfrom(n, j, c, xc, s);
+        double vc = xc;   // unscaled x of the lane's column (alpha at j + 1, 0 outside (j, n))
         for (int64_t t = tlo; t < thi; t++) {
             const bool more = t + 1 < thi;
             const bool last_in_strip = (J == MPS * I + MPS - 1);
@@ -1142,7 +1105,7 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
             const int64_t cn = Jn * MC + 4 * wave + cg;
             double xcn = 0.0, xsn = 0.0;
             if (more) {
-                xcn = xraw(xj, n, j, cn);
+                xcn = xcol(xj, n, j, cn);
                 if (!RES || t != tlo) tall_load(A, lda, n, In, cn, p, an);   // (RES: tile tlo + 1 is loaded)
                 if (In != I) xsn = xraw(xj, n, j, In * MR + tid);
             }
@@ -1224,7 +1187,7 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
                         tot = (sh.mred[0][tid] + sh.mred[1][tid]) + (sh.mred[2][tid] + sh.mred[3][tid]);
                     }
                     if (r < n) st_co(prow_of(w, j) + cur_slot * w.ldp + r, tot);
-                    if (more) sh.vstrip[tid] = vfrom(n, j, In * MR + tid, settle(xsn, xj + In * MR + tid, err, poll_code(POLL_SETTLE, j)), s);
+                    if (more) sh.vstrip[tid] = settle(xsn, xj + In * MR + tid, err, poll_code(POLL_SETTLE, j));
                 }
                 __syncthreads();
                 cur_slot = 0;   // this rank owns the first tile of every later strip it touches
@@ -1232,7 +1195,7 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
             if (more) {
 #pragma unroll
                 for (int k = 0; k < 16; k++) a[k] = an[k];
-                vc = vfrom(n, j, cn, settle(xcn, xj + cn, err, poll_code(POLL_SETTLE, j)), s);
+                vc = settle(xcn, xj + cn, err, poll_code(POLL_SETTLE, j));
             }
             I = In;
             J = Jn;
@@ -1240,6 +1203,51 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         }
         clk_inner(clk, 10);
     }
+    // ---- (1) the reflector scalars of column j, after the matvec, which needs none of them: from the
+    // block norm partials, formed by every rank in the same fixed order (wave 0, strided lanes, a
+    // butterfly), so all ranks hold bit-identical values.  Having seen every block's norm partial of
+    // column j also tells this rank that every row pass of column j has finished (the slot emptying
+    // below relies on it) ----
+    if (wave == 0) {
+        double pn0 = 0.0, pn1 = 0.0, araw = 0.0;
+        bool ok = alive;
+        for (unsigned spins = 0; ok; spins++) {
+            pn0 = ld_co(pnj + q0);
+            pn1 = ld_co(pnj + q1);
+            araw = ld_co(xj + (j + 1));
+            unsigned miss = 0;
+            if ((in0 && !filled(pn0)) || (in1 && !filled(pn1))) miss |= 32;
+            if (!filled(araw)) miss |= 64;
+            if (__all(miss == 0)) break;
+            if (!poll_more(spins, err, poll_code(POLL_B, j))) {
+                poll_report(err, miss, j + 1);
+                ok = false;
+            }
+        }
+        if (ok) {
+            double acc = (in0 ? pn0 : 0.0) + (in1 ? pn1 : 0.0);
+            for (int64_t q = lane + 128; q < nq; q += 64) {
+                const double x = ld_co(pnj + b0 + q);
+                acc += settle(x, pnj + b0 + q, err, poll_code(POLL_SETTLE, j));
+            }
+            acc = wave_sum(acc);
+            const ColScal sc = reflector_scalars(araw, acc);
+            if (lane == 0) {
+                sh.cur = sc;
+                if (g == 0) {
+                    tau_out[j] = sc.tau;
+                    eta_out[j] = sc.beta;
+                }
+            }
+        }
+        alive = ok;
+    }
+    if (__syncthreads_or(!alive)) return false;
+    clk_inner(clk, 8);
+    // ---- empty this rank's t slots of column j + 2 (copy (j + 2) mod 3 = (j - 1) mod 3): column
+    // j - 1's t was consumed by the row passes of column j, all finished (above) ----
+    if (lane == 0)
+        for (int e = e0; e < 2 * NBMAX; e += 4 * G) st_co(tv_of(w, j + 2) + e, empty_slot());
     if (!mv) return true;
     // ---- empty this rank's matvec hand-over slots of column j + 2 (copy (j + 2) mod 3 = (j - 1) mod 3),
     // after its own partials of column j are on their way, off the path from the poll to the matvec.
