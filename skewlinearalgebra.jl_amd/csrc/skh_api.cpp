@@ -649,8 +649,8 @@ int zsytrd_blocked(int64_t n, double2* dA, int64_t lda, double2* dtau, double* d
     const int64_t nts = (n + skh::MR - 1) / skh::MR + 1;
     const int64_t nseg = grid + 2;
     const int64_t gp = (grid + 1) & ~int64_t(1);
-    // W | V2 | (Pnorm | PS | X) x 4 | (Tv | Pmu | Prow | Pcol) x 3 | Dfin, in doubles
-    const int64_t ostride = ntp + 2 * nt * 2 * nb + 2 * ld;
+    // W | V2 | (Pnorm | PS | X | Pq) x 4 | (Tv | Pmu | Prow | Pcol) x 3 | Dfin, in doubles
+    const int64_t ostride = ntp + 2 * nt * 2 * nb + 2 * ld + 2 * ntp;
     const int64_t hstride = 2 * 2 * nb + gp + 2 * nseg * ld + 2 * nts * ld;
     const size_t doubles = (size_t)(2 * 2 * ld * nb + skh::OWN_COPIES * ostride + skh::HAND_COPIES * hstride + 2 * ld + 64);
     double* ws = (double*)g_zbws.get(doubles * sizeof(double));
@@ -663,9 +663,11 @@ int zsytrd_blocked(int64_t n, double2* dA, int64_t lda, double2* dtau, double* d
     w.Pnorm = own0;
     w.PS = (double2*)(own0 + ntp);
     w.X = (double2*)(own0 + ntp + 2 * nt * 2 * nb);
+    w.Pq = (double2*)(own0 + ntp + 2 * nt * 2 * nb + 2 * ld);
     w.pn_stride = ostride;
     w.ps_stride = ostride / 2;
     w.x_stride = ostride / 2;
+    w.pq_stride = ostride / 2;
     double* hand0 = own0 + skh::OWN_COPIES * ostride;
     w.Tv = (double2*)hand0;
     w.Pmu = hand0 + 2 * 2 * nb;

@@ -66,6 +66,8 @@ __device__ __forceinline__ void stc(double2* p, cd v)
 struct ZScal {
     cd tau, scale;
     double beta, unit;
+    cd corr;   // unit - scale * alpha: v = scale * x + corr * e_{j+1} for the unscaled x (x_{j+1} = alpha)
+    cd q;      // x^H A22 e_{j+1} (the blocked panel kernel's m from the unscaled matvec)
 };
 
 __device__ __forceinline__ ZScal zreflector(cd alpha, double xn2)
@@ -87,6 +89,10 @@ __device__ __forceinline__ ZScal zreflector(cd alpha, double xn2)
         s.scale = cd{0.0, 0.0};
         s.unit = 0.0;
     }
+    // b is real with the sign opposite to Re alpha and |b| >= |alpha|, so |alpha - b|^2 >= |alpha|^2 + |b|^2
+    // and |scale * alpha| <= 1/sqrt(2): no cancellation
+    s.corr = cd{s.unit - (s.scale.re * alpha.re - s.scale.im * alpha.im), -(s.scale.re * alpha.im + s.scale.im * alpha.re)};
+    s.q = cd{0.0, 0.0};
     return s;
 }
 

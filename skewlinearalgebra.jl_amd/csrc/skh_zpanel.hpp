@@ -46,8 +46,9 @@ struct ZPWork {
     double* Pnorm;   // [nt] per copy: per-block sums of |x|^2 of the reflector tail
     double2* PS;     // [nt][2 ZNB] per copy: per-block W^H x and V^H x (rows >= j+2)
     double2* X;      // [ld] per copy: x_j, the updated column j below the diagonal
-    int64_t pn_stride, ps_stride, x_stride;   // copy strides (column mod 4)
-    double2* Tv;     // [2 ZNB] per copy: scale * (W^H x, V^H x) sums; the consumer adds the head
+    double2* Pq;     // [nt] per copy: per-block sums of conj(x_r) A[r, j+1], rows >= j+1
+    int64_t pn_stride, ps_stride, x_stride, pq_stride;   // copy strides (column mod 4)
+    double2* Tv;     // [2 ZNB] per copy: W^H x, V^H x sums (unscaled; the consumer scales, adds the head)
     double* Pmu;     // [grid] per copy: per-workgroup Im(v^H A22 v) partials
     double2* Prow;   // [nseg][ldp] per copy: row partials of the matvec
     double2* Pcol;   // [nts][ldp] per copy: column partials
@@ -59,6 +60,7 @@ struct ZPWork {
 __device__ __forceinline__ double* pnorm_of(const ZPWork& w, int64_t j) { return w.Pnorm + own_copy(j) * w.pn_stride; }
 __device__ __forceinline__ double2* ps_of(const ZPWork& w, int64_t j) { return w.PS + own_copy(j) * w.ps_stride; }
 __device__ __forceinline__ double2* x_of(const ZPWork& w, int64_t j) { return w.X + own_copy(j) * w.x_stride; }
+__device__ __forceinline__ double2* pq_of(const ZPWork& w, int64_t j) { return w.Pq + own_copy(j) * w.pq_stride; }
 __device__ __forceinline__ double2* tv_of(const ZPWork& w, int64_t j) { return w.Tv + hand_copy(j) * w.tv_stride; }
 __device__ __forceinline__ double* pmu_of(const ZPWork& w, int64_t j) { return w.Pmu + hand_copy(j) * w.mu_stride; }
 __device__ __forceinline__ double2* prow_of(const ZPWork& w, int64_t j) { return w.Prow + hand_copy(j) * w.pr_stride; }
@@ -145,6 +147,7 @@ __device__ bool zrow_pass(double2* __restrict__ A, int64_t lda, int64_t n, int64
         if (wave == 0) {
             if (rin) cempty(x_of(w, j + 2) + r);
             if (lane == 0) st_co(pnorm_of(w, j + 2) + b, empty_slot());
+            if (lane == 1) cempty(pq_of(w, j + 2) + b);
         }
         if (lane < KQ) {
             double2* ps2 = ps_of(w, j + 2) + b * (2 * ZNB) + k0 + lane;
@@ -160,6 +163,8 @@ __device__ bool zrow_pass(double2* __restrict__ A, int64_t lda, int64_t n, int64
         wreg[u] = rr.w[(wave * KQ + u) * 64 + lane];
     }
     const cd xo = rr.xprev, xold = rr.xnext;
+    // column j + 1 from its diagonal (as at the launch's start): q's partial here, the next pass's input
+    const cd xn1 = (doA && rin && r >= j + 1 && j + 1 < n) ? zc::ld(A + r + (j + 1) * lda) : czero();
     cd pp = czero();
     cd tv1 = czero(), tv2 = czero(), wraw = czero(), vraw = czero(), sp[4] = {czero(), czero(), czero(), czero()};
     double pmu = 0.0;
@@ -277,9 +282,9 @@ __device__ bool zrow_pass(double2* __restrict__ A, int64_t lda, int64_t n, int64
     }
     // ---- 2. shared inputs into LDS, the special row of w_{j-1} ----
     if (doC && tid < im) {
-        // t of column j-1 = unit * conj(row j of W, V) + scale * (sums over rows > j)
-        sh.t[tid] = cadd(cd{cs.unit * wraw.re, -cs.unit * wraw.im}, tv1);
-        sh.t[ZNB + tid] = cadd(cd{cs.unit * vraw.re, -cs.unit * vraw.im}, tv2);
+        // t of column j-1 = unit * conj(row j of W, V) + scale * (sums over rows > j of W^H x, V^H x)
+        sh.t[tid] = cadd(cd{cs.unit * wraw.re, -cs.unit * wraw.im}, cmul(cs.scale, tv1));
+        sh.t[ZNB + tid] = cadd(cd{cs.unit * vraw.re, -cs.unit * vraw.im}, cmul(cs.scale, tv2));
         sh.wrow[tid] = wraw;
         sh.vrow[tid] = vraw;
     }
@@ -295,10 +300,15 @@ __device__ bool zrow_pass(double2* __restrict__ A, int64_t lda, int64_t n, int64
             cor = cadd(cor, csub(cmul(sh.vrow[k], sh.t[k]), cmul(sh.wrow[k], sh.t[ZNB + k])));
             tt += cmulc(sh.t[k], sh.t[ZNB + k]).im;   // t1 conj(t2), its imaginary part
         }
-        acc = cwave_sum(cadd(acc, cor));
+        // the column partials are of the unscaled x
+        acc = cwave_sum(cadd(cmul(cs.scale, acc), cor));
         tt = wave_sum(tt);
         if (lane == 0) {
-            const double m = (sh.mu[0] + sh.mu[1]) + (sh.mu[2] + sh.mu[3]) + 2.0 * tt;
+            // Im(v^H A22 v) for v = s x + c e_{j}: |s|^2 Im(x^H A22 x) + 2 Im(conj(s) c q) + |c|^2 Im(A[j, j])
+            const double s2 = cs.scale.re * cs.scale.re + cs.scale.im * cs.scale.im;
+            const double c2 = cs.corr.re * cs.corr.re + cs.corr.im * cs.corr.im;
+            const cd scq = cmul(cconj(cs.scale), cmul(cs.corr, cs.q));
+            const double m = s2 * ((sh.mu[0] + sh.mu[1]) + (sh.mu[2] + sh.mu[3])) + 2.0 * scq.im + c2 * ajj.im + 2.0 * tt;
             const double t2 = cs.tau.re * cs.tau.re + cs.tau.im * cs.tau.im;
             const cd c = cd{0.0, -0.5 * t2 * m};
             const cd yj = cadd(acc, cd{ajj.re * cs.unit, ajj.im * cs.unit});
@@ -341,7 +351,9 @@ __device__ bool zrow_pass(double2* __restrict__ A, int64_t lda, int64_t n, int64
                 }
             }
         }
-        if (doC) sh.red[wave][lane] = rowC ? cadd(pp, c4) : czero();
+        // y_r = s (A22 x)_r + c A[r, j] (xold = A[r, j] as at the launch's start); pp is this wave's part
+        // of the partials, the c term is wave 0's
+        if (doC) sh.red[wave][lane] = rowC ? cadd(cadd(cmul(cs.scale, pp), (wave == 0) ? cmul(cs.corr, xold) : czero()), c4) : czero();
         if (doA) sh.red2[wave][lane] = p4;
     }
     __syncthreads();
@@ -383,6 +395,10 @@ __device__ bool zrow_pass(double2* __restrict__ A, int64_t lda, int64_t n, int64
             const double ss = wave_sum(tail ? zc::norm2(x) : 0.0);
             if (lane == 0) st_co(pnorm_of(w, j) + b, ss);
         }
+        if (wave == 1) {   // the block's part of q = x^H A22 e_{j+1} (rows >= j + 1)
+            const cd qb = cwave_sum((live && r >= j + 1) ? cmulc(xn1, x) : czero());
+            if (lane == 0) cst(pq_of(w, j) + b, qb);
+        }
         // W^H x and V^H x of the block's rows >= j+2: values 0..7 Re / 8..15 Im of conj(W) x,
         // 16..23 Re / 24..31 Im of conj(V) x (one k-group of KQ = 8 per wave)
         const cd xt = tail ? x : czero();
@@ -419,7 +435,7 @@ __device__ bool zrow_pass(double2* __restrict__ A, int64_t lda, int64_t n, int64
         }
     }
     rr.xprev = x;
-    rr.xnext = (doA && rin && r >= j + 1 && j + 1 < n) ? zc::ld(A + r + (j + 1) * lda) : czero();
+    rr.xnext = xn1;
     __syncthreads();
     return true;
 }
@@ -444,22 +460,23 @@ __device__ __forceinline__ void ztall_load(const double2* __restrict__ A, int64_
     }
 }
 
-__device__ __forceinline__ cd zvfrom(int64_t n, int64_t j, int64_t c, cd x, const ZScal& s)
-{
-    if (c <= j || c >= n) return czero();
-    if (c == j + 1) return cd{s.unit, 0.0};
-    return cmul(s.scale, x);
-}
-
+// unscaled x_c of column j on the rows of a strip (rows past j + 1; row j + 1 meets only masked columns)
 __device__ __forceinline__ cd zxraw(const double2* xj, int64_t n, int64_t j, int64_t c)
 {
     return (c > j + 1 && c < n) ? cld(xj + c) : czero();
 }
 
+// unscaled x_c of column j for a tile column: alpha at c = j + 1, 0 outside (j, n)
+__device__ __forceinline__ cd zxcol(const double2* xj, int64_t n, int64_t j, int64_t c)
+{
+    return (c > j && c < n) ? cld(xj + c) : czero();
+}
+
 // ---------------------------------------------------------------------------------------
-// Phase B of column j: scalars, t, and the matvec y = A22 v over the trailing block's lower
-// triangle with its diagonal (row sums A[r,c] v_c for r >= c, column sums -conj(A[r,c]) v_r for
-// r > c, m = sum 2 Im(conj(v_r) A[r,c] v_c) (r > c) + Im(A[c,c]) |v_c|^2).
+// Phase B of column j: t, the matvec y = A22 x over the trailing block's lower triangle with its
+// diagonal, on the unscaled x (row sums A[r,c] x_c for r >= c, column sums -conj(A[r,c]) x_r for
+// r > c, m partials sum 2 Im(conj(x_r) A[r,c] x_c) (r > c) + Im(A[c,c]) |x_c|^2), then the scalars.
+// The row pass forms A22 v = s A22 x + c A22 e_{j+1} (v = s x + c e_{j+1}) and m from the same parts.
 // ---------------------------------------------------------------------------------------
 __device__ bool zphase_b(const double2* __restrict__ A, int64_t lda, int64_t n, int64_t c0, int64_t j, int i,
                          const ZPWork& w, ZPShared& sh, double2* tau_out, double* eta_out, int g, int G,
@@ -481,6 +498,7 @@ __device__ bool zphase_b(const double2* __restrict__ A, int64_t lda, int64_t n, 
         ztall_load(A, lda, n, I, c, p, a);
     }
     const double* pnj = pnorm_of(w, j);
+    const double2* pqj = pq_of(w, j);
     const double2* psj = ps_of(w, j);
     const double2* xj = x_of(w, j);
     const int e0 = 4 * g + wave;
@@ -488,19 +506,14 @@ __device__ bool zphase_b(const double2* __restrict__ A, int64_t lda, int64_t n, 
     const int64_t rs = I * MR + tid;
     const bool in0 = lane < nq, in1 = lane + 64 < nq;
     const int64_t q0 = b0 + (in0 ? lane : 0), q1 = b0 + (in1 ? lane + 64 : 0);
-    const bool xcin = mv && c > j + 1 && c < n, xsin = mv && rs > j + 1 && rs < n;
+    const bool xcin = mv && c > j && c < n, xsin = mv && rs > j + 1 && rs < n;
     const double2* pxc = xj + (xcin ? c : j + 1);
     const double2* pxs = xj + (xsin ? rs : j + 1);
     const int64_t e0c = t0 ? e0 : 0;
-    double pn0 = 0.0, pn1 = 0.0;
-    cd araw = czero(), ps0 = czero(), ps1 = czero(), xc = czero(), xs = czero();
+    cd ps0 = czero(), ps1 = czero(), xc = czero(), xs = czero();
     bool alive = true;
+    // ---- poll this wave's t partials and the first tile's x (the matvec runs on the unscaled x) ----
     for (unsigned spins = 0;; spins++) {
-        if (wave == 0) {
-            pn0 = ld_co(pnj + q0);
-            pn1 = ld_co(pnj + q1);
-            araw = cld(xj + (j + 1));
-        }
         if (t0) {
             ps0 = cld(psj + q0 * (2 * ZNB) + e0c);
             ps1 = cld(psj + q1 * (2 * ZNB) + e0c);
@@ -510,10 +523,6 @@ __device__ bool zphase_b(const double2* __restrict__ A, int64_t lda, int64_t n, 
             xs = cld(pxs);
         }
         unsigned miss = 0;
-        if (wave == 0) {
-            if ((in0 && !filled(pn0)) || (in1 && !filled(pn1))) miss |= 32;
-            if (!cfilled(araw)) miss |= 64;
-        }
         if (t0 && ((in0 && !cfilled(ps0)) || (in1 && !cfilled(ps1)))) miss |= 128;
         if (xcin && !cfilled(xc)) miss |= 256;
         if (xsin && !cfilled(xs)) miss |= 512;
@@ -524,38 +533,14 @@ __device__ bool zphase_b(const double2* __restrict__ A, int64_t lda, int64_t n, 
             break;
         }
     }
-    if (!in0) {
-        pn0 = 0.0;
-        ps0 = czero();
-    }
-    if (!in1) {
-        pn1 = 0.0;
-        ps1 = czero();
-    }
+    if (!in0) ps0 = czero();
+    if (!in1) ps1 = czero();
     if (!xcin) xc = czero();
     if (!xsin) xs = czero();
     drain();
-    // ---- (1) scalars ----
-    if (wave == 0 && alive) {
-        double acc = pn0 + pn1;
-        for (int64_t q = lane + 128; q < nq; q += 64) acc += settle(ld_co(pnj + b0 + q), pnj + b0 + q, err, poll_code(POLL_SETTLE, j));
-        acc = wave_sum(acc);
-        if (lane == 0) {
-            const ZScal s = zc::zreflector(araw, acc);
-            sh.cur = s;
-            if (g == 0) {
-                tau_out[j] = make_double2(s.tau.re, s.tau.im);
-                eta_out[j] = s.beta;
-            }
-        }
-    }
-    if (__syncthreads_or(!alive)) return false;
-    if (lane == 0)
-        for (int e = e0; e < 2 * ZNB; e += 4 * G) cempty(tv_of(w, j + 2) + e);
-    const ZScal s = sh.cur;
-    if (mv) sh.vstrip[tid] = zvfrom(n, j, rs, xs, s);
-    // ---- (2) t of the live elements of this wave ----
-    for (int e = e0; e < 2 * ZNB; e += 4 * G) {
+    if (mv) sh.vstrip[tid] = xs;
+    // ---- (1) t of the live elements of this wave (unscaled: the consumer applies scale) ----
+    for (int e = e0; e < 2 * ZNB && alive; e += 4 * G) {
         if ((e % ZNB) >= i) continue;   // wave-uniform
         cd acc;
         if (e == e0) {
@@ -578,100 +563,154 @@ __device__ bool zphase_b(const double2* __restrict__ A, int64_t lda, int64_t n, 
             acc = cadd(acc, v);
         }
         acc = cwave_sum(acc);
-        if (lane == 0) cst(tv_of(w, j) + e, cmul(s.scale, acc));
+        if (lane == 0) cst(tv_of(w, j) + e, acc);
     }
-    // ---- (3) matvec ----
-    if (!mv) return true;
-    __syncthreads();   // v of the first strip is in LDS
-    cd racc[16];
+    if (__syncthreads_or(!alive)) return false;   // x of the first strip is in LDS
+    // ---- (2) matvec on the unscaled x ----
+    if (mv) {
+        cd racc[16];
 #pragma unroll
-    for (int k = 0; k < 16; k++) racc[k] = czero();
-    double mu = 0.0;
-    cd vc = zvfrom(n, j, c, xc, s);
-    for (int64_t t = tlo; t < thi; t++) {
-        const bool more = t + 1 < thi;
-        const bool last_in_strip = (J == MPS * I + MPS - 1);
-        const int64_t In = last_in_strip ? I + 1 : I;
-        const int64_t Jn = last_in_strip ? tg.Jlo : J + 1;
-        const int64_t cn = Jn * MC + 4 * wave + cg;
-        cd an[16], xcn = czero(), xsn = czero();
-        if (more) {
-            xcn = zxraw(xj, n, j, cn);
-            ztall_load(A, lda, n, In, cn, p, an);
-            if (In != I) xsn = zxraw(xj, n, j, In * MR + tid);
-        }
-        cd cp = czero();
-        {
-            const bool cok = c > j && c < n;
-            const int64_t lo = cok ? c - I * MR : (int64_t)MR, hi = n - I * MR;
-#pragma unroll
-            for (int k = 0; k < 16; k++) {
-                const int rl = ztile_row(k, p);
-                const bool below = rl > lo && rl < hi;
-                const bool diag = cok && rl == lo;
-                const cd av = (below || diag) ? a[k] : czero();
-                racc[k] = cadd(racc[k], cmul(av, vc));
-                const cd vr = sh.vstrip[rl];
-                if (below) {
-                    cp = cadd(cp, cmul(cconj(av), vr));              // conj(A[r,c]) v_r
-                    mu += 2.0 * cmulc(cmul(av, vc), vr).im;          // Im(conj(v_r) A[r,c] v_c)
-                } else if (diag) {
-                    mu += av.im * zc::norm2(vc);
-                }
+        for (int k = 0; k < 16; k++) racc[k] = czero();
+        double mu = 0.0;
+        cd vc = xc;   // alpha at j + 1, 0 outside (j, n)
+        for (int64_t t = tlo; t < thi; t++) {
+            const bool more = t + 1 < thi;
+            const bool last_in_strip = (J == MPS * I + MPS - 1);
+            const int64_t In = last_in_strip ? I + 1 : I;
+            const int64_t Jn = last_in_strip ? tg.Jlo : J + 1;
+            const int64_t cn = Jn * MC + 4 * wave + cg;
+            cd an[16], xcn = czero(), xsn = czero();
+            if (more) {
+                xcn = zxcol(xj, n, j, cn);
+                ztall_load(A, lda, n, In, cn, p, an);
+                if (In != I) xsn = zxraw(xj, n, j, In * MR + tid);
             }
-        }
-        cp = cd{row16_sum(cp.re), row16_sum(cp.im)};
-        if (p == 0 && c > j && c < n) cst(pcol_of(w, j) + I * w.ldp + c, cd{-cp.re, -cp.im});
-        if (!more || In != I) {
-            // flush: each (wave, column group) writes its 16 rows to its own LDS row, thread t sums
-            // the 16 values of strip row t in a fixed order (an LDS transposition)
+            cd cp = czero();
             {
-                cd* mr = sh.mred[wave * 4 + cg];
+                const bool cok = c > j && c < n;
+                const int64_t lo = cok ? c - I * MR : (int64_t)MR, hi = n - I * MR;
 #pragma unroll
                 for (int k = 0; k < 16; k++) {
-                    mr[ztile_row(k, p)] = racc[k];
-                    racc[k] = czero();
+                    const int rl = ztile_row(k, p);
+                    const bool below = rl > lo && rl < hi;
+                    const bool diag = cok && rl == lo;
+                    const cd av = (below || diag) ? a[k] : czero();
+                    racc[k] = cadd(racc[k], cmul(av, vc));
+                    const cd vr = sh.vstrip[rl];
+                    if (below) {
+                        cp = cadd(cp, cmul(cconj(av), vr));              // conj(A[r,c]) v_r
+                        mu += 2.0 * cmulc(cmul(av, vc), vr).im;          // Im(conj(v_r) A[r,c] v_c)
+                    } else if (diag) {
+                        mu += av.im * zc::norm2(vc);
+                    }
                 }
             }
-            __syncthreads();
-            {
-                const int64_t r = I * MR + tid;
-                cd q4[4];
+            cp = cd{row16_sum(cp.re), row16_sum(cp.im)};
+            if (p == 0 && c > j && c < n) cst(pcol_of(w, j) + I * w.ldp + c, cd{-cp.re, -cp.im});
+            if (!more || In != I) {
+                // flush: each (wave, column group) writes its 16 rows to its own LDS row, thread t sums
+                // the 16 values of strip row t in a fixed order (an LDS transposition)
+                {
+                    cd* mr = sh.mred[wave * 4 + cg];
 #pragma unroll
-                for (int q = 0; q < 4; q++)
-                    q4[q] = cadd(cadd(sh.mred[4 * q][tid], sh.mred[4 * q + 1][tid]), cadd(sh.mred[4 * q + 2][tid], sh.mred[4 * q + 3][tid]));
-                if (r < n) cst(prow_of(w, j) + cur_slot * w.ldp + r, cadd(cadd(q4[0], q4[1]), cadd(q4[2], q4[3])));
-                if (more) {
-                    const double2* px = xj + In * MR + tid;
-                    xsn.re = settle(xsn.re, reinterpret_cast<const double*>(px), err, poll_code(POLL_SETTLE, j));
-                    xsn.im = settle(xsn.im, reinterpret_cast<const double*>(px) + 1, err, poll_code(POLL_SETTLE, j));
-                    sh.vstrip[tid] = zvfrom(n, j, In * MR + tid, xsn, s);
+                    for (int k = 0; k < 16; k++) {
+                        mr[ztile_row(k, p)] = racc[k];
+                        racc[k] = czero();
+                    }
                 }
-            }
-            __syncthreads();
-            cur_slot = 0;
-        }
-        if (more) {
+                __syncthreads();
+                {
+                    const int64_t r = I * MR + tid;
+                    cd q4[4];
 #pragma unroll
-            for (int k = 0; k < 16; k++) a[k] = an[k];
-            const double2* px = xj + cn;
-            xcn.re = settle(xcn.re, reinterpret_cast<const double*>(px), err, poll_code(POLL_SETTLE, j));
-            xcn.im = settle(xcn.im, reinterpret_cast<const double*>(px) + 1, err, poll_code(POLL_SETTLE, j));
-            vc = zvfrom(n, j, cn, xcn, s);
+                    for (int q = 0; q < 4; q++)
+                        q4[q] = cadd(cadd(sh.mred[4 * q][tid], sh.mred[4 * q + 1][tid]), cadd(sh.mred[4 * q + 2][tid], sh.mred[4 * q + 3][tid]));
+                    if (r < n) cst(prow_of(w, j) + cur_slot * w.ldp + r, cadd(cadd(q4[0], q4[1]), cadd(q4[2], q4[3])));
+                    if (more) {
+                        const double2* px = xj + In * MR + tid;
+                        xsn.re = settle(xsn.re, reinterpret_cast<const double*>(px), err, poll_code(POLL_SETTLE, j));
+                        xsn.im = settle(xsn.im, reinterpret_cast<const double*>(px) + 1, err, poll_code(POLL_SETTLE, j));
+                        sh.vstrip[tid] = xsn;
+                    }
+                }
+                __syncthreads();
+                cur_slot = 0;
+            }
+            if (more) {
+#pragma unroll
+                for (int k = 0; k < 16; k++) a[k] = an[k];
+                const double2* px = xj + cn;
+                xcn.re = settle(xcn.re, reinterpret_cast<const double*>(px), err, poll_code(POLL_SETTLE, j));
+                xcn.im = settle(xcn.im, reinterpret_cast<const double*>(px) + 1, err, poll_code(POLL_SETTLE, j));
+                vc = xcn;
+            }
+            I = In;
+            J = Jn;
+            c = cn;
         }
-        I = In;
-        J = Jn;
-        c = cn;
+        // this workgroup's partial of m (of the unscaled x)
+        mu = wave_sum(mu);
+        if (lane == 0) sh.mu[wave] = mu;
+        __syncthreads();
+        if (tid == 0) st_co(pmu_of(w, j) + g, (sh.mu[0] + sh.mu[1]) + (sh.mu[2] + sh.mu[3]));
     }
-    // this workgroup's partial of m
-    mu = wave_sum(mu);
-    if (lane == 0) sh.mu[wave] = mu;
-    __syncthreads();
-    if (tid == 0) st_co(pmu_of(w, j) + g, (sh.mu[0] + sh.mu[1]) + (sh.mu[2] + sh.mu[3]));
+    // ---- (3) scalars of column j after the matvec, which needs none of them (the next row pass's
+    // inputs, with q = x^H A22 e_{j+1} from the block partials).  Having seen every block's norm
+    // partial of column j tells this rank that every row pass of column j has finished, which the slot
+    // emptying below relies on ----
+    if (wave == 0) {
+        double pn0 = 0.0, pn1 = 0.0;
+        cd araw = czero(), pq0 = czero(), pq1 = czero();
+        bool ok = alive;
+        for (unsigned spins = 0; ok; spins++) {
+            pn0 = ld_co(pnj + q0);
+            pn1 = ld_co(pnj + q1);
+            pq0 = cld(pqj + q0);
+            pq1 = cld(pqj + q1);
+            araw = cld(xj + (j + 1));
+            unsigned miss = 0;
+            if ((in0 && !(filled(pn0) && cfilled(pq0))) || (in1 && !(filled(pn1) && cfilled(pq1)))) miss |= 32;
+            if (!cfilled(araw)) miss |= 64;
+            if (__all(miss == 0)) break;
+            if (!poll_more(spins, err, poll_code(POLL_B, j))) {
+                poll_report(err, miss, j + 1);
+                ok = false;
+            }
+        }
+        if (ok) {
+            double acc = (in0 ? pn0 : 0.0) + (in1 ? pn1 : 0.0);
+            cd qa = cadd(in0 ? pq0 : czero(), in1 ? pq1 : czero());
+            for (int64_t q = lane + 128; q < nq; q += 64) {
+                acc += settle(ld_co(pnj + b0 + q), pnj + b0 + q, err, poll_code(POLL_SETTLE, j));
+                const double2* pp = pqj + b0 + q;
+                cd v = cld(pp);
+                v.re = settle(v.re, reinterpret_cast<const double*>(pp), err, poll_code(POLL_SETTLE, j));
+                v.im = settle(v.im, reinterpret_cast<const double*>(pp) + 1, err, poll_code(POLL_SETTLE, j));
+                qa = cadd(qa, v);
+            }
+            acc = wave_sum(acc);
+            qa = cwave_sum(qa);
+            if (lane == 0) {
+                ZScal sc = zc::zreflector(araw, acc);
+                sc.q = qa;
+                sh.cur = sc;
+                if (g == 0) {
+                    tau_out[j] = make_double2(sc.tau.re, sc.tau.im);
+                    eta_out[j] = sc.beta;
+                }
+            }
+        }
+        alive = ok;
+    }
+    if (__syncthreads_or(!alive)) return false;
+    // column j - 1's t was consumed by the row passes of column j, all finished (above)
+    if (lane == 0)
+        for (int e = e0; e < 2 * ZNB; e += 4 * G) cempty(tv_of(w, j + 2) + e);
+    if (!mv) return true;
     // ---- empty this rank's matvec hand-over slots of column j + 2 (copy (j + 2) mod 3 = (j - 1) mod 3)
     // after its own outputs of column j are on their way, off the path from the poll to the matvec:
     // column j - 1's values there were consumed by the row passes of column j, which all finished
-    // before their norm partials of column j, which wave 0 saw before the first barrier above; the
+    // before their norm partials of column j, which wave 0 saw before the barrier above; the
     // next phase B drains these stores before storing anything of column j + 1 ----
     {
         double2* prn = prow_of(w, j + 2);
