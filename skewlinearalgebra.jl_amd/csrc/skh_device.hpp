@@ -392,14 +392,18 @@ __host__ __device__ inline TallGrid tall_grid(int64_t n, int64_t s0)
 // Weighted split of the T matvec tiles over Ge ranks: the first O ranks (the panel's row-block owners,
 // which also run the row pass and start their matvec after it) take WNUM / WDEN of the others' share,
 // so that they do not finish it last.  Rank g's range starts at floor(T vpos(g) / Lv) with vpos(g) =
-// WNUM min(g, O) + WDEN max(g - O, 0) and Lv = vpos(Ge).  Applied only while T >= 2 Ge (every range then
-// holds at least one tile), else the split is uniform.  64-bit arithmetic (computed once per launch).
+// WNUM min(g, O) + WDEN max(g - O, 0) and Lv = vpos(Ge).  Applied only while T >= 2 Ge (with
+// WNUM / WDEN >= 1/2 every range then holds at least one tile), else the split is uniform.  64-bit
+// arithmetic (computed once per launch).  5/8 measured best at n = 8192 (186.6 ms; 3/4 187.3, 7/8 189.1,
+// 1/1 193.0).
 #ifndef SKH_OWNER_WNUM
-#define SKH_OWNER_WNUM 3
+#define SKH_OWNER_WNUM 5
 #endif
 #ifndef SKH_OWNER_WDEN
-#define SKH_OWNER_WDEN 4
+#define SKH_OWNER_WDEN 8
 #endif
+static_assert(2 * SKH_OWNER_WNUM >= SKH_OWNER_WDEN && SKH_OWNER_WNUM <= SKH_OWNER_WDEN,
+              "owner weight must lie in [1/2, 1]: a smaller one can leave an owner without tiles");
 struct TileSplit {
     uint64_t T, Ge, O, wn, wd, Lv;
 };
