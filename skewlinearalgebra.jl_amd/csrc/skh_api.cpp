@@ -649,17 +649,20 @@ int zsytrd_blocked(int64_t n, double2* dA, int64_t lda, double2* dtau, double* d
     const int64_t nts = (n + skh::MR - 1) / skh::MR + 1;
     const int64_t nseg = grid + 2;
     const int64_t gp = (grid + 1) & ~int64_t(1);
-    // W | V2 | (Pnorm | PS | X | Pq) x 4 | (Tv | Pmu | Prow | Pcol) x 3 | Dfin, in doubles
+    // (W | V2) x 2 | (Pnorm | PS | X | Pq) x 4 | (Tv | Pmu | Prow | Pcol) x 3 | Dfin, in doubles; the two
+    // W | V2 pairs alternate by panel parity, so each rank-2k launch (reading its panel's W) can fill
+    // the next panel's slots
+    const int64_t wpair = 2 * 2 * ld * nb;
     const int64_t ostride = ntp + 2 * nt * 2 * nb + 2 * ld + 2 * ntp;
     const int64_t hstride = 2 * 2 * nb + gp + 2 * nseg * ld + 2 * nts * ld;
-    const size_t doubles = (size_t)(2 * 2 * ld * nb + skh::OWN_COPIES * ostride + skh::HAND_COPIES * hstride + 2 * ld + 64);
+    const size_t doubles = (size_t)(2 * wpair + skh::OWN_COPIES * ostride + skh::HAND_COPIES * hstride + 2 * ld + 64);
     double* ws = (double*)g_zbws.get(doubles * sizeof(double));
     if (!ws) return set_err(SKH_ERR_HIP, "hipMalloc of %zu bytes of workspace failed", doubles * 8);
     skh::zp::ZPWork w;
     w.W = (double2*)ws;
     w.V2 = w.W + ld * nb;
     w.ldw = ld;
-    double* own0 = ws + 2 * 2 * ld * nb;
+    double* own0 = ws + 2 * wpair;
     w.Pnorm = own0;
     w.PS = (double2*)(own0 + ntp);
     w.X = (double2*)(own0 + ntp + 2 * nt * 2 * nb);
@@ -679,39 +682,64 @@ int zsytrd_blocked(int64_t n, double2* dA, int64_t lda, double2* dtau, double* d
     w.pr_stride = hstride / 2;
     w.pc_stride = hstride / 2;
     w.Dfin = (double2*)(hand0 + skh::HAND_COPIES * hstride);
-    for (int64_t c0 = 0; c0 < kb_end; c0 += nb) {
-        const int nbp = (int)std::min<int64_t>(nb, kb_end - c0);
+    auto zgrid = [&](int64_t c0) {
         const skh::TallGrid tg = skh::tall_grid(n, c0 + 1);
         const int64_t blocks = (n + skh::TB - 1) / skh::TB - c0 / skh::TB;
-        const int g = (int)std::min<int64_t>(grid, std::max<int64_t>(std::max<int64_t>(8, blocks), tg.T));
-        // the slots polled before any producer of this launch refills them start empty
-        {
-            int64_t maxslot = 1;
-            for (int64_t I = tg.I0; I < tg.nts; I++) maxslot = std::max(maxslot, skh::tall_slots(tg, I, g));
+        return (int)std::min<int64_t>(grid, std::max<int64_t>(std::max<int64_t>(8, blocks), tg.T));
+    };
+    auto zpanel_w = [&](int64_t c0) {
+        skh::zp::ZPWork pw = w;
+        if ((c0 / nb) & 1) {
+            pw.W = (double2*)(ws + wpair);
+            pw.V2 = pw.W + ld * nb;
+        }
+        return pw;
+    };
+    // the slots polled before any producer of the panel's launch refills them start empty
+    auto zpanel_fill = [&](int64_t c0, int nbp, int g, const skh::zp::ZPWork& pw, FillSegs& fs) {
+        const skh::TallGrid tg = skh::tall_grid(n, c0 + 1);
+        int64_t maxslot = 1;
+        for (int64_t I = tg.I0; I < tg.nts; I++) maxslot = std::max(maxslot, skh::tall_slots(tg, I, g));
+        int ns = 0;
+        auto seg = [&](double* p, int64_t rows, int64_t ldr, int64_t lo, int64_t hi) {
+            fs.p[ns] = p;
+            fs.rows[ns] = rows;
+            fs.ld[ns] = ldr;
+            fs.lo[ns] = lo;
+            fs.hi[ns] = hi;
+            ns++;
+        };
+        for (int64_t cc = c0; cc < c0 + 2; cc++) {
+            double* h = hand0 + (cc % skh::HAND_COPIES) * hstride;
+            seg(h, 1, 0, 0, 2 * 2 * nb + gp);                                   // t and m partials
+            seg(h + 2 * 2 * nb + gp, maxslot, 2 * ld, 2 * c0, 2 * n);           // row partials
+            seg(h + 2 * 2 * nb + gp + 2 * nseg * ld, tg.nts, 2 * ld, 2 * c0, 2 * n);   // column partials
+            seg(own0 + (cc & (skh::OWN_COPIES - 1)) * ostride, 1, 0, 0, ostride);
+        }
+        seg(hand0 + ((c0 + 2) % skh::HAND_COPIES) * hstride, 1, 0, 0, 2 * 2 * nb);
+        seg((double*)pw.W, nbp, 2 * ld, 2 * c0, 2 * n);
+        seg((double*)pw.V2, nbp, 2 * ld, 2 * c0, 2 * n);
+        return ns;
+    };
+    for (int64_t c0 = 0; c0 < kb_end; c0 += nb) {
+        const int nbp = (int)std::min<int64_t>(nb, kb_end - c0);
+        const int g = zgrid(c0);
+        const skh::zp::ZPWork pw = zpanel_w(c0);
+        if (c0 == 0) {   // later panels' slots are filled by the rank-2k launch before them
             FillSegs fs;
-            int ns = 0;
-            auto seg = [&](double* p, int64_t rows, int64_t ldr, int64_t lo, int64_t hi) {
-                fs.p[ns] = p;
-                fs.rows[ns] = rows;
-                fs.ld[ns] = ldr;
-                fs.lo[ns] = lo;
-                fs.hi[ns] = hi;
-                ns++;
-            };
-            for (int64_t cc = c0; cc < c0 + 2; cc++) {
-                double* h = hand0 + (cc % skh::HAND_COPIES) * hstride;
-                seg(h, 1, 0, 0, 2 * 2 * nb + gp);                                   // t and m partials
-                seg(h + 2 * 2 * nb + gp, maxslot, 2 * ld, 2 * c0, 2 * n);           // row partials
-                seg(h + 2 * 2 * nb + gp + 2 * nseg * ld, tg.nts, 2 * ld, 2 * c0, 2 * n);   // column partials
-                seg(own0 + (cc & (skh::OWN_COPIES - 1)) * ostride, 1, 0, 0, ostride);
-            }
-            seg(hand0 + ((c0 + 2) % skh::HAND_COPIES) * hstride, 1, 0, 0, 2 * 2 * nb);
-            seg((double*)w.W, nbp, 2 * ld, 2 * c0, 2 * n);
-            seg((double*)w.V2, nbp, 2 * ld, 2 * c0, 2 * n);
+            const int ns = zpanel_fill(c0, nbp, g, pw, fs);
             HIPCHK((hipError_t)skh_internal_fill_empty(&fs, ns, st));
         }
-        HIPCHK((hipError_t)skh_internal_launch_zpanel(dA, lda, n, c0, nbp, &w, dtau, deta, ctl.err, g, st));
-        HIPCHK((hipError_t)skh_internal_launch_zrank2k(dA, lda, w.W, w.ldw, n, c0 + nbp, c0, nbp, st));
+        HIPCHK((hipError_t)skh_internal_launch_zpanel(dA, lda, n, c0, nbp, &pw, dtau, deta, ctl.err, g, st));
+        FillSegs fnext;
+        int nsnext = 0;
+        const int64_t c1 = c0 + nb;
+        if (c1 < kb_end) {
+            const int nbp1 = (int)std::min<int64_t>(nb, kb_end - c1);
+            nsnext = zpanel_fill(c1, nbp1, zgrid(c1), zpanel_w(c1), fnext);
+        }
+        HIPCHK((hipError_t)skh_internal_launch_zrank2k(dA, lda, pw.W, pw.ldw, n, c0 + nbp, c0, nbp, nsnext ? &fnext : nullptr,
+                                                       nsnext, st));
     }
     // the panel columns' final diagonal entries into A (no later launch reads them)
     HIPCHK((hipError_t)skh_internal_launch_zdfin(dA, lda, w.Dfin, 0, (int)kb_end, st));

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 
+#include "skh_fill.hpp"
 #include "skh_internal.h"
 #include "skh_sync.hpp"
 #include "skh_zengine.hpp"
@@ -136,9 +137,13 @@ __device__ __forceinline__ void zr2k_mfma(v4d re[2][4], v4d im[2][4], const doub
 
 __global__ __launch_bounds__(64, 2) void k_zrank2k(double2* A, int64_t lda, const double2* U, int64_t ldu, int64_t n,
                                                    int64_t e, int64_t c0, int nbp, int64_t b0, int64_t nt,
-                                                   int64_t total, int64_t per)
+                                                   int64_t total, int64_t per, FillSegs fs)
 {
     const int64_t bb = blockIdx.x;
+    if (bb >= 8 * per) {   // workgroups past the update fill the next panel's hand-over slots
+        fill_chunk(fs, bb - 8 * per);
+        return;
+    }
     const int64_t t = (bb % 8) * per + bb / 8;
     if (t >= total) return;
     int64_t ti, tj;
@@ -331,17 +336,21 @@ extern "C" int skh_internal_launch_zdfin(double2* A, int64_t lda, const double2*
 }
 
 // trailing update of rows / columns >= e after the panel [c0, c0 + nbp): 64 x 64 tile blocks of the
-// lower triangle from block e / 64, two waves each, dealt to the XCDs as k_rank2k
+// lower triangle from block e / 64, two waves each, dealt to the XCDs as k_rank2k; with fs (nseg > 0)
+// extra workgroups of the same launch fill those segments with the empty marker
 extern "C" int skh_internal_launch_zrank2k(double2* A, int64_t lda, const double2* U, int64_t ldu, int64_t n, int64_t e,
-                                           int64_t c0, int nbp, hipStream_t st)
+                                           int64_t c0, int nbp, const FillSegs* fs, int nseg, hipStream_t st)
 {
-    if (e >= n) return 0;
+    FillSegs f;
+    const int64_t nfill = fill_plan(fs, nseg, f);
+    if (nfill < 0) return (int)hipErrorInvalidValue;
     const int64_t b0 = e / R2K_BLK;
-    const int64_t nt = (n + R2K_BLK - 1) / R2K_BLK - b0;
+    const int64_t nt = e < n ? (n + R2K_BLK - 1) / R2K_BLK - b0 : 0;
     const int64_t total = nt * (nt + 1);   // two 32-row wave tiles per 64 x 64 block
     const int64_t per = (total + 7) / 8;
-    hipLaunchKernelGGL(k_zrank2k, dim3((unsigned)(per * 8)), dim3(64), 0, st, A, lda, U, ldu, n, e, c0, nbp, b0, nt, total,
-                       per);
+    if (per * 8 + nfill == 0) return 0;
+    hipLaunchKernelGGL(k_zrank2k, dim3((unsigned)(per * 8 + nfill)), dim3(64), 0, st, A, lda, U, ldu, n, e, c0, nbp, b0, nt,
+                       total, per, f);
     return (int)hipGetLastError();
 }
 

@@ -55,8 +55,6 @@ int skh_internal_zpanel_occupancy(int* blocks_per_cu);
 int skh_internal_launch_zpanel(double2* A, int64_t lda, int64_t n, int64_t c0, int nbp, const skh::zp::ZPWork* w,
                                double2* tau, double* eta, int* err, int grid, hipStream_t st);
 int skh_internal_launch_zdfin(double2* A, int64_t lda, const double2* Dfin, int64_t c0, int nbp, hipStream_t st);
-int skh_internal_launch_zrank2k(double2* A, int64_t lda, const double2* U, int64_t ldu, int64_t n, int64_t e,
-                                int64_t c0, int nbp, hipStream_t st);
 int skh_internal_zdiag(const double2* A, int64_t lda, int64_t n, double* dvim, hipStream_t st);
 int skh_internal_zform_v(const double2* F, int64_t ldf, int64_t n, int64_t c0, int nbp, double2* V, int64_t ldv,
                          hipStream_t st);
@@ -137,6 +135,9 @@ struct FillSegs {
 };
 int skh_internal_fill_empty(const FillSegs* fs, int nseg, hipStream_t st);
 int skh_internal_panel_occupancy(int nbmax, int* blocks_per_cu);
+// the complex rank-2k update after a panel; with fs (nseg > 0) the same launch also fills those segments
+int skh_internal_launch_zrank2k(double2* A, int64_t lda, const double2* U, int64_t ldu, int64_t n, int64_t e,
+                                int64_t c0, int nbp, const FillSegs* fs, int nseg, hipStream_t st);
 // the rank-2k update after a panel; with fs (nseg > 0) the same launch also fills those segments
 int skh_internal_launch_rank2k(double* A, int64_t lda, const double* U, int64_t ldu, int64_t n, int64_t e,
                                int64_t c0, int nbp, const FillSegs* fs, int nseg, hipStream_t st);

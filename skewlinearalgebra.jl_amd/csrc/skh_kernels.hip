@@ -14,6 +14,7 @@
 #include <algorithm>
 
 #include "skh_device.hpp"
+#include "skh_fill.hpp"
 #include "skh_internal.h"
 #include "skh_sync.hpp"
 
@@ -86,21 +87,6 @@ __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int
         clk_outer(clk, 0);
         for (int k = 0; k < 13; k++) ph[k] += s_clk[k];
     }
-}
-
-// entries of fill workgroup f (64 threads, FILL_CHUNK consecutive entries of one row of one segment)
-constexpr int FILL_CHUNK = 1024;
-__device__ void fill_chunk(const FillSegs& fs, int64_t f)
-{
-    int k = 0;
-    while (k + 1 < fs.nseg && f >= fs.first[k + 1]) k++;
-    const int64_t local = f - fs.first[k];
-    const int64_t row = local / fs.chunks[k];
-    const int64_t lo = fs.lo[k] + (local % fs.chunks[k]) * FILL_CHUNK;
-    const int64_t hi = min(fs.hi[k], lo + FILL_CHUNK);
-    double* s = fs.p[k] + row * fs.ld[k];
-    const double e = empty_slot();
-    for (int64_t r = lo + threadIdx.x; r < hi; r += 64) s[r] = e;
 }
 
 // One wave per 32x64 tile; tile blocks (64x64, two tiles each) of the trailing triangle
@@ -378,19 +364,9 @@ extern "C" int skh_internal_panel_occupancy(int nbmax, int* blocks_per_cu)
 extern "C" int skh_internal_launch_rank2k(double* A, int64_t lda, const double* U, int64_t ldu, int64_t n, int64_t e,
                                           int64_t c0, int nbp, const FillSegs* fs, int nseg, hipStream_t st)
 {
-    if (nseg < 0 || nseg > FillSegs::MAXSEG || (nseg > 0 && !fs)) return (int)hipErrorInvalidValue;
-    FillSegs f{};
-    int64_t nfill = 0;
-    if (nseg > 0) {
-        f = *fs;
-        for (int k = 0; k < nseg; k++) {
-            f.first[k] = nfill;
-            f.chunks[k] = std::max<int64_t>(1, (f.hi[k] - f.lo[k] + FILL_CHUNK - 1) / FILL_CHUNK);
-            nfill += f.rows[k] * f.chunks[k];
-        }
-        f.first[nseg] = nfill;
-        f.nseg = nseg;
-    }
+    FillSegs f;
+    const int64_t nfill = fill_plan(fs, nseg, f);
+    if (nfill < 0) return (int)hipErrorInvalidValue;
     const int64_t b0 = e / R2K_BLK, nt = e < n ? (n + R2K_BLK - 1) / R2K_BLK - b0 : 0;
     const int64_t total = nt * (nt + 1) / 2 * R2K_WPB;   // wave tiles of the blocks of the triangle
     const int64_t per = (total + 7) / 8;
