@@ -894,10 +894,10 @@ __device__ bool row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
 
 // ---------------------------------------------------------------------------------------
 // Phase B of column j, once the row pass's outputs of column j it needs are in:
-//   (1) the sums over rows >= j+2 of W^T x and V^T x of column j from the per-block partials
-//       (unscaled), element e of t (W part e < NBMAX, V part NBMAX + k) by wave e mod 4 of rank
-//       (e / 4) mod G for the whole launch, live while k < i (the consumer scales them and adds
-//       unit * row j+1);
+//   (1) after the matvec (2): the sums over rows >= j+2 of W^T x and V^T x of column j from the
+//       per-block partials (unscaled), element e of t (W part e < NBMAX, V part NBMAX + k) by wave
+//       e mod 4 of rank (e / 4) mod G for the whole launch, live while k < i (the consumer scales
+//       them and adds unit * row j+1);
 //   (2) the skew matvec partials of column j over the strictly lower triangle of
 //       A[j+1:n, j+1:n] on MR x MC tiles (strip-major), split into G contiguous tile ranges, one
 //       per rank, on the unscaled x (x_{j+1} = alpha): the consumer forms
@@ -912,8 +912,8 @@ __device__ bool row_pass(double* __restrict__ A, int64_t lda, int64_t n, int64_t
 //       norm partials (the next row pass's inputs), then the slot emptying.
 // The first tile is loaded before the poll (tile data is not written while any rank can still read
 // it: a column c > j is stored as a factor in row pass c + 1, after every rank's matvec of column
-// j has been consumed); the poll of (1) and the first tile's x entries is one memory round trip
-// once they are stored.
+// j has been consumed); the poll of the first tile's x entries is one memory round trip once they
+// are stored.
 // Only ranks with a part in column j take it (tiles, a live t element, or rows of w_j to finish
 // in the next row pass): every value such a rank reads is consumed by a row pass, which is what
 // lets producers refill the slots with the empty marker safely.
@@ -1035,9 +1035,17 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
     const double* pxc = xj + (xcin ? c : j + 1);
     const double* pxs = xj + (xsin ? rs : j + 1);
     const int64_t e0c = t0 ? e0 : 0;
+    // t is formed after the matvec (its partials not polled here), so the ranks that own t elements
+    // (also row-block owners) do not wait for every block's row pass before their matvec (n = 8192:
+    // 186.8 -> 185.7 ms, nb = 64: 213.6 -> 207.2 ms); SKH_T_AFTER=0 builds the form before it
+#ifndef SKH_T_AFTER
+#define SKH_T_AFTER 1
+#endif
+    constexpr bool TA = SKH_T_AFTER;
+    const bool tp = t0 && !TA;
     bool alive = true;
     for (unsigned spins = 0;; spins++) {
-        if (t0) {   // wave-uniform branches: no per-lane exec mask around the loads
+        if (tp) {   // wave-uniform branches: no per-lane exec mask around the loads
             ps0 = ld_co(psj + q0 * (2 * NBMAX) + e0c);
             ps1 = ld_co(psj + q1 * (2 * NBMAX) + e0c);
         }
@@ -1046,7 +1054,7 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
             xs = ld_co(pxs);
         }
         unsigned miss = 0;
-        if (t0 && ((in0 && !filled(ps0)) || (in1 && !filled(ps1)))) miss |= 128;
+        if (tp && ((in0 && !filled(ps0)) || (in1 && !filled(ps1)))) miss |= 128;
         if (xcin && !filled(xc)) miss |= 256;
         if (xsin && !filled(xs)) miss |= 512;
         if (__all(miss == 0)) break;
@@ -1066,10 +1074,11 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
     clk_inner(clk, 12);
     if (mv) sh.vstrip[tid] = xs;
     // ---- (2) t of the live elements of this wave (unscaled: the consumer applies scale) ----
+    auto form_t = [&]() {
     for (int e = e0; e < 2 * NBMAX && alive; e += 4 * G) {
         if ((e % NBMAX) >= i) continue;   // wave-uniform
         double acc;
-        if (e == e0) {
+        if (e == e0 && !TA) {
             acc = ps0 + ps1;
         } else {
             acc = 0.0;
@@ -1085,6 +1094,8 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         acc = wave_sum(acc);
         if (lane == 0) st_co(tv_of(w, j) + e, acc);
     }
+    };
+    if (!TA) form_t();
     clk_inner(clk, 9);
     if (__syncthreads_or(!alive)) return false;   // x of the first strip is in LDS
     // ---- (3) matvec ----
@@ -1207,6 +1218,7 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
         }
         clk_inner(clk, 10);
     }
+    if (TA) form_t();
     // ---- (1) the reflector scalars of column j, after the matvec, which needs none of them: from the
     // block norm partials, formed by every rank in the same fixed order (wave 0, strided lanes, a
     // butterfly), so all ranks hold bit-identical values.  Having seen every block's norm partial of
