@@ -249,7 +249,7 @@ int get_ctl(Ctl& c)
 }
 
 // ----------------------------------------------------------------------------------------
-// Large engine: cooperative panel kernel + MFMA rank-2k update per panel.
+// Large engine: persistent panel kernel (every workgroup resident) + MFMA rank-2k update per panel.
 // The workspace and control block are the library's cached buffers, or (ws_ext, ctl_ext) given by
 // the caller: ws_cap doubles and CTL_BYTES, nothing allocated then.
 // ----------------------------------------------------------------------------------------
@@ -1179,8 +1179,9 @@ int check_barrier_flag()
         if ((herr & 1) && (herr >> 1) != 0) {
             static const char* site[] = {"?", "row pass", "phase B", "phase B column entry"};
             return set_err(SKH_ERR_BARRIER, "a bounded wait of the panel kernel ran out (%s, column %d mod 65536, "
-                           "workgroup %d; inputs missing 0x%x, index %d)", site[(herr >> 1) & 3], (herr >> 3) & 0xFFFF,
-                           (herr >> 19) & 0x3FF, hv[1], hv[2] - 1);
+                           "workgroup %d; inputs missing 0x%x, index %d); if other work shares the GPU, "
+                           "SKH_PANEL_COOPERATIVE=1 keeps the panel's workgroups co-resident", site[(herr >> 1) & 3],
+                           (herr >> 3) & 0xFFFF, (herr >> 19) & 0x3FF, hv[1], hv[2] - 1);
         }
         return set_err(SKH_ERR_BARRIER, "a bounded wait of a cooperative kernel ran out (lost workgroup)");
     }
@@ -1313,7 +1314,7 @@ int zpf_run(int64_t batch, const int64_t* ns, const int64_t* offs, const double2
             order.push_back(b);
         }
     }
-    // large even members: one cooperative launch each over the whole GPU
+    // large even members: the large engine over the whole GPU, one after another
     if (!big.empty()) {
         Ctl ctl;
         int rc = get_ctl(ctl);

@@ -11,7 +11,7 @@
 // v^H y = i m is imaginary (A22 + V W^H - W V^H is skew-Hermitian), so it is c v with
 // c = -|tau|^2 (i m) / 2 and m = Im(v^H A22 v) + 2 Im(t2^H t1), t1 = W^H v, t2 = V^H v.
 //
-// The structure is the real panel kernel's (skh_device.hpp): cooperative persistent launch per
+// The structure is the real panel kernel's (skh_device.hpp): persistent launch (all co-resident) per
 // panel, no grid barrier (every hand-over value sits in a slot holding the empty marker until it is
 // stored, and is polled; the same copies, refill rules and drains), a row pass per 64-row block
 // and a phase B per column with the skew matvec over 256 x 16 tiles of the trailing lower
@@ -19,6 +19,10 @@
 //   * the diagonal of the trailing block (imaginary, not zero) belongs to the matvec: the tile
 //     holding (c, c) adds A[c,c] v_c to row c and Im(A[c,c]) |v_c|^2 to m;
 //   * the matvec also produces the per-workgroup partials of m, polled by every row pass;
+//   * the matvec runs on the unscaled column x (x_{j+1} = alpha), as in the real kernel: with
+//     v = s x + c e_{j+1} (s = 1/(alpha - beta), c = unit - s alpha) the row pass forms
+//     A22 v = s A22 x + c A22 e_{j+1} and m = |s|^2 Im(x^H A22 x) + 2 Im(conj(s) c q)
+//     + |c|^2 Im(A[j+1, j+1]) with q = x^H A22 e_{j+1} from per-block partials of the row pass;
 //   * A is not written on its diagonal during a launch: the panel columns' final diagonal goes to
 //     Dfin and is copied into A after the launch (the special row of w reads A[j, j] as it was at
 //     the launch's start);
