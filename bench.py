@@ -60,7 +60,7 @@ def log(*a):
 CSRC = os.path.join(HERE, "skewlinearalgebra.jl_amd", "csrc")
 # the sources a kernel's code comes from (its traffic profile is valid for exactly these bytes)
 KERNEL_SOURCES = {
-    "k_panel": ("skh_device.hpp", "skh_sync.hpp", "skh_kernels.hip"),
+    "k_panel": ("skh_device.hpp", "skh_sync.hpp", "skh_fill.hpp", "skh_kernels.hip"),
     "k_compact": ("skh_device.hpp", "skh_sync.hpp", "skh_compact.hpp", "skh_compact.hip"),
     "k_cblock": ("skh_device.hpp", "skh_sync.hpp", "skh_cblock.hpp", "skh_compact.hip"),
 }
