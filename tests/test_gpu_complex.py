@@ -139,6 +139,42 @@ def test_zsytrd_structured():
         check_zfact(A, F, tau, eta, dvim)
 
 
+def zdegenerate(n, seed):
+    """Skew-Hermitian matrices whose reflectors hit the householder! special cases inside the blocked
+    panels: a tridiagonal one with an imaginary diagonal (every tail zero: x = alpha e_1, where the
+    unscaled matvec's m takes q = conj(alpha) A[j+1, j+1]), the same plus a dense trailing block, and
+    a block-diagonal one (zero columns at the block edges: tau = 0)."""
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal(n - 1) + 1j * rng.standard_normal(n - 1)
+    T = np.diag(1j * rng.standard_normal(n)) + np.diag(a, -1) - np.diag(np.conj(a), 1)
+    Z = T.copy()
+    h = n // 2 + 5
+    Z[h:, h:] += zskew(n - h, seed + 1)
+    B = np.zeros((n, n), complex)
+    r = 0
+    for size in rng.integers(1, 140, size=n):
+        if r >= n:
+            break
+        m = int(min(size, n - r))
+        B[r:r + m, r:r + m] = zskew(m, seed + 2 + r)
+        r += m
+    return [T, Z, B]
+
+
+@pytest.mark.parametrize("n", [700, 1111])
+def test_zsytrd_blocked_degenerate(n):
+    for k, A in enumerate(zdegenerate(n, 60 + n)):
+        say("zsytrd blocked degenerate n=%d case %d" % (n, k))
+        F, tau, eta, dvim = gpu_zsytrd(A, _lib.ENGINE_LARGE)
+        Fo, tauo, etao = oracle.skewblockedhess(A)
+        nA = max(1.0, np.linalg.norm(A))
+        np.testing.assert_allclose(eta, etao, rtol=0, atol=1e-11 * nA)
+        np.testing.assert_array_equal(tau != 0, tauo != 0)
+        np.testing.assert_allclose(tau, tauo, rtol=0, atol=1e-11)
+        np.testing.assert_allclose(dvim, np.imag(np.diag(Fo)), rtol=0, atol=1e-11 * nA)
+        check_zfact(A, F, tau, eta, dvim)
+
+
 def test_zorgtr_device_vs_oracle():
     say("zorgtr")
     import torch

@@ -155,6 +155,51 @@ def test_sytrd_medium(n):
     check_fact(A, F, tau, eta)
 
 
+def degenerate_skew(n, seed):
+    """Skew matrices whose reflectors hit the householder! special cases inside multi-column panels:
+    an already tridiagonal one (every tail zero, tau = 2 where alpha != 0), a block-diagonal one
+    (zero coupling at the block edges: tau = 0) and a tridiagonal one with a dense trailing block
+    (zero tails up to the block, then full reflectors in the same panels).  Zeroed rows and columns
+    are avoided: they make the matrix singular, and past a numerically zero eta the reflectors are
+    fixed by rounding noise (the issue-130 case below)."""
+    rng = np.random.default_rng(seed)
+    e = rng.standard_normal(n - 1)
+    e[rng.integers(0, n - 1, size=max(1, n // 50))] = 0.0
+    T = np.diag(e, -1) - np.diag(e, 1)
+    B = np.zeros((n, n))
+    r = 0
+    for size in rng.integers(1, 140, size=n):
+        if r >= n:
+            break
+        m = int(min(size, n - r))
+        G = rng.standard_normal((m, m))
+        B[r:r + m, r:r + m] = G - G.T
+        r += m
+    e2 = rng.standard_normal(n - 1)
+    Z = np.diag(e2, -1) - np.diag(e2, 1)
+    h = n // 2 + 7
+    Z[h:, h:] += skew_gauss(n - h, seed + 1)
+    return [T, B, Z]
+
+
+@pytest.mark.parametrize("n,nb", [(700, 32), (1000, 64), (513, 128)])
+def test_sytrd_degenerate_columns_large(n, nb):
+    """Zero reflector tails and zero columns in the middle of the large engine's panels (the matvec
+    runs on the unscaled column and its consumer applies scale and corr = unit - scale * alpha, which
+    must give the reference's tau = 2 and tau = 0 reflectors): the oracle's eta, the same zero
+    pattern of tau, and the factorization."""
+    for k, A in enumerate(degenerate_skew(n, 40 + n)):
+        say("sytrd degenerate n=%d nb=%d case %d" % (n, nb, k))
+        F, tau, eta, nz = gpu_sytrd(A, nb)
+        Fo, tauo, etao = oracle.skewblockedhess(A)
+        nA = max(1.0, np.linalg.norm(A))
+        np.testing.assert_allclose(eta, etao, rtol=0, atol=1e-11 * nA)
+        np.testing.assert_array_equal(tau != 0, tauo != 0)
+        np.testing.assert_allclose(tau, tauo, rtol=0, atol=1e-11)
+        assert nz == int(np.count_nonzero(tau))
+        check_fact(A, F, tau, eta)
+
+
 def test_edge_conventions(golden):
     say("edge conventions")
     cases = [np.array(golden["issue121"]["A"], float), np.array(golden["readme"]["A"], float)]
