@@ -89,6 +89,15 @@ int skh_sytrd_d_device_noalloc(int64_t n, double *dA, int64_t lda, double *dtau,
                                const skh_opts *opts);
 
 /*
+ * Status of the device work of the last _device entry call (the allocation-free one included):
+ * waits for `stream`, then returns SKH_ERR_BARRIER (details in skh_last_error) if one of its
+ * persistent kernels abandoned a bounded wait (its results are then invalid), else 0.  The _device
+ * entries of the large engine do not wait for their kernels, so this is where they report it; each
+ * such call clears the flag when it starts.  No counterpart in the reference (its calls are synchronous).
+ */
+int skh_device_status(void *stream);
+
+/*
  * log|Pf(A)| and sign for a real skew-symmetric A (A is not modified).
  * Replaces _logabspfaffian!(A::SkewHermitian{<:Real}) (reference src/pfaffian.jl:167-179):
  * odd n -> (-Inf, 0).

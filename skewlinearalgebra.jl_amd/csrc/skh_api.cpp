@@ -1167,13 +1167,9 @@ int grad_run(int64_t batch, const int64_t* ns, const int64_t* offs, const double
     return 0;
 }
 
-int check_barrier_flag()
+// the error words of a control block (flag, inputs missing, index + 1) as a status
+int barrier_status(const int hv[3])
 {
-    Ctl ctl;
-    int rc = get_ctl(ctl);
-    if (rc) return rc;
-    int hv[3] = {0, 0, 0};
-    HIPCHK(hipMemcpy(hv, ctl.err, sizeof(hv), hipMemcpyDeviceToHost));
     const int herr = hv[0];
     if (herr) {
         if ((herr & 1) && (herr >> 1) != 0) {
@@ -1186,6 +1182,16 @@ int check_barrier_flag()
         return set_err(SKH_ERR_BARRIER, "a bounded wait of a cooperative kernel ran out (lost workgroup)");
     }
     return 0;
+}
+
+int check_barrier_flag()
+{
+    Ctl ctl;
+    int rc = get_ctl(ctl);
+    if (rc) return rc;
+    int hv[3] = {0, 0, 0};
+    HIPCHK(hipMemcpy(hv, ctl.err, sizeof(hv), hipMemcpyDeviceToHost));
+    return barrier_status(hv);
 }
 
 int sytrd_device_locked(int64_t n, double* dA, int64_t lda, double* dtau, double* deta, hipStream_t st,
@@ -1666,6 +1672,28 @@ int skh_sytrd_d_device_noalloc(int64_t n, double* dA, int64_t lda, double* dtau,
     g_timer.on = false;
     g_timer.phases = false;
     return sytrd_large(n, dA, lda, dtau, deta, pick_nb(n, opts), (hipStream_t)stream, ws, cap, ctl);
+}
+
+int skh_device_status(void* stream)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc = ensure_init_locked(g_device);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    rc = check_barrier_flag();
+    if (rc) return rc;
+    unsigned* ctl = nullptr;
+    double* ws = nullptr;
+    size_t cap = 0;
+    if (!g_noalloc_status && !skh_internal_noalloc_scratch(g_device, &ctl, &ws, &cap)) {
+        Ctl c;
+        ctl_at(ctl, c);
+        int hv[3] = {0, 0, 0};
+        HIPCHK(hipMemcpy(hv, c.err, sizeof(hv), hipMemcpyDeviceToHost));
+        rc = barrier_status(hv);
+        if (rc) return rc;
+    }
+    return 0;
 }
 
 int skh_sytrd_d(int64_t n, double* A, int64_t lda, double* tau, double* eta, int64_t* ntau_nonzero,

@@ -50,6 +50,7 @@ def lib():
         "skh_sytrd_d": ([i64, vp, i64, vp, vp, _i64p, op], ctypes.c_int),
         "skh_sytrd_d_device": ([i64, vp, i64, vp, vp, vp, op], ctypes.c_int),
         "skh_sytrd_d_device_noalloc": ([i64, vp, i64, vp, vp, vp, op], ctypes.c_int),
+        "skh_device_status": ([vp], ctypes.c_int),
         "skh_logpf_d": ([i64, vp, i64, _dp, _dp, op], ctypes.c_int),
         "skh_logpf_d_device": ([i64, vp, i64, vp, vp, vp, op], ctypes.c_int),
         "skh_logpf_batched_d": ([i64, vp, vp, vp, vp, vp, op], ctypes.c_int),

@@ -57,6 +57,7 @@ def test_n8192_default_width_residual_orthogonality_eigvals():
     o = _lib.opts(0, skl.ENGINE_LARGE)
     _lib.check("skh_sytrd_d_device", L.skh_sytrd_d_device(n, work.data_ptr(), n, tau.data_ptr(), eta.data_ptr(), st,
                                                          ctypes.byref(o)))
+    _lib.check("skh_device_status", L.skh_device_status(st))   # the factorization abandoned no bounded wait
     dQ = torch.empty((n, n), dtype=torch.float64, device=dev)
     _lib.check("skh_orgtr_d_device", L.skh_orgtr_d_device(n, work.data_ptr(), n, tau.data_ptr(), dQ.data_ptr(), n, st))
     torch.cuda.synchronize()

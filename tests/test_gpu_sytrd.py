@@ -640,6 +640,7 @@ def test_sytrd_n8192_invariants(nb):
     st = torch.cuda.current_stream().cuda_stream
     rc = _lib.lib().skh_sytrd_d_device(n, work.data_ptr(), n, tau.data_ptr(), eta.data_ptr(), st, ctypes.byref(o))
     _lib.check("skh_sytrd_d_device", rc)
+    _lib.check("skh_device_status", _lib.lib().skh_device_status(st))   # no bounded wait abandoned
     torch.cuda.synchronize()
     del work
     e = eta[: n - 1].cpu().numpy()

@@ -26,6 +26,7 @@ def _factor(torch, entry, A, ld, nb):
     rc = getattr(_lib.lib(), entry)(n, work.data_ptr(), ld, tau.data_ptr(), eta.data_ptr(),
                                     torch.cuda.current_stream().cuda_stream, ctypes.byref(o))
     _lib.check(entry, rc)
+    _lib.check("skh_device_status", _lib.lib().skh_device_status(torch.cuda.current_stream().cuda_stream))
     torch.cuda.synchronize()
     F = work.cpu().numpy()[:, :n].T   # column-major factors as an n x n array
     return F, tau[: n - 1].cpu().numpy(), eta[: n - 1].cpu().numpy()
