@@ -486,6 +486,7 @@ def complex_bench(torch, _lib, steps, n=4096, seed=314159):
     step()
     torch.cuda.synchronize()
     ms = timed(torch, step, max(3, steps))
+    _lib.check("skh_device_status", L.skh_device_status(st))   # the last timed call abandoned no bounded wait
     dQ = torch.empty((n, n), dtype=torch.complex128, device=dev)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
@@ -771,6 +772,7 @@ def main():
     for k in range(args.steps):
         step()
         _lib.check("skh_diag_timings", L.skh_diag_timings(out6))
+        _lib.check("skh_device_status", L.skh_device_status(st))   # no bounded wait abandoned
         tm = [a + b for a, b in zip(tm, out6)]
         log("step %d done" % k)
     ev1.record()
@@ -851,6 +853,7 @@ def main():
         e1.record()
         torch.cuda.synchronize()
         _lib.check("skh_sytrd_d_device", rc)
+        _lib.check("skh_device_status", L.skh_device_status(st))
         _lib.check("skh_diag_timings", L.skh_diag_timings(out6))
         t_ms = e0.elapsed_time(e1)
         sweep[str(nbv)] = {"ms": t_ms, "tflops": flops / (t_ms * 1e-3) / 1e12,
