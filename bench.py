@@ -290,8 +290,8 @@ def batched_bench(torch, skl, _lib, steps):
     plan.status(st)
     same = bool(torch.equal(lp, ref_lp))
     # per-matrix GPU loop on every 16th matrix (all 16 sizes represented): one skh_logpf_d_device
-    # call per matrix, the library's single-matrix engine choice (one workgroup to n = 192, the
-    # cooperative engine above), as a user looping logabspfaffian over device matrices would
+    # call per matrix, the library's single-matrix engine choice (one workgroup to n = 320, the
+    # large engine above), as a user looping logabspfaffian over device matrices would
     o = _lib.opts()
     L = _lib.lib()
     sample = np.arange(0, count, 16)

@@ -1,6 +1,6 @@
 // skh_noalloc.hip -- the fixed device scratch of skh_sytrd_d_device_noalloc.
 //
-// One file-scope device array per GPU holds what the large engine (cooperative panel kernel +
+// One file-scope device array per GPU holds what the large engine (persistent panel kernel +
 // MFMA rank-2k update) otherwise takes from the library's cached buffers: the 2 KiB control
 // block (error flag, phase clocks) followed by the workspace (W and V of the panel, the copies of
 // the norm and W^T x / V^T x partials and of the column, t, the matvec row and column partials).

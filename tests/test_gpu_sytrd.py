@@ -462,7 +462,7 @@ def test_autograd_function():
 
 @pytest.mark.parametrize("nb", [32, 64])
 def test_sytrd_n4096_vs_oracle(nb):
-    """Large enough that every workgroup of the cooperative grid exchanges data each column."""
+    """Large enough that every workgroup of the panel grid exchanges data each column."""
     say("sytrd n=4096 nb=%d" % nb)
     n = 4096
     A = skew_gauss(n, 4096 + nb)
@@ -578,7 +578,7 @@ def test_complex_eigvals_value_range():
 
 @pytest.mark.parametrize("n,cap", [(1000, 4), (2049, 16)])
 def test_sytrd_small_grid_several_blocks(n, cap):
-    """A capped cooperative grid makes workgroups own several 64-row blocks (the load-based row
+    """A capped panel grid makes workgroups own several 64-row blocks (the load-based row
     pass beside the LDS-resident one); the factorization must not change beyond rounding."""
     say("sytrd small grid n=%d cap=%d" % (n, cap))
     A = skew_gauss(n, 77)
@@ -620,7 +620,7 @@ def tri_fourth_moment(e):
 
 @pytest.mark.parametrize("nb", [32, 64, 128])
 def test_sytrd_n8192_invariants(nb):
-    """Full-size factorization, the whole cooperative grid (512 workgroups) in the first panels.
+    """Full-size factorization, the whole panel grid (512 workgroups) in the first panels.
     A = Q T Q^T fixes ||T||_F = ||A||_F, tr(T^4) = ||A^2||_F^2 and log|det A| = 2 log|Pf T|;
     the right-hand sides come from A itself on the GPU (a product and an LU), so a reflector
     built from stale or misplaced data fails one of them."""
@@ -657,7 +657,7 @@ def test_sytrd_n8192_invariants(nb):
 
 @pytest.mark.parametrize("nb", [32, 64])
 def test_sytrd_repeatable_and_clock_neutral(nb):
-    """The cooperative engine hands partial sums between workgroups through tagged values and
+    """The large engine hands partial sums between workgroups through polled slots and
     parity buffers: every result must be independent of timing.  Three factorizations of the same
     matrix (one with the per-kernel events and the phase clock on) give bit-identical factors,
     tau and eta."""
