@@ -52,6 +52,13 @@ __global__ __launch_bounds__(WG, 1) void k_zfused(double2* A, int64_t lda, int64
 
 constexpr int ZR2K_TR = 32;   // rows of a complex rank-2k wave tile (two per 64 x 64 block)
 
+// lane offsets of the complex wave tile's operand rows (32 x 64 tile: two 16-row blocks of L, four of
+// R), fixed here, apart from the real update's tile shape
+struct ZR2kRows {
+    uint32_t lv[2], lu[2];   // L rows r0 + 16 bi + lane%16, V and W layouts
+    uint32_t rv[4], ru[4];   // R rows q0 + 16 bj + lane%16
+};
+
 // One panel of the blocked complex reduction (device code in skh_zpanel.hpp): co-resident, one
 // workgroup per CU, one 64-row block per workgroup (grid >= blocks of the trailing matrix).
 __global__ __launch_bounds__(WG, 1) void k_zpanel(double2* A, int64_t lda, int64_t n, int64_t c0, int nbp, zp::ZPWork w,
@@ -98,7 +105,7 @@ __global__ void k_zdfin(double2* A, int64_t lda, const double2* Dfin, int64_t c0
 // operands are the two halves of one 16-byte load).
 __device__ __forceinline__ void zr2k_operands(const double2* __restrict__ A, int64_t lda, const double2* __restrict__ U,
                                               int64_t ldu, int64_t c0, int nbp, int nkh, int s, int lq,
-                                              const R2kRows& o, double2 lf[2], double2 rf[4])
+                                              const ZR2kRows& o, double2 lf[2], double2 rf[4])
 {
     const int h = s >= nkh;
     const int ks = s - h * nkh;
@@ -151,7 +158,7 @@ __global__ __launch_bounds__(64, 2) void k_zrank2k(double2* A, int64_t lda, cons
     const int64_t r0 = (b0 + ti) * R2K_BLK + (t & 1) * ZR2K_TR, q0 = (b0 + tj) * R2K_BLK;
     if (r0 + ZR2K_TR - 1 < q0) return;   // the upper half of a diagonal block: nothing on or below the diagonal
     const int lane = threadIdx.x & 63, lc = lane & 15, lq = lane >> 4;
-    R2kRows o;
+    ZR2kRows o;
 #pragma unroll
     for (int b = 0; b < 2; b++) {
         const int64_t rl = min(r0 + b * 16 + lc, n - 1);

@@ -1303,7 +1303,7 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
 //   A[r,c] += sum_k L[r,k] R[c,k]   for r > c >= e,
 //   L = [V U], R = [U -V], V[:,k] = A[:, c0+k] (reflector k, its unit element in place),
 //   U[:,k] = W[:,k], k < nbp.
-// One wave owns a 32x64 output tile (rows r0.., cols q0..) and runs without LDS or
+// One wave owns a 32x32 output tile (rows r0.., cols q0..) and runs without LDS or
 // barriers: its operands stream from L2 straight into the MFMA operand registers, one k-step
 // ahead, through a uniform base pointer plus a 32-bit lane offset.  Independent waves drift
 // out of phase, so one wave's C traffic overlaps another's MFMAs (an LDS-staged 128x128
@@ -1317,8 +1317,11 @@ __device__ bool phase_b(const double* __restrict__ A, int64_t lda, int64_t n, in
 #ifndef SKH_R2K_TR
 #define SKH_R2K_TR 32
 #endif
+// 32 x 32 wave tiles: four per 64 x 64 block, about 2.6 tiles per wave slot over an n = 8192
+// factorization's launches and 4 waves per SIMD (32 x 64: 15.14 ms of rank-2k per factorization,
+// 32 x 32: 14.40 ms, 16 x 64: 18.21 ms)
 #ifndef SKH_R2K_TC
-#define SKH_R2K_TC 64
+#define SKH_R2K_TC 32
 #endif
 constexpr int R2K_BLK = 64;          // edge of a tile block (the unit of the supertile order)
 constexpr int R2K_TR = SKH_R2K_TR;   // rows of a wave tile

@@ -89,7 +89,7 @@ __global__ __launch_bounds__(WG, (KQ <= 16) ? 2 : 1) void k_panel(double* A, int
     }
 }
 
-// One wave per 32x64 tile; tile blocks (64x64, two tiles each) of the trailing triangle
+// One wave per R2K_TR x R2K_TC tile; tile blocks (64x64, R2K_WPB tiles each) of the trailing triangle
 // from block b0 = e/64, consecutive indices on one XCD (blocks are dealt to XCDs round robin).
 // Workgroups from 8 per on fill fs (the next panel's hand-over slots) instead.
 __global__ __launch_bounds__(64, 2) void k_rank2k(double* A, int64_t lda, const double* U, int64_t ldu, int64_t n,
