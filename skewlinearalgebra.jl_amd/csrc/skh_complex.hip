@@ -50,13 +50,20 @@ __global__ __launch_bounds__(WG, 1) void k_zfused(double2* A, int64_t lda, int64
     }
 }
 
-constexpr int ZR2K_TR = 32;   // rows of a complex rank-2k wave tile (two per 64 x 64 block)
+constexpr int ZR2K_TR = 32;   // rows of a complex rank-2k wave tile (two row halves per 64 x 64 block)
+#ifndef SKH_ZR2K_BJ
+#define SKH_ZR2K_BJ 2
+#endif
+constexpr int ZBJ = SKH_ZR2K_BJ;           // 16-column blocks of a wave tile: 2 (32 x 32) or 4 (32 x 64)
+constexpr int ZCPB = 4 / ZBJ;              // wave tiles across a 64 x 64 block
+constexpr int ZWPB = 2 * ZCPB;             // wave tiles per 64 x 64 block
+static_assert(ZBJ == 2 || ZBJ == 4, "complex rank-2k wave tiles are 32 x 32 or 32 x 64");
 
-// lane offsets of the complex wave tile's operand rows (32 x 64 tile: two 16-row blocks of L, four of
-// R), fixed here, apart from the real update's tile shape
+// lane offsets of the complex wave tile's operand rows (two 16-row blocks of L, ZBJ of R), apart from
+// the real update's tile shape
 struct ZR2kRows {
-    uint32_t lv[2], lu[2];   // L rows r0 + 16 bi + lane%16, V and W layouts
-    uint32_t rv[4], ru[4];   // R rows q0 + 16 bj + lane%16
+    uint32_t lv[2], lu[2];       // L rows r0 + 16 bi + lane%16, V and W layouts
+    uint32_t rv[ZBJ], ru[ZBJ];   // R rows q0 + 16 bj + lane%16
 };
 
 // One panel of the blocked complex reduction (device code in skh_zpanel.hpp): co-resident, one
@@ -100,12 +107,12 @@ __global__ void k_zdfin(double2* A, int64_t lda, const double2* Dfin, int64_t c0
 // Complex skew rank-2k update after a panel (src/hessenberg.jl:158-166, complex case):
 // A[r, c] += sum_k L[r, k] conj(R[c, k]) for r >= c >= e, L = [V W], R = [W -V], on the diagonal the
 // imaginary part only (the reference adds Omega - Omega^H there).  One wave per 32 x 64 tile, as
-// k_rank2k, each 16 x 16 block transposed: Re += Rre Lre^T + Rim Lim^T, Im += Rre Lim^T - Rim Lre^T
+// k_rank2k (32 x 32 by default, SKH_ZR2K_BJ), each 16 x 16 block transposed: Re += Rre Lre^T + Rim Lim^T, Im += Rre Lim^T - Rim Lre^T
 // (four v_mfma_f64_16x16x4f64 per block and k-step; real and imaginary parts of the interleaved
 // operands are the two halves of one 16-byte load).
 __device__ __forceinline__ void zr2k_operands(const double2* __restrict__ A, int64_t lda, const double2* __restrict__ U,
                                               int64_t ldu, int64_t c0, int nbp, int nkh, int s, int lq,
-                                              const ZR2kRows& o, double2 lf[2], double2 rf[4])
+                                              const ZR2kRows& o, double2 lf[2], double2 rf[ZBJ])
 {
     const int h = s >= nkh;
     const int ks = s - h * nkh;
@@ -117,24 +124,24 @@ __device__ __forceinline__ void zr2k_operands(const double2* __restrict__ A, int
 #pragma unroll
         for (int b = 0; b < 2; b++) lf[b] = ok ? pv[o.lv[b]] : z;
 #pragma unroll
-        for (int b = 0; b < 4; b++) rf[b] = ok ? pu[o.ru[b]] : z;
+        for (int b = 0; b < ZBJ; b++) rf[b] = ok ? pu[o.ru[b]] : z;
     } else {
 #pragma unroll
         for (int b = 0; b < 2; b++) lf[b] = ok ? pu[o.lu[b]] : z;
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
+        for (int b = 0; b < ZBJ; b++) {
             const double2 t = ok ? pv[o.rv[b]] : z;
             rf[b] = make_double2(-t.x, -t.y);
         }
     }
 }
 
-__device__ __forceinline__ void zr2k_mfma(v4d re[2][4], v4d im[2][4], const double2 lf[2], const double2 rf[4])
+__device__ __forceinline__ void zr2k_mfma(v4d re[2][ZBJ], v4d im[2][ZBJ], const double2 lf[2], const double2 rf[ZBJ])
 {
 #pragma unroll
     for (int bi = 0; bi < 2; bi++)
 #pragma unroll
-        for (int bj = 0; bj < 4; bj++) {
+        for (int bj = 0; bj < ZBJ; bj++) {
             re[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(rf[bj].x, lf[bi].x, re[bi][bj], 0, 0, 0);
             re[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(rf[bj].y, lf[bi].y, re[bi][bj], 0, 0, 0);
             im[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(rf[bj].x, lf[bi].y, im[bi][bj], 0, 0, 0);
@@ -154,9 +161,11 @@ __global__ __launch_bounds__(64, 2) void k_zrank2k(double2* A, int64_t lda, cons
     const int64_t t = (bb % 8) * per + bb / 8;
     if (t >= total) return;
     int64_t ti, tj;
-    super_ij(t >> 1, nt, ti, tj);
-    const int64_t r0 = (b0 + ti) * R2K_BLK + (t & 1) * ZR2K_TR, q0 = (b0 + tj) * R2K_BLK;
-    if (r0 + ZR2K_TR - 1 < q0) return;   // the upper half of a diagonal block: nothing on or below the diagonal
+    super_ij(t / ZWPB, nt, ti, tj);
+    const int wt = (int)(t % ZWPB);   // row half wt / ZCPB, column part wt % ZCPB of the block
+    const int64_t r0 = (b0 + ti) * R2K_BLK + (wt / ZCPB) * ZR2K_TR;
+    const int64_t q0 = (b0 + tj) * R2K_BLK + (wt % ZCPB) * (16 * ZBJ);
+    if (r0 + ZR2K_TR - 1 < q0) return;   // above the diagonal of a diagonal block: nothing on or below it
     const int lane = threadIdx.x & 63, lc = lane & 15, lq = lane >> 4;
     ZR2kRows o;
 #pragma unroll
@@ -166,7 +175,7 @@ __global__ __launch_bounds__(64, 2) void k_zrank2k(double2* A, int64_t lda, cons
         o.lu[b] = (uint32_t)(rl + lq * ldu);
     }
 #pragma unroll
-    for (int b = 0; b < 4; b++) {
+    for (int b = 0; b < ZBJ; b++) {
         const int64_t rr = min(q0 + b * 16 + lc, n - 1);
         o.rv[b] = (uint32_t)(rr + lq * lda);
         o.ru[b] = (uint32_t)(rr + lq * ldu);
@@ -174,12 +183,12 @@ __global__ __launch_bounds__(64, 2) void k_zrank2k(double2* A, int64_t lda, cons
     double2* C = A + r0 + q0 * lda;
     const uint32_t co = (uint32_t)(lc + lq * lda);
     // strictly below the diagonal and inside the trailing block everywhere: plain loads and stores
-    const bool inner = (q0 + R2K_BLK - 1 < r0) && (r0 + ZR2K_TR - 1 < n) && (q0 >= e);
-    v4d re[2][4], im[2][4];
+    const bool inner = (q0 + 16 * ZBJ - 1 < r0) && (r0 + ZR2K_TR - 1 < n) && (q0 >= e);
+    v4d re[2][ZBJ], im[2][ZBJ];
 #pragma unroll
     for (int bi = 0; bi < 2; bi++)
 #pragma unroll
-        for (int bj = 0; bj < 4; bj++) {
+        for (int bj = 0; bj < ZBJ; bj++) {
             const int64_t row = r0 + lc + bi * 16;
             v4d cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -195,7 +204,7 @@ __global__ __launch_bounds__(64, 2) void k_zrank2k(double2* A, int64_t lda, cons
             im[bi][bj] = ci;
         }
     const int nkh = (nbp + 3) / 4, S = 2 * nkh;
-    double2 la[2], ra[4], lb[2], rb[4];
+    double2 la[2], ra[ZBJ], lb[2], rb[ZBJ];
     zr2k_operands(A, lda, U, ldu, c0, nbp, nkh, 0, lq, o, la, ra);
     for (int s = 0; s < S; s += 2) {
         if (s + 1 < S) zr2k_operands(A, lda, U, ldu, c0, nbp, nkh, s + 1, lq, o, lb, rb);
@@ -208,7 +217,7 @@ __global__ __launch_bounds__(64, 2) void k_zrank2k(double2* A, int64_t lda, cons
     for (int bi = 0; bi < 2; bi++) {
         const int64_t row = r0 + lc + bi * 16;
 #pragma unroll
-        for (int bj = 0; bj < 4; bj++)
+        for (int bj = 0; bj < ZBJ; bj++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int64_t col = q0 + lq + bj * 16 + 4 * q;
@@ -353,7 +362,7 @@ extern "C" int skh_internal_launch_zrank2k(double2* A, int64_t lda, const double
     if (nfill < 0) return (int)hipErrorInvalidValue;
     const int64_t b0 = e / R2K_BLK;
     const int64_t nt = e < n ? (n + R2K_BLK - 1) / R2K_BLK - b0 : 0;
-    const int64_t total = nt * (nt + 1);   // two 32-row wave tiles per 64 x 64 block
+    const int64_t total = nt * (nt + 1) / 2 * ZWPB;   // wave tiles of the blocks of the triangle
     const int64_t per = (total + 7) / 8;
     if (per * 8 + nfill == 0) return 0;
     hipLaunchKernelGGL(k_zrank2k, dim3((unsigned)(per * 8 + nfill)), dim3(64), 0, st, A, lda, U, ldu, n, e, c0, nbp, b0, nt,
