@@ -1612,7 +1612,7 @@ int skh_finalize(void)
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_inited) return 0;
     (void)hipStreamSynchronize(g_stream);
-    for (Buf* b : {&g_mat, &g_ws, &g_vec, &g_ctl, &g_bin, &g_bout, &g_q, &g_ev, &g_cslot, &g_big, &g_bctl, &g_cwbuf, &g_zws, &g_zq, &g_zc, &g_zx, &g_zpslot, &g_zpctl, &g_zpin,
+    for (Buf* b : {&g_mat, &g_ws, &g_vec, &g_ctl, &g_bin, &g_bout, &g_q, &g_ev, &g_cslot, &g_big, &g_bctl, &g_cwbuf, &g_zws, &g_zq, &g_zc, &g_zx, &g_zbws, &g_zpslot, &g_zpctl, &g_zpin,
                    &g_zpout, &g_chctl, &g_chin, &g_chout, &g_chp,
                    &g_chws, &g_rotq, &g_rotk, &g_rotcs, &g_rotseg, &g_fvec, &g_fpiv, &g_fz, &g_stq})
         b->release();

@@ -119,6 +119,32 @@ def test_zsytrd_cooperative_launch_same_bits():
     check_zfact(A, *r1)
 
 
+def test_finalize_and_reinit_same_bits():
+    """skh_finalize releases every cached workspace (the blocked complex panels' included) and the
+    stream; the next call initialises the library again and must give the same bits."""
+    say("finalize and re-init")
+    A = zskew(600, 17)
+    r0 = gpu_zsytrd(A, _lib.ENGINE_LARGE)
+    Ar = np.asfortranarray(np.real(A) - np.real(A).T)
+    Fr0 = np.array(Ar, order="F", copy=True)
+    n = Ar.shape[0]
+    tau0 = np.zeros(n - 1)
+    eta0 = np.zeros(n - 1)
+    o = _lib.opts(0, _lib.ENGINE_LARGE)
+    _lib.check("skh_sytrd_d", _lib.lib().skh_sytrd_d(n, Fr0.ctypes.data, n, tau0.ctypes.data, eta0.ctypes.data, None,
+                                                      ctypes.byref(o)))
+    assert _lib.lib().skh_finalize() == 0
+    r1 = gpu_zsytrd(A, _lib.ENGINE_LARGE)
+    Fr1 = np.array(Ar, order="F", copy=True)
+    tau1 = np.zeros(n - 1)
+    eta1 = np.zeros(n - 1)
+    _lib.check("skh_sytrd_d", _lib.lib().skh_sytrd_d(n, Fr1.ctypes.data, n, tau1.ctypes.data, eta1.ctypes.data, None,
+                                                      ctypes.byref(o)))
+    for a, b in zip(r0, r1):
+        assert np.array_equal(a, b)
+    assert np.array_equal(Fr0, Fr1) and np.array_equal(tau0, tau1) and np.array_equal(eta0, eta1)
+
+
 def test_zsytrd_structured():
     say("zsytrd structured")
     cases = [np.zeros((6, 6), complex)]
